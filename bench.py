@@ -1,0 +1,206 @@
+"""Benchmark of the single-GPU NEP MD step: atom-steps/s on the 256,000-atom
+Cu fcc NEP4 configuration (Bussi NVT, 300 K, dt 2 fs).
+
+A "step" is one velocity-Verlet MD step: integrate, Verlet-list check (and
+rebuild when an atom moved more than half the skin), typewise filter, fused
+descriptor/ANN/angular-partial kernel, force/virial gather, integrate,
+compensated thermo and the Bussi velocity rescale.  Inputs are resident in
+HBM before the timed region.  Prints one JSON line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s (1 GPU, NEP4 Cu 256k); served frames/s + p99 latency at fixed force err"
+MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_c1_cu.txt")
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (vector and MFMA), AMD public figure
+PEAK_HBM_GBS = 8000.0
+
+
+def flops_per_atom(info, n_r, n_a):
+    """Algorithmic FLOP per atom-step of the reference algorithm (the
+    canonical per-unit formulas of the measurement plan), split by kernel."""
+    KR, KA = info.basis_size_radial + 1, info.basis_size_angular + 1
+    NR, NA = info.n_max_radial + 1, info.n_max_angular + 1
+    L = info.l_max
+    ABC = (L + 1) ** 2 - 1
+    D, H = info.dim, info.num_neurons
+    q45 = info.num_L - L
+    f_desc = n_r * (3 * KR + 2 * NR * KR + 10) + n_a * (3 * KA + 2 * NA * KA + 2 * NA * ABC + 60) + NA * (3 * ABC + 30)
+    f_ann = 4 * D * H + 2 * H
+    f_ang = n_a * (6 * KA + 4 * NA * KA + NA * (20 * ABC + 60 * q45) + 20)
+    f_rad = n_r * (6 * KR + 4 * NR * KR + 6 * NR + 30)
+    f_gather = 30 * n_a
+    return {"fused": f_desc + f_ann + f_ang, "gather": f_rad + f_gather, "desc": f_desc, "ann": f_ann,
+            "ang": f_ang, "rad": f_rad}
+
+
+def measured_traffic(kernel_prefix: str, atoms: int):
+    """HBM bytes per dispatch of the kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), when it covers this
+    workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "cu256k_hbm_traffic.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if d.get("atoms") != atoms:
+        return None
+    for name, v in d["kernels"].items():
+        if name.startswith(kernel_prefix):
+            return v["read_bytes"] + v["write_bytes"]
+    return None
+
+
+def cpu_baseline(budget_s=10.0):
+    """The reference's own FP64 CPU evaluator (oracle/_ref, nep_cpu.cpp) when
+    it was built, else the FP64 oracle port, timed on one host core on a
+    6,912-atom sample of the same Cu configuration (forces + velocity Verlet)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    W = importlib.import_module("dmg-md_amd.workloads")
+    s = W.copper_fcc(n=12, seed=7)
+    kind = "port"
+    try:
+        import ref
+        if ref.available():
+            ev = ref.RefNEP(MODEL)
+            kind = "reference"
+        else:
+            ev = O.OracleNEP(MODEL)
+    except Exception:
+        ev = O.OracleNEP(MODEL)
+    x, v, m = s["pos"].copy(), s["vel"].copy(), s["mass"]
+    h, pbc = s["box"].h, s["box"].pbc
+    dt = 2.0 / 10.18051
+    f = ev.compute(h, pbc, s["types"], x)["force"]
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        O.velocity_verlet(True, dt, m, x, v, np.ascontiguousarray(f))
+        O.wrap_positions(h, pbc, x)
+        f = ev.compute(h, pbc, s["types"], x)["force"]
+        O.velocity_verlet(False, dt, m, x, v, np.ascontiguousarray(f))
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    n = len(s["types"])
+    return {"value": n * steps / el, "unit": "atom-steps/s", "cores": 1, "kind": kind,
+            "sample": f"Cu fcc 12^3 = {n} atoms, {steps} NVE steps (force + velocity Verlet), "
+                      f"{el:.1f} s on 1 host core, FP64"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cells", type=int, default=40, help="fcc cells per edge (40 -> 256,000 atoms)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (the known-size "
+                         "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 5}))
+        return
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(f"bench.py measures one GPU's share of the domain; --gpus {max(args.gpus, world)} is not "
+              "supported (multi-GPU halo exchange is out of scope for this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    dmgx = importlib.import_module("dmg-md_amd")
+    W = importlib.import_module("dmg-md_amd.workloads")
+    s = W.copper_fcc(n=args.cells, seed=1)
+    N = len(s["types"])
+    nep = dmgx.NEP(MODEL)
+    md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    ens = dict(ensemble="nvt_bdp", t_init=300.0, t_final=300.0, t_coupling=100.0, seed=12345678)
+    print(f"bench: Cu fcc {args.cells}^3 = {N} atoms, warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    md.run(args.warmup, 2.0, **ens)
+    torch.cuda.synchronize()
+    nep.profile_reset()
+    nep.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    th = md.run(args.steps, 2.0, **ens)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nep.set_profiling(False)
+    prof = nep.profile()
+    means = nep.neighbor_means()
+    stats = nep.neighbor_stats()
+    elapsed = t1 - t0
+    value = N * args.steps / elapsed
+    fl = flops_per_atom(nep.info, means["radial"], means["angular"])
+    n_fused, ms_fused = prof["descriptor"]
+    avg_fused = ms_fused / max(n_fused, 1)
+    achieved = fl["fused"] * N / (avg_fused * 1e-3) / 1e12
+    kernel_ms = {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}
+    print(f"bench: {args.steps} steps in {elapsed:.3f} s, T={th[0]:.1f} K, rebuilds={stats['rebuilds']}, "
+          f"per-kernel avg ms {kernel_ms}, mean rows {means}", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 force/descriptor/ANN, f64 positions/integrator/thermo",
+        "data": "synthetic (seeded Cu fcc lattice + 0.03 A jitter, seeded random NEP4 weights)",
+        "config": {
+            "workload": f"Cu fcc {args.cells}^3 cells ({N} atoms, a=3.615 A), NEP4 n_max 6/4 basis 8/8 "
+                        "l_max 4/2/0 rc 5.0/4.0 A, 50 tanh neurons; Bussi NVT 300 K, tau 100 steps, dt 2 fs",
+            "atoms": N, "ensemble": "nvt_bdp", "dt_fs": 2.0, "ann": "fp32",
+            "mean_neighbors": {k: round(v, 2) for k, v in means.items() if k != "centres"},
+            "verlet_rebuilds_in_run": stats["rebuilds"],
+            "kernel_avg_ms": kernel_ms,
+            "final_temperature_K": th[0],
+        },
+        "roofline": {
+            "kernel": "k_descriptor (fused descriptor + ANN + adjoint + angular partial forces)",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": measured_traffic("dmgx::k_descriptor", N),
+            "flop_per_atom": fl["fused"],
+            "avg_launch_ms": avg_fused,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,628 @@
+// engine.cpp -- host orchestration of the NEP step (see engine.hpp).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace dmgx {
+
+namespace {
+
+constexpr double kCoulomb = 14.399645;  // K_C_SP, eV*A (common.cuh:57)
+
+// covalent radii for the typewise ZBL cutoff (nep_utilities.cuh:175-186),
+// a data table that has to be identical to the reference's
+const double kCovalentRadius[94] = {
+  0.426667, 0.613333, 1.6,     1.25333, 1.02667, 1.0,     0.946667, 0.84,    0.853333,
+  0.893333, 1.86667,  1.66667, 1.50667, 1.38667, 1.46667, 1.36,     1.32,    1.28,
+  2.34667,  2.05333,  1.77333, 1.62667, 1.61333, 1.46667, 1.42667,  1.38667, 1.33333,
+  1.32,     1.34667,  1.45333, 1.49333, 1.45333, 1.53333, 1.46667,  1.52,    1.56,
+  2.52,     2.22667,  1.96,    1.85333, 1.76,    1.65333, 1.53333,  1.50667, 1.50667,
+  1.44,     1.53333,  1.64,    1.70667, 1.68,    1.68,    1.64,     1.76,    1.74667,
+  2.78667,  2.34667,  2.16,    1.96,    2.10667, 2.09333, 2.08,     2.06667, 2.01333,
+  2.02667,  2.01333,  2.0,     1.98667, 1.98667, 1.97333, 2.04,     1.94667, 1.82667,
+  1.74667,  1.64,     1.57333, 1.54667, 1.48,    1.49333, 1.50667,  1.76,    1.73333,
+  1.73333,  1.81333,  1.74667, 1.84,    1.89333, 2.68,    2.41333,  2.22667, 2.10667,
+  2.02667,  2.04,     2.05333, 2.06667};
+
+void check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+void dalloc(T** p, size_t n, const char* what)
+{
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    throw std::runtime_error(std::string("device allocation failed: ") + what);
+  }
+}
+
+template <class T>
+void dfree(T** p)
+{
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+}
+
+}  // namespace
+
+void box_inverse(const double* h, double* hi)
+{
+  const double det = h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
+                     h[2] * (h[3] * h[7] - h[4] * h[6]);
+  hi[0] = (h[4] * h[8] - h[5] * h[7]) / det;
+  hi[1] = (h[2] * h[7] - h[1] * h[8]) / det;
+  hi[2] = (h[1] * h[5] - h[2] * h[4]) / det;
+  hi[3] = (h[5] * h[6] - h[3] * h[8]) / det;
+  hi[4] = (h[0] * h[8] - h[2] * h[6]) / det;
+  hi[5] = (h[2] * h[3] - h[0] * h[5]) / det;
+  hi[6] = (h[3] * h[7] - h[4] * h[6]) / det;
+  hi[7] = (h[1] * h[6] - h[0] * h[7]) / det;
+  hi[8] = (h[0] * h[4] - h[1] * h[3]) / det;
+}
+
+double box_volume(const double* h)
+{
+  return std::fabs(h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
+                   h[2] * (h[3] * h[7] - h[4] * h[6]));
+}
+
+// ---------------------------------------------------------------------------
+// Profiler
+// ---------------------------------------------------------------------------
+
+Profiler::~Profiler()
+{
+  for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t Profiler::take()
+{
+  if (used_ == pool_.size()) {
+    hipEvent_t e;
+    check(hipEventCreate(&e), "event create");
+    pool_.push_back(e);
+  }
+  return pool_[used_++];
+}
+
+void Profiler::begin(int kernel, hipStream_t s)
+{
+  if (!on_) return;
+  Rec r{kernel, take(), nullptr};
+  check(hipEventRecord(r.a, s), "event record");
+  recs_.push_back(r);
+  open_ = (int)recs_.size() - 1;
+}
+
+void Profiler::end(hipStream_t s)
+{
+  if (!on_ || open_ < 0) return;
+  recs_[open_].b = take();
+  check(hipEventRecord(recs_[open_].b, s), "event record");
+  open_ = -1;
+}
+
+void Profiler::collect()
+{
+  for (const Rec& r : recs_) {
+    if (!r.b) continue;
+    check(hipEventSynchronize(r.b), "event sync");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, r.a, r.b), "event time");
+    launches_[r.kernel] += 1;
+    total_[r.kernel] += ms;
+  }
+  recs_.clear();
+  used_ = 0;
+}
+
+void Profiler::read(int kernel, int* launches, double* total_ms)
+{
+  collect();
+  if (kernel < 0 || kernel >= kProfCount) throw std::invalid_argument("unknown kernel id");
+  *launches = launches_[kernel];
+  *total_ms = total_[kernel];
+}
+
+void Profiler::reset()
+{
+  collect();
+  for (int k = 0; k < kProfCount; ++k) {
+    launches_[k] = 0;
+    total_[k] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// NepEngine
+// ---------------------------------------------------------------------------
+
+NepEngine::NepEngine(const std::string& path)
+{
+  model_.load(path);
+  const NepModel& m = model_;
+  NepModel::Image im = m.device_image();
+  // ZBL pair table [T][T][12]: a_inv, zizj, r_in, r_out, (a_k, b_k) x4
+  const int T = m.num_types;
+  std::vector<float> zt((size_t)T * T * 12, 0.0f);
+  if (m.zbl_enabled) {
+    const double uni[8] = {0.18175, 3.1998, 0.50986, 0.94229, 0.28022, 0.4029, 0.02817, 0.20162};
+    for (int a = 0; a < T; ++a)
+      for (int b = 0; b < T; ++b) {
+        float* z = &zt[(size_t)(a * T + b) * 12];
+        const int za = m.Z[a], zb = m.Z[b];
+        z[0] = (float)((std::pow((double)za, 0.23) + std::pow((double)zb, 0.23)) * 2.134563);
+        z[1] = (float)(kCoulomb * za * zb);
+        double r1 = m.zbl_rc_inner, r2 = m.zbl_rc_outer;
+        double ab[8];
+        std::copy(uni, uni + 8, ab);
+        if (m.zbl_flexible) {
+          const int lo = std::min(a, b), hi = std::max(a, b);
+          const int idx = lo * T - (lo * (lo - 1)) / 2 + (hi - lo);
+          const double* p = &m.zbl_para[10 * idx];
+          r1 = p[0];
+          r2 = p[1];
+          std::copy(p + 2, p + 10, ab);
+        } else if (m.zbl_typewise) {
+          r2 = std::min((kCovalentRadius[za - 1] + kCovalentRadius[zb - 1]) * m.zbl_factor, r2);
+          r1 = 0.0;
+        }
+        z[2] = (float)r1;
+        z[3] = (float)r2;
+        for (int k = 0; k < 8; ++k) z[4 + k] = (float)ab[k];
+      }
+  }
+  const size_t zoff = im.data.size();
+  im.data.insert(im.data.end(), zt.begin(), zt.end());
+  dalloc(&params_, im.data.size(), "parameters");
+  check(hipMemcpy(params_, im.data.data(), im.data.size() * sizeof(float), hipMemcpyHostToDevice),
+        "upload parameters");
+  upload_angular_constants(derive_angular_constants());
+  dm_.T = T;
+  dm_.nR = m.nR;
+  dm_.nA = m.nA;
+  dm_.kR = m.kR;
+  dm_.kA = m.kA;
+  dm_.L = m.L_max;
+  dm_.has222 = m.has222;
+  dm_.has1111 = m.has1111;
+  dm_.numL = m.num_L;
+  dm_.D = m.dim;
+  dm_.H = m.H;
+  dm_.version = m.version;
+  dm_.zbl = m.zbl_enabled ? 1 : 0;
+  for (int t = 0; t < kMaxTypes; ++t) {
+    dm_.rcR[t] = t < T ? m.rcR[t] : 0.0;
+    dm_.rcA[t] = t < T ? m.rcA[t] : 0.0;
+  }
+  dm_.w0 = params_ + im.off_w0;
+  dm_.b0 = params_ + im.off_b0;
+  dm_.w1 = params_ + im.off_w1;
+  dm_.w1x = params_ + im.off_w1x;
+  dm_.b1 = params_ + im.off_b1;
+  dm_.cR = params_ + im.off_cR;
+  dm_.cA = params_ + im.off_cA;
+  dm_.qs = params_ + im.off_qs;
+  dm_.zblp = params_ + zoff;
+  check(hipHostMalloc((void**)&hflags_, 8 * sizeof(int), hipHostMallocDefault), "pinned flags");
+  dalloc(&w_.flags, 8, "flags");
+  check(hipMemset(w_.flags, 0, 8 * sizeof(int)), "clear flags");
+}
+
+NepEngine::~NepEngine()
+{
+  free_all();
+  dfree(&params_);
+  dfree(&w_.flags);
+  if (hflags_) (void)hipHostFree(hflags_);
+}
+
+void NepEngine::free_all()
+{
+  dfree(&w_.cell_of);
+  dfree(&w_.cell_atoms);
+  dfree(&w_.vn);
+  dfree(&w_.vl);
+  dfree(&w_.vrev);
+  dfree(&w_.x0);
+  dfree(&w_.nr);
+  dfree(&w_.na);
+  dfree(&w_.edge);
+  dfree(&w_.eq);
+  dfree(&w_.vsa);
+  dfree(&w_.bR);
+  dfree(&w_.f12);
+  dfree(&w_.f12t);
+  dfree(&w_.cell_count);
+  dfree(&w_.cell_start);
+  dfree(&w_.cell_fill);
+  w_.cap = 0;
+  w_.ncell_cap = 0;
+  have_list_ = false;
+}
+
+void NepEngine::allocate(int capacity, int mv)
+{
+  const size_t cap = (size_t)std::max(capacity, 1);
+  const size_t rows = cap * (size_t)mv;
+  dalloc(&w_.cell_of, cap, "cell_of");
+  dalloc(&w_.cell_atoms, cap, "cell_atoms");
+  dalloc(&w_.vn, cap, "vn");
+  dalloc(&w_.vl, rows, "verlet rows");
+  dalloc(&w_.vrev, rows, "verlet reverse");
+  dalloc(&w_.x0, 3 * cap, "x0");
+  dalloc(&w_.nr, cap, "nr");
+  dalloc(&w_.na, cap, "na");
+  dalloc(&w_.edge, rows, "edges");
+  dalloc(&w_.eq, rows, "edge reverse");
+  dalloc(&w_.vsa, rows, "verlet->angular");
+  check(hipMemset(w_.vsa, 0xff, rows * sizeof(int)), "clear verlet->angular");
+  dalloc(&w_.bR, cap * model_.num_types * model_.kR, "radial adjoint");
+  dalloc(&w_.f12, rows, "angular partials");
+  dalloc(&w_.f12t, rows, "angular partials (transposed)");
+  w_.cap = (int)cap;
+  w_.MV = mv;
+  w_.MR = mv;
+  w_.MA = mv;
+  have_list_ = false;
+}
+
+void NepEngine::reserve(int capacity)
+{
+  if (capacity < 0) throw std::invalid_argument("negative capacity");
+  if (capacity <= w_.cap && w_.MV > 0) return;
+  const int mv = std::max(w_.MV, (int)std::ceil(model_.MN_R * std::pow((model_.rcR_max + kSkin) / model_.rcR_max, 3)));
+  allocate(std::max(capacity, w_.cap), std::max(mv, 8));
+}
+
+void NepEngine::ensure_cells(int ncell)
+{
+  if (ncell <= w_.ncell_cap) return;
+  dalloc(&w_.cell_count, (size_t)ncell, "cell counts");
+  dalloc(&w_.cell_start, (size_t)ncell + 1, "cell starts");
+  dalloc(&w_.cell_fill, (size_t)ncell, "cell fill");
+  check(hipMemset(w_.cell_fill, 0, sizeof(int) * ncell), "clear cell fill");
+  w_.ncell_cap = ncell;
+}
+
+DevBox NepEngine::make_devbox(const Box& box) const
+{
+  DevBox b{};
+  std::memcpy(b.h, box.h, sizeof(b.h));
+  box_inverse(box.h, b.hi);
+  for (int d = 0; d < 3; ++d) b.pbc[d] = box.pbc[d] ? 1 : 0;
+  const double* h = box.h;
+  b.ortho = (h[1] == 0 && h[2] == 0 && h[3] == 0 && h[5] == 0 && h[6] == 0 && h[7] == 0) ? 1 : 0;
+  const double vol = box_volume(h);
+  if (!(vol > 0.0) || !std::isfinite(vol)) throw std::invalid_argument("box has non-positive volume");
+  const double a[3] = {h[0], h[3], h[6]}, bb[3] = {h[1], h[4], h[7]}, c[3] = {h[2], h[5], h[8]};
+  const double* pairs[3][2] = {{bb, c}, {c, a}, {a, bb}};
+  const double edge = 0.5 * (model_.rcR_max + kSkin);
+  for (int d = 0; d < 3; ++d) {
+    const double* u = pairs[d][0];
+    const double* v = pairs[d][1];
+    const double x = u[1] * v[2] - u[2] * v[1], y = u[2] * v[0] - u[0] * v[2], z = u[0] * v[1] - u[1] * v[0];
+    b.thick[d] = vol / std::sqrt(x * x + y * y + z * z);
+    b.nc[d] = std::max(1, (int)std::floor(b.thick[d] / edge));
+  }
+  return b;
+}
+
+void NepEngine::rebuild(const DevBox& b, int N, int nd1, int nd2, const double* pos, const uint64_t* gid,
+                        hipStream_t s)
+{
+  const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
+  ensure_cells(ncell);
+  const double rv = model_.rcR_max + kSkin;
+  const double rv2 = rv * rv;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    check(hipMemsetAsync(w_.flags, 0, 8 * sizeof(int), s), "clear flags");
+    check(launch_cell_count(b, N, pos, w_, s), "cell count");
+    check(launch_cell_scan(ncell, w_, s), "cell scan");
+    check(launch_cell_fill(N, w_, s), "cell fill");
+    check(launch_verlet_build(b, N, nd1, nd2, pos, rv2, w_, s), "verlet build");
+    check(hipMemcpyAsync(hflags_, w_.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s), "read flags");
+    check(hipStreamSynchronize(s), "verlet build");
+    if (hflags_[kFlagOutOfBox])
+      throw std::invalid_argument("a position lies outside the box in a non-periodic direction");
+    if (hflags_[kFlagVerletOverflow] > w_.MV) {
+      const int mv = (int)std::ceil(hflags_[kFlagVerletOverflow] * 1.2) + 8;
+      if (mv > 4096) throw std::runtime_error("Verlet row overflow: more than 4096 neighbours within rc + skin");
+      allocate(w_.cap, mv);
+      continue;
+    }
+    check(launch_verlet_sort(N, nd1, nd2, gid, w_, s), "verlet sort");
+    check(launch_verlet_reverse(N, nd1, nd2, gid, w_, s), "verlet reverse");
+    check(launch_save_positions(N, pos, w_, s), "save positions");
+    ++rebuilds_;
+    have_list_ = true;
+    return;
+  }
+  throw std::runtime_error("Verlet rows kept overflowing");
+}
+
+void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                        const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
+                        unsigned flags, hipStream_t s)
+{
+  if (N < 0 || n1 < 0 || n2 < n1 || n2 > N || nd1 < 0 || nd2 < nd1 || nd2 > N)
+    throw std::invalid_argument("centre range is outside [0, N)");
+  if (n1 < nd1 || n2 > nd2) throw std::invalid_argument("force centres must be dependency centres");
+  if (N > 0 && (!type || !pos || !pe || !force || !virial)) throw std::invalid_argument("null array");
+  const bool by_gid = (flags & 8u) != 0;
+  if (by_gid && !gid) throw std::invalid_argument("DMGX_SORT_BY_GID needs global_id");
+  if (N == 0 || nd2 == nd1) return;
+  DevBox b = make_devbox(box);
+  for (int d = 0; d < 3; ++d)
+    if (b.pbc[d] && b.thick[d] <= 2.5 * (model_.rcR_max + kSkin))
+      throw std::invalid_argument(
+          "box thinner than 2.5 (rc + skin) in a periodic direction (use the small-box path)");
+  const long long ncell = (long long)b.nc[0] * b.nc[1] * b.nc[2];
+  if (ncell > (1LL << 28)) throw std::invalid_argument("cell grid too large");
+  if (N > w_.cap || w_.MV == 0) reserve(std::max(N, (int)(w_.cap * 1.25)));
+  bool need = !have_list_ || box_changed_ || N != last_N_ || nd1 != last_nd1_ || nd2 != last_nd2_ ||
+              by_gid != last_gid_sort_ || (flags & 2u) || std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
+              std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0;
+  if (!need) {
+    check(hipMemsetAsync(w_.flags + kFlagMoved, 0, sizeof(int), s), "clear moved");
+    prof_.begin(kProfCheck, s);
+    check(launch_check_moved(b, N, pos, w_, s), "displacement check");
+    prof_.end(s);
+    check(hipMemcpyAsync(hflags_, w_.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s), "read flags");
+    check(hipStreamSynchronize(s), "displacement check");
+    if (hflags_[kFlagBadType]) {
+      check(hipMemsetAsync(w_.flags + kFlagBadType, 0, sizeof(int), s), "clear flag");
+      throw std::invalid_argument("an atom type lies outside [0, num_types) (previous call)");
+    }
+    need = hflags_[kFlagMoved] != 0;
+  }
+  if (need) {
+    prof_.begin(kProfRebuild, s);
+    rebuild(b, N, nd1, nd2, pos, by_gid ? gid : nullptr, s);
+    prof_.end(s);
+    box_changed_ = false;
+    last_N_ = N;
+    last_nd1_ = nd1;
+    last_nd2_ = nd2;
+    last_gid_sort_ = by_gid;
+    std::memcpy(last_h_, box.h, sizeof(last_h_));
+    std::memcpy(last_pbc_, box.pbc, sizeof(last_pbc_));
+  }
+  const bool acc = (flags & 1u) != 0;
+  prof_.begin(kProfFilter, s);
+  check(launch_filter(b, dm_, N, nd1, nd2, type, pos, w_, s), "filter");
+  prof_.end(s);
+  prof_.begin(kProfDescriptor, s);
+  check(launch_descriptor(dm_, N, nd1, nd2, type, pe, acc, w_, s), "descriptor");
+  prof_.end(s);
+  prof_.begin(kProfGather, s);
+  check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, pe, force, virial, acc, w_, s), "gather");
+  prof_.end(s);
+  last_N_stats_ = N;
+  last_c1_ = nd1;
+  last_c2_ = nd2;
+}
+
+void NepEngine::stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds)
+{
+  const int n = last_c2_ - last_c1_;
+  int mr = 0, ma = 0, mv = 0;
+  if (n > 0 && have_list_) {
+    std::vector<int> a(n), b(n), c(n);
+    check(hipDeviceSynchronize(), "sync");
+    check(hipMemcpy(a.data(), w_.nr + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
+    check(hipMemcpy(b.data(), w_.na + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
+    check(hipMemcpy(c.data(), w_.vn + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
+    mr = *std::max_element(a.begin(), a.end());
+    ma = *std::max_element(b.begin(), b.end());
+    mv = *std::max_element(c.begin(), c.end());
+  }
+  if (max_radial) *max_radial = mr;
+  if (max_angular) *max_angular = ma;
+  if (max_verlet) *max_verlet = mv;
+  if (rebuilds) *rebuilds = rebuilds_;
+}
+
+void NepEngine::totals(long long* sum_radial, long long* sum_angular, long long* sum_verlet, int* centres)
+{
+  const int n = last_c2_ - last_c1_;
+  long long a = 0, b = 0, c = 0;
+  if (n > 0 && have_list_) {
+    std::vector<int> r(n), g(n), v(n);
+    check(hipDeviceSynchronize(), "sync");
+    check(hipMemcpy(r.data(), w_.nr + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
+    check(hipMemcpy(g.data(), w_.na + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
+    check(hipMemcpy(v.data(), w_.vn + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
+    for (int k = 0; k < n; ++k) {
+      a += r[k];
+      b += g[k];
+      c += v[k];
+    }
+  }
+  if (sum_radial) *sum_radial = a;
+  if (sum_angular) *sum_angular = b;
+  if (sum_verlet) *sum_verlet = c;
+  if (centres) *centres = n;
+}
+
+void NepEngine::copy_verlet_rows(int c1, int c2, int max_nb, int* counts, int* rows)
+{
+  if (!have_list_) throw std::runtime_error("no neighbour list yet");
+  if (c1 < last_c1_ || c2 > last_c2_ || c2 < c1) throw std::invalid_argument("rows outside the centre range");
+  const int N = last_N_stats_;
+  check(hipDeviceSynchronize(), "sync");
+  std::vector<int> vn(N), vl((size_t)w_.MV * N);
+  check(hipMemcpy(vn.data(), w_.vn, N * sizeof(int), hipMemcpyDeviceToHost), "copy");
+  check(hipMemcpy(vl.data(), w_.vl, vl.size() * sizeof(int), hipMemcpyDeviceToHost), "copy");
+  for (int i = c1; i < c2; ++i) {
+    const int n = std::min(vn[i], w_.MV);
+    counts[i - c1] = vn[i];
+    if (n > max_nb) throw std::invalid_argument("max_nb too small");
+    std::vector<int> r(n);
+    for (int p = 0; p < n; ++p) r[p] = vl[(size_t)p * N + i];
+    std::sort(r.begin(), r.end());
+    std::copy(r.begin(), r.end(), rows + (size_t)(i - c1) * max_nb);
+  }
+}
+
+void NepEngine::copy_pair_rows(int which, int c1, int c2, int max_nb, int* counts, int* rows)
+{
+  if (!have_list_) throw std::runtime_error("no neighbour list yet");
+  if (c1 < last_c1_ || c2 > last_c2_ || c2 < c1) throw std::invalid_argument("rows outside the centre range");
+  const int N = last_N_stats_;
+  check(hipDeviceSynchronize(), "sync");
+  std::vector<int> nr(N), na(N);
+  std::vector<float4> e((size_t)w_.MR * N);
+  check(hipMemcpy(nr.data(), w_.nr, N * sizeof(int), hipMemcpyDeviceToHost), "copy");
+  check(hipMemcpy(na.data(), w_.na, N * sizeof(int), hipMemcpyDeviceToHost), "copy");
+  check(hipMemcpy(e.data(), w_.edge, e.size() * sizeof(float4), hipMemcpyDeviceToHost), "copy");
+  for (int i = c1; i < c2; ++i) {
+    const int n = which == 0 ? nr[i] : na[i];
+    counts[i - c1] = n;
+    if (n > max_nb) throw std::invalid_argument("max_nb too small");
+    std::vector<int> r(n);
+    for (int k = 0; k < n; ++k) {
+      const int slot = k < na[i] ? k : w_.MR - 1 - (k - na[i]);
+      float w = e[(size_t)slot * N + i].w;
+      int j;
+      std::memcpy(&j, &w, sizeof(int));
+      r[k] = j;
+    }
+    std::sort(r.begin(), r.end());
+    std::copy(r.begin(), r.end(), rows + (size_t)(i - c1) * max_nb);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MdState
+// ---------------------------------------------------------------------------
+
+MdState::MdState(NepEngine* nep, const Box& box, int N, const int* type, const double* mass, const double* pos,
+                 const double* vel)
+    : nep_(nep), box_(box), N_(N)
+{
+  if (N <= 0) throw std::invalid_argument("MD needs at least one atom");
+  for (int i = 0; i < N; ++i) {
+    if (type[i] < 0 || type[i] >= nep->model().num_types) throw std::invalid_argument("atom type out of range");
+    if (!(mass[i] > 0.0)) throw std::invalid_argument("masses must be positive");
+  }
+  check(hipStreamCreate(&stream_), "stream");
+  dalloc(&pos_, 3 * (size_t)N, "positions");
+  dalloc(&vel_, 3 * (size_t)N, "velocities");
+  dalloc(&force_, 3 * (size_t)N, "forces");
+  dalloc(&pe_, (size_t)N, "potential");
+  dalloc(&vir_, 9 * (size_t)N, "virial");
+  dalloc(&mass_, (size_t)N, "mass");
+  dalloc(&type_, (size_t)N, "type");
+  dalloc(&tw_.partial, (size_t)thermo_blocks(N) * 16, "thermo partials");
+  dalloc(&tw_.thermo, 8, "thermo");
+  check(hipMemcpy(pos_, pos, 3 * sizeof(double) * N, hipMemcpyHostToDevice), "upload");
+  check(hipMemcpy(vel_, vel, 3 * sizeof(double) * N, hipMemcpyHostToDevice), "upload");
+  check(hipMemcpy(mass_, mass, sizeof(double) * N, hipMemcpyHostToDevice), "upload");
+  check(hipMemcpy(type_, type, sizeof(int) * N, hipMemcpyHostToDevice), "upload");
+  check(hipMemset(force_, 0, 3 * sizeof(double) * N), "clear");
+  check(hipMemset(pe_, 0, sizeof(double) * N), "clear");
+  check(hipMemset(vir_, 0, 9 * sizeof(double) * N), "clear");
+  nep_->reserve(N);
+}
+
+MdState::~MdState()
+{
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  dfree(&pos_);
+  dfree(&vel_);
+  dfree(&force_);
+  dfree(&pe_);
+  dfree(&vir_);
+  dfree(&mass_);
+  dfree(&type_);
+  dfree(&tw_.partial);
+  dfree(&tw_.thermo);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void MdState::arrays(double** pos, double** vel, double** force, double** pe, double** vir, double** mass,
+                     int** type)
+{
+  if (pos) *pos = pos_;
+  if (vel) *vel = vel_;
+  if (force) *force = force_;
+  if (pe) *pe = pe_;
+  if (vir) *vir = vir_;
+  if (mass) *mass = mass_;
+  if (type) *type = type_;
+}
+
+void MdState::force(unsigned flags)
+{
+  DevBox b{};
+  std::memcpy(b.h, box_.h, sizeof(b.h));
+  box_inverse(box_.h, b.hi);
+  for (int d = 0; d < 3; ++d) b.pbc[d] = box_.pbc[d];
+  check(launch_wrap(b, N_, pos_, stream_), "wrap");
+  nep_->compute(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
+}
+
+void MdState::thermo(double* out)
+{
+  Profiler& p = nep_->profiler();
+  p.begin(kProfThermo, stream_);
+  check(launch_thermo(N_, box_volume(box_.h), mass_, pe_, vel_, vir_, tw_, stream_), "thermo");
+  p.end(stream_);
+  check(hipMemcpyAsync(out, tw_.thermo, 8 * sizeof(double), hipMemcpyDeviceToHost, stream_), "thermo copy");
+  check(hipStreamSynchronize(stream_), "thermo");
+}
+
+void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out)
+{
+  if (steps < 0) throw std::invalid_argument("negative step count");
+  const double dt = dt_fs / kTimeUnit;
+  Controller ctl(ens, N_, dt);
+  double th[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (!have_force_) {
+    force(flags);
+    have_force_ = true;
+  }
+  for (int step = 0; step < steps; ++step) {
+    if (ctl.needs_pre_force_ke()) {
+      thermo(th);
+      const double ke = 0.5 * th[0] * 3.0 * N_ * kBoltzmann;
+      if (auto f = ctl.before_force(dt, ke, step, steps))
+        check(launch_scale(3 * N_, *f, vel_, stream_), "scale velocity");
+    }
+    Profiler& p = nep_->profiler();
+    p.begin(kProfIntegrate, stream_);
+    check(launch_velocity_verlet(1, N_, dt, mass_, pos_, vel_, force_, stream_), "vv1");
+    p.end(stream_);
+    force(flags);
+    p.begin(kProfIntegrate, stream_);
+    check(launch_velocity_verlet(0, N_, dt, mass_, pos_, vel_, force_, stream_), "vv2");
+    p.end(stream_);
+    const bool last = step == steps - 1;
+    if (ctl.needs_thermo() || last) thermo(th);
+    if (auto f = ctl.after_force(th, step, steps)) check(launch_scale(3 * N_, *f, vel_, stream_), "scale velocity");
+    if (ctl.has_barostat()) {
+      const double s = ctl.barostat_scale(th, box_volume(box_.h), step, steps);
+      if (s != 1.0) {
+        for (double& v : box_.h) v *= s;
+        check(launch_scale(3 * N_, s, pos_, stream_), "scale positions");
+        nep_->notify_box_changed();
+      }
+    }
+  }
+  if (steps == 0) thermo(th);
+  if (thermo_out) std::memcpy(thermo_out, th, sizeof(th));
+}
+
+}  // namespace dmgx

@@ -1,0 +1,130 @@
+// engine.hpp -- host orchestration of the NEP step on one GPU.
+//
+// NepEngine mirrors the reference NEP object (src/gpumd_compat/nep.cuh):
+// parse once, size workspaces, compute(box, ranges, SoA arrays) with a
+// Verlet list reused across calls until an atom moved more than skin/2 or the
+// box changed (neighbor.cu:537-582).  MdState mirrors run_segment
+// (src/runtime.cu:680-849) for one GPU without MPI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ensemble.hpp"
+#include "kernels.hpp"
+#include "nep_model.hpp"
+
+struct dmgx_box;
+
+namespace dmgx {
+
+struct Box {
+  double h[9];
+  int pbc[3];
+};
+
+// Per-kernel HIP-event timing on the launch stream (off unless enabled).
+enum ProfKernel { kProfFilter = 0, kProfDescriptor, kProfGather, kProfRebuild, kProfCheck, kProfIntegrate,
+                  kProfThermo, kProfCount };
+
+class Profiler {
+ public:
+  ~Profiler();
+  void enable(bool on) { on_ = on; }
+  bool on() const { return on_; }
+  // returns an event index pair to record around a launch
+  void begin(int kernel, hipStream_t s);
+  void end(hipStream_t s);
+  // synchronises, folds recorded pairs into the totals and clears them
+  void collect();
+  void read(int kernel, int* launches, double* total_ms);
+  void reset();
+
+ private:
+  bool on_ = false;
+  struct Rec {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> pool_;
+  size_t used_ = 0;
+  int open_ = -1;
+  int launches_[kProfCount] = {};
+  double total_[kProfCount] = {};
+  hipEvent_t take();
+};
+
+class NepEngine {
+ public:
+  explicit NepEngine(const std::string& path);
+  ~NepEngine();
+  NepEngine(const NepEngine&) = delete;
+  NepEngine& operator=(const NepEngine&) = delete;
+
+  const NepModel& model() const { return model_; }
+  void reserve(int capacity);
+  // Throws std::invalid_argument / std::runtime_error; see dmgx_nep_compute.
+  void compute(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type, const double* pos,
+               const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
+               hipStream_t stream);
+  void notify_box_changed() { box_changed_ = true; }
+  void stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds);
+  void totals(long long* sum_radial, long long* sum_angular, long long* sum_verlet, int* centres);
+  void copy_verlet_rows(int c1, int c2, int max_nb, int* counts, int* rows);
+  void copy_pair_rows(int which, int c1, int c2, int max_nb, int* counts, int* rows);
+  Profiler& profiler() { return prof_; }
+
+ private:
+  DevBox make_devbox(const Box& box) const;
+  void ensure_cells(int ncell);
+  void allocate(int capacity, int mv);
+  void free_all();
+  void rebuild(const DevBox& b, int N, int nd1, int nd2, const double* pos, const uint64_t* gid,
+               hipStream_t s);
+
+  NepModel model_;
+  DevModel dm_{};
+  float* params_ = nullptr;
+  DevWork w_{};
+  int* hflags_ = nullptr;  // pinned host mirror of w_.flags
+  bool have_list_ = false, box_changed_ = true, last_gid_sort_ = false;
+  int last_N_ = -1, last_nd1_ = -1, last_nd2_ = -1;
+  double last_h_[9] = {};
+  int last_pbc_[3] = {};
+  int rebuilds_ = 0;
+  int last_N_stats_ = 0, last_c1_ = 0, last_c2_ = 0;
+  Profiler prof_;
+};
+
+class MdState {
+ public:
+  MdState(NepEngine* nep, const Box& box, int N, const int* type, const double* mass, const double* pos,
+          const double* vel);
+  ~MdState();
+  MdState(const MdState&) = delete;
+  MdState& operator=(const MdState&) = delete;
+  void run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out);
+  void arrays(double** pos, double** vel, double** force, double** pe, double** vir, double** mass, int** type);
+  const Box& box() const { return box_; }
+
+ private:
+  void force(unsigned flags);
+  void thermo(double* out);
+  NepEngine* nep_;
+  Box box_;
+  int N_;
+  hipStream_t stream_ = nullptr;
+  double *pos_ = nullptr, *vel_ = nullptr, *force_ = nullptr, *pe_ = nullptr, *vir_ = nullptr, *mass_ = nullptr;
+  int* type_ = nullptr;
+  DevWork tw_{};  // thermo scratch only
+  bool have_force_ = false;
+};
+
+double box_volume(const double* h);
+void box_inverse(const double* h, double* hi);
+
+}  // namespace dmgx

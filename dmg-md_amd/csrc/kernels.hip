@@ -1,0 +1,1295 @@
+// kernels.hip -- HIP kernels of the NEP molecular-dynamics step for gfx950.
+//
+// Design (CDNA4, wave64):
+//  * Verlet list with a 1 A skin, rebuilt only when an atom moved more than
+//    skin/2 (neighbor.cu:446-582); rows hold slot indices, sorted ascending
+//    (or by global id), with a precomputed reverse position per entry.
+//  * Every step a filter pass turns each Verlet row into one compact edge row:
+//    angular edges first (slots 0..na-1, ascending), radial-only edges from the
+//    end of the row downward, each edge a float4 (FP64 minimum-image r_ij
+//    rounded to float, neighbour slot).  All later kernels stream these rows
+//    with coalesced loads instead of re-gathering positions.
+//  * One fused per-atom kernel does descriptor -> ANN -> adjoints -> angular
+//    partial forces: the angular moments s[n][lm] stay in registers, are
+//    turned in place into the adjoint dE/ds, and each angular edge's
+//    dE_i/dr_ij is written both at i's slot and (transposed) at j's slot for
+//    i, so the gather kernel reads both directions with coalesced loads.
+//  * The gather kernel forms the radial pair force from both centres'
+//    basis-space adjoints, adds the angular partials and ZBL, and writes FP64
+//    force/virial.  No floating-point atomics anywhere: results are
+//    deterministic and independent of launch order.
+//  * Integrator and thermo in FP64, thermo sums compensated (two-sum).
+#include "kernels.hpp"
+
+#include <cmath>
+
+namespace dmgx {
+
+__constant__ float c_c3[24];
+__constant__ float c_zc[kMaxL + 1][kMaxL + 1][kMaxL + 1];
+__constant__ float c_c4[5];
+__constant__ float c_c5;
+
+void upload_angular_constants(const AngularConstants& a)
+{
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_c3), a.c3, sizeof(a.c3));
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_zc), a.zc, sizeof(a.zc));
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_c4), a.c4, sizeof(a.c4));
+  (void)hipMemcpyToSymbol(HIP_SYMBOL(c_c5), &a.c5, sizeof(a.c5));
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// Workgroup -> atom tile mapping that keeps consecutive tiles on one XCD:
+// the dispatcher deals workgroups round-robin over the 8 XCDs, so workgroup
+// b runs on XCD b%8; tile = (b%8)*per_xcd + b/8 gives each XCD one contiguous
+// (spatially coherent) range of atoms and its own L2 working set.
+__device__ __forceinline__ int xcd_tile(int nblocks)
+{
+  const int per = (nblocks + 7) >> 3;
+  const int b = blockIdx.x;
+  return (b & 7) * per + (b >> 3);
+}
+
+__host__ inline int xcd_grid(int nblocks) { return ((nblocks + 7) >> 3) << 3; }
+
+// FP64 minimum image (box.cuh:77-123), no FMA contraction so that the
+// predicate is bit-identical to the FP64 oracle.
+__device__ __forceinline__ void mic(const DevBox& b, double& x, double& y, double& z)
+{
+#pragma clang fp contract(off)
+  if (b.ortho) {
+    if (b.pbc[0]) {
+      const double L = b.h[0];
+      if (x < -L * 0.5) x += L;
+      else if (x > L * 0.5) x -= L;
+    }
+    if (b.pbc[1]) {
+      const double L = b.h[4];
+      if (y < -L * 0.5) y += L;
+      else if (y > L * 0.5) y -= L;
+    }
+    if (b.pbc[2]) {
+      const double L = b.h[8];
+      if (z < -L * 0.5) z += L;
+      else if (z > L * 0.5) z -= L;
+    }
+  } else {
+    double sx = b.hi[0] * x + b.hi[1] * y + b.hi[2] * z;
+    double sy = b.hi[3] * x + b.hi[4] * y + b.hi[5] * z;
+    double sz = b.hi[6] * x + b.hi[7] * y + b.hi[8] * z;
+    if (b.pbc[0]) sx -= rint(sx);
+    if (b.pbc[1]) sy -= rint(sy);
+    if (b.pbc[2]) sz -= rint(sz);
+    x = b.h[0] * sx + b.h[1] * sy + b.h[2] * sz;
+    y = b.h[3] * sx + b.h[4] * sy + b.h[5] * sz;
+    z = b.h[6] * sx + b.h[7] * sy + b.h[8] * sz;
+  }
+}
+
+__device__ __forceinline__ double dist2(double x, double y, double z)
+{
+#pragma clang fp contract(off)
+  return (x * x + y * y) + z * z;
+}
+
+// cell coordinates of a position; returns false when a non-periodic
+// coordinate lies outside the box
+__device__ __forceinline__ bool cell_coords(const DevBox& b, double x, double y, double z, int* c)
+{
+  const double s[3] = {b.hi[0] * x + b.hi[1] * y + b.hi[2] * z, b.hi[3] * x + b.hi[4] * y + b.hi[5] * z,
+                       b.hi[6] * x + b.hi[7] * y + b.hi[8] * z};
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double u = s[d];
+    if (b.pbc[d]) {
+      u -= floor(u);
+    } else if (u < 0.0 || u >= 1.0) {
+      ok = false;
+      u = fmin(fmax(u, 0.0), 0.999999999);
+    }
+    int k = (int)(u * b.nc[d]);
+    c[d] = k < 0 ? 0 : (k >= b.nc[d] ? b.nc[d] - 1 : k);
+  }
+  return ok;
+}
+
+// type index clamped into [0, T) (invalid caller types are flagged by the filter)
+__device__ __forceinline__ int safe_type(int t, int T) { return (t < 0 || t >= T) ? 0 : t; }
+
+// cosine cutoff (nep_utilities.cuh:317-342)
+__device__ __forceinline__ void cutoff_fc(float rc, float rcinv, float d, float& fc)
+{
+  fc = d < rc ? 0.5f * cosf(3.14159265f * d * rcinv) + 0.5f : 0.0f;
+}
+__device__ __forceinline__ void cutoff_fc_fcp(float rc, float rcinv, float d, float& fc, float& fcp)
+{
+  if (d < rc) {
+    float sn, cs;
+    sincosf(3.14159265f * d * rcinv, &sn, &cs);
+    fc = 0.5f * cs + 0.5f;
+    fcp = -1.57079633f * sn * rcinv;
+  } else {
+    fc = 0.0f;
+    fcp = 0.0f;
+  }
+}
+
+// Chebyshev radial basis f_k = (T_k(x)+1)/2 fc, x = 2(d/rc-1)^2-1
+template <int K>
+__device__ __forceinline__ void cheb(int kn, float rcinv, float d, float fc, float* f)
+{
+  const float t = d * rcinv - 1.0f;
+  const float x = 2.0f * t * t - 1.0f;
+  const float hfc = 0.5f * fc;
+  float tm2 = 1.0f, tm1 = x;
+  f[0] = fc;
+  if (K > 1) f[1] = (x + 1.0f) * hfc;
+#pragma unroll
+  for (int k = 2; k < K; ++k) {
+    if (k >= kn) break;
+    const float tk = 2.0f * x * tm1 - tm2;
+    tm2 = tm1;
+    tm1 = tk;
+    f[k] = (tk + 1.0f) * hfc;
+  }
+}
+
+// basis and d/dd (T_k' = k U_{k-1})
+template <int K>
+__device__ __forceinline__ void cheb_d(int kn, float rcinv, float d, float fc, float fcp, float* f, float* fp)
+{
+  const float t = d * rcinv - 1.0f;
+  const float x = 2.0f * t * t - 1.0f;
+  const float dx = 4.0f * t * rcinv;  // dx/dd
+  f[0] = fc;
+  fp[0] = fcp;
+  float tm2 = 1.0f, tm1 = x, um2 = 1.0f, um1 = 2.0f * x;
+  if (K > 1) {
+    const float v = 0.5f * (x + 1.0f);
+    f[1] = v * fc;
+    fp[1] = 0.5f * dx * fc + v * fcp;
+  }
+#pragma unroll
+  for (int k = 2; k < K; ++k) {
+    if (k >= kn) break;
+    const float tk = 2.0f * x * tm1 - tm2;
+    const float v = 0.5f * (tk + 1.0f);
+    const float der = 0.5f * k * um1 * dx;  // (T_k'/2) dx/dd, T_k' = k U_{k-1}
+    f[k] = v * fc;
+    fp[k] = der * fc + v * fcp;
+    tm2 = tm1;
+    tm1 = tk;
+    const float uk = 2.0f * x * um1 - um2;
+    um2 = um1;
+    um1 = uk;
+  }
+}
+
+// real solid harmonics Y_{L,m}(u), L=1..LM, packed at L*L-1 (+2m-1 re, +2m im)
+template <int LM>
+__device__ __forceinline__ void harmonics(float ux, float uy, float uz, float* Y)
+{
+  float zp[LM + 1];
+  zp[0] = 1.0f;
+#pragma unroll
+  for (int k = 1; k <= LM; ++k) zp[k] = zp[k - 1] * uz;
+  float pr[LM + 1], pim[LM + 1];
+  pr[0] = 1.0f;
+  pim[0] = 0.0f;
+#pragma unroll
+  for (int k = 1; k <= LM; ++k) {
+    pr[k] = pr[k - 1] * ux - pim[k - 1] * uy;
+    pim[k] = pr[k - 1] * uy + pim[k - 1] * ux;
+  }
+#pragma unroll
+  for (int L = 1; L <= LM; ++L) {
+#pragma unroll
+    for (int m = 0; m <= L; ++m) {
+      float Z = 0.0f;
+#pragma unroll
+      for (int k = (L - m) & 1; k <= L - m; k += 2) Z += c_zc[L][m][k] * zp[k];
+      const int b = L * L - 1;
+      if (m == 0) {
+        Y[b] = Z;
+      } else {
+        Y[b + 2 * m - 1] = Z * pr[m];
+        Y[b + 2 * m] = Z * pim[m];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour build
+// ---------------------------------------------------------------------------
+
+__global__ void k_cell_count(DevBox b, int N, const double* __restrict__ pos, int* __restrict__ cell_of,
+                             int* __restrict__ count, int* __restrict__ flags)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int c[3];
+  if (!cell_coords(b, pos[i], pos[N + i], pos[2 * N + i], c)) atomicOr(&flags[kFlagOutOfBox], 1);
+  const int id = c[0] + b.nc[0] * (c[1] + b.nc[1] * c[2]);
+  cell_of[i] = id;
+  atomicAdd(&count[id], 1);
+}
+
+// three-phase exclusive scan (1024 counts per workgroup)
+__global__ void k_scan_local(int n, const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum)
+{
+  __shared__ int sh[256];
+  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+  int v[4], sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    sum += v[k];
+  }
+  sh[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - sum;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) bsum[blockIdx.x] = sh[255];
+}
+
+__global__ void k_scan_blocks(int nb, int* __restrict__ bsum, int* __restrict__ total_out)
+{
+  __shared__ int sh[1024];
+  // one workgroup of 1024 threads; nb <= 1024*chunk handled by a serial chunk per thread
+  const int chunk = (nb + 1023) / 1024;
+  const int b0 = threadIdx.x * chunk;
+  int sum = 0;
+  for (int k = 0; k < chunk; ++k)
+    if (b0 + k < nb) sum += bsum[b0 + k];
+  sh[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - sum;
+  for (int k = 0; k < chunk; ++k)
+    if (b0 + k < nb) {
+      const int v = bsum[b0 + k];
+      bsum[b0 + k] = run;
+      run += v;
+    }
+  if (threadIdx.x == 1023) *total_out = sh[1023];
+}
+
+__global__ void k_scan_add(int n, int* __restrict__ out, const int* __restrict__ bsum)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += bsum[i >> 10];
+}
+
+__global__ void k_cell_fill(int N, const int* __restrict__ cell_of, const int* __restrict__ start,
+                            int* __restrict__ fill, int* __restrict__ atoms)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int c = cell_of[i];
+  atoms[start[c] + atomicAdd(&fill[c], 1)] = i;
+}
+
+// One thread per centre: scan the 5x5x5 cells around it (cell edge >= (rc+skin)/2)
+// and keep every slot with FP64 d2 < (rc+skin)^2 (neighbor.cu:141-218).
+__global__ __launch_bounds__(kBlock) void k_verlet_build(DevBox b, int N, int c1, int c2,
+                                                         const double* __restrict__ pos, double rv2,
+                                                         DevWork w)
+{
+  const int i = c1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c2) return;
+  const double xi = pos[i], yi = pos[N + i], zi = pos[2 * N + i];
+  int c[3];
+  cell_coords(b, xi, yi, zi, c);
+  const int rng[3] = {b.pbc[0] ? 2 : 2, b.pbc[1] ? 2 : 2, b.pbc[2] ? 2 : 2};
+  int cnt = 0;
+  for (int dz = -rng[2]; dz <= rng[2]; ++dz) {
+    int cz = c[2] + dz;
+    if (b.pbc[2]) cz = (cz + b.nc[2]) % b.nc[2];
+    else if (cz < 0 || cz >= b.nc[2]) continue;
+    for (int dy = -rng[1]; dy <= rng[1]; ++dy) {
+      int cy = c[1] + dy;
+      if (b.pbc[1]) cy = (cy + b.nc[1]) % b.nc[1];
+      else if (cy < 0 || cy >= b.nc[1]) continue;
+      for (int dx = -rng[0]; dx <= rng[0]; ++dx) {
+        int cx = c[0] + dx;
+        if (b.pbc[0]) cx = (cx + b.nc[0]) % b.nc[0];
+        else if (cx < 0 || cx >= b.nc[0]) continue;
+        const int cell = cx + b.nc[0] * (cy + b.nc[1] * cz);
+        const int e = w.cell_start[cell + 1];
+        for (int k = w.cell_start[cell]; k < e; ++k) {
+          const int j = w.cell_atoms[k];
+          if (j == i) continue;
+          double x = pos[j] - xi, y = pos[N + j] - yi, z = pos[2 * N + j] - zi;
+          mic(b, x, y, z);
+          if (dist2(x, y, z) < rv2) {
+            if (cnt < w.MV) w.vl[(size_t)cnt * N + i] = j;
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  w.vn[i] = cnt;
+  if (cnt > w.MV) atomicMax(&w.flags[kFlagVerletOverflow], cnt);
+}
+
+__device__ __forceinline__ bool key_less(const uint64_t* gid, int a, int b)
+{
+  if (gid) {
+    const uint64_t ga = gid[a], gb = gid[b];
+    if (ga != gb) return ga < gb;
+  }
+  return a < b;
+}
+
+// One wavefront per row: rank sort in LDS (keys unique: slot, or (gid, slot)).
+__global__ __launch_bounds__(256) void k_verlet_sort(int N, int c1, int c2, const uint64_t* __restrict__ gid,
+                                                     DevWork w)
+{
+  extern __shared__ int sh[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = c1 + blockIdx.x * 4 + wave;
+  int* row = sh + wave * w.MV;
+  const int n = i < c2 ? w.vn[i] : 0;
+  const int nn = n < w.MV ? n : w.MV;
+  for (int k = lane; k < nn; k += 64) row[k] = w.vl[(size_t)k * N + i];
+  __syncthreads();
+  for (int k = lane; k < nn; k += 64) {
+    const int v = row[k];
+    int rank = 0;
+    for (int t = 0; t < nn; ++t) rank += key_less(gid, row[t], v) ? 1 : 0;
+    w.vl[(size_t)rank * N + i] = v;
+  }
+}
+
+// Position of the centre inside each neighbour's row (binary search on the
+// row key); -1 when the neighbour has no row (outside the centre range).
+__global__ __launch_bounds__(kBlock) void k_verlet_reverse(int N, int c1, int c2,
+                                                           const uint64_t* __restrict__ gid, DevWork w)
+{
+  const int i = c1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c2) return;
+  const int n = min(w.vn[i], w.MV);
+  for (int p = 0; p < n; ++p) {
+    const int j = w.vl[(size_t)p * N + i];
+    int q = -1;
+    if (j >= c1 && j < c2) {
+      int lo = 0, hi = min(w.vn[j], w.MV) - 1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = w.vl[(size_t)mid * N + j];
+        if (v == i) {
+          q = mid;
+          break;
+        }
+        if (key_less(gid, v, i)) lo = mid + 1;
+        else hi = mid - 1;
+      }
+    }
+    w.vrev[(size_t)p * N + i] = q;
+  }
+}
+
+__global__ void k_save_positions(int N, const double* __restrict__ pos, double* __restrict__ x0)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * N) x0[i] = pos[i];
+}
+
+// any atom displaced by more than skin/2 since the last rebuild (neighbor.cu:446-490)
+__global__ void k_check_moved(DevBox b, int N, const double* __restrict__ pos, const double* __restrict__ x0,
+                              double lim2, int* __restrict__ flags)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool moved = false;
+  if (i < N) {
+    double x = pos[i] - x0[i], y = pos[N + i] - x0[N + i], z = pos[2 * N + i] - x0[2 * N + i];
+    mic(b, x, y, z);
+    moved = dist2(x, y, z) > lim2;
+  }
+  if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(&flags[kFlagMoved], 1);
+}
+
+// ---------------------------------------------------------------------------
+// per-step typewise filter (nep.cu:503-561) into compact edge rows
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int N, int c1, int c2, int nblocks,
+                                                   const int* __restrict__ type, const double* __restrict__ pos,
+                                                   DevWork w)
+{
+  const int i = c1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
+  if (i >= c2) return;
+  const double xi = pos[i], yi = pos[N + i], zi = pos[2 * N + i];
+  int ti = type[i];
+  if (ti < 0 || ti >= m.T) {
+    atomicOr(&w.flags[kFlagBadType], 1);
+    ti = 0;
+  }
+  const int nv = min(w.vn[i], w.MV);
+  int na = 0, nro = 0;
+  for (int p = 0; p < nv; ++p) {
+    const int j = w.vl[(size_t)p * N + i];
+    double x = pos[j] - xi, y = pos[N + j] - yi, z = pos[2 * N + j] - zi;
+    mic(b, x, y, z);
+    const double d2 = dist2(x, y, z);
+    int tj = type[j];
+    if (tj < 0 || tj >= m.T) {
+      atomicOr(&w.flags[kFlagBadType], 1);
+      tj = 0;
+    }
+    const double rr = (m.rcR[ti] + m.rcR[tj]) * 0.5;
+    if (!(d2 < rr * rr)) continue;
+    const double ra = (m.rcA[ti] + m.rcA[tj]) * 0.5;
+    const float4 rec = make_float4((float)x, (float)y, (float)z, __int_as_float(j));
+    if (d2 < ra * ra) {
+      if (na < w.MA && na + nro < w.MR) {
+        w.edge[(size_t)na * N + i] = rec;
+        w.eq[(size_t)na * N + i] = w.vrev[(size_t)p * N + i];
+        w.vsa[(size_t)p * N + i] = na;
+      }
+      ++na;
+    } else {
+      if (na + nro < w.MR) w.edge[(size_t)(w.MR - 1 - nro) * N + i] = rec;
+      ++nro;
+    }
+  }
+  w.nr[i] = min(na + nro, w.MR);
+  w.na[i] = min(na, min(w.MA, w.MR));
+  if (na + nro > w.MR) atomicMax(&w.flags[kFlagRadialOverflow], na + nro);
+  if (na > w.MA) atomicMax(&w.flags[kFlagAngularOverflow], na);
+}
+
+// ---------------------------------------------------------------------------
+// fused descriptor + ANN + adjoint + angular partial forces
+// ---------------------------------------------------------------------------
+
+template <int NR, int KR, int NA, int KA, int LM, bool Q222, bool Q1111, bool ONE, bool EXACT>
+struct Cfg {
+  static constexpr int nr = NR, kr = KR, na = NA, ka = KA, lm = LM;
+  static constexpr bool q222 = Q222, q1111 = Q1111, one = ONE, exact = EXACT;
+  static constexpr int nabc = (LM + 1) * (LM + 1) - 1;
+  static constexpr int numl = LM + (Q222 ? 1 : 0) + (Q1111 ? 1 : 0);
+  static constexpr int dim = NR + NA * numl;
+};
+
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1, int c2, int nblocks,
+                                                       const int* __restrict__ type, double* __restrict__ pe,
+                                                       int accumulate, DevWork w)
+{
+  constexpr int NR = C::nr, KR = C::kr, NA = C::na, KA = C::ka, LM = C::lm, NABC = C::nabc, DMAX = C::dim;
+  const int i = c1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
+  if (i >= c2) return;
+  const int nR = C::exact ? NR : m.nR, nA = C::exact ? NA : m.nA;
+  const int kR = C::exact ? KR : m.kR, kA = C::exact ? KA : m.kA;
+  const bool h222 = C::exact ? C::q222 : (m.has222 != 0);
+  const bool h1111 = C::exact ? C::q1111 : (m.has1111 != 0);
+  const int D = C::exact ? DMAX : m.D;
+  const int T = C::one ? 1 : m.T;
+  const int ti = C::one ? 0 : safe_type(type[i], T);
+  float rcRf[kMaxTypes], rcAf[kMaxTypes];
+  // per-type float cutoffs (the FP64 values rounded once)
+  if (C::one) {
+    rcRf[0] = (float)m.rcR[0];
+    rcAf[0] = (float)m.rcA[0];
+  }
+  const int nr = w.nr[i], na = w.na[i];
+  float q[DMAX];
+#pragma unroll
+  for (int d = 0; d < DMAX; ++d) q[d] = 0.0f;
+
+  // ---- radial descriptor: q_n = sum_j sum_k c[ti,tj,n,k] f_k(r_ij)
+  float Fs[KR];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) Fs[k] = 0.0f;
+  for (int e = 0; e < nr; ++e) {
+    const int slot = e < na ? e : w.MR - 1 - (e - na);
+    const float4 r = w.edge[(size_t)slot * N + i];
+    const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+    float rc;
+    int tj = 0;
+    if (C::one) {
+      rc = rcRf[0];
+    } else {
+      tj = safe_type(type[__float_as_int(r.w)], T);
+      rc = (float)((m.rcR[ti] + m.rcR[tj]) * 0.5);
+    }
+    const float rcinv = 1.0f / rc;
+    float fc;
+    cutoff_fc(rc, rcinv, d, fc);
+    float f[KR];
+    cheb<KR>(kR, rcinv, d, fc, f);
+    if (C::one) {
+#pragma unroll
+      for (int k = 0; k < KR; ++k)
+        if (k < kR) Fs[k] += f[k];
+    } else {
+      const float* c = m.cR + (size_t)(ti * T + tj) * nR * kR;
+#pragma unroll
+      for (int n = 0; n < NR; ++n) {
+        if (n >= nR) break;
+        float g = 0.0f;
+#pragma unroll
+        for (int k = 0; k < KR; ++k)
+          if (k < kR) g += c[n * kR + k] * f[k];
+        q[n] += g;
+      }
+    }
+  }
+  if (C::one) {
+#pragma unroll
+    for (int n = 0; n < NR; ++n) {
+      if (n >= nR) break;
+      float g = 0.0f;
+#pragma unroll
+      for (int k = 0; k < KR; ++k)
+        if (k < kR) g += m.cR[n * kR + k] * Fs[k];
+      q[n] = g;
+    }
+  }
+
+  // ---- angular moments s[n][lm] = sum_j g_n(r_ij) Y_lm(u_ij)
+  float s[NA][NABC];
+#pragma unroll
+  for (int n = 0; n < NA; ++n)
+#pragma unroll
+    for (int a = 0; a < NABC; ++a) s[n][a] = 0.0f;
+  for (int e = 0; e < na; ++e) {
+    const float4 r = w.edge[(size_t)e * N + i];
+    const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+    const float dinv = 1.0f / d;
+    float rc;
+    int tj = 0;
+    if (C::one) {
+      rc = rcAf[0];
+    } else {
+      tj = safe_type(type[__float_as_int(r.w)], T);
+      rc = (float)((m.rcA[ti] + m.rcA[tj]) * 0.5);
+    }
+    const float rcinv = 1.0f / rc;
+    float fc;
+    cutoff_fc(rc, rcinv, d, fc);
+    float f[KA];
+    cheb<KA>(kA, rcinv, d, fc, f);
+    const float* c = m.cA + (size_t)(ti * T + tj) * nA * kA;
+    float g[NA];
+#pragma unroll
+    for (int n = 0; n < NA; ++n) {
+      g[n] = 0.0f;
+      if (n < nA) {
+#pragma unroll
+        for (int k = 0; k < KA; ++k)
+          if (k < kA) g[n] += c[n * kA + k] * f[k];
+      }
+    }
+    float Y[NABC];
+    harmonics<LM>(r.x * dinv, r.y * dinv, r.z * dinv, Y);
+#pragma unroll
+    for (int n = 0; n < NA; ++n)
+#pragma unroll
+      for (int a = 0; a < NABC; ++a) s[n][a] += g[n] * Y[a];
+  }
+
+  // ---- invariants
+#pragma unroll
+  for (int n = 0; n < NA; ++n) {
+    if (n >= nA) break;
+#pragma unroll
+    for (int L = 1; L <= LM; ++L) {
+      float v = 0.0f;
+#pragma unroll
+      for (int a = L * L - 1; a < (L + 1) * (L + 1) - 1; ++a) v += c_c3[a] * s[n][a] * s[n][a];
+      q[nR + (L - 1) * nA + n] = v;
+    }
+    int li = LM;
+    if (h222) {
+      const float* x = &s[n][3];
+      q[nR + li * nA + n] = c_c4[0] * x[0] * x[0] * x[0] + c_c4[1] * x[0] * (x[1] * x[1] + x[2] * x[2]) +
+                            c_c4[2] * x[0] * (x[3] * x[3] + x[4] * x[4]) +
+                            c_c4[3] * x[3] * (x[2] * x[2] - x[1] * x[1]) + c_c4[4] * x[1] * x[2] * x[4];
+      ++li;
+    }
+    if (h1111) {
+      const float r2 = s[n][0] * s[n][0] + s[n][1] * s[n][1] + s[n][2] * s[n][2];
+      q[nR + li * nA + n] = c_c5 * r2 * r2;
+    }
+  }
+
+  // ---- ANN (nep_utilities.cuh:209-310): E = sum_h w1 tanh(w0.q - b0) - bias
+#pragma unroll
+  for (int d = 0; d < DMAX; ++d)
+    if (d < D) q[d] *= m.qs[d];
+  const int H = m.H;
+  const float* w0 = m.w0 + (size_t)ti * H * D;
+  const float* b0 = m.b0 + (size_t)ti * H;
+  const float* w1 = m.w1 + (size_t)ti * H;
+  float Fp[DMAX];
+#pragma unroll
+  for (int d = 0; d < DMAX; ++d) Fp[d] = 0.0f;
+  float E = 0.0f;
+  for (int h = 0; h < H; ++h) {
+    const float* wr = w0 + (size_t)h * D;
+    float a = 0.0f;
+#pragma unroll
+    for (int d = 0; d < DMAX; ++d)
+      if (d < D) a += wr[d] * q[d];
+    const float x = tanhf(a - b0[h]);
+    E += w1[h] * x;
+    const float gq = w1[h] * (1.0f - x * x);
+#pragma unroll
+    for (int d = 0; d < DMAX; ++d)
+      if (d < D) Fp[d] += gq * wr[d];
+  }
+  E -= m.b1[0];
+  if (m.version == 5) E -= m.w1x[ti];
+#pragma unroll
+  for (int d = 0; d < DMAX; ++d)
+    if (d < D) Fp[d] *= m.qs[d];
+  pe[i] = accumulate ? pe[i] + (double)E : (double)E;
+
+  // ---- radial adjoint in basis space: bR[i][t][k] = sum_n Fp_n c[ti,t,n,k]
+  for (int t = 0; t < T; ++t) {
+    const float* c = m.cR + (size_t)(ti * T + t) * nR * kR;
+    float* out = w.bR + ((size_t)i * T + t) * kR;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      if (k >= kR) break;
+      float v = 0.0f;
+#pragma unroll
+      for (int n = 0; n < NR; ++n)
+        if (n < nR) v += Fp[n] * c[n * kR + k];
+      out[k] = v;
+    }
+  }
+
+  // ---- angular adjoint a[n][lm] = dE/ds[n][lm], written over s
+#pragma unroll
+  for (int n = 0; n < NA; ++n) {
+    if (n >= nA) break;
+    float extra[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) extra[a] = 0.0f;
+    int li = LM;
+    if (h222) {
+      const float F = Fp[nR + li * nA + n];
+      const float* x = &s[n][3];
+      extra[3] = F * (3.0f * c_c4[0] * x[0] * x[0] + c_c4[1] * (x[1] * x[1] + x[2] * x[2]) +
+                      c_c4[2] * (x[3] * x[3] + x[4] * x[4]));
+      extra[4] = F * (2.0f * c_c4[1] * x[0] * x[1] - 2.0f * c_c4[3] * x[3] * x[1] + c_c4[4] * x[2] * x[4]);
+      extra[5] = F * (2.0f * c_c4[1] * x[0] * x[2] + 2.0f * c_c4[3] * x[3] * x[2] + c_c4[4] * x[1] * x[4]);
+      extra[6] = F * (2.0f * c_c4[2] * x[0] * x[3] + c_c4[3] * (x[2] * x[2] - x[1] * x[1]));
+      extra[7] = F * (2.0f * c_c4[2] * x[0] * x[4] + c_c4[4] * x[1] * x[2]);
+      ++li;
+    }
+    if (h1111) {
+      const float F = Fp[nR + li * nA + n];
+      const float r2 = s[n][0] * s[n][0] + s[n][1] * s[n][1] + s[n][2] * s[n][2];
+      const float k5 = 4.0f * c_c5 * F * r2;
+      extra[0] = k5 * s[n][0];
+      extra[1] = k5 * s[n][1];
+      extra[2] = k5 * s[n][2];
+    }
+#pragma unroll
+    for (int L = 1; L <= LM; ++L) {
+      const float F2 = 2.0f * Fp[nR + (L - 1) * nA + n];
+#pragma unroll
+      for (int a = L * L - 1; a < (L + 1) * (L + 1) - 1; ++a) {
+        float v = F2 * c_c3[a] * s[n][a];
+        if (a < 8) v += extra[a];
+        s[n][a] = v;
+      }
+    }
+  }
+
+  // ---- angular partial forces dE_i/dr_ij
+  for (int e = 0; e < na; ++e) {
+    const float4 r = w.edge[(size_t)e * N + i];
+    const int j = __float_as_int(r.w);
+    const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+    const float dinv = 1.0f / d;
+    const float ux = r.x * dinv, uy = r.y * dinv, uz = r.z * dinv;
+    float rc;
+    int tj = 0;
+    if (C::one) {
+      rc = rcAf[0];
+    } else {
+      tj = safe_type(type[j], T);
+      rc = (float)((m.rcA[ti] + m.rcA[tj]) * 0.5);
+    }
+    const float rcinv = 1.0f / rc;
+    float fc, fcp;
+    cutoff_fc_fcp(rc, rcinv, d, fc, fcp);
+    float f[KA], fp[KA];
+    cheb_d<KA>(kA, rcinv, d, fc, fcp, f, fp);
+    const float* c = m.cA + (size_t)(ti * T + tj) * nA * kA;
+    float g[NA], gp[NA];
+#pragma unroll
+    for (int n = 0; n < NA; ++n) {
+      g[n] = 0.0f;
+      gp[n] = 0.0f;
+      if (n < nA) {
+#pragma unroll
+        for (int k = 0; k < KA; ++k)
+          if (k < kA) {
+            g[n] += c[n * kA + k] * f[k];
+            gp[n] += c[n * kA + k] * fp[k];
+          }
+      }
+    }
+    // contract the adjoint with the basis: A = sum_n a g_n, B = sum_n a g'_n,
+    // then P = sum B Y and G = sum A dY/du (Y = Z(uz) Re/Im (ux+i uy)^m)
+    float zp[LM + 1];
+    zp[0] = 1.0f;
+#pragma unroll
+    for (int k = 1; k <= LM; ++k) zp[k] = zp[k - 1] * uz;
+    float pr[LM + 1], pim[LM + 1];
+    pr[0] = 1.0f;
+    pim[0] = 0.0f;
+#pragma unroll
+    for (int k = 1; k <= LM; ++k) {
+      pr[k] = pr[k - 1] * ux - pim[k - 1] * uy;
+      pim[k] = pr[k - 1] * uy + pim[k - 1] * ux;
+    }
+    float P = 0.0f, Gx = 0.0f, Gy = 0.0f, Gz = 0.0f;
+#pragma unroll
+    for (int L = 1; L <= LM; ++L) {
+#pragma unroll
+      for (int mm = 0; mm <= L; ++mm) {
+        float Z = 0.0f, dZ = 0.0f;
+#pragma unroll
+        for (int k = (L - mm) & 1; k <= L - mm; k += 2) {
+          Z += c_zc[L][mm][k] * zp[k];
+          if (k > 0) dZ += c_zc[L][mm][k] * (float)k * zp[k - 1];
+        }
+        const int b = L * L - 1;
+        if (mm == 0) {
+          float A = 0.0f, B = 0.0f;
+#pragma unroll
+          for (int n = 0; n < NA; ++n) {
+            A += s[n][b] * g[n];
+            B += s[n][b] * gp[n];
+          }
+          P += B * Z;
+          Gz += A * dZ;
+        } else {
+          const int ir = b + 2 * mm - 1, ii = b + 2 * mm;
+          float Ar = 0.0f, Ai = 0.0f, Br = 0.0f, Bi = 0.0f;
+#pragma unroll
+          for (int n = 0; n < NA; ++n) {
+            Ar += s[n][ir] * g[n];
+            Ai += s[n][ii] * g[n];
+            Br += s[n][ir] * gp[n];
+            Bi += s[n][ii] * gp[n];
+          }
+          P += Z * (Br * pr[mm] + Bi * pim[mm]);
+          const float mz = (float)mm * Z;
+          Gx += mz * (Ar * pr[mm - 1] + Ai * pim[mm - 1]);
+          Gy += mz * (Ai * pr[mm - 1] - Ar * pim[mm - 1]);
+          Gz += dZ * (Ar * pr[mm] + Ai * pim[mm]);
+        }
+      }
+    }
+    const float gu = Gx * ux + Gy * uy + Gz * uz;
+    const float4 fo = make_float4(P * ux + (Gx - gu * ux) * dinv, P * uy + (Gy - gu * uy) * dinv,
+                                  P * uz + (Gz - gu * uz) * dinv, 0.0f);
+    w.f12[(size_t)e * N + i] = fo;
+    const int qv = w.eq[(size_t)e * N + i];
+    if (qv >= 0 && qv < w.MV && j >= c1 && j < c2) {
+      const int e2 = w.vsa[(size_t)qv * N + j];
+      if (e2 >= 0 && e2 < w.MA) w.f12t[(size_t)e2 * N + j] = fo;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gather: radial pair force, angular partials, ZBL -> FP64 force/virial
+// ---------------------------------------------------------------------------
+
+template <int KR, bool ONE>
+__global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int N, int n1, int n2, int nd1, int nd2,
+                                                   int nblocks, const int* __restrict__ type,
+                                                   double* __restrict__ pe, double* __restrict__ force,
+                                                   double* __restrict__ virial, int accumulate, DevWork w)
+{
+  const int i = n1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
+  if (i >= n2) return;
+  const int T = ONE ? 1 : m.T;
+  const int kR = m.kR;
+  const int ti = ONE ? 0 : safe_type(type[i], T);
+  const int nr = w.nr[i], na = w.na[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  float bi[KR];
+  if (ONE) {
+#pragma unroll
+    for (int k = 0; k < KR; ++k) bi[k] = k < kR ? w.bR[(size_t)i * kR + k] : 0.0f;
+  }
+  for (int e = 0; e < nr; ++e) {
+    const int slot = e < na ? e : w.MR - 1 - (e - na);
+    const float4 r = w.edge[(size_t)slot * N + i];
+    const int j = __float_as_int(r.w);
+    const int tj = ONE ? 0 : safe_type(type[j], T);
+    const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+    const float rc = (float)((m.rcR[ti] + m.rcR[tj]) * 0.5);
+    const float rcinv = 1.0f / rc;
+    float fc, fcp;
+    cutoff_fc_fcp(rc, rcinv, d, fc, fcp);
+    float f[KR], fp[KR];
+    cheb_d<KR>(kR, rcinv, d, fc, fcp, f, fp);
+    const bool jdep = j >= nd1 && j < nd2;
+    const float* bj = w.bR + ((size_t)j * T + ti) * kR;
+    const float* bo = w.bR + ((size_t)i * T + tj) * kR;
+    float Gi = 0.0f, Gj = 0.0f;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      if (k >= kR) break;
+      Gi += (ONE ? bi[k] : bo[k]) * fp[k];
+      if (jdep) Gj += bj[k] * fp[k];
+    }
+    const float dinv = 1.0f / d;
+    const float s = (Gi + Gj) * dinv;
+    fx += (double)(s * r.x);
+    fy += (double)(s * r.y);
+    fz += (double)(s * r.z);
+    // virial_i += r_ij (x) dE_j/dr_ji,  dE_j/dr_ji = -Gj r_ij/d
+    const float gj = -Gj * dinv;
+    const float fjx = gj * r.x, fjy = gj * r.y, fjz = gj * r.z;
+    v[0] += (double)(r.x * fjx);
+    v[1] += (double)(r.y * fjy);
+    v[2] += (double)(r.z * fjz);
+    v[3] += (double)(r.x * fjy);
+    v[4] += (double)(r.x * fjz);
+    v[5] += (double)(r.y * fjz);
+    v[6] += (double)(r.y * fjx);
+    v[7] += (double)(r.z * fjx);
+    v[8] += (double)(r.z * fjy);
+  }
+  float zpe = 0.0f;
+  for (int e = 0; e < na; ++e) {
+    const float4 r = w.edge[(size_t)e * N + i];
+    const int j = __float_as_int(r.w);
+    const float4 a = w.f12[(size_t)e * N + i];
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j >= nd1 && j < nd2) b = w.f12t[(size_t)e * N + i];
+    fx += (double)(a.x - b.x);
+    fy += (double)(a.y - b.y);
+    fz += (double)(a.z - b.z);
+    v[0] += (double)(r.x * b.x);
+    v[1] += (double)(r.y * b.y);
+    v[2] += (double)(r.z * b.z);
+    v[3] += (double)(r.x * b.y);
+    v[4] += (double)(r.x * b.z);
+    v[5] += (double)(r.y * b.z);
+    v[6] += (double)(r.y * b.x);
+    v[7] += (double)(r.z * b.x);
+    v[8] += (double)(r.z * b.y);
+    if (m.zbl) {
+      // screened-Coulomb ZBL with cosine switch (nep_utilities.cuh:347-433)
+      const int tj = ONE ? 0 : safe_type(type[j], T);
+      const float* zp = m.zblp + (size_t)(ti * m.T + tj) * 12;
+      const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+      const float dinv = 1.0f / d;
+      const float x = d * zp[0];
+      float phi = 0.0f, dphi = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float t = zp[4 + 2 * k] * expf(-zp[5 + 2 * k] * x);
+        phi += t;
+        dphi -= zp[5 + 2 * k] * t;
+      }
+      const float vv = zp[1] * phi * dinv;
+      const float vp = zp[1] * dphi * zp[0] * dinv - vv * dinv;
+      const float r1 = zp[2], r2 = zp[3];
+      float fc = 0.0f, fcp = 0.0f;
+      if (d < r1) {
+        fc = 1.0f;
+      } else if (d < r2) {
+        const float wv = 3.14159265f / (r2 - r1);
+        float sn, cs;
+        sincosf(wv * (d - r1), &sn, &cs);
+        fc = 0.5f * cs + 0.5f;
+        fcp = -0.5f * wv * sn;
+      }
+      const float fz_ = vv * fc, fpz = vp * fc + vv * fcp;
+      const float s = fpz * dinv;
+      fx += (double)(s * r.x);
+      fy += (double)(s * r.y);
+      fz += (double)(s * r.z);
+      const float h = -0.5f * s;
+      v[0] += (double)(h * r.x * r.x);
+      v[1] += (double)(h * r.y * r.y);
+      v[2] += (double)(h * r.z * r.z);
+      v[3] += (double)(h * r.x * r.y);
+      v[4] += (double)(h * r.x * r.z);
+      v[5] += (double)(h * r.y * r.z);
+      v[6] += (double)(h * r.y * r.x);
+      v[7] += (double)(h * r.z * r.x);
+      v[8] += (double)(h * r.z * r.y);
+      zpe += 0.5f * fz_;
+    }
+  }
+  if (accumulate) {
+    force[i] += fx;
+    force[N + i] += fy;
+    force[2 * N + i] += fz;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) virial[(size_t)c * N + i] += v[c];
+  } else {
+    force[i] = fx;
+    force[N + i] = fy;
+    force[2 * N + i] = fz;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) virial[(size_t)c * N + i] = v[c];
+  }
+  if (m.zbl) pe[i] += (double)zpe;
+}
+
+// ---------------------------------------------------------------------------
+// integrator and thermo (FP64)
+// ---------------------------------------------------------------------------
+
+__global__ void k_velocity_verlet(int first_half, int N, double dt, const double* __restrict__ mass,
+                                  double* __restrict__ pos, double* __restrict__ vel,
+                                  const double* __restrict__ force)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double half = dt * 0.5;
+  const double im = 1.0 / mass[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double v = vel[c * N + i] + force[c * N + i] * im * half;
+    vel[c * N + i] = v;
+    if (first_half) pos[c * N + i] += v * dt;
+  }
+}
+
+__global__ void k_wrap(DevBox b, int N, double* __restrict__ pos)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double x = pos[i], y = pos[N + i], z = pos[2 * N + i];
+  double s[3] = {b.hi[0] * x + b.hi[1] * y + b.hi[2] * z, b.hi[3] * x + b.hi[4] * y + b.hi[5] * z,
+                 b.hi[6] * x + b.hi[7] * y + b.hi[8] * z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!b.pbc[d]) continue;
+    if (s[d] < 0.0) s[d] += 1.0;
+    else if (s[d] > 1.0) s[d] -= 1.0;
+  }
+  pos[i] = b.h[0] * s[0] + b.h[1] * s[1] + b.h[2] * s[2];
+  pos[N + i] = b.h[3] * s[0] + b.h[4] * s[1] + b.h[5] * s[2];
+  pos[2 * N + i] = b.h[6] * s[0] + b.h[7] * s[1] + b.h[8] * s[2];
+}
+
+// error-free two-sum accumulation (s, c) += x
+struct KSum {
+  double s, c;
+};
+__device__ __forceinline__ void ksum_add(KSum& k, double x)
+{
+  const double t = k.s + x;
+  const double bp = t - k.s;
+  const double err = (k.s - (t - bp)) + (x - bp);
+  k.s = t;
+  k.c += err;
+}
+__device__ __forceinline__ void ksum_merge(KSum& a, const KSum& b)
+{
+  ksum_add(a, b.s);
+  a.c += b.c;
+}
+
+constexpr int kThermoThreads = 256;
+
+int thermo_blocks(int N)
+{
+  int nb = (N + kThermoThreads - 1) / kThermoThreads;
+  return nb < 1 ? 1 : (nb > 512 ? 512 : nb);
+}
+
+// compensated reduction of 8 quantities over a 256-thread workgroup: wave
+// shuffles, then the 4 wave results through LDS; the total lands in thread 0
+__device__ __forceinline__ void block_reduce8(KSum* acc)
+{
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      KSum o{__shfl_down(acc[q].s, off), __shfl_down(acc[q].c, off)};
+      ksum_merge(acc[q], o);
+    }
+  __shared__ double sh[kThermoThreads / 64][8][2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      sh[wave][q][0] = acc[q].s;
+      sh[wave][q][1] = acc[q].c;
+    }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      for (int w = 1; w < kThermoThreads / 64; ++w) ksum_merge(acc[q], KSum{sh[w][q][0], sh[w][q][1]});
+}
+
+// per-workgroup compensated partial sums of the 8 thermo quantities
+// (find_owned_thermo_sums, runtime.cu:442-493)
+__global__ __launch_bounds__(kThermoThreads) void k_thermo_partial(int N, const double* __restrict__ mass,
+                                                                   const double* __restrict__ pe,
+                                                                   const double* __restrict__ vel,
+                                                                   const double* __restrict__ vir,
+                                                                   double* __restrict__ partial)
+{
+  KSum acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = {0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const double m = mass[i];
+    const double vx = vel[i], vy = vel[N + i], vz = vel[2 * N + i];
+    ksum_add(acc[0], m * (vx * vx + vy * vy + vz * vz));
+    ksum_add(acc[1], pe[i]);
+    ksum_add(acc[2], vir[i] + m * vx * vx);
+    ksum_add(acc[3], vir[N + i] + m * vy * vy);
+    ksum_add(acc[4], vir[2 * N + i] + m * vz * vz);
+    ksum_add(acc[5], vir[3 * N + i] + m * vx * vy);
+    ksum_add(acc[6], vir[4 * N + i] + m * vx * vz);
+    ksum_add(acc[7], vir[5 * N + i] + m * vy * vz);
+  }
+  block_reduce8(acc);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      partial[(blockIdx.x * 8 + q) * 2 + 0] = acc[q].s;
+      partial[(blockIdx.x * 8 + q) * 2 + 1] = acc[q].c;
+    }
+}
+
+// normalize_global_thermo (runtime_internal.hpp:704-716): T = sum m v^2/(3 N kB), stress/V
+__global__ __launch_bounds__(kThermoThreads) void k_thermo_final(int nb, int N, double volume,
+                                                                 const double* __restrict__ partial,
+                                                                 double* __restrict__ thermo)
+{
+  KSum acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ksum_merge(acc[q], KSum{partial[(b * 8 + q) * 2], partial[(b * 8 + q) * 2 + 1]});
+  block_reduce8(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      double v = acc[q].s + acc[q].c;
+      if (q == 0) v /= 3.0 * N * 8.617343e-5;
+      else if (q >= 2) v /= volume;
+      thermo[q] = v;
+    }
+  }
+}
+
+__global__ void k_scale(int n, double factor, double* __restrict__ data)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) data[i] *= factor;
+}
+
+__global__ void k_check_scaled(int n, double factor, const double* __restrict__ data, int* __restrict__ flag)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !isfinite(data[i] * factor)) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+
+static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
+
+hipError_t launch_cell_count(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s)
+{
+  const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
+  hipError_t e = hipMemsetAsync(w.cell_count, 0, sizeof(int) * ncell, s);
+  if (e != hipSuccess) return e;
+  if (N > 0) k_cell_count<<<nblk(N), kBlock, 0, s>>>(b, N, pos, w.cell_of, w.cell_count, w.flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_scan(int ncell, DevWork& w, hipStream_t s)
+{
+  const int nb = (ncell + 1023) / 1024;
+  // block sums reuse cell_fill as scratch (nb <= ncell)
+  k_scan_local<<<nb, 256, 0, s>>>(ncell, w.cell_count, w.cell_start, w.cell_fill);
+  k_scan_blocks<<<1, 1024, 0, s>>>(nb, w.cell_fill, w.cell_start + ncell);
+  k_scan_add<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, w.cell_start, w.cell_fill);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipMemsetAsync(w.cell_fill, 0, sizeof(int) * ncell, s);
+}
+
+hipError_t launch_cell_fill(int N, DevWork& w, hipStream_t s)
+{
+  if (N > 0) k_cell_fill<<<nblk(N), kBlock, 0, s>>>(N, w.cell_of, w.cell_start, w.cell_fill, w.cell_atoms);
+  return hipGetLastError();
+}
+
+hipError_t launch_verlet_build(const DevBox& b, int N, int c1, int c2, const double* pos, double rv2,
+                               DevWork& w, hipStream_t s)
+{
+  if (c2 > c1) k_verlet_build<<<nblk(c2 - c1), kBlock, 0, s>>>(b, N, c1, c2, pos, rv2, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_verlet_sort(int N, int c1, int c2, const uint64_t* gid, DevWork& w, hipStream_t s)
+{
+  if (c2 > c1) {
+    const size_t lds = sizeof(int) * 4 * (size_t)w.MV;
+    k_verlet_sort<<<(c2 - c1 + 3) / 4, 256, lds, s>>>(N, c1, c2, gid, w);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_verlet_reverse(int N, int c1, int c2, const uint64_t* gid, DevWork& w, hipStream_t s)
+{
+  if (c2 > c1) k_verlet_reverse<<<nblk(c2 - c1), kBlock, 0, s>>>(N, c1, c2, gid, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_save_positions(int N, const double* pos, DevWork& w, hipStream_t s)
+{
+  if (N > 0) k_save_positions<<<nblk(3 * N), kBlock, 0, s>>>(N, pos, w.x0);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_moved(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s)
+{
+  const double half = 0.5 * kSkin;
+  if (N > 0) k_check_moved<<<nblk(N), kBlock, 0, s>>>(b, N, pos, w.x0, half * half, w.flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type,
+                         const double* pos, DevWork& w, hipStream_t s)
+{
+  if (c2 > c1) {
+    const int nb = nblk(c2 - c1);
+    k_filter<<<xcd_grid(nb), kBlock, 0, s>>>(b, m, N, c1, c2, nb, type, pos, w);
+  }
+  return hipGetLastError();
+}
+
+template <class C>
+static void run_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc,
+                           DevWork& w, hipStream_t s)
+{
+  const int nb = nblk(c2 - c1);
+  k_descriptor<C><<<xcd_grid(nb), kBlock, 0, s>>>(m, N, c1, c2, nb, type, pe, acc ? 1 : 0, w);
+}
+
+hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
+                             bool acc, DevWork& w, hipStream_t s)
+{
+  if (c2 <= c1) return hipSuccess;
+  const bool one = m.T == 1;
+  const bool q2 = m.has222 != 0, q1 = m.has1111 != 0;
+  auto is = [&](int nr, int kr, int na, int ka, int L, bool a, bool b, bool o) {
+    return m.nR == nr && m.kR == kr && m.nA == na && m.kA == ka && m.L == L && q2 == a && q1 == b && one == o;
+  };
+  // exact instantiations of the configurations in use; generic otherwise
+  if (is(5, 9, 5, 9, 4, true, true, true))
+    run_descriptor<Cfg<5, 9, 5, 9, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s);
+  else if (is(7, 9, 5, 9, 4, true, false, true))
+    run_descriptor<Cfg<7, 9, 5, 9, 4, true, false, true, true>>(m, N, c1, c2, type, pe, acc, w, s);
+  else if (is(13, 17, 9, 13, 4, true, true, true))
+    run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s);
+  else if (is(7, 11, 7, 11, 4, true, true, false))
+    run_descriptor<Cfg<7, 11, 7, 11, 4, true, true, false, true>>(m, N, c1, c2, type, pe, acc, w, s);
+  else if (is(5, 9, 5, 9, 4, true, false, false))
+    run_descriptor<Cfg<5, 9, 5, 9, 4, true, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s);
+  else if (is(9, 9, 7, 9, 4, false, false, false))
+    run_descriptor<Cfg<9, 9, 7, 9, 4, false, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s);
+  else if (m.L <= 4 && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
+    run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, 4, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
+                                                                                  w, s);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                         double* pe, double* force, double* virial, bool acc, DevWork& w, hipStream_t s)
+{
+  if (n2 <= n1) return hipSuccess;
+  const int nb = nblk(n2 - n1);
+  const int g = xcd_grid(nb);
+  const int a = acc ? 1 : 0;
+  if (m.T == 1 && m.kR == 9)
+    k_gather<9, true><<<g, kBlock, 0, s>>>(m, N, n1, n2, nd1, nd2, nb, type, pe, force, virial, a, w);
+  else if (m.T == 1)
+    k_gather<kMaxK, true><<<g, kBlock, 0, s>>>(m, N, n1, n2, nd1, nd2, nb, type, pe, force, virial, a, w);
+  else if (m.kR <= 11)
+    k_gather<11, false><<<g, kBlock, 0, s>>>(m, N, n1, n2, nd1, nd2, nb, type, pe, force, virial, a, w);
+  else
+    k_gather<kMaxK, false><<<g, kBlock, 0, s>>>(m, N, n1, n2, nd1, nd2, nb, type, pe, force, virial, a, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_velocity_verlet(int first_half, int N, double dt, const double* mass, double* pos,
+                                  double* vel, const double* force, hipStream_t s)
+{
+  if (N > 0) k_velocity_verlet<<<nblk(N), kBlock, 0, s>>>(first_half, N, dt, mass, pos, vel, force);
+  return hipGetLastError();
+}
+
+hipError_t launch_wrap(const DevBox& b, int N, double* pos, hipStream_t s)
+{
+  if (N > 0) k_wrap<<<nblk(N), kBlock, 0, s>>>(b, N, pos);
+  return hipGetLastError();
+}
+
+hipError_t launch_thermo(int N, double volume, const double* mass, const double* pe, const double* vel,
+                         const double* virial, DevWork& w, hipStream_t s)
+{
+  const int nb = thermo_blocks(N);
+  k_thermo_partial<<<nb, kThermoThreads, 0, s>>>(N, mass, pe, vel, virial, w.partial);
+  k_thermo_final<<<1, kThermoThreads, 0, s>>>(nb, N, volume, w.partial, w.thermo);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(int n, double factor, double* data, hipStream_t s)
+{
+  if (n > 0) k_scale<<<nblk(n), kBlock, 0, s>>>(n, factor, data);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_scaled(int n, double factor, const double* data, int* flag, hipStream_t s)
+{
+  if (n > 0) k_check_scaled<<<nblk(n), kBlock, 0, s>>>(n, factor, data, flag);
+  return hipGetLastError();
+}
+
+}  // namespace dmgx

@@ -1,0 +1,117 @@
+// kernels.hpp -- device-side structures and launch wrappers of the NEP step.
+//
+// All kernels live in kernels.hip (one translation unit, so the __constant__
+// angular tables are shared); host code calls the launch_* wrappers below.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nep_model.hpp"
+
+namespace dmgx {
+
+constexpr int kBlock = 256;  // threads per workgroup for per-atom kernels
+constexpr double kSkin = 1.0;  // Verlet skin (neighbor.cuh:259)
+
+// Box with its inverse, as the reference's Box (box.cuh:53-70) plus the
+// cell grid of the Verlet build.
+struct DevBox {
+  double h[9];
+  double hi[9];
+  double thick[3];
+  int pbc[3];
+  int ortho;
+  int nc[3];   // cells per direction
+};
+
+// Model parameters as kernel arguments; tables live in device memory.
+struct DevModel {
+  int T, nR, nA, kR, kA, L, has222, has1111, numL, D, H, version;
+  int zbl;
+  double rcR[kMaxTypes], rcA[kMaxTypes];  // FP64 per-type cutoffs (predicate)
+  const float* w0;    // [T][H][D]
+  const float* b0;    // [T][H]
+  const float* w1;    // [T][H]
+  const float* w1x;   // [T] NEP5 typewise output bias
+  const float* b1;    // [1]
+  const float* cR;    // [T][T][nR][kR]
+  const float* cA;    // [T][T][nA][kA]
+  const float* qs;    // [D]
+  const float* zblp;  // [T][T][12] a_inv, zizj, r_in, r_out, (a_k, b_k) x4
+};
+
+// Per-atom / per-edge workspaces (device pointers), sized for `cap` slots.
+struct DevWork {
+  int cap = 0;
+  int MV = 0, MR = 0, MA = 0;  // row capacities: Verlet, radial, angular
+  int ncell_cap = 0;
+  int* cell_of = nullptr;      // [cap]
+  int* cell_count = nullptr;   // [ncell]
+  int* cell_start = nullptr;   // [ncell+1]
+  int* cell_fill = nullptr;    // [ncell]
+  int* cell_atoms = nullptr;   // [cap]
+  int* vn = nullptr;           // [cap] Verlet row length
+  int* vl = nullptr;           // [MV*cap] Verlet rows (ELL, slot-major)
+  int* vrev = nullptr;         // [MV*cap] position of the centre in the neighbour's row
+  double* x0 = nullptr;        // [3*cap] positions at the last rebuild
+  int* nr = nullptr;           // [cap] radial row length
+  int* na = nullptr;           // [cap] angular row length (prefix of the radial row)
+  float4* edge = nullptr;      // [MR*cap] (rx, ry, rz, bits(j))
+  int* eq = nullptr;           // [MA*cap] reverse Verlet position of an angular edge
+  int* vsa = nullptr;          // [MV*cap] Verlet position -> angular slot
+  float* bR = nullptr;         // [cap*T*kR] radial adjoint in basis space
+  float4* f12 = nullptr;       // [MA*cap] dE_i/dr_ij of angular edges
+  float4* f12t = nullptr;      // [MA*cap] dE_j/dr_ji stored at i's slot of j
+  int* flags = nullptr;        // [8] device status words
+  double* partial = nullptr;   // thermo block partials
+  double* thermo = nullptr;    // [8]
+};
+
+// Status words in DevWork::flags
+enum : int {
+  kFlagVerletOverflow = 0,  // max Verlet row length seen
+  kFlagRadialOverflow = 1,  // max radial row length seen
+  kFlagAngularOverflow = 2, // max angular row length seen
+  kFlagMoved = 3,           // an atom moved more than skin/2
+  kFlagOutOfBox = 4,        // a position lies outside the box in a non-periodic direction
+  kFlagNonFinite = 5,       // non-finite scaled position
+  kFlagBadType = 6,         // a type index outside [0, num_types)
+};
+
+void upload_angular_constants(const AngularConstants& a);
+
+// Neighbour build (per rebuild)
+hipError_t launch_cell_count(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s);
+hipError_t launch_cell_scan(int ncell, DevWork& w, hipStream_t s);
+hipError_t launch_cell_fill(int N, DevWork& w, hipStream_t s);
+hipError_t launch_verlet_build(const DevBox& b, int N, int c1, int c2, const double* pos, double rv2,
+                               DevWork& w, hipStream_t s);
+hipError_t launch_verlet_sort(int N, int c1, int c2, const uint64_t* gid, DevWork& w, hipStream_t s);
+hipError_t launch_verlet_reverse(int N, int c1, int c2, const uint64_t* gid, DevWork& w, hipStream_t s);
+hipError_t launch_save_positions(int N, const double* pos, DevWork& w, hipStream_t s);
+hipError_t launch_check_moved(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s);
+
+// Force evaluation (per step)
+hipError_t launch_filter(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type,
+                         const double* pos, DevWork& w, hipStream_t s);
+hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
+                             bool accumulate, DevWork& w, hipStream_t s);
+hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                         double* pe, double* force, double* virial, bool accumulate, DevWork& w,
+                         hipStream_t s);
+
+// Integrator / thermo
+hipError_t launch_velocity_verlet(int first_half, int N, double dt, const double* mass, double* pos,
+                                  double* vel, const double* force, hipStream_t s);
+hipError_t launch_wrap(const DevBox& b, int N, double* pos, hipStream_t s);
+hipError_t launch_thermo(int N, double volume, const double* mass, const double* pe, const double* vel,
+                         const double* virial, DevWork& w, hipStream_t s);
+hipError_t launch_scale(int n, double factor, double* data, hipStream_t s);
+hipError_t launch_check_scaled(int n, double factor, const double* data, int* flag, hipStream_t s);
+
+// Number of thermo block partials launch_thermo uses for N atoms.
+int thermo_blocks(int N);
+
+}  // namespace dmgx

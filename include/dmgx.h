@@ -1,0 +1,229 @@
+/*
+ * dmgx.h -- C ABI of the MI355X-native NEP molecular-dynamics step.
+ *
+ * The library (libdmgx.so, built from dmg-md_amd/csrc) is a drop-in for one
+ * GPU's share of the reference's NEP force evaluator and integrator.  Every
+ * entry point below names the reference interface it replaces (file:line in
+ * the reference tree).  Conventions:
+ *   - every function returns 0 on success or a negative DMGX_E* code; the
+ *     message of the last failure on the calling thread is dmgx_last_error();
+ *     nothing here exits the process or throws across the ABI;
+ *   - array arguments are caller-owned DEVICE pointers (hipMalloc'd) unless
+ *     the name ends in _host; per-atom arrays are structure-of-arrays with
+ *     stride N exactly as the reference lays them out (nep.cu:1282-1292):
+ *       position/force double[3N] x|y|z, potential double[N],
+ *       virial double[9N] xx yy zz xy xz yz yx zx zy, type int[N] (0-based
+ *       index into the nep.txt symbol list), global_id uint64[N];
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *     every launch of a call goes to that stream;
+ *   - one dmgx_nep handle per host thread; calls on one handle are not
+ *     reentrant (the reference is single-threaded too, nep.cu:1311).
+ */
+#ifndef DMGX_H
+#define DMGX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMGX_OK 0
+#define DMGX_EINVAL -1     /* bad argument or input file */
+#define DMGX_ENOMEM -2     /* device allocation failed */
+#define DMGX_EDEVICE -3    /* HIP runtime / kernel launch error */
+#define DMGX_EOVERFLOW -4  /* a neighbour row exceeded its capacity */
+#define DMGX_EUNSUPPORTED -5
+
+/* Simulation box: h is the reference's cpu_h[0..8] (row-major 3x3 matrix whose
+ * COLUMNS are the cell vectors a, b, c; box.cuh:53-70), pbc 1 = periodic. */
+typedef struct dmgx_box {
+  double h[9];
+  int pbc[3];
+} dmgx_box;
+
+typedef struct dmgx_nep dmgx_nep;
+
+/* Per-model dimensions, the params() view of the reference (nep.cuh:80-137). */
+typedef struct dmgx_nep_info {
+  int version;            /* 4 or 5 */
+  int num_types;
+  int n_max_radial, n_max_angular;
+  int basis_size_radial, basis_size_angular;
+  int l_max, num_L, dim, num_neurons;
+  int zbl_enabled, zbl_flexible;
+  int MN_radial, MN_angular; /* enlarged x1.25 as nep.cu:267-268 */
+  double rc_radial_max, rc_angular_max;
+  int atomic_number[16];     /* first 16 types */
+} dmgx_nep_info;
+
+/* Flags of dmgx_nep_compute */
+#define DMGX_ACCUMULATE 1u     /* add into pe/force/virial (the reference's +=
+                                  contract, nep.cu:745,869); default overwrite */
+#define DMGX_FORCE_REBUILD 2u  /* rebuild the Verlet list regardless of skin */
+#define DMGX_ANN_BF16 4u       /* ANN on bf16 MFMA with fp32 accumulation */
+#define DMGX_SORT_BY_GID 8u    /* order rows by global_id (domain rows,
+                                  neighbor.cuh:164-196); needs global_id */
+
+/* NEP::NEP(const char*) -- parse nep.txt once and upload the parameters
+ * (nep.cu:135-415).  Supports nep4, nep4_zbl, nep5, nep5_zbl with l_max <= 4
+ * and the q_222 / q_1111 many-body invariants. */
+int dmgx_nep_create(const char* nep_txt_path, dmgx_nep** out);
+
+/* ~NEP (nep.cu:455) */
+void dmgx_nep_destroy(dmgx_nep* nep);
+
+/* NEP::params()/zbl_params() view (nep.cuh:80-137) */
+int dmgx_nep_get_info(const dmgx_nep* nep, dmgx_nep_info* out);
+
+/* NEP::allocate_workspace(int) (nep.cu:432-451): size every per-atom workspace
+ * for up to `capacity` local slots.  compute() grows it on demand. */
+int dmgx_nep_reserve(dmgx_nep* nep, int capacity);
+
+/* Potential::compute (potential.cuh:82-88) and NEP::compute_domain
+ * (nep.cuh:178-189) in one call.
+ *   [0,N)      every local slot is a neighbour candidate (the SoA stride);
+ *   [nd1,nd2)  dependency centres: Verlet rows, descriptors, Fp, partials;
+ *   [n1,n2)    force centres: force, virial (and ZBL energy).
+ * The plain large-box call is n1=nd1=0, n2=nd2=N.  global_id may be NULL
+ * unless DMGX_SORT_BY_GID is set.  Positions must lie inside the box in
+ * non-periodic directions.  Outputs are overwritten unless DMGX_ACCUMULATE. */
+int dmgx_nep_compute(dmgx_nep* nep, const dmgx_box* box, int N, int n1, int n2, int nd1, int nd2,
+                     const int* type, const double* position, const uint64_t* global_id,
+                     double* potential, double* force, double* virial, unsigned flags,
+                     void* stream);
+
+/* NEP::notify_box_changed (nep.cuh:170, neighbor.cuh:224): the next compute
+ * rebuilds the Verlet list. */
+int dmgx_nep_notify_box_changed(dmgx_nep* nep);
+
+/* Neighbour-list statistics of the last compute (the neighbor.out record of
+ * nep.cu:1156-1176): max radial/angular/Verlet row lengths, rebuild count. */
+int dmgx_nep_neighbor_stats(dmgx_nep* nep, int* max_radial, int* max_angular, int* max_verlet,
+                            int* num_rebuilds);
+/* Summed row lengths over the dependency centres of the last compute (mean
+ * neighbour counts for throughput accounting; no reference counterpart). */
+int dmgx_nep_neighbor_totals(dmgx_nep* nep, long long* sum_radial, long long* sum_angular,
+                             long long* sum_verlet, int* num_centres);
+
+/* Per-kernel timing with HIP events recorded on the launch stream (no
+ * counterpart in the reference; the measurement harness uses it).  Kernel
+ * ids: 0 filter, 1 fused descriptor/ANN/angular-partial, 2 gather,
+ * 3 Verlet rebuild (all its kernels), 4 displacement check, 5 velocity
+ * Verlet, 6 thermo.  profile_read synchronises and returns the launch count
+ * and summed milliseconds since the last reset. */
+int dmgx_nep_set_profiling(dmgx_nep* nep, int enable);
+int dmgx_nep_profile_read(dmgx_nep* nep, int kernel, int* launches, double* total_ms);
+int dmgx_nep_profile_reset(dmgx_nep* nep);
+
+/* Copies the current Verlet rows of centres [c1,c2) to host memory, sorted
+ * ascending by slot (test access to the row contract of neighbor.cu:141-218):
+ * counts_host[c2-c1], rows_host[(c2-c1)*max_nb]. */
+int dmgx_nep_copy_verlet_rows(dmgx_nep* nep, int c1, int c2, int max_nb, int* counts_host,
+                              int* rows_host);
+/* Same for the per-step radial (which=0) or angular (which=1) rows. */
+int dmgx_nep_copy_pair_rows(dmgx_nep* nep, int which, int c1, int c2, int max_nb,
+                            int* counts_host, int* rows_host);
+
+/* ---------------------------------------------------------------------- */
+/* Integrator and thermodynamics (FP64)                                    */
+/* ---------------------------------------------------------------------- */
+
+/* velocity_verlet kernel (runtime.cu:392-421): v += f/m * dt/2, and on the
+ * first half x += v*dt.  dt in natural units (fs / 10.18051). */
+int dmgx_velocity_verlet(int first_half, int N, double dt, const double* mass, double* position,
+                         double* velocity, const double* force, void* stream);
+
+/* wrap_positions (runtime_internal.hpp:648-681): fold into the box through
+ * fractional coordinates. */
+int dmgx_wrap_positions(const dmgx_box* box, int N, double* position, void* stream);
+
+/* find_owned_thermo_sums + normalize_global_thermo (runtime.cu:442-535,
+ * runtime_internal.hpp:704-716) with compensated FP64 sums:
+ * thermo_host[8] = T, sum pe, (sum virial_ab + m v_a v_b)/V for xx yy zz xy
+ * xz yz.  Synchronises the stream. */
+int dmgx_thermo(const dmgx_box* box, int N, const double* mass, const double* potential,
+                const double* velocity, const double* virial, double* thermo_host, void* stream);
+
+/* scale_owned_velocity (runtime.cu:495-510) */
+int dmgx_scale_velocity(int N, double factor, double* velocity, void* stream);
+
+/* preflight_scaled_positions + scale_positions (box_control.cu:22-46,128-174):
+ * isotropic x <- factor*x; returns DMGX_EINVAL if a scaled position is not
+ * finite (positions are left unchanged in that case). */
+int dmgx_scale_positions(int N, double factor, double* position, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Ensemble scalars (host FP64; bit-compatible with the reference)         */
+/* ---------------------------------------------------------------------- */
+
+typedef struct dmgx_rng dmgx_rng; /* std::mt19937 stream of SvrState */
+
+/* SvrState(seed) (thermostat_svr.cpp:29) */
+int dmgx_rng_create(uint32_t seed, dmgx_rng** out);
+void dmgx_rng_destroy(dmgx_rng* rng);
+/* gasdev, gamdev, resamplekin_sumnoises, resamplekin (thermostat_svr.cpp:31-124) */
+double dmgx_gasdev(dmgx_rng* rng);
+double dmgx_gamdev(int ia, dmgx_rng* rng);
+double dmgx_resamplekin_sumnoises(int nn, dmgx_rng* rng);
+double dmgx_resamplekin(double kk, double sigma, int ndeg, double taut, dmgx_rng* rng);
+
+/* make_nhc_state / nhc_half_step (thermostat_nhc.cpp:30-117); chain state is
+ * position[4], momentum[4], mass[4]. */
+typedef struct dmgx_nhc {
+  double position[4], momentum[4], mass[4];
+} dmgx_nhc;
+int dmgx_nhc_init(int atom_count, double temperature, double damping, double dt, dmgx_nhc* out);
+double dmgx_nhc_half_step(dmgx_nhc* state, double ek2, double kT, double dN, double dt_half);
+
+/* npt_scr_isotropic_scale (npt_scr.cpp:21-34) */
+double dmgx_npt_scr_scale(double target_pressure, double pressure_coupling,
+                          double pressure_instant, double target_temperature, double volume,
+                          double gaussian);
+
+/* ---------------------------------------------------------------------- */
+/* MD segment (runtime.cu:680-849 run_segment, single GPU, no MPI)          */
+/* ---------------------------------------------------------------------- */
+
+typedef struct dmgx_md dmgx_md;
+
+#define DMGX_ENS_NVE 0
+#define DMGX_ENS_NVT_BER 1
+#define DMGX_ENS_NVT_BDP 2
+#define DMGX_ENS_NVT_NHC 3
+#define DMGX_ENS_NPT_BER 4
+#define DMGX_ENS_NPT_SCR 5
+
+typedef struct dmgx_ensemble {
+  int kind;               /* DMGX_ENS_* */
+  double t_init, t_final; /* K */
+  double t_coupling;      /* Tdamp in steps (run.in ensemble line) */
+  double p_target_gpa, bulk_modulus_gpa, p_coupling; /* NPT */
+  uint32_t seed;          /* BDP/SCR RNG seed (default 12345678) */
+} dmgx_ensemble;
+
+/* Creates an MD state bound to a NEP handle; copies the caller's host arrays
+ * (positions/velocities in A and A/natural-time, masses in amu) into device
+ * memory owned by the state. */
+int dmgx_md_create(dmgx_nep* nep, const dmgx_box* box, int N, const int* type_host,
+                   const double* mass_host, const double* position_host,
+                   const double* velocity_host, dmgx_md** out);
+void dmgx_md_destroy(dmgx_md* md);
+/* Runs `steps` velocity-Verlet steps of dt_fs femtoseconds under `ens`,
+ * evaluating forces on the device; thermo of the last step -> thermo_host[8]
+ * (may be NULL).  flags as dmgx_nep_compute (DMGX_ANN_BF16). */
+int dmgx_md_run(dmgx_md* md, int steps, double dt_fs, const dmgx_ensemble* ens, unsigned flags,
+                double* thermo_host);
+/* Device pointers of the state (position, velocity, force, potential, virial,
+ * mass, type) for zero-copy inspection. */
+int dmgx_md_arrays(dmgx_md* md, double** position, double** velocity, double** force,
+                   double** potential, double** virial, double** mass, int** type);
+int dmgx_md_box(dmgx_md* md, dmgx_box* box_out);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* dmgx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMGX_H */

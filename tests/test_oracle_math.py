@@ -1,0 +1,189 @@
+"""Pins of the FP64 oracle that come from the mathematics rather than from
+reference outputs: forces are minus the gradient of the energy, descriptors
+are rotation invariant, momentum is conserved, the per-atom virial sums to
+the box virial, the angular constants equal their closed forms (Legendre
+addition theorem, Wigner-3j couplings), and the ZBL pair term is the
+textbook universal screened Coulomb potential."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import BASELINE, POTENTIALS
+from oracle import OracleNEP, constant_tables
+
+C_NEP = os.path.join(BASELINE, "inputs", "potentials", "nep_C.txt")
+W_NEP = os.path.join(BASELINE, "inputs", "potentials", "nep_water.txt")
+Z_NEP = os.path.join(BASELINE, "inputs", "potentials", "nep_BaTiO3_zbl.txt")
+
+
+def cluster(n, L, seed, dmin=1.1):
+    rng = np.random.default_rng(seed)
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(0, L, 3)
+        if all(np.linalg.norm(((p - q) + L / 2) % L - L / 2) > dmin for q in pts):
+            pts.append(p)
+    return np.ascontiguousarray(np.array(pts).T)
+
+
+@pytest.mark.parametrize("path,ntypes", [(C_NEP, 1), (W_NEP, 2), (Z_NEP, 3)])
+def test_force_is_minus_energy_gradient(path, ntypes):
+    nep = OracleNEP(path)
+    L = 15.0
+    pos = cluster(18, L, 3, dmin=1.3)
+    t = (np.arange(pos.shape[1]) % ntypes).astype(np.int32)
+    h = np.diag([L, L, L])
+    o = nep.compute(h, [1, 1, 1], t, pos)
+    eps = 1e-5
+    for i in (0, 5, 11):
+        for c in range(3):
+            p1 = pos.copy(); p1[c, i] += eps
+            p2 = pos.copy(); p2[c, i] -= eps
+            e1 = nep.compute(h, [1, 1, 1], t, p1)["pe"].sum()
+            e2 = nep.compute(h, [1, 1, 1], t, p2)["pe"].sum()
+            fd = -(e1 - e2) / (2 * eps)
+            assert abs(fd - o["force"][c, i]) < 2e-6 * max(1.0, abs(fd)), (i, c, fd, o["force"][c, i])
+
+
+def test_virial_is_strain_derivative():
+    """sum of the per-atom virial = -dE/d(strain) (homogeneous deformation)."""
+    nep = OracleNEP(C_NEP)
+    L = 16.0
+    pos = cluster(24, L, 7, dmin=1.3)
+    t = np.zeros(pos.shape[1], np.int32)
+    h = np.diag([L, L, L])
+    W = nep.compute(h, [1, 1, 1], t, pos)["virial"].sum(1)
+    eps = 1e-6
+    # xx and xy components: W_ab = -dE/de_ab with x -> (I + e) x applied to box and positions
+    for (a, b, k) in ((0, 0, 0), (0, 1, 3), (2, 1, 8)):
+        E = []
+        for s in (+1, -1):
+            F = np.eye(3)
+            F[a, b] += s * eps
+            E.append(nep.compute(F @ h, [1, 1, 1], t, F @ pos)["pe"].sum())
+        dEde = (E[0] - E[1]) / (2 * eps)
+        assert abs(-dEde - W[k]) < 1e-5 * max(1.0, abs(dEde)), (a, b, -dEde, W[k])
+
+
+def test_momentum_conservation_and_translation():
+    nep = OracleNEP(W_NEP)
+    L = 14.0
+    pos = cluster(40, L, 9)
+    t = (np.arange(pos.shape[1]) % 2).astype(np.int32)
+    h = np.diag([L, L, L])
+    o = nep.compute(h, [1, 1, 1], t, pos)
+    assert np.abs(o["force"].sum(1)).max() < 1e-10
+    o2 = nep.compute(h, [1, 1, 1], t, (pos + np.array([[0.3], [-1.7], [2.2]])))
+    assert np.abs(o2["pe"] - o["pe"]).max() < 1e-10
+
+
+def test_descriptor_rotation_and_permutation_invariance():
+    nep = OracleNEP(C_NEP)
+    pos = cluster(14, 10.0, 4, dmin=1.3) - 5.0
+    t = np.zeros(pos.shape[1], np.int32)
+    h = np.diag([80.0, 80.0, 80.0])  # isolated cluster in a large box
+    q0 = nep.compute(h, [0, 0, 0], t, pos + 40, want_descriptor=True)["q"]
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    R = R @ np.array([[1, 0, 0], [0, np.cos(1.1), -np.sin(1.1)], [0, np.sin(1.1), np.cos(1.1)]])
+    q1 = nep.compute(h, [0, 0, 0], t, R @ pos + 40, want_descriptor=True)["q"]
+    assert np.abs(q1 - q0).max() < 1e-10 * max(1.0, np.abs(q0).max())
+    perm = np.random.default_rng(1).permutation(pos.shape[1])
+    q2 = nep.compute(h, [0, 0, 0], t, pos[:, perm] + 40, want_descriptor=True)["q"]
+    assert np.abs(q2 - q0[perm]).max() < 1e-10 * max(1.0, np.abs(q0).max())
+
+
+def test_angular_constants_closed_forms():
+    sp = pytest.importorskip("sympy")
+    from sympy.physics.wigner import wigner_3j
+    c3, c4, c5, zc = constant_tables()
+    z = sp.symbols("z")
+    for L in range(1, 5):
+        for m in range(L + 1):
+            p = sp.Poly(sp.diff(sp.legendre(L, z), z, m), z)
+            coeffs = [int(sp.Rational(c) / sp.gcd_list([sp.Rational(x) for x in p.all_coeffs() if x != 0]))
+                      for c in p.all_coeffs()[::-1]]
+            got = [int(v) for v in zc[L * (L + 1) // 2 + m][:len(coeffs)]]
+            assert got == coeffs, (L, m)
+    # q_L through the Legendre addition theorem: sum_jk g_j g_k P_L(u_j.u_k) (2L+1)/(4 pi)
+    rng = np.random.default_rng(0)
+    u = rng.normal(size=(6, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    g = rng.normal(size=6)
+    for L in range(1, 5):
+        direct = sum(g[a] * g[b] * np.polynomial.legendre.legval(u[a] @ u[b], [0] * L + [1])
+                     for a in range(6) for b in range(6)) * (2 * L + 1) / (4 * np.pi)
+        s = np.zeros(2 * L + 1)
+        for a in range(6):
+            x, y, zz = u[a]
+            for m in range(L + 1):
+                Z = sum(zc[L * (L + 1) // 2 + m][k] * zz ** k for k in range(L - m + 1))
+                P = (x + 1j * y) ** m
+                if m == 0:
+                    s[0] += g[a] * Z
+                else:
+                    s[2 * m - 1] += g[a] * Z * P.real
+                    s[2 * m] += g[a] * Z * P.imag
+        qL = sum(c3[L * L - 1 + k] * s[k] ** 2 for k in range(2 * L + 1))
+        assert abs(qL - direct) < 1e-10 * max(1.0, abs(direct)), L
+    # q_222 and q_1111 weights from the Wigner-3j coupling of complex moments
+    rt14 = np.sqrt(14.0) / (448 * np.pi ** 1.5)
+    assert np.allclose(c4, np.array([-5, -90, 45, 270, -540]) * rt14, rtol=1e-14)
+    s1 = rng.normal(size=3)
+    A1 = {0: np.sqrt(3 / (4 * np.pi)) * s1[0],
+          1: -np.sqrt(3 / (8 * np.pi)) * (s1[1] + 1j * s1[2])}
+    A1[-1] = -np.conj(A1[1])
+    q = 0.0
+    for Lc in (0, 1, 2):
+        for m1 in (-1, 0, 1):
+            for m2 in (-1, 0, 1):
+                for m3 in (-1, 0, 1):
+                    m4 = -m1 - m2 - m3
+                    M = m1 + m2
+                    if abs(m4) > 1 or abs(M) > Lc:
+                        continue
+                    w = float(wigner_3j(1, 1, Lc, m1, m2, -M)) * float(wigner_3j(Lc, 1, 1, M, m3, m4))
+                    q += (-1) ** (Lc - M) * w * A1[m1] * A1[m2] * A1[m3] * A1[m4]
+    assert abs(q.real - c5 * (s1 ** 2).sum() ** 2) < 1e-12 and abs(q.imag) < 1e-12
+
+
+def test_zbl_pair_is_universal_screened_coulomb():
+    """Two atoms well inside rc_inner: E = K Zi Zj phi(r/a)/r with the
+    universal ZBL screening function, and F = -dE/dr."""
+    nep = OracleNEP(Z_NEP)
+    h = np.diag([30.0, 30.0, 30.0])
+    r = 0.5
+    pos = np.array([[10.0, 10.0 + r], [10.0, 10.0], [10.0, 10.0]])
+    t = np.array([0, 1], np.int32)  # Ba, Ti
+    o = nep.compute(h, [1, 1, 1], t, pos)
+    # separate the ZBL term: the NEP part at the same geometry from a
+    # no-ZBL copy of the model
+    txt = open(Z_NEP).read().splitlines()
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as fh:
+        fh.write("\n".join([txt[0].replace("nep4_zbl", "nep4")] + txt[2:]) + "\n")
+        plain = fh.name
+    o0 = OracleNEP(plain).compute(h, [1, 1, 1], t, pos)
+    os.unlink(plain)
+    Zi, Zj = 56, 22
+    a = 0.46850 / (Zi ** 0.23 + Zj ** 0.23)
+    x = r / a
+    phi = (0.18175 * np.exp(-3.1998 * x) + 0.50986 * np.exp(-0.94229 * x) +
+           0.28022 * np.exp(-0.4029 * x) + 0.02817 * np.exp(-0.20162 * x))
+    E = 14.399645 * Zi * Zj * phi / r
+    ez = (o["pe"] - o0["pe"]).sum()
+    assert abs(ez - E) < 2e-4 * E  # 0.46850 = 1/2.134563 to 5 digits
+    fz = o["force"][0, 1] - o0["force"][0, 1]
+    dr = 1e-6
+    x1, x2 = (r + dr) / a, (r - dr) / a
+    ph = lambda x: (0.18175 * np.exp(-3.1998 * x) + 0.50986 * np.exp(-0.94229 * x) +
+                    0.28022 * np.exp(-0.4029 * x) + 0.02817 * np.exp(-0.20162 * x))
+    dE = 14.399645 * Zi * Zj * (ph(x1) / (r + dr) - ph(x2) / (r - dr)) / (2 * dr)
+    assert abs(fz + dE) < 2e-4 * abs(dE)
+
+
+def test_seeded_models_parse_and_are_finite():
+    for name in ("nep_c0_carbon.txt", "nep_c1_cu.txt", "nep_c4_carbon_zbl.txt"):
+        nep = OracleNEP(os.path.join(POTENTIALS, name))
+        assert nep.version == 4
