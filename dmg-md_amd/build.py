@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdmgx.so")
-SOURCES = ["kernels.hip", "nep_model.cpp", "ensemble.cpp", "engine.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "neighbor.hip", "nep_model.cpp", "ensemble.cpp", "engine.cpp", "capi.cpp"]
 ARCH = os.environ.get("DMGX_OFFLOAD_ARCH", "gfx950")
 
 
@@ -22,7 +22,7 @@ def command(out: str = LIB) -> list:
         hipcc, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
         # packed FP32 (v_pk_*_f32) stays off: scalar FP32 only
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-Wno-unused-command-line-argument",
+        "-Wno-unused-command-line-argument", "-Wl,-z,defs",
         "-I", os.path.join(ROOT, "include"), "-I", CSRC,
         *[os.path.join(CSRC, s) for s in SOURCES],
         "-o", out,

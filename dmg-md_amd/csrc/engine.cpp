@@ -215,9 +215,9 @@ NepEngine::NepEngine(const std::string& path)
   dm_.cA = params_ + im.off_cA;
   dm_.qs = params_ + im.off_qs;
   dm_.zblp = params_ + zoff;
-  check(hipHostMalloc((void**)&hflags_, 8 * sizeof(int), hipHostMallocDefault), "pinned flags");
-  dalloc(&w_.flags, 8, "flags");
-  check(hipMemset(w_.flags, 0, 8 * sizeof(int)), "clear flags");
+  check(hipHostMalloc((void**)&hflags_, kNumFlags * sizeof(int), hipHostMallocDefault), "pinned flags");
+  dalloc(&w_.flags, kNumFlags, "flags");
+  check(hipMemset(w_.flags, 0, kNumFlags * sizeof(int)), "clear flags");
 }
 
 NepEngine::~NepEngine()
@@ -319,40 +319,7 @@ DevBox NepEngine::make_devbox(const Box& box) const
   return b;
 }
 
-void NepEngine::rebuild(const DevBox& b, int N, int nd1, int nd2, const double* pos, const uint64_t* gid,
-                        hipStream_t s)
-{
-  const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
-  ensure_cells(ncell);
-  const double rv = model_.rcR_max + kSkin;
-  const double rv2 = rv * rv;
-  for (int attempt = 0; attempt < 4; ++attempt) {
-    check(hipMemsetAsync(w_.flags, 0, 8 * sizeof(int), s), "clear flags");
-    check(launch_cell_count(b, N, pos, w_, s), "cell count");
-    check(launch_cell_scan(ncell, w_, s), "cell scan");
-    check(launch_cell_fill(N, w_, s), "cell fill");
-    check(launch_verlet_build(b, N, nd1, nd2, pos, rv2, w_, s), "verlet build");
-    check(hipMemcpyAsync(hflags_, w_.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s), "read flags");
-    check(hipStreamSynchronize(s), "verlet build");
-    if (hflags_[kFlagOutOfBox])
-      throw std::invalid_argument("a position lies outside the box in a non-periodic direction");
-    if (hflags_[kFlagVerletOverflow] > w_.MV) {
-      const int mv = (int)std::ceil(hflags_[kFlagVerletOverflow] * 1.2) + 8;
-      if (mv > 4096) throw std::runtime_error("Verlet row overflow: more than 4096 neighbours within rc + skin");
-      allocate(w_.cap, mv);
-      continue;
-    }
-    check(launch_verlet_sort(N, nd1, nd2, gid, w_, s), "verlet sort");
-    check(launch_verlet_reverse(N, nd1, nd2, gid, w_, s), "verlet reverse");
-    check(launch_save_positions(N, pos, w_, s), "save positions");
-    ++rebuilds_;
-    have_list_ = true;
-    return;
-  }
-  throw std::runtime_error("Verlet rows kept overflowing");
-}
-
-void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
+void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
                         const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
                         unsigned flags, hipStream_t s)
 {
@@ -371,34 +338,28 @@ void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2,
   const long long ncell = (long long)b.nc[0] * b.nc[1] * b.nc[2];
   if (ncell > (1LL << 28)) throw std::invalid_argument("cell grid too large");
   if (N > w_.cap || w_.MV == 0) reserve(std::max(N, (int)(w_.cap * 1.25)));
-  bool need = !have_list_ || box_changed_ || N != last_N_ || nd1 != last_nd1_ || nd2 != last_nd2_ ||
-              by_gid != last_gid_sort_ || (flags & 2u) || std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
-              std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0;
-  if (!need) {
-    check(hipMemsetAsync(w_.flags + kFlagMoved, 0, sizeof(int), s), "clear moved");
-    prof_.begin(kProfCheck, s);
-    check(launch_check_moved(b, N, pos, w_, s), "displacement check");
-    prof_.end(s);
-    check(hipMemcpyAsync(hflags_, w_.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s), "read flags");
-    check(hipStreamSynchronize(s), "displacement check");
-    if (hflags_[kFlagBadType]) {
-      check(hipMemsetAsync(w_.flags + kFlagBadType, 0, sizeof(int), s), "clear flag");
-      throw std::invalid_argument("an atom type lies outside [0, num_types) (previous call)");
-    }
-    need = hflags_[kFlagMoved] != 0;
-  }
-  if (need) {
-    prof_.begin(kProfRebuild, s);
-    rebuild(b, N, nd1, nd2, pos, by_gid ? gid : nullptr, s);
-    prof_.end(s);
-    box_changed_ = false;
-    last_N_ = N;
-    last_nd1_ = nd1;
-    last_nd2_ = nd2;
-    last_gid_sort_ = by_gid;
-    std::memcpy(last_h_, box.h, sizeof(last_h_));
-    std::memcpy(last_pbc_, box.pbc, sizeof(last_pbc_));
-  }
+  ensure_cells((int)ncell);
+  const bool forced = !have_list_ || box_changed_ || N != last_N_ || nd1 != last_nd1_ || nd2 != last_nd2_ ||
+                      by_gid != last_gid_sort_ || (flags & 2u) ||
+                      std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
+                      std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0;
+  // rebuild decision on the device: forced by the host, or any atom moved > skin/2
+  prof_.begin(kProfCheck, s);
+  if (forced) check(launch_request_rebuild(w_, s), "rebuild request");
+  else check(launch_check_moved(b, N, pos, w_, s), "displacement check");
+  prof_.end(s);
+  const double rv = model_.rcR_max + kSkin;
+  prof_.begin(kProfRebuild, s);
+  check(launch_rebuild(b, N, nd1, nd2, pos, rv * rv, by_gid ? gid : nullptr, w_, s), "rebuild");
+  prof_.end(s);
+  have_list_ = true;
+  box_changed_ = false;
+  last_N_ = N;
+  last_nd1_ = nd1;
+  last_nd2_ = nd2;
+  last_gid_sort_ = by_gid;
+  std::memcpy(last_h_, box.h, sizeof(last_h_));
+  std::memcpy(last_pbc_, box.pbc, sizeof(last_pbc_));
   const bool acc = (flags & 1u) != 0;
   prof_.begin(kProfFilter, s);
   check(launch_filter(b, dm_, N, nd1, nd2, type, pos, w_, s), "filter");
@@ -414,13 +375,57 @@ void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2,
   last_c2_ = nd2;
 }
 
+const int* NepEngine::read_flags(hipStream_t s)
+{
+  check(hipMemcpyAsync(hflags_, w_.flags, kNumFlags * sizeof(int), hipMemcpyDeviceToHost, s), "read flags");
+  check(hipStreamSynchronize(s), "synchronise");
+  return hflags_;
+}
+
+bool NepEngine::settle(hipStream_t s)
+{
+  read_flags(s);
+  if (hflags_[kFlagBadType]) {
+    check(hipMemsetAsync(w_.flags + kFlagBadType, 0, sizeof(int), s), "clear flag");
+    throw std::invalid_argument("an atom type lies outside [0, num_types)");
+  }
+  if (hflags_[kFlagOutOfBox]) {
+    check(hipMemsetAsync(w_.flags + kFlagOutOfBox, 0, sizeof(int), s), "clear flag");
+    box_changed_ = true;
+    throw std::invalid_argument("a position lies outside the box in a non-periodic direction");
+  }
+  if (hflags_[kFlagVerletOverflow] > w_.MV) {
+    const int mv = (int)std::ceil(hflags_[kFlagVerletOverflow] * 1.2) + 8;
+    if (mv > 4096) throw std::runtime_error("Verlet row overflow: more than 4096 neighbours within rc + skin");
+    check(hipMemsetAsync(w_.flags + kFlagVerletOverflow, 0, sizeof(int), s), "clear flag");
+    check(hipStreamSynchronize(s), "synchronise");
+    allocate(w_.cap, mv);  // drops the list: the next enqueue rebuilds
+    return true;
+  }
+  return false;
+}
+
+void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                        const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
+                        unsigned flags, hipStream_t s)
+{
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    enqueue(box, N, n1, n2, nd1, nd2, type, pos, gid, pe, force, virial, flags, s);
+    if (!settle(s)) return;
+    if (flags & 1u)
+      throw std::runtime_error(
+          "Verlet row overflow: accumulated outputs are invalid (workspace grown; clear them and call again)");
+  }
+  throw std::runtime_error("Verlet rows kept overflowing");
+}
+
 void NepEngine::stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds)
 {
   const int n = last_c2_ - last_c1_;
   int mr = 0, ma = 0, mv = 0;
+  check(hipDeviceSynchronize(), "sync");
   if (n > 0 && have_list_) {
     std::vector<int> a(n), b(n), c(n);
-    check(hipDeviceSynchronize(), "sync");
     check(hipMemcpy(a.data(), w_.nr + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
     check(hipMemcpy(b.data(), w_.na + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
     check(hipMemcpy(c.data(), w_.vn + last_c1_, n * sizeof(int), hipMemcpyDeviceToHost), "copy");
@@ -428,10 +433,12 @@ void NepEngine::stats(int* max_radial, int* max_angular, int* max_verlet, int* r
     ma = *std::max_element(b.begin(), b.end());
     mv = *std::max_element(c.begin(), c.end());
   }
+  int f[kNumFlags];
+  check(hipMemcpy(f, w_.flags, sizeof(f), hipMemcpyDeviceToHost), "copy");
   if (max_radial) *max_radial = mr;
   if (max_angular) *max_angular = ma;
   if (max_verlet) *max_verlet = mv;
-  if (rebuilds) *rebuilds = rebuilds_;
+  if (rebuilds) *rebuilds = f[kFlagRebuildCount];
 }
 
 void NepEngine::totals(long long* sum_radial, long long* sum_angular, long long* sum_verlet, int* centres)
@@ -549,6 +556,14 @@ MdState::~MdState()
   dfree(&type_);
   dfree(&tw_.partial);
   dfree(&tw_.thermo);
+  dfree(&ck_pos_);
+  dfree(&ck_vel_);
+  dfree(&ck_force_);
+  dfree(&ck_pe_);
+  dfree(&ck_vir_);
+  dfree(&rand_);
+  dfree(&factor_);
+  if (hrand_) (void)hipHostFree(hrand_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -564,24 +579,58 @@ void MdState::arrays(double** pos, double** vel, double** force, double** pe, do
   if (type) *type = type_;
 }
 
-void MdState::force(unsigned flags)
+void MdState::wrap()
 {
   DevBox b{};
   std::memcpy(b.h, box_.h, sizeof(b.h));
   box_inverse(box_.h, b.hi);
   for (int d = 0; d < 3; ++d) b.pbc[d] = box_.pbc[d];
   check(launch_wrap(b, N_, pos_, stream_), "wrap");
+}
+
+void MdState::force(unsigned flags)
+{
+  wrap();
   nep_->compute(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
 }
 
-void MdState::thermo(double* out)
+void MdState::enqueue_thermo()
 {
   Profiler& p = nep_->profiler();
   p.begin(kProfThermo, stream_);
   check(launch_thermo(N_, box_volume(box_.h), mass_, pe_, vel_, vir_, tw_, stream_), "thermo");
   p.end(stream_);
+}
+
+void MdState::thermo(double* out)
+{
+  enqueue_thermo();
   check(hipMemcpyAsync(out, tw_.thermo, 8 * sizeof(double), hipMemcpyDeviceToHost, stream_), "thermo copy");
   check(hipStreamSynchronize(stream_), "thermo");
+}
+
+void MdState::vv(int first_half, double dt)
+{
+  Profiler& p = nep_->profiler();
+  p.begin(kProfIntegrate, stream_);
+  check(launch_velocity_verlet(first_half, N_, dt, mass_, pos_, vel_, force_, stream_), "velocity Verlet");
+  p.end(stream_);
+}
+
+void MdState::checkpoint(bool save)
+{
+  struct {
+    double* live;
+    double** copy;
+    size_t n;
+  } buf[5] = {{pos_, &ck_pos_, 3}, {vel_, &ck_vel_, 3}, {force_, &ck_force_, 3}, {pe_, &ck_pe_, 1},
+              {vir_, &ck_vir_, 9}};
+  for (auto& e : buf) {
+    const size_t bytes = e.n * (size_t)N_ * sizeof(double);
+    if (!*e.copy) dalloc(e.copy, e.n * (size_t)N_, "checkpoint");
+    if (save) check(hipMemcpyAsync(*e.copy, e.live, bytes, hipMemcpyDeviceToDevice, stream_), "checkpoint");
+    else check(hipMemcpyAsync(e.live, *e.copy, bytes, hipMemcpyDeviceToDevice, stream_), "restore");
+  }
 }
 
 void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out)
@@ -594,34 +643,84 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
     force(flags);
     have_force_ = true;
   }
-  for (int step = 0; step < steps; ++step) {
-    if (ctl.needs_pre_force_ke()) {
+  // NHC needs the pre-force kinetic energy and NPT the pressure on the host
+  // every step (scalar chain/box state): those stay on the synchronous path.
+  if (ctl.needs_pre_force_ke() || ctl.has_barostat()) {
+    for (int step = 0; step < steps; ++step) {
+      if (ctl.needs_pre_force_ke()) {
+        thermo(th);
+        const double ke = 0.5 * th[0] * 3.0 * N_ * kBoltzmann;
+        if (auto f = ctl.before_force(dt, ke, step, steps))
+          check(launch_scale(3 * N_, *f, vel_, stream_), "scale velocity");
+      }
+      vv(1, dt);
+      force(flags);
+      vv(0, dt);
       thermo(th);
-      const double ke = 0.5 * th[0] * 3.0 * N_ * kBoltzmann;
-      if (auto f = ctl.before_force(dt, ke, step, steps))
-        check(launch_scale(3 * N_, *f, vel_, stream_), "scale velocity");
-    }
-    Profiler& p = nep_->profiler();
-    p.begin(kProfIntegrate, stream_);
-    check(launch_velocity_verlet(1, N_, dt, mass_, pos_, vel_, force_, stream_), "vv1");
-    p.end(stream_);
-    force(flags);
-    p.begin(kProfIntegrate, stream_);
-    check(launch_velocity_verlet(0, N_, dt, mass_, pos_, vel_, force_, stream_), "vv2");
-    p.end(stream_);
-    const bool last = step == steps - 1;
-    if (ctl.needs_thermo() || last) thermo(th);
-    if (auto f = ctl.after_force(th, step, steps)) check(launch_scale(3 * N_, *f, vel_, stream_), "scale velocity");
-    if (ctl.has_barostat()) {
-      const double s = ctl.barostat_scale(th, box_volume(box_.h), step, steps);
-      if (s != 1.0) {
-        for (double& v : box_.h) v *= s;
-        check(launch_scale(3 * N_, s, pos_, stream_), "scale positions");
-        nep_->notify_box_changed();
+      if (auto f = ctl.after_force(th, step, steps)) check(launch_scale(3 * N_, *f, vel_, stream_), "scale velocity");
+      if (ctl.has_barostat()) {
+        const double sc = ctl.barostat_scale(th, box_volume(box_.h), step, steps);
+        if (sc != 1.0) {
+          for (double& v : box_.h) v *= sc;
+          check(launch_scale(3 * N_, sc, pos_, stream_), "scale positions");
+          nep_->notify_box_changed();
+        }
       }
     }
+    if (steps == 0) thermo(th);
+    if (thermo_out) std::memcpy(thermo_out, th, sizeof(th));
+    return;
   }
-  if (steps == 0) thermo(th);
+  // Device-resident chunks: the rebuild decision and the thermostat factor
+  // are formed on the device, so a chunk is enqueued without a host
+  // synchronisation; the status flags are read once per chunk and a Verlet
+  // overflow rolls the chunk back to its checkpoint.
+  int kind = 0;
+  double coupling = 0.0;
+  if (ens.kind == Ensemble::nvt_ber) {
+    coupling = 1.0 / ens.t_coupling;
+    kind = coupling > 1.0e-5 ? 1 : 0;
+  } else if (ens.kind == Ensemble::nvt_bdp) {
+    kind = 2;
+    coupling = ens.t_coupling > 0.1 ? std::exp(-1.0 / ens.t_coupling) : 0.0;
+  }
+  constexpr int kChunk = 128;
+  if (!rand_) dalloc(&rand_, 2 * kChunk, "thermostat draws");
+  if (!factor_) dalloc(&factor_, 1, "thermostat factor");
+  if (!hrand_) check(hipHostMalloc((void**)&hrand_, 2 * kChunk * sizeof(double), hipHostMallocDefault), "pinned");
+  for (int c0 = 0; c0 < steps;) {
+    const int n = std::min(kChunk, steps - c0);
+    const Rng saved = ctl.rng();
+    checkpoint(true);
+    bool redo = false;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      if (kind == 2) {
+        check(hipStreamSynchronize(stream_), "staging");  // hrand_ reuse
+        for (int k = 0; k < n; ++k) ctl.draw_bdp(&hrand_[2 * k], &hrand_[2 * k + 1]);
+        check(hipMemcpyAsync(rand_, hrand_, 2 * n * sizeof(double), hipMemcpyHostToDevice, stream_), "draws");
+      }
+      for (int k = 0; k < n; ++k) {
+        const int step = c0 + k;
+        vv(1, dt);
+        wrap();
+        nep_->enqueue(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
+        vv(0, dt);
+        if (kind != 0) {
+          enqueue_thermo();
+          check(launch_ensemble_scale(kind, N_, coupling, ctl.target_temperature(step, steps), rand_ + 2 * k,
+                                      factor_, tw_.thermo, vel_, stream_),
+                "thermostat");
+        }
+      }
+      redo = nep_->settle(stream_);
+      if (!redo) break;
+      checkpoint(false);
+      ctl.set_rng(saved);
+    }
+    if (redo) throw std::runtime_error("Verlet rows kept overflowing");
+    c0 += n;
+  }
+  thermo(th);
   if (thermo_out) std::memcpy(thermo_out, th, sizeof(th));
 }
 

@@ -71,6 +71,15 @@ class NepEngine {
   void compute(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type, const double* pos,
                const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
                hipStream_t stream);
+  // Enqueues one evaluation without any host synchronisation (the Verlet
+  // rebuild decision is taken on the device).  settle() then synchronises,
+  // raises on invalid input, and returns true when a Verlet row overflowed:
+  // the workspace has been grown and the enqueued results must be redone.
+  void enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type, const double* pos,
+               const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
+               hipStream_t stream);
+  bool settle(hipStream_t stream);
+  const int* read_flags(hipStream_t stream);
   void notify_box_changed() { box_changed_ = true; }
   void stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds);
   void totals(long long* sum_radial, long long* sum_angular, long long* sum_verlet, int* centres);
@@ -83,8 +92,6 @@ class NepEngine {
   void ensure_cells(int ncell);
   void allocate(int capacity, int mv);
   void free_all();
-  void rebuild(const DevBox& b, int N, int nd1, int nd2, const double* pos, const uint64_t* gid,
-               hipStream_t s);
 
   NepModel model_;
   DevModel dm_{};
@@ -95,7 +102,6 @@ class NepEngine {
   int last_N_ = -1, last_nd1_ = -1, last_nd2_ = -1;
   double last_h_[9] = {};
   int last_pbc_[3] = {};
-  int rebuilds_ = 0;
   int last_N_stats_ = 0, last_c1_ = 0, last_c2_ = 0;
   Profiler prof_;
 };
@@ -113,7 +119,11 @@ class MdState {
 
  private:
   void force(unsigned flags);
+  void wrap();
+  void vv(int first_half, double dt);
+  void enqueue_thermo();
   void thermo(double* out);
+  void checkpoint(bool save);
   NepEngine* nep_;
   Box box_;
   int N_;
@@ -122,6 +132,10 @@ class MdState {
   int* type_ = nullptr;
   DevWork tw_{};  // thermo scratch only
   bool have_force_ = false;
+  double *ck_pos_ = nullptr, *ck_vel_ = nullptr, *ck_force_ = nullptr, *ck_pe_ = nullptr, *ck_vir_ = nullptr;
+  double* rand_ = nullptr;    // per-step thermostat draws of a chunk (device)
+  double* hrand_ = nullptr;   // pinned staging of the draws
+  double* factor_ = nullptr;  // device velocity factor
 };
 
 double box_volume(const double* h);

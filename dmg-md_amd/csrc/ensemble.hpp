@@ -73,6 +73,15 @@ class Controller {
   std::optional<double> after_force(const double* thermo, int step, int steps);
   // isotropic box factor (thermo before velocity scaling)
   double barostat_scale(const double* thermo, double volume, int step, int steps);
+  // The state-independent Bussi draws of one step, in resamplekin's order:
+  // the Gaussian rr, then the sum of ndeg-1 squared Gaussians.
+  void draw_bdp(double* rr, double* sumsq)
+  {
+    *rr = gasdev(rng_);
+    *sumsq = resamplekin_sumnoises(3 * n_ - 1, rng_);
+  }
+  const Rng& rng() const { return rng_; }
+  void set_rng(const Rng& r) { rng_ = r; }
 
  private:
   EnsembleConfig cfg_;

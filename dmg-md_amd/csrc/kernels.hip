@@ -19,7 +19,7 @@
 //    force/virial.  No floating-point atomics anywhere: results are
 //    deterministic and independent of launch order.
 //  * Integrator and thermo in FP64, thermo sums compensated (two-sum).
-#include "kernels.hpp"
+#include "device_common.hpp"
 
 #include <cmath>
 
@@ -41,84 +41,6 @@ void upload_angular_constants(const AngularConstants& a)
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-
-// Workgroup -> atom tile mapping that keeps consecutive tiles on one XCD:
-// the dispatcher deals workgroups round-robin over the 8 XCDs, so workgroup
-// b runs on XCD b%8; tile = (b%8)*per_xcd + b/8 gives each XCD one contiguous
-// (spatially coherent) range of atoms and its own L2 working set.
-__device__ __forceinline__ int xcd_tile(int nblocks)
-{
-  const int per = (nblocks + 7) >> 3;
-  const int b = blockIdx.x;
-  return (b & 7) * per + (b >> 3);
-}
-
-__host__ inline int xcd_grid(int nblocks) { return ((nblocks + 7) >> 3) << 3; }
-
-// FP64 minimum image (box.cuh:77-123), no FMA contraction so that the
-// predicate is bit-identical to the FP64 oracle.
-__device__ __forceinline__ void mic(const DevBox& b, double& x, double& y, double& z)
-{
-#pragma clang fp contract(off)
-  if (b.ortho) {
-    if (b.pbc[0]) {
-      const double L = b.h[0];
-      if (x < -L * 0.5) x += L;
-      else if (x > L * 0.5) x -= L;
-    }
-    if (b.pbc[1]) {
-      const double L = b.h[4];
-      if (y < -L * 0.5) y += L;
-      else if (y > L * 0.5) y -= L;
-    }
-    if (b.pbc[2]) {
-      const double L = b.h[8];
-      if (z < -L * 0.5) z += L;
-      else if (z > L * 0.5) z -= L;
-    }
-  } else {
-    double sx = b.hi[0] * x + b.hi[1] * y + b.hi[2] * z;
-    double sy = b.hi[3] * x + b.hi[4] * y + b.hi[5] * z;
-    double sz = b.hi[6] * x + b.hi[7] * y + b.hi[8] * z;
-    if (b.pbc[0]) sx -= rint(sx);
-    if (b.pbc[1]) sy -= rint(sy);
-    if (b.pbc[2]) sz -= rint(sz);
-    x = b.h[0] * sx + b.h[1] * sy + b.h[2] * sz;
-    y = b.h[3] * sx + b.h[4] * sy + b.h[5] * sz;
-    z = b.h[6] * sx + b.h[7] * sy + b.h[8] * sz;
-  }
-}
-
-__device__ __forceinline__ double dist2(double x, double y, double z)
-{
-#pragma clang fp contract(off)
-  return (x * x + y * y) + z * z;
-}
-
-// cell coordinates of a position; returns false when a non-periodic
-// coordinate lies outside the box
-__device__ __forceinline__ bool cell_coords(const DevBox& b, double x, double y, double z, int* c)
-{
-  const double s[3] = {b.hi[0] * x + b.hi[1] * y + b.hi[2] * z, b.hi[3] * x + b.hi[4] * y + b.hi[5] * z,
-                       b.hi[6] * x + b.hi[7] * y + b.hi[8] * z};
-  bool ok = true;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double u = s[d];
-    if (b.pbc[d]) {
-      u -= floor(u);
-    } else if (u < 0.0 || u >= 1.0) {
-      ok = false;
-      u = fmin(fmax(u, 0.0), 0.999999999);
-    }
-    int k = (int)(u * b.nc[d]);
-    c[d] = k < 0 ? 0 : (k >= b.nc[d] ? b.nc[d] - 1 : k);
-  }
-  return ok;
-}
-
-// type index clamped into [0, T) (invalid caller types are flagged by the filter)
-__device__ __forceinline__ int safe_type(int t, int T) { return (t < 0 || t >= T) ? 0 : t; }
 
 // cosine cutoff (nep_utilities.cuh:317-342)
 __device__ __forceinline__ void cutoff_fc(float rc, float rcinv, float d, float& fc)
@@ -221,213 +143,6 @@ __device__ __forceinline__ void harmonics(float ux, float uy, float uz, float* Y
       }
     }
   }
-}
-
-// ---------------------------------------------------------------------------
-// neighbour build
-// ---------------------------------------------------------------------------
-
-__global__ void k_cell_count(DevBox b, int N, const double* __restrict__ pos, int* __restrict__ cell_of,
-                             int* __restrict__ count, int* __restrict__ flags)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  int c[3];
-  if (!cell_coords(b, pos[i], pos[N + i], pos[2 * N + i], c)) atomicOr(&flags[kFlagOutOfBox], 1);
-  const int id = c[0] + b.nc[0] * (c[1] + b.nc[1] * c[2]);
-  cell_of[i] = id;
-  atomicAdd(&count[id], 1);
-}
-
-// three-phase exclusive scan (1024 counts per workgroup)
-__global__ void k_scan_local(int n, const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum)
-{
-  __shared__ int sh[256];
-  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
-  int v[4], sum = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    v[k] = base + k < n ? in[base + k] : 0;
-    sum += v[k];
-  }
-  sh[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = sh[threadIdx.x] - sum;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) out[base + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == 255) bsum[blockIdx.x] = sh[255];
-}
-
-__global__ void k_scan_blocks(int nb, int* __restrict__ bsum, int* __restrict__ total_out)
-{
-  __shared__ int sh[1024];
-  // one workgroup of 1024 threads; nb <= 1024*chunk handled by a serial chunk per thread
-  const int chunk = (nb + 1023) / 1024;
-  const int b0 = threadIdx.x * chunk;
-  int sum = 0;
-  for (int k = 0; k < chunk; ++k)
-    if (b0 + k < nb) sum += bsum[b0 + k];
-  sh[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = sh[threadIdx.x] - sum;
-  for (int k = 0; k < chunk; ++k)
-    if (b0 + k < nb) {
-      const int v = bsum[b0 + k];
-      bsum[b0 + k] = run;
-      run += v;
-    }
-  if (threadIdx.x == 1023) *total_out = sh[1023];
-}
-
-__global__ void k_scan_add(int n, int* __restrict__ out, const int* __restrict__ bsum)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += bsum[i >> 10];
-}
-
-__global__ void k_cell_fill(int N, const int* __restrict__ cell_of, const int* __restrict__ start,
-                            int* __restrict__ fill, int* __restrict__ atoms)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int c = cell_of[i];
-  atoms[start[c] + atomicAdd(&fill[c], 1)] = i;
-}
-
-// One thread per centre: scan the 5x5x5 cells around it (cell edge >= (rc+skin)/2)
-// and keep every slot with FP64 d2 < (rc+skin)^2 (neighbor.cu:141-218).
-__global__ __launch_bounds__(kBlock) void k_verlet_build(DevBox b, int N, int c1, int c2,
-                                                         const double* __restrict__ pos, double rv2,
-                                                         DevWork w)
-{
-  const int i = c1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c2) return;
-  const double xi = pos[i], yi = pos[N + i], zi = pos[2 * N + i];
-  int c[3];
-  cell_coords(b, xi, yi, zi, c);
-  const int rng[3] = {b.pbc[0] ? 2 : 2, b.pbc[1] ? 2 : 2, b.pbc[2] ? 2 : 2};
-  int cnt = 0;
-  for (int dz = -rng[2]; dz <= rng[2]; ++dz) {
-    int cz = c[2] + dz;
-    if (b.pbc[2]) cz = (cz + b.nc[2]) % b.nc[2];
-    else if (cz < 0 || cz >= b.nc[2]) continue;
-    for (int dy = -rng[1]; dy <= rng[1]; ++dy) {
-      int cy = c[1] + dy;
-      if (b.pbc[1]) cy = (cy + b.nc[1]) % b.nc[1];
-      else if (cy < 0 || cy >= b.nc[1]) continue;
-      for (int dx = -rng[0]; dx <= rng[0]; ++dx) {
-        int cx = c[0] + dx;
-        if (b.pbc[0]) cx = (cx + b.nc[0]) % b.nc[0];
-        else if (cx < 0 || cx >= b.nc[0]) continue;
-        const int cell = cx + b.nc[0] * (cy + b.nc[1] * cz);
-        const int e = w.cell_start[cell + 1];
-        for (int k = w.cell_start[cell]; k < e; ++k) {
-          const int j = w.cell_atoms[k];
-          if (j == i) continue;
-          double x = pos[j] - xi, y = pos[N + j] - yi, z = pos[2 * N + j] - zi;
-          mic(b, x, y, z);
-          if (dist2(x, y, z) < rv2) {
-            if (cnt < w.MV) w.vl[(size_t)cnt * N + i] = j;
-            ++cnt;
-          }
-        }
-      }
-    }
-  }
-  w.vn[i] = cnt;
-  if (cnt > w.MV) atomicMax(&w.flags[kFlagVerletOverflow], cnt);
-}
-
-__device__ __forceinline__ bool key_less(const uint64_t* gid, int a, int b)
-{
-  if (gid) {
-    const uint64_t ga = gid[a], gb = gid[b];
-    if (ga != gb) return ga < gb;
-  }
-  return a < b;
-}
-
-// One wavefront per row: rank sort in LDS (keys unique: slot, or (gid, slot)).
-__global__ __launch_bounds__(256) void k_verlet_sort(int N, int c1, int c2, const uint64_t* __restrict__ gid,
-                                                     DevWork w)
-{
-  extern __shared__ int sh[];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = c1 + blockIdx.x * 4 + wave;
-  int* row = sh + wave * w.MV;
-  const int n = i < c2 ? w.vn[i] : 0;
-  const int nn = n < w.MV ? n : w.MV;
-  for (int k = lane; k < nn; k += 64) row[k] = w.vl[(size_t)k * N + i];
-  __syncthreads();
-  for (int k = lane; k < nn; k += 64) {
-    const int v = row[k];
-    int rank = 0;
-    for (int t = 0; t < nn; ++t) rank += key_less(gid, row[t], v) ? 1 : 0;
-    w.vl[(size_t)rank * N + i] = v;
-  }
-}
-
-// Position of the centre inside each neighbour's row (binary search on the
-// row key); -1 when the neighbour has no row (outside the centre range).
-__global__ __launch_bounds__(kBlock) void k_verlet_reverse(int N, int c1, int c2,
-                                                           const uint64_t* __restrict__ gid, DevWork w)
-{
-  const int i = c1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c2) return;
-  const int n = min(w.vn[i], w.MV);
-  for (int p = 0; p < n; ++p) {
-    const int j = w.vl[(size_t)p * N + i];
-    int q = -1;
-    if (j >= c1 && j < c2) {
-      int lo = 0, hi = min(w.vn[j], w.MV) - 1;
-      while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const int v = w.vl[(size_t)mid * N + j];
-        if (v == i) {
-          q = mid;
-          break;
-        }
-        if (key_less(gid, v, i)) lo = mid + 1;
-        else hi = mid - 1;
-      }
-    }
-    w.vrev[(size_t)p * N + i] = q;
-  }
-}
-
-__global__ void k_save_positions(int N, const double* __restrict__ pos, double* __restrict__ x0)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 3 * N) x0[i] = pos[i];
-}
-
-// any atom displaced by more than skin/2 since the last rebuild (neighbor.cu:446-490)
-__global__ void k_check_moved(DevBox b, int N, const double* __restrict__ pos, const double* __restrict__ x0,
-                              double lim2, int* __restrict__ flags)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool moved = false;
-  if (i < N) {
-    double x = pos[i] - x0[i], y = pos[N + i] - x0[N + i], z = pos[2 * N + i] - x0[2 * N + i];
-    mic(b, x, y, z);
-    moved = dist2(x, y, z) > lim2;
-  }
-  if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(&flags[kFlagMoved], 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1112,6 +827,37 @@ __global__ __launch_bounds__(kThermoThreads) void k_thermo_final(int nb, int N, 
   }
 }
 
+// velocity factor from this step's normalised thermo, computed where the
+// thermo lives (same expressions as Controller::after_force, no contraction)
+__global__ void k_ensemble_factor(int kind, int N, double coupling, double t_target,
+                                  const double* __restrict__ rand2, const double* __restrict__ thermo,
+                                  double* __restrict__ factor)
+{
+#pragma clang fp contract(off)
+  double f = 1.0;
+  const double T = thermo[0];
+  if (kind == 1) {
+    f = sqrt(1.0 + coupling * (t_target / T - 1.0));
+  } else if (kind == 2) {
+    // resamplekin (thermostat_svr.cpp:107-124) with rr and the chi-square sum drawn on the host
+    const int ndeg = 3 * N;
+    const double kk = T * (ndeg * 8.617343e-5 * 0.5);  // old_KE *= ndeg * K_B * 0.5
+    const double sigma = ndeg * 8.617343e-5 * t_target * 0.5;
+    const double fac = coupling;  // exp(-1/taut) or 0, formed on the host
+    const double rr = rand2[0], sum = rand2[1];
+    const double knew = kk + (1.0 - fac) * (sigma * (sum + rr * rr) / ndeg - kk) +
+                        2.0 * rr * sqrt(kk * sigma / ndeg * (1.0 - fac) * fac);
+    f = sqrt(knew / kk);
+  }
+  factor[0] = f;
+}
+
+__global__ void k_scale_by(int n, const double* __restrict__ factor, double* __restrict__ data)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) data[i] *= factor[0];
+}
+
 __global__ void k_scale(int n, double factor, double* __restrict__ data)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1129,68 +875,6 @@ __global__ void k_check_scaled(int n, double factor, const double* __restrict__ 
 // ---------------------------------------------------------------------------
 
 static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
-
-hipError_t launch_cell_count(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s)
-{
-  const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
-  hipError_t e = hipMemsetAsync(w.cell_count, 0, sizeof(int) * ncell, s);
-  if (e != hipSuccess) return e;
-  if (N > 0) k_cell_count<<<nblk(N), kBlock, 0, s>>>(b, N, pos, w.cell_of, w.cell_count, w.flags);
-  return hipGetLastError();
-}
-
-hipError_t launch_cell_scan(int ncell, DevWork& w, hipStream_t s)
-{
-  const int nb = (ncell + 1023) / 1024;
-  // block sums reuse cell_fill as scratch (nb <= ncell)
-  k_scan_local<<<nb, 256, 0, s>>>(ncell, w.cell_count, w.cell_start, w.cell_fill);
-  k_scan_blocks<<<1, 1024, 0, s>>>(nb, w.cell_fill, w.cell_start + ncell);
-  k_scan_add<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, w.cell_start, w.cell_fill);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  return hipMemsetAsync(w.cell_fill, 0, sizeof(int) * ncell, s);
-}
-
-hipError_t launch_cell_fill(int N, DevWork& w, hipStream_t s)
-{
-  if (N > 0) k_cell_fill<<<nblk(N), kBlock, 0, s>>>(N, w.cell_of, w.cell_start, w.cell_fill, w.cell_atoms);
-  return hipGetLastError();
-}
-
-hipError_t launch_verlet_build(const DevBox& b, int N, int c1, int c2, const double* pos, double rv2,
-                               DevWork& w, hipStream_t s)
-{
-  if (c2 > c1) k_verlet_build<<<nblk(c2 - c1), kBlock, 0, s>>>(b, N, c1, c2, pos, rv2, w);
-  return hipGetLastError();
-}
-
-hipError_t launch_verlet_sort(int N, int c1, int c2, const uint64_t* gid, DevWork& w, hipStream_t s)
-{
-  if (c2 > c1) {
-    const size_t lds = sizeof(int) * 4 * (size_t)w.MV;
-    k_verlet_sort<<<(c2 - c1 + 3) / 4, 256, lds, s>>>(N, c1, c2, gid, w);
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_verlet_reverse(int N, int c1, int c2, const uint64_t* gid, DevWork& w, hipStream_t s)
-{
-  if (c2 > c1) k_verlet_reverse<<<nblk(c2 - c1), kBlock, 0, s>>>(N, c1, c2, gid, w);
-  return hipGetLastError();
-}
-
-hipError_t launch_save_positions(int N, const double* pos, DevWork& w, hipStream_t s)
-{
-  if (N > 0) k_save_positions<<<nblk(3 * N), kBlock, 0, s>>>(N, pos, w.x0);
-  return hipGetLastError();
-}
-
-hipError_t launch_check_moved(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s)
-{
-  const double half = 0.5 * kSkin;
-  if (N > 0) k_check_moved<<<nblk(N), kBlock, 0, s>>>(b, N, pos, w.x0, half * half, w.flags);
-  return hipGetLastError();
-}
 
 hipError_t launch_filter(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type,
                          const double* pos, DevWork& w, hipStream_t s)
@@ -1283,6 +967,14 @@ hipError_t launch_thermo(int N, double volume, const double* mass, const double*
 hipError_t launch_scale(int n, double factor, double* data, hipStream_t s)
 {
   if (n > 0) k_scale<<<nblk(n), kBlock, 0, s>>>(n, factor, data);
+  return hipGetLastError();
+}
+
+hipError_t launch_ensemble_scale(int kind, int N, double coupling, double t_target, const double* rand2,
+                                 double* factor, const double* thermo, double* vel, hipStream_t s)
+{
+  k_ensemble_factor<<<1, 1, 0, s>>>(kind, N, coupling, t_target, rand2, thermo, factor);
+  k_scale_by<<<nblk(3 * N), kBlock, 0, s>>>(3 * N, factor, vel);
   return hipGetLastError();
 }
 

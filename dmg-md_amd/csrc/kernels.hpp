@@ -78,20 +78,21 @@ enum : int {
   kFlagOutOfBox = 4,        // a position lies outside the box in a non-periodic direction
   kFlagNonFinite = 5,       // non-finite scaled position
   kFlagBadType = 6,         // a type index outside [0, num_types)
+  kFlagRebuild = 7,         // a Verlet rebuild is requested for this step
+  kFlagRebuildCount = 8,    // rebuilds done since the flags were allocated
+  kNumFlags = 16,
 };
 
 void upload_angular_constants(const AngularConstants& a);
 
-// Neighbour build (per rebuild)
-hipError_t launch_cell_count(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s);
-hipError_t launch_cell_scan(int ncell, DevWork& w, hipStream_t s);
-hipError_t launch_cell_fill(int N, DevWork& w, hipStream_t s);
-hipError_t launch_verlet_build(const DevBox& b, int N, int c1, int c2, const double* pos, double rv2,
-                               DevWork& w, hipStream_t s);
-hipError_t launch_verlet_sort(int N, int c1, int c2, const uint64_t* gid, DevWork& w, hipStream_t s);
-hipError_t launch_verlet_reverse(int N, int c1, int c2, const uint64_t* gid, DevWork& w, hipStream_t s);
-hipError_t launch_save_positions(int N, const double* pos, DevWork& w, hipStream_t s);
+// Neighbour build (neighbor.hip).  launch_rebuild enqueues the whole cell
+// list + Verlet build; every kernel returns at once unless flags[kFlagRebuild]
+// is set, which launch_check_moved (displacement > skin/2) or
+// launch_request_rebuild (first call, box change) do on the device.
+hipError_t launch_request_rebuild(DevWork& w, hipStream_t s);
 hipError_t launch_check_moved(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s);
+hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, const double* pos, double rv2,
+                          const uint64_t* gid, DevWork& w, hipStream_t s);
 
 // Force evaluation (per step)
 hipError_t launch_filter(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type,
@@ -109,6 +110,12 @@ hipError_t launch_wrap(const DevBox& b, int N, double* pos, hipStream_t s);
 hipError_t launch_thermo(int N, double volume, const double* mass, const double* pe, const double* vel,
                          const double* virial, DevWork& w, hipStream_t s);
 hipError_t launch_scale(int n, double factor, double* data, hipStream_t s);
+// Device-side velocity factor from the thermo of this step (no host read):
+// kind 1 Berendsen sqrt(1 + c (T_t/T - 1)); kind 2 Bussi with the step's
+// pre-drawn Gaussian rr and chi-square sum R (the same expression and
+// rounding as the host resamplekin).  Writes factor[0] and rescales v.
+hipError_t launch_ensemble_scale(int kind, int N, double coupling, double t_target, const double* rand2,
+                                 double* factor, const double* thermo, double* vel, hipStream_t s);
 hipError_t launch_check_scaled(int n, double factor, const double* data, int* flag, hipStream_t s);
 
 // Number of thermo block partials launch_thermo uses for N atoms.
