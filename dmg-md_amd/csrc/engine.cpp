@@ -230,7 +230,9 @@ NepEngine::~NepEngine()
 
 void NepEngine::free_all()
 {
+  dfree(&w_.xq);
   dfree(&w_.cell_of);
+  dfree(&w_.cell_rank);
   dfree(&w_.cell_atoms);
   dfree(&w_.vn);
   dfree(&w_.vl);
@@ -243,10 +245,8 @@ void NepEngine::free_all()
   dfree(&w_.vsa);
   dfree(&w_.bR);
   dfree(&w_.f12);
-  dfree(&w_.f12t);
   dfree(&w_.cell_count);
   dfree(&w_.cell_start);
-  dfree(&w_.cell_fill);
   w_.cap = 0;
   w_.ncell_cap = 0;
   have_list_ = false;
@@ -256,7 +256,9 @@ void NepEngine::allocate(int capacity, int mv)
 {
   const size_t cap = (size_t)std::max(capacity, 1);
   const size_t rows = cap * (size_t)mv;
+  dalloc(&w_.xq, cap, "packed positions");
   dalloc(&w_.cell_of, cap, "cell_of");
+  dalloc(&w_.cell_rank, cap, "cell rank");
   dalloc(&w_.cell_atoms, cap, "cell_atoms");
   dalloc(&w_.vn, cap, "vn");
   dalloc(&w_.vl, rows, "verlet rows");
@@ -270,7 +272,6 @@ void NepEngine::allocate(int capacity, int mv)
   check(hipMemset(w_.vsa, 0xff, rows * sizeof(int)), "clear verlet->angular");
   dalloc(&w_.bR, cap * model_.num_types * model_.kR, "radial adjoint");
   dalloc(&w_.f12, rows, "angular partials");
-  dalloc(&w_.f12t, rows, "angular partials (transposed)");
   w_.cap = (int)cap;
   w_.MV = mv;
   w_.MR = mv;
@@ -291,8 +292,7 @@ void NepEngine::ensure_cells(int ncell)
   if (ncell <= w_.ncell_cap) return;
   dalloc(&w_.cell_count, (size_t)ncell, "cell counts");
   dalloc(&w_.cell_start, (size_t)ncell + 1, "cell starts");
-  dalloc(&w_.cell_fill, (size_t)ncell, "cell fill");
-  check(hipMemset(w_.cell_fill, 0, sizeof(int) * ncell), "clear cell fill");
+  check(hipMemset(w_.cell_count, 0, sizeof(int) * ncell), "clear cell counts");
   w_.ncell_cap = ncell;
 }
 
@@ -345,12 +345,11 @@ void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2,
                       std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0;
   // rebuild decision on the device: forced by the host, or any atom moved > skin/2
   prof_.begin(kProfCheck, s);
-  if (forced) check(launch_request_rebuild(w_, s), "rebuild request");
-  else check(launch_check_moved(b, N, pos, w_, s), "displacement check");
+  check(launch_pack(b, N, model_.num_types, pos, type, forced, w_, s), "pack / displacement check");
   prof_.end(s);
   const double rv = model_.rcR_max + kSkin;
   prof_.begin(kProfRebuild, s);
-  check(launch_rebuild(b, N, nd1, nd2, pos, rv * rv, by_gid ? gid : nullptr, w_, s), "rebuild");
+  check(launch_rebuild(b, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w_, s), "rebuild");
   prof_.end(s);
   have_list_ = true;
   box_changed_ = false;
@@ -362,13 +361,13 @@ void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2,
   std::memcpy(last_pbc_, box.pbc, sizeof(last_pbc_));
   const bool acc = (flags & 1u) != 0;
   prof_.begin(kProfFilter, s);
-  check(launch_filter(b, dm_, N, nd1, nd2, type, pos, w_, s), "filter");
+  check(launch_filter(b, dm_, nd1, nd2, w_, s), "filter");
   prof_.end(s);
   prof_.begin(kProfDescriptor, s);
   check(launch_descriptor(dm_, N, nd1, nd2, type, pe, acc, w_, s), "descriptor");
   prof_.end(s);
   prof_.begin(kProfGather, s);
-  check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, pe, force, virial, acc, w_, s), "gather");
+  check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s), "gather");
   prof_.end(s);
   last_N_stats_ = N;
   last_c1_ = nd1;
@@ -477,7 +476,7 @@ void NepEngine::copy_verlet_rows(int c1, int c2, int max_nb, int* counts, int* r
     counts[i - c1] = vn[i];
     if (n > max_nb) throw std::invalid_argument("max_nb too small");
     std::vector<int> r(n);
-    for (int p = 0; p < n; ++p) r[p] = vl[(size_t)p * N + i];
+    for (int p = 0; p < n; ++p) r[p] = vl[(size_t)i * w_.MV + p];
     std::sort(r.begin(), r.end());
     std::copy(r.begin(), r.end(), rows + (size_t)(i - c1) * max_nb);
   }
@@ -501,7 +500,7 @@ void NepEngine::copy_pair_rows(int which, int c1, int c2, int max_nb, int* count
     std::vector<int> r(n);
     for (int k = 0; k < n; ++k) {
       const int slot = k < na[i] ? k : w_.MR - 1 - (k - na[i]);
-      float w = e[(size_t)slot * N + i].w;
+      float w = e[(size_t)i * w_.MR + slot].w;
       int j;
       std::memcpy(&j, &w, sizeof(int));
       r[k] = j;

@@ -4,20 +4,21 @@
 //  * Verlet list with a 1 A skin, rebuilt only when an atom moved more than
 //    skin/2 (neighbor.cu:446-582); rows hold slot indices, sorted ascending
 //    (or by global id), with a precomputed reverse position per entry.
-//  * Every step a filter pass turns each Verlet row into one compact edge row:
-//    angular edges first (slots 0..na-1, ascending), radial-only edges from the
-//    end of the row downward, each edge a float4 (FP64 minimum-image r_ij
-//    rounded to float, neighbour slot).  All later kernels stream these rows
-//    with coalesced loads instead of re-gathering positions.
+//  * Every step a filter pass (16 lanes per centre, ballot compaction) turns
+//    each Verlet row into one compact edge row: angular edges first (slots
+//    0..na-1, ascending), radial-only edges from the end of the row
+//    downward, each edge a float4 (FP64 minimum-image r_ij rounded to float,
+//    neighbour slot).  Later kernels stream these rows instead of
+//    re-gathering positions.  All rows are row-major ([i*M + slot]).
 //  * One fused per-atom kernel does descriptor -> ANN -> adjoints -> angular
 //    partial forces: the angular moments s[n][lm] stay in registers, are
 //    turned in place into the adjoint dE/ds, and each angular edge's
-//    dE_i/dr_ij is written both at i's slot and (transposed) at j's slot for
-//    i, so the gather kernel reads both directions with coalesced loads.
-//  * The gather kernel forms the radial pair force from both centres'
-//    basis-space adjoints, adds the angular partials and ZBL, and writes FP64
-//    force/virial.  No floating-point atomics anywhere: results are
-//    deterministic and independent of launch order.
+//    dE_i/dr_ij is written at i's slot.
+//  * The gather kernel (8 lanes per atom) forms the radial pair force from
+//    both centres' basis-space adjoints, adds the angular partials of both
+//    directions (dE_j/dr_ji read from j's row through the reverse index) and
+//    ZBL, and writes FP64 force/virial.  No floating-point atomics anywhere:
+//    results are deterministic and independent of launch order.
 //  * Integrator and thermo in FP64, thermo sums compensated (two-sum).
 #include "device_common.hpp"
 
@@ -60,7 +61,8 @@ __device__ __forceinline__ void cutoff_fc_fcp(float rc, float rcinv, float d, fl
   }
 }
 
-// Chebyshev radial basis f_k = (T_k(x)+1)/2 fc, x = 2(d/rc-1)^2-1
+// Chebyshev radial basis f_k = (T_k(x)+1)/2 fc, x = 2(d/rc-1)^2-1; entries
+// k >= kn are zero
 template <int K>
 __device__ __forceinline__ void cheb(int kn, float rcinv, float d, float fc, float* f)
 {
@@ -72,11 +74,10 @@ __device__ __forceinline__ void cheb(int kn, float rcinv, float d, float fc, flo
   if (K > 1) f[1] = (x + 1.0f) * hfc;
 #pragma unroll
   for (int k = 2; k < K; ++k) {
-    if (k >= kn) break;
     const float tk = 2.0f * x * tm1 - tm2;
     tm2 = tm1;
     tm1 = tk;
-    f[k] = (tk + 1.0f) * hfc;
+    f[k] = k < kn ? (tk + 1.0f) * hfc : 0.0f;
   }
 }
 
@@ -97,12 +98,11 @@ __device__ __forceinline__ void cheb_d(int kn, float rcinv, float d, float fc, f
   }
 #pragma unroll
   for (int k = 2; k < K; ++k) {
-    if (k >= kn) break;
     const float tk = 2.0f * x * tm1 - tm2;
     const float v = 0.5f * (tk + 1.0f);
     const float der = 0.5f * k * um1 * dx;  // (T_k'/2) dx/dd, T_k' = k U_{k-1}
-    f[k] = v * fc;
-    fp[k] = der * fc + v * fcp;
+    f[k] = k < kn ? v * fc : 0.0f;
+    fp[k] = k < kn ? der * fc + v * fcp : 0.0f;
     tm2 = tm1;
     tm1 = tk;
     const float uk = 2.0f * x * um1 - um2;
@@ -145,54 +145,145 @@ __device__ __forceinline__ void harmonics(float ux, float uy, float uz, float* Y
   }
 }
 
+// Screened-Coulomb ZBL pair energy f(d) and df/dd with the cosine switch
+// between r_in and r_out (nep_utilities.cuh:347-433); zp is one pair's
+// {1/a, Zi Zj K_C, r_in, r_out, (c_k, d_k) x4} record.
+__device__ __forceinline__ void zbl_pair(const float* __restrict__ zp, float d, float dinv, float& f, float& fp)
+{
+  const float x = d * zp[0];
+  float phi = 0.0f, dphi = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float t = zp[4 + 2 * k] * expf(-zp[5 + 2 * k] * x);
+    phi += t;
+    dphi -= zp[5 + 2 * k] * t;
+  }
+  const float vv = zp[1] * phi * dinv;
+  const float vp = zp[1] * dphi * zp[0] * dinv - vv * dinv;
+  const float r1 = zp[2], r2 = zp[3];
+  float fc = 0.0f, fcp = 0.0f;
+  if (d < r1) {
+    fc = 1.0f;
+  } else if (d < r2) {
+    const float wv = 3.14159265f / (r2 - r1);
+    float sn, cs;
+    sincosf(wv * (d - r1), &sn, &cs);
+    fc = 0.5f * cs + 0.5f;
+    fcp = -0.5f * wv * sn;
+  }
+  f = vv * fc;
+  fp = vp * fc + vv * fcp;
+}
+
 // ---------------------------------------------------------------------------
 // per-step typewise filter (nep.cu:503-561) into compact edge rows
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int N, int c1, int c2, int nblocks,
-                                                   const int* __restrict__ type, const double* __restrict__ pos,
-                                                   DevWork w)
+// kFilterLanes lanes per centre walk its Verlet row kFilterLanes entries per
+// round; the kept entries of each round are compacted with a wavefront
+// ballot, so every store of the kernel is contiguous inside the row.  The
+// loads of kFilterBatch rounds (row entries, their reverse positions, then
+// the neighbours' packed positions) are issued before any of them is used,
+// so a wave waits for two memory latencies per batch instead of two per
+// round.  Types were validated when the step's positions were packed; here
+// they are only clamped into range.  Pair cutoffs are FP64
+// ((rc_i + rc_j) / 2)^2, as the reference's typewise test (nep.cu:503-561).
+constexpr int kFilterLanes = 16;
+constexpr int kFilterBatch = 3;
+constexpr int kGatherLanes = 4;
+
+template <bool ONE>
+__global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1, int c2, int nblocks, DevWork w)
 {
-  const int i = c1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
-  if (i >= c2) return;
-  const double xi = pos[i], yi = pos[N + i], zi = pos[2 * N + i];
-  int ti = type[i];
-  if (ti < 0 || ti >= m.T) {
-    atomicOr(&w.flags[kFlagBadType], 1);
-    ti = 0;
+  constexpr int G = kFilterLanes, PER = kBlock / G, RB = kFilterBatch;
+  __shared__ double cut2[ONE ? 1 : 2 * kMaxTypes * kMaxTypes];  // [ti][tj] radial, then angular
+  const int T = m.T;
+  double rr2 = 0.0, ra2 = 0.0;
+  if (ONE) {
+    const double rr = (m.rcR[0] + m.rcR[0]) * 0.5, ra = (m.rcA[0] + m.rcA[0]) * 0.5;
+    rr2 = rr * rr;
+    ra2 = ra * ra;
+  } else {
+    for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+      const int p = k / T, q = k % T;
+      const double rr = (m.rcR[p] + m.rcR[q]) * 0.5, ra = (m.rcA[p] + m.rcA[q]) * 0.5;
+      cut2[k] = rr * rr;
+      cut2[T * T + k] = ra * ra;
+    }
+    __syncthreads();
   }
-  const int nv = min(w.vn[i], w.MV);
+  const int g = threadIdx.x / G, lg = threadIdx.x % G, gw = g % (64 / G);
+  const int i = c1 + xcd_tile(nblocks) * PER + g;
+  const bool active = i < c2;
+  double4 pi = make_double4(0, 0, 0, 0);
+  int ti = 0, nv = 0;
+  if (active) {
+    pi = w.xq[i];
+    ti = ONE ? 0 : safe_type((int)__double_as_longlong(pi.w), T);
+    nv = min(w.vn[i], w.MV);
+  }
+  const size_t vrow = (size_t)i * w.MV, erow = (size_t)i * w.MR, arow = (size_t)i * w.MA;
+  const unsigned below = (1u << lg) - 1u;
   int na = 0, nro = 0;
-  for (int p = 0; p < nv; ++p) {
-    const int j = w.vl[(size_t)p * N + i];
-    double x = pos[j] - xi, y = pos[N + j] - yi, z = pos[2 * N + j] - zi;
-    mic(b, x, y, z);
-    const double d2 = dist2(x, y, z);
-    int tj = type[j];
-    if (tj < 0 || tj >= m.T) {
-      atomicOr(&w.flags[kFlagBadType], 1);
-      tj = 0;
-    }
-    const double rr = (m.rcR[ti] + m.rcR[tj]) * 0.5;
-    if (!(d2 < rr * rr)) continue;
-    const double ra = (m.rcA[ti] + m.rcA[tj]) * 0.5;
-    const float4 rec = make_float4((float)x, (float)y, (float)z, __int_as_float(j));
-    if (d2 < ra * ra) {
-      if (na < w.MA && na + nro < w.MR) {
-        w.edge[(size_t)na * N + i] = rec;
-        w.eq[(size_t)na * N + i] = w.vrev[(size_t)p * N + i];
-        w.vsa[(size_t)p * N + i] = na;
+  for (int p0 = 0; __any(p0 < nv); p0 += G * RB) {
+    int jj[RB], qq[RB];
+    double4 pj[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int p = p0 + r * G + lg;
+      jj[r] = -1;
+      qq[r] = -1;
+      if (p < nv) {
+        jj[r] = w.vl[vrow + p];
+        qq[r] = w.vrev[vrow + p];
       }
-      ++na;
-    } else {
-      if (na + nro < w.MR) w.edge[(size_t)(w.MR - 1 - nro) * N + i] = rec;
-      ++nro;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) pj[r] = jj[r] >= 0 ? w.xq[jj[r]] : make_double4(0, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int p = p0 + r * G + lg;
+      bool isA = false, isR = false;
+      float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (jj[r] >= 0) {
+        double x = pj[r].x - pi.x, y = pj[r].y - pi.y, z = pj[r].z - pi.z;
+        mic(b, x, y, z);
+        const double d2 = dist2(x, y, z);
+        double cr = rr2, ca = ra2;
+        if (!ONE) {
+          const int tj = safe_type((int)__double_as_longlong(pj[r].w), T);
+          cr = cut2[ti * T + tj];
+          ca = cut2[T * T + ti * T + tj];
+        }
+        if (d2 < cr) {
+          isA = d2 < ca;
+          isR = !isA;
+          rec = make_float4((float)x, (float)y, (float)z, __int_as_float(jj[r]));
+        }
+      }
+      const unsigned gA = (unsigned)(__ballot(isA) >> (gw * G)) & ((1u << G) - 1u);
+      const unsigned gR = (unsigned)(__ballot(isR) >> (gw * G)) & ((1u << G) - 1u);
+      if (isA) {
+        const int a = na + __popc(gA & below);
+        if (a < w.MA) {
+          w.edge[erow + a] = rec;
+          w.eq[arow + a] = qq[r];
+          w.vsa[vrow + p] = a;
+        }
+      }
+      if (isR) {
+        const int slot = w.MR - 1 - (nro + __popc(gR & below));
+        if (slot >= 0) w.edge[erow + slot] = rec;
+      }
+      na += __popc(gA);
+      nro += __popc(gR);
     }
   }
-  w.nr[i] = min(na + nro, w.MR);
-  w.na[i] = min(na, min(w.MA, w.MR));
-  if (na + nro > w.MR) atomicMax(&w.flags[kFlagRadialOverflow], na + nro);
-  if (na > w.MA) atomicMax(&w.flags[kFlagAngularOverflow], na);
+  // na + nro <= nv <= MV = MR = MA: the two ends of a row never meet
+  if (active && lg == 0) {
+    w.nr[i] = min(na + nro, w.MR);
+    w.na[i] = min(na, w.MA);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -230,6 +321,8 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
     rcAf[0] = (float)m.rcA[0];
   }
   const int nr = w.nr[i], na = w.na[i];
+  const float4* __restrict__ erow = w.edge + (size_t)i * w.MR;
+  float4* __restrict__ frow = w.f12 + (size_t)i * w.MA;
   float q[DMAX];
 #pragma unroll
   for (int d = 0; d < DMAX; ++d) q[d] = 0.0f;
@@ -238,9 +331,11 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
   float Fs[KR];
 #pragma unroll
   for (int k = 0; k < KR; ++k) Fs[k] = 0.0f;
+  // each loop below loads the next edge record before using the current one
+  float4 rn = erow[0 < na ? 0 : w.MR - 1];
   for (int e = 0; e < nr; ++e) {
-    const int slot = e < na ? e : w.MR - 1 - (e - na);
-    const float4 r = w.edge[(size_t)slot * N + i];
+    const float4 r = rn;
+    if (e + 1 < nr) rn = erow[e + 1 < na ? e + 1 : w.MR - 1 - (e + 1 - na)];
     const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
     float rc;
     int tj = 0;
@@ -263,24 +358,26 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
       const float* c = m.cR + (size_t)(ti * T + tj) * nR * kR;
 #pragma unroll
       for (int n = 0; n < NR; ++n) {
-        if (n >= nR) break;
-        float g = 0.0f;
+        if (n < nR) {
+          float g = 0.0f;
 #pragma unroll
-        for (int k = 0; k < KR; ++k)
-          if (k < kR) g += c[n * kR + k] * f[k];
-        q[n] += g;
+          for (int k = 0; k < KR; ++k)
+            if (k < kR) g += c[n * kR + k] * f[k];
+          q[n] += g;
+        }
       }
     }
   }
   if (C::one) {
 #pragma unroll
     for (int n = 0; n < NR; ++n) {
-      if (n >= nR) break;
-      float g = 0.0f;
+      if (n < nR) {
+        float g = 0.0f;
 #pragma unroll
-      for (int k = 0; k < KR; ++k)
-        if (k < kR) g += m.cR[n * kR + k] * Fs[k];
-      q[n] = g;
+        for (int k = 0; k < KR; ++k)
+          if (k < kR) g += m.cR[n * kR + k] * Fs[k];
+        q[n] = g;
+      }
     }
   }
 
@@ -290,8 +387,11 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
   for (int n = 0; n < NA; ++n)
 #pragma unroll
     for (int a = 0; a < NABC; ++a) s[n][a] = 0.0f;
+  float zpe = 0.0f;
+  rn = erow[0];
   for (int e = 0; e < na; ++e) {
-    const float4 r = w.edge[(size_t)e * N + i];
+    const float4 r = rn;
+    if (e + 1 < na) rn = erow[e + 1];
     const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
     const float dinv = 1.0f / d;
     float rc;
@@ -301,6 +401,13 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
     } else {
       tj = safe_type(type[__float_as_int(r.w)], T);
       rc = (float)((m.rcA[ti] + m.rcA[tj]) * 0.5);
+    }
+    if (m.zbl) {
+      // half of each ZBL pair energy belongs to the centre; ZBL pairs are
+      // taken from the angular rows, as the reference's ZBL pass does
+      float zf, zfp;
+      zbl_pair(m.zblp + (size_t)(ti * m.T + tj) * 12, d, dinv, zf, zfp);
+      zpe += 0.5f * zf;
     }
     const float rcinv = 1.0f / rc;
     float fc;
@@ -381,7 +488,8 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
 #pragma unroll
   for (int d = 0; d < DMAX; ++d)
     if (d < D) Fp[d] *= m.qs[d];
-  pe[i] = accumulate ? pe[i] + (double)E : (double)E;
+  const double etot = (double)E + (double)zpe;
+  pe[i] = accumulate ? pe[i] + etot : etot;
 
   // ---- radial adjoint in basis space: bR[i][t][k] = sum_n Fp_n c[ti,t,n,k]
   for (int t = 0; t < T; ++t) {
@@ -389,12 +497,13 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
     float* out = w.bR + ((size_t)i * T + t) * kR;
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
-      if (k >= kR) break;
-      float v = 0.0f;
+      if (k < kR) {
+        float v = 0.0f;
 #pragma unroll
-      for (int n = 0; n < NR; ++n)
-        if (n < nR) v += Fp[n] * c[n * kR + k];
-      out[k] = v;
+        for (int n = 0; n < NR; ++n)
+          if (n < nR) v += Fp[n] * c[n * kR + k];
+        out[k] = v;
+      }
     }
   }
 
@@ -438,8 +547,10 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
   }
 
   // ---- angular partial forces dE_i/dr_ij
+  rn = erow[0];
   for (int e = 0; e < na; ++e) {
-    const float4 r = w.edge[(size_t)e * N + i];
+    const float4 r = rn;
+    if (e + 1 < na) rn = erow[e + 1];
     const int j = __float_as_int(r.w);
     const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
     const float dinv = 1.0f / d;
@@ -528,12 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
     const float gu = Gx * ux + Gy * uy + Gz * uz;
     const float4 fo = make_float4(P * ux + (Gx - gu * ux) * dinv, P * uy + (Gy - gu * uy) * dinv,
                                   P * uz + (Gz - gu * uz) * dinv, 0.0f);
-    w.f12[(size_t)e * N + i] = fo;
-    const int qv = w.eq[(size_t)e * N + i];
-    if (qv >= 0 && qv < w.MV && j >= c1 && j < c2) {
-      const int e2 = w.vsa[(size_t)qv * N + j];
-      if (e2 >= 0 && e2 < w.MA) w.f12t[(size_t)e2 * N + j] = fo;
-    }
+    frow[e] = fo;
   }
 }
 
@@ -541,129 +647,183 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
 // gather: radial pair force, angular partials, ZBL -> FP64 force/virial
 // ---------------------------------------------------------------------------
 
-template <int KR, bool ONE>
-__global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int N, int n1, int n2, int nd1, int nd2,
-                                                   int nblocks, const int* __restrict__ type,
-                                                   double* __restrict__ pe, double* __restrict__ force,
+template <int KR, bool ONE, int G>
+__global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, int nd1, int nd2, int nblocks,
+                                                   const int* __restrict__ type, int N, double* __restrict__ force,
                                                    double* __restrict__ virial, int accumulate, DevWork w)
 {
-  const int i = n1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
-  if (i >= n2) return;
+  // G lanes per atom share its edges; the per-lane FP64 sums are combined
+  // with a butterfly over the group at the end.  Each lane takes its edges
+  // RB at a time: the RB edge records are loaded, then the RB neighbour
+  // adjoint rows, then all of them are used, so the loads of a batch overlap.
+  constexpr int PER = kBlock / G;
+  constexpr int RB = KR <= 9 ? 4 : 2;
+  const int g = threadIdx.x / G, lg = threadIdx.x % G;
+  const int i = n1 + xcd_tile(nblocks) * PER + g;
+  const bool active = i < n2;
   const int T = ONE ? 1 : m.T;
   const int kR = m.kR;
-  const int ti = ONE ? 0 : safe_type(type[i], T);
-  const int nr = w.nr[i], na = w.na[i];
+  int ti = 0, nr = 0, na = 0;
+  if (active) {
+    ti = ONE ? 0 : safe_type(type[i], T);
+    nr = w.nr[i];
+    na = w.na[i];
+  }
+  const float4* __restrict__ erow = w.edge + (size_t)i * w.MR;
+  const float4* __restrict__ frow = w.f12 + (size_t)i * w.MA;
+  const int* __restrict__ qrow = w.eq + (size_t)i * w.MA;
+  const float* __restrict__ bR = w.bR;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   float bi[KR];
-  if (ONE) {
 #pragma unroll
-    for (int k = 0; k < KR; ++k) bi[k] = k < kR ? w.bR[(size_t)i * kR + k] : 0.0f;
+  for (int k = 0; k < KR; ++k) bi[k] = 0.0f;
+  if (ONE && active) {
+#pragma unroll
+    for (int k = 0; k < KR; ++k)
+      if (k < kR) bi[k] = bR[(size_t)i * kR + k];
   }
-  for (int e = 0; e < nr; ++e) {
-    const int slot = e < na ? e : w.MR - 1 - (e - na);
-    const float4 r = w.edge[(size_t)slot * N + i];
-    const int j = __float_as_int(r.w);
-    const int tj = ONE ? 0 : safe_type(type[j], T);
-    const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
-    const float rc = (float)((m.rcR[ti] + m.rcR[tj]) * 0.5);
-    const float rcinv = 1.0f / rc;
-    float fc, fcp;
-    cutoff_fc_fcp(rc, rcinv, d, fc, fcp);
-    float f[KR], fp[KR];
-    cheb_d<KR>(kR, rcinv, d, fc, fcp, f, fp);
-    const bool jdep = j >= nd1 && j < nd2;
-    const float* bj = w.bR + ((size_t)j * T + ti) * kR;
-    const float* bo = w.bR + ((size_t)i * T + tj) * kR;
-    float Gi = 0.0f, Gj = 0.0f;
+  const float rcR0 = (float)((m.rcR[0] + m.rcR[0]) * 0.5);
+  // radial: F_i += (G_i + G_j) r_ij/d with G_c = sum_k b_c[k] f'_k(d)
+  for (int e0 = lg; e0 < nr; e0 += G * RB) {
+    float4 rr[RB];
+    int jj[RB], tt[RB];
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      if (k >= kR) break;
-      Gi += (ONE ? bi[k] : bo[k]) * fp[k];
-      if (jdep) Gj += bj[k] * fp[k];
+    for (int r = 0; r < RB; ++r) {
+      const int e = e0 + r * G;
+      rr[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+      jj[r] = -1;
+      if (e < nr) {
+        rr[r] = erow[e < na ? e : w.MR - 1 - (e - na)];
+        jj[r] = __float_as_int(rr[r].w);
+      }
     }
-    const float dinv = 1.0f / d;
-    const float s = (Gi + Gj) * dinv;
-    fx += (double)(s * r.x);
-    fy += (double)(s * r.y);
-    fz += (double)(s * r.z);
-    // virial_i += r_ij (x) dE_j/dr_ji,  dE_j/dr_ji = -Gj r_ij/d
-    const float gj = -Gj * dinv;
-    const float fjx = gj * r.x, fjy = gj * r.y, fjz = gj * r.z;
-    v[0] += (double)(r.x * fjx);
-    v[1] += (double)(r.y * fjy);
-    v[2] += (double)(r.z * fjz);
-    v[3] += (double)(r.x * fjy);
-    v[4] += (double)(r.x * fjz);
-    v[5] += (double)(r.y * fjz);
-    v[6] += (double)(r.y * fjx);
-    v[7] += (double)(r.z * fjx);
-    v[8] += (double)(r.z * fjy);
-  }
-  float zpe = 0.0f;
-  for (int e = 0; e < na; ++e) {
-    const float4 r = w.edge[(size_t)e * N + i];
-    const int j = __float_as_int(r.w);
-    const float4 a = w.f12[(size_t)e * N + i];
-    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (j >= nd1 && j < nd2) b = w.f12t[(size_t)e * N + i];
-    fx += (double)(a.x - b.x);
-    fy += (double)(a.y - b.y);
-    fz += (double)(a.z - b.z);
-    v[0] += (double)(r.x * b.x);
-    v[1] += (double)(r.y * b.y);
-    v[2] += (double)(r.z * b.z);
-    v[3] += (double)(r.x * b.y);
-    v[4] += (double)(r.x * b.z);
-    v[5] += (double)(r.y * b.z);
-    v[6] += (double)(r.y * b.x);
-    v[7] += (double)(r.z * b.x);
-    v[8] += (double)(r.z * b.y);
-    if (m.zbl) {
-      // screened-Coulomb ZBL with cosine switch (nep_utilities.cuh:347-433)
-      const int tj = ONE ? 0 : safe_type(type[j], T);
-      const float* zp = m.zblp + (size_t)(ti * m.T + tj) * 12;
-      const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
-      const float dinv = 1.0f / d;
-      const float x = d * zp[0];
-      float phi = 0.0f, dphi = 0.0f;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float t = zp[4 + 2 * k] * expf(-zp[5 + 2 * k] * x);
-        phi += t;
-        dphi -= zp[5 + 2 * k] * t;
+    for (int r = 0; r < RB; ++r) tt[r] = (ONE || jj[r] < 0) ? 0 : safe_type(type[jj[r]], T);
+    float bj[RB][KR], bo[ONE ? 1 : RB][KR];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const bool jdep = jj[r] >= nd1 && jj[r] < nd2;
+      const float* pj = bR + ((size_t)(jdep ? jj[r] : 0) * T + ti) * kR;
+#pragma unroll
+      for (int k = 0; k < KR; ++k) bj[r][k] = (jdep && k < kR) ? pj[k] : 0.0f;
+      if (!ONE) {
+        const float* po = bR + ((size_t)i * T + tt[r]) * kR;
+#pragma unroll
+        for (int k = 0; k < KR; ++k) bo[ONE ? 0 : r][k] = (jj[r] >= 0 && k < kR) ? po[k] : 0.0f;
       }
-      const float vv = zp[1] * phi * dinv;
-      const float vp = zp[1] * dphi * zp[0] * dinv - vv * dinv;
-      const float r1 = zp[2], r2 = zp[3];
-      float fc = 0.0f, fcp = 0.0f;
-      if (d < r1) {
-        fc = 1.0f;
-      } else if (d < r2) {
-        const float wv = 3.14159265f / (r2 - r1);
-        float sn, cs;
-        sincosf(wv * (d - r1), &sn, &cs);
-        fc = 0.5f * cs + 0.5f;
-        fcp = -0.5f * wv * sn;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const float4 q = rr[r];
+      const float d = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+      const float rc = ONE ? rcR0 : (float)((m.rcR[ti] + m.rcR[tt[r]]) * 0.5);
+      const float rcinv = 1.0f / rc;
+      float fc, fcp;
+      cutoff_fc_fcp(rc, rcinv, d, fc, fcp);
+      float f[KR], fp[KR];
+      cheb_d<KR>(kR, rcinv, d, fc, fcp, f, fp);
+      float Gi = 0.0f, Gj = 0.0f;
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        Gi += (ONE ? bi[k] : bo[ONE ? 0 : r][k]) * fp[k];
+        Gj += bj[r][k] * fp[k];
       }
-      const float fz_ = vv * fc, fpz = vp * fc + vv * fcp;
-      const float s = fpz * dinv;
-      fx += (double)(s * r.x);
-      fy += (double)(s * r.y);
-      fz += (double)(s * r.z);
-      const float h = -0.5f * s;
-      v[0] += (double)(h * r.x * r.x);
-      v[1] += (double)(h * r.y * r.y);
-      v[2] += (double)(h * r.z * r.z);
-      v[3] += (double)(h * r.x * r.y);
-      v[4] += (double)(h * r.x * r.z);
-      v[5] += (double)(h * r.y * r.z);
-      v[6] += (double)(h * r.y * r.x);
-      v[7] += (double)(h * r.z * r.x);
-      v[8] += (double)(h * r.z * r.y);
-      zpe += 0.5f * fz_;
+      // lanes past the row end carry r = 0 and contribute nothing
+      const float dinv = jj[r] >= 0 ? 1.0f / d : 0.0f;
+      const float sc = (Gi + Gj) * dinv;
+      fx += (double)(sc * q.x);
+      fy += (double)(sc * q.y);
+      fz += (double)(sc * q.z);
+      // virial_i += r_ij (x) dE_j/dr_ji,  dE_j/dr_ji = -Gj r_ij/d
+      const float gj = -Gj * dinv;
+      const float fjx = gj * q.x, fjy = gj * q.y, fjz = gj * q.z;
+      v[0] += (double)(q.x * fjx);
+      v[1] += (double)(q.y * fjy);
+      v[2] += (double)(q.z * fjz);
+      v[3] += (double)(q.x * fjy);
+      v[4] += (double)(q.x * fjz);
+      v[5] += (double)(q.y * fjz);
+      v[6] += (double)(q.y * fjx);
+      v[7] += (double)(q.z * fjx);
+      v[8] += (double)(q.z * fjy);
     }
   }
+  // angular: F_i += dE_i/dr_ij - dE_j/dr_ji; the second term is read from
+  // j's row at the slot where i sits (Verlet reverse position -> angular slot)
+  constexpr int AB = 3;
+  for (int e0 = lg; e0 < na; e0 += G * AB) {
+    float4 rr[AB], aa[AB], bb[AB];
+    int jj[AB], qv[AB], e2[AB];
+#pragma unroll
+    for (int r = 0; r < AB; ++r) {
+      const int e = e0 + r * G;
+      rr[r] = aa[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+      jj[r] = -1;
+      qv[r] = -1;
+      if (e < na) {
+        rr[r] = erow[e];
+        aa[r] = frow[e];
+        qv[r] = qrow[e];
+        jj[r] = __float_as_int(rr[r].w);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < AB; ++r) {
+      e2[r] = -1;
+      if (jj[r] >= nd1 && jj[r] < nd2 && qv[r] >= 0 && qv[r] < w.MV) e2[r] = w.vsa[(size_t)jj[r] * w.MV + qv[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < AB; ++r)
+      bb[r] = (e2[r] >= 0 && e2[r] < w.MA) ? w.f12[(size_t)jj[r] * w.MA + e2[r]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < AB; ++r) {
+      const float4 q = rr[r];
+      const bool jdep = jj[r] >= nd1 && jj[r] < nd2;
+      float gx = aa[r].x - bb[r].x, gy = aa[r].y - bb[r].y, gz = aa[r].z - bb[r].z;
+      float bx = bb[r].x, by = bb[r].y, bz = bb[r].z;
+      if (m.zbl && jj[r] >= 0) {
+        // each directed pair carries half of the ZBL pair gradient
+        // (nep_utilities.cuh:347-433 switch, nep.cu ZBL pass)
+        const int tj = ONE ? 0 : safe_type(type[jj[r]], T);
+        const float d = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+        const float dinv = 1.0f / d;
+        float zf, zfp;
+        zbl_pair(m.zblp + (size_t)(ti * m.T + tj) * 12, d, dinv, zf, zfp);
+        const float h = 0.5f * zfp * dinv;  // dE_i/dr_ij = h r_ij
+        const float c = jdep ? 2.0f * h : h;
+        gx += c * q.x;
+        gy += c * q.y;
+        gz += c * q.z;
+        if (jdep) {  // dE_j/dr_ji = -h r_ij
+          bx -= h * q.x;
+          by -= h * q.y;
+          bz -= h * q.z;
+        }
+      }
+      fx += (double)gx;
+      fy += (double)gy;
+      fz += (double)gz;
+      v[0] += (double)(q.x * bx);
+      v[1] += (double)(q.y * by);
+      v[2] += (double)(q.z * bz);
+      v[3] += (double)(q.x * by);
+      v[4] += (double)(q.x * bz);
+      v[5] += (double)(q.y * bz);
+      v[6] += (double)(q.y * bx);
+      v[7] += (double)(q.z * bx);
+      v[8] += (double)(q.z * by);
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    fx += __shfl_xor(fx, off, G);
+    fy += __shfl_xor(fy, off, G);
+    fz += __shfl_xor(fz, off, G);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) v[c] += __shfl_xor(v[c], off, G);
+  }
+  if (!active || lg != 0) return;
   if (accumulate) {
     force[i] += fx;
     force[N + i] += fy;
@@ -677,7 +837,6 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int N, int n1, in
 #pragma unroll
     for (int c = 0; c < 9; ++c) virial[(size_t)c * N + i] = v[c];
   }
-  if (m.zbl) pe[i] += (double)zpe;
 }
 
 // ---------------------------------------------------------------------------
@@ -876,12 +1035,13 @@ __global__ void k_check_scaled(int n, double factor, const double* __restrict__ 
 
 static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 
-hipError_t launch_filter(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type,
-                         const double* pos, DevWork& w, hipStream_t s)
+hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s)
 {
   if (c2 > c1) {
-    const int nb = nblk(c2 - c1);
-    k_filter<<<xcd_grid(nb), kBlock, 0, s>>>(b, m, N, c1, c2, nb, type, pos, w);
+    constexpr int per = kBlock / kFilterLanes;
+    const int nb = (c2 - c1 + per - 1) / per;
+    if (m.T == 1) k_filter<true><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+    else k_filter<false><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
   }
   return hipGetLastError();
 }
@@ -925,20 +1085,21 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
 }
 
 hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
-                         double* pe, double* force, double* virial, bool acc, DevWork& w, hipStream_t s)
+                         double* force, double* virial, bool acc, DevWork& w, hipStream_t s)
 {
   if (n2 <= n1) return hipSuccess;
-  const int nb = nblk(n2 - n1);
+  constexpr int G = kGatherLanes, per = kBlock / G;
+  const int nb = (n2 - n1 + per - 1) / per;
   const int g = xcd_grid(nb);
   const int a = acc ? 1 : 0;
   if (m.T == 1 && m.kR == 9)
-    k_gather<9, true><<<g, kBlock, 0, s>>>(m, N, n1, n2, nd1, nd2, nb, type, pe, force, virial, a, w);
+    k_gather<9, true, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
   else if (m.T == 1)
-    k_gather<kMaxK, true><<<g, kBlock, 0, s>>>(m, N, n1, n2, nd1, nd2, nb, type, pe, force, virial, a, w);
+    k_gather<kMaxK, true, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
   else if (m.kR <= 11)
-    k_gather<11, false><<<g, kBlock, 0, s>>>(m, N, n1, n2, nd1, nd2, nb, type, pe, force, virial, a, w);
+    k_gather<11, false, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
   else
-    k_gather<kMaxK, false><<<g, kBlock, 0, s>>>(m, N, n1, n2, nd1, nd2, nb, type, pe, force, virial, a, w);
+    k_gather<kMaxK, false, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
   return hipGetLastError();
 }
 

@@ -43,28 +43,30 @@ struct DevModel {
 };
 
 // Per-atom / per-edge workspaces (device pointers), sized for `cap` slots.
+// Row arrays are row-major: entry p of centre i lives at [i*M + p].
 struct DevWork {
   int cap = 0;
   int MV = 0, MR = 0, MA = 0;  // row capacities: Verlet, radial, angular
   int ncell_cap = 0;
+  double4* xq = nullptr;       // [cap] per-step (x, y, z, type bits)
   int* cell_of = nullptr;      // [cap]
-  int* cell_count = nullptr;   // [ncell]
+  int* cell_rank = nullptr;    // [cap] rank of the atom inside its cell
+  int* cell_count = nullptr;   // [ncell] zero between rebuilds
   int* cell_start = nullptr;   // [ncell+1]
-  int* cell_fill = nullptr;    // [ncell]
   int* cell_atoms = nullptr;   // [cap]
   int* vn = nullptr;           // [cap] Verlet row length
-  int* vl = nullptr;           // [MV*cap] Verlet rows (ELL, slot-major)
-  int* vrev = nullptr;         // [MV*cap] position of the centre in the neighbour's row
+  int* vl = nullptr;           // [cap*MV] Verlet rows
+  int* vrev = nullptr;         // [cap*MV] position of the centre in the neighbour's row
   double* x0 = nullptr;        // [3*cap] positions at the last rebuild
   int* nr = nullptr;           // [cap] radial row length
-  int* na = nullptr;           // [cap] angular row length (prefix of the radial row)
-  float4* edge = nullptr;      // [MR*cap] (rx, ry, rz, bits(j))
-  int* eq = nullptr;           // [MA*cap] reverse Verlet position of an angular edge
-  int* vsa = nullptr;          // [MV*cap] Verlet position -> angular slot
+  int* na = nullptr;           // [cap] angular row length
+  float4* edge = nullptr;      // [cap*MR] (rx, ry, rz, bits(j)): angular edges at
+                               // 0..na-1, radial-only edges at MR-1 downward
+  int* eq = nullptr;           // [cap*MA] reverse Verlet position of an angular edge
+  int* vsa = nullptr;          // [cap*MV] Verlet position -> angular slot
   float* bR = nullptr;         // [cap*T*kR] radial adjoint in basis space
-  float4* f12 = nullptr;       // [MA*cap] dE_i/dr_ij of angular edges
-  float4* f12t = nullptr;      // [MA*cap] dE_j/dr_ji stored at i's slot of j
-  int* flags = nullptr;        // [8] device status words
+  float4* f12 = nullptr;       // [cap*MA] dE_i/dr_ij of angular edges
+  int* flags = nullptr;        // [kNumFlags] device status words
   double* partial = nullptr;   // thermo block partials
   double* thermo = nullptr;    // [8]
 };
@@ -85,23 +87,23 @@ enum : int {
 
 void upload_angular_constants(const AngularConstants& a);
 
-// Neighbour build (neighbor.hip).  launch_rebuild enqueues the whole cell
-// list + Verlet build; every kernel returns at once unless flags[kFlagRebuild]
-// is set, which launch_check_moved (displacement > skin/2) or
-// launch_request_rebuild (first call, box change) do on the device.
+// Neighbour build (neighbor.hip).  launch_pack packs (x, y, z, type) for
+// the step, validates types and, unless `forced`, sets flags[kFlagRebuild]
+// when an atom moved more than skin/2; `forced` sets it unconditionally.
+// launch_rebuild enqueues the whole cell list + Verlet build; every kernel of
+// it returns at once unless flags[kFlagRebuild] is set.
 hipError_t launch_request_rebuild(DevWork& w, hipStream_t s);
-hipError_t launch_check_moved(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s);
-hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, const double* pos, double rv2,
-                          const uint64_t* gid, DevWork& w, hipStream_t s);
+hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const int* type, bool forced, DevWork& w,
+                       hipStream_t s);
+hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, const uint64_t* gid, DevWork& w,
+                          hipStream_t s);
 
 // Force evaluation (per step)
-hipError_t launch_filter(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type,
-                         const double* pos, DevWork& w, hipStream_t s);
+hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s);
 hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
                              bool accumulate, DevWork& w, hipStream_t s);
 hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
-                         double* pe, double* force, double* virial, bool accumulate, DevWork& w,
-                         hipStream_t s);
+                         double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s);
 
 // Integrator / thermo
 hipError_t launch_velocity_verlet(int first_half, int N, double dt, const double* mass, double* pos,

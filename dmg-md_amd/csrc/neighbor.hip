@@ -3,160 +3,132 @@
 //
 // Every kernel of the rebuild is launched every step and returns at once
 // unless the device flag flags[kFlagRebuild] is set (by the displacement
-// check or by the host for a first call / box change).  The host therefore
-// never has to read the decision back: a whole MD chunk is enqueued without a
-// synchronisation.  Behaviour follows the reference's Neighbor class
-// (neighbor.cu:90-134 cell list, :141-218 rows, :446-582 skin policy,
-// neighbor.cuh:133-196 row order): rows hold every slot j != i with
-// FP64 d^2 < (rc + skin)^2, sorted ascending by slot (or by global id), plus
-// the position of the centre inside each neighbour's row.
+// check in k_pack or by the host for a first call / box change).  The host
+// therefore never has to read the decision back: a whole MD chunk is enqueued
+// without a synchronisation.  Behaviour follows the reference's Neighbor
+// class (neighbor.cu:90-134 cell list, :141-218 rows, :446-582 skin policy,
+// neighbor.cuh:133-196 row order): rows hold every slot j != i with FP64
+// d^2 < (rc + skin)^2, sorted ascending by slot (or by global id), plus the
+// position of the centre inside each neighbour's row.
+//
+// Layout: rows are row-major (vl[i*MV + p]) so that a group of lanes that
+// works on one centre reads and writes contiguous memory.  Build, sort and
+// reverse search run with kRowLanes lanes per centre (4 centres per wave).
 #include "device_common.hpp"
 
 namespace dmgx {
 
 namespace {
 
+constexpr int kRowLanes = 16;  // lanes cooperating on one Verlet row
+
 __global__ void k_set_flag(int* __restrict__ flag, int value) { *flag = value; }
 
-__global__ void k_zero_if(const int* __restrict__ go, int* __restrict__ buf, int n)
+// Per-step packing of positions and types into one 32-byte record per atom
+// (x, y, z, type bits) so that every neighbour gather is a single 32-byte
+// access; also validates the types and, unless a rebuild is forced, tests
+// whether any atom moved more than skin/2 since the last rebuild
+// (neighbor.cu:446-490).
+__global__ __launch_bounds__(kBlock) void k_pack(DevBox b, int N, int T, const double* __restrict__ pos,
+                                                 const int* __restrict__ type, const double* __restrict__ x0,
+                                                 double lim2, int forced, double4* __restrict__ xq,
+                                                 int* __restrict__ flags)
 {
-  if (!*go) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) buf[i] = 0;
+  if (forced && i == 0) flags[kFlagRebuild] = 1;
+  bool moved = false, bad = false;
+  if (i < N) {
+    const double x = pos[i], y = pos[N + i], z = pos[2 * N + i];
+    const int t = type[i];
+    bad = t < 0 || t >= T;
+    xq[i] = make_double4(x, y, z, __longlong_as_double((long long)t));
+    if (!forced) {
+      double dx = x - x0[i], dy = y - x0[N + i], dz = z - x0[2 * N + i];
+      mic(b, dx, dy, dz);
+      moved = !(dist2(dx, dy, dz) <= lim2);  // NaN displacements request a rebuild too
+    }
+  }
+  const bool lead = (threadIdx.x & 63) == 0;
+  if (__any(moved) && lead) atomicOr(&flags[kFlagRebuild], 1);
+  if (__any(bad) && lead) atomicOr(&flags[kFlagBadType], 1);
 }
 
-__global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const double* __restrict__ pos,
-                             int* __restrict__ cell_of, int* __restrict__ count, int* __restrict__ flags)
+// cell of every atom, its rank inside the cell, and the positions the next
+// displacement checks compare against
+__global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const double4* __restrict__ xq,
+                             int* __restrict__ cell_of, int* __restrict__ rank, int* __restrict__ count,
+                             double* __restrict__ x0, int* __restrict__ flags)
 {
   if (!*go) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const double4 p = xq[i];
   int c[3];
-  if (!cell_coords(b, pos[i], pos[N + i], pos[2 * N + i], c)) atomicOr(&flags[kFlagOutOfBox], 1);
+  if (!cell_coords(b, p.x, p.y, p.z, c)) atomicOr(&flags[kFlagOutOfBox], 1);
   const int id = c[0] + b.nc[0] * (c[1] + b.nc[1] * c[2]);
   cell_of[i] = id;
-  atomicAdd(&count[id], 1);
+  rank[i] = atomicAdd(&count[id], 1);
+  x0[i] = p.x;
+  x0[N + i] = p.y;
+  x0[2 * N + i] = p.z;
 }
 
-// three-phase exclusive scan, 1024 counts per workgroup
-__global__ void k_scan_local(const int* __restrict__ go, int n, const int* __restrict__ in, int* __restrict__ out,
-                             int* __restrict__ bsum)
+// Exclusive scan of the cell counts by one workgroup in tiles of 4096, which
+// also clears the counts for the next rebuild; start[ncell] = total.
+__global__ __launch_bounds__(1024) void k_scan_cells(const int* __restrict__ go, int ncell,
+                                                     int* __restrict__ count, int* __restrict__ start)
 {
   if (!*go) return;
-  __shared__ int sh[256];
-  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
-  int v[4], sum = 0;
+  __shared__ int wsum[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < ncell; base += 4096) {
+    const int k0 = base + threadIdx.x * 4;
+    int v[4], s = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    v[k] = base + k < n ? in[base + k] : 0;
-    sum += v[k];
-  }
-  sh[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = sh[threadIdx.x] - sum;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) out[base + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == 255) bsum[blockIdx.x] = sh[255];
-}
-
-__global__ void k_scan_blocks(const int* __restrict__ go, int nb, int* __restrict__ bsum, int* __restrict__ total)
-{
-  if (!*go) return;
-  __shared__ int sh[1024];
-  const int chunk = (nb + 1023) / 1024;
-  const int b0 = threadIdx.x * chunk;
-  int sum = 0;
-  for (int k = 0; k < chunk; ++k)
-    if (b0 + k < nb) sum += bsum[b0 + k];
-  sh[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = sh[threadIdx.x] - sum;
-  for (int k = 0; k < chunk; ++k)
-    if (b0 + k < nb) {
-      const int v = bsum[b0 + k];
-      bsum[b0 + k] = run;
-      run += v;
+    for (int k = 0; k < 4; ++k) {
+      v[k] = k0 + k < ncell ? count[k0 + k] : 0;
+      s += v[k];
     }
-  if (threadIdx.x == 1023) *total = sh[1023];
-}
-
-__global__ void k_scan_add(const int* __restrict__ go, int n, int* __restrict__ out, const int* __restrict__ bsum)
-{
-  if (!*go) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += bsum[i >> 10];
+    // inclusive scan of s over the wavefront
+    int x = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int before = carry, tile_total = 0;
+    for (int q = 0; q < 16; ++q) {
+      if (q < wave) before += wsum[q];
+      tile_total += wsum[q];
+    }
+    int run = before + x - s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k0 + k < ncell) {
+        start[k0 + k] = run;
+        count[k0 + k] = 0;
+      }
+      run += v[k];
+    }
+    carry += tile_total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) start[ncell] = carry;
 }
 
 __global__ void k_cell_fill(const int* __restrict__ go, int N, const int* __restrict__ cell_of,
-                            const int* __restrict__ start, int* __restrict__ fill, int* __restrict__ atoms)
+                            const int* __restrict__ rank, const int* __restrict__ start, int* __restrict__ atoms)
 {
   if (!*go) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const int c = cell_of[i];
-  atoms[start[c] + atomicAdd(&fill[c], 1)] = i;
+  atoms[start[cell_of[i]] + rank[i]] = i;
 }
 
-// One thread per centre: scan the 5x5x5 cells around it (cell edge >= (rc+skin)/2)
-// and keep every slot with FP64 d2 < (rc+skin)^2.  Rows past the capacity MV
-// are counted but not written; the count is reported through the flags.
-__global__ __launch_bounds__(kBlock) void k_verlet_build(const int* __restrict__ go, DevBox b, int N, int c1,
-                                                         int c2, const double* __restrict__ pos, double rv2,
-                                                         DevWork w)
-{
-  if (!*go) return;
-  const int i = c1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c2) return;
-  const double xi = pos[i], yi = pos[N + i], zi = pos[2 * N + i];
-  int c[3];
-  cell_coords(b, xi, yi, zi, c);
-  int cnt = 0;
-  for (int dz = -2; dz <= 2; ++dz) {
-    int cz = c[2] + dz;
-    if (b.pbc[2]) cz = (cz + b.nc[2]) % b.nc[2];
-    else if (cz < 0 || cz >= b.nc[2]) continue;
-    for (int dy = -2; dy <= 2; ++dy) {
-      int cy = c[1] + dy;
-      if (b.pbc[1]) cy = (cy + b.nc[1]) % b.nc[1];
-      else if (cy < 0 || cy >= b.nc[1]) continue;
-      for (int dx = -2; dx <= 2; ++dx) {
-        int cx = c[0] + dx;
-        if (b.pbc[0]) cx = (cx + b.nc[0]) % b.nc[0];
-        else if (cx < 0 || cx >= b.nc[0]) continue;
-        const int cell = cx + b.nc[0] * (cy + b.nc[1] * cz);
-        const int e = w.cell_start[cell + 1];
-        for (int k = w.cell_start[cell]; k < e; ++k) {
-          const int j = w.cell_atoms[k];
-          if (j == i) continue;
-          double x = pos[j] - xi, y = pos[N + j] - yi, z = pos[2 * N + j] - zi;
-          mic(b, x, y, z);
-          if (dist2(x, y, z) < rv2) {
-            if (cnt < w.MV) w.vl[(size_t)cnt * N + i] = j;
-            ++cnt;
-          }
-        }
-      }
-    }
-  }
-  w.vn[i] = cnt;
-  if (cnt > w.MV) atomicMax(&w.flags[kFlagVerletOverflow], cnt);
-}
-
+// Row key order: slot, or (global id, slot).
 __device__ __forceinline__ bool key_less(const uint64_t* gid, int a, int b)
 {
   if (gid) {
@@ -166,44 +138,132 @@ __device__ __forceinline__ bool key_less(const uint64_t* gid, int a, int b)
   return a < b;
 }
 
-// One wavefront per row: rank sort in LDS (keys unique: slot, or (gid, slot)).
-__global__ __launch_bounds__(256) void k_verlet_sort(const int* __restrict__ go, int N, int c1, int c2,
-                                                     const uint64_t* __restrict__ gid, DevWork w)
+// One kRowLanes group per centre: the 5x5 rows of cells around it (cell edge
+// >= (rc+skin)/2, so +-2 cells cover rc+skin) are scanned as contiguous atom
+// ranges (one per row, two where a periodic row wraps), accepted slots are
+// compacted into LDS with a ballot, rank-sorted there on (key, slot) and
+// written out as one contiguous row.  Rows past the capacity MV are counted
+// but not stored; the count is reported through the flags.
+__global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go, DevBox b, int c1, int c2,
+                                                      int groups, const double4* __restrict__ xq, double rv2,
+                                                      const uint64_t* __restrict__ gid, DevWork w)
 {
   if (!*go) return;
-  extern __shared__ int sh[];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = c1 + blockIdx.x * 4 + wave;
-  int* row = sh + wave * w.MV;
-  const int n = i < c2 ? w.vn[i] : 0;
-  const int nn = n < w.MV ? n : w.MV;
-  for (int k = lane; k < nn; k += 64) row[k] = w.vl[(size_t)k * N + i];
+  extern __shared__ uint64_t shk[];
+  const int g = threadIdx.x / kRowLanes, lg = threadIdx.x % kRowLanes;
+  const int gw = g % (64 / kRowLanes);  // group inside the wavefront
+  uint64_t* key = shk + (size_t)g * 2 * w.MV;
+  int* row = reinterpret_cast<int*>(key + w.MV);
+  int* sorted = row + w.MV;
+  const int i = c1 + blockIdx.x * groups + g;
+  const bool active = i < c2;
+  double4 pi = make_double4(0, 0, 0, 0);
+  int c[3] = {0, 0, 0};
+  if (active) {
+    pi = xq[i];
+    cell_coords(b, pi.x, pi.y, pi.z, c);
+  }
+  int cnt = 0;
+  for (int dz = -2; dz <= 2; ++dz) {
+    for (int dy = -2; dy <= 2; ++dy) {
+      int cz = c[2] + dz, cy = c[1] + dy;
+      bool ok = active;
+      if (b.pbc[2]) cz = (cz + b.nc[2]) % b.nc[2];
+      else ok = ok && cz >= 0 && cz < b.nc[2];
+      if (b.pbc[1]) cy = (cy + b.nc[1]) % b.nc[1];
+      else ok = ok && cy >= 0 && cy < b.nc[1];
+      // cells [lo, hi] of this row, split in two ranges where the row wraps
+      int lo0 = c[0] - 2, hi0 = c[0] + 2, lo1 = 0, hi1 = -1;
+      if (b.pbc[0]) {
+        if (lo0 < 0) {
+          lo1 = lo0 + b.nc[0];
+          hi1 = b.nc[0] - 1;
+          lo0 = 0;
+        } else if (hi0 >= b.nc[0]) {
+          lo1 = 0;
+          hi1 = hi0 - b.nc[0];
+          hi0 = b.nc[0] - 1;
+        }
+      } else {
+        lo0 = max(lo0, 0);
+        hi0 = min(hi0, b.nc[0] - 1);
+      }
+      const int base = b.nc[0] * (cy + b.nc[1] * cz);
+      for (int part = 0; part < 2; ++part) {
+        int s = 0, e = 0;
+        const int lo = part ? lo1 : lo0, hi = part ? hi1 : hi0;
+        if (ok && hi >= lo) {
+          s = w.cell_start[base + lo];
+          e = w.cell_start[base + hi + 1];
+        }
+        for (int k0 = s; __any(k0 < e); k0 += kRowLanes) {
+          const int k = k0 + lg;
+          bool take = false;
+          int j = 0;
+          if (k < e) {
+            j = w.cell_atoms[k];
+            if (j != i) {
+              const double4 pj = xq[j];
+              double x = pj.x - pi.x, y = pj.y - pi.y, z = pj.z - pi.z;
+              mic(b, x, y, z);
+              take = dist2(x, y, z) < rv2;
+            }
+          }
+          const unsigned gm = (unsigned)(__ballot(take) >> (gw * kRowLanes)) & ((1u << kRowLanes) - 1u);
+          if (take) {
+            const int at = cnt + __popc(gm & ((1u << lg) - 1u));
+            if (at < w.MV) {
+              row[at] = j;
+              key[at] = gid ? gid[j] : 0ull;
+            }
+          }
+          cnt += __popc(gm);
+        }
+      }
+    }
+  }
   __syncthreads();
-  for (int k = lane; k < nn; k += 64) {
+  const int n = min(cnt, w.MV);
+  for (int k = lg; k < n; k += kRowLanes) {
+    const uint64_t kv = key[k];
     const int v = row[k];
-    int rank = 0;
-    for (int t = 0; t < nn; ++t) rank += key_less(gid, row[t], v) ? 1 : 0;
-    w.vl[(size_t)rank * N + i] = v;
+    int r = 0;
+    for (int t = 0; t < n; ++t) {
+      const uint64_t kt = key[t];
+      r += (kt < kv || (kt == kv && row[t] < v)) ? 1 : 0;
+    }
+    sorted[r] = v;
+  }
+  __syncthreads();
+  if (active) {
+    for (int k = lg; k < n; k += kRowLanes) w.vl[(size_t)i * w.MV + k] = sorted[k];
+    if (lg == 0) {
+      w.vn[i] = cnt;
+      if (cnt > w.MV) atomicMax(&w.flags[kFlagVerletOverflow], cnt);
+    }
   }
 }
 
 // Position of the centre inside each neighbour's row (binary search on the
 // row key); -1 when the neighbour has no row (outside the centre range).
-__global__ __launch_bounds__(kBlock) void k_verlet_reverse(const int* __restrict__ go, int N, int c1, int c2,
-                                                           const uint64_t* __restrict__ gid, DevWork w)
+__global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ go, int c1, int c2,
+                                                        const uint64_t* __restrict__ gid, DevWork w)
 {
   if (!*go) return;
-  const int i = c1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = c1 + (blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes;
+  const int lg = threadIdx.x % kRowLanes;
   if (i >= c2) return;
   const int n = min(w.vn[i], w.MV);
-  for (int p = 0; p < n; ++p) {
-    const int j = w.vl[(size_t)p * N + i];
+  const int* ri = w.vl + (size_t)i * w.MV;
+  for (int p = lg; p < n; p += kRowLanes) {
+    const int j = ri[p];
     int q = -1;
     if (j >= c1 && j < c2) {
+      const int* rj = w.vl + (size_t)j * w.MV;
       int lo = 0, hi = min(w.vn[j], w.MV) - 1;
       while (lo <= hi) {
         const int mid = (lo + hi) >> 1;
-        const int v = w.vl[(size_t)mid * N + j];
+        const int v = rj[mid];
         if (v == i) {
           q = mid;
           break;
@@ -212,16 +272,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_reverse(const int* __restrict
         else hi = mid - 1;
       }
     }
-    w.vrev[(size_t)p * N + i] = q;
+    w.vrev[(size_t)i * w.MV + p] = q;
   }
-}
-
-__global__ void k_save_positions(const int* __restrict__ go, int N, const double* __restrict__ pos,
-                                 double* __restrict__ x0)
-{
-  if (!*go) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 3 * N) x0[i] = pos[i];
 }
 
 // closes a rebuild: clears the request and counts it
@@ -233,22 +285,17 @@ __global__ void k_rebuild_done(int* __restrict__ flags)
   }
 }
 
-// any atom displaced by more than skin/2 since the last rebuild requests one
-// (neighbor.cu:446-490); the reduction is an OR over the wavefront
-__global__ void k_check_moved(DevBox b, int N, const double* __restrict__ pos, const double* __restrict__ x0,
-                              double lim2, int* __restrict__ flags)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool moved = false;
-  if (i < N) {
-    double x = pos[i] - x0[i], y = pos[N + i] - x0[N + i], z = pos[2 * N + i] - x0[2 * N + i];
-    mic(b, x, y, z);
-    moved = !(dist2(x, y, z) <= lim2);  // NaN displacements request a rebuild too
-  }
-  if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(&flags[kFlagRebuild], 1);
-}
-
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
+
+// centres per build workgroup: 16 groups of kRowLanes lanes unless the LDS
+// rows (16 bytes per entry) would exceed 64 KiB
+int build_groups(int mv)
+{
+  const size_t per = 16 * (size_t)mv;
+  int groups = 256 / kRowLanes;
+  while (groups > 1 && per * groups > 64 * 1024) groups >>= 1;
+  return groups;
+}
 
 }  // namespace
 
@@ -258,33 +305,34 @@ hipError_t launch_request_rebuild(DevWork& w, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_check_moved(const DevBox& b, int N, const double* pos, DevWork& w, hipStream_t s)
+hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const int* type, bool forced, DevWork& w,
+                       hipStream_t s)
 {
   const double half = 0.5 * kSkin;
-  if (N > 0) k_check_moved<<<nblk(N), kBlock, 0, s>>>(b, N, pos, w.x0, half * half, w.flags);
+  if (N > 0)
+    k_pack<<<nblk(N), kBlock, 0, s>>>(b, N, T, pos, type, w.x0, half * half, forced ? 1 : 0, w.xq, w.flags);
+  else if (forced)
+    k_set_flag<<<1, 1, 0, s>>>(w.flags + kFlagRebuild, 1);
   return hipGetLastError();
 }
 
-hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, const double* pos, double rv2,
-                          const uint64_t* gid, DevWork& w, hipStream_t s)
+hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, const uint64_t* gid, DevWork& w,
+                          hipStream_t s)
 {
   const int* go = w.flags + kFlagRebuild;
   const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
-  const int nb = (ncell + 1023) / 1024;
-  k_zero_if<<<(ncell + 255) / 256, 256, 0, s>>>(go, w.cell_count, ncell);
-  if (N > 0) k_cell_count<<<nblk(N), kBlock, 0, s>>>(go, b, N, pos, w.cell_of, w.cell_count, w.flags);
-  // block sums use cell_fill as scratch (nb <= ncell), cleared again before the fill
-  k_scan_local<<<nb, 256, 0, s>>>(go, ncell, w.cell_count, w.cell_start, w.cell_fill);
-  k_scan_blocks<<<1, 1024, 0, s>>>(go, nb, w.cell_fill, w.cell_start + ncell);
-  k_scan_add<<<(ncell + 255) / 256, 256, 0, s>>>(go, ncell, w.cell_start, w.cell_fill);
-  k_zero_if<<<(ncell + 255) / 256, 256, 0, s>>>(go, w.cell_fill, ncell);
-  if (N > 0) k_cell_fill<<<nblk(N), kBlock, 0, s>>>(go, N, w.cell_of, w.cell_start, w.cell_fill, w.cell_atoms);
+  if (N > 0)
+    k_cell_count<<<nblk(N), kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, w.flags);
+  k_scan_cells<<<1, 1024, 0, s>>>(go, ncell, w.cell_count, w.cell_start);
+  if (N > 0) k_cell_fill<<<nblk(N), kBlock, 0, s>>>(go, N, w.cell_of, w.cell_rank, w.cell_start, w.cell_atoms);
   if (c2 > c1) {
-    k_verlet_build<<<nblk(c2 - c1), kBlock, 0, s>>>(go, b, N, c1, c2, pos, rv2, w);
-    k_verlet_sort<<<(c2 - c1 + 3) / 4, 256, sizeof(int) * 4 * (size_t)w.MV, s>>>(go, N, c1, c2, gid, w);
-    k_verlet_reverse<<<nblk(c2 - c1), kBlock, 0, s>>>(go, N, c1, c2, gid, w);
+    const int groups = build_groups(w.MV);
+    const int nb = (c2 - c1 + groups - 1) / groups;
+    const size_t lds = 16 * (size_t)w.MV * groups;
+    k_verlet_build<<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
+    constexpr int rows_per_block = 256 / kRowLanes;
+    k_verlet_reverse<<<(c2 - c1 + rows_per_block - 1) / rows_per_block, 256, 0, s>>>(go, c1, c2, gid, w);
   }
-  if (N > 0) k_save_positions<<<nblk(3 * N), kBlock, 0, s>>>(go, N, pos, w.x0);
   k_rebuild_done<<<1, 1, 0, s>>>(w.flags);
   return hipGetLastError();
 }
