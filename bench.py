@@ -110,6 +110,9 @@ def main():
     ap.add_argument("--cells", type=int, default=40, help="fcc cells per edge (40 -> 256,000 atoms)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--ann", choices=["fp32", "valu", "bf16"], default="fp32",
+                    help="ANN arithmetic: fp32 on the matrix cores (default), fp32 on the vector ALUs, "
+                         "or bfloat16 operands on the matrix cores")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -139,7 +142,8 @@ def main():
     N = len(s["types"])
     nep = dmgx.NEP(MODEL)
     md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
-    ens = dict(ensemble="nvt_bdp", t_init=300.0, t_final=300.0, t_coupling=100.0, seed=12345678)
+    ens = dict(ensemble="nvt_bdp", t_init=300.0, t_final=300.0, t_coupling=100.0, seed=12345678,
+               ann_bf16=args.ann == "bf16", ann_valu=args.ann == "valu")
     print(f"bench: Cu fcc {args.cells}^3 = {N} atoms, warmup {args.warmup} steps", file=sys.stderr, flush=True)
     md.run(args.warmup, 2.0, **ens)
     torch.cuda.synchronize()
@@ -175,12 +179,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 force/descriptor/ANN, f64 positions/integrator/thermo",
+        "dtype": ("f32 force/descriptor/ANN, f64 positions/integrator/thermo" if args.ann != "bf16" else
+                  "f32 force/descriptor, bf16 ANN operands, f64 positions/integrator/thermo"),
         "data": "synthetic (seeded Cu fcc lattice + 0.03 A jitter, seeded random NEP4 weights)",
         "config": {
             "workload": f"Cu fcc {args.cells}^3 cells ({N} atoms, a=3.615 A), NEP4 n_max 6/4 basis 8/8 "
                         "l_max 4/2/0 rc 5.0/4.0 A, 50 tanh neurons; Bussi NVT 300 K, tau 100 steps, dt 2 fs",
-            "atoms": N, "ensemble": "nvt_bdp", "dt_fs": 2.0, "ann": "fp32",
+            "atoms": N, "ensemble": "nvt_bdp", "dt_fs": 2.0,
+            "ann": {"fp32": "fp32 (matrix cores)", "valu": "fp32 (vector ALUs)",
+                    "bf16": "bf16 operands, fp32 accumulation (matrix cores)"}[args.ann],
             "mean_neighbors": {k: round(v, 2) for k, v in means.items() if k != "centres"},
             "verlet_rebuilds_in_run": stats["rebuilds"],
             "kernel_avg_ms": kernel_ms,

@@ -53,6 +53,7 @@ ACCUMULATE = 1
 FORCE_REBUILD = 2
 ANN_BF16 = 4
 SORT_BY_GID = 8
+ANN_VALU = 16
 
 VP = ctypes.c_void_p
 I = ctypes.c_int

@@ -193,6 +193,7 @@ NepEngine::NepEngine(const std::string& path)
   dm_.nR = m.nR;
   dm_.nA = m.nA;
   dm_.kR = m.kR;
+  dm_.kRp = (m.kR + 3) & ~3;
   dm_.kA = m.kA;
   dm_.L = m.L_max;
   dm_.has222 = m.has222;
@@ -224,6 +225,12 @@ NepEngine::~NepEngine()
 {
   free_all();
   dfree(&params_);
+  dfree(&sb_pos_);
+  dfree(&sb_type_);
+  dfree(&sb_gid_);
+  dfree(&sb_pe_);
+  dfree(&sb_f_);
+  dfree(&sb_vir_);
   dfree(&w_.flags);
   if (hflags_) (void)hipHostFree(hflags_);
 }
@@ -256,6 +263,8 @@ void NepEngine::allocate(int capacity, int mv)
 {
   const size_t cap = (size_t)std::max(capacity, 1);
   const size_t rows = cap * (size_t)mv;
+  if (rows >= (size_t)INT32_MAX)
+    throw std::invalid_argument("atoms x Verlet row capacity must stay below 2^31");
   dalloc(&w_.xq, cap, "packed positions");
   dalloc(&w_.cell_of, cap, "cell_of");
   dalloc(&w_.cell_rank, cap, "cell rank");
@@ -269,8 +278,7 @@ void NepEngine::allocate(int capacity, int mv)
   dalloc(&w_.edge, rows, "edges");
   dalloc(&w_.eq, rows, "edge reverse");
   dalloc(&w_.vsa, rows, "verlet->angular");
-  check(hipMemset(w_.vsa, 0xff, rows * sizeof(int)), "clear verlet->angular");
-  dalloc(&w_.bR, cap * model_.num_types * model_.kR, "radial adjoint");
+  dalloc(&w_.bR, cap * model_.num_types * (size_t)((model_.kR + 3) & ~3), "radial adjoint");
   dalloc(&w_.f12, rows, "angular partials");
   w_.cap = (int)cap;
   w_.MV = mv;
@@ -319,38 +327,31 @@ DevBox NepEngine::make_devbox(const Box& box) const
   return b;
 }
 
-void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
-                        const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
-                        unsigned flags, hipStream_t s)
+NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                                const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
+                                unsigned flags)
 {
+  Plan p;
   if (N < 0 || n1 < 0 || n2 < n1 || n2 > N || nd1 < 0 || nd2 < nd1 || nd2 > N)
     throw std::invalid_argument("centre range is outside [0, N)");
   if (n1 < nd1 || n2 > nd2) throw std::invalid_argument("force centres must be dependency centres");
   if (N > 0 && (!type || !pos || !pe || !force || !virial)) throw std::invalid_argument("null array");
   const bool by_gid = (flags & 8u) != 0;
   if (by_gid && !gid) throw std::invalid_argument("DMGX_SORT_BY_GID needs global_id");
-  if (N == 0 || nd2 == nd1) return;
-  DevBox b = make_devbox(box);
+  if (N == 0 || nd2 == nd1) return p;
+  p.box = make_devbox(box);
   for (int d = 0; d < 3; ++d)
-    if (b.pbc[d] && b.thick[d] <= 2.5 * (model_.rcR_max + kSkin))
+    if (p.box.pbc[d] && p.box.thick[d] <= 2.5 * (model_.rcR_max + kSkin))
       throw std::invalid_argument(
-          "box thinner than 2.5 (rc + skin) in a periodic direction (use the small-box path)");
-  const long long ncell = (long long)b.nc[0] * b.nc[1] * b.nc[2];
+          "box thinner than 2.5 (rc + skin) in a periodic direction");
+  const long long ncell = (long long)p.box.nc[0] * p.box.nc[1] * p.box.nc[2];
   if (ncell > (1LL << 28)) throw std::invalid_argument("cell grid too large");
   if (N > w_.cap || w_.MV == 0) reserve(std::max(N, (int)(w_.cap * 1.25)));
   ensure_cells((int)ncell);
-  const bool forced = !have_list_ || box_changed_ || N != last_N_ || nd1 != last_nd1_ || nd2 != last_nd2_ ||
-                      by_gid != last_gid_sort_ || (flags & 2u) ||
-                      std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
-                      std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0;
-  // rebuild decision on the device: forced by the host, or any atom moved > skin/2
-  prof_.begin(kProfCheck, s);
-  check(launch_pack(b, N, model_.num_types, pos, type, forced, w_, s), "pack / displacement check");
-  prof_.end(s);
-  const double rv = model_.rcR_max + kSkin;
-  prof_.begin(kProfRebuild, s);
-  check(launch_rebuild(b, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w_, s), "rebuild");
-  prof_.end(s);
+  p.forced = !have_list_ || box_changed_ || N != last_N_ || nd1 != last_nd1_ || nd2 != last_nd2_ ||
+             by_gid != last_gid_sort_ || (flags & 2u) || std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
+             std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0;
+  p.empty = false;
   have_list_ = true;
   box_changed_ = false;
   last_N_ = N;
@@ -359,12 +360,27 @@ void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2,
   last_gid_sort_ = by_gid;
   std::memcpy(last_h_, box.h, sizeof(last_h_));
   std::memcpy(last_pbc_, box.pbc, sizeof(last_pbc_));
+  return p;
+}
+
+void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                             const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
+                             hipStream_t s)
+{
+  if (p.empty) return;
+  const bool by_gid = (flags & 8u) != 0;
+  const double rv = model_.rcR_max + kSkin;
+  prof_.begin(kProfRebuild, s);
+  check(launch_rebuild(p.box, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w_, s), "rebuild");
+  prof_.end(s);
   const bool acc = (flags & 1u) != 0;
   prof_.begin(kProfFilter, s);
-  check(launch_filter(b, dm_, nd1, nd2, w_, s), "filter");
+  check(launch_filter(p.box, dm_, nd1, nd2, w_, s), "filter");
   prof_.end(s);
   prof_.begin(kProfDescriptor, s);
-  check(launch_descriptor(dm_, N, nd1, nd2, type, pe, acc, w_, s), "descriptor");
+  // ANN: FP32 on the matrix cores unless BF16 operands or the vector path are asked for
+  const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
+  check(launch_descriptor(dm_, N, nd1, nd2, type, pe, acc, ann, w_, s), "descriptor");
   prof_.end(s);
   prof_.begin(kProfGather, s);
   check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s), "gather");
@@ -372,6 +388,76 @@ void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2,
   last_N_stats_ = N;
   last_c1_ = nd1;
   last_c2_ = nd2;
+}
+
+bool NepEngine::small_box(const Box& box, int reps[3]) const
+{
+  const DevBox b = make_devbox(box);
+  const double need = 2.5 * (model_.rcR_max + kSkin);
+  bool any = false;
+  for (int d = 0; d < 3; ++d) {
+    reps[d] = 1;
+    if (b.pbc[d] && b.thick[d] <= need) {
+      reps[d] = (int)std::floor(need / b.thick[d]) + 1;
+      any = true;
+    }
+  }
+  return any;
+}
+
+void NepEngine::enqueue_small(const Box& box, const int reps[3], int N, const int* type, const double* pos,
+                              const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
+                              hipStream_t s)
+{
+  const long long R = (long long)reps[0] * reps[1] * reps[2];
+  const long long n2 = (long long)N * R;
+  if (n2 > (1LL << 30)) throw std::invalid_argument("small box: too many periodic replicas");
+  if (n2 > sb_cap_) {
+    dalloc(&sb_pos_, 3 * (size_t)n2, "replica positions");
+    dalloc(&sb_type_, (size_t)n2, "replica types");
+    dalloc(&sb_gid_, (size_t)n2, "replica ids");
+    dalloc(&sb_pe_, (size_t)n2, "replica potential");
+    dalloc(&sb_f_, 3 * (size_t)n2, "replica forces");
+    dalloc(&sb_vir_, 9 * (size_t)n2, "replica virial");
+    sb_cap_ = n2;
+  }
+  Box big = box;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) big.h[3 * r + c] *= reps[c];
+  const bool by_gid = (flags & 8u) != 0;
+  check(launch_replicate(N, reps, make_devbox(box), pos, type, by_gid ? gid : nullptr, sb_pos_, sb_type_, sb_gid_, s),
+        "replicate");
+  const int M = (int)n2;
+  const uint64_t* gid2 = by_gid ? sb_gid_ : nullptr;
+  const unsigned f2 = flags & ~1u;  // the replicas' outputs are overwritten, then folded
+  const Plan p = plan(big, M, 0, N, 0, M, sb_type_, sb_pos_, gid2, sb_pe_, sb_f_, sb_vir_, f2);
+  prof_.begin(kProfCheck, s);
+  check(launch_pack(p.box, M, model_.num_types, sb_pos_, sb_type_, p.forced, w_, s), "pack / displacement check");
+  prof_.end(s);
+  launch_force(p, M, 0, N, 0, M, sb_type_, gid2, sb_pe_, sb_f_, sb_vir_, f2, s);
+  check(launch_fold(N, n2, (flags & 1u) != 0, sb_pe_, sb_f_, sb_vir_, pe, force, virial, s), "fold replicas");
+}
+
+void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                        const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
+                        unsigned flags, hipStream_t s)
+{
+  int reps[3];
+  if (N > 0 && nd2 > nd1 && small_box(box, reps)) {
+    if (n1 != 0 || nd1 != 0 || n2 != N || nd2 != N)
+      throw std::invalid_argument("a periodic box thinner than 2.5 (rc + skin) needs all atoms as centres");
+    if (!type || !pos || !pe || !force || !virial) throw std::invalid_argument("null array");
+    if ((flags & 8u) && !gid) throw std::invalid_argument("DMGX_SORT_BY_GID needs global_id");
+    enqueue_small(box, reps, N, type, pos, gid, pe, force, virial, flags, s);
+    return;
+  }
+  const Plan p = plan(box, N, n1, n2, nd1, nd2, type, pos, gid, pe, force, virial, flags);
+  if (p.empty) return;
+  // rebuild decision on the device: forced by the host, or any atom moved > skin/2
+  prof_.begin(kProfCheck, s);
+  check(launch_pack(p.box, N, model_.num_types, pos, type, p.forced, w_, s), "pack / displacement check");
+  prof_.end(s);
+  launch_force(p, N, n1, n2, nd1, nd2, type, gid, pe, force, virial, flags, s);
 }
 
 const int* NepEngine::read_flags(hipStream_t s)
@@ -644,7 +730,8 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
   }
   // NHC needs the pre-force kinetic energy and NPT the pressure on the host
   // every step (scalar chain/box state): those stay on the synchronous path.
-  if (ctl.needs_pre_force_ke() || ctl.has_barostat()) {
+  int reps[3];
+  if (ctl.needs_pre_force_ke() || ctl.has_barostat() || nep_->small_box(box_, reps)) {
     for (int step = 0; step < steps; ++step) {
       if (ctl.needs_pre_force_ke()) {
         thermo(th);
@@ -698,19 +785,36 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
         for (int k = 0; k < n; ++k) ctl.draw_bdp(&hrand_[2 * k], &hrand_[2 * k + 1]);
         check(hipMemcpyAsync(rand_, hrand_, 2 * n * sizeof(double), hipMemcpyHostToDevice, stream_), "draws");
       }
+      // per step: fused first half (pending factor, kick, drift, wrap, pack,
+      // displacement test), force, fused second half (+ thermo partials),
+      // and the thermo reduction that also forms the next factor
+      const double volume = box_volume(box_.h);
+      bool pending = false;
       for (int k = 0; k < n; ++k) {
         const int step = c0 + k;
-        vv(1, dt);
-        wrap();
-        nep_->enqueue(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
-        vv(0, dt);
+        const NepEngine::Plan plan =
+            nep_->plan(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u);
+        Profiler& p = nep_->profiler();
+        p.begin(kProfIntegrate, stream_);
+        check(launch_md_first(plan.box, N_, nep_->model().num_types, dt, mass_, pos_, vel_, force_,
+                              pending ? factor_ : nullptr, type_, plan.forced, nep_->work(), stream_),
+              "velocity Verlet (first half)");
+        p.end(stream_);
+        nep_->launch_force(plan, N_, 0, N_, 0, N_, type_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
+        p.begin(kProfIntegrate, stream_);
+        check(launch_md_second(N_, dt, mass_, vel_, force_, pe_, vir_, kind != 0, tw_, stream_),
+              "velocity Verlet (second half)");
+        p.end(stream_);
         if (kind != 0) {
-          enqueue_thermo();
-          check(launch_ensemble_scale(kind, N_, coupling, ctl.target_temperature(step, steps), rand_ + 2 * k,
-                                      factor_, tw_.thermo, vel_, stream_),
+          p.begin(kProfThermo, stream_);
+          check(launch_thermo_final(N_, volume, kind, coupling, ctl.target_temperature(step, steps), rand_ + 2 * k,
+                                    factor_, tw_, stream_),
                 "thermostat");
+          p.end(stream_);
+          pending = true;
         }
       }
+      if (pending) check(launch_scale_by(3 * N_, factor_, vel_, stream_), "thermostat scale");
       redo = nep_->settle(stream_);
       if (!redo) break;
       checkpoint(false);

@@ -79,6 +79,23 @@ class NepEngine {
                const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
                hipStream_t stream);
   bool settle(hipStream_t stream);
+  // enqueue() in two halves for callers that pack the step's positions
+  // themselves (the MD loop fuses the pack into its integrator kernel):
+  // plan() validates and takes the host-side rebuild decision; launch_force()
+  // enqueues the rebuild, filter, descriptor and gather kernels.
+  struct Plan {
+    DevBox box;
+    bool forced = false;
+    bool empty = true;
+  };
+  Plan plan(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type, const double* pos,
+            const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags);
+  void launch_force(const Plan& p, int N, int n1, int n2, int nd1, int nd2, const int* type, const uint64_t* gid,
+                    double* pe, double* force, double* virial, unsigned flags, hipStream_t stream);
+  DevWork& work() { return w_; }
+  // A periodic direction thinner than 2.5 (rc + skin) is evaluated on a
+  // replicated supercell; reps[d] is the replication per box vector.
+  bool small_box(const Box& box, int reps[3]) const;
   const int* read_flags(hipStream_t stream);
   void notify_box_changed() { box_changed_ = true; }
   void stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds);
@@ -89,6 +106,9 @@ class NepEngine {
 
  private:
   DevBox make_devbox(const Box& box) const;
+  void enqueue_small(const Box& box, const int reps[3], int N, const int* type, const double* pos,
+                     const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
+                     hipStream_t stream);
   void ensure_cells(int ncell);
   void allocate(int capacity, int mv);
   void free_all();
@@ -104,6 +124,11 @@ class NepEngine {
   int last_pbc_[3] = {};
   int last_N_stats_ = 0, last_c1_ = 0, last_c2_ = 0;
   Profiler prof_;
+  // replicated small-box system (atoms x replicas)
+  long long sb_cap_ = 0;
+  double *sb_pos_ = nullptr, *sb_pe_ = nullptr, *sb_f_ = nullptr, *sb_vir_ = nullptr;
+  int* sb_type_ = nullptr;
+  uint64_t* sb_gid_ = nullptr;
 };
 
 class MdState {

@@ -175,6 +175,24 @@ __device__ __forceinline__ void zbl_pair(const float* __restrict__ zp, float d, 
   fp = vp * fc + vv * fcp;
 }
 
+// Loads the first kp floats of a 16-byte aligned row as float4s into out[0..KP)
+// (kp <= KP, both multiples of 4); entries past kp are left as they are.
+template <int KP>
+__device__ __forceinline__ void load_row4(const float* __restrict__ row, int kp, float* out)
+{
+  const float4* q = reinterpret_cast<const float4*>(row);
+#pragma unroll
+  for (int c = 0; c < KP / 4; ++c) {
+    if (4 * c < kp) {
+      const float4 t = q[c];
+      out[4 * c] = t.x;
+      out[4 * c + 1] = t.y;
+      out[4 * c + 2] = t.z;
+      out[4 * c + 3] = t.w;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // per-step typewise filter (nep.cu:503-561) into compact edge rows
 // ---------------------------------------------------------------------------
@@ -188,14 +206,29 @@ __device__ __forceinline__ void zbl_pair(const float* __restrict__ zp, float d, 
 // round.  Types were validated when the step's positions were packed; here
 // they are only clamped into range.  Pair cutoffs are FP64
 // ((rc_i + rc_j) / 2)^2, as the reference's typewise test (nep.cu:503-561).
-constexpr int kFilterLanes = 16;
-constexpr int kFilterBatch = 3;
-constexpr int kGatherLanes = 4;
+#ifndef DMGX_FILTER_LANES
+#define DMGX_FILTER_LANES 16
+#endif
+#ifndef DMGX_FILTER_BATCH
+#define DMGX_FILTER_BATCH 7
+#endif
+#ifndef DMGX_GATHER_LANES
+#define DMGX_GATHER_LANES 8
+#endif
+#ifndef DMGX_GATHER_BATCH
+#define DMGX_GATHER_BATCH 2
+#endif
+constexpr int kFilterLanes = DMGX_FILTER_LANES;
+constexpr int kFilterBatch = DMGX_FILTER_BATCH;
+constexpr int kGatherLanes = DMGX_GATHER_LANES;
+constexpr int kGatherBatch = DMGX_GATHER_BATCH;  // radial edges per lane per batch (kR <= 9)
 
 template <bool ONE>
 __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1, int c2, int nblocks, DevWork w)
 {
   constexpr int G = kFilterLanes, PER = kBlock / G, RB = kFilterBatch;
+  static_assert(G >= 1 && G <= 32 && (G & (G - 1)) == 0, "filter lanes: a power of two up to 32");
+  constexpr unsigned long long gmask = (1ull << G) - 1ull;
   __shared__ double cut2[ONE ? 1 : 2 * kMaxTypes * kMaxTypes];  // [ti][tj] radial, then angular
   const int T = m.T;
   double rr2 = 0.0, ra2 = 0.0;
@@ -211,6 +244,11 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
       cut2[T * T + k] = ra * ra;
     }
     __syncthreads();
+  }
+  // the rebuild kernels of this step ran before this one: close the request
+  if (blockIdx.x == 0 && threadIdx.x == 0 && w.flags[kFlagRebuild]) {
+    w.flags[kFlagRebuild] = 0;
+    w.flags[kFlagRebuildCount] += 1;
   }
   const int g = threadIdx.x / G, lg = threadIdx.x % G, gw = g % (64 / G);
   const int i = c1 + xcd_tile(nblocks) * PER + g;
@@ -261,15 +299,14 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
           rec = make_float4((float)x, (float)y, (float)z, __int_as_float(jj[r]));
         }
       }
-      const unsigned gA = (unsigned)(__ballot(isA) >> (gw * G)) & ((1u << G) - 1u);
-      const unsigned gR = (unsigned)(__ballot(isR) >> (gw * G)) & ((1u << G) - 1u);
-      if (isA) {
-        const int a = na + __popc(gA & below);
-        if (a < w.MA) {
-          w.edge[erow + a] = rec;
-          w.eq[arow + a] = qq[r];
-          w.vsa[vrow + p] = a;
-        }
+      const unsigned gA = (unsigned)((__ballot(isA) >> (gw * G)) & gmask);
+      const unsigned gR = (unsigned)((__ballot(isR) >> (gw * G)) & gmask);
+      // every row entry gets its angular slot or -1: one contiguous store
+      const int a = na + __popc(gA & below);
+      if (jj[r] >= 0) w.vsa[vrow + p] = (isA && a < w.MA) ? a : -1;
+      if (isA && a < w.MA) {
+        w.edge[erow + a] = rec;
+        w.eq[arow + a] = qq[r] >= 0 ? jj[r] * w.MV + qq[r] : -1;
       }
       if (isR) {
         const int slot = w.MR - 1 - (nro + __popc(gR & below));
@@ -290,14 +327,152 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
 // fused descriptor + ANN + adjoint + angular partial forces
 // ---------------------------------------------------------------------------
 
-template <int NR, int KR, int NA, int KA, int LM, bool Q222, bool Q1111, bool ONE, bool EXACT>
+// ANN modes of the descriptor kernel
+enum : int { kAnnValu = 0, kAnnMfma32 = 1, kAnnMfmaBf16 = 2 };
+
+template <int NR, int KR, int NA, int KA, int LM, bool Q222, bool Q1111, bool ONE, bool EXACT, int ANN = kAnnValu>
 struct Cfg {
-  static constexpr int nr = NR, kr = KR, na = NA, ka = KA, lm = LM;
+  static constexpr int nr = NR, kr = KR, na = NA, ka = KA, lm = LM, ann = ANN;
   static constexpr bool q222 = Q222, q1111 = Q1111, one = ONE, exact = EXACT;
   static constexpr int nabc = (LM + 1) * (LM + 1) - 1;
   static constexpr int numl = LM + (Q222 ? 1 : 0) + (Q1111 ? 1 : 0);
   static constexpr int dim = NR + NA * numl;
 };
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+// float -> bfloat16, round to nearest even (inputs are finite)
+__device__ __forceinline__ short to_bf16(float x)
+{
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// LDS floats per wavefront of the matrix-core ANN: 64 rows of DMAX+1 (odd
+// stride), then 64 energies
+template <int DMAX>
+constexpr int ann_lds_floats() { return 64 * (DMAX + 1) + 64; }
+
+// ANN of the 64 atoms of a wavefront on the matrix cores (single-type
+// model, H <= 64).  Lane l holds atom l's scaled descriptor q[0..D).
+//   GEMM 1: Z^T[h][atom] = sum_d W0[h][d] Q[atom][d]   (A = W0, B = Q^T)
+//   x = tanh(Z - b0), E = sum_h w1 x, G = w1 (1 - x^2)
+//   GEMM 2: Fp[atom][d] = sum_h G[atom][h] W0[h][d]     (A = G, B = W0)
+// With the 16x16 output layout (lane l, register r) -> (row 4(l/16)+r,
+// column l%16), the GEMM-1 accumulator of lane l holds atom 16nt + l%16 at
+// hidden units 16mt + 4(l/16) + r; ordering GEMM 2's contraction index as
+// (mt, r) makes those registers its A operand as they are.  Fp rows and the
+// energies go back to their lanes through LDS.  BF16 rounds Q, W0 and G to
+// bfloat16 (16x16x16 bf16 MFMA, FP32 accumulation); FP32 uses the 16x16x4
+// FP32 MFMA (exact products).
+template <int DMAX, bool BF16>
+__device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* q, float* Fp, float& E,
+                                         float* __restrict__ lds)
+{
+  constexpr int DS = DMAX + 1;
+  constexpr int DT = (DMAX + 15) / 16;
+  const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  float* Qs = lds;
+  float* Es = lds + 64 * DS;
+#pragma unroll
+  for (int d = 0; d < DMAX; ++d) Qs[lane * DS + d] = d < D ? q[d] : 0.0f;
+  __syncthreads();
+  const int H = m.H, HT = (H + 15) >> 4;
+  const float* __restrict__ W0 = m.w0;
+  const float* __restrict__ b0 = m.b0;
+  const float* __restrict__ w1 = m.w1;
+#pragma unroll 1
+  for (int nt = 0; nt < 4; ++nt) {
+    const float* qrow = Qs + (16 * nt + lr) * DS;
+    f32x4 facc[DT];
+#pragma unroll
+    for (int dt = 0; dt < DT; ++dt) facc[dt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float ep = 0.0f;
+    // one tile of 16 hidden units at a time: GEMM 1, activation, and its
+    // share of GEMM 2, so that only one accumulator tile is live
+#pragma unroll 1
+    for (int mt = 0; mt < HT; ++mt) {
+      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+      const int ha = 16 * mt + lr;
+      if (!BF16) {
+#pragma unroll
+        for (int ks = 0; ks < (DMAX + 3) / 4; ++ks) {
+          const int d = 4 * ks + lg;
+          const float bq = d < D ? qrow[d] : 0.0f;
+          const float a = (ha < H && d < D) ? W0[ha * D + d] : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bq, acc, 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int ks = 0; ks < (DMAX + 15) / 16; ++ks) {
+          s16x4 bq, a;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int d = 16 * ks + 4 * lg + c;
+            bq[c] = to_bf16(d < D ? qrow[d] : 0.0f);
+            a[c] = to_bf16((ha < H && d < D) ? W0[ha * D + d] : 0.0f);
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
+        }
+      }
+      float G[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int h = 16 * mt + 4 * lg + r;
+        G[r] = 0.0f;
+        if (h < H) {
+          const float x = tanhf(acc[r] - b0[h]);
+          ep += w1[h] * x;
+          G[r] = w1[h] * (1.0f - x * x);
+        }
+      }
+      if (!BF16) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int h = 16 * mt + 4 * lg + r;
+#pragma unroll
+          for (int dt = 0; dt < DT; ++dt) {
+            const int d = 16 * dt + lr;
+            const float bw = (h < H && d < D) ? W0[h * D + d] : 0.0f;
+            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(G[r], bw, facc[dt], 0, 0, 0);
+          }
+        }
+      } else {
+        s16x4 a;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = to_bf16(G[r]);
+#pragma unroll
+        for (int dt = 0; dt < DT; ++dt) {
+          const int d = 16 * dt + lr;
+          s16x4 bw;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int h = 16 * mt + 4 * lg + c;
+            bw[c] = to_bf16((h < H && d < D) ? W0[h * D + d] : 0.0f);
+          }
+          facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, facc[dt], 0, 0, 0);
+        }
+      }
+    }
+    ep += __shfl_xor(ep, 16);
+    ep += __shfl_xor(ep, 32);
+    if (lg == 0) Es[16 * nt + lr] = ep;
+    // this tile's Fp rows replace its (already consumed) Q rows
+#pragma unroll
+    for (int dt = 0; dt < DT; ++dt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int d = 16 * dt + lr;
+        if (d < DMAX) Qs[(16 * nt + 4 * lg + r) * DS + d] = facc[dt][r];
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < DMAX; ++d) Fp[d] = d < D ? Qs[lane * DS + d] : 0.0f;
+  E = Es[lane];
+}
 
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1, int c2, int nblocks,
@@ -305,8 +480,12 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
                                                        int accumulate, DevWork w)
 {
   constexpr int NR = C::nr, KR = C::kr, NA = C::na, KA = C::ka, LM = C::lm, NABC = C::nabc, DMAX = C::dim;
-  const int i = c1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
-  if (i >= c2) return;
+  const int i0 = c1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
+  const bool active = i0 < c2;
+  // the matrix-core ANN needs every lane of the wavefront: lanes past the
+  // range carry an empty row of a valid atom and write nothing
+  if (C::ann == kAnnValu && !active) return;
+  const int i = active ? i0 : c1;
   const int nR = C::exact ? NR : m.nR, nA = C::exact ? NA : m.nA;
   const int kR = C::exact ? KR : m.kR, kA = C::exact ? KA : m.kA;
   const bool h222 = C::exact ? C::q222 : (m.has222 != 0);
@@ -320,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
     rcRf[0] = (float)m.rcR[0];
     rcAf[0] = (float)m.rcA[0];
   }
-  const int nr = w.nr[i], na = w.na[i];
+  const int nr = active ? w.nr[i] : 0, na = active ? w.na[i] : 0;
   const float4* __restrict__ erow = w.edge + (size_t)i * w.MR;
   float4* __restrict__ frow = w.f12 + (size_t)i * w.MA;
   float q[DMAX];
@@ -462,26 +641,31 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
 #pragma unroll
   for (int d = 0; d < DMAX; ++d)
     if (d < D) q[d] *= m.qs[d];
-  const int H = m.H;
-  const float* w0 = m.w0 + (size_t)ti * H * D;
-  const float* b0 = m.b0 + (size_t)ti * H;
-  const float* w1 = m.w1 + (size_t)ti * H;
   float Fp[DMAX];
-#pragma unroll
-  for (int d = 0; d < DMAX; ++d) Fp[d] = 0.0f;
   float E = 0.0f;
-  for (int h = 0; h < H; ++h) {
-    const float* wr = w0 + (size_t)h * D;
-    float a = 0.0f;
+  if (C::ann != kAnnValu) {
+    __shared__ float ann_lds[kBlock / 64][ann_lds_floats<DMAX>()];
+    ann_mfma<DMAX, C::ann == kAnnMfmaBf16>(m, D, q, Fp, E, ann_lds[threadIdx.x >> 6]);
+  } else {
+    const int H = m.H;
+    const float* w0 = m.w0 + (size_t)ti * H * D;
+    const float* b0 = m.b0 + (size_t)ti * H;
+    const float* w1 = m.w1 + (size_t)ti * H;
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d)
-      if (d < D) a += wr[d] * q[d];
-    const float x = tanhf(a - b0[h]);
-    E += w1[h] * x;
-    const float gq = w1[h] * (1.0f - x * x);
+    for (int d = 0; d < DMAX; ++d) Fp[d] = 0.0f;
+    for (int h = 0; h < H; ++h) {
+      const float* wr = w0 + (size_t)h * D;
+      float a = 0.0f;
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d)
-      if (d < D) Fp[d] += gq * wr[d];
+      for (int d = 0; d < DMAX; ++d)
+        if (d < D) a += wr[d] * q[d];
+      const float x = tanhf(a - b0[h]);
+      E += w1[h] * x;
+      const float gq = w1[h] * (1.0f - x * x);
+#pragma unroll
+      for (int d = 0; d < DMAX; ++d)
+        if (d < D) Fp[d] += gq * wr[d];
+    }
   }
   E -= m.b1[0];
   if (m.version == 5) E -= m.w1x[ti];
@@ -489,21 +673,22 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
   for (int d = 0; d < DMAX; ++d)
     if (d < D) Fp[d] *= m.qs[d];
   const double etot = (double)E + (double)zpe;
-  pe[i] = accumulate ? pe[i] + etot : etot;
+  if (active) pe[i] = accumulate ? pe[i] + etot : etot;
 
   // ---- radial adjoint in basis space: bR[i][t][k] = sum_n Fp_n c[ti,t,n,k]
   for (int t = 0; t < T; ++t) {
     const float* c = m.cR + (size_t)(ti * T + t) * nR * kR;
-    float* out = w.bR + ((size_t)i * T + t) * kR;
+    float* out = w.bR + ((size_t)i * T + t) * m.kRp;
+    if (!active) break;
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
+    for (int k = 0; k < ((KR + 3) & ~3); ++k) {
+      float v = 0.0f;
       if (k < kR) {
-        float v = 0.0f;
 #pragma unroll
         for (int n = 0; n < NR; ++n)
           if (n < nR) v += Fp[n] * c[n * kR + k];
-        out[k] = v;
       }
+      if (k < m.kRp) out[k] = v;
     }
   }
 
@@ -657,7 +842,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
   // RB at a time: the RB edge records are loaded, then the RB neighbour
   // adjoint rows, then all of them are used, so the loads of a batch overlap.
   constexpr int PER = kBlock / G;
-  constexpr int RB = KR <= 9 ? 4 : 2;
+  constexpr int RB = KR <= 9 ? kGatherBatch : 2;
+  static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "gather lanes: a power of two");
   const int g = threadIdx.x / G, lg = threadIdx.x % G;
   const int i = n1 + xcd_tile(nblocks) * PER + g;
   const bool active = i < n2;
@@ -669,20 +855,18 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
     nr = w.nr[i];
     na = w.na[i];
   }
+  constexpr int KP = (KR + 3) & ~3;  // padded row: KP/4 float4 loads
+  const int kRp = m.kRp;
   const float4* __restrict__ erow = w.edge + (size_t)i * w.MR;
   const float4* __restrict__ frow = w.f12 + (size_t)i * w.MA;
   const int* __restrict__ qrow = w.eq + (size_t)i * w.MA;
   const float* __restrict__ bR = w.bR;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  float bi[KR];
+  float bi[KP];
 #pragma unroll
-  for (int k = 0; k < KR; ++k) bi[k] = 0.0f;
-  if (ONE && active) {
-#pragma unroll
-    for (int k = 0; k < KR; ++k)
-      if (k < kR) bi[k] = bR[(size_t)i * kR + k];
-  }
+  for (int k = 0; k < KP; ++k) bi[k] = 0.0f;
+  if (ONE && active) load_row4<KP>(bR + (size_t)i * kRp, kRp, bi);
   const float rcR0 = (float)((m.rcR[0] + m.rcR[0]) * 0.5);
   // radial: F_i += (G_i + G_j) r_ij/d with G_c = sum_k b_c[k] f'_k(d)
   for (int e0 = lg; e0 < nr; e0 += G * RB) {
@@ -700,17 +884,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) tt[r] = (ONE || jj[r] < 0) ? 0 : safe_type(type[jj[r]], T);
-    float bj[RB][KR], bo[ONE ? 1 : RB][KR];
+    float bj[RB][KP], bo[ONE ? 1 : RB][KP];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-      const bool jdep = jj[r] >= nd1 && jj[r] < nd2;
-      const float* pj = bR + ((size_t)(jdep ? jj[r] : 0) * T + ti) * kR;
 #pragma unroll
-      for (int k = 0; k < KR; ++k) bj[r][k] = (jdep && k < kR) ? pj[k] : 0.0f;
+      for (int k = 0; k < KP; ++k) bj[r][k] = 0.0f;
+      if (jj[r] >= nd1 && jj[r] < nd2) load_row4<KP>(bR + ((size_t)jj[r] * T + ti) * kRp, kRp, bj[r]);
       if (!ONE) {
-        const float* po = bR + ((size_t)i * T + tt[r]) * kR;
 #pragma unroll
-        for (int k = 0; k < KR; ++k) bo[ONE ? 0 : r][k] = (jj[r] >= 0 && k < kR) ? po[k] : 0.0f;
+        for (int k = 0; k < KP; ++k) bo[ONE ? 0 : r][k] = 0.0f;
+        if (jj[r] >= 0) load_row4<KP>(bR + ((size_t)i * T + tt[r]) * kRp, kRp, bo[ONE ? 0 : r]);
       }
     }
 #pragma unroll
@@ -759,8 +942,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
     for (int r = 0; r < AB; ++r) {
       const int e = e0 + r * G;
       rr[r] = aa[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-      jj[r] = -1;
-      qv[r] = -1;
+      jj[r] = qv[r] = -1;
       if (e < na) {
         rr[r] = erow[e];
         aa[r] = frow[e];
@@ -768,14 +950,17 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
         jj[r] = __float_as_int(rr[r].w);
       }
     }
+    const int lim = w.cap * w.MV;
 #pragma unroll
     for (int r = 0; r < AB; ++r) {
       e2[r] = -1;
-      if (jj[r] >= nd1 && jj[r] < nd2 && qv[r] >= 0 && qv[r] < w.MV) e2[r] = w.vsa[(size_t)jj[r] * w.MV + qv[r]];
+      if (jj[r] >= nd1 && jj[r] < nd2 && qv[r] >= 0 && qv[r] < lim) e2[r] = w.vsa[qv[r]];
     }
 #pragma unroll
-    for (int r = 0; r < AB; ++r)
-      bb[r] = (e2[r] >= 0 && e2[r] < w.MA) ? w.f12[(size_t)jj[r] * w.MA + e2[r]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int r = 0; r < AB; ++r) {
+      bb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (e2[r] >= 0 && e2[r] < w.MA) bb[r] = w.f12[(size_t)jj[r] * w.MA + e2[r]];
+    }
 #pragma unroll
     for (int r = 0; r < AB; ++r) {
       const float4 q = rr[r];
@@ -879,6 +1064,65 @@ __global__ void k_wrap(DevBox b, int N, double* __restrict__ pos)
   pos[2 * N + i] = b.h[6] * s[0] + b.h[7] * s[1] + b.h[8] * s[2];
 }
 
+// First half of a velocity-Verlet step of the MD loop in one kernel: the
+// previous step's pending thermostat factor (v *= f), the half kick and
+// drift, the periodic wrap, the step's packed (x, y, z, type) record and the
+// displacement test against the last rebuild.  The FP64 expressions are
+// those of k_scale_by, k_velocity_verlet, k_wrap and the neighbour pack, in
+// that order and without contraction, so the results are bit-identical to
+// running them as separate kernels.
+__global__ __launch_bounds__(kBlock) void k_md_first(DevBox b, int N, int T, double dt,
+                                                     const double* __restrict__ mass, double* __restrict__ pos,
+                                                     double* __restrict__ vel, const double* __restrict__ force,
+                                                     const double* __restrict__ factor,
+                                                     const int* __restrict__ type, const double* __restrict__ x0,
+                                                     double lim2, int forced, double4* __restrict__ xq,
+                                                     int* __restrict__ flags)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (forced && i == 0) flags[kFlagRebuild] = 1;
+  bool moved = false, bad = false;
+  if (i < N) {
+    const double half = dt * 0.5;
+    const double im = 1.0 / mass[i];
+    double x[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double v = vel[c * N + i];
+      if (factor) v *= factor[0];
+      v = v + force[c * N + i] * im * half;
+      vel[c * N + i] = v;
+      x[c] = pos[c * N + i] + v * dt;
+    }
+    double sf[3] = {b.hi[0] * x[0] + b.hi[1] * x[1] + b.hi[2] * x[2], b.hi[3] * x[0] + b.hi[4] * x[1] + b.hi[5] * x[2],
+                    b.hi[6] * x[0] + b.hi[7] * x[1] + b.hi[8] * x[2]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!b.pbc[d]) continue;
+      if (sf[d] < 0.0) sf[d] += 1.0;
+      else if (sf[d] > 1.0) sf[d] -= 1.0;
+    }
+    const double px = b.h[0] * sf[0] + b.h[1] * sf[1] + b.h[2] * sf[2];
+    const double py = b.h[3] * sf[0] + b.h[4] * sf[1] + b.h[5] * sf[2];
+    const double pz = b.h[6] * sf[0] + b.h[7] * sf[1] + b.h[8] * sf[2];
+    pos[i] = px;
+    pos[N + i] = py;
+    pos[2 * N + i] = pz;
+    const int t = type[i];
+    bad = t < 0 || t >= T;
+    xq[i] = make_double4(px, py, pz, __longlong_as_double((long long)t));
+    if (!forced) {
+      double dx = px - x0[i], dy = py - x0[N + i], dz = pz - x0[2 * N + i];
+      mic(b, dx, dy, dz);
+      moved = !(dist2(dx, dy, dz) <= lim2);
+    }
+  }
+  const bool lead = (threadIdx.x & 63) == 0;
+  if (__any(moved) && lead) atomicOr(&flags[kFlagRebuild], 1);
+  if (__any(bad) && lead) atomicOr(&flags[kFlagBadType], 1);
+}
+
 // error-free two-sum accumulation (s, c) += x
 struct KSum {
   double s, c;
@@ -931,6 +1175,29 @@ __device__ __forceinline__ void block_reduce8(KSum* acc)
       for (int w = 1; w < kThermoThreads / 64; ++w) ksum_merge(acc[q], KSum{sh[w][q][0], sh[w][q][1]});
 }
 
+// velocity factor from this step's normalised temperature, formed where the
+// thermo lives (same expressions as Controller::after_force, no contraction)
+__device__ __forceinline__ double ensemble_factor(int kind, int N, double coupling, double t_target,
+                                                  const double* __restrict__ rand2, double T)
+{
+#pragma clang fp contract(off)
+  double f = 1.0;
+  if (kind == 1) {
+    f = sqrt(1.0 + coupling * (t_target / T - 1.0));
+  } else if (kind == 2) {
+    // resamplekin (thermostat_svr.cpp:107-124) with rr and the chi-square sum drawn on the host
+    const int ndeg = 3 * N;
+    const double kk = T * (ndeg * 8.617343e-5 * 0.5);  // old_KE *= ndeg * K_B * 0.5
+    const double sigma = ndeg * 8.617343e-5 * t_target * 0.5;
+    const double fac = coupling;  // exp(-1/taut) or 0, formed on the host
+    const double rr = rand2[0], sum = rand2[1];
+    const double knew = kk + (1.0 - fac) * (sigma * (sum + rr * rr) / ndeg - kk) +
+                        2.0 * rr * sqrt(kk * sigma / ndeg * (1.0 - fac) * fac);
+    f = sqrt(knew / kk);
+  }
+  return f;
+}
+
 // per-workgroup compensated partial sums of the 8 thermo quantities
 // (find_owned_thermo_sums, runtime.cu:442-493)
 __global__ __launch_bounds__(kThermoThreads) void k_thermo_partial(int N, const double* __restrict__ mass,
@@ -939,6 +1206,7 @@ __global__ __launch_bounds__(kThermoThreads) void k_thermo_partial(int N, const 
                                                                    const double* __restrict__ vir,
                                                                    double* __restrict__ partial)
 {
+#pragma clang fp contract(off)
   KSum acc[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = {0.0, 0.0};
@@ -963,10 +1231,61 @@ __global__ __launch_bounds__(kThermoThreads) void k_thermo_partial(int N, const 
     }
 }
 
+// Second half kick of the MD step fused with the per-workgroup thermo
+// partial sums of the kicked velocities (the sums of k_thermo_partial).
+template <bool THERMO>
+__global__ __launch_bounds__(kThermoThreads) void k_md_second(int N, double dt, const double* __restrict__ mass,
+                                                              double* __restrict__ vel,
+                                                              const double* __restrict__ force,
+                                                              const double* __restrict__ pe,
+                                                              const double* __restrict__ vir,
+                                                              double* __restrict__ partial)
+{
+#pragma clang fp contract(off)
+  KSum acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = {0.0, 0.0};
+  const double half = dt * 0.5;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const double m = mass[i];
+    const double im = 1.0 / m;
+    double v3[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double v = vel[c * N + i] + force[c * N + i] * im * half;
+      vel[c * N + i] = v;
+      v3[c] = v;
+    }
+    if (THERMO) {
+      const double vx = v3[0], vy = v3[1], vz = v3[2];
+      ksum_add(acc[0], m * (vx * vx + vy * vy + vz * vz));
+      ksum_add(acc[1], pe[i]);
+      ksum_add(acc[2], vir[i] + m * vx * vx);
+      ksum_add(acc[3], vir[N + i] + m * vy * vy);
+      ksum_add(acc[4], vir[2 * N + i] + m * vz * vz);
+      ksum_add(acc[5], vir[3 * N + i] + m * vx * vy);
+      ksum_add(acc[6], vir[4 * N + i] + m * vx * vz);
+      ksum_add(acc[7], vir[5 * N + i] + m * vy * vz);
+    }
+  }
+  if (THERMO) {
+    block_reduce8(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        partial[(blockIdx.x * 8 + q) * 2 + 0] = acc[q].s;
+        partial[(blockIdx.x * 8 + q) * 2 + 1] = acc[q].c;
+      }
+  }
+}
+
 // normalize_global_thermo (runtime_internal.hpp:704-716): T = sum m v^2/(3 N kB), stress/V
 __global__ __launch_bounds__(kThermoThreads) void k_thermo_final(int nb, int N, double volume,
                                                                  const double* __restrict__ partial,
-                                                                 double* __restrict__ thermo)
+                                                                 double* __restrict__ thermo, int kind,
+                                                                 double coupling, double t_target,
+                                                                 const double* __restrict__ rand2,
+                                                                 double* __restrict__ factor)
 {
   KSum acc[8];
 #pragma unroll
@@ -982,33 +1301,16 @@ __global__ __launch_bounds__(kThermoThreads) void k_thermo_final(int nb, int N, 
       if (q == 0) v /= 3.0 * N * 8.617343e-5;
       else if (q >= 2) v /= volume;
       thermo[q] = v;
+      if (q == 0 && factor) factor[0] = ensemble_factor(kind, N, coupling, t_target, rand2, v);
     }
   }
 }
 
-// velocity factor from this step's normalised thermo, computed where the
-// thermo lives (same expressions as Controller::after_force, no contraction)
 __global__ void k_ensemble_factor(int kind, int N, double coupling, double t_target,
                                   const double* __restrict__ rand2, const double* __restrict__ thermo,
                                   double* __restrict__ factor)
 {
-#pragma clang fp contract(off)
-  double f = 1.0;
-  const double T = thermo[0];
-  if (kind == 1) {
-    f = sqrt(1.0 + coupling * (t_target / T - 1.0));
-  } else if (kind == 2) {
-    // resamplekin (thermostat_svr.cpp:107-124) with rr and the chi-square sum drawn on the host
-    const int ndeg = 3 * N;
-    const double kk = T * (ndeg * 8.617343e-5 * 0.5);  // old_KE *= ndeg * K_B * 0.5
-    const double sigma = ndeg * 8.617343e-5 * t_target * 0.5;
-    const double fac = coupling;  // exp(-1/taut) or 0, formed on the host
-    const double rr = rand2[0], sum = rand2[1];
-    const double knew = kk + (1.0 - fac) * (sigma * (sum + rr * rr) / ndeg - kk) +
-                        2.0 * rr * sqrt(kk * sigma / ndeg * (1.0 - fac) * fac);
-    f = sqrt(knew / kk);
-  }
-  factor[0] = f;
+  factor[0] = ensemble_factor(kind, N, coupling, t_target, rand2, thermo[0]);
 }
 
 __global__ void k_scale_by(int n, const double* __restrict__ factor, double* __restrict__ data)
@@ -1054,8 +1356,23 @@ static void run_descriptor(const DevModel& m, int N, int c1, int c2, const int* 
   k_descriptor<C><<<xcd_grid(nb), kBlock, 0, s>>>(m, N, c1, c2, nb, type, pe, acc ? 1 : 0, w);
 }
 
+// single-type exact configuration with the ANN mode picked at run time:
+// matrix cores for H <= 64 (FP32 or BF16 operands), vector ALUs otherwise
+template <int NR, int KR, int NA, int KA, bool Q2, bool Q1>
+static void run_single(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
+                       DevWork& w, hipStream_t s)
+{
+  const int mode = m.H <= 64 ? ann : kAnnValu;
+  if (mode == kAnnMfma32)
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfma32>>(m, N, c1, c2, type, pe, acc, w, s);
+  else if (mode == kAnnMfmaBf16)
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaBf16>>(m, N, c1, c2, type, pe, acc, w, s);
+  else
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s);
+}
+
 hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
-                             bool acc, DevWork& w, hipStream_t s)
+                             bool acc, int ann, DevWork& w, hipStream_t s)
 {
   if (c2 <= c1) return hipSuccess;
   const bool one = m.T == 1;
@@ -1065,9 +1382,9 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
   };
   // exact instantiations of the configurations in use; generic otherwise
   if (is(5, 9, 5, 9, 4, true, true, true))
-    run_descriptor<Cfg<5, 9, 5, 9, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_single<5, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s);
   else if (is(7, 9, 5, 9, 4, true, false, true))
-    run_descriptor<Cfg<7, 9, 5, 9, 4, true, false, true, true>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_single<7, 9, 5, 9, true, false>(m, N, c1, c2, type, pe, acc, ann, w, s);
   else if (is(13, 17, 9, 13, 4, true, true, true))
     run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s);
   else if (is(7, 11, 7, 11, 4, true, true, false))
@@ -1121,7 +1438,7 @@ hipError_t launch_thermo(int N, double volume, const double* mass, const double*
 {
   const int nb = thermo_blocks(N);
   k_thermo_partial<<<nb, kThermoThreads, 0, s>>>(N, mass, pe, vel, virial, w.partial);
-  k_thermo_final<<<1, kThermoThreads, 0, s>>>(nb, N, volume, w.partial, w.thermo);
+  k_thermo_final<<<1, kThermoThreads, 0, s>>>(nb, N, volume, w.partial, w.thermo, 0, 0.0, 0.0, nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -1136,6 +1453,107 @@ hipError_t launch_ensemble_scale(int kind, int N, double coupling, double t_targ
 {
   k_ensemble_factor<<<1, 1, 0, s>>>(kind, N, coupling, t_target, rand2, thermo, factor);
   k_scale_by<<<nblk(3 * N), kBlock, 0, s>>>(3 * N, factor, vel);
+  return hipGetLastError();
+}
+
+hipError_t launch_md_first(const DevBox& b, int N, int T, double dt, const double* mass, double* pos, double* vel,
+                           const double* force, const double* factor, const int* type, bool forced, DevWork& w,
+                           hipStream_t s)
+{
+  const double half = 0.5 * kSkin;
+  if (N > 0)
+    k_md_first<<<nblk(N), kBlock, 0, s>>>(b, N, T, dt, mass, pos, vel, force, factor, type, w.x0, half * half,
+                                          forced ? 1 : 0, w.xq, w.flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_md_second(int N, double dt, const double* mass, double* vel, const double* force, const double* pe,
+                            const double* virial, bool thermo, DevWork& tw, hipStream_t s)
+{
+  if (N <= 0) return hipSuccess;
+  const int nb = thermo_blocks(N);
+  if (thermo)
+    k_md_second<true><<<nb, kThermoThreads, 0, s>>>(N, dt, mass, vel, force, pe, virial, tw.partial);
+  else
+    k_md_second<false><<<nb, kThermoThreads, 0, s>>>(N, dt, mass, vel, force, pe, virial, tw.partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_thermo_final(int N, double volume, int kind, double coupling, double t_target,
+                               const double* rand2, double* factor, DevWork& tw, hipStream_t s)
+{
+  k_thermo_final<<<1, kThermoThreads, 0, s>>>(thermo_blocks(N), N, volume, tw.partial, tw.thermo, kind, coupling,
+                                                t_target, rand2, kind ? factor : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_by(int n, const double* factor, double* data, hipStream_t s)
+{
+  if (n > 0) k_scale_by<<<nblk(n), kBlock, 0, s>>>(n, factor, data);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// small boxes: periodic replicas and the fold of their results
+// ---------------------------------------------------------------------------
+
+// Replica k = (ia, ib, ic) of atom i lands at slot k*N + i, shifted by
+// ia*a + ib*b + ic*c (the box columns); replica 0 is the original.
+__global__ void k_replicate(int N, int r0, int r1, int r2, DevBox b, const double* __restrict__ pos,
+                            const int* __restrict__ type, const uint64_t* __restrict__ gid,
+                            double* __restrict__ pos2, int* __restrict__ type2, uint64_t* __restrict__ gid2)
+{
+#pragma clang fp contract(off)
+  const int R = r0 * r1 * r2;
+  const long long n2 = (long long)N * R;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n2) return;
+  const int k = (int)(t / N), i = (int)(t % N);
+  const int ia = k % r0, ib = (k / r0) % r1, ic = k / (r0 * r1);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    pos2[c * n2 + t] = pos[c * N + i] + (ia * b.h[3 * c] + ib * b.h[3 * c + 1] + ic * b.h[3 * c + 2]);
+  type2[t] = type[i];
+  if (gid) gid2[t] = gid[i];
+}
+
+// Copies (or adds) the first N entries of each component of the replicated
+// outputs (stride n2) into the caller's arrays (stride N).
+__global__ void k_fold(int N, long long n2, int accumulate, const double* __restrict__ pe2,
+                       const double* __restrict__ f2, const double* __restrict__ v2, double* __restrict__ pe,
+                       double* __restrict__ force, double* __restrict__ virial)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (accumulate) {
+    pe[i] += pe2[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) force[c * N + i] += f2[c * n2 + i];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) virial[(size_t)c * N + i] += v2[c * n2 + i];
+  } else {
+    pe[i] = pe2[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) force[c * N + i] = f2[c * n2 + i];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) virial[(size_t)c * N + i] = v2[c * n2 + i];
+  }
+}
+
+hipError_t launch_replicate(int N, const int reps[3], const DevBox& b, const double* pos, const int* type,
+                            const uint64_t* gid, double* pos2, int* type2, uint64_t* gid2, hipStream_t s)
+{
+  const long long n2 = (long long)N * reps[0] * reps[1] * reps[2];
+  if (n2 > 0)
+    k_replicate<<<(unsigned)((n2 + kBlock - 1) / kBlock), kBlock, 0, s>>>(N, reps[0], reps[1], reps[2], b, pos,
+                                                                        type, gid, pos2, type2, gid2);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold(int N, long long n2, bool accumulate, const double* pe2, const double* f2, const double* v2,
+                       double* pe, double* force, double* virial, hipStream_t s)
+{
+  if (N > 0) k_fold<<<nblk(N), kBlock, 0, s>>>(N, n2, accumulate ? 1 : 0, pe2, f2, v2, pe, force, virial);
   return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ struct DevBox {
 // Model parameters as kernel arguments; tables live in device memory.
 struct DevModel {
   int T, nR, nA, kR, kA, L, has222, has1111, numL, D, H, version;
+  int kRp;  // kR rounded up to a multiple of 4 (row pitch of the radial adjoint)
   int zbl;
   double rcR[kMaxTypes], rcA[kMaxTypes];  // FP64 per-type cutoffs (predicate)
   const float* w0;    // [T][H][D]
@@ -62,9 +63,10 @@ struct DevWork {
   int* na = nullptr;           // [cap] angular row length
   float4* edge = nullptr;      // [cap*MR] (rx, ry, rz, bits(j)): angular edges at
                                // 0..na-1, radial-only edges at MR-1 downward
-  int* eq = nullptr;           // [cap*MA] reverse Verlet position of an angular edge
-  int* vsa = nullptr;          // [cap*MV] Verlet position -> angular slot
-  float* bR = nullptr;         // [cap*T*kR] radial adjoint in basis space
+  int* eq = nullptr;           // [cap*MA] of an angular edge i->j: j*MV + (position of i in
+                               // j's Verlet row), an index into vsa; -1 if j has no row
+  int* vsa = nullptr;          // [cap*MV] Verlet position -> angular slot, -1 if not angular
+  float* bR = nullptr;         // [cap*T*kRp] radial adjoint in basis space
   float4* f12 = nullptr;       // [cap*MA] dE_i/dr_ij of angular edges
   int* flags = nullptr;        // [kNumFlags] device status words
   double* partial = nullptr;   // thermo block partials
@@ -91,7 +93,8 @@ void upload_angular_constants(const AngularConstants& a);
 // the step, validates types and, unless `forced`, sets flags[kFlagRebuild]
 // when an atom moved more than skin/2; `forced` sets it unconditionally.
 // launch_rebuild enqueues the whole cell list + Verlet build; every kernel of
-// it returns at once unless flags[kFlagRebuild] is set.
+// it returns at once unless flags[kFlagRebuild] is set.  The request is
+// closed (and counted) by the filter kernel that follows.
 hipError_t launch_request_rebuild(DevWork& w, hipStream_t s);
 hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const int* type, bool forced, DevWork& w,
                        hipStream_t s);
@@ -100,8 +103,11 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
 
 // Force evaluation (per step)
 hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s);
+// ann: 0 FP32 on the vector ALUs, 1 FP32 on the matrix cores, 2 BF16 operands
+// on the matrix cores; modes 1 and 2 apply to single-type models of the
+// exact configurations with at most 64 neurons, the vector path otherwise.
 hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
-                             bool accumulate, DevWork& w, hipStream_t s);
+                             bool accumulate, int ann, DevWork& w, hipStream_t s);
 hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
                          double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s);
 
@@ -119,6 +125,29 @@ hipError_t launch_scale(int n, double factor, double* data, hipStream_t s);
 hipError_t launch_ensemble_scale(int kind, int N, double coupling, double t_target, const double* rand2,
                                  double* factor, const double* thermo, double* vel, hipStream_t s);
 hipError_t launch_check_scaled(int n, double factor, const double* data, int* flag, hipStream_t s);
+
+// Fused MD-loop kernels.  launch_md_first: v *= factor[0] (if factor), half
+// kick, drift, wrap, pack (x, y, z, type) into w.xq and the displacement test
+// (or a forced rebuild request).  launch_md_second: half kick, plus the thermo
+// partial sums into tw.partial when `thermo`.  launch_thermo_final: tw.thermo
+// from the partials and, when kind != 0, the velocity factor of that
+// thermostat into factor[0] (see launch_ensemble_scale for kind).
+hipError_t launch_md_first(const DevBox& b, int N, int T, double dt, const double* mass, double* pos, double* vel,
+                           const double* force, const double* factor, const int* type, bool forced, DevWork& w,
+                           hipStream_t s);
+hipError_t launch_md_second(int N, double dt, const double* mass, double* vel, const double* force, const double* pe,
+                            const double* virial, bool thermo, DevWork& tw, hipStream_t s);
+hipError_t launch_thermo_final(int N, double volume, int kind, double coupling, double t_target,
+                               const double* rand2, double* factor, DevWork& tw, hipStream_t s);
+hipError_t launch_scale_by(int n, const double* factor, double* data, hipStream_t s);
+
+// Small boxes: replicate N atoms into reps[0]*reps[1]*reps[2] periodic copies
+// (copy k = ia + r0*(ib + r1*ic) at slots k*N..k*N+N-1, shifted by
+// ia*a + ib*b + ic*c), and fold the first N results of the copies back.
+hipError_t launch_replicate(int N, const int reps[3], const DevBox& b, const double* pos, const int* type,
+                            const uint64_t* gid, double* pos2, int* type2, uint64_t* gid2, hipStream_t s);
+hipError_t launch_fold(int N, long long n2, bool accumulate, const double* pe2, const double* f2, const double* v2,
+                       double* pe, double* force, double* virial, hipStream_t s);
 
 // Number of thermo block partials launch_thermo uses for N atoms.
 int thermo_blocks(int N);

@@ -276,15 +276,6 @@ __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ 
   }
 }
 
-// closes a rebuild: clears the request and counts it
-__global__ void k_rebuild_done(int* __restrict__ flags)
-{
-  if (flags[kFlagRebuild]) {
-    flags[kFlagRebuild] = 0;
-    flags[kFlagRebuildCount] += 1;
-  }
-}
-
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 
 // centres per build workgroup: 16 groups of kRowLanes lanes unless the LDS
@@ -333,7 +324,6 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
     constexpr int rows_per_block = 256 / kRowLanes;
     k_verlet_reverse<<<(c2 - c1 + rows_per_block - 1) / rows_per_block, 256, 0, s>>>(go, c1, c2, gid, w);
   }
-  k_rebuild_done<<<1, 1, 0, s>>>(w.flags);
   return hipGetLastError();
 }
 

@@ -105,25 +105,29 @@ class NEP:
         check(lib().dmgx_nep_reserve(self._h, int(capacity)))
 
     def compute(self, box: Box, type, position, potential, force, virial, *, accumulate=False,
-                ann_bf16=False, force_rebuild=False, stream=None) -> None:
+                ann_bf16=False, ann_valu=False, force_rebuild=False, stream=None) -> None:
         """Potential::compute: every atom is a force and dependency centre."""
         N = type.numel()
         self.compute_domain(box, N, type, position, potential, force, virial, n1=0, n2=N, nd1=0,
-                            nd2=N, accumulate=accumulate, ann_bf16=ann_bf16,
+                            nd2=N, accumulate=accumulate, ann_bf16=ann_bf16, ann_valu=ann_valu,
                             force_rebuild=force_rebuild, stream=stream)
 
     def compute_domain(self, box: Box, num_atoms: int, type, position, potential, force, virial,
                        global_id=None, *, n1=0, n2=None, nd1=0, nd2=None, sort_by_gid=False,
-                       accumulate=False, ann_bf16=False, force_rebuild=False, stream=None) -> None:
+                       accumulate=False, ann_bf16=False, ann_valu=False, force_rebuild=False,
+                       stream=None) -> None:
         """NEP::compute_domain: [n1,n2) force centres, [nd1,nd2) dependency
-        centres, every slot of [0,num_atoms) a neighbour candidate."""
+        centres, every slot of [0,num_atoms) a neighbour candidate.  The ANN
+        runs in FP32 on the matrix cores where the model allows (single type,
+        <= 64 neurons); ann_valu keeps it on the vector ALUs, ann_bf16 uses
+        bfloat16 operands."""
         import torch
         N = int(num_atoms)
         n2 = N if n2 is None else n2
         nd2 = N if nd2 is None else nd2
         flags = ((_lib.ACCUMULATE if accumulate else 0) | (_lib.ANN_BF16 if ann_bf16 else 0) |
                  (_lib.FORCE_REBUILD if force_rebuild else 0) |
-                 (_lib.SORT_BY_GID if sort_by_gid else 0))
+                 (_lib.SORT_BY_GID if sort_by_gid else 0) | (_lib.ANN_VALU if ann_valu else 0))
         b = box.to_c()
         check(lib().dmgx_nep_compute(
             self._h, ctypes.byref(b), N, n1, n2, nd1, nd2,
@@ -214,13 +218,14 @@ class MD:
 
     def run(self, steps: int, dt_fs: float, ensemble: str = "nve", t_init=0.0, t_final=None,
             t_coupling=100.0, p_target_gpa=0.0, bulk_modulus_gpa=0.0, p_coupling=1000.0,
-            seed=12345678, ann_bf16=False) -> np.ndarray:
+            seed=12345678, ann_bf16=False, ann_valu=False) -> np.ndarray:
         e = _lib.Ensemble(ENSEMBLES[ensemble], float(t_init), float(t_init if t_final is None else t_final),
                           float(t_coupling), float(p_target_gpa), float(bulk_modulus_gpa),
                           float(p_coupling), int(seed))
         th = np.zeros(8)
         check(lib().dmgx_md_run(self._h, int(steps), float(dt_fs), ctypes.byref(e),
-                                _lib.ANN_BF16 if ann_bf16 else 0, th.ctypes.data_as(_lib.PD)))
+                                (_lib.ANN_BF16 if ann_bf16 else 0) | (_lib.ANN_VALU if ann_valu else 0),
+                                th.ctypes.data_as(_lib.PD)))
         return th
 
     def arrays(self) -> dict:

@@ -61,9 +61,13 @@ typedef struct dmgx_nep_info {
 #define DMGX_ACCUMULATE 1u     /* add into pe/force/virial (the reference's +=
                                   contract, nep.cu:745,869); default overwrite */
 #define DMGX_FORCE_REBUILD 2u  /* rebuild the Verlet list regardless of skin */
-#define DMGX_ANN_BF16 4u       /* ANN on bf16 MFMA with fp32 accumulation */
+#define DMGX_ANN_BF16 4u       /* ANN with bf16 operands on the matrix cores, fp32
+                                  accumulation (single-type models, <= 64 neurons;
+                                  other models keep the fp32 ANN) */
 #define DMGX_SORT_BY_GID 8u    /* order rows by global_id (domain rows,
                                   neighbor.cuh:164-196); needs global_id */
+#define DMGX_ANN_VALU 16u      /* fp32 ANN on the vector ALUs; the default fp32 ANN
+                                  runs on the matrix cores where the model allows */
 
 /* NEP::NEP(const char*) -- parse nep.txt once and upload the parameters
  * (nep.cu:135-415).  Supports nep4, nep4_zbl, nep5, nep5_zbl with l_max <= 4

@@ -35,6 +35,7 @@ def check(err, scale_f=1.0, tol_e=TOL_E):
     ("single_large_nve", "main", "nep_C.txt", 1.0),
     ("multi_nvt_restart", "initial", "nep_water.txt", 1.0),
     ("nep_zbl_boundary", "main", "nep_BaTiO3_zbl.txt", 50.0),  # 1.2e4 eV/A ZBL pair in FP32
+    ("single_small_static", "main", "nep_C.txt", 1.0),  # small box: replicated supercell
 ])
 def test_golden_systems_against_oracle_and_gpumd(dmgx, case, stage, pot, fscale):
     g = read_xyz(os.path.join(BASELINE, "goldens", case, stage, "initial.xyz"))
@@ -217,3 +218,151 @@ def test_rejects_host_tensors_and_bad_ranges(dmgx, workloads):
                     torch.zeros(3 * N, dtype=torch.float64), torch.zeros(9 * N, dtype=torch.float64))
     with pytest.raises(dmgx.DmgxError):
         gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"], n1=10, n2=20, nd1=15, nd2=N)
+
+
+@pytest.mark.parametrize("ensemble", ["nvt_ber", "nvt_bdp"])
+def test_thermostat_chunks_match_stepwise_calls(dmgx, workloads, ensemble):
+    """The device-resident MD chunk (fused half steps, device thermo and
+    thermostat factor) against the same steps made of separate C-ABI calls
+    (compute, velocity Verlet, wrap, thermo, scale) with the factor formed on
+    the host from the library's own Bussi draws: same trajectory to FP64
+    rounding (the FP64 operations are the same, in the same order)."""
+    import ctypes
+    import math
+    import torch
+    from ctypes import c_void_p as P
+    s, path = _system(workloads, "c0")
+    N = len(s["types"])
+    steps, dt_fs, t_target, tau, seed = 12, 1.0, 500.0, 10.0, 777
+    md = dmgx.MD(dmgx.NEP(path), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    md.run(steps, dt_fs, ensemble, t_init=t_target, t_coupling=tau, seed=seed)
+    got = md.arrays()
+
+    L = dmgx.lib()
+    nep = dmgx.NEP(path)
+    b = s["box"].to_c()
+    dx, dv = to_dev(s["pos"], torch.float64), to_dev(s["vel"], torch.float64)
+    dm, dt_ = to_dev(s["mass"], torch.float64), to_dev(s["types"], torch.int32)
+    pe = torch.zeros(N, dtype=torch.float64, device="cuda")
+    f = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+    vir = torch.zeros(9 * N, dtype=torch.float64, device="cuda")
+    rng = ctypes.c_void_p()
+    assert L.dmgx_rng_create(seed, ctypes.byref(rng)) == 0
+    dt = dt_fs / 10.18051
+    ck = dmgx._lib.check
+
+    def force():
+        ck(L.dmgx_wrap_positions(ctypes.byref(b), N, P(dx.data_ptr()), None))
+        nep.compute(s["box"], dt_, dx, pe, f, vir)
+
+    force()
+    th = np.zeros(8)
+    for _ in range(steps):
+        ck(L.dmgx_velocity_verlet(1, N, dt, P(dm.data_ptr()), P(dx.data_ptr()), P(dv.data_ptr()),
+                                  P(f.data_ptr()), None))
+        force()
+        ck(L.dmgx_velocity_verlet(0, N, dt, P(dm.data_ptr()), P(dx.data_ptr()), P(dv.data_ptr()),
+                                  P(f.data_ptr()), None))
+        ck(L.dmgx_thermo(ctypes.byref(b), N, P(dm.data_ptr()), P(pe.data_ptr()), P(dv.data_ptr()),
+                         P(vir.data_ptr()), th.ctypes.data_as(dmgx._lib.PD), None))
+        T = th[0]
+        if ensemble == "nvt_ber":
+            fac = math.sqrt(1.0 + (1.0 / tau) * (t_target / T - 1.0))
+        else:
+            rr = L.dmgx_gasdev(rng)
+            sm = L.dmgx_resamplekin_sumnoises(3 * N - 1, rng)
+            ndeg = 3 * N
+            kk = T * (ndeg * 8.617343e-5 * 0.5)
+            sigma = ndeg * 8.617343e-5 * t_target * 0.5
+            c = math.exp(-1.0 / tau)
+            knew = (kk + (1.0 - c) * (sigma * (sm + rr * rr) / ndeg - kk)
+                    + 2.0 * rr * math.sqrt(kk * sigma / ndeg * (1.0 - c) * c))
+            fac = math.sqrt(knew / kk)
+        ck(L.dmgx_scale_velocity(N, fac, P(dv.data_ptr()), None))
+    L.dmgx_rng_destroy(rng)
+    torch.cuda.synchronize()
+    x_ref = dx.cpu().numpy().reshape(3, N)
+    v_ref = dv.cpu().numpy().reshape(3, N)
+    assert np.abs(got["position"] - x_ref).max() <= 1e-12
+    assert np.abs(got["velocity"] - v_ref).max() <= 1e-12
+    if ensemble == "nvt_ber":  # the weak-coupling thermostat moved T towards the target
+        t0 = float((s["mass"] * (s["vel"].reshape(3, N) ** 2).sum(0)).sum() / (3 * N * 8.617343e-5))
+        assert abs(th[0] - t_target) < abs(t0 - t_target)
+
+
+def _small_cells(workloads):
+    """Tiny periodic cells (thinner than 2.5 (rc + skin)): the 8-atom cubic
+    diamond cell and the 2-atom triclinic primitive cell, both jittered."""
+    rng = np.random.default_rng(11)
+    a = 3.567
+    frac = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
+                     [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
+    cubic = (frac * a).T + rng.normal(0, 0.05, (3, 8))
+    h_cubic = np.diag([a, a, a])
+    h_prim = 0.5 * a * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]]).T  # columns a, b, c
+    prim = (h_prim @ np.array([[0, 0, 0], [.25, .25, .25]]).T) + rng.normal(0, 0.05, (3, 2))
+    return [(h_cubic, cubic), (h_prim, prim)]
+
+
+def test_small_boxes_against_oracle(dmgx, workloads):
+    """Boxes thinner than 2.5 (rc + skin) go through periodic replicas; the
+    oracle sums the periodic images directly."""
+    nep = dmgx.NEP(C0)
+    orc = OracleNEP(C0, round_params_to_float=True)
+    for h, pos in _small_cells(workloads):
+        n = pos.shape[1]
+        t = np.zeros(n, np.int32)
+        box = dmgx.Box(h, (1, 1, 1))
+        out = gpu_compute(dmgx, nep, box, t, pos)
+        ref = orc.compute(h, (1, 1, 1), t, pos)
+        check(errors(out, ref), tol_e=1e-5 * np.abs(ref["pe"]).mean())
+        assert abs(out[1].sum(1)).max() < 1e-5
+
+
+def test_small_box_nve_matches_oracle(dmgx, workloads):
+    """A few NVE steps of the 8-atom cell through MD (per-step path)."""
+    import oracle as O
+    h, pos = _small_cells(workloads)[0]
+    n = pos.shape[1]
+    t = np.zeros(n, np.int32)
+    mass = np.full(n, 12.011)
+    vel = np.random.default_rng(3).normal(0, 0.01, (3, n))
+    vel -= vel.mean(1, keepdims=True)
+    box = dmgx.Box(h, (1, 1, 1))
+    md = dmgx.MD(dmgx.NEP(C0), box, t, mass, pos, vel)
+    steps, dt = 5, 1.0
+    md.run(steps, dt, "nve")
+    got = md.arrays()
+    orc = OracleNEP(C0, round_params_to_float=True)
+    x, v = pos.copy(), vel.copy()
+    O.wrap_positions(h, (1, 1, 1), x)
+    o = orc.compute(h, (1, 1, 1), t, x)
+    dtn = dt / 10.18051
+    for _ in range(steps):
+        O.velocity_verlet(True, dtn, mass, x, v, o["force"])
+        O.wrap_positions(h, (1, 1, 1), x)
+        o = orc.compute(h, (1, 1, 1), t, x)
+        O.velocity_verlet(False, dtn, mass, x, v, o["force"])
+    assert np.abs(got["position"] - x).max() < 1e-7
+    assert np.abs(got["velocity"] - v).max() < 1e-7
+
+
+@pytest.mark.parametrize("which", ["c0", "c1"])
+def test_ann_modes(dmgx, workloads, which):
+    """The FP32 ANN on the matrix cores (default) against the FP32 ANN on the
+    vector ALUs (same products in FP32, different summation order), and the
+    BF16-operand ANN against both the FP32 result and the oracle at its own
+    (bfloat16) tolerance."""
+    s, path = _system(workloads, which)
+    nep = dmgx.NEP(path)
+    mf = gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"])
+    va = gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"], ann_valu=True)
+    bf = gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"], ann_bf16=True)
+    d = errors(mf, dict(pe=va[0], force=va[1], virial=va[2]))
+    assert d["force"] < 1e-5 and d["energy_atom_max"] < 1e-5 and d["virial"] < 1e-5, d
+    ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
+    e = errors(bf, ref)
+    print(which, "bf16 vs oracle", e)
+    fmax = np.abs(ref["force"]).max()
+    assert e["force"] < 2e-2 * max(fmax, 1.0), e
+    assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
