@@ -49,6 +49,17 @@ class Nhc(ctypes.Structure):
                 ("mass", ctypes.c_double * 4)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_atoms", ctypes.c_longlong), ("max_requests", ctypes.c_int), ("flags", ctypes.c_uint),
+                ("margin", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_longlong) for n in ("submitted", "completed", "frames", "atom_frames")] + [
+        (n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "elapsed_s", "frames_per_s")] + [
+        (n, ctypes.c_longlong) for n in ("iterations", "relayouts")]
+
+
 ACCUMULATE = 1
 FORCE_REBUILD = 2
 ANN_BF16 = 4
@@ -96,6 +107,13 @@ _SIGNATURES = {
     "dmgx_md_run": (I, [VP, I, D, ctypes.POINTER(Ensemble), U, PD]),
     "dmgx_md_arrays": (I, [VP] + [ctypes.POINTER(VP)] * 7),
     "dmgx_md_box": (I, [VP, ctypes.POINTER(Box)]),
+    "dmgx_server_create": (I, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
+    "dmgx_server_destroy": (None, [VP]),
+    "dmgx_server_submit": (I, [VP, ctypes.POINTER(Box), I, PI, PD, PD, PD, I, D,
+                               ctypes.POINTER(ctypes.c_longlong)]),
+    "dmgx_server_poll": (I, [VP, ctypes.c_longlong, I, PD, PD, PD, PD, PD, PI]),
+    "dmgx_server_drain": (I, [VP]),
+    "dmgx_server_get_stats": (I, [VP, ctypes.POINTER(ServerStats)]),
 }
 
 

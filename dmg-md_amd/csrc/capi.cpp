@@ -13,6 +13,7 @@
 
 #include "engine.hpp"
 #include "ensemble.hpp"
+#include "serve.hpp"
 
 struct dmgx_nep {
   dmgx::NepEngine engine;
@@ -23,6 +24,11 @@ struct dmgx_md {
   dmgx::MdState state;
   template <class... A>
   explicit dmgx_md(A&&... a) : state(std::forward<A>(a)...) {}
+};
+
+struct dmgx_server {
+  dmgx::Server server;
+  dmgx_server(dmgx::NepEngine* nep, const dmgx::ServeConfig& cfg) : server(nep, cfg) {}
 };
 
 struct dmgx_rng {
@@ -390,6 +396,72 @@ int dmgx_md_box(dmgx_md* md, dmgx_box* out)
     if (!md || !out) throw std::invalid_argument("null argument");
     std::memcpy(out->h, md->state.box().h, sizeof(out->h));
     std::memcpy(out->pbc, md->state.box().pbc, sizeof(out->pbc));
+  });
+}
+
+int dmgx_server_create(dmgx_nep* nep, const dmgx_server_config* cfg, dmgx_server** out)
+{
+  return guarded([&] {
+    if (!nep || !cfg || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    dmgx::ServeConfig c;
+    c.max_atoms = cfg->max_atoms;
+    c.max_requests = cfg->max_requests;
+    c.flags = cfg->flags;
+    c.margin = cfg->margin;
+    if (!(c.margin >= 0.0)) throw std::invalid_argument("negative shell margin");
+    *out = new dmgx_server(&nep->engine, c);
+  });
+}
+
+void dmgx_server_destroy(dmgx_server* server) { delete server; }
+
+int dmgx_server_submit(dmgx_server* server, const dmgx_box* box, int N, const int* type_host,
+                       const double* mass_host, const double* position_host, const double* velocity_host, int steps,
+                       double dt_fs, long long* id_out)
+{
+  return guarded([&] {
+    if (!server || !box || !id_out) throw std::invalid_argument("null argument");
+    *id_out = server->server.submit(to_box(box), N, type_host, mass_host, position_host, velocity_host, steps, dt_fs);
+  });
+}
+
+int dmgx_server_poll(dmgx_server* server, long long id, int wait, double* potential_host, double* force_host,
+                     double* position_host, double* velocity_host, double* latency_ms, int* done)
+{
+  return guarded([&] {
+    if (!server || !done) throw std::invalid_argument("null argument");
+    *done = server->server.poll(id, wait != 0, potential_host, force_host, position_host, velocity_host, latency_ms)
+                ? 1
+                : 0;
+  });
+}
+
+int dmgx_server_drain(dmgx_server* server)
+{
+  return guarded([&] {
+    if (!server) throw std::invalid_argument("null handle");
+    server->server.drain();
+  });
+}
+
+int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out)
+{
+  return guarded([&] {
+    if (!server || !out) throw std::invalid_argument("null argument");
+    const dmgx::ServeStats s = server->server.stats();
+    out->submitted = s.submitted;
+    out->completed = s.completed;
+    out->frames = s.frames;
+    out->atom_frames = s.atom_frames;
+    out->p50_ms = s.p50_ms;
+    out->p99_ms = s.p99_ms;
+    out->mean_ms = s.mean_ms;
+    out->max_ms = s.max_ms;
+    out->elapsed_s = s.elapsed_s;
+    out->frames_per_s = s.frames_per_s;
+    out->iterations = s.iterations;
+    out->relayouts = s.relayouts;
   });
 }
 

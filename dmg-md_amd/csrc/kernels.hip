@@ -1557,6 +1557,133 @@ hipError_t launch_fold(int N, long long n2, bool accumulate, const double* pe2, 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// serving loop: masked integrator, ghost refresh, relayout and extraction
+// ---------------------------------------------------------------------------
+
+// velocity-Verlet half step of the owned atoms whose request takes an MD
+// step this iteration (the FP64 expressions of k_velocity_verlet)
+__global__ void k_serve_kick(int first_half, int n, int Nt, const int* __restrict__ atom_req,
+                             const int* __restrict__ req_md, const double* __restrict__ req_dt,
+                             const double* __restrict__ mass, double* __restrict__ pos, double* __restrict__ vel,
+                             const double* __restrict__ force)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = atom_req[i];
+  if (!req_md[r]) return;
+  const double dt = req_dt[r];
+  const double half = dt * 0.5;
+  const double im = 1.0 / mass[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double v = vel[c * Nt + i] + force[c * Nt + i] * im * half;
+    vel[c * Nt + i] = v;
+    if (first_half) pos[c * Nt + i] += v * dt;
+  }
+}
+
+// ghost k (slot n_owned + k) = its owner + a box shift
+__global__ void k_serve_ghosts(int n_owned, int n_ghost, int Nt, const int* __restrict__ owner,
+                               const double* __restrict__ shift, double* __restrict__ pos)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_ghost) return;
+  const int g = n_owned + k, o = owner[k];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pos[c * Nt + g] = pos[c * Nt + o] + shift[3 * k + c];
+}
+
+// owned atoms of continuing requests move to their new slots (and regions)
+__global__ void k_serve_remap(int n, int Nt_new, int Nt_old, const int* __restrict__ src,
+                              const double* __restrict__ dtr, const double* __restrict__ pos_o,
+                              const double* __restrict__ vel_o, const double* __restrict__ f_o,
+                              double* __restrict__ pos_n, double* __restrict__ vel_n, double* __restrict__ f_n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = src[i];
+  if (s < 0) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    pos_n[c * Nt_new + i] = pos_o[c * Nt_old + s] + dtr[3 * i + c];
+    vel_n[c * Nt_new + i] = vel_o[c * Nt_old + s];
+    f_n[c * Nt_new + i] = f_o[c * Nt_old + s];
+  }
+}
+
+// newly admitted atoms: position and velocity from the staging upload
+__global__ void k_serve_fresh(int nf, int Nt, const int* __restrict__ dst, const double* __restrict__ fresh,
+                              double* __restrict__ pos, double* __restrict__ vel, double* __restrict__ force)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  const int i = dst[k];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    pos[c * Nt + i] = fresh[6 * k + c];
+    vel[c * Nt + i] = fresh[6 * k + 3 + c];
+    force[c * Nt + i] = 0.0;
+  }
+}
+
+// finished atoms -> packed (position - region shift, velocity, force, potential)
+__global__ void k_serve_extract(int no, int Nt, const int* __restrict__ slot, const double* __restrict__ shift,
+                                const double* __restrict__ pos, const double* __restrict__ vel,
+                                const double* __restrict__ force, const double* __restrict__ pe,
+                                double* __restrict__ out)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= no) return;
+  const int i = slot[k];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    out[10 * k + c] = pos[c * Nt + i] - shift[3 * k + c];
+    out[10 * k + 3 + c] = vel[c * Nt + i];
+    out[10 * k + 6 + c] = force[c * Nt + i];
+  }
+  out[10 * k + 9] = pe[i];
+}
+
+hipError_t launch_serve_kick(int first_half, int n, int Nt, const int* atom_req, const int* req_md,
+                             const double* req_dt, const double* mass, double* pos, double* vel, const double* force,
+                             hipStream_t s)
+{
+  if (n > 0) k_serve_kick<<<nblk(n), kBlock, 0, s>>>(first_half, n, Nt, atom_req, req_md, req_dt, mass, pos, vel, force);
+  return hipGetLastError();
+}
+
+hipError_t launch_serve_ghosts(int n_owned, int n_ghost, int Nt, const int* owner, const double* shift, double* pos,
+                               hipStream_t s)
+{
+  if (n_ghost > 0) k_serve_ghosts<<<nblk(n_ghost), kBlock, 0, s>>>(n_owned, n_ghost, Nt, owner, shift, pos);
+  return hipGetLastError();
+}
+
+hipError_t launch_serve_remap(int n, int Nt_new, int Nt_old, const int* src, const double* dtr, const double* pos_o,
+                              const double* vel_o, const double* f_o, double* pos_n, double* vel_n, double* f_n,
+                              hipStream_t s)
+{
+  if (n > 0)
+    k_serve_remap<<<nblk(n), kBlock, 0, s>>>(n, Nt_new, Nt_old, src, dtr, pos_o, vel_o, f_o, pos_n, vel_n, f_n);
+  return hipGetLastError();
+}
+
+hipError_t launch_serve_fresh(int nf, int Nt, const int* dst, const double* fresh, double* pos, double* vel,
+                              double* force, hipStream_t s)
+{
+  if (nf > 0) k_serve_fresh<<<nblk(nf), kBlock, 0, s>>>(nf, Nt, dst, fresh, pos, vel, force);
+  return hipGetLastError();
+}
+
+hipError_t launch_serve_extract(int no, int Nt, const int* slot, const double* shift, const double* pos,
+                                const double* vel, const double* force, const double* pe, double* out, hipStream_t s)
+{
+  if (no > 0) k_serve_extract<<<nblk(no), kBlock, 0, s>>>(no, Nt, slot, shift, pos, vel, force, pe, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_check_scaled(int n, double factor, const double* data, int* flag, hipStream_t s)
 {
   if (n > 0) k_check_scaled<<<nblk(n), kBlock, 0, s>>>(n, factor, data, flag);

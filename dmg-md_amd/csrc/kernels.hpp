@@ -149,6 +149,24 @@ hipError_t launch_replicate(int N, const int reps[3], const DevBox& b, const dou
 hipError_t launch_fold(int N, long long n2, bool accumulate, const double* pe2, const double* f2, const double* v2,
                        double* pe, double* force, double* virial, hipStream_t s);
 
+// Serving loop (arrays of the batch have stride Nt = owned + ghost slots):
+// masked half kick of owned atoms whose request takes an MD step; ghost
+// positions = owner + shift; remap of continuing owned atoms to their new
+// slots (+ translation change); scatter of newly admitted atoms; packing of
+// finished atoms' (position - shift, velocity, force, potential).
+hipError_t launch_serve_kick(int first_half, int n, int Nt, const int* atom_req, const int* req_md,
+                             const double* req_dt, const double* mass, double* pos, double* vel, const double* force,
+                             hipStream_t s);
+hipError_t launch_serve_ghosts(int n_owned, int n_ghost, int Nt, const int* owner, const double* shift, double* pos,
+                               hipStream_t s);
+hipError_t launch_serve_remap(int n, int Nt_new, int Nt_old, const int* src, const double* dtr, const double* pos_o,
+                              const double* vel_o, const double* f_o, double* pos_n, double* vel_n, double* f_n,
+                              hipStream_t s);
+hipError_t launch_serve_fresh(int nf, int Nt, const int* dst, const double* fresh, double* pos, double* vel,
+                              double* force, hipStream_t s);
+hipError_t launch_serve_extract(int no, int Nt, const int* slot, const double* shift, const double* pos,
+                                const double* vel, const double* force, const double* pe, double* out, hipStream_t s);
+
 // Number of thermo block partials launch_thermo uses for N atoms.
 int thermo_blocks(int N);
 

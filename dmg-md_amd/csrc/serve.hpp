@@ -1,0 +1,133 @@
+// serve.hpp -- the serving loop: many independent periodic systems (requests)
+// evaluated together on one GPU with continuous batching.
+//
+// No reference counterpart: the reference runs one system per process.  A
+// request is a periodic box with N atoms and either a single-point force
+// evaluation (steps = 0) or `steps` NVE velocity-Verlet steps.  At submit
+// the request gets a shell of periodic images of its atoms: images within
+// rc + skin + margin of the box become descriptor-only centres, images out to
+// 2 rc + skin + margin are neighbour coordinates only.  Active requests are
+// placed in disjoint, translated regions of one non-periodic batch box (gaps
+// wider than the Verlet radius), laid out as
+//     [owned atoms of every request | descriptor ghosts | coordinate ghosts]
+// so a single force evaluation of the batch (force centres = owned atoms,
+// dependency centres = owned + descriptor ghosts, the domain contract of
+// NEP::compute_domain) gives every request exactly its periodic result.
+// Ghost positions are refreshed from their owners (+ box shift) every step.
+//
+// A worker thread owns the GPU work: each iteration admits waiting requests
+// (FIFO) while the batch stays within the HBM budget (atoms incl. ghosts),
+// re-lays the batch out when its membership changed, advances every active
+// request by one frame (its first force evaluation, or one MD step), and
+// retires the finished ones, recording submit-to-result latency.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace dmgx {
+
+struct ServeConfig {
+  long long max_atoms = 4000000;  // owned + ghost atoms in flight
+  int max_requests = 4096;
+  unsigned flags = 0;             // force flags (ANN mode)
+  double margin = 1.0;            // extra shell thickness (A) for motion during a request
+};
+
+struct ServeStats {
+  long long submitted = 0, completed = 0, frames = 0, atom_frames = 0;
+  double p50_ms = 0, p99_ms = 0, mean_ms = 0, max_ms = 0, elapsed_s = 0, frames_per_s = 0;
+  long long iterations = 0, relayouts = 0;
+};
+
+// The Server owns the NepEngine's GPU work while it lives: the engine must not
+// be used by other threads meanwhile.  Requests overlapping on the GPU are the
+// point; the results of each are those of its periodic system alone.
+class Server {
+ public:
+  Server(NepEngine* nep, const ServeConfig& cfg);
+  ~Server();
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  // host arrays are copied; position/velocity SoA [3*N]; velocity may be null
+  long long submit(const Box& box, int N, const int* type, const double* mass, const double* pos,
+                   const double* vel, int steps, double dt_fs);
+  // true when done; then copies results (each may be null): potential[N],
+  // force[3N], position[3N] (wrapped into the request's box), velocity[3N]
+  bool poll(long long id, bool wait, double* potential, double* force, double* position, double* velocity,
+            double* latency_ms);
+  ServeStats stats();
+  // blocks until every submitted request is done
+  void drain();
+
+ private:
+  struct Request;
+  void worker();
+  bool admit();
+  void relayout();
+  void iterate();
+  void retire();
+
+  NepEngine* nep_;
+  ServeConfig cfg_;
+  hipStream_t stream_ = nullptr;
+  std::thread thread_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  bool stop_ = false;
+  std::string error_;  // first worker failure, reported by poll/drain/stats
+  long long next_id_ = 1;
+  std::deque<std::shared_ptr<Request>> waiting_;
+  std::map<long long, std::shared_ptr<Request>> all_;
+  std::vector<std::shared_ptr<Request>> active_;  // worker only
+  bool dirty_ = true;                               // membership changed since the last layout
+  double t_first_ = -1, t_last_ = 0;
+  std::vector<double> latencies_;
+  ServeStats st_;
+
+  // device buffer with its capacity (elements)
+  template <class T>
+  struct DBuf {
+    T* p = nullptr;
+    long long cap = 0;
+    void need(long long n, const char* what);
+    void release();
+  };
+
+  // batch layout (worker only)
+  int n_owned_ = 0, n_dep_ = 0, n_ghost_ = 0, n_total_ = 0;
+  Box batch_box_{};
+  DBuf<double> pos_, vel_, force_, pe_, vir_, mass_;
+  DBuf<double> pos2_, vel2_, force2_;  // relayout targets (swapped in)
+  DBuf<int> type_;
+  DBuf<int> atom_req_;     // owned slot -> active request index
+  DBuf<int> req_md_;       // active request -> 1 when it takes an MD step this iteration
+  DBuf<double> req_dt_;    // active request -> dt (natural units)
+  DBuf<int> g_owner_;      // ghost k -> owned slot
+  DBuf<double> g_shift_;   // ghost k -> box shift [3]
+  DBuf<int> remap_src_;    // new owned slot -> old slot (continuing) or -1
+  DBuf<double> remap_dt_;  // new owned slot -> translation change [3]
+  DBuf<int> fresh_dst_;    // newly admitted atom -> owned slot
+  DBuf<double> fresh_;     // newly admitted atoms: position [3] + velocity [3]
+  DBuf<int> out_slot_;     // finished atoms -> owned slot
+  DBuf<double> out_shift_; // finished atoms -> translation to remove [3]
+  DBuf<double> out_;       // finished atoms: position, velocity, force, potential (10 doubles)
+  char* h_stage_ = nullptr;  // pinned staging for uploads and results
+  size_t h_stage_bytes_ = 0;
+  void stage(size_t bytes);
+  bool forced_rebuild_ = true;
+};
+
+}  // namespace dmgx

@@ -246,6 +246,71 @@ class MD:
         return Box(np.array(b.h[:]).reshape(3, 3), tuple(b.pbc[:]))
 
 
+class Server:
+    """Continuous-batching server of independent periodic systems (no
+    reference counterpart): each request is a single point (steps=0) or an
+    NVE run of `steps` steps; requests share the GPU in one batch and each
+    gets the result of its own periodic system.  The NEP handle is owned by
+    the server while it lives."""
+
+    def __init__(self, nep: NEP, max_atoms: int = 4_000_000, max_requests: int = 4096, ann_bf16=False,
+                 ann_valu=False, margin: float = 1.0):
+        cfg = _lib.ServerConfig(int(max_atoms), int(max_requests),
+                                (_lib.ANN_BF16 if ann_bf16 else 0) | (_lib.ANN_VALU if ann_valu else 0),
+                                float(margin))
+        h = ctypes.c_void_p()
+        check(lib().dmgx_server_create(nep.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.nep = nep
+        self._n = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.dmgx_server_destroy(h)
+            self._h = None
+
+    def submit(self, box: Box, types, masses, positions, velocities=None, steps: int = 0,
+               dt_fs: float = 1.0) -> int:
+        t = np.ascontiguousarray(types, np.int32)
+        N = t.shape[0]
+        m = np.ascontiguousarray(masses, np.float64)
+        x = np.ascontiguousarray(positions, np.float64).reshape(3, N)
+        v = None if velocities is None else np.ascontiguousarray(velocities, np.float64).reshape(3, N)
+        b = box.to_c()
+        rid = ctypes.c_longlong()
+        check(lib().dmgx_server_submit(self._h, ctypes.byref(b), N, t.ctypes.data_as(_lib.PI),
+                                       m.ctypes.data_as(_lib.PD), x.ctypes.data_as(_lib.PD),
+                                       None if v is None else v.ctypes.data_as(_lib.PD), int(steps),
+                                       float(dt_fs), ctypes.byref(rid)))
+        self._n[rid.value] = N
+        return rid.value
+
+    def poll(self, rid: int, wait: bool = False):
+        """None while pending; else potential[N], force[3,N], position[3,N],
+        velocity[3,N] and latency_ms."""
+        N = self._n[rid]
+        pe, f, x, v = np.zeros(N), np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N)
+        lat = ctypes.c_double()
+        done = ctypes.c_int()
+        check(lib().dmgx_server_poll(self._h, ctypes.c_longlong(rid), 1 if wait else 0,
+                                     pe.ctypes.data_as(_lib.PD), f.ctypes.data_as(_lib.PD),
+                                     x.ctypes.data_as(_lib.PD), v.ctypes.data_as(_lib.PD), ctypes.byref(lat),
+                                     ctypes.byref(done)))
+        if not done.value:
+            return None
+        return {"potential": pe, "force": f.reshape(3, N), "position": x.reshape(3, N),
+                "velocity": v.reshape(3, N), "latency_ms": lat.value}
+
+    def drain(self) -> None:
+        check(lib().dmgx_server_drain(self._h))
+
+    def stats(self) -> dict:
+        s = _lib.ServerStats()
+        check(lib().dmgx_server_get_stats(self._h, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in s._fields_}
+
+
 _hip = None
 
 

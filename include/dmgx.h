@@ -224,6 +224,52 @@ int dmgx_md_arrays(dmgx_md* md, double** position, double** velocity, double** f
                    double** potential, double** virial, double** mass, int** type);
 int dmgx_md_box(dmgx_md* md, dmgx_box* box_out);
 
+/* ---------------------------------------------------------------------- */
+/* Serving loop: continuous batching of independent periodic systems.      */
+/* No reference counterpart (the reference runs one system per process).    */
+/* ---------------------------------------------------------------------- */
+
+typedef struct dmgx_server dmgx_server;
+
+typedef struct dmgx_server_config {
+  long long max_atoms;   /* atoms in flight, periodic-image shells included */
+  int max_requests;      /* requests in flight */
+  unsigned flags;        /* DMGX_ANN_BF16 / DMGX_ANN_VALU */
+  double margin;         /* extra shell thickness (A) for motion during a request */
+} dmgx_server_config;
+
+typedef struct dmgx_server_stats {
+  long long submitted, completed;
+  long long frames;      /* single points + MD steps served */
+  long long atom_frames; /* the same, weighted by the request's atom count */
+  double p50_ms, p99_ms, mean_ms, max_ms; /* submit-to-result latency */
+  double elapsed_s;      /* first submit to last completion */
+  double frames_per_s;
+  long long iterations, relayouts;
+} dmgx_server_stats;
+
+/* Starts a server (one worker thread, one stream) on a NEP handle; the
+ * handle must not be used elsewhere while the server lives. */
+int dmgx_server_create(dmgx_nep* nep, const dmgx_server_config* cfg, dmgx_server** out);
+/* Stops the worker (pending requests are dropped) and frees the server. */
+void dmgx_server_destroy(dmgx_server* server);
+/* Submits a periodic system (host arrays, copied): steps = 0 is a single
+ * point, steps > 0 that many NVE velocity-Verlet steps of dt_fs. velocity may
+ * be NULL (zero). */
+int dmgx_server_submit(dmgx_server* server, const dmgx_box* box, int N, const int* type_host,
+                       const double* mass_host, const double* position_host,
+                       const double* velocity_host, int steps, double dt_fs, long long* id_out);
+/* *done = 1 once the request finished; then its results are copied to the
+ * host arrays given (each may be NULL): potential[N], force[3N],
+ * position[3N] (wrapped into its box), velocity[3N], latency in ms.  wait != 0
+ * blocks until it finishes. */
+int dmgx_server_poll(dmgx_server* server, long long id, int wait, double* potential_host,
+                     double* force_host, double* position_host, double* velocity_host,
+                     double* latency_ms, int* done);
+/* Blocks until every submitted request has finished. */
+int dmgx_server_drain(dmgx_server* server);
+int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* dmgx_last_error(void);
 

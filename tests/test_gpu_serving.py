@@ -1,0 +1,95 @@
+"""The serving loop: every request's result is its own periodic system's
+result, alone or in the batch (single points against the FP64 oracle, an NVE
+request against the standalone MD path), through the C ABI."""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from conftest import POTENTIALS
+from gpu_helpers import errors
+from oracle import OracleNEP
+
+pytestmark = pytest.mark.gpu
+
+C0 = os.path.join(POTENTIALS, "nep_c0_carbon.txt")
+TOL_F, TOL_EA, TOL_V = 1e-4, 2e-5, 2e-4
+
+
+def _diamond(reps, jitter, seed):
+    a = 3.567
+    frac = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
+                     [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
+    cells = np.array([[i, j, k] for i in range(reps) for j in range(reps) for k in range(reps)])
+    pos = ((cells[:, None, :] + frac[None, :, :]).reshape(-1, 3) * a).T
+    rng = np.random.default_rng(seed)
+    pos = pos + rng.normal(0, jitter, pos.shape)
+    return np.diag([a * reps] * 3), pos
+
+
+def _check_single(out, h, pos, orc):
+    n = pos.shape[1]
+    ref = orc.compute(h, (1, 1, 1), np.zeros(n, np.int32), pos)
+    # the server returns no virial; compare energy and force
+    err = errors((out["potential"], out["force"], np.zeros((9, n))),
+                 dict(pe=ref["pe"], force=ref["force"], virial=np.zeros((9, n))))
+    assert err["force"] <= TOL_F, err
+    assert err["energy_atom_max"] <= TOL_EA, err
+
+
+def test_server_requests_match_standalone(dmgx, workloads):
+    orc = OracleNEP(C0, round_params_to_float=True)
+    srv = dmgx.Server(dmgx.NEP(C0), max_atoms=2_000_000)
+    # a 64-atom NVE request runs first; a large single point joins mid-run
+    h64, p64 = _diamond(2, 0.05, 1)
+    v64 = np.random.default_rng(2).normal(0, 0.01, p64.shape)
+    v64 -= v64.mean(1, keepdims=True)
+    m64 = np.full(64, 12.011)
+    box64 = dmgx.Box(h64, (1, 1, 1))
+    steps = 100
+    r_md = srv.submit(box64, np.zeros(64, np.int32), m64, p64, v64, steps=steps, dt_fs=1.0)
+    time.sleep(0.005)
+    s = workloads.carbon_diamond_4096(seed=1)
+    r_big = srv.submit(s["box"], s["types"], s["mass"], s["pos"])
+    h8, p8 = _diamond(1, 0.05, 3)
+    r_8 = srv.submit(dmgx.Box(h8, (1, 1, 1)), np.zeros(8, np.int32), np.full(8, 12.011), p8)
+    h216, p216 = _diamond(3, 0.05, 4)
+    r_216 = srv.submit(dmgx.Box(h216, (1, 1, 1)), np.zeros(216, np.int32), np.full(216, 12.011), p216)
+    srv.drain()
+    st = srv.stats()
+    assert st["completed"] == 4 and st["frames"] == steps + 3, st
+    _check_single(srv.poll(r_big), s["box"].h, s["pos"], orc)
+    _check_single(srv.poll(r_8), h8, p8, orc)
+    _check_single(srv.poll(r_216), h216, p216, orc)
+    # the NVE request against the standalone MD of the same system
+    got = srv.poll(r_md)
+    md = dmgx.MD(dmgx.NEP(C0), box64, np.zeros(64, np.int32), m64, p64, v64)
+    md.run(steps, 1.0, "nve")
+    ref = md.arrays()
+    L = h64[0, 0]
+    d = got["position"] - ref["position"]
+    d -= np.round(d / L) * L
+    assert np.abs(d).max() < 1e-6
+    assert np.abs(got["velocity"] - ref["velocity"]).max() < 1e-6
+    assert got["latency_ms"] > 0
+
+
+def test_server_many_small_requests(dmgx, workloads):
+    """Many concurrent single points of different sizes: all correct, and the
+    loop reports its frames and latency percentiles."""
+    orc = OracleNEP(C0, round_params_to_float=True)
+    srv = dmgx.Server(dmgx.NEP(C0), max_atoms=400_000)
+    cases = []
+    for k in range(24):
+        reps = 1 + k % 3
+        h, p = _diamond(reps, 0.05, 100 + k)
+        n = p.shape[1]
+        rid = srv.submit(dmgx.Box(h, (1, 1, 1)), np.zeros(n, np.int32), np.full(n, 12.011), p)
+        cases.append((rid, h, p))
+    srv.drain()
+    for rid, h, p in cases[::5]:
+        _check_single(srv.poll(rid), h, p, orc)
+    st = srv.stats()
+    assert st["completed"] == 24 and st["frames"] == 24
+    assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
