@@ -102,6 +102,101 @@ def cpu_baseline(budget_s=10.0):
                       f"{el:.1f} s on 1 host core, FP64"}
 
 
+C0_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_c0_carbon.txt")
+
+
+def _carbon_request(rng, n_min=64, n_max=20000):
+    """A carbon diamond supercell of log-uniform size in [n_min, n_max]
+    atoms (cubic n^3 cells), 30% vacancies, 0.05 A jitter."""
+    a = 3.567
+    frac = [[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
+            [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]]
+    import numpy as np
+    target = float(np.exp(rng.uniform(np.log(n_min / 0.7), np.log(n_max / 0.7))))
+    n = max(2, int(round((target / 8) ** (1 / 3))))
+    cells = np.array([[i, j, k] for i in range(n) for j in range(n) for k in range(n)])
+    pos = ((cells[:, None, :] + np.array(frac)[None]).reshape(-1, 3) * a)
+    keep = rng.random(len(pos)) >= 0.3
+    pos = pos[keep].T + rng.normal(0, 0.05, (3, int(keep.sum())))
+    N = pos.shape[1]
+    vel = rng.normal(0, 0.01, (3, N))
+    vel -= vel.mean(1, keepdims=True)
+    return np.diag([a * n] * 3), pos, vel
+
+
+def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
+    """Served frames/s and latency percentiles of the continuous-batching
+    server on a bounded sample of the serving workload (carbon requests of
+    64..20000 atoms with 30% vacancies, half single points and half 50-step
+    NVE, Poisson arrivals, BF16 ANN), beside a plain sequential FP32 loop
+    (one request at a time through NEP.compute / MD.run) on the first
+    requests of the same stream; plus the BF16 force error against FP32."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(requests):
+        h, p, v = _carbon_request(rng)
+        reqs.append((h, p, v, 50 if k % 2 else 0))
+    gaps = rng.exponential(1.0 / rate_hz, requests)
+    srv = dmgx.Server(dmgx.NEP(C0_MODEL), max_atoms=6_000_000, ann_bf16=True)
+    ids = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for (h, p, v, st), g in zip(reqs, gaps):
+        t_next += g
+        while time.perf_counter() < t_next:
+            time.sleep(min(1e-4, max(0.0, t_next - time.perf_counter())))
+        n = p.shape[1]
+        ids.append(srv.submit(dmgx.Box(h), np.zeros(n, np.int32), np.full(n, 12.011), p, v, steps=st, dt_fs=0.5))
+    srv.drain()
+    st = srv.stats()
+    # BF16 force error against FP32 on single points of the stream
+    err = 0.0
+    nep32 = dmgx.NEP(C0_MODEL)
+    for k in range(0, min(requests, 16), 2):
+        h, p, v, _ = reqs[k]
+        n = p.shape[1]
+        got = srv.poll(ids[k])
+        t = torch.zeros(n, dtype=torch.int32, device="cuda")
+        x = torch.tensor(p.reshape(-1), dtype=torch.float64, device="cuda")
+        pe = torch.zeros(n, dtype=torch.float64, device="cuda")
+        f = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+        vir = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
+        nep32.compute(dmgx.Box(h), t, x, pe, f, vir)
+        torch.cuda.synchronize()
+        err = max(err, float(np.abs(got["force"] - f.cpu().numpy().reshape(3, n)).max()))
+    # the plain path: one request at a time, FP32
+    frames = 0
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for h, p, v, stp in reqs[:plain_requests]:
+        n = p.shape[1]
+        if stp == 0:
+            t = torch.zeros(n, dtype=torch.int32, device="cuda")
+            x = torch.tensor(p.reshape(-1), dtype=torch.float64, device="cuda")
+            pe = torch.zeros(n, dtype=torch.float64, device="cuda")
+            f = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+            vir = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
+            nep32.compute(dmgx.Box(h), t, x, pe, f, vir)
+            torch.cuda.synchronize()
+            frames += 1
+        else:
+            md = dmgx.MD(nep32, dmgx.Box(h), np.zeros(n, np.int32), np.full(n, 12.011), p, v)
+            md.run(stp, 0.5, "nve")
+            frames += stp
+    plain = frames / (time.perf_counter() - tp)
+    return {"workload": f"{requests} carbon requests, 64..20000 atoms (log-uniform, 30% vacancies), "
+                        f"half single points / half 50-step NVE at 0.5 fs, Poisson arrivals {rate_hz:.0f}/s, "
+                        "c0 NEP4 model, bf16 ANN",
+            "frames_per_s": st["frames_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+            "atom_frames_per_s": st["atom_frames"] / max(st["elapsed_s"], 1e-9),
+            "requests": st["completed"], "relayouts": st["relayouts"], "iterations": st["iterations"],
+            "bf16_force_err_max_eV_per_A": err,
+            "plain_fp32_frames_per_s": plain, "plain_sample_requests": plain_requests,
+            "speedup_vs_plain": st["frames_per_s"] / plain if plain > 0 else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +204,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cells", type=int, default=40, help="fcc cells per edge (40 -> 256,000 atoms)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--ann", choices=["fp32", "valu", "bf16"], default="fp32",
                     help="ANN arithmetic: fp32 on the matrix cores (default), fp32 on the vector ALUs, "
@@ -168,6 +264,7 @@ def main():
     print(f"bench: {args.steps} steps in {elapsed:.3f} s, T={th[0]:.1f} K, rebuilds={stats['rebuilds']}, "
           f"per-kernel avg ms {kernel_ms}, mean rows {means}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    serving = None if args.no_serve else serve_bench(dmgx)
     out = {
         "metric": METRIC,
         "value": value,
@@ -205,6 +302,7 @@ def main():
             "avg_launch_ms": avg_fused,
         },
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out))
 

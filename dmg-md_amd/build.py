@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdmgx.so")
-SOURCES = ["kernels.hip", "neighbor.hip", "nep_model.cpp", "ensemble.cpp", "engine.cpp", "serve.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "neighbor.hip", "nep_model.cpp", "ensemble.cpp", "engine.cpp", "serve.cpp", "frontend.cpp", "capi.cpp"]
 ARCH = os.environ.get("DMGX_OFFLOAD_ARCH", "gfx950")
 
 
@@ -29,6 +29,15 @@ def command(out: str = LIB) -> list:
     ]
 
 
+CLI = os.path.join(HERE, "dmgx-run")
+
+
+def cli_command(out: str = CLI) -> list:
+    """The dmgx-run executable: a thin C++ caller of dmgx_run_directory."""
+    return ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "dmgx_run.cpp"),
+            "-L", HERE, "-ldmgx", "-Wl,-rpath,$ORIGIN", "-o", out]
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = command(tmp)
@@ -39,6 +48,14 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libdmgx.so")
     os.replace(tmp, LIB)
+    cmd = cli_command(CLI + ".tmp")
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building dmgx-run")
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

@@ -13,6 +13,7 @@
 
 #include "engine.hpp"
 #include "ensemble.hpp"
+#include "frontend.hpp"
 #include "serve.hpp"
 
 struct dmgx_nep {
@@ -462,6 +463,14 @@ int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out)
     out->frames_per_s = s.frames_per_s;
     out->iterations = s.iterations;
     out->relayouts = s.relayouts;
+  });
+}
+
+int dmgx_run_directory(const char* directory, const char* run_file, const char* model_file)
+{
+  return guarded([&] {
+    if (!directory) throw std::invalid_argument("null directory");
+    dmgx::run_directory(directory, run_file ? run_file : "run.in", model_file ? model_file : "model.xyz");
   });
 }
 

@@ -675,6 +675,7 @@ void MdState::wrap()
 
 void MdState::force(unsigned flags)
 {
+  ++force_calls_;
   wrap();
   nep_->compute(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
 }
@@ -718,7 +719,8 @@ void MdState::checkpoint(bool save)
   }
 }
 
-void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out)
+void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out, int every,
+                  const Output& on_output)
 {
   if (steps < 0) throw std::invalid_argument("negative step count");
   const double dt = dt_fs / kTimeUnit;
@@ -752,6 +754,10 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
           nep_->notify_box_changed();
         }
       }
+      if (on_output && ((every > 0 && (step + 1) % every == 0) || step + 1 == steps)) {
+        check(hipStreamSynchronize(stream_), "output");
+        on_output(step + 1, th);
+      }
     }
     if (steps == 0) thermo(th);
     if (thermo_out) std::memcpy(thermo_out, th, sizeof(th));
@@ -775,7 +781,8 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
   if (!factor_) dalloc(&factor_, 1, "thermostat factor");
   if (!hrand_) check(hipHostMalloc((void**)&hrand_, 2 * kChunk * sizeof(double), hipHostMallocDefault), "pinned");
   for (int c0 = 0; c0 < steps;) {
-    const int n = std::min(kChunk, steps - c0);
+    int n = std::min(kChunk, steps - c0);
+    if (on_output && every > 0) n = std::min(n, every - c0 % every);
     const Rng saved = ctl.rng();
     checkpoint(true);
     bool redo = false;
@@ -821,7 +828,16 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
       ctl.set_rng(saved);
     }
     if (redo) throw std::runtime_error("Verlet rows kept overflowing");
+    force_calls_ += n;
     c0 += n;
+    if (on_output && ((every > 0 && c0 % every == 0) || c0 == steps)) {
+      // the last step's thermo before its rescale (NVE: nothing rescales)
+      double tho[8];
+      if (kind == 0) enqueue_thermo();
+      check(hipMemcpyAsync(tho, tw_.thermo, sizeof(tho), hipMemcpyDeviceToHost, stream_), "thermo copy");
+      check(hipStreamSynchronize(stream_), "output");
+      on_output(c0, tho);
+    }
   }
   thermo(th);
   if (thermo_out) std::memcpy(thermo_out, th, sizeof(th));

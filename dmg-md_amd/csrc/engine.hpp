@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -138,7 +139,14 @@ class MdState {
   ~MdState();
   MdState(const MdState&) = delete;
   MdState& operator=(const MdState&) = delete;
-  void run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out);
+  // Every `every` steps (every > 0) and after the last step, `on_output`
+  // gets the number of steps done and that step's thermo as computed before
+  // the thermostat rescale (the state itself is after it), the point where
+  // the reference writes its thermo rows and dumps (runtime.cu:790-840).
+  using Output = std::function<void(int steps_done, const double* thermo)>;
+  void run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out, int every = 0,
+           const Output& on_output = Output());
+  long long force_calls() const { return force_calls_; }
   void arrays(double** pos, double** vel, double** force, double** pe, double** vir, double** mass, int** type);
   const Box& box() const { return box_; }
 
@@ -161,6 +169,7 @@ class MdState {
   double* rand_ = nullptr;    // per-step thermostat draws of a chunk (device)
   double* hrand_ = nullptr;   // pinned staging of the draws
   double* factor_ = nullptr;  // device velocity factor
+  long long force_calls_ = 0;
 };
 
 double box_volume(const double* h);

@@ -270,6 +270,18 @@ int dmgx_server_poll(dmgx_server* server, long long id, int wait, double* potent
 int dmgx_server_drain(dmgx_server* server);
 int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out);
 
+/* ---------------------------------------------------------------------- */
+/* Input front end (dmgmd_main.cpp + run_m1_replicated, runtime.cu)         */
+/* ---------------------------------------------------------------------- */
+
+/* Runs a run.in / model.xyz input directory: commands potential, velocity,
+ * time_step, ensemble (nve, nvt_ber, nvt_bdp, nvt_nhc, npt_ber, npt_scr),
+ * dump_thermo, dump_xyz, dump_restart, run (run_parser.cpp:158-315); writes
+ * thermo.out, the dump_xyz files, restart.xyz and neighbor.out into the
+ * directory in the reference's formats (runtime.cu:1148-1313).  run_file and
+ * model_file default to "run.in" and "model.xyz" when NULL. */
+int dmgx_run_directory(const char* directory, const char* run_file, const char* model_file);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* dmgx_last_error(void);
 

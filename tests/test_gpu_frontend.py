@@ -1,0 +1,134 @@
+"""End-to-end replay of the reference's golden cases through the run.in /
+model.xyz front end: every output file the reference wrote (thermo.out,
+dump_xyz files, restart.xyz, neighbor.out) is reproduced and compared with
+the reference's GPUMD outputs at the reference manifest's own tolerances
+(tests/baseline/manifest.json); a multi-stage case resumes from the
+restart.xyz this build wrote in its previous stage."""
+import json
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from conftest import BASELINE
+
+pytestmark = pytest.mark.gpu
+
+MANIFEST = json.load(open(os.path.join(BASELINE, "manifest.json")))
+TOL = MANIFEST["tolerances"]
+
+
+def close(a, b, tol, what):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    bad = np.abs(a - b) > tol["atol"] + tol["rtol"] * np.abs(b)
+    assert not bad.any(), (what, float(np.abs(a - b).max()), a[bad][:3], b[bad][:3])
+
+
+def thermo_rows(path):
+    """Rows per segment (a '# dump_thermo' header starts a segment)."""
+    segs = []
+    for line in open(path):
+        if line.startswith("# dump_thermo"):
+            segs.append([])
+        elif not line.startswith("#") and line.strip():
+            segs[-1].append([float(x) for x in line.split()])
+    return [np.array(s) for s in segs]
+
+
+def kv(line, key):
+    i = line.find(key + "=")
+    rest = line[i + len(key) + 1:]
+    if rest.startswith('"'):
+        return rest[1:rest.index('"', 1)]
+    return rest.split()[0]
+
+
+def xyz_frames(path):
+    lines = open(path).read().splitlines()
+    frames, k = [], 0
+    while k < len(lines):
+        n = int(lines[k])
+        head = lines[k + 1]
+        props = kv(head, "Properties").split(":")
+        atoms = [l.split() for l in lines[k + 2:k + 2 + n]]
+        cols, off = {}, 1
+        for j in range(0, len(props), 3):
+            name, w = props[j].lower(), int(props[j + 2])
+            if name != "species":
+                cols[name] = np.array([[float(a[off + c]) for c in range(w)] for a in atoms])
+                off += w
+        frames.append({"head": head, "props": kv(head, "Properties"), "species": [a[0] for a in atoms],
+                       "cols": cols})
+        k += 2 + n
+    return frames
+
+
+def compare_xyz(got_path, ref_path, restart=False):
+    g, r = xyz_frames(got_path), xyz_frames(ref_path)
+    assert len(g) == len(r)
+    for fg, fr in zip(g, r):
+        assert fg["props"] == fr["props"] and fg["species"] == fr["species"]
+        close([float(x) for x in kv(fg["head"], "Lattice").split()],
+              [float(x) for x in kv(fr["head"], "Lattice").split()], TOL["lattice"], "lattice")
+        assert kv(fg["head"], "pbc") == kv(fr["head"], "pbc")
+        if restart:
+            close(fg["cols"]["pos"], fr["cols"]["pos"], TOL["restart_position"], "restart pos")
+            close(fg["cols"]["vel"], fr["cols"]["vel"], TOL["restart_velocity"], "restart vel")
+            close(fg["cols"]["mass"], fr["cols"]["mass"], TOL["restart_mass"], "restart mass")
+            continue
+        close(float(kv(fg["head"], "Time")), float(kv(fr["head"], "Time")), TOL["time"], "time")
+        close(float(kv(fg["head"], "energy")), float(kv(fr["head"], "energy")), TOL["energy"], "energy")
+        for key, tol in (("virial", TOL["virial"]), ("stress", TOL["xyz_stress"])):
+            close([float(x) for x in kv(fg["head"], key).split()], [float(x) for x in kv(fr["head"], key).split()],
+                  tol, key)
+        for name, tol in (("pos", TOL["position"]), ("vel", TOL["velocity"]), ("forces", TOL["force"]),
+                          ("energy_atom", TOL["energy"]), ("virial", TOL["virial"])):
+            if name in fr["cols"]:
+                close(fg["cols"][name], fr["cols"][name], tol, name)
+
+
+def compare_thermo(got_path, ref_path):
+    g, r = thermo_rows(got_path), thermo_rows(ref_path)
+    assert [len(s) for s in g] == [len(s) for s in r]
+    for sg, sr in zip(g, r):
+        close(sg[:, 0], sr[:, 0], TOL["temperature"], "T")
+        close(sg[:, 1:3], sr[:, 1:3], TOL["energy"], "KE PE")
+        close(sg[:, 3:9], sr[:, 3:9], TOL["thermo_stress"], "stress")
+        close(sg[:, 9:], sr[:, 9:], TOL["lattice"], "box")
+
+
+@pytest.mark.parametrize("case", sorted(MANIFEST["cases"]))
+def test_golden_case_replays_end_to_end(dmgx, tmp_path, case):
+    import importlib
+    cli = importlib.import_module("dmg-md_amd.cli")
+    c = MANIFEST["cases"][case]
+    prev = None
+    for stage in c["stages"]:
+        d = tmp_path / stage["name"]
+        d.mkdir()
+        run_src = os.path.join(BASELINE, stage["run"])
+        shutil.copy(run_src, d / os.path.basename(run_src))
+        pot_name = [l.split()[1] for l in open(run_src) if l.split()[:1] == ["potential"]][0]
+        shutil.copy(os.path.join(BASELINE, c["potential"]), d / pot_name)
+        if "model" in stage:
+            shutil.copy(os.path.join(BASELINE, stage["model"]), d / "model.xyz")
+        else:
+            src_stage, src_file = stage["model_from_stage"]
+            shutil.copy(tmp_path / src_stage / src_file, d / "model.xyz")
+        cli.run_directory(str(d), os.path.basename(run_src), "model.xyz")
+        gold = os.path.join(BASELINE, "goldens", case, stage["name"])
+        for name, spec in stage["outputs"].items():
+            got, ref = str(d / name), os.path.join(gold, name)
+            assert os.path.exists(got), name
+            if spec["kind"] == "thermo":
+                compare_thermo(got, ref)
+            elif spec["kind"] == "xyz":
+                compare_xyz(got, ref)
+            elif spec["kind"] == "restart":
+                compare_xyz(got, ref, restart=True)
+            else:
+                assert open(got).read() == open(ref).read(), name
+        prev = d
+    assert prev is not None
