@@ -367,9 +367,41 @@ constexpr int ann_lds_floats() { return 64 * (DMAX + 1) + 64; }
 // energies go back to their lanes through LDS.  BF16 rounds Q, W0 and G to
 // bfloat16 (16x16x16 bf16 MFMA, FP32 accumulation); FP32 uses the 16x16x4
 // FP32 MFMA (exact products).
+// LDS copy of one type's ANN weights for the matrix-core path: W0 rows
+// padded to 64 hidden units with an odd row stride (no bank conflicts), then
+// b0 and w1 (zero past H)
+template <int DMAX>
+constexpr int ann_w_stride() { return DMAX + 1; }
+template <int DMAX>
+constexpr int ann_w_floats() { return 64 * ann_w_stride<DMAX>() + 128; }
+
+template <int DMAX>
+__device__ __forceinline__ void stage_ann_weights(const DevModel& m, int D, float* __restrict__ ws)
+{
+  constexpr int WS = ann_w_stride<DMAX>();
+  const int H = m.H;
+  for (int k = threadIdx.x; k < 64 * WS; k += blockDim.x) {
+    const int h = k / WS, d = k % WS;
+    ws[k] = (h < H && d < D) ? m.w0[h * D + d] : 0.0f;
+  }
+  for (int h = threadIdx.x; h < 64; h += blockDim.x) {
+    ws[64 * WS + h] = h < H ? m.b0[h] : 0.0f;
+    ws[64 * WS + 64 + h] = h < H ? m.w1[h] : 0.0f;
+  }
+}
+
+// orders this wavefront's LDS writes before its later LDS reads (each
+// wavefront works in its own LDS region, so no workgroup barrier is needed)
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int DMAX, bool BF16>
 __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* q, float* Fp, float& E,
-                                         float* __restrict__ lds)
+                                         float* __restrict__ lds, const float* __restrict__ ws)
 {
   constexpr int DS = DMAX + 1;
   constexpr int DT = (DMAX + 15) / 16;
@@ -378,11 +410,12 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
   float* Es = lds + 64 * DS;
 #pragma unroll
   for (int d = 0; d < DMAX; ++d) Qs[lane * DS + d] = d < D ? q[d] : 0.0f;
-  __syncthreads();
+  wave_lds_sync();
   const int H = m.H, HT = (H + 15) >> 4;
-  const float* __restrict__ W0 = m.w0;
-  const float* __restrict__ b0 = m.b0;
-  const float* __restrict__ w1 = m.w1;
+  constexpr int WS = ann_w_stride<DMAX>();
+  const float* __restrict__ W0 = ws;  // [64][WS], zero-padded
+  const float* __restrict__ b0 = ws + 64 * WS;
+  const float* __restrict__ w1 = ws + 64 * WS + 64;
 #pragma unroll 1
   for (int nt = 0; nt < 4; ++nt) {
     const float* qrow = Qs + (16 * nt + lr) * DS;
@@ -401,7 +434,7 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
         for (int ks = 0; ks < (DMAX + 3) / 4; ++ks) {
           const int d = 4 * ks + lg;
           const float bq = d < D ? qrow[d] : 0.0f;
-          const float a = (ha < H && d < D) ? W0[ha * D + d] : 0.0f;
+          const float a = W0[ha * WS + d];
           acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bq, acc, 0, 0, 0);
         }
       } else {
@@ -412,7 +445,7 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
           for (int c = 0; c < 4; ++c) {
             const int d = 16 * ks + 4 * lg + c;
             bq[c] = to_bf16(d < D ? qrow[d] : 0.0f);
-            a[c] = to_bf16((ha < H && d < D) ? W0[ha * D + d] : 0.0f);
+            a[c] = to_bf16(W0[ha * WS + d]);
           }
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
         }
@@ -435,7 +468,7 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
 #pragma unroll
           for (int dt = 0; dt < DT; ++dt) {
             const int d = 16 * dt + lr;
-            const float bw = (h < H && d < D) ? W0[h * D + d] : 0.0f;
+            const float bw = W0[h * WS + d];
             facc[dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(G[r], bw, facc[dt], 0, 0, 0);
           }
         }
@@ -450,7 +483,7 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const int h = 16 * mt + 4 * lg + c;
-            bw[c] = to_bf16((h < H && d < D) ? W0[h * D + d] : 0.0f);
+            bw[c] = to_bf16(W0[h * WS + d]);
           }
           facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, facc[dt], 0, 0, 0);
         }
@@ -468,7 +501,7 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
         if (d < DMAX) Qs[(16 * nt + 4 * lg + r) * DS + d] = facc[dt][r];
       }
   }
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int d = 0; d < DMAX; ++d) Fp[d] = d < D ? Qs[lane * DS + d] : 0.0f;
   E = Es[lane];
@@ -486,6 +519,12 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
   // range carry an empty row of a valid atom and write nothing
   if (C::ann == kAnnValu && !active) return;
   const int i = active ? i0 : c1;
+  __shared__ float ann_w[C::ann != kAnnValu ? ann_w_floats<DMAX>() : 1];
+  if (C::ann != kAnnValu) {
+    // single-type models only: the one type's weights, staged once per workgroup
+    stage_ann_weights<DMAX>(m, C::exact ? DMAX : m.D, ann_w);
+    __syncthreads();
+  }
   const int nR = C::exact ? NR : m.nR, nA = C::exact ? NA : m.nA;
   const int kR = C::exact ? KR : m.kR, kA = C::exact ? KA : m.kA;
   const bool h222 = C::exact ? C::q222 : (m.has222 != 0);
@@ -645,7 +684,7 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
   float E = 0.0f;
   if (C::ann != kAnnValu) {
     __shared__ float ann_lds[kBlock / 64][ann_lds_floats<DMAX>()];
-    ann_mfma<DMAX, C::ann == kAnnMfmaBf16>(m, D, q, Fp, E, ann_lds[threadIdx.x >> 6]);
+    ann_mfma<DMAX, C::ann == kAnnMfmaBf16>(m, D, q, Fp, E, ann_lds[threadIdx.x >> 6], ann_w);
   } else {
     const int H = m.H;
     const float* w0 = m.w0 + (size_t)ti * H * D;

@@ -49,12 +49,16 @@ def test_golden_systems_against_oracle_and_gpumd(dmgx, case, stage, pot, fscale)
     # per-atom floor (the reference's own FP32 outputs sit at 5e-6 eV/atom from
     # FP64), so the energy bound here is the reference manifest's 1e-5 relative.
     check(errors(out, ref), fscale, tol_e=1e-5 * np.abs(ref["pe"]).mean())
-    # and against the reference engine's own FP32 outputs at its manifest tolerances
+    # and against the reference engine's own FP32 outputs at its manifest
+    # tolerances; the virial's absolute floor is raised from the manifest's
+    # 1e-6 eV (bitwise reruns of one binary) to 1e-5 eV, the FP32 floor of a
+    # second implementation: GPUMD's per-atom virials of single_small_static
+    # are 4.4e-6 eV from the FP64 oracle themselves
     pe, f, v = out
     assert np.all(np.abs(f.T - g["forces"]) <= 1e-6 + 1e-4 * np.abs(g["forces"]))
     assert np.all(np.abs(pe - g["energy_atom"]) <= 1e-8 + 1e-5 * np.abs(g["energy_atom"]))
     gv = g["virial"][:, XYZ_VIRIAL_TO_SOA]
-    assert np.all(np.abs(v.T - gv) <= 1e-6 + 1e-4 * np.abs(gv))
+    assert np.all(np.abs(v.T - gv) <= 1e-5 + 1e-4 * np.abs(gv))
 
 
 def _system(workloads, which):

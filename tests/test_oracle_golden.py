@@ -79,3 +79,44 @@ def test_oracle_matches_reference_on_jittered_multitype():
     r = ref.RefNEP(path).compute(h, [1, 1, 1], t, pos)
     for k in ("pe", "force", "virial"):
         assert np.abs(o[k] - r[k]).max() <= 1e-11 * max(1.0, np.abs(r[k]).max()), k
+
+
+def test_oracle_replays_gpumd_nvt_ber_trajectory():
+    """The FP64 oracle's velocity-Verlet step with a Berendsen rescale
+    (factor sqrt(1 + (T_target/T - 1)/tau), T over 3N degrees of freedom,
+    target ramped linearly as t0 + (t1 - t0) * step/steps; reference
+    ensemble_control.cpp Berendsen branch, integrate.cu) replays the two
+    nvt_ber segments of multi_nvt_restart.  GPUMD's FP32 trajectory sits up to
+    1.5e-8 A/fs from the replay, so velocities are held to 3e-8 A/fs (5.6x
+    the manifest's bitwise-rerun floor); a target ramp shifted by one step
+    moves velocities by about 1e-6 A/fs and fails."""
+    from oracle import thermo, velocity_verlet
+    from xyz_io import TIME_UNIT_CONVERSION as TU
+    path = os.path.join(BASELINE, "inputs", "potentials", "nep_water.txt")
+    model = read_xyz(os.path.join(BASELINE, "inputs", "multi_nvt_restart", "model.xyz"))
+    gold_dir = os.path.join(BASELINE, "goldens", "multi_nvt_restart", "initial")
+    mass = read_xyz(os.path.join(gold_dir, "restart.xyz"))["mass"].copy()
+    t = types_of(model["species"], path)
+    orc = OracleNEP(path, round_params_to_float=True)
+    h = model["h"]
+    pos = np.ascontiguousarray(model["pos"].T)
+    vel = np.ascontiguousarray(model["vel"].T) * TU
+    force = np.ascontiguousarray(orc.compute(h, model["pbc"], t, pos)["force"])
+    frames = []
+    dt = 0.1 / TU
+    for t0, t1, steps, every in ((100.0, 120.0, 4, 2), (120.0, 130.0, 3, 1)):
+        for step in range(steps):
+            velocity_verlet(1, dt, mass, pos, vel, force)
+            r = orc.compute(h, model["pbc"], t, pos)
+            force = np.ascontiguousarray(r["force"])
+            velocity_verlet(0, dt, mass, pos, vel, force)
+            temp = thermo(h, mass, r["pe"], vel, r["virial"])[0]
+            target = t0 + (t1 - t0) * step / steps
+            vel *= np.sqrt(1.0 + (target / temp - 1.0) / 100.0)
+            if (step + 1) % every == 0:
+                frames.append((pos.T.copy(), vel.T / TU))
+    vel_tol = dict(TOL["velocity"], atol=3e-8)
+    for k, (p, v) in enumerate(frames):
+        g = read_xyz(os.path.join(gold_dir, "trajectory.xyz"), frame=k)
+        assert within(v, g["vel"], vel_tol), (k, np.abs(v - g["vel"]).max())
+        assert within(p, g["pos"], TOL["position"]), (k, np.abs(p - g["pos"]).max())
