@@ -57,7 +57,8 @@ class ServerConfig(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_longlong) for n in ("submitted", "completed", "frames", "atom_frames")] + [
         (n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "elapsed_s", "frames_per_s")] + [
-        (n, ctypes.c_longlong) for n in ("iterations", "relayouts")]
+        (n, ctypes.c_longlong) for n in ("iterations", "relayouts", "max_slots")] + [
+        (n, ctypes.c_double) for n in ("relayout_s", "iterate_s", "retire_s")]
 
 
 ACCUMULATE = 1

@@ -463,6 +463,10 @@ int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out)
     out->frames_per_s = s.frames_per_s;
     out->iterations = s.iterations;
     out->relayouts = s.relayouts;
+    out->max_slots = s.max_slots;
+    out->relayout_s = s.relayout_s;
+    out->iterate_s = s.iterate_s;
+    out->retire_s = s.retire_s;
   });
 }
 

@@ -329,7 +329,7 @@ DevBox NepEngine::make_devbox(const Box& box) const
 
 NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
                                 const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
-                                unsigned flags)
+                                unsigned flags, const ImageMap* map)
 {
   Plan p;
   if (N < 0 || n1 < 0 || n2 < n1 || n2 > N || nd1 < 0 || nd2 < nd1 || nd2 > N)
@@ -339,6 +339,10 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
   const bool by_gid = (flags & 8u) != 0;
   if (by_gid && !gid) throw std::invalid_argument("DMGX_SORT_BY_GID needs global_id");
   if (N == 0 || nd2 == nd1) return p;
+  const ImageMap im = map ? *map : ImageMap{};
+  if (im.owner && (!im.image || !im.gstart)) throw std::invalid_argument("incomplete image map");
+  if (im.owner && (n1 != 0 || nd1 != 0 || gid))
+    throw std::invalid_argument("an image map needs the owned slots first and rows ordered by slot");
   p.box = make_devbox(box);
   for (int d = 0; d < 3; ++d)
     if (p.box.pbc[d] && p.box.thick[d] <= 2.5 * (model_.rcR_max + kSkin))
@@ -350,8 +354,11 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
   ensure_cells((int)ncell);
   p.forced = !have_list_ || box_changed_ || N != last_N_ || nd1 != last_nd1_ || nd2 != last_nd2_ ||
              by_gid != last_gid_sort_ || (flags & 2u) || std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
-             std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0;
+             std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0 || im.owner != last_map_.owner ||
+             im.image != last_map_.image || im.gstart != last_map_.gstart;
   p.empty = false;
+  w_.map = im;
+  last_map_ = im;
   have_list_ = true;
   box_changed_ = false;
   last_N_ = N;
@@ -440,10 +447,11 @@ void NepEngine::enqueue_small(const Box& box, const int reps[3], int N, const in
 
 void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
                         const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
-                        unsigned flags, hipStream_t s)
+                        unsigned flags, hipStream_t s, const ImageMap* map)
 {
   int reps[3];
   if (N > 0 && nd2 > nd1 && small_box(box, reps)) {
+    if (map && map->owner) throw std::invalid_argument("an image map needs a box without thin periodic directions");
     if (n1 != 0 || nd1 != 0 || n2 != N || nd2 != N)
       throw std::invalid_argument("a periodic box thinner than 2.5 (rc + skin) needs all atoms as centres");
     if (!type || !pos || !pe || !force || !virial) throw std::invalid_argument("null array");
@@ -451,7 +459,7 @@ void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2,
     enqueue_small(box, reps, N, type, pos, gid, pe, force, virial, flags, s);
     return;
   }
-  const Plan p = plan(box, N, n1, n2, nd1, nd2, type, pos, gid, pe, force, virial, flags);
+  const Plan p = plan(box, N, n1, n2, nd1, nd2, type, pos, gid, pe, force, virial, flags, map);
   if (p.empty) return;
   // rebuild decision on the device: forced by the host, or any atom moved > skin/2
   prof_.begin(kProfCheck, s);
@@ -492,10 +500,10 @@ bool NepEngine::settle(hipStream_t s)
 
 void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
                         const double* pos, const uint64_t* gid, double* pe, double* force, double* virial,
-                        unsigned flags, hipStream_t s)
+                        unsigned flags, hipStream_t s, const ImageMap* map)
 {
   for (int attempt = 0; attempt < 4; ++attempt) {
-    enqueue(box, N, n1, n2, nd1, nd2, type, pos, gid, pe, force, virial, flags, s);
+    enqueue(box, N, n1, n2, nd1, nd2, type, pos, gid, pe, force, virial, flags, s, map);
     if (!settle(s)) return;
     if (flags & 1u)
       throw std::runtime_error(

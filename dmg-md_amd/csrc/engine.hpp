@@ -69,16 +69,18 @@ class NepEngine {
   const NepModel& model() const { return model_; }
   void reserve(int capacity);
   // Throws std::invalid_argument / std::runtime_error; see dmgx_nep_compute.
+  // `map` (optional) marks the slots past the owned ones as periodic images
+  // of owned slots (ImageMap); the serving loop's batches use it.
   void compute(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type, const double* pos,
                const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
-               hipStream_t stream);
+               hipStream_t stream, const ImageMap* map = nullptr);
   // Enqueues one evaluation without any host synchronisation (the Verlet
   // rebuild decision is taken on the device).  settle() then synchronises,
   // raises on invalid input, and returns true when a Verlet row overflowed:
   // the workspace has been grown and the enqueued results must be redone.
   void enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type, const double* pos,
                const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
-               hipStream_t stream);
+               hipStream_t stream, const ImageMap* map = nullptr);
   bool settle(hipStream_t stream);
   // enqueue() in two halves for callers that pack the step's positions
   // themselves (the MD loop fuses the pack into its integrator kernel):
@@ -90,7 +92,8 @@ class NepEngine {
     bool empty = true;
   };
   Plan plan(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type, const double* pos,
-            const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags);
+            const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
+            const ImageMap* map = nullptr);
   void launch_force(const Plan& p, int N, int n1, int n2, int nd1, int nd2, const int* type, const uint64_t* gid,
                     double* pe, double* force, double* virial, unsigned flags, hipStream_t stream);
   DevWork& work() { return w_; }
@@ -123,6 +126,7 @@ class NepEngine {
   int last_N_ = -1, last_nd1_ = -1, last_nd2_ = -1;
   double last_h_[9] = {};
   int last_pbc_[3] = {};
+  ImageMap last_map_;
   int last_N_stats_ = 0, last_c1_ = 0, last_c2_ = 0;
   Profiler prof_;
   // replicated small-box system (atoms x replicas)

@@ -223,7 +223,7 @@ constexpr int kFilterBatch = DMGX_FILTER_BATCH;
 constexpr int kGatherLanes = DMGX_GATHER_LANES;
 constexpr int kGatherBatch = DMGX_GATHER_BATCH;  // radial edges per lane per batch (kR <= 9)
 
-template <bool ONE>
+template <bool ONE, bool MAP>
 __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1, int c2, int nblocks, DevWork w)
 {
   constexpr int G = kFilterLanes, PER = kBlock / G, RB = kFilterBatch;
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
   const unsigned below = (1u << lg) - 1u;
   int na = 0, nro = 0;
   for (int p0 = 0; __any(p0 < nv); p0 += G * RB) {
-    int jj[RB], qq[RB];
+    int jj[RB], qq[RB], jo[RB];
     double4 pj[RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
@@ -276,8 +276,13 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
         qq[r] = w.vrev[vrow + p];
       }
     }
+    // with an image map an entry stands for its owner in the edge record
+    // and the reverse index; its own slot only gives the position
 #pragma unroll
-    for (int r = 0; r < RB; ++r) pj[r] = jj[r] >= 0 ? w.xq[jj[r]] : make_double4(0, 0, 0, 0);
+    for (int r = 0; r < RB; ++r) {
+      pj[r] = jj[r] >= 0 ? w.xq[jj[r]] : make_double4(0, 0, 0, 0);
+      jo[r] = (MAP && jj[r] >= 0) ? w.map.owner[jj[r]] : jj[r];
+    }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int p = p0 + r * G + lg;
@@ -296,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
         if (d2 < cr) {
           isA = d2 < ca;
           isR = !isA;
-          rec = make_float4((float)x, (float)y, (float)z, __int_as_float(jj[r]));
+          rec = make_float4((float)x, (float)y, (float)z, __int_as_float(jo[r]));
         }
       }
       const unsigned gA = (unsigned)((__ballot(isA) >> (gw * G)) & gmask);
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
       if (jj[r] >= 0) w.vsa[vrow + p] = (isA && a < w.MA) ? a : -1;
       if (isA && a < w.MA) {
         w.edge[erow + a] = rec;
-        w.eq[arow + a] = qq[r] >= 0 ? jj[r] * w.MV + qq[r] : -1;
+        w.eq[arow + a] = qq[r] >= 0 ? jo[r] * w.MV + qq[r] : -1;
       }
       if (isR) {
         const int slot = w.MR - 1 - (nro + __popc(gR & below));
@@ -1381,8 +1386,13 @@ hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, Dev
   if (c2 > c1) {
     constexpr int per = kBlock / kFilterLanes;
     const int nb = (c2 - c1 + per - 1) / per;
-    if (m.T == 1) k_filter<true><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
-    else k_filter<false><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+    if (w.map.owner) {
+      if (m.T == 1) k_filter<true, true><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+      else k_filter<false, true><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+    } else {
+      if (m.T == 1) k_filter<true, false><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+      else k_filter<false, false><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+    }
   }
   return hipGetLastError();
 }

@@ -43,6 +43,30 @@ struct DevModel {
   const float* zblp;  // [T][T][12] a_inv, zizj, r_in, r_out, (a_k, b_k) x4
 };
 
+// Periodic-image map of a batch whose ghost slots are images of owned slots
+// (the serving loop's periodic requests).  owner[j] is the owned slot that j
+// is an image of (owner[j] == j for an owned slot), image[j] the lattice shift
+// (a, b, c) of that image as image_code(a, b, c), and the ghosts of owned slot
+// i occupy slots [gstart[i], gstart[i+1]).  With a map, a row entry j stands
+// for its owner everywhere except for its position: the owner's descriptor,
+// radial adjoint and angular partials are the image's, so ghosts need no rows
+// and no descriptors of their own.
+struct ImageMap {
+  const int* owner = nullptr;   // [N]
+  const int* image = nullptr;   // [N]
+  const int* gstart = nullptr;  // [owned + 1], absolute slots
+};
+constexpr int kImageMax = 15;  // |a|, |b|, |c| <= kImageMax
+__host__ __device__ inline int image_code(int a, int b, int c)
+{
+  return (a + 16) | ((b + 16) << 5) | ((c + 16) << 10);
+}
+// the code of the opposite shift (-a, -b, -c)
+__host__ __device__ inline int image_neg(int code)
+{
+  return (32 - (code & 31)) | ((32 - ((code >> 5) & 31)) << 5) | ((32 - ((code >> 10) & 31)) << 10);
+}
+
 // Per-atom / per-edge workspaces (device pointers), sized for `cap` slots.
 // Row arrays are row-major: entry p of centre i lives at [i*M + p].
 struct DevWork {
@@ -68,6 +92,7 @@ struct DevWork {
   int* vsa = nullptr;          // [cap*MV] Verlet position -> angular slot, -1 if not angular
   float* bR = nullptr;         // [cap*T*kRp] radial adjoint in basis space
   float4* f12 = nullptr;       // [cap*MA] dE_i/dr_ij of angular edges
+  ImageMap map;                // periodic-image map of the current call (empty: none)
   int* flags = nullptr;        // [kNumFlags] device status words
   double* partial = nullptr;   // thermo block partials
   double* thermo = nullptr;    // [8]

@@ -255,20 +255,38 @@ __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ 
   if (i >= c2) return;
   const int n = min(w.vn[i], w.MV);
   const int* ri = w.vl + (size_t)i * w.MV;
+  const ImageMap& im = w.map;
   for (int p = lg; p < n; p += kRowLanes) {
     const int j = ri[p];
+    // the reverse of i -> j is found in the row of j; with an image map, j
+    // stands for its owner o = owner[j] shifted by S, and the reverse entry
+    // is o -> (the image of i shifted by -S), one of i's own ghost slots
+    int o = j, t = i;
+    if (im.owner) {
+      o = im.owner[j];
+      const int c = im.image[j];
+      if (c != image_code(0, 0, 0)) {
+        const int want = image_neg(c);
+        t = -1;
+        for (int g = im.gstart[i]; g < im.gstart[i + 1]; ++g)
+          if (im.image[g] == want) {
+            t = g;
+            break;
+          }
+      }
+    }
     int q = -1;
-    if (j >= c1 && j < c2) {
-      const int* rj = w.vl + (size_t)j * w.MV;
-      int lo = 0, hi = min(w.vn[j], w.MV) - 1;
+    if (t >= 0 && o >= c1 && o < c2) {
+      const int* ro = w.vl + (size_t)o * w.MV;
+      int lo = 0, hi = min(w.vn[o], w.MV) - 1;
       while (lo <= hi) {
         const int mid = (lo + hi) >> 1;
-        const int v = rj[mid];
-        if (v == i) {
+        const int v = ro[mid];
+        if (v == t) {
           q = mid;
           break;
         }
-        if (key_less(gid, v, i)) lo = mid + 1;
+        if (key_less(gid, v, t)) lo = mid + 1;
         else hi = mid - 1;
       }
     }

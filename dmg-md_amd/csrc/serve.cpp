@@ -90,11 +90,10 @@ struct Server::Request {
   std::vector<double> mass, pos, vel;  // pos wrapped into the box, SoA
   int steps = 0;
   double dt = 0;
-  // periodic images: owner (local index) and Cartesian shift; the first
-  // n_dep are descriptor centres, the rest neighbour coordinates only
-  std::vector<int> g_owner;
+  // periodic images within rc + skin + margin of the box, grouped by owner:
+  // owner (local index), image code and Cartesian shift
+  std::vector<int> g_owner, g_image;
   std::vector<double> g_shift;
-  int n_dep = 0;
   double lo[3] = {0, 0, 0}, hi_ext[3] = {0, 0, 0};  // extent of owned + ghost positions
   // progress
   int state = 0;  // 0: needs its first force evaluation, 1: MD steps, 2: done
@@ -133,7 +132,8 @@ Server::~Server()
   for (DBuf<double>* b : {&pos_, &vel_, &force_, &pe_, &vir_, &mass_, &pos2_, &vel2_, &force2_, &req_dt_, &g_shift_,
                           &remap_dt_, &fresh_, &out_shift_, &out_})
     b->release();
-  for (DBuf<int>* b : {&type_, &atom_req_, &req_md_, &g_owner_, &remap_src_, &fresh_dst_, &out_slot_}) b->release();
+  for (DBuf<int>* b : {&type_, &atom_req_, &req_md_, &owner_, &image_, &gstart_, &remap_src_, &fresh_dst_, &out_slot_})
+    b->release();
   if (h_stage_) (void)hipHostFree(h_stage_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -164,11 +164,14 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
   if (vel) std::copy(vel, vel + 3 * (size_t)N, r->vel.begin());
   // wrap into the box and collect the periodic images of the shell
   const double rc = nep_->model().rcR_max;
-  const double d1 = rc + kSkin + cfg_.margin, d2 = 2.0 * rc + kSkin + 2.0 * cfg_.margin;
+  const double d1 = rc + kSkin + cfg_.margin;
   double th[3];
   thickness(box.h, th);
   int K[3];
-  for (int d = 0; d < 3; ++d) K[d] = box.pbc[d] ? (int)std::ceil(d2 / th[d]) : 0;
+  for (int d = 0; d < 3; ++d) {
+    K[d] = box.pbc[d] ? (int)std::ceil(d1 / th[d]) : 0;
+    if (K[d] > kImageMax) throw std::invalid_argument("request box too thin for its periodic image shell");
+  }
   std::vector<double> frac(3 * (size_t)N);
   for (int i = 0; i < N; ++i) {
     double x[3] = {pos[i], pos[N + i], pos[2 * N + i]}, s[3];
@@ -181,38 +184,30 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
     to_cart(box.h, s, x);
     for (int d = 0; d < 3; ++d) r->pos[(size_t)d * N + i] = x[d];
   }
-  std::vector<int> dep_o, crd_o;
-  std::vector<double> dep_s, crd_s;
-  for (int ia = -K[0]; ia <= K[0]; ++ia)
-    for (int ib = -K[1]; ib <= K[1]; ++ib)
-      for (int ic = -K[2]; ic <= K[2]; ++ic) {
-        if (ia == 0 && ib == 0 && ic == 0) continue;
-        const double sh[3] = {box.h[0] * ia + box.h[1] * ib + box.h[2] * ic, box.h[3] * ia + box.h[4] * ib + box.h[5] * ic,
-                              box.h[6] * ia + box.h[7] * ib + box.h[8] * ic};
-        const int img[3] = {ia, ib, ic};
-        for (int i = 0; i < N; ++i) {
-          // distance of the image from the box, per face pair (a lower bound
-          // of the Euclidean distance: the shell is a superset)
+  // every image within d1 of the box (per face pair: a lower bound of the
+  // Euclidean distance, so the shell is a superset of what the rows need);
+  // an owned atom's Verlet neighbours at any rebuild lie within
+  // rc + skin of the box plus the request's motion, covered by the margin
+  for (int i = 0; i < N; ++i)
+    for (int ia = -K[0]; ia <= K[0]; ++ia)
+      for (int ib = -K[1]; ib <= K[1]; ++ib)
+        for (int ic = -K[2]; ic <= K[2]; ++ic) {
+          if (ia == 0 && ib == 0 && ic == 0) continue;
+          const int img[3] = {ia, ib, ic};
           double far = 0.0;
           for (int d = 0; d < 3; ++d) {
             const double u = frac[3 * i + d] + img[d];
             const double e = std::max(0.0, std::max(-u, u - 1.0)) * th[d];
             far = std::max(far, e);
           }
-          if (far <= d1) {
-            dep_o.push_back(i);
-            dep_s.insert(dep_s.end(), sh, sh + 3);
-          } else if (far <= d2) {
-            crd_o.push_back(i);
-            crd_s.insert(crd_s.end(), sh, sh + 3);
-          }
+          if (far > d1) continue;
+          const double sh[3] = {box.h[0] * ia + box.h[1] * ib + box.h[2] * ic,
+                                box.h[3] * ia + box.h[4] * ib + box.h[5] * ic,
+                                box.h[6] * ia + box.h[7] * ib + box.h[8] * ic};
+          r->g_owner.push_back(i);
+          r->g_image.push_back(image_code(ia, ib, ic));
+          r->g_shift.insert(r->g_shift.end(), sh, sh + 3);
         }
-      }
-  r->n_dep = (int)dep_o.size();
-  r->g_owner = dep_o;
-  r->g_owner.insert(r->g_owner.end(), crd_o.begin(), crd_o.end());
-  r->g_shift = dep_s;
-  r->g_shift.insert(r->g_shift.end(), crd_s.begin(), crd_s.end());
   for (int d = 0; d < 3; ++d) {
     r->lo[d] = 1e300;
     r->hi_ext[d] = -1e300;
@@ -319,12 +314,20 @@ void Server::worker()
       admit();
     }
     try {
+      const double t0 = now_s();
       if (dirty_) relayout();
       bool any = false;
       for (auto& r : active_) any = any || r->live;
-      if (!any) continue;
-      iterate();
-      retire();
+      const double t1 = now_s();
+      if (any) iterate();
+      const double t2 = now_s();
+      if (any) retire();
+      const double t3 = now_s();
+      std::lock_guard<std::mutex> lk(mu_);
+      st_.relayout_s += t1 - t0;
+      st_.iterate_s += t2 - t1;
+      st_.retire_s += t3 - t2;
+      st_.max_slots = std::max<long long>(st_.max_slots, n_total_);
     } catch (const std::exception& e) {
       std::lock_guard<std::mutex> lk(mu_);
       error_ = e.what();
@@ -384,13 +387,12 @@ void Server::relayout()
   const int R = (int)active_.size();
   int side = 1;
   while ((long long)side * side * side < R) ++side;
-  long long n_owned = 0, n_dep = 0, n_crd = 0;
+  long long n_owned = 0, n_ghost = 0;
   for (auto& r : active_) {
     n_owned += r->n;
-    n_dep += r->n_dep;
-    n_crd += r->ghosts() - r->n_dep;
+    n_ghost += r->ghosts();
   }
-  const long long n_ghost = n_dep + n_crd, Nt = n_owned + n_ghost;
+  const long long Nt = n_owned + n_ghost;
   if (Nt >= (1LL << 30)) throw std::runtime_error("serving batch too large");
   const int old_Nt = n_total_;
   // device arrays; position, velocity and force are double-buffered across the remap
@@ -406,15 +408,18 @@ void Server::relayout()
   remap_dt_.need(3 * n_owned, "serve remap shift");
   req_md_.need(R, "serve request flags");
   req_dt_.need(R, "serve request dt");
-  g_owner_.need(n_ghost, "serve ghost owners");
+  owner_.need(Nt, "serve slot owners");
+  image_.need(Nt, "serve slot images");
+  gstart_.need(n_owned + 1, "serve ghost ranges");
   g_shift_.need(3 * n_ghost, "serve ghost shifts");
   // host side of the layout
-  std::vector<int> h_type((size_t)Nt), h_req((size_t)n_owned), h_src((size_t)n_owned), h_gown((size_t)n_ghost);
+  std::vector<int> h_type((size_t)Nt), h_req((size_t)n_owned), h_src((size_t)n_owned), h_own((size_t)Nt),
+      h_img((size_t)Nt, image_code(0, 0, 0)), h_gst((size_t)n_owned + 1);
   std::vector<double> h_mass((size_t)Nt), h_dtr(3 * (size_t)n_owned, 0.0), h_gsh(3 * (size_t)n_ghost);
   std::vector<int> h_fdst;
   std::vector<double> h_fresh;
   std::vector<double> h_dt((size_t)std::max(R, 1));
-  long long o = 0, gd = n_owned, gc = n_owned + n_dep;
+  long long o = 0, gk = n_owned;
   for (int q = 0; q < R; ++q) {
     Request& r = *active_[q];
     const int gx = q % side, gy = (q / side) % side, gz = q / (side * side);
@@ -424,6 +429,7 @@ void Server::relayout()
     h_dt[q] = r.dt;
     for (int k = 0; k < r.n; ++k) {
       const long long i = o + k;
+      h_own[i] = (int)i;
       h_type[i] = r.type[k];
       h_mass[i] = r.mass[k];
       h_req[i] = q;
@@ -437,27 +443,32 @@ void Server::relayout()
         for (int d = 0; d < 3; ++d) h_fresh.push_back(r.vel[(size_t)d * r.n + k]);
       }
     }
-    for (long long g = 0; g < r.ghosts(); ++g) {
-      const long long slotg = g < r.n_dep ? gd++ : gc++;
-      const long long k = slotg - n_owned;
-      const int own = (int)(o + r.g_owner[g]);
-      h_gown[k] = own;
-      for (int d = 0; d < 3; ++d) h_gsh[3 * k + d] = r.g_shift[3 * g + d];
-      h_type[slotg] = r.type[r.g_owner[g]];
-      h_mass[slotg] = r.mass[r.g_owner[g]];
+    // ghosts in owner order: owned slot o + a owns [h_gst[o + a], h_gst[o + a + 1])
+    long long g = 0;
+    for (int a = 0; a < r.n; ++a) {
+      h_gst[o + a] = (int)gk;
+      for (; g < r.ghosts() && r.g_owner[g] == a; ++g, ++gk) {
+        const long long k = gk - n_owned;
+        h_own[gk] = (int)(o + a);
+        h_img[gk] = r.g_image[g];
+        for (int d = 0; d < 3; ++d) h_gsh[3 * k + d] = r.g_shift[3 * g + d];
+        h_type[gk] = r.type[a];
+        h_mass[gk] = r.mass[a];
+      }
     }
     r.slot0 = (int)o;
     for (int d = 0; d < 3; ++d) r.trans[d] = tnew[d];
     r.placed = true;
     o += r.n;
   }
+  h_gst[n_owned] = (int)gk;
   const long long nf = (long long)h_fdst.size();
   fresh_dst_.need(nf, "serve fresh slots");
   fresh_.need(6 * nf, "serve fresh atoms");
   // uploads through one pinned staging buffer (the stream is idle here)
   const size_t bytes = h_type.size() * 4 + h_mass.size() * 8 + h_req.size() * 4 + h_src.size() * 4 +
-                       h_dtr.size() * 8 + h_gown.size() * 4 + h_gsh.size() * 8 + h_fdst.size() * 4 +
-                       h_fresh.size() * 8 + h_dt.size() * 8 + 64 * 16;
+                       h_dtr.size() * 8 + h_own.size() * 4 + h_img.size() * 4 + h_gst.size() * 4 +
+                       h_gsh.size() * 8 + h_fdst.size() * 4 + h_fresh.size() * 8 + h_dt.size() * 8 + 64 * 16;
   stage(bytes);
   char* hs = h_stage_;
   size_t off = 0;
@@ -473,7 +484,9 @@ void Server::relayout()
   up(atom_req_.p, h_req.data(), h_req.size() * 4);
   up(remap_src_.p, h_src.data(), h_src.size() * 4);
   up(remap_dt_.p, h_dtr.data(), h_dtr.size() * 8);
-  up(g_owner_.p, h_gown.data(), h_gown.size() * 4);
+  up(owner_.p, h_own.data(), h_own.size() * 4);
+  up(image_.p, h_img.data(), h_img.size() * 4);
+  up(gstart_.p, h_gst.data(), h_gst.size() * 4);
   up(g_shift_.p, h_gsh.data(), h_gsh.size() * 8);
   up(fresh_dst_.p, h_fdst.data(), h_fdst.size() * 4);
   up(fresh_.p, h_fresh.data(), h_fresh.size() * 8);
@@ -490,7 +503,6 @@ void Server::relayout()
   std::swap(vel_, vel2_);
   std::swap(force_, force2_);
   n_owned_ = (int)n_owned;
-  n_dep_ = (int)n_dep;
   n_ghost_ = (int)n_ghost;
   n_total_ = (int)Nt;
   std::memset(&batch_box_, 0, sizeof(batch_box_));
@@ -521,11 +533,16 @@ void Server::iterate()
     check(launch_serve_kick(1, n_owned_, n_total_, atom_req_.p, req_md_.p, req_dt_.p, mass_.p, pos_.p, vel_.p,
                             force_.p, stream_),
           "serve kick 1");
-  check(launch_serve_ghosts(n_owned_, n_ghost_, n_total_, g_owner_.p, g_shift_.p, pos_.p, stream_), "serve ghosts");
+  check(launch_serve_ghosts(n_owned_, n_ghost_, n_total_, owner_.p + n_owned_, g_shift_.p, pos_.p, stream_),
+        "serve ghosts");
   unsigned flags = cfg_.flags & ~1u;
   if (forced_rebuild_) flags |= 2u;
-  nep_->compute(batch_box_, n_total_, 0, n_owned_, 0, n_owned_ + n_dep_, type_.p, pos_.p, nullptr, pe_.p, force_.p,
-                vir_.p, flags, stream_);
+  ImageMap map;
+  map.owner = owner_.p;
+  map.image = image_.p;
+  map.gstart = gstart_.p;
+  nep_->compute(batch_box_, n_total_, 0, n_owned_, 0, n_owned_, type_.p, pos_.p, nullptr, pe_.p, force_.p,
+                vir_.p, flags, stream_, &map);
   forced_rebuild_ = false;
   if (any_md)
     check(launch_serve_kick(0, n_owned_, n_total_, atom_req_.p, req_md_.p, req_dt_.p, mass_.p, pos_.p, vel_.p,

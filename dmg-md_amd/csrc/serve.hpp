@@ -4,16 +4,17 @@
 // No reference counterpart: the reference runs one system per process.  A
 // request is a periodic box with N atoms and either a single-point force
 // evaluation (steps = 0) or `steps` NVE velocity-Verlet steps.  At submit
-// the request gets a shell of periodic images of its atoms: images within
-// rc + skin + margin of the box become descriptor-only centres, images out to
-// 2 rc + skin + margin are neighbour coordinates only.  Active requests are
-// placed in disjoint, translated regions of one non-periodic batch box (gaps
-// wider than the Verlet radius), laid out as
-//     [owned atoms of every request | descriptor ghosts | coordinate ghosts]
-// so a single force evaluation of the batch (force centres = owned atoms,
-// dependency centres = owned + descriptor ghosts, the domain contract of
-// NEP::compute_domain) gives every request exactly its periodic result.
-// Ghost positions are refreshed from their owners (+ box shift) every step.
+// the request gets a shell of the periodic images of its atoms that lie
+// within rc + skin + margin of its box.  Active requests are placed in
+// disjoint, translated regions of one non-periodic batch box (gaps wider than
+// the Verlet radius), laid out as
+//     [owned atoms of every request | their image ghosts, grouped by owner]
+// and the batch is evaluated with an ImageMap: owned atoms are the only
+// force and descriptor centres, and a ghost in a row stands for its owner
+// (the owner's descriptor, radial adjoint and angular partials are the
+// image's by periodicity), so each request gets exactly its periodic result
+// while ghosts cost positions only.  Ghost positions are refreshed from their
+// owners (+ box shift) every step.
 //
 // A worker thread owns the GPU work: each iteration admits waiting requests
 // (FIFO) while the batch stays within the HBM budget (atoms incl. ghosts),
@@ -49,6 +50,8 @@ struct ServeStats {
   long long submitted = 0, completed = 0, frames = 0, atom_frames = 0;
   double p50_ms = 0, p99_ms = 0, mean_ms = 0, max_ms = 0, elapsed_s = 0, frames_per_s = 0;
   long long iterations = 0, relayouts = 0;
+  long long max_slots = 0;
+  double relayout_s = 0, iterate_s = 0, retire_s = 0;
 };
 
 // The Server owns the NepEngine's GPU work while it lives: the engine must not
@@ -107,7 +110,7 @@ class Server {
   };
 
   // batch layout (worker only)
-  int n_owned_ = 0, n_dep_ = 0, n_ghost_ = 0, n_total_ = 0;
+  int n_owned_ = 0, n_ghost_ = 0, n_total_ = 0;
   Box batch_box_{};
   DBuf<double> pos_, vel_, force_, pe_, vir_, mass_;
   DBuf<double> pos2_, vel2_, force2_;  // relayout targets (swapped in)
@@ -115,7 +118,9 @@ class Server {
   DBuf<int> atom_req_;     // owned slot -> active request index
   DBuf<int> req_md_;       // active request -> 1 when it takes an MD step this iteration
   DBuf<double> req_dt_;    // active request -> dt (natural units)
-  DBuf<int> g_owner_;      // ghost k -> owned slot
+  DBuf<int> owner_;        // slot -> owned slot it is an image of (itself for owned slots)
+  DBuf<int> image_;        // slot -> image code (image_code(0, 0, 0) for owned slots)
+  DBuf<int> gstart_;       // owned slot -> first ghost slot of its images [n_owned + 1]
   DBuf<double> g_shift_;   // ghost k -> box shift [3]
   DBuf<int> remap_src_;    // new owned slot -> old slot (continuing) or -1
   DBuf<double> remap_dt_;  // new owned slot -> translation change [3]

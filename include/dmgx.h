@@ -246,6 +246,8 @@ typedef struct dmgx_server_stats {
   double elapsed_s;      /* first submit to last completion */
   double frames_per_s;
   long long iterations, relayouts;
+  long long max_slots;   /* largest batch: owned atoms + image ghosts */
+  double relayout_s, iterate_s, retire_s; /* worker wall time per phase */
 } dmgx_server_stats;
 
 /* Starts a server (one worker thread, one stream) on a NEP handle; the

@@ -93,3 +93,48 @@ def test_server_many_small_requests(dmgx, workloads):
     st = srv.stats()
     assert st["completed"] == 24 and st["frames"] == 24
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+
+
+def test_server_image_map_edge_cases(dmgx):
+    """Requests whose rows hold many images of the same atom, its own images
+    among them: the 2-atom triclinic primitive diamond cell (thickness
+    2.06 A, images out to 4 cells), a sheared 64-atom cell, a ZBL model with a
+    close pair, and a 2-type water cell thinner than the Verlet radius; all
+    in one batch, each against the FP64 oracle of its periodic system."""
+    from conftest import BASELINE
+    rng = np.random.default_rng(21)
+    a = 3.567
+    h_prim = 0.5 * a * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]]).T
+    p_prim = h_prim @ np.array([[0, 0, 0], [.25, .25, .25]]).T + rng.normal(0, 0.05, (3, 2))
+    h64, p64 = _diamond(2, 0.05, 22)
+    shear = np.array([[1, 0.21, -0.13], [0, 1, 0.17], [0, 0, 1]])
+    h_sh, p_sh = shear @ h64, shear @ p64
+    hz, pz = _diamond(2, 0.05, 23)
+    pz[:, 1] = pz[:, 0] + np.array([0.95, 0.4, -0.3])  # a 1.07 A pair inside the ZBL switch
+    water = os.path.join(BASELINE, "inputs", "potentials", "nep_water.txt")
+    wat = open(os.path.join(BASELINE, "inputs", "multi_nvt_restart", "model.xyz")).read().splitlines()[2:]
+    sym = open(water).readline().split()[2:]
+    tw = np.array([sym.index(l.split()[0]) for l in wat], np.int32)
+    pw = np.array([[float(x) for x in l.split()[1:4]] for l in wat]).T
+    hw = np.diag([8.0, 8.5, 9.0])
+    cases = [(C0, h_prim, np.zeros(2, np.int32), p_prim), (C0, h_sh, np.zeros(64, np.int32), p_sh),
+             (os.path.join(POTENTIALS, "nep_c4_carbon_zbl.txt"), hz, np.zeros(64, np.int32), pz),
+             (water, hw, tw, pw)]
+    by_model = {}
+    for path, h, t, p in cases:
+        by_model.setdefault(path, []).append((h, t, p))
+    for path, items in by_model.items():
+        orc = OracleNEP(path, round_params_to_float=True)
+        srv = dmgx.Server(dmgx.NEP(path), max_atoms=200_000)
+        ids = [srv.submit(dmgx.Box(h, (1, 1, 1)), t, np.full(len(t), 12.0), p) for h, t, p in items]
+        srv.drain()
+        for rid, (h, t, p) in zip(ids, items):
+            out = srv.poll(rid)
+            ref = orc.compute(h, (1, 1, 1), t, p)
+            n = len(t)
+            err = errors((out["potential"], out["force"], np.zeros((9, n))),
+                         dict(pe=ref["pe"], force=ref["force"], virial=np.zeros((9, n))))
+            # the ZBL pair carries 40 eV/A: FP32 rounding scales with the force
+            tol_f = max(TOL_F, 2e-6 * np.abs(ref["force"]).max())
+            assert err["force"] <= tol_f, (os.path.basename(path), n, err)
+            assert err["energy_atom_max"] <= TOL_EA, (os.path.basename(path), n, err)
