@@ -254,6 +254,7 @@ void NepEngine::free_all()
   dfree(&w_.f12);
   dfree(&w_.cell_count);
   dfree(&w_.cell_start);
+  dfree(&w_.cell_tiles);
   w_.cap = 0;
   w_.ncell_cap = 0;
   have_list_ = false;
@@ -300,6 +301,7 @@ void NepEngine::ensure_cells(int ncell)
   if (ncell <= w_.ncell_cap) return;
   dalloc(&w_.cell_count, (size_t)ncell, "cell counts");
   dalloc(&w_.cell_start, (size_t)ncell + 1, "cell starts");
+  dalloc(&w_.cell_tiles, 2 * ((size_t)ncell / 4096 + 2), "cell scan tiles");
   check(hipMemset(w_.cell_count, 0, sizeof(int) * ncell), "clear cell counts");
   w_.ncell_cap = ncell;
 }
@@ -350,8 +352,13 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
           "box thinner than 2.5 (rc + skin) in a periodic direction");
   const long long ncell = (long long)p.box.nc[0] * p.box.nc[1] * p.box.nc[2];
   if (ncell > (1LL << 28)) throw std::invalid_argument("cell grid too large");
-  if (N > w_.cap || w_.MV == 0) reserve(std::max(N, (int)(w_.cap * 1.25)));
-  ensure_cells((int)ncell);
+  // geometric growth (a serving batch grows request by request); rows stay
+  // addressable by int (cap * MV < 2^31)
+  if (N > w_.cap || w_.MV == 0) {
+    const long long lim = (long long)(INT32_MAX - 1) / std::max(w_.MV, 128);
+    reserve((int)std::max<long long>(N, std::min<long long>(2LL * w_.cap, lim)));
+  }
+  ensure_cells((int)std::max<long long>(ncell, std::min<long long>(2LL * w_.ncell_cap, 1LL << 28)));
   p.forced = !have_list_ || box_changed_ || N != last_N_ || nd1 != last_nd1_ || nd2 != last_nd2_ ||
              by_gid != last_gid_sort_ || (flags & 2u) || std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
              std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0 || im.owner != last_map_.owner ||

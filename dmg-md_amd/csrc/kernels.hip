@@ -1645,54 +1645,108 @@ __global__ void k_serve_ghosts(int n_owned, int n_ghost, int Nt, const int* __re
 }
 
 // owned atoms of continuing requests move to their new slots (and regions)
-__global__ void k_serve_remap(int n, int Nt_new, int Nt_old, const int* __restrict__ src,
-                              const double* __restrict__ dtr, const double* __restrict__ pos_o,
-                              const double* __restrict__ vel_o, const double* __restrict__ f_o,
-                              double* __restrict__ pos_n, double* __restrict__ vel_n, double* __restrict__ f_n)
+// Assembly of a new serving layout, one thread per slot: the slot's request
+// is found by binary search over the requests' first slots (both slot ranges
+// are in request order and tile [0, n_owned) and [n_owned, Nt)).
+__global__ void k_serve_assemble(int R, const ServeSlots* __restrict__ req, ServeLayout from, ServeLayout to,
+                                 int* __restrict__ atom_req)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int s = src[i];
-  if (s < 0) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Nt = to.Nt, no = to.n_owned;
+  if (s == 0) to.gstart[no] = Nt;
+  if (s >= Nt) return;
+  const bool owned = s < no;
+  int lo = 0, hi = R - 1;
+  while (lo < hi) {  // last request whose range starts at or before s
+    const int mid = (lo + hi + 1) >> 1;
+    if ((owned ? req[mid].new_o : req[mid].new_g) <= s) lo = mid;
+    else hi = mid - 1;
+  }
+  const ServeSlots q = req[lo];
+  if (owned) {
+    const int k = s - q.new_o;
+    atom_req[s] = lo;
+    to.owner[s] = s;
+    to.image[s] = image_code(0, 0, 0);
+    if (q.old_o >= 0) {
+      const int src = q.old_o + k;
+      to.type[s] = from.type[src];
+      to.mass[s] = from.mass[src];
+      to.gstart[s] = q.new_g + (from.gstart[src] - q.old_g);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    pos_n[c * Nt_new + i] = pos_o[c * Nt_old + s] + dtr[3 * i + c];
-    vel_n[c * Nt_new + i] = vel_o[c * Nt_old + s];
-    f_n[c * Nt_new + i] = f_o[c * Nt_old + s];
+      for (int c = 0; c < 3; ++c) {
+        to.pos[c * Nt + s] = from.pos[c * from.Nt + src] + q.dtr[c];
+        to.vel[c * Nt + s] = from.vel[c * from.Nt + src];
+        to.force[c * Nt + s] = from.force[c * from.Nt + src];
+      }
+    } else {
+      to.type[s] = q.pi[k];
+      to.gstart[s] = q.new_g + q.pi[q.n + k];
+      to.mass[s] = q.pd[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        to.pos[c * Nt + s] = q.pd[q.n + 3 * k + c] + q.dtr[c];
+        to.vel[c * Nt + s] = q.pd[4 * q.n + 3 * k + c];
+        to.force[c * Nt + s] = 0.0;
+      }
+    }
+  } else {
+    const int k = s - q.new_g;
+    const int g = s - no;
+    if (q.old_g >= 0) {
+      const int src = q.old_g + k;
+      to.owner[s] = q.new_o + (from.owner[src] - q.old_o);
+      to.image[s] = from.image[src];
+      to.type[s] = from.type[src];
+      to.mass[s] = from.mass[src];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) to.gshift[3 * g + c] = from.gshift[3 * (src - from.n_owned) + c];
+    } else {
+      const int a = q.pi[2 * q.n + k];
+      to.owner[s] = q.new_o + a;
+      to.image[s] = q.pi[2 * q.n + q.ng + k];
+      to.type[s] = q.pi[a];
+      to.mass[s] = q.pd[a];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) to.gshift[3 * g + c] = q.pd[7 * q.n + 3 * k + c];
+    }
   }
 }
 
-// newly admitted atoms: position and velocity from the staging upload
-__global__ void k_serve_fresh(int nf, int Nt, const int* __restrict__ dst, const double* __restrict__ fresh,
-                              double* __restrict__ pos, double* __restrict__ vel, double* __restrict__ force)
-{
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nf) return;
-  const int i = dst[k];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    pos[c * Nt + i] = fresh[6 * k + c];
-    vel[c * Nt + i] = fresh[6 * k + 3 + c];
-    force[c * Nt + i] = 0.0;
-  }
-}
-
-// finished atoms -> packed (position - region shift, velocity, force, potential)
-__global__ void k_serve_extract(int no, int Nt, const int* __restrict__ slot, const double* __restrict__ shift,
+// finished requests -> their result blocks, one thread per atom; the
+// request is found by binary search over the blocks' first atoms
+__global__ void k_serve_extract(int R, const ServeOut* __restrict__ req, long long no, int Nt,
                                 const double* __restrict__ pos, const double* __restrict__ vel,
                                 const double* __restrict__ force, const double* __restrict__ pe,
                                 double* __restrict__ out)
 {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= no) return;
-  const int i = slot[k];
+  int lo = 0, hi = R - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (req[mid].out / 10 <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  const ServeOut& q = req[lo];
+  const int a = (int)(k - q.out / 10), n = q.n, i = q.slot0 + a;
+  double* blk = out + q.out;
+  blk[a] = pe[i];
+  double x[3], f[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    out[10 * k + c] = pos[c * Nt + i] - shift[3 * k + c];
-    out[10 * k + 3 + c] = vel[c * Nt + i];
-    out[10 * k + 6 + c] = force[c * Nt + i];
+    x[c] = pos[c * Nt + i] - q.trans[c];
+    blk[n + c * n + a] = force[c * Nt + i];
+    blk[7 * n + c * n + a] = vel[c * Nt + i];
   }
-  out[10 * k + 9] = pe[i];
+  // back into the request's box: fractional coordinates, wrapped where periodic
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    f[c] = q.hi[3 * c] * x[0] + q.hi[3 * c + 1] * x[1] + q.hi[3 * c + 2] * x[2];
+    if (q.pbc[c]) f[c] -= floor(f[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) blk[4 * n + c * n + a] = q.h[3 * c] * f[0] + q.h[3 * c + 1] * f[1] + q.h[3 * c + 2] * f[2];
 }
 
 hipError_t launch_serve_kick(int first_half, int n, int Nt, const int* atom_req, const int* req_md,
@@ -1710,26 +1764,20 @@ hipError_t launch_serve_ghosts(int n_owned, int n_ghost, int Nt, const int* owne
   return hipGetLastError();
 }
 
-hipError_t launch_serve_remap(int n, int Nt_new, int Nt_old, const int* src, const double* dtr, const double* pos_o,
-                              const double* vel_o, const double* f_o, double* pos_n, double* vel_n, double* f_n,
-                              hipStream_t s)
+hipError_t launch_serve_assemble(int R, const ServeSlots* req, const ServeLayout& from, const ServeLayout& to,
+                                 int* atom_req, hipStream_t s)
 {
-  if (n > 0)
-    k_serve_remap<<<nblk(n), kBlock, 0, s>>>(n, Nt_new, Nt_old, src, dtr, pos_o, vel_o, f_o, pos_n, vel_n, f_n);
+  if (R > 0 && to.Nt > 0) k_serve_assemble<<<nblk(to.Nt), kBlock, 0, s>>>(R, req, from, to, atom_req);
   return hipGetLastError();
 }
 
-hipError_t launch_serve_fresh(int nf, int Nt, const int* dst, const double* fresh, double* pos, double* vel,
-                              double* force, hipStream_t s)
+hipError_t launch_serve_extract(int R, const ServeOut* req, long long no, int Nt, const double* pos,
+                                const double* vel, const double* force, const double* pe, double* out,
+                                hipStream_t s)
 {
-  if (nf > 0) k_serve_fresh<<<nblk(nf), kBlock, 0, s>>>(nf, Nt, dst, fresh, pos, vel, force);
-  return hipGetLastError();
-}
-
-hipError_t launch_serve_extract(int no, int Nt, const int* slot, const double* shift, const double* pos,
-                                const double* vel, const double* force, const double* pe, double* out, hipStream_t s)
-{
-  if (no > 0) k_serve_extract<<<nblk(no), kBlock, 0, s>>>(no, Nt, slot, shift, pos, vel, force, pe, out);
+  if (R > 0 && no > 0)
+    k_serve_extract<<<(unsigned)((no + kBlock - 1) / kBlock), kBlock, 0, s>>>(R, req, no, Nt, pos, vel, force, pe,
+                                                                             out);
   return hipGetLastError();
 }
 

@@ -79,6 +79,7 @@ struct DevWork {
   int* cell_count = nullptr;   // [ncell] zero between rebuilds
   int* cell_start = nullptr;   // [ncell+1]
   int* cell_atoms = nullptr;   // [cap]
+  int* cell_tiles = nullptr;   // [ncell/4096 + 2] tile sums, then offsets, of the multi-block scan
   int* vn = nullptr;           // [cap] Verlet row length
   int* vl = nullptr;           // [cap*MV] Verlet rows
   int* vrev = nullptr;         // [cap*MV] position of the centre in the neighbour's row
@@ -176,21 +177,61 @@ hipError_t launch_fold(int N, long long n2, bool accumulate, const double* pe2, 
 
 // Serving loop (arrays of the batch have stride Nt = owned + ghost slots):
 // masked half kick of owned atoms whose request takes an MD step; ghost
-// positions = owner + shift; remap of continuing owned atoms to their new
-// slots (+ translation change); scatter of newly admitted atoms; packing of
-// finished atoms' (position - shift, velocity, force, potential).
+// positions = owner + shift; assembly of a new layout; packing of finished
+// atoms' (position - shift, velocity, force, potential).
 hipError_t launch_serve_kick(int first_half, int n, int Nt, const int* atom_req, const int* req_md,
                              const double* req_dt, const double* mass, double* pos, double* vel, const double* force,
                              hipStream_t s);
 hipError_t launch_serve_ghosts(int n_owned, int n_ghost, int Nt, const int* owner, const double* shift, double* pos,
                                hipStream_t s);
-hipError_t launch_serve_remap(int n, int Nt_new, int Nt_old, const int* src, const double* dtr, const double* pos_o,
-                              const double* vel_o, const double* f_o, double* pos_n, double* vel_n, double* f_n,
-                              hipStream_t s);
-hipError_t launch_serve_fresh(int nf, int Nt, const int* dst, const double* fresh, double* pos, double* vel,
-                              double* force, hipStream_t s);
-hipError_t launch_serve_extract(int no, int Nt, const int* slot, const double* shift, const double* pos,
-                                const double* vel, const double* force, const double* pe, double* out, hipStream_t s);
+
+// One request of a new serving layout: where its owned atoms and image
+// ghosts go, and where they come from: the previous layout for a continuing
+// request, or the request's own upload (a "pack") for a newly admitted one.
+// A pack holds, for n atoms and ng ghosts, the ints type[n], first ghost
+// (request-local)[n], ghost owner (request-local)[ng], image code[ng], then
+// the doubles mass[n], position[3n] and velocity[3n] (per atom x, y, z, in
+// the request's own frame) and ghost shift[3ng].
+struct ServeSlots {
+  int new_o, new_g;   // first owned / ghost slot in the new layout
+  int old_o, old_g;   // first owned / ghost slot in the previous layout, -1 if new
+  int n, ng;          // owned atoms, ghosts
+  const int* pi;      // a new request's pack: ints
+  const double* pd;   // a new request's pack: doubles
+  double dtr[3];      // translation added to the positions
+};
+// pack layout: ints then doubles, the doubles at an 8-byte boundary
+inline size_t serve_pack_ints(int n, int ng) { return 2 * (size_t)n + 2 * (size_t)ng; }
+inline size_t serve_pack_doubles(int n, int ng) { return 7 * (size_t)n + 3 * (size_t)ng; }
+inline size_t serve_pack_double_offset(int n, int ng) { return (serve_pack_ints(n, ng) * 4 + 7) & ~size_t(7); }
+inline size_t serve_pack_bytes(int n, int ng)
+{
+  return serve_pack_double_offset(n, ng) + 8 * serve_pack_doubles(n, ng);
+}
+// The per-slot arrays of one layout.  gshift is per ghost ([3*(slot - n_owned)]).
+struct ServeLayout {
+  int Nt = 0, n_owned = 0;
+  double *pos = nullptr, *vel = nullptr, *force = nullptr, *mass = nullptr, *gshift = nullptr;
+  int *type = nullptr, *owner = nullptr, *image = nullptr, *gstart = nullptr;
+};
+// Writes every slot of `to` (R requests in slot order; ghost positions are
+// left to launch_serve_ghosts) and atom_req[owned slot] = request index.
+hipError_t launch_serve_assemble(int R, const ServeSlots* req, const ServeLayout& from, const ServeLayout& to,
+                                 int* atom_req, hipStream_t s);
+// One finished request for launch_serve_extract: its owned slots, the
+// offset of its block in the output, its placement translation and its box.
+struct ServeOut {
+  int slot0, n;
+  long long out;  // first double of its block of 10 n
+  int pbc[3];
+  double trans[3];
+  double h[9], hi[9];
+};
+// Writes, per finished request, the block [potential n | force 3n | position
+// 3n | velocity 3n] (SoA, position wrapped into the request's own box).
+hipError_t launch_serve_extract(int R, const ServeOut* req, long long no, int Nt, const double* pos,
+                                const double* vel, const double* force, const double* pe, double* out,
+                                hipStream_t s);
 
 // Number of thermo block partials launch_thermo uses for N atoms.
 int thermo_blocks(int N);

@@ -21,6 +21,7 @@ namespace dmgx {
 namespace {
 
 constexpr int kRowLanes = 16;  // lanes cooperating on one Verlet row
+constexpr int kScanOneBlock = 262144;  // larger cell grids are scanned by many workgroups
 
 __global__ void k_set_flag(int* __restrict__ flag, int value) { *flag = value; }
 
@@ -117,6 +118,57 @@ __global__ __launch_bounds__(1024) void k_scan_cells(const int* __restrict__ go,
     __syncthreads();
   }
   if (threadIdx.x == 0) start[ncell] = carry;
+}
+
+// Multi-block form of the same scan for large grids: each workgroup scans
+// one tile of 4096 cells (and clears its counts) and stores the tile total;
+// k_scan_cells then scans the tile totals into offsets and k_scan_offsets
+// adds each tile's offset.
+__global__ __launch_bounds__(1024) void k_scan_tiles(const int* __restrict__ go, int ncell, int* __restrict__ count,
+                                                     int* __restrict__ start, int* __restrict__ tile_sum)
+{
+  if (!*go) return;
+  __shared__ int wsum[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k0 = blockIdx.x * 4096 + threadIdx.x * 4;
+  int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = k0 + k < ncell ? count[k0 + k] : 0;
+    s += v[k];
+  }
+  int x = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int q = 0; q < 16; ++q) {
+    if (q < wave) before += wsum[q];
+    total += wsum[q];
+  }
+  int run = before + x - s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k0 + k < ncell) {
+      start[k0 + k] = run;
+      count[k0 + k] = 0;
+    }
+    run += v[k];
+  }
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ void k_scan_offsets(const int* __restrict__ go, int ncell, const int* __restrict__ tile_off,
+                               int* __restrict__ start)
+{
+  if (!*go) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ncell) start[k] += tile_off[k / 4096];
+  if (k == 0) start[ncell] = tile_off[(ncell + 4095) / 4096];
 }
 
 __global__ void k_cell_fill(const int* __restrict__ go, int N, const int* __restrict__ cell_of,
@@ -332,7 +384,16 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
   const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
   if (N > 0)
     k_cell_count<<<nblk(N), kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, w.flags);
-  k_scan_cells<<<1, 1024, 0, s>>>(go, ncell, w.cell_count, w.cell_start);
+  if (ncell <= kScanOneBlock) {
+    k_scan_cells<<<1, 1024, 0, s>>>(go, ncell, w.cell_count, w.cell_start);
+  } else {
+    const int tiles = (ncell + 4095) / 4096;
+    int* tile_sum = w.cell_tiles;
+    int* tile_off = w.cell_tiles + tiles + 1;
+    k_scan_tiles<<<tiles, 1024, 0, s>>>(go, ncell, w.cell_count, w.cell_start, tile_sum);
+    k_scan_cells<<<1, 1024, 0, s>>>(go, tiles, tile_sum, tile_off);
+    k_scan_offsets<<<nblk(ncell), kBlock, 0, s>>>(go, ncell, tile_off, w.cell_start);
+  }
   if (N > 0) k_cell_fill<<<nblk(N), kBlock, 0, s>>>(go, N, w.cell_of, w.cell_rank, w.cell_start, w.cell_atoms);
   if (c2 > c1) {
     const int groups = build_groups(w.MV);

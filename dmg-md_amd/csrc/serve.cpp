@@ -81,19 +81,73 @@ void Server::stage(size_t bytes)
   h_stage_bytes_ = bytes;
 }
 
+// Device buffers for request packs, reused by size class (powers of two from
+// 4 KiB): freeing device memory would synchronise the device under the
+// serving loop, so buffers are only freed with the server.
+void* Server::pack_get(size_t bytes, size_t* cap)
+{
+  size_t c = 4096;
+  while (c < bytes) c <<= 1;
+  {
+    std::lock_guard<std::mutex> lk(pack_mu_);
+    auto it = pack_free_.find(c);
+    if (it != pack_free_.end()) {
+      void* p = it->second;
+      pack_free_.erase(it);
+      *cap = c;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, c) != hipSuccess) throw std::runtime_error("out of device memory for a request upload");
+  *cap = c;
+  return p;
+}
+
+void Server::pack_put(void* p, size_t cap)
+{
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(pack_mu_);
+  pack_free_.emplace(cap, p);
+}
+
+// pinned host buffers for building packs, reused the same way
+void* Server::pin_get(size_t bytes, size_t* cap)
+{
+  size_t c = 4096;
+  while (c < bytes) c <<= 1;
+  {
+    std::lock_guard<std::mutex> lk(pack_mu_);
+    auto it = pin_free_.find(c);
+    if (it != pin_free_.end()) {
+      void* p = it->second;
+      pin_free_.erase(it);
+      *cap = c;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess)
+    throw std::runtime_error("out of pinned host memory for a request upload");
+  *cap = c;
+  return p;
+}
+
 struct Server::Request {
   long long id = 0;
   Box box{};
   double hi[9] = {};
   int n = 0;
-  std::vector<int> type;
-  std::vector<double> mass, pos, vel;  // pos wrapped into the box, SoA
+  int ng = 0;  // periodic images within rc + skin + margin of the box
   int steps = 0;
   double dt = 0;
-  // periodic images within rc + skin + margin of the box, grouped by owner:
-  // owner (local index), image code and Cartesian shift
-  std::vector<int> g_owner, g_image;
-  std::vector<double> g_shift;
+  // the request's atoms and images on the device until its first layout
+  // (ServeSlots pack layout), from the server's pack pool
+  void* pack = nullptr;
+  size_t pack_cap = 0;
+  void* hpack = nullptr;  // its pinned host copy, until the upload is consumed
+  size_t hpack_cap = 0;
+  hipEvent_t ready = nullptr;  // the upload's completion on the upload stream
   double lo[3] = {0, 0, 0}, hi_ext[3] = {0, 0, 0};  // extent of owned + ghost positions
   // progress
   int state = 0;  // 0: needs its first force evaluation, 1: MD steps, 2: done
@@ -103,13 +157,28 @@ struct Server::Request {
   std::string error;
   // placement in the current layout
   double trans[3] = {0, 0, 0};
-  int slot0 = -1;
+  int slot0 = -1, gslot0 = -1;  // first owned / ghost slot in the layout
   bool placed = false;
   // results
   std::vector<double> r_pe, r_force, r_pos, r_vel;
-  long long ghosts() const { return (long long)g_owner.size(); }
+  long long ghosts() const { return ng; }
   long long footprint() const { return n + ghosts(); }
 };
+
+// returns a request's upload buffers and event once the upload is consumed
+// (or never will be)
+void Server::release_upload(Request& r)
+{
+  if (r.ready) (void)hipEventDestroy(r.ready);
+  r.ready = nullptr;
+  pack_put(r.pack, r.pack_cap);
+  r.pack = nullptr;
+  if (r.hpack) {
+    std::lock_guard<std::mutex> lk(pack_mu_);
+    pin_free_.emplace(r.hpack_cap, r.hpack);
+  }
+  r.hpack = nullptr;
+}
 
 Server::Server(NepEngine* nep, const ServeConfig& cfg) : nep_(nep), cfg_(cfg)
 {
@@ -117,6 +186,7 @@ Server::Server(NepEngine* nep, const ServeConfig& cfg) : nep_(nep), cfg_(cfg)
   if (cfg_.max_atoms <= 0 || cfg_.max_requests <= 0) throw std::invalid_argument("bad serving limits");
   if (cfg_.max_atoms >= (1LL << 30)) throw std::invalid_argument("max_atoms must stay below 2^30");
   check(hipStreamCreate(&stream_), "serving stream");
+  check(hipStreamCreate(&upload_), "serving upload stream");
   thread_ = std::thread([this] { worker(); });
 }
 
@@ -129,11 +199,19 @@ Server::~Server()
   cv_.notify_all();
   if (thread_.joinable()) thread_.join();
   if (stream_) (void)hipStreamSynchronize(stream_);
-  for (DBuf<double>* b : {&pos_, &vel_, &force_, &pe_, &vir_, &mass_, &pos2_, &vel2_, &force2_, &req_dt_, &g_shift_,
-                          &remap_dt_, &fresh_, &out_shift_, &out_})
+  for (DBuf<double>* b : {&pos_, &vel_, &force_, &pe_, &vir_, &mass_, &pos2_, &vel2_, &force2_, &mass2_, &req_dt_,
+                          &g_shift_, &g_shift2_, &out_})
     b->release();
-  for (DBuf<int>* b : {&type_, &atom_req_, &req_md_, &owner_, &image_, &gstart_, &remap_src_, &fresh_dst_, &out_slot_})
+  for (DBuf<int>* b : {&type_, &type2_, &atom_req_, &req_md_, &owner_, &owner2_, &image_, &image2_, &gstart_,
+                       &gstart2_})
     b->release();
+  slots_.release();
+  outq_.release();
+  if (upload_) (void)hipStreamSynchronize(upload_);
+  for (auto& kv : all_) release_upload(*kv.second);  // every request, waiting ones included
+  for (auto& kv : pack_free_) (void)hipFree(kv.second);
+  for (auto& kv : pin_free_) (void)hipHostFree(kv.second);
+  if (upload_) (void)hipStreamDestroy(upload_);
   if (h_stage_) (void)hipHostFree(h_stage_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -144,6 +222,7 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
   if (N <= 0 || !type || !mass || !pos) throw std::invalid_argument("a request needs atoms, types, masses, positions");
   if (steps < 0) throw std::invalid_argument("negative step count");
   if (steps > 0 && !(dt_fs > 0.0)) throw std::invalid_argument("MD requests need dt_fs > 0");
+  const double t_call = now_s();
   auto r = std::make_shared<Request>();
   r->box = box;
   const double vol = box_volume(box.h);
@@ -152,16 +231,12 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
   r->n = N;
   r->steps = steps;
   r->dt = dt_fs / kTimeUnit;
-  r->type.assign(type, type + N);
-  r->mass.assign(mass, mass + N);
   const int T = nep_->model().num_types;
   for (int i = 0; i < N; ++i) {
     if (type[i] < 0 || type[i] >= T) throw std::invalid_argument("atom type out of range");
     if (!(mass[i] > 0.0)) throw std::invalid_argument("masses must be positive");
   }
-  r->pos.resize(3 * (size_t)N);
-  r->vel.assign(3 * (size_t)N, 0.0);
-  if (vel) std::copy(vel, vel + 3 * (size_t)N, r->vel.begin());
+  std::vector<double> wpos(3 * (size_t)N);  // wrapped into the box, SoA
   // wrap into the box and collect the periodic images of the shell
   const double rc = nep_->model().rcR_max;
   const double d1 = rc + kSkin + cfg_.margin;
@@ -182,8 +257,10 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
       frac[3 * i + d] = s[d];
     }
     to_cart(box.h, s, x);
-    for (int d = 0; d < 3; ++d) r->pos[(size_t)d * N + i] = x[d];
+    for (int d = 0; d < 3; ++d) wpos[(size_t)d * N + i] = x[d];
   }
+  std::vector<int> g_owner, g_image;
+  std::vector<double> g_shift;
   // every image within d1 of the box (per face pair: a lower bound of the
   // Euclidean distance, so the shell is a superset of what the rows need);
   // an owned atom's Verlet neighbours at any rebuild lie within
@@ -204,10 +281,13 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
           const double sh[3] = {box.h[0] * ia + box.h[1] * ib + box.h[2] * ic,
                                 box.h[3] * ia + box.h[4] * ib + box.h[5] * ic,
                                 box.h[6] * ia + box.h[7] * ib + box.h[8] * ic};
-          r->g_owner.push_back(i);
-          r->g_image.push_back(image_code(ia, ib, ic));
-          r->g_shift.insert(r->g_shift.end(), sh, sh + 3);
+          g_owner.push_back(i);
+          g_image.push_back(image_code(ia, ib, ic));
+          g_shift.insert(g_shift.end(), sh, sh + 3);
         }
+  if (g_owner.size() >= (1u << 30)) throw std::invalid_argument("request has too many periodic images");
+  r->ng = (int)g_owner.size();
+  const int NG = r->ng;
   for (int d = 0; d < 3; ++d) {
     r->lo[d] = 1e300;
     r->hi_ext[d] = -1e300;
@@ -219,21 +299,55 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
     }
   };
   for (int i = 0; i < N; ++i) {
-    const double x[3] = {r->pos[i], r->pos[(size_t)N + i], r->pos[2 * (size_t)N + i]};
+    const double x[3] = {wpos[i], wpos[(size_t)N + i], wpos[2 * (size_t)N + i]};
     extend(x);
   }
-  for (size_t g = 0; g < r->g_owner.size(); ++g) {
-    const int o = r->g_owner[g];
-    const double x[3] = {r->pos[o] + r->g_shift[3 * g], r->pos[(size_t)N + o] + r->g_shift[3 * g + 1],
-                         r->pos[2 * (size_t)N + o] + r->g_shift[3 * g + 2]};
+  for (int g = 0; g < NG; ++g) {
+    const int o = g_owner[g];
+    const double x[3] = {wpos[o] + g_shift[3 * g], wpos[(size_t)N + o] + g_shift[3 * g + 1],
+                         wpos[2 * (size_t)N + o] + g_shift[3 * g + 2]};
     extend(x);
   }
   if (r->footprint() > cfg_.max_atoms)
     throw std::invalid_argument("request (with its periodic shell) exceeds the serving atom budget");
+  // the pack: built here and uploaded on the submitting thread, so the
+  // serving loop only copies it device-to-device when it places the request
+  const size_t doff = serve_pack_double_offset(N, NG), bytes = serve_pack_bytes(N, NG);
+  r->hpack = pin_get(bytes, &r->hpack_cap);
+  char* host = static_cast<char*>(r->hpack);
+  int* pi = reinterpret_cast<int*>(host);
+  double* pd = reinterpret_cast<double*>(host + doff);
+  std::copy(type, type + N, pi);
+  for (int g = 0, a = 0; a < N; ++a) {
+    pi[N + a] = g;
+    while (g < NG && g_owner[g] == a) ++g;
+  }
+  std::copy(g_owner.begin(), g_owner.end(), pi + 2 * (size_t)N);
+  std::copy(g_image.begin(), g_image.end(), pi + 2 * (size_t)N + NG);
+  std::copy(mass, mass + N, pd);
+  for (int a = 0; a < N; ++a)
+    for (int d = 0; d < 3; ++d) {
+      pd[N + 3 * (size_t)a + d] = wpos[(size_t)d * N + a];
+      pd[4 * (size_t)N + 3 * (size_t)a + d] = vel ? vel[(size_t)d * N + a] : 0.0;
+    }
+  std::copy(g_shift.begin(), g_shift.end(), pd + 7 * (size_t)N);
+  // asynchronous upload: the serving stream waits for `ready` before it
+  // places the request
+  r->pack = pack_get(bytes, &r->pack_cap);
+  hipError_t e = hipEventCreateWithFlags(&r->ready, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMemcpyAsync(r->pack, host, bytes, hipMemcpyHostToDevice, upload_);
+  if (e == hipSuccess) e = hipEventRecord(r->ready, upload_);
+  if (e != hipSuccess) {
+    release_upload(*r);
+    throw std::runtime_error(std::string("request upload: ") + hipGetErrorString(e));
+  }
   std::lock_guard<std::mutex> lk(mu_);
-  if (!error_.empty()) throw std::runtime_error("serving loop failed: " + error_);
+  if (!error_.empty()) {
+    release_upload(*r);
+    throw std::runtime_error("serving loop failed: " + error_);
+  }
   r->id = next_id_++;
-  r->t_submit = now_s();
+  r->t_submit = t_call;
   if (t_first_ < 0) t_first_ = r->t_submit;
   waiting_.push_back(r);
   all_[r->id] = r;
@@ -394,81 +508,62 @@ void Server::relayout()
   }
   const long long Nt = n_owned + n_ghost;
   if (Nt >= (1LL << 30)) throw std::runtime_error("serving batch too large");
-  const int old_Nt = n_total_;
-  // device arrays; position, velocity and force are double-buffered across the remap
+  // the new layout's arrays (double-buffered: the previous layout is the source)
   pos2_.need(3 * Nt, "serve positions");
   vel2_.need(3 * Nt, "serve velocities");
   force2_.need(3 * Nt, "serve forces");
+  mass2_.need(Nt, "serve masses");
+  type2_.need(Nt, "serve types");
+  owner2_.need(Nt, "serve slot owners");
+  image2_.need(Nt, "serve slot images");
+  gstart2_.need(n_owned + 1, "serve ghost ranges");
+  g_shift2_.need(3 * n_ghost, "serve ghost shifts");
   pe_.need(Nt, "serve potential");
   vir_.need(9 * Nt, "serve virial");
-  mass_.need(Nt, "serve masses");
-  type_.need(Nt, "serve types");
   atom_req_.need(n_owned, "serve atom->request");
-  remap_src_.need(n_owned, "serve remap");
-  remap_dt_.need(3 * n_owned, "serve remap shift");
   req_md_.need(R, "serve request flags");
   req_dt_.need(R, "serve request dt");
-  owner_.need(Nt, "serve slot owners");
-  image_.need(Nt, "serve slot images");
-  gstart_.need(n_owned + 1, "serve ghost ranges");
-  g_shift_.need(3 * n_ghost, "serve ghost shifts");
-  // host side of the layout
-  std::vector<int> h_type((size_t)Nt), h_req((size_t)n_owned), h_src((size_t)n_owned), h_own((size_t)Nt),
-      h_img((size_t)Nt, image_code(0, 0, 0)), h_gst((size_t)n_owned + 1);
-  std::vector<double> h_mass((size_t)Nt), h_dtr(3 * (size_t)n_owned, 0.0), h_gsh(3 * (size_t)n_ghost);
-  std::vector<int> h_fdst;
-  std::vector<double> h_fresh;
+  slots_.need(R, "serve layout table");
+  // host side: one table row per request; new requests bring their packs
+  std::vector<ServeSlots> tab((size_t)R);
   std::vector<double> h_dt((size_t)std::max(R, 1));
+  std::vector<Request*> consumed;
   long long o = 0, gk = n_owned;
   for (int q = 0; q < R; ++q) {
     Request& r = *active_[q];
-    const int gx = q % side, gy = (q / side) % side, gz = q / (side * side);
+    const int g3[3] = {q % side, (q / side) % side, q / (side * side)};
     double tnew[3];
-    const int g3[3] = {gx, gy, gz};
     for (int d = 0; d < 3; ++d) tnew[d] = g3[d] * slot[d] + 0.5 * gap - r.lo[d];
     h_dt[q] = r.dt;
-    for (int k = 0; k < r.n; ++k) {
-      const long long i = o + k;
-      h_own[i] = (int)i;
-      h_type[i] = r.type[k];
-      h_mass[i] = r.mass[k];
-      h_req[i] = q;
-      if (r.placed) {
-        h_src[i] = r.slot0 + k;
-        for (int d = 0; d < 3; ++d) h_dtr[3 * i + d] = tnew[d] - r.trans[d];
-      } else {
-        h_src[i] = -1;
-        h_fdst.push_back((int)i);
-        for (int d = 0; d < 3; ++d) h_fresh.push_back(r.pos[(size_t)d * r.n + k] + tnew[d]);
-        for (int d = 0; d < 3; ++d) h_fresh.push_back(r.vel[(size_t)d * r.n + k]);
-      }
-    }
-    // ghosts in owner order: owned slot o + a owns [h_gst[o + a], h_gst[o + a + 1])
-    long long g = 0;
-    for (int a = 0; a < r.n; ++a) {
-      h_gst[o + a] = (int)gk;
-      for (; g < r.ghosts() && r.g_owner[g] == a; ++g, ++gk) {
-        const long long k = gk - n_owned;
-        h_own[gk] = (int)(o + a);
-        h_img[gk] = r.g_image[g];
-        for (int d = 0; d < 3; ++d) h_gsh[3 * k + d] = r.g_shift[3 * g + d];
-        h_type[gk] = r.type[a];
-        h_mass[gk] = r.mass[a];
-      }
+    ServeSlots& t = tab[q];
+    t.new_o = (int)o;
+    t.new_g = (int)gk;
+    t.n = r.n;
+    t.ng = r.ng;
+    t.pi = nullptr;
+    t.pd = nullptr;
+    if (r.placed) {
+      t.old_o = r.slot0;
+      t.old_g = r.gslot0;
+      for (int d = 0; d < 3; ++d) t.dtr[d] = tnew[d] - r.trans[d];
+    } else {
+      if (!r.pack) throw std::logic_error("serving: a new request has no pack");
+      t.old_o = t.old_g = -1;
+      check(hipStreamWaitEvent(stream_, r.ready, 0), "serve wait for upload");
+      t.pi = static_cast<const int*>(r.pack);
+      t.pd = reinterpret_cast<const double*>(static_cast<const char*>(r.pack) + serve_pack_double_offset(r.n, r.ng));
+      for (int d = 0; d < 3; ++d) t.dtr[d] = tnew[d];
+      consumed.push_back(&r);
     }
     r.slot0 = (int)o;
+    r.gslot0 = (int)gk;
     for (int d = 0; d < 3; ++d) r.trans[d] = tnew[d];
     r.placed = true;
     o += r.n;
+    gk += r.ng;
   }
-  h_gst[n_owned] = (int)gk;
-  const long long nf = (long long)h_fdst.size();
-  fresh_dst_.need(nf, "serve fresh slots");
-  fresh_.need(6 * nf, "serve fresh atoms");
   // uploads through one pinned staging buffer (the stream is idle here)
-  const size_t bytes = h_type.size() * 4 + h_mass.size() * 8 + h_req.size() * 4 + h_src.size() * 4 +
-                       h_dtr.size() * 8 + h_own.size() * 4 + h_img.size() * 4 + h_gst.size() * 4 +
-                       h_gsh.size() * 8 + h_fdst.size() * 4 + h_fresh.size() * 8 + h_dt.size() * 8 + 64 * 16;
+  const size_t bytes = tab.size() * sizeof(ServeSlots) + h_dt.size() * 8 + 4 * 16;
   stage(bytes);
   char* hs = h_stage_;
   size_t off = 0;
@@ -479,29 +574,43 @@ void Server::relayout()
     check(hipMemcpyAsync(dev, hs + off, n, hipMemcpyHostToDevice, stream_), "serve upload");
     off += n;
   };
-  up(type_.p, h_type.data(), h_type.size() * 4);
-  up(mass_.p, h_mass.data(), h_mass.size() * 8);
-  up(atom_req_.p, h_req.data(), h_req.size() * 4);
-  up(remap_src_.p, h_src.data(), h_src.size() * 4);
-  up(remap_dt_.p, h_dtr.data(), h_dtr.size() * 8);
-  up(owner_.p, h_own.data(), h_own.size() * 4);
-  up(image_.p, h_img.data(), h_img.size() * 4);
-  up(gstart_.p, h_gst.data(), h_gst.size() * 4);
-  up(g_shift_.p, h_gsh.data(), h_gsh.size() * 8);
-  up(fresh_dst_.p, h_fdst.data(), h_fdst.size() * 4);
-  up(fresh_.p, h_fresh.data(), h_fresh.size() * 8);
+  up(slots_.p, tab.data(), tab.size() * sizeof(ServeSlots));
   up(req_dt_.p, h_dt.data(), h_dt.size() * 8);
-  // continuing atoms move from the old layout, new atoms come from staging
-  if (old_Nt > 0 && pos_.p)
-    check(launch_serve_remap((int)n_owned, (int)Nt, old_Nt, remap_src_.p, remap_dt_.p, pos_.p, vel_.p, force_.p,
-                             pos2_.p, vel2_.p, force2_.p, stream_),
-          "serve remap");
-  check(launch_serve_fresh((int)nf, (int)Nt, fresh_dst_.p, fresh_.p, pos2_.p, vel2_.p, force2_.p, stream_),
-        "serve fresh");
+  ServeLayout from, to;
+  from.Nt = n_total_;
+  from.n_owned = n_owned_;
+  from.pos = pos_.p;
+  from.vel = vel_.p;
+  from.force = force_.p;
+  from.mass = mass_.p;
+  from.gshift = g_shift_.p;
+  from.type = type_.p;
+  from.owner = owner_.p;
+  from.image = image_.p;
+  from.gstart = gstart_.p;
+  to.Nt = (int)Nt;
+  to.n_owned = (int)n_owned;
+  to.pos = pos2_.p;
+  to.vel = vel2_.p;
+  to.force = force2_.p;
+  to.mass = mass2_.p;
+  to.gshift = g_shift2_.p;
+  to.type = type2_.p;
+  to.owner = owner2_.p;
+  to.image = image2_.p;
+  to.gstart = gstart2_.p;
+  check(launch_serve_assemble(R, slots_.p, from, to, atom_req_.p, stream_), "serve assemble");
   check(hipStreamSynchronize(stream_), "serve relayout");
+  for (Request* r : consumed) release_upload(*r);
   std::swap(pos_, pos2_);
   std::swap(vel_, vel2_);
   std::swap(force_, force2_);
+  std::swap(mass_, mass2_);
+  std::swap(type_, type2_);
+  std::swap(owner_, owner2_);
+  std::swap(image_, image2_);
+  std::swap(gstart_, gstart2_);
+  std::swap(g_shift_, g_shift2_);
   n_owned_ = (int)n_owned;
   n_ghost_ = (int)n_ghost;
   n_total_ = (int)Nt;
@@ -579,58 +688,46 @@ void Server::retire()
   slk.unlock();
   if (!fin.empty()) {
     long long no = 0;
-    for (Request* r : fin) no += r->n;
-    out_slot_.need(no, "serve out slots");
-    out_shift_.need(3 * no, "serve out shifts");
-    out_.need(10 * no, "serve results");
-    std::vector<int> slots((size_t)no);
-    std::vector<double> shifts(3 * (size_t)no);
-    long long k = 0;
-    for (Request* r : fin)
-      for (int a = 0; a < r->n; ++a, ++k) {
-        slots[k] = r->slot0 + a;
-        for (int d = 0; d < 3; ++d) shifts[3 * k + d] = r->trans[d];
+    std::vector<ServeOut> tab(fin.size());
+    for (size_t q = 0; q < fin.size(); ++q) {
+      const Request& r = *fin[q];
+      ServeOut& t = tab[q];
+      t.slot0 = r.slot0;
+      t.n = r.n;
+      t.out = 10 * no;
+      for (int d = 0; d < 3; ++d) {
+        t.pbc[d] = r.box.pbc[d] ? 1 : 0;
+        t.trans[d] = r.trans[d];
       }
-    const size_t o2 = (slots.size() * 4 + 15) & ~size_t(15);
-    const size_t o3 = (o2 + shifts.size() * 8 + 15) & ~size_t(15);
+      std::memcpy(t.h, r.box.h, sizeof(t.h));
+      std::memcpy(t.hi, r.hi, sizeof(t.hi));
+      no += r.n;
+    }
+    outq_.need((long long)tab.size(), "serve out table");
+    out_.need(10 * no, "serve results");
+    const size_t o2 = (tab.size() * sizeof(ServeOut) + 15) & ~size_t(15);
     check(hipStreamSynchronize(stream_), "serve step");  // the staging buffer is free again
-    stage(o3 + 10 * (size_t)no * 8);
+    stage(o2 + 10 * (size_t)no * 8);
     char* hs = h_stage_;
-    std::memcpy(hs, slots.data(), slots.size() * 4);
-    std::memcpy(hs + o2, shifts.data(), shifts.size() * 8);
-    check(hipMemcpyAsync(out_slot_.p, hs, slots.size() * 4, hipMemcpyHostToDevice, stream_), "serve out upload");
-    check(hipMemcpyAsync(out_shift_.p, hs + o2, shifts.size() * 8, hipMemcpyHostToDevice, stream_),
+    std::memcpy(hs, tab.data(), tab.size() * sizeof(ServeOut));
+    check(hipMemcpyAsync(outq_.p, hs, tab.size() * sizeof(ServeOut), hipMemcpyHostToDevice, stream_),
           "serve out upload");
-    check(launch_serve_extract((int)no, n_total_, out_slot_.p, out_shift_.p, pos_.p, vel_.p, force_.p, pe_.p, out_.p,
+    check(launch_serve_extract((int)tab.size(), outq_.p, no, n_total_, pos_.p, vel_.p, force_.p, pe_.p, out_.p,
                                stream_),
           "serve extract");
-    check(hipMemcpyAsync(hs + o3, out_.p, 10 * (size_t)no * 8, hipMemcpyDeviceToHost, stream_), "serve results");
+    check(hipMemcpyAsync(hs + o2, out_.p, 10 * (size_t)no * 8, hipMemcpyDeviceToHost, stream_), "serve results");
     check(hipStreamSynchronize(stream_), "serve results");
-    const double* res = reinterpret_cast<const double*>(hs + o3);
+    const double* res = reinterpret_cast<const double*>(hs + o2);
     const double t = now_s();
     std::lock_guard<std::mutex> lk(mu_);
-    k = 0;
-    for (Request* r : fin) {
+    for (size_t q = 0; q < fin.size(); ++q) {
+      Request* r = fin[q];
       const size_t n = (size_t)r->n;
-      r->r_pe.resize(n);
-      r->r_force.resize(3 * n);
-      r->r_pos.resize(3 * n);
-      r->r_vel.resize(3 * n);
-      for (size_t a = 0; a < n; ++a, ++k) {
-        const double* v = res + 10 * k;
-        // positions back into the request's own box
-        double x[3] = {v[0], v[1], v[2]}, s[3];
-        to_frac(r->hi, x, s);
-        for (int d = 0; d < 3; ++d)
-          if (r->box.pbc[d]) s[d] -= std::floor(s[d]);
-        to_cart(r->box.h, s, x);
-        for (int d = 0; d < 3; ++d) {
-          r->r_pos[d * n + a] = x[d];
-          r->r_vel[d * n + a] = v[3 + d];
-          r->r_force[d * n + a] = v[6 + d];
-        }
-        r->r_pe[a] = v[9];
-      }
+      const double* blk = res + tab[q].out;
+      r->r_pe.assign(blk, blk + n);
+      r->r_force.assign(blk + n, blk + 4 * n);
+      r->r_pos.assign(blk + 4 * n, blk + 7 * n);
+      r->r_vel.assign(blk + 7 * n, blk + 10 * n);
       r->state = 2;
       r->live = false;
       r->t_done = t;

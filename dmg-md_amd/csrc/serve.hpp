@@ -113,22 +113,28 @@ class Server {
   int n_owned_ = 0, n_ghost_ = 0, n_total_ = 0;
   Box batch_box_{};
   DBuf<double> pos_, vel_, force_, pe_, vir_, mass_;
-  DBuf<double> pos2_, vel2_, force2_;  // relayout targets (swapped in)
   DBuf<int> type_;
-  DBuf<int> atom_req_;     // owned slot -> active request index
-  DBuf<int> req_md_;       // active request -> 1 when it takes an MD step this iteration
-  DBuf<double> req_dt_;    // active request -> dt (natural units)
   DBuf<int> owner_;        // slot -> owned slot it is an image of (itself for owned slots)
   DBuf<int> image_;        // slot -> image code (image_code(0, 0, 0) for owned slots)
   DBuf<int> gstart_;       // owned slot -> first ghost slot of its images [n_owned + 1]
   DBuf<double> g_shift_;   // ghost k -> box shift [3]
-  DBuf<int> remap_src_;    // new owned slot -> old slot (continuing) or -1
-  DBuf<double> remap_dt_;  // new owned slot -> translation change [3]
-  DBuf<int> fresh_dst_;    // newly admitted atom -> owned slot
-  DBuf<double> fresh_;     // newly admitted atoms: position [3] + velocity [3]
-  DBuf<int> out_slot_;     // finished atoms -> owned slot
-  DBuf<double> out_shift_; // finished atoms -> translation to remove [3]
-  DBuf<double> out_;       // finished atoms: position, velocity, force, potential (10 doubles)
+  // the next layout's per-slot arrays, assembled on the device and swapped in
+  DBuf<double> pos2_, vel2_, force2_, mass2_, g_shift2_;
+  DBuf<int> type2_, owner2_, image2_, gstart2_;
+  DBuf<int> atom_req_;     // owned slot -> active request index
+  DBuf<int> req_md_;       // active request -> 1 when it takes an MD step this iteration
+  DBuf<double> req_dt_;    // active request -> dt (natural units)
+  DBuf<ServeSlots> slots_; // layout table (one row per active request)
+  DBuf<ServeOut> outq_;    // finished requests (extract table)
+  DBuf<double> out_;       // finished requests' result blocks (10 doubles per atom)
+  hipStream_t upload_ = nullptr;  // request uploads (submitting threads)
+  std::mutex pack_mu_;
+  std::multimap<size_t, void*> pack_free_;  // reusable device pack buffers by capacity
+  std::multimap<size_t, void*> pin_free_;   // reusable pinned host buffers by capacity
+  void* pack_get(size_t bytes, size_t* cap);
+  void pack_put(void* p, size_t cap);
+  void* pin_get(size_t bytes, size_t* cap);
+  void release_upload(Request& r);
   char* h_stage_ = nullptr;  // pinned staging for uploads and results
   size_t h_stage_bytes_ = 0;
   void stage(size_t bytes);
