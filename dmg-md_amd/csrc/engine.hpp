@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <functional>
 #include <string>
@@ -68,6 +69,8 @@ class NepEngine {
 
   const NepModel& model() const { return model_; }
   void reserve(int capacity);
+  // sizes the cell grid arrays for at least ncell cells
+  void reserve_cells(long long ncell) { ensure_cells((int)std::min<long long>(ncell, 1LL << 28)); }
   // Throws std::invalid_argument / std::runtime_error; see dmgx_nep_compute.
   // `map` (optional) marks the slots past the owned ones as periodic images
   // of owned slots (ImageMap); the serving loop's batches use it.

@@ -159,11 +159,19 @@ struct Server::Request {
   double trans[3] = {0, 0, 0};
   int slot0 = -1, gslot0 = -1;  // first owned / ghost slot in the layout
   bool placed = false;
-  // results
-  std::vector<double> r_pe, r_force, r_pos, r_vel;
+  // results: block [potential n | force 3n | position 3n | velocity 3n]
+  std::shared_ptr<ResultBlock> res;
+  long long res_off = 0;
   long long ghosts() const { return ng; }
   long long footprint() const { return n + ghosts(); }
 };
+
+void Server::pin_put(void* p, size_t cap)
+{
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(pack_mu_);
+  pin_free_.emplace(cap, p);
+}
 
 // returns a request's upload buffers and event once the upload is consumed
 // (or never will be)
@@ -173,10 +181,7 @@ void Server::release_upload(Request& r)
   r.ready = nullptr;
   pack_put(r.pack, r.pack_cap);
   r.pack = nullptr;
-  if (r.hpack) {
-    std::lock_guard<std::mutex> lk(pack_mu_);
-    pin_free_.emplace(r.hpack_cap, r.hpack);
-  }
+  pin_put(r.hpack, r.hpack_cap);
   r.hpack = nullptr;
 }
 
@@ -185,6 +190,10 @@ Server::Server(NepEngine* nep, const ServeConfig& cfg) : nep_(nep), cfg_(cfg)
   if (!nep) throw std::invalid_argument("null potential");
   if (cfg_.max_atoms <= 0 || cfg_.max_requests <= 0) throw std::invalid_argument("bad serving limits");
   if (cfg_.max_atoms >= (1LL << 30)) throw std::invalid_argument("max_atoms must stay below 2^30");
+  // the batch grows request by request: size the engine once, up front,
+  // instead of reallocating (and synchronising the device) as it grows
+  nep_->reserve((int)std::min(cfg_.max_atoms, cfg_.reserve_atoms));
+  nep_->reserve_cells(1LL << 23);
   check(hipStreamCreate(&stream_), "serving stream");
   check(hipStreamCreate(&upload_), "serving upload stream");
   thread_ = std::thread([this] { worker(); });
@@ -209,6 +218,10 @@ Server::~Server()
   outq_.release();
   if (upload_) (void)hipStreamSynchronize(upload_);
   for (auto& kv : all_) release_upload(*kv.second);  // every request, waiting ones included
+  // dropping the requests returns their result blocks to the pinned pool
+  waiting_.clear();
+  active_.clear();
+  all_.clear();
   for (auto& kv : pack_free_) (void)hipFree(kv.second);
   for (auto& kv : pin_free_) (void)hipHostFree(kv.second);
   if (upload_) (void)hipStreamDestroy(upload_);
@@ -370,12 +383,12 @@ bool Server::poll(long long id, bool wait, double* potential, double* force, dou
   }
   if (!r->error.empty()) throw std::runtime_error(r->error);
   const size_t n = (size_t)r->n;
-  if (potential) std::copy(r->r_pe.begin(), r->r_pe.end(), potential);
-  if (force) std::copy(r->r_force.begin(), r->r_force.end(), force);
-  if (position) std::copy(r->r_pos.begin(), r->r_pos.end(), position);
-  if (velocity) std::copy(r->r_vel.begin(), r->r_vel.end(), velocity);
+  const double* blk = static_cast<const double*>(r->res->p) + r->res_off;
+  if (potential) std::copy(blk, blk + n, potential);
+  if (force) std::copy(blk + n, blk + 4 * n, force);
+  if (position) std::copy(blk + 4 * n, blk + 7 * n, position);
+  if (velocity) std::copy(blk + 7 * n, blk + 10 * n, velocity);
   if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
-  (void)n;
   return true;
 }
 
@@ -705,29 +718,26 @@ void Server::retire()
     }
     outq_.need((long long)tab.size(), "serve out table");
     out_.need(10 * no, "serve results");
-    const size_t o2 = (tab.size() * sizeof(ServeOut) + 15) & ~size_t(15);
     check(hipStreamSynchronize(stream_), "serve step");  // the staging buffer is free again
-    stage(o2 + 10 * (size_t)no * 8);
-    char* hs = h_stage_;
-    std::memcpy(hs, tab.data(), tab.size() * sizeof(ServeOut));
-    check(hipMemcpyAsync(outq_.p, hs, tab.size() * sizeof(ServeOut), hipMemcpyHostToDevice, stream_),
+    stage(tab.size() * sizeof(ServeOut));
+    std::memcpy(h_stage_, tab.data(), tab.size() * sizeof(ServeOut));
+    check(hipMemcpyAsync(outq_.p, h_stage_, tab.size() * sizeof(ServeOut), hipMemcpyHostToDevice, stream_),
           "serve out upload");
     check(launch_serve_extract((int)tab.size(), outq_.p, no, n_total_, pos_.p, vel_.p, force_.p, pe_.p, out_.p,
                                stream_),
           "serve extract");
-    check(hipMemcpyAsync(hs + o2, out_.p, 10 * (size_t)no * 8, hipMemcpyDeviceToHost, stream_), "serve results");
+    // results go straight into a pinned block that the requests keep
+    auto blk = std::make_shared<ResultBlock>();
+    blk->srv = this;
+    blk->p = pin_get(10 * (size_t)no * 8, &blk->cap);
+    check(hipMemcpyAsync(blk->p, out_.p, 10 * (size_t)no * 8, hipMemcpyDeviceToHost, stream_), "serve results");
     check(hipStreamSynchronize(stream_), "serve results");
-    const double* res = reinterpret_cast<const double*>(hs + o2);
     const double t = now_s();
     std::lock_guard<std::mutex> lk(mu_);
     for (size_t q = 0; q < fin.size(); ++q) {
       Request* r = fin[q];
-      const size_t n = (size_t)r->n;
-      const double* blk = res + tab[q].out;
-      r->r_pe.assign(blk, blk + n);
-      r->r_force.assign(blk + n, blk + 4 * n);
-      r->r_pos.assign(blk + 4 * n, blk + 7 * n);
-      r->r_vel.assign(blk + 7 * n, blk + 10 * n);
+      r->res = blk;
+      r->res_off = tab[q].out;
       r->state = 2;
       r->live = false;
       r->t_done = t;

@@ -44,6 +44,7 @@ struct ServeConfig {
   int max_requests = 4096;
   unsigned flags = 0;             // force flags (ANN mode)
   double margin = 1.0;            // extra shell thickness (A) for motion during a request
+  long long reserve_atoms = 1LL << 21;  // engine workspace sized up front (capped by max_atoms)
 };
 
 struct ServeStats {
@@ -134,7 +135,16 @@ class Server {
   void* pack_get(size_t bytes, size_t* cap);
   void pack_put(void* p, size_t cap);
   void* pin_get(size_t bytes, size_t* cap);
+  void pin_put(void* p, size_t cap);
   void release_upload(Request& r);
+  // a pinned block of finished requests' results, shared by those requests
+  // until they are dropped with the server
+  struct ResultBlock {
+    Server* srv = nullptr;
+    void* p = nullptr;
+    size_t cap = 0;
+    ~ResultBlock() { srv->pin_put(p, cap); }
+  };
   char* h_stage_ = nullptr;  // pinned staging for uploads and results
   size_t h_stage_bytes_ = 0;
   void stage(size_t bytes);
