@@ -634,6 +634,7 @@ MdState::MdState(NepEngine* nep, const Box& box, int N, const int* type, const d
   dalloc(&type_, (size_t)N, "type");
   dalloc(&tw_.partial, (size_t)thermo_blocks(N) * 16, "thermo partials");
   dalloc(&tw_.thermo, 8, "thermo");
+  dalloc(&vmax2_, 1, "max speed");
   check(hipMemcpy(pos_, pos, 3 * sizeof(double) * N, hipMemcpyHostToDevice), "upload");
   check(hipMemcpy(vel_, vel, 3 * sizeof(double) * N, hipMemcpyHostToDevice), "upload");
   check(hipMemcpy(mass_, mass, sizeof(double) * N, hipMemcpyHostToDevice), "upload");
@@ -656,6 +657,7 @@ MdState::~MdState()
   dfree(&type_);
   dfree(&tw_.partial);
   dfree(&tw_.thermo);
+  dfree(&vmax2_);
   dfree(&ck_pos_);
   dfree(&ck_vel_);
   dfree(&ck_force_);
@@ -737,9 +739,26 @@ void MdState::checkpoint(bool save)
 void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out, int every,
                   const Output& on_output)
 {
+  RunOptions opt;
+  opt.every = every;
+  opt.on_output = on_output;
+  run(steps, dt_fs, ens, flags, thermo_out, opt);
+}
+
+void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out,
+                  const RunOptions& opt)
+{
   if (steps < 0) throw std::invalid_argument("negative step count");
+  if (opt.max_distance < 0.0 || !std::isfinite(opt.max_distance))
+    throw std::invalid_argument("maximum distance per step should be > 0");
+  if (opt.correct_every < 0) throw std::invalid_argument("negative correction interval");
   const double dt = dt_fs / kTimeUnit;
+  const int every = opt.every;
+  const Output& on_output = opt.on_output;
+  const bool adaptive = opt.max_distance > 0.0;
+  const bool correct = opt.correct_every > 0 && opt.on_correct;
   Controller ctl(ens, N_, dt);
+  elapsed_ = 0.0;
   double th[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (!have_force_) {
     force(flags);
@@ -748,17 +767,37 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
   // NHC needs the pre-force kinetic energy and NPT the pressure on the host
   // every step (scalar chain/box state): those stay on the synchronous path.
   int reps[3];
-  if (ctl.needs_pre_force_ke() || ctl.has_barostat() || nep_->small_box(box_, reps)) {
+  if (ctl.needs_pre_force_ke() || ctl.has_barostat() || nep_->small_box(box_, reps) || adaptive || correct) {
     for (int step = 0; step < steps; ++step) {
+      if (correct && step % opt.correct_every == 0) {
+        check(hipStreamSynchronize(stream_), "velocity correction");
+        opt.on_correct(step);
+      }
+      double dts = dt;
+      if (adaptive) {
+        // the largest speed at the step's start (its bits: non-negative
+        // doubles order as their bit patterns)
+        check(launch_max_speed2(N_, vel_, vmax2_, stream_), "max speed");
+        unsigned long long bits = 0;
+        check(hipMemcpyAsync(&bits, vmax2_, sizeof(bits), hipMemcpyDeviceToHost, stream_), "max speed");
+        check(hipStreamSynchronize(stream_), "max speed");
+        double v2;
+        std::memcpy(&v2, &bits, sizeof(v2));
+        if (v2 != 0.0) {
+          const double limited = opt.max_distance / std::sqrt(v2);
+          if (limited < dt) dts = limited;
+        }
+      }
+      elapsed_ += dts;
       if (ctl.needs_pre_force_ke()) {
         thermo(th);
         const double ke = 0.5 * th[0] * 3.0 * N_ * kBoltzmann;
-        if (auto f = ctl.before_force(dt, ke, step, steps))
+        if (auto f = ctl.before_force(dts, ke, step, steps))
           check(launch_scale(3 * N_, *f, vel_, stream_), "scale velocity");
       }
-      vv(1, dt);
+      vv(1, dts);
       force(flags);
-      vv(0, dt);
+      vv(0, dts);
       thermo(th);
       if (auto f = ctl.after_force(th, step, steps)) check(launch_scale(3 * N_, *f, vel_, stream_), "scale velocity");
       if (ctl.has_barostat()) {
@@ -845,6 +884,7 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
     if (redo) throw std::runtime_error("Verlet rows kept overflowing");
     force_calls_ += n;
     c0 += n;
+    elapsed_ += dt * n;
     if (on_output && ((every > 0 && c0 % every == 0) || c0 == steps)) {
       // the last step's thermo before its rescale (NVE: nothing rescales)
       double tho[8];

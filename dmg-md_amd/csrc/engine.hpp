@@ -153,6 +153,23 @@ class MdState {
   using Output = std::function<void(int steps_done, const double* thermo)>;
   void run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out, int every = 0,
            const Output& on_output = Output());
+  // Options of a run beyond the ensemble: output as above; an adaptive step,
+  // each step's dt being min(dt, max_distance / max |v|) over the velocities
+  // at the step's start (runtime.cu:594-618); and `on_correct(step)` called
+  // before every step whose index is a multiple of correct_every, with the
+  // stream idle (the reference's correct_velocity trigger, runtime.cu:749-756).
+  // Either of the last two runs the loop step by step.
+  struct RunOptions {
+    int every = 0;
+    Output on_output;
+    double max_distance = 0.0;  // A; 0 keeps the step fixed
+    int correct_every = 0;
+    std::function<void(int step)> on_correct;
+  };
+  void run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned flags, double* thermo_out,
+           const RunOptions& opt);
+  // natural time units advanced by the last run (sum of its steps' dt)
+  double elapsed() const { return elapsed_; }
   long long force_calls() const { return force_calls_; }
   void arrays(double** pos, double** vel, double** force, double** pe, double** vir, double** mass, int** type);
   const Box& box() const { return box_; }
@@ -177,6 +194,8 @@ class MdState {
   double* hrand_ = nullptr;   // pinned staging of the draws
   double* factor_ = nullptr;  // device velocity factor
   long long force_calls_ = 0;
+  unsigned long long* vmax2_ = nullptr;  // device: bits of the largest |v|^2 (adaptive step)
+  double elapsed_ = 0.0;
 };
 
 double box_volume(const double* h);

@@ -99,11 +99,12 @@ struct DumpXyz {
 };
 
 struct Cmd {
-  enum Kind { potential, velocity, time_step, ensemble, dump_thermo, dump_xyz, dump_restart, run } kind;
+  enum Kind { potential, velocity, time_step, ensemble, dump_thermo, dump_xyz, dump_restart, run, correct_velocity } kind;
   Where at;
   std::string text;        // potential file
   double a = 0, b = 0, c = 0, d = 0, e = 0, f = 0;
   int i = 0;               // interval / steps / seed
+  int j = -1;              // correct_velocity grouping method (-1: all atoms)
   bool has_seed = false;
   EnsembleConfig ens;
   DumpXyz xyz;
@@ -131,10 +132,14 @@ Cmd parse_command(const std::vector<std::string>& t, const Where& at)
       c.has_seed = true;
     }
   } else if (k == "time_step") {
-    if (t.size() == 3) fail(at, "time_step with a maximum distance per step (adaptive) is not supported by this build");
-    need(2, "time_step should have 1 parameter");
+    // time_step dt [maximum distance per step] (run_parser.cpp:184-197)
+    if (t.size() != 2 && t.size() != 3) fail(at, "time_step should have 1 or 2 parameters");
     c.kind = Cmd::time_step;
     c.a = to_real(t[1], at, "time_step");
+    if (t.size() == 3) {
+      c.b = to_real(t[2], at, "maximum distance per step");
+      if (c.b <= 0.0) fail(at, "maximum distance per step should be > 0");
+    }
   } else if (k == "ensemble") {
     if (t.size() < 2) fail(at, "ensemble type is missing");
     c.kind = Cmd::ensemble;
@@ -218,7 +223,15 @@ Cmd parse_command(const std::vector<std::string>& t, const Where& at)
     c.i = to_int(t[1], at, "number of steps");
     if (c.i < 0) fail(at, "number of steps should be >= 0");
   } else if (k == "correct_velocity") {
-    fail(at, "correct_velocity is not supported by this build");
+    // correct_velocity interval [grouping method] (run_parser.cpp:256-271)
+    if (t.size() != 2 && t.size() != 3) fail(at, "correct_velocity should have 1 or 2 parameters");
+    c.kind = Cmd::correct_velocity;
+    c.i = to_int(t[1], at, "velocity correction interval");
+    if (c.i < 10) fail(at, "velocity correction interval should be >= 10");
+    if (t.size() == 3) {
+      c.j = to_int(t[2], at, "velocity correction grouping method");
+      if (c.j < 0) fail(at, "velocity correction grouping method should be >= 0");
+    }
   } else {
     fail(at, "unsupported command '" + k + "'");
   }
@@ -408,30 +421,92 @@ Model parse_model(const std::string& path, const std::vector<std::string>& symbo
 // Maxwell-Boltzmann velocities at temperature T: Gaussian components from
 // std::mt19937_64, centre-of-mass momentum removed, rescaled to exactly T
 // (a fresh stream of this build; not the reference's generator)
-void random_velocities(Model& m, double T, uint64_t seed)
+// Removes the total linear momentum of `atoms`, then their rigid rotation
+// about their centre of mass: omega = I^-1 L, v -= omega x (r - r_c); the
+// rotation is kept when the inertia tensor is singular (runtime.cu:979-1076).
+void correct_velocity_subset(const std::vector<double>& mass, const double* pos, double* vel, int N,
+                             const std::vector<int>& atoms)
 {
-  std::mt19937_64 rng(seed);
-  std::normal_distribution<double> g(0.0, 1.0);
-  const int N = m.N;
-  double p[3] = {0, 0, 0}, mt = 0;
-  for (int i = 0; i < N; ++i) {
-    for (int d = 0; d < 3; ++d) {
-      const double v = g(rng) / std::sqrt(m.mass[i]);
-      m.vel[(size_t)d * N + i] = v;
-      p[d] += m.mass[i] * v;
-    }
-    mt += m.mass[i];
+  if (atoms.empty()) return;
+  double p[3] = {0, 0, 0}, mt = 0.0;
+  for (int a : atoms) {
+    mt += mass[a];
+    for (int d = 0; d < 3; ++d) p[d] += mass[a] * vel[(size_t)d * N + a];
   }
-  double ke2 = 0;
+  if (mt != 0.0) {
+    for (double& x : p) x /= mt;
+    for (int a : atoms)
+      for (int d = 0; d < 3; ++d) vel[(size_t)d * N + a] -= p[d];
+  }
+  double c[3] = {0, 0, 0};
+  mt = 0.0;
+  for (int a : atoms) {
+    mt += mass[a];
+    for (int d = 0; d < 3; ++d) c[d] += pos[(size_t)d * N + a] * mass[a];
+  }
+  for (double& x : c) x /= mt;
+  double L[3] = {0, 0, 0}, I[3][3] = {};
+  for (int a : atoms) {
+    const double m = mass[a];
+    const double dx = pos[a] - c[0], dy = pos[(size_t)N + a] - c[1], dz = pos[2 * (size_t)N + a] - c[2];
+    const double vx = vel[a], vy = vel[(size_t)N + a], vz = vel[2 * (size_t)N + a];
+    L[0] += m * (dy * vz - dz * vy);
+    L[1] += m * (dz * vx - dx * vz);
+    L[2] += m * (dx * vy - dy * vx);
+    I[0][0] += m * (dy * dy + dz * dz);
+    I[1][1] += m * (dx * dx + dz * dz);
+    I[2][2] += m * (dx * dx + dy * dy);
+    I[0][1] -= m * dx * dy;
+    I[1][2] -= m * dy * dz;
+    I[0][2] -= m * dx * dz;
+  }
+  I[1][0] = I[0][1];
+  I[2][1] = I[1][2];
+  I[2][0] = I[0][2];
+  // cofactor inverse (I is symmetric)
+  double C[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) {
+      const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, q1 = (q + 1) % 3, q2 = (q + 2) % 3;
+      C[q][r] = I[r1][q1] * I[r2][q2] - I[r1][q2] * I[r2][q1];
+    }
+  const double det = I[0][0] * C[0][0] + I[0][1] * C[1][0] + I[0][2] * C[2][0];
+  if (det > -1.0e-10 && det < 1.0e-10) return;
+  double w[3] = {0, 0, 0};
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) w[r] += C[r][q] / det * L[q];
+  for (int a : atoms) {
+    const double dx = pos[a] - c[0], dy = pos[(size_t)N + a] - c[1], dz = pos[2 * (size_t)N + a] - c[2];
+    vel[a] -= w[1] * dz - w[2] * dy;
+    vel[(size_t)N + a] -= w[2] * dx - w[0] * dz;
+    vel[2 * (size_t)N + a] -= w[0] * dy - w[1] * dx;
+  }
+}
+
+// The `velocity T [seed S]` initialisation (runtime.cu:1078-1109): every
+// component uniform in [-1, 1] from the C library's rand() (seeded per
+// component with S + 3 i + axis when a seed is given, so the draw is the
+// reference's on the same C library), momentum and rotation removed, then
+// scaled to T over 3N degrees of freedom.
+void random_velocities(Model& m, double T, bool has_seed, int seed)
+{
+  const int N = m.N;
   for (int i = 0; i < N; ++i)
     for (int d = 0; d < 3; ++d) {
-      double& v = m.vel[(size_t)d * N + i];
-      v -= p[d] / mt;
-      ke2 += m.mass[i] * v * v;
+      if (has_seed) std::srand((unsigned)(seed + i * 3 + d));
+      m.vel[(size_t)d * N + i] = -1.0 + (std::rand() * 2.0) / (double)RAND_MAX;
     }
-  const double t_now = ke2 / (3.0 * N * kBoltzmann);
-  const double s = t_now > 0 ? std::sqrt(T / t_now) : 0.0;
-  for (double& v : m.vel) v *= s;
+  std::vector<int> all((size_t)N);
+  for (int i = 0; i < N; ++i) all[i] = i;
+  correct_velocity_subset(m.mass, m.pos.data(), m.vel.data(), N, all);
+  double t = 0.0;
+  for (int i = 0; i < N; ++i) {
+    const double vx = m.vel[i], vy = m.vel[(size_t)N + i], vz = m.vel[2 * (size_t)N + i];
+    t += m.mass[i] * (vx * vx + vy * vy + vz * vz);
+  }
+  t /= 3.0 * kBoltzmann * N;
+  const double f = std::sqrt(T / t);
+  for (double& v : m.vel) v *= f;
 }
 
 // ---------------------------------------------------------------------------
@@ -564,6 +639,46 @@ void write_neighbor(const std::string& dir, long long call, NepEngine& nep)
       << "), angular(max=" << nep.model().MN_A << ",actual=" << ma << ")." << std::endl;
 }
 
+// The whole program is checked before anything runs, so an unsupported
+// later run cannot leave a partial trajectory (validate_runtime_ensembles,
+// run_parser.cpp:344-386): correct_velocity and an adaptive time_step only
+// with nve / nvt_ber, and a finite positive time_step for the thermostats
+// and barostats that integrate their own state.  n_label < 0 skips the
+// grouping-method range (the model is not read yet).
+void validate_program(const std::vector<Cmd>& cmds, int n_label)
+{
+  Ensemble kind = Ensemble::nve;
+  bool adaptive = false;
+  int correct_group = -2;  // -2: no correct_velocity since the last run
+  double dt_fs = 1.0;
+  for (const Cmd& c : cmds) {
+    if (c.kind == Cmd::time_step) {
+      dt_fs = c.a;
+      adaptive = c.b > 0.0;
+    } else if (c.kind == Cmd::correct_velocity) {
+      correct_group = c.j;
+    } else if (c.kind == Cmd::ensemble) {
+      kind = c.ens.kind;
+    } else if (c.kind == Cmd::run) {
+      const bool npt = kind == Ensemble::npt_ber || kind == Ensemble::npt_scr;
+      const bool bdp_nhc = kind == Ensemble::nvt_bdp || kind == Ensemble::nvt_nhc;
+      const std::string name = kind == Ensemble::npt_ber   ? "npt_ber"
+                               : kind == Ensemble::npt_scr ? "npt_scr"
+                               : kind == Ensemble::nvt_bdp ? "nvt_bdp"
+                                                           : "nvt_nhc";
+      const bool correct = correct_group != -2;
+      if ((npt || bdp_nhc) && correct) fail(c.at, "ensemble " + name + " does not support correct_velocity");
+      if ((npt || bdp_nhc) && adaptive) fail(c.at, "ensemble " + name + " does not support adaptive time_step");
+      if ((npt || bdp_nhc) && !(dt_fs > 0.0 && std::isfinite(dt_fs)))
+        fail(c.at, "ensemble " + name + " requires a finite positive time_step");
+      if (correct && n_label >= 0 && correct_group >= n_label)
+        fail(c.at, "correct_velocity grouping method is out of range");
+      adaptive = false;
+      correct_group = -2;
+    }
+  }
+}
+
 void hcheck(hipError_t e, const char* what)
 {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -601,12 +716,18 @@ void run_directory(const std::string& dir, const std::string& run_file, const st
       potential = c.text;
     }
   if (potential.empty()) throw InputError(dir + "/" + run_file + " does not contain a potential command");
+  validate_program(cmds, -1);  // before any device work
   NepEngine nep(potential[0] == '/' ? potential : dir + "/" + potential);
   Model m = parse_model(dir + "/" + model_file, nep.model().symbols);
-  if (!m.has_vel) random_velocities(m, 300.0, 12345678u);
+  validate_program(cmds, m.n_label);
+  // no velocities in the model: 300 K (the reference seeds this draw from the
+  // clock; here it is seeded, so runs repeat)
+  if (!m.has_vel) random_velocities(m, 300.0, true, 12345678);
   std::unique_ptr<MdState> md(new MdState(&nep, m.box, m.N, m.type.data(), m.mass.data(), m.pos.data(), m.vel.data()));
 
   double dt_fs = 1.0;
+  double max_distance = 0.0;      // adaptive time_step (A per step), this run only
+  int correct_every = 0, correct_group = -1;  // correct_velocity, this run only
   EnsembleConfig ens;
   int thermo_every = 0, restart_every = 0;
   std::vector<DumpXyz> xyz;
@@ -618,7 +739,7 @@ void run_directory(const std::string& dir, const std::string& run_file, const st
         break;
       case Cmd::velocity:
         if (!m.has_vel) {
-          random_velocities(m, c.a, c.has_seed ? (uint64_t)c.i : 12345678u);
+          random_velocities(m, c.a, c.has_seed, c.i);
           double* vel;
           md->arrays(nullptr, &vel, nullptr, nullptr, nullptr, nullptr, nullptr);
           hcheck(hipMemcpy(vel, m.vel.data(), m.vel.size() * 8, hipMemcpyHostToDevice), "velocity upload");
@@ -626,6 +747,11 @@ void run_directory(const std::string& dir, const std::string& run_file, const st
         break;
       case Cmd::time_step:
         dt_fs = c.a;
+        max_distance = c.b;
+        break;
+      case Cmd::correct_velocity:
+        correct_every = c.i;
+        correct_group = c.j;
         break;
       case Cmd::ensemble:
         ens = c.ens;
@@ -645,10 +771,7 @@ void run_directory(const std::string& dir, const std::string& run_file, const st
       case Cmd::run: {
         const int steps = c.i;
         const double dt_natural = dt_fs / kTimeUnit;
-        if ((ens.kind == Ensemble::nvt_bdp || ens.kind == Ensemble::nvt_nhc || ens.kind == Ensemble::npt_ber ||
-             ens.kind == Ensemble::npt_scr) &&
-            !(dt_fs > 0))
-          fail(c.at, "this ensemble requires a finite positive time_step");
+
         if (thermo_every) write_thermo_header(dir, thermo_every, m.N, dt_natural);
         int every = thermo_every;
         for (const DumpXyz& d : xyz) every = every ? std::gcd(every, d.interval) : d.interval;
@@ -656,8 +779,13 @@ void run_directory(const std::string& dir, const std::string& run_file, const st
         const double t0 = time_natural;
         int done_before = 0;
         double t_now = t0;
+        const bool adaptive = max_distance > 0.0;
         auto on_output = [&](int done, const double* th) {
-          for (int k = done_before; k < done; ++k) t_now += dt_natural;
+          if (adaptive) {
+            t_now = t0 + md->elapsed();
+          } else {
+            for (int k = done_before; k < done; ++k) t_now += dt_natural;
+          }
           done_before = done;
           if (md->force_calls() - 1 >= next_neighbor_call) {
             write_neighbor(dir, next_neighbor_call, nep);
@@ -676,7 +804,38 @@ void run_directory(const std::string& dir, const std::string& run_file, const st
         };
         if (steps > 0) {
           double th[8];
-          md->run(steps, dt_fs, ens, 0u, th, every > 0 ? every : steps, on_output);
+          MdState::RunOptions opt;
+          opt.every = every > 0 ? every : steps;
+          opt.on_output = on_output;
+          opt.max_distance = max_distance;
+          if (correct_every > 0) {
+            // correct_device_velocity (runtime.cu:620-656): per group of the
+            // chosen grouping method, or all atoms
+            opt.correct_every = correct_every;
+            opt.on_correct = [&](int) {
+              double *pos, *vel;
+              md->arrays(&pos, &vel, nullptr, nullptr, nullptr, nullptr, nullptr);
+              std::vector<double> hp(3 * (size_t)m.N), hv(3 * (size_t)m.N);
+              hcheck(hipMemcpy(hp.data(), pos, hp.size() * 8, hipMemcpyDeviceToHost), "position download");
+              hcheck(hipMemcpy(hv.data(), vel, hv.size() * 8, hipMemcpyDeviceToHost), "velocity download");
+              if (correct_group < 0) {
+                std::vector<int> all((size_t)m.N);
+                for (int i = 0; i < m.N; ++i) all[i] = i;
+                correct_velocity_subset(m.mass, hp.data(), hv.data(), m.N, all);
+              } else {
+                const int* lab = m.labels.data() + (size_t)correct_group * m.N;
+                const int gmax = *std::max_element(lab, lab + m.N);
+                for (int g = 0; g <= gmax; ++g) {
+                  std::vector<int> sub;
+                  for (int i = 0; i < m.N; ++i)
+                    if (lab[i] == g) sub.push_back(i);
+                  correct_velocity_subset(m.mass, hp.data(), hv.data(), m.N, sub);
+                }
+              }
+              hcheck(hipMemcpy(vel, hv.data(), hv.size() * 8, hipMemcpyHostToDevice), "velocity upload");
+            };
+          }
+          md->run(steps, dt_fs, ens, 0u, th, opt);
         } else {
           double th[8];
           md->run(0, dt_fs, ens, 0u, th);
@@ -685,10 +844,18 @@ void run_directory(const std::string& dir, const std::string& run_file, const st
           write_neighbor(dir, 0, nep);
           next_neighbor_call = 1000;
         }
-        for (int k = done_before; k < steps; ++k) t_now += dt_natural;
+        if (adaptive) {
+          t_now = t0 + (steps > 0 ? md->elapsed() : 0.0);
+        } else {
+          for (int k = done_before; k < steps; ++k) t_now += dt_natural;
+        }
         time_natural = t_now;
         thermo_every = restart_every = 0;
         xyz.clear();
+        // both runtimes clear these at each run boundary (runtime.cu:950-952)
+        max_distance = 0.0;
+        correct_every = 0;
+        correct_group = -1;
         break;
       }
     }

@@ -1781,6 +1781,30 @@ hipError_t launch_serve_extract(int R, const ServeOut* req, long long no, int Nt
   return hipGetLastError();
 }
 
+// largest |v|^2: a wavefront maximum, then one atomicMax per wavefront on
+// the bit pattern (non-negative doubles order as their bits)
+__global__ void k_max_speed2(int N, const double* __restrict__ vel, unsigned long long* __restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v2 = 0.0;
+  if (i < N) {
+    const double x = vel[i], y = vel[N + i], z = vel[2 * N + i];
+    v2 = x * x + y * y + z * z;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v2 = fmax(v2, __shfl_xor(v2, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v2));
+}
+
+__global__ void k_clear_u64(unsigned long long* p) { *p = 0ull; }
+
+hipError_t launch_max_speed2(int N, const double* vel, unsigned long long* out, hipStream_t s)
+{
+  k_clear_u64<<<1, 1, 0, s>>>(out);
+  if (N > 0) k_max_speed2<<<nblk(N), kBlock, 0, s>>>(N, vel, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_check_scaled(int n, double factor, const double* data, int* flag, hipStream_t s)
 {
   if (n > 0) k_check_scaled<<<nblk(n), kBlock, 0, s>>>(n, factor, data, flag);

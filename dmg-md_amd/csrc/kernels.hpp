@@ -144,6 +144,8 @@ hipError_t launch_wrap(const DevBox& b, int N, double* pos, hipStream_t s);
 hipError_t launch_thermo(int N, double volume, const double* mass, const double* pe, const double* vel,
                          const double* virial, DevWork& w, hipStream_t s);
 hipError_t launch_scale(int n, double factor, double* data, hipStream_t s);
+// out[0] = bit pattern of max_i |v_i|^2 (velocities SoA [3N])
+hipError_t launch_max_speed2(int N, const double* vel, unsigned long long* out, hipStream_t s);
 // Device-side velocity factor from the thermo of this step (no host read):
 // kind 1 Berendsen sqrt(1 + c (T_t/T - 1)); kind 2 Bussi with the step's
 // pre-drawn Gaussian rr and chi-square sum R (the same expression and

@@ -19,13 +19,22 @@ def _run(dmgx, tmp_path, text):
     ("potential nep.txt\nensemble nvt_ber 300 300\n", "should have 3 parameters"),
     ("potential nep.txt\nensemble nvt_ber 300 300 0.5\n", "temperature coupling should be >= 1"),
     ("potential nep.txt\nensemble npt_ber 300 300 100 0 0 1000\n", "bulk modulus should be > 0"),
-    ("potential nep.txt\ntime_step 1 0.1\n", "adaptive"),
+    ("potential nep.txt\ntime_step 1 0\n", "maximum distance per step should be > 0"),
+    ("potential nep.txt\ntime_step 1 0.1 3\n", "time_step should have 1 or 2 parameters"),
+    ("potential nep.txt\ncorrect_velocity 5\n", "velocity correction interval should be >= 10"),
+    ("potential nep.txt\ncorrect_velocity 10 -1\n", "grouping method should be >= 0"),
+    ("potential nep.txt\ncorrect_velocity\n", "correct_velocity should have 1 or 2 parameters"),
+    ("potential nep.txt\nensemble nvt_bdp 300 300 100\ncorrect_velocity 10\nrun 10\n",
+     "ensemble nvt_bdp does not support correct_velocity"),
+    ("potential nep.txt\ntime_step 1 0.05\nensemble nvt_nhc 300 300 100\nrun 10\n",
+     "ensemble nvt_nhc does not support adaptive time_step"),
+    ("potential nep.txt\ntime_step 1 0.05\nensemble npt_scr 300 300 100 0 100 1000\nrun 10\n",
+     "does not support adaptive time_step"),
     ("potential nep.txt\ndump_thermo 0\n", "thermo dump interval should be > 0"),
     ("potential nep.txt\ndump_xyz 10 out.xyz precision quad\n", "invalid dump_xyz precision"),
     ("potential nep.txt\ndump_xyz 10 out.xyz bogus\n", "unrecognized dump_xyz argument"),
     ("potential nep.txt\nrun ten\n", "number of steps should be an integer"),
     ("potential a.txt\npotential b.txt\n", "multiple potential commands"),
-    ("potential nep.txt\ncorrect_velocity 100\n", "correct_velocity"),
 ])
 def test_run_in_errors(dmgx, tmp_path, text, needle):
     st, msg = _run(dmgx, tmp_path, text)

@@ -150,3 +150,168 @@ def test_golden_case_replays_end_to_end(dmgx, tmp_path, case):
                 assert open(got).read() == open(ref).read(), name
         prev = d
     assert prev is not None
+
+
+# ---------------------------------------------------------------------------
+# velocity / correct_velocity / adaptive time_step through the front end
+# ---------------------------------------------------------------------------
+
+def _c0_dir(tmp_path, run_in, n_cells=2, vel=None, groups=None, seed=3):
+    """A jittered carbon diamond model.xyz (c0 model) with optional velocities
+    (A/fs) and one grouping method, plus run.in."""
+    from conftest import POTENTIALS
+    shutil.copy(os.path.join(POTENTIALS, "nep_c0_carbon.txt"), tmp_path / "nep.txt")
+    a = 3.567
+    frac = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
+                     [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
+    cells = np.array([[i, j, k] for i in range(n_cells) for j in range(n_cells) for k in range(n_cells)])
+    pos = ((cells[:, None, :] + frac[None]).reshape(-1, 3) * a)
+    pos = pos + np.random.default_rng(seed).normal(0, 0.05, pos.shape)
+    L = a * n_cells
+    props = ("species:S:1:pos:R:3:mass:R:1" + (":vel:R:3" if vel is not None else "")
+             + (":group:I:1" if groups is not None else ""))
+    lines = [str(len(pos)), f'pbc="T T T" Lattice="{L} 0 0 0 {L} 0 0 0 {L}" Properties={props}']
+    for i, p in enumerate(pos):
+        row = "C " + " ".join(f"{x:.10f}" for x in p) + " 12.011"
+        if vel is not None:
+            row += " " + " ".join(f"{x:.12e}" for x in vel[i])
+        if groups is not None:
+            row += f" {groups[i]}"
+        lines.append(row)
+    (tmp_path / "model.xyz").write_text("\n".join(lines) + "\n")
+    (tmp_path / "run.in").write_text(run_in)
+    return pos
+
+
+def _rigid_residual(v_from, v_to, pos, mass, scale=False):
+    """Least-squares residual of v_from = (alpha) v_to + u + w x (r - c):
+    zero when v_from and v_to differ by a rigid motion (and a scale)."""
+    c = (mass[:, None] * pos).sum(0) / mass.sum()
+    d = pos - c
+    n = len(pos)
+    cols = []
+    if scale:
+        cols.append(v_to.reshape(-1))
+    for k in range(3):
+        e = np.zeros((n, 3))
+        e[:, k] = 1.0
+        cols.append(e.reshape(-1))
+    for k in range(3):
+        w = np.zeros(3)
+        w[k] = 1.0
+        cols.append(np.cross(w, d).reshape(-1))
+    A = np.stack(cols, 1)
+    x, *_ = np.linalg.lstsq(A, v_from.reshape(-1), rcond=None)
+    return float(np.abs(A @ x - v_from.reshape(-1)).max()), x
+
+
+def _momenta(vel, pos, mass):
+    c = (mass[:, None] * pos).sum(0) / mass.sum()
+    P = (mass[:, None] * vel).sum(0)
+    Lm = (mass[:, None] * np.cross(pos - c, vel)).sum(0)
+    return P, Lm
+
+
+def test_velocity_command_draw_correction_and_temperature(dmgx, tmp_path):
+    """`velocity 300 seed 7` (runtime.cu:1078-1109): uniform draws from the C
+    library's rand() seeded per component with 7 + 3 i + axis, momentum and
+    rotation removed, scaled to 300 K over 3N degrees of freedom."""
+    import ctypes
+    import importlib
+    cli = importlib.import_module("dmg-md_amd.cli")
+    pos = _c0_dir(tmp_path, "potential nep.txt\nvelocity 300 seed 7\ntime_step 0\nensemble nve\n"
+                            "dump_restart 1\nrun 1\n")
+    cli.run_directory(str(tmp_path), "run.in", "model.xyz")
+    from xyz_io import read_xyz, TIME_UNIT_CONVERSION as TU
+    r = read_xyz(str(tmp_path / "restart.xyz"))
+    # the draw is corrected about the model's positions as read (unwrapped)
+    v, m = r["vel"], r["mass"]
+    n = len(m)
+    P, Lm = _momenta(v, pos, m)
+    assert np.abs(P).max() < 1e-12 and np.abs(Lm).max() < 1e-10, (P, Lm)
+    T = (m[:, None] * (v * TU) ** 2).sum() / (3 * n * 8.617343e-5)
+    assert abs(T - 300.0) < 1e-6, T
+    libc = ctypes.CDLL(None)
+    rand_max = 2147483647
+    draw = np.zeros((n, 3))
+    for i in range(n):
+        for d in range(3):
+            libc.srand(ctypes.c_uint(7 + 3 * i + d))
+            draw[i, d] = -1.0 + libc.rand() * 2.0 / rand_max
+    res, x = _rigid_residual(draw, v, pos, m, scale=True)
+    assert res < 1e-9, res
+
+
+def test_correct_velocity_removes_momentum_per_group(dmgx, tmp_path):
+    """correct_velocity 10 0 with time_step 0: before step 0 each group's
+    velocities lose exactly a rigid motion, leaving zero linear and angular
+    momentum per group (runtime.cu:620-656, 1002-1076)."""
+    import importlib
+    cli = importlib.import_module("dmg-md_amd.cli")
+    rng = np.random.default_rng(9)
+    n = 64
+    vel = rng.normal(0, 0.01, (n, 3)) + np.array([0.004, -0.002, 0.003])
+    groups = (np.arange(n) % 3 == 0).astype(int)
+    pos = _c0_dir(tmp_path, "potential nep.txt\ntime_step 0\nensemble nve\ncorrect_velocity 10 0\n"
+                            "dump_restart 1\nrun 1\n", vel=vel, groups=groups)
+    cli.run_directory(str(tmp_path), "run.in", "model.xyz")
+    from xyz_io import read_xyz
+    r = read_xyz(str(tmp_path / "restart.xyz"))
+    # the correction acts on the device positions, wrapped into the box (with
+    # time_step 0 those are the restart's)
+    v, m, p = r["vel"], r["mass"], r["pos"]
+    for g in (0, 1):
+        s = groups == g
+        P, Lm = _momenta(v[s], p[s], m[s])
+        assert np.abs(P).max() < 1e-12 and np.abs(Lm).max() < 1e-10, (g, P, Lm)
+        res, _ = _rigid_residual(vel[s] - v[s], np.zeros_like(v[s]), p[s], m[s])
+        assert res < 1e-12, (g, res)
+
+
+def test_adaptive_time_step_matches_oracle_replay(dmgx, tmp_path):
+    """`time_step 1 0.005`: each step's dt is min(1 fs, 0.005 A / max |v|)
+    over the step's starting velocities (runtime.cu:594-618); positions,
+    velocities and the dump's Time against an FP64 oracle replay with the same
+    rule."""
+    import importlib
+    import oracle as O
+    from conftest import POTENTIALS
+    from oracle import OracleNEP
+    from xyz_io import read_xyz, TIME_UNIT_CONVERSION as TU
+    cli = importlib.import_module("dmg-md_amd.cli")
+    rng = np.random.default_rng(4)
+    n = 64
+    vel = rng.normal(0, 0.012, (n, 3))
+    vel -= vel.mean(0)
+    steps = 6
+    pos = _c0_dir(tmp_path, "potential nep.txt\ntime_step 1 0.005\nensemble nve\n"
+                            f"dump_xyz 1 traj.xyz precision double velocity\nrun {steps}\n", vel=vel)
+    cli.run_directory(str(tmp_path), "run.in", "model.xyz")
+    orc = OracleNEP(os.path.join(POTENTIALS, "nep_c0_carbon.txt"), round_params_to_float=True)
+    L = 3.567 * 2
+    h = np.diag([L, L, L])
+    mass = np.full(n, 12.011)
+    t = np.zeros(n, np.int32)
+    x = np.ascontiguousarray(pos.T)
+    v = np.ascontiguousarray(vel.T) * TU
+    O.wrap_positions(h, (1, 1, 1), x)
+    f = np.ascontiguousarray(orc.compute(h, (1, 1, 1), t, x)["force"])
+    time = 0.0
+    reduced = 0
+    for k in range(steps):
+        vmax = np.sqrt((v * v).sum(0).max())
+        dt = min(1.0 / TU, 0.005 / vmax)
+        reduced += dt < 1.0 / TU
+        time += dt
+        O.velocity_verlet(True, dt, mass, x, v, f)
+        O.wrap_positions(h, (1, 1, 1), x)
+        f = np.ascontiguousarray(orc.compute(h, (1, 1, 1), t, x)["force"])
+        O.velocity_verlet(False, dt, mass, x, v, f)
+        g = read_xyz(str(tmp_path / "traj.xyz"), frame=k)
+        # Time is printed with 8 decimals (the manifest's time tolerance)
+        assert abs(float(g["header"]["time"]) - time * TU) < 5e-9, (k, g["header"]["time"], time * TU)
+        d = g["pos"] - x.T
+        d -= L * np.round(d / L)
+        assert np.abs(d).max() < 1e-7, (k, np.abs(d).max())
+        assert np.abs(g["vel"] - v.T / TU).max() < 2e-8, (k, np.abs(g["vel"] - v.T / TU).max())
+    assert reduced == steps  # every step was limited by the distance bound
