@@ -658,6 +658,8 @@ MdState::~MdState()
   dfree(&tw_.partial);
   dfree(&tw_.thermo);
   dfree(&vmax2_);
+  dfree(&unwrapped_);
+  dfree(&ck_unwrapped_);
   dfree(&ck_pos_);
   dfree(&ck_vel_);
   dfree(&ck_force_);
@@ -716,8 +718,18 @@ void MdState::vv(int first_half, double dt)
 {
   Profiler& p = nep_->profiler();
   p.begin(kProfIntegrate, stream_);
-  check(launch_velocity_verlet(first_half, N_, dt, mass_, pos_, vel_, force_, stream_), "velocity Verlet");
+  check(launch_velocity_verlet(first_half, N_, dt, mass_, pos_, vel_, force_, stream_, unwrapped_),
+        "velocity Verlet");
   p.end(stream_);
+}
+
+void MdState::enable_unwrapped()
+{
+  if (unwrapped_) return;
+  dalloc(&unwrapped_, 3 * (size_t)N_, "unwrapped positions");
+  check(hipMemcpyAsync(unwrapped_, pos_, 3 * (size_t)N_ * sizeof(double), hipMemcpyDeviceToDevice, stream_),
+        "unwrapped positions");
+  check(hipStreamSynchronize(stream_), "unwrapped positions");
 }
 
 void MdState::checkpoint(bool save)
@@ -726,9 +738,10 @@ void MdState::checkpoint(bool save)
     double* live;
     double** copy;
     size_t n;
-  } buf[5] = {{pos_, &ck_pos_, 3}, {vel_, &ck_vel_, 3}, {force_, &ck_force_, 3}, {pe_, &ck_pe_, 1},
-              {vir_, &ck_vir_, 9}};
+  } buf[6] = {{pos_, &ck_pos_, 3}, {vel_, &ck_vel_, 3}, {force_, &ck_force_, 3}, {pe_, &ck_pe_, 1},
+              {vir_, &ck_vir_, 9}, {unwrapped_, &ck_unwrapped_, 3}};
   for (auto& e : buf) {
+    if (!e.live) continue;
     const size_t bytes = e.n * (size_t)N_ * sizeof(double);
     if (!*e.copy) dalloc(e.copy, e.n * (size_t)N_, "checkpoint");
     if (save) check(hipMemcpyAsync(*e.copy, e.live, bytes, hipMemcpyDeviceToDevice, stream_), "checkpoint");
@@ -858,7 +871,7 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
         Profiler& p = nep_->profiler();
         p.begin(kProfIntegrate, stream_);
         check(launch_md_first(plan.box, N_, nep_->model().num_types, dt, mass_, pos_, vel_, force_,
-                              pending ? factor_ : nullptr, type_, plan.forced, nep_->work(), stream_),
+                              pending ? factor_ : nullptr, type_, plan.forced, nep_->work(), stream_, unwrapped_),
               "velocity Verlet (first half)");
         p.end(stream_);
         nep_->launch_force(plan, N_, 0, N_, 0, N_, type_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);

@@ -170,6 +170,12 @@ class MdState {
            const RunOptions& opt);
   // natural time units advanced by the last run (sum of its steps' dt)
   double elapsed() const { return elapsed_; }
+  // Tracks unwrapped positions from now on (starting from the current
+  // positions; a second call keeps the running values): every drift is also
+  // added to them (AtomsState::enable_unwrapped, update_unwrapped,
+  // runtime.cu:80-90,423-440).  unwrapped() is null until enabled.
+  void enable_unwrapped();
+  const double* unwrapped() const { return unwrapped_; }
   long long force_calls() const { return force_calls_; }
   void arrays(double** pos, double** vel, double** force, double** pe, double** vir, double** mass, int** type);
   const Box& box() const { return box_; }
@@ -195,6 +201,8 @@ class MdState {
   double* factor_ = nullptr;  // device velocity factor
   long long force_calls_ = 0;
   unsigned long long* vmax2_ = nullptr;  // device: bits of the largest |v|^2 (adaptive step)
+  double* unwrapped_ = nullptr;          // device [3N] when enabled
+  double* ck_unwrapped_ = nullptr;
   double elapsed_ = 0.0;
 };
 

@@ -95,7 +95,8 @@ struct DumpXyz {
   std::string file;
   bool dbl = false;
   bool mass = false, charge = false, velocity = false, force = false, potential = false, virial = false,
-       labels = false;
+       labels = false, unwrapped = false;
+  int gmethod = -1, gid = -1;  // `group method id`: only that group's atoms
 };
 
 struct Cmd {
@@ -211,8 +212,15 @@ Cmd parse_command(const std::vector<std::string>& t, const Where& at)
         c.xyz.virial = true;
       } else if (w == "group_labels") {
         c.xyz.labels = true;
-      } else if (w == "group" || w == "unwrapped_position") {
-        fail(at, "dump_xyz option '" + w + "' is not supported by this build");
+      } else if (w == "unwrapped_position") {
+        c.xyz.unwrapped = true;
+      } else if (w == "group") {
+        if (c.xyz.gmethod >= 0) fail(at, "option 'group' is specified more than once in dump_xyz");
+        if (q + 2 >= t.size()) fail(at, "option 'group' requires a grouping method and group ID");
+        c.xyz.gmethod = to_int(t[q + 1], at, "grouping method");
+        c.xyz.gid = to_int(t[q + 2], at, "group ID");
+        if (c.xyz.gmethod < 0 || c.xyz.gid < 0) fail(at, "grouping method and group ID should be >= 0");
+        q += 2;
       } else {
         fail(at, "unrecognized dump_xyz argument '" + w + "'");
       }
@@ -514,7 +522,7 @@ void random_velocities(Model& m, double T, bool has_seed, int seed)
 // ---------------------------------------------------------------------------
 
 struct Snapshot {
-  std::vector<double> pos, vel, force, pe, vir;
+  std::vector<double> pos, vel, force, pe, vir, unwrapped;
 };
 
 void write_tensor(FILE* f, const char* name, const char* fmt, const double* v)
@@ -566,7 +574,13 @@ void write_xyz(const std::string& dir, const DumpXyz& d, int steps_done, double 
   const char* fmt = d.dbl ? " %.17g" : " %.9g";
   const int N = m.N;
   const double* h = box.h;
-  std::fprintf(f, "%d\nTime=%.8f", N, time_natural * kTimeUnit);
+  // `group method id`: only that group's atoms (the header sums stay over all
+  // atoms; output_order, runtime.cu:1111-1137)
+  std::vector<int> order;
+  const int* lab = d.gmethod >= 0 ? m.labels.data() + (size_t)d.gmethod * N : nullptr;
+  for (int i = 0; i < N; ++i)
+    if (!lab || lab[i] == d.gid) order.push_back(i);
+  std::fprintf(f, "%zu\nTime=%.8f", order.size(), time_natural * kTimeUnit);
   std::fprintf(f, " pbc=\"%c %c %c\"", box.pbc[0] ? 'T' : 'F', box.pbc[1] ? 'T' : 'F', box.pbc[2] ? 'T' : 'F');
   const double lat[9] = {h[0], h[3], h[6], h[1], h[4], h[7], h[2], h[5], h[8]};
   write_tensor(f, "Lattice", fmt, lat);
@@ -585,12 +599,13 @@ void write_xyz(const std::string& dir, const DumpXyz& d, int steps_done, double 
   if (d.velocity) std::fprintf(f, ":vel:R:3");
   if (d.force) std::fprintf(f, ":forces:R:3");
   if (d.potential) std::fprintf(f, ":energy_atom:R:1");
+  if (d.unwrapped) std::fprintf(f, ":unwrapped_position:R:3");
   if (d.virial) std::fprintf(f, ":virial:R:9");
   if (d.labels) std::fprintf(f, ":group:I:%d", m.n_label);
   std::fprintf(f, "\n");
   // per-atom virial as a row-major 3x3 from the xx yy zz xy xz yz yx zx zy order
   const int vi[9] = {0, 3, 4, 6, 1, 5, 7, 8, 2};
-  for (int i = 0; i < N; ++i) {
+  for (int i : order) {
     std::fprintf(f, "%s", m.species[i].c_str());
     for (int c = 0; c < 3; ++c) std::fprintf(f, fmt, s.pos[(size_t)c * N + i]);
     if (d.mass) std::fprintf(f, fmt, m.mass[i]);
@@ -600,6 +615,8 @@ void write_xyz(const std::string& dir, const DumpXyz& d, int steps_done, double 
     if (d.force)
       for (int c = 0; c < 3; ++c) std::fprintf(f, fmt, s.force[(size_t)c * N + i]);
     if (d.potential) std::fprintf(f, fmt, s.pe[i]);
+    if (d.unwrapped)
+      for (int c = 0; c < 3; ++c) std::fprintf(f, fmt, s.unwrapped[(size_t)c * N + i]);
     if (d.virial)
       for (int c : vi) std::fprintf(f, fmt, s.vir[(size_t)c * N + i]);
     if (d.labels)
@@ -701,6 +718,10 @@ Snapshot snapshot(MdState& md, int N)
   hcheck(hipMemcpy(s.force.data(), force, s.force.size() * 8, hipMemcpyDeviceToHost), "snapshot");
   hcheck(hipMemcpy(s.pe.data(), pe, s.pe.size() * 8, hipMemcpyDeviceToHost), "snapshot");
   hcheck(hipMemcpy(s.vir.data(), vir, s.vir.size() * 8, hipMemcpyDeviceToHost), "snapshot");
+  if (md.unwrapped()) {
+    s.unwrapped.resize(3 * (size_t)N);
+    hcheck(hipMemcpy(s.unwrapped.data(), md.unwrapped(), s.unwrapped.size() * 8, hipMemcpyDeviceToHost), "snapshot");
+  }
   return s;
 }
 
@@ -766,6 +787,12 @@ void run_directory(const std::string& dir, const std::string& run_file, const st
         break;
       case Cmd::dump_xyz:
         if (c.xyz.labels && m.n_label == 0) fail(c.at, "cannot output group labels without model groups");
+        if (c.xyz.gmethod >= 0) {
+          if (c.xyz.gmethod >= m.n_label) fail(c.at, "dump_xyz grouping method is out of range");
+          const int* lab = m.labels.data() + (size_t)c.xyz.gmethod * m.N;
+          if (c.xyz.gid > *std::max_element(lab, lab + m.N)) fail(c.at, "dump_xyz group ID is out of range");
+        }
+        if (c.xyz.unwrapped) md->enable_unwrapped();
         xyz.push_back(c.xyz);
         break;
       case Cmd::run: {

@@ -1072,9 +1072,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
 // integrator and thermo (FP64)
 // ---------------------------------------------------------------------------
 
+// (the first half also adds the drift to `unwrapped` when given:
+// update_unwrapped, runtime.cu:423-440)
 __global__ void k_velocity_verlet(int first_half, int N, double dt, const double* __restrict__ mass,
                                   double* __restrict__ pos, double* __restrict__ vel,
-                                  const double* __restrict__ force)
+                                  const double* __restrict__ force, double* __restrict__ unwrapped)
 {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1085,7 +1087,11 @@ __global__ void k_velocity_verlet(int first_half, int N, double dt, const double
   for (int c = 0; c < 3; ++c) {
     const double v = vel[c * N + i] + force[c * N + i] * im * half;
     vel[c * N + i] = v;
-    if (first_half) pos[c * N + i] += v * dt;
+    if (first_half) {
+      const double x0 = pos[c * N + i], x1 = x0 + v * dt;
+      pos[c * N + i] = x1;
+      if (unwrapped) unwrapped[c * N + i] += x1 - x0;
+    }
   }
 }
 
@@ -1121,7 +1127,7 @@ __global__ __launch_bounds__(kBlock) void k_md_first(DevBox b, int N, int T, dou
                                                      const double* __restrict__ factor,
                                                      const int* __restrict__ type, const double* __restrict__ x0,
                                                      double lim2, int forced, double4* __restrict__ xq,
-                                                     int* __restrict__ flags)
+                                                     int* __restrict__ flags, double* __restrict__ unwrapped)
 {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1137,7 +1143,9 @@ __global__ __launch_bounds__(kBlock) void k_md_first(DevBox b, int N, int T, dou
       if (factor) v *= factor[0];
       v = v + force[c * N + i] * im * half;
       vel[c * N + i] = v;
-      x[c] = pos[c * N + i] + v * dt;
+      const double p0 = pos[c * N + i];
+      x[c] = p0 + v * dt;
+      if (unwrapped) unwrapped[c * N + i] += x[c] - p0;
     }
     double sf[3] = {b.hi[0] * x[0] + b.hi[1] * x[1] + b.hi[2] * x[2], b.hi[3] * x[0] + b.hi[4] * x[1] + b.hi[5] * x[2],
                     b.hi[6] * x[0] + b.hi[7] * x[1] + b.hi[8] * x[2]};
@@ -1470,9 +1478,9 @@ hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int 
 }
 
 hipError_t launch_velocity_verlet(int first_half, int N, double dt, const double* mass, double* pos,
-                                  double* vel, const double* force, hipStream_t s)
+                                  double* vel, const double* force, hipStream_t s, double* unwrapped)
 {
-  if (N > 0) k_velocity_verlet<<<nblk(N), kBlock, 0, s>>>(first_half, N, dt, mass, pos, vel, force);
+  if (N > 0) k_velocity_verlet<<<nblk(N), kBlock, 0, s>>>(first_half, N, dt, mass, pos, vel, force, unwrapped);
   return hipGetLastError();
 }
 
@@ -1507,12 +1515,12 @@ hipError_t launch_ensemble_scale(int kind, int N, double coupling, double t_targ
 
 hipError_t launch_md_first(const DevBox& b, int N, int T, double dt, const double* mass, double* pos, double* vel,
                            const double* force, const double* factor, const int* type, bool forced, DevWork& w,
-                           hipStream_t s)
+                           hipStream_t s, double* unwrapped)
 {
   const double half = 0.5 * kSkin;
   if (N > 0)
     k_md_first<<<nblk(N), kBlock, 0, s>>>(b, N, T, dt, mass, pos, vel, force, factor, type, w.x0, half * half,
-                                          forced ? 1 : 0, w.xq, w.flags);
+                                          forced ? 1 : 0, w.xq, w.flags, unwrapped);
   return hipGetLastError();
 }
 

@@ -138,8 +138,9 @@ hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int 
                          double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s);
 
 // Integrator / thermo
+// first half: kick + drift (and the drift added to `unwrapped` when given)
 hipError_t launch_velocity_verlet(int first_half, int N, double dt, const double* mass, double* pos,
-                                  double* vel, const double* force, hipStream_t s);
+                                  double* vel, const double* force, hipStream_t s, double* unwrapped = nullptr);
 hipError_t launch_wrap(const DevBox& b, int N, double* pos, hipStream_t s);
 hipError_t launch_thermo(int N, double volume, const double* mass, const double* pe, const double* vel,
                          const double* virial, DevWork& w, hipStream_t s);
@@ -162,7 +163,7 @@ hipError_t launch_check_scaled(int n, double factor, const double* data, int* fl
 // thermostat into factor[0] (see launch_ensemble_scale for kind).
 hipError_t launch_md_first(const DevBox& b, int N, int T, double dt, const double* mass, double* pos, double* vel,
                            const double* force, const double* factor, const int* type, bool forced, DevWork& w,
-                           hipStream_t s);
+                           hipStream_t s, double* unwrapped = nullptr);
 hipError_t launch_md_second(int N, double dt, const double* mass, double* vel, const double* force, const double* pe,
                             const double* virial, bool thermo, DevWork& tw, hipStream_t s);
 hipError_t launch_thermo_final(int N, double volume, int kind, double coupling, double t_target,

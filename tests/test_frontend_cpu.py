@@ -24,6 +24,9 @@ def _run(dmgx, tmp_path, text):
     ("potential nep.txt\ncorrect_velocity 5\n", "velocity correction interval should be >= 10"),
     ("potential nep.txt\ncorrect_velocity 10 -1\n", "grouping method should be >= 0"),
     ("potential nep.txt\ncorrect_velocity\n", "correct_velocity should have 1 or 2 parameters"),
+    ("potential nep.txt\ndump_xyz 1 a.xyz group 0\n", "option 'group' requires a grouping method and group ID"),
+    ("potential nep.txt\ndump_xyz 1 a.xyz group 0 -1\n", "grouping method and group ID should be >= 0"),
+    ("potential nep.txt\ndump_xyz 1 a.xyz group 0 1 group 0 1\n", "option 'group' is specified more than once"),
     ("potential nep.txt\nensemble nvt_bdp 300 300 100\ncorrect_velocity 10\nrun 10\n",
      "ensemble nvt_bdp does not support correct_velocity"),
     ("potential nep.txt\ntime_step 1 0.05\nensemble nvt_nhc 300 300 100\nrun 10\n",

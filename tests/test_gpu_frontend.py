@@ -315,3 +315,38 @@ def test_adaptive_time_step_matches_oracle_replay(dmgx, tmp_path):
         assert np.abs(d).max() < 1e-7, (k, np.abs(d).max())
         assert np.abs(g["vel"] - v.T / TU).max() < 2e-8, (k, np.abs(g["vel"] - v.T / TU).max())
     assert reduced == steps  # every step was limited by the distance bound
+
+
+def test_dump_xyz_group_filter_and_unwrapped_positions(dmgx, tmp_path):
+    """`dump_xyz ... group 0 1 unwrapped_position`: only group 1's atoms are
+    written, and each unwrapped position differs from the wrapped one by whole
+    box vectors while changing smoothly from the raw input coordinates
+    (update_unwrapped, runtime.cu:423-440; output_order, runtime.cu:1111-1137)."""
+    import importlib
+    from xyz_io import read_xyz
+    cli = importlib.import_module("dmg-md_amd.cli")
+    rng = np.random.default_rng(12)
+    n = 64
+    vel = rng.normal(0, 0.02, (n, 3))
+    vel -= vel.mean(0)
+    groups = (np.arange(n) % 4 == 1).astype(int)
+    steps = 8
+    pos = _c0_dir(tmp_path, "potential nep.txt\ntime_step 1\nensemble nve\n"
+                            "dump_xyz 1 g.xyz precision double group 0 1 unwrapped_position velocity\n"
+                            f"run {steps}\n", vel=vel, groups=groups)
+    cli.run_directory(str(tmp_path), "run.in", "model.xyz")
+    L = 3.567 * 2
+    sel = np.flatnonzero(groups == 1)
+    prev = pos[sel]
+    crossed = 0
+    for k in range(steps):
+        g = read_xyz(str(tmp_path / "g.xyz"), frame=k)
+        assert g["n"] == len(sel)
+        u, p = g["unwrapped_position"], g["pos"]
+        shift = (u - p) / L
+        assert np.abs(shift - np.round(shift)).max() < 1e-9, k
+        crossed += int(np.abs(np.round(shift)).sum())
+        step = np.abs(u - prev).max()
+        assert step < 0.1, (k, step)  # |v| dt with |v| < 0.1 A/fs
+        prev = u
+    assert crossed > 0  # some atoms of the group sit across a face
