@@ -272,8 +272,10 @@ def main():
     print(f"bench: Cu fcc {args.cells}^3 = {N} atoms, warmup {args.warmup} steps", file=sys.stderr, flush=True)
     md.run(args.warmup, 2.0, **ens)
     torch.cuda.synchronize()
+    # timed region: only the dominant kernel is timed, with HIP events
+    # attached to its own dispatch (no extra packets in the stream)
     nep.profile_reset()
-    nep.set_profiling(True)
+    nep.set_profiling(True, dispatch_only=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     th = md.run(args.steps, 2.0, **ens)
@@ -289,7 +291,14 @@ def main():
     n_fused, ms_fused = prof["descriptor"]
     avg_fused = ms_fused / max(n_fused, 1)
     achieved = fl["fused"] * N / (avg_fused * 1e-3) / 1e12
-    kernel_ms = {k: round(v[1] / max(v[0], 1), 4) for k, v in prof.items()}
+    # per-kernel breakdown from a separate, untimed profiled stretch (events
+    # around every kernel group separate the kernels by a few microseconds)
+    nep.profile_reset()
+    nep.set_profiling(True)
+    md.run(min(args.steps, 100), 2.0, **ens)
+    torch.cuda.synchronize()
+    nep.set_profiling(False)
+    kernel_ms = {k: round(v[1] / max(v[0], 1), 4) for k, v in nep.profile().items()}
     print(f"bench: {args.steps} steps in {elapsed:.3f} s, T={th[0]:.1f} K, rebuilds={stats['rebuilds']}, "
           f"per-kernel avg ms {kernel_ms}, mean rows {means}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
@@ -317,6 +326,7 @@ def main():
             "mean_neighbors": {k: round(v, 2) for k, v in means.items() if k != "centres"},
             "verlet_rebuilds_in_run": stats["rebuilds"],
             "kernel_avg_ms": kernel_ms,
+            "kernel_avg_ms_from": "a separate profiled stretch after the timed steps",
             "final_temperature_K": th[0],
         },
         "roofline": {

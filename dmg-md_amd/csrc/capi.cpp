@@ -170,7 +170,8 @@ int dmgx_nep_set_profiling(dmgx_nep* nep, int enable)
 {
   return guarded([&] {
     if (!nep) throw std::invalid_argument("null handle");
-    nep->engine.profiler().enable(enable != 0);
+    if (enable < 0 || enable > 2) throw std::invalid_argument("profiling mode is 0, 1 or 2");
+    nep->engine.profiler().enable(enable);
   });
 }
 

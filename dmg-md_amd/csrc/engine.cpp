@@ -98,7 +98,7 @@ hipEvent_t Profiler::take()
 
 void Profiler::begin(int kernel, hipStream_t s)
 {
-  if (!on_) return;
+  if (mode_ != 1) return;
   Rec r{kernel, take(), nullptr};
   check(hipEventRecord(r.a, s), "event record");
   recs_.push_back(r);
@@ -107,10 +107,21 @@ void Profiler::begin(int kernel, hipStream_t s)
 
 void Profiler::end(hipStream_t s)
 {
-  if (!on_ || open_ < 0) return;
+  if (mode_ != 1 || open_ < 0) return;
   recs_[open_].b = take();
   check(hipEventRecord(recs_[open_].b, s), "event record");
   open_ = -1;
+}
+
+bool Profiler::attach(int kernel, hipEvent_t* start, hipEvent_t* stop)
+{
+  *start = *stop = nullptr;
+  if (mode_ != 2 || kernel != kProfDescriptor) return false;
+  Rec r{kernel, take(), take()};
+  recs_.push_back(r);
+  *start = r.a;
+  *stop = r.b;
+  return true;
 }
 
 void Profiler::collect()
@@ -358,7 +369,8 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
     const long long lim = (long long)(INT32_MAX - 1) / std::max(w_.MV, 128);
     reserve((int)std::max<long long>(N, std::min<long long>(2LL * w_.cap, lim)));
   }
-  ensure_cells((int)std::max<long long>(ncell, std::min<long long>(2LL * w_.ncell_cap, 1LL << 28)));
+  if (ncell > w_.ncell_cap)
+    ensure_cells((int)std::max<long long>(ncell, std::min<long long>(2LL * w_.ncell_cap, 1LL << 28)));
   p.forced = !have_list_ || box_changed_ || N != last_N_ || nd1 != last_nd1_ || nd2 != last_nd2_ ||
              by_gid != last_gid_sort_ || (flags & 2u) || std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
              std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0 || im.owner != last_map_.owner ||
@@ -391,11 +403,13 @@ void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int 
   prof_.begin(kProfFilter, s);
   check(launch_filter(p.box, dm_, nd1, nd2, w_, s), "filter");
   prof_.end(s);
-  prof_.begin(kProfDescriptor, s);
+  hipEvent_t ev0, ev1;
+  const bool attached = prof_.attach(kProfDescriptor, &ev0, &ev1);
+  if (!attached) prof_.begin(kProfDescriptor, s);
   // ANN: FP32 on the matrix cores unless BF16 operands or the vector path are asked for
   const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
-  check(launch_descriptor(dm_, N, nd1, nd2, type, pe, acc, ann, w_, s), "descriptor");
-  prof_.end(s);
+  check(launch_descriptor(dm_, N, nd1, nd2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
+  if (!attached) prof_.end(s);
   prof_.begin(kProfGather, s);
   check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s), "gather");
   prof_.end(s);

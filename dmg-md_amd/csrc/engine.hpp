@@ -35,18 +35,22 @@ enum ProfKernel { kProfFilter = 0, kProfDescriptor, kProfGather, kProfRebuild, k
 class Profiler {
  public:
   ~Profiler();
-  void enable(bool on) { on_ = on; }
-  bool on() const { return on_; }
-  // returns an event index pair to record around a launch
+  // 0 off, 1 events recorded around every kernel group, 2 the fused
+  // descriptor kernel only, with events attached to its dispatch
+  void enable(int mode) { mode_ = mode; }
+  bool on() const { return mode_ != 0; }
+  // mode 1: event pair recorded around a launch (group)
   void begin(int kernel, hipStream_t s);
   void end(hipStream_t s);
+  // mode 2: an event pair for the dispatch of `kernel` itself, if it is timed
+  bool attach(int kernel, hipEvent_t* start, hipEvent_t* stop);
   // synchronises, folds recorded pairs into the totals and clears them
   void collect();
   void read(int kernel, int* launches, double* total_ms);
   void reset();
 
  private:
-  bool on_ = false;
+  int mode_ = 0;
   struct Rec {
     int kernel;
     hipEvent_t a, b;

@@ -22,6 +22,8 @@
 //  * Integrator and thermo in FP64, thermo sums compensated (two-sum).
 #include "device_common.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <cmath>
 
 namespace dmgx {
@@ -218,6 +220,12 @@ __device__ __forceinline__ void load_row4(const float* __restrict__ row, int kp,
 #ifndef DMGX_GATHER_BATCH
 #define DMGX_GATHER_BATCH 2
 #endif
+// the fused descriptor kernel's workgroup (256: three workgroups per CU by
+// LDS; 512, which would allow four wavefronts per SIMD, measured no faster)
+#ifndef DMGX_DESC_BLOCK
+#define DMGX_DESC_BLOCK 256
+#endif
+constexpr int kDescBlock = DMGX_DESC_BLOCK;
 constexpr int kFilterLanes = DMGX_FILTER_LANES;
 constexpr int kFilterBatch = DMGX_FILTER_BATCH;
 constexpr int kGatherLanes = DMGX_GATHER_LANES;
@@ -513,7 +521,7 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
 }
 
 template <class C>
-__global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1, int c2, int nblocks,
+__global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, int c1, int c2, int nblocks,
                                                        const int* __restrict__ type, double* __restrict__ pe,
                                                        int accumulate, DevWork w)
 {
@@ -688,7 +696,7 @@ __global__ __launch_bounds__(kBlock) void k_descriptor(DevModel m, int N, int c1
   float Fp[DMAX];
   float E = 0.0f;
   if (C::ann != kAnnValu) {
-    __shared__ float ann_lds[kBlock / 64][ann_lds_floats<DMAX>()];
+    __shared__ float ann_lds[kDescBlock / 64][ann_lds_floats<DMAX>()];
     ann_mfma<DMAX, C::ann == kAnnMfmaBf16>(m, D, q, Fp, E, ann_lds[threadIdx.x >> 6], ann_w);
   } else {
     const int H = m.H;
@@ -1407,29 +1415,33 @@ hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, Dev
 
 template <class C>
 static void run_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc,
-                           DevWork& w, hipStream_t s)
+                           DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
-  const int nb = nblk(c2 - c1);
-  k_descriptor<C><<<xcd_grid(nb), kBlock, 0, s>>>(m, N, c1, c2, nb, type, pe, acc ? 1 : 0, w);
+  const int nb = (c2 - c1 + kDescBlock - 1) / kDescBlock;
+  if (e0 || e1)
+    hipExtLaunchKernelGGL(k_descriptor<C>, dim3(xcd_grid(nb)), dim3(kDescBlock), 0, s, e0, e1, 0, m, N, c1, c2, nb,
+                          type, pe, acc ? 1 : 0, w);
+  else
+    k_descriptor<C><<<xcd_grid(nb), kDescBlock, 0, s>>>(m, N, c1, c2, nb, type, pe, acc ? 1 : 0, w);
 }
 
 // single-type exact configuration with the ANN mode picked at run time:
 // matrix cores for H <= 64 (FP32 or BF16 operands), vector ALUs otherwise
 template <int NR, int KR, int NA, int KA, bool Q2, bool Q1>
 static void run_single(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
-                       DevWork& w, hipStream_t s)
+                       DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
   const int mode = m.H <= 64 ? ann : kAnnValu;
   if (mode == kAnnMfma32)
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfma32>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfma32>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (mode == kAnnMfmaBf16)
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaBf16>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaBf16>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
 }
 
 hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
-                             bool acc, int ann, DevWork& w, hipStream_t s)
+                             bool acc, int ann, DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
   if (c2 <= c1) return hipSuccess;
   const bool one = m.T == 1;
@@ -1439,20 +1451,20 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
   };
   // exact instantiations of the configurations in use; generic otherwise
   if (is(5, 9, 5, 9, 4, true, true, true))
-    run_single<5, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s);
+    run_single<5, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, false, true))
-    run_single<7, 9, 5, 9, true, false>(m, N, c1, c2, type, pe, acc, ann, w, s);
+    run_single<7, 9, 5, 9, true, false>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(13, 17, 9, 13, 4, true, true, true))
-    run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(7, 11, 7, 11, 4, true, true, false))
-    run_descriptor<Cfg<7, 11, 7, 11, 4, true, true, false, true>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_descriptor<Cfg<7, 11, 7, 11, 4, true, true, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(5, 9, 5, 9, 4, true, false, false))
-    run_descriptor<Cfg<5, 9, 5, 9, 4, true, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_descriptor<Cfg<5, 9, 5, 9, 4, true, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(9, 9, 7, 9, 4, false, false, false))
-    run_descriptor<Cfg<9, 9, 7, 9, 4, false, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s);
+    run_descriptor<Cfg<9, 9, 7, 9, 4, false, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (m.L <= 4 && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
     run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, 4, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
-                                                                                  w, s);
+                                                                                  w, s, e0, e1);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

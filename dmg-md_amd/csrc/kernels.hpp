@@ -132,8 +132,10 @@ hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, Dev
 // ann: 0 FP32 on the vector ALUs, 1 FP32 on the matrix cores, 2 BF16 operands
 // on the matrix cores; modes 1 and 2 apply to single-type models of the
 // exact configurations with at most 64 neurons, the vector path otherwise.
+// start/stop (optional) are recorded with the kernel's own dispatch
 hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
-                             bool accumulate, int ann, DevWork& w, hipStream_t s);
+                             bool accumulate, int ann, DevWork& w, hipStream_t s, hipEvent_t start = nullptr,
+                             hipEvent_t stop = nullptr);
 hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
                          double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s);
 

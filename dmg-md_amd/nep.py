@@ -155,8 +155,11 @@ class NEP:
 
     KERNELS = ("filter", "descriptor", "gather", "rebuild", "check", "integrate", "thermo")
 
-    def set_profiling(self, on: bool) -> None:
-        check(lib().dmgx_nep_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on: bool, dispatch_only: bool = False) -> None:
+        """on: HIP-event timing of every kernel group (events recorded on the
+        stream around each); dispatch_only: the fused descriptor kernel only,
+        with events attached to its dispatch (the stream is left as it is)."""
+        check(lib().dmgx_nep_set_profiling(self._h, (2 if dispatch_only else 1) if on else 0))
 
     def profile_reset(self) -> None:
         check(lib().dmgx_nep_profile_reset(self._h))

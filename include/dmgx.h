@@ -110,12 +110,16 @@ int dmgx_nep_neighbor_stats(dmgx_nep* nep, int* max_radial, int* max_angular, in
 int dmgx_nep_neighbor_totals(dmgx_nep* nep, long long* sum_radial, long long* sum_angular,
                              long long* sum_verlet, int* num_centres);
 
-/* Per-kernel timing with HIP events recorded on the launch stream (no
- * counterpart in the reference; the measurement harness uses it).  Kernel
- * ids: 0 filter, 1 fused descriptor/ANN/angular-partial, 2 gather,
- * 3 Verlet rebuild (all its kernels), 4 displacement check, 5 velocity
- * Verlet, 6 thermo.  profile_read synchronises and returns the launch count
- * and summed milliseconds since the last reset. */
+/* Per-kernel timing with HIP events (no counterpart in the reference; the
+ * measurement harness uses it).  Kernel ids: 0 filter, 1 fused
+ * descriptor/ANN/angular-partial, 2 gather, 3 Verlet rebuild (all its
+ * kernels), 4 displacement check, 5 velocity Verlet, 6 thermo.  enable: 0
+ * off; 1 every kernel group, events recorded on the launch stream around it
+ * (each record separates the kernels it sits between by a few microseconds);
+ * 2 the fused descriptor kernel only, its events attached to its own dispatch
+ * (hipExtLaunchKernelGGL), which leaves the stream undisturbed.  profile_read
+ * synchronises and returns the launch count and summed milliseconds since
+ * the last reset. */
 int dmgx_nep_set_profiling(dmgx_nep* nep, int enable);
 int dmgx_nep_profile_read(dmgx_nep* nep, int kernel, int* launches, double* total_ms);
 int dmgx_nep_profile_reset(dmgx_nep* nep);
