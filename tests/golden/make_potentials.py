@@ -80,7 +80,69 @@ def make(name, spec):
     print(name, "dim", dim, "MN", mnr, mna)
 
 
+def two_type_sample(n=120, L=12.0, dmin=1.0, seed=0):
+    """A random 2-type periodic configuration with a minimum separation."""
+    rng = np.random.default_rng(seed)
+    pos = []
+    while len(pos) < n:
+        x = rng.uniform(0, L, 3)
+        if all(np.linalg.norm((x - y) - L * np.round((x - y) / L)) > dmin for y in pos):
+            pos.append(x)
+    pos = np.array(pos).T.copy()
+    types = (np.arange(n) % 3 == 0).astype(np.int32)
+    return dict(box=W.Box.orthogonal(L, L, L), pos=pos, types=types)
+
+
+# Coverage models (no benchmark configuration uses them): NEP5, and the
+# flexible and typewise ZBL forms, each on a 2-type (C, H) system.
+# name: (version, zbl line or None, flexible ZBL per-pair table or None, seed)
+UNIVERSAL = [0.18175, 3.1998, 0.50986, 0.94229, 0.28022, 0.4029, 0.02817, 0.20162]
+COVERAGE = {
+    "nep5_ch.txt": (5, None, None, 201),
+    "nep4_zbl_flexible_ch.txt": (4, "zbl 0 0", [[1.0, 2.0] + UNIVERSAL, [0.8, 1.7] + UNIVERSAL,
+                                                [0.6, 1.4] + UNIVERSAL], 202),
+    "nep5_zbl_typewise_ch.txt": (5, "zbl 0.7 1.6 0.6", None, 203),
+}
+
+
+def make_coverage(name, spec):
+    version, zbl, flex, seed = spec
+    syms, rcR, rcA, nmax, basis, lmax, H = ["C", "H"], 4.5, 3.5, (4, 3), (6, 6), "4 2 1", 20
+    T = len(syms)
+    L, q222, q1111 = [int(x) for x in lmax.split()]
+    numL = L + (1 if q222 else 0) + (1 if q1111 else 0)
+    dim = (nmax[0] + 1) + (nmax[1] + 1) * numL
+    rng = np.random.default_rng(seed)
+    per_type = (dim + 2) * H + (1 if version == 5 else 0)
+    n_ann = per_type * T + 1
+    n_desc = T * T * ((nmax[0] + 1) * (basis[0] + 1) + (nmax[1] + 1) * (basis[1] + 1))
+    ann = f32(rng.normal(0.0, 0.1, n_ann))
+    desc = f32(rng.normal(0.0, 0.1, n_desc))
+    sysd = two_type_sample(seed=seed)
+    rows = neighbor_rows(sysd["box"].h, sysd["box"].pbc, sysd["pos"], rcR, max_nb=1024)
+    rows_a = neighbor_rows(sysd["box"].h, sysd["box"].pbc, sysd["pos"], rcA, max_nb=1024)
+    mnr = int(np.ceil(1.25 * max(len(r) for r in rows)))
+    mna = int(np.ceil(1.25 * max(len(r) for r in rows_a)))
+    head = [f"nep{version}{'_zbl' if zbl else ''} {T} {' '.join(syms)}"]
+    if zbl:
+        head.append(zbl)
+    head += [f"cutoff {rcR} {rcA} {mnr} {mna}", f"n_max {nmax[0]} {nmax[1]}",
+             f"basis_size {basis[0]} {basis[1]}", f"l_max {lmax}", f"ANN {H} 0"]
+    tail = f32(np.array(flex).reshape(-1)) if flex else np.zeros(0)
+    path = os.path.join(OUT, name)
+    write(path, head, np.concatenate([ann, desc, np.ones(dim), tail]))
+    o = OracleNEP(path).compute(sysd["box"].h, sysd["box"].pbc, sysd["types"], sysd["pos"],
+                                want_descriptor=True)
+    q = o["q"]
+    rngq = q.max(0) - q.min(0)
+    scaler = f32(np.where(rngq > 0, 1.0 / np.where(rngq > 0, rngq, 1.0), 1.0))
+    write(path, head, np.concatenate([ann, desc, scaler, tail]))
+    print(name, "dim", dim, "MN", mnr, mna)
+
+
 if __name__ == "__main__":
     os.makedirs(OUT, exist_ok=True)
     for n, s in SPECS.items():
         make(n, s)
+    for n, s in COVERAGE.items():
+        make_coverage(n, s)

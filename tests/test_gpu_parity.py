@@ -370,3 +370,32 @@ def test_ann_modes(dmgx, workloads, which):
     fmax = np.abs(ref["force"]).max()
     assert e["force"] < 2e-2 * max(fmax, 1.0), e
     assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
+
+
+@pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt"])
+@pytest.mark.parametrize("tiles", [1, 3])
+def test_nep5_and_zbl_forms_against_oracle(dmgx, name, tiles):
+    """NEP5, the flexible per-pair ZBL table and the typewise ZBL cutoff on a
+    random 2-type (C, H) system with pairs down to 0.5 A: the 11 A cell
+    (replicated small-box path) and its 3x3x3 tiling (large-box path).  The
+    FP32 bars scale with the largest force and energy, which the ZBL pairs
+    make large."""
+    from test_oracle_golden import coverage_sample
+    h, pos, t = coverage_sample()
+    if tiles > 1:
+        L = h[0, 0]
+        shifts = np.array([[a, b, c] for a in range(tiles) for b in range(tiles) for c in range(tiles)]) * L
+        pos = np.ascontiguousarray(np.concatenate([pos + s[:, None] for s in shifts], axis=1))
+        t = np.tile(t, tiles ** 3)
+        h = h * tiles
+    path = os.path.join(POTENTIALS, name)
+    nep = dmgx.NEP(path)
+    out = gpu_compute(dmgx, nep, dmgx.Box(h, (1, 1, 1)), t, pos)
+    ref = OracleNEP(path, round_params_to_float=True).compute(h, (1, 1, 1), t, pos)
+    err = errors(out, ref)
+    fmax = np.abs(ref["force"]).max()
+    emax = np.abs(ref["pe"]).max()
+    assert err["force"] <= max(TOL_F, 2e-6 * fmax), (err, fmax)
+    assert err["energy_atom_max"] <= max(TOL_EA, 2e-7 * emax), (err, emax)
+    assert err["energy_per_atom"] <= max(TOL_E, 2e-7 * emax), (err, emax)
+    assert err["virial"] <= max(TOL_V, 2e-6 * np.abs(ref["virial"]).max()), err

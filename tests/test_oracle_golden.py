@@ -120,3 +120,42 @@ def test_oracle_replays_gpumd_nvt_ber_trajectory():
         g = read_xyz(os.path.join(gold_dir, "trajectory.xyz"), frame=k)
         assert within(v, g["vel"], vel_tol), (k, np.abs(v - g["vel"]).max())
         assert within(p, g["pos"], TOL["position"]), (k, np.abs(p - g["pos"]).max())
+
+
+COVERAGE_MODELS = ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt"]
+
+
+def coverage_sample(seed=7, n=100, L=11.0, dmin=0.5):
+    """Random 2-type (C, H) periodic configuration; dmin 0.5 A puts pairs
+    inside every ZBL range of the coverage models."""
+    rng = np.random.default_rng(seed)
+    pos = []
+    while len(pos) < n:
+        x = rng.uniform(0, L, 3)
+        if all(np.linalg.norm((x - y) - L * np.round((x - y) / L)) > dmin for y in pos):
+            pos.append(x)
+    pos = np.ascontiguousarray(np.array(pos).T)
+    return np.diag([L, L, L]), pos, (np.arange(n) % 3 == 0).astype(np.int32)
+
+
+@pytest.mark.parametrize("name", COVERAGE_MODELS)
+def test_oracle_matches_reference_on_nep5_and_zbl_forms(name):
+    """NEP5 (typewise output bias), the flexible per-pair ZBL table and the
+    typewise ZBL cutoff: no reference golden covers them, so the oracle is
+    pinned to the reference's own FP64 evaluator (nep_cpu.cpp) on a random
+    2-type system with short pairs."""
+    from conftest import POTENTIALS
+    ref = pytest.importorskip("ref")
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    path = os.path.join(POTENTIALS, name)
+    h, pos, t = coverage_sample()
+    d = pos[:, :, None] - pos[:, None, :]
+    d -= h[0, 0] * np.round(d / h[0, 0])
+    r = np.sqrt((d * d).sum(0))[np.triu_indices(pos.shape[1], 1)]
+    assert (r < 0.9).sum() > 0  # pairs inside the typewise ZBL cutoffs
+    o = OracleNEP(path).compute(h, [1, 1, 1], t, pos)
+    rr = ref.RefNEP(path).compute(h, [1, 1, 1], t, pos)
+    for k in ("pe", "force", "virial"):
+        scale = max(1.0, np.abs(rr[k]).max())
+        assert np.abs(o[k] - rr[k]).max() <= 1e-12 * scale, (k, np.abs(o[k] - rr[k]).max(), scale)
