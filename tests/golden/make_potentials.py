@@ -95,19 +95,23 @@ def two_type_sample(n=120, L=12.0, dmin=1.0, seed=0):
 
 # Coverage models (no benchmark configuration uses them): NEP5, and the
 # flexible and typewise ZBL forms, each on a 2-type (C, H) system.
-# name: (version, zbl line or None, flexible ZBL per-pair table or None, seed)
+# name: (version, zbl line or None, flexible ZBL per-pair table or None, seed,
+#        per-type (rc_radial, rc_angular) or None)
 UNIVERSAL = [0.18175, 3.1998, 0.50986, 0.94229, 0.28022, 0.4029, 0.02817, 0.20162]
 COVERAGE = {
-    "nep5_ch.txt": (5, None, None, 201),
+    "nep5_ch.txt": (5, None, None, 201, None),
     "nep4_zbl_flexible_ch.txt": (4, "zbl 0 0", [[1.0, 2.0] + UNIVERSAL, [0.8, 1.7] + UNIVERSAL,
-                                                [0.6, 1.4] + UNIVERSAL], 202),
-    "nep5_zbl_typewise_ch.txt": (5, "zbl 0.7 1.6 0.6", None, 203),
+                                                [0.6, 1.4] + UNIVERSAL], 202, None),
+    "nep5_zbl_typewise_ch.txt": (5, "zbl 0.7 1.6 0.6", None, 203, None),
+    "nep4_typecut_ch.txt": (4, None, None, 204, [(4.5, 3.5), (3.6, 2.9)]),
 }
 
 
 def make_coverage(name, spec):
-    version, zbl, flex, seed = spec
+    version, zbl, flex, seed, cut = spec
     syms, rcR, rcA, nmax, basis, lmax, H = ["C", "H"], 4.5, 3.5, (4, 3), (6, 6), "4 2 1", 20
+    if cut:
+        rcR, rcA = max(c[0] for c in cut), max(c[1] for c in cut)
     T = len(syms)
     L, q222, q1111 = [int(x) for x in lmax.split()]
     numL = L + (1 if q222 else 0) + (1 if q1111 else 0)
@@ -126,7 +130,8 @@ def make_coverage(name, spec):
     head = [f"nep{version}{'_zbl' if zbl else ''} {T} {' '.join(syms)}"]
     if zbl:
         head.append(zbl)
-    head += [f"cutoff {rcR} {rcA} {mnr} {mna}", f"n_max {nmax[0]} {nmax[1]}",
+    radii = " ".join(f"{a} {b}" for a, b in cut) if cut else f"{rcR} {rcA}"
+    head += [f"cutoff {radii} {mnr} {mna}", f"n_max {nmax[0]} {nmax[1]}",
              f"basis_size {basis[0]} {basis[1]}", f"l_max {lmax}", f"ANN {H} 0"]
     tail = f32(np.array(flex).reshape(-1)) if flex else np.zeros(0)
     path = os.path.join(OUT, name)

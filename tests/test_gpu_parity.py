@@ -372,7 +372,8 @@ def test_ann_modes(dmgx, workloads, which):
     assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
 
 
-@pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt"])
+@pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt",
+                                  "nep4_typecut_ch.txt"])
 @pytest.mark.parametrize("tiles", [1, 3])
 def test_nep5_and_zbl_forms_against_oracle(dmgx, name, tiles):
     """NEP5, the flexible per-pair ZBL table and the typewise ZBL cutoff on a

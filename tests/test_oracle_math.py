@@ -15,6 +15,7 @@ from oracle import OracleNEP, constant_tables
 C_NEP = os.path.join(BASELINE, "inputs", "potentials", "nep_C.txt")
 W_NEP = os.path.join(BASELINE, "inputs", "potentials", "nep_water.txt")
 Z_NEP = os.path.join(BASELINE, "inputs", "potentials", "nep_BaTiO3_zbl.txt")
+TC_NEP = os.path.join(POTENTIALS, "nep4_typecut_ch.txt")  # per-type cutoffs C 4.5/3.5, H 3.6/2.9
 
 
 def cluster(n, L, seed, dmin=1.1):
@@ -27,7 +28,7 @@ def cluster(n, L, seed, dmin=1.1):
     return np.ascontiguousarray(np.array(pts).T)
 
 
-@pytest.mark.parametrize("path,ntypes", [(C_NEP, 1), (W_NEP, 2), (Z_NEP, 3)])
+@pytest.mark.parametrize("path,ntypes", [(C_NEP, 1), (W_NEP, 2), (Z_NEP, 3), (TC_NEP, 2)])
 def test_force_is_minus_energy_gradient(path, ntypes):
     nep = OracleNEP(path)
     L = 15.0
@@ -187,3 +188,19 @@ def test_seeded_models_parse_and_are_finite():
     for name in ("nep_c0_carbon.txt", "nep_c1_cu.txt", "nep_c4_carbon_zbl.txt"):
         nep = OracleNEP(os.path.join(POTENTIALS, name))
         assert nep.version == 4
+
+
+def test_per_type_cutoffs_use_the_pair_mean():
+    """`cutoff rcR_C rcA_C rcR_H rcA_H ...`: a C-H pair interacts within the
+    mean of the two types' radii (nep.cu find_neighbor_list_large_box: pair
+    cutoffs are type-pair averages).  An isolated C-H pair between the mean
+    radial cutoff (4.05 A) and the larger one (4.5 A) leaves both atoms'
+    energies exactly those of isolated atoms; just inside the mean it does not."""
+    nep = OracleNEP(TC_NEP)
+    L = 40.0
+    h = np.diag([L, L, L])
+    t = np.array([0, 1], np.int32)
+    far = nep.compute(h, [1, 1, 1], t, np.array([[5.0, 25.0], [5.0, 5.0], [5.0, 5.0]]))["pe"]
+    for d, same in ((4.2, True), (4.45, True), (3.95, False)):
+        pe = nep.compute(h, [1, 1, 1], t, np.array([[5.0, 5.0 + d], [5.0, 5.0], [5.0, 5.0]]))["pe"]
+        assert np.array_equal(pe, far) == same, (d, pe, far)
