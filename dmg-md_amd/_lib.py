@@ -51,7 +51,7 @@ class Nhc(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_atoms", ctypes.c_longlong), ("max_requests", ctypes.c_int), ("flags", ctypes.c_uint),
-                ("margin", ctypes.c_double)]
+                ("margin", ctypes.c_double), ("use_graph", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):

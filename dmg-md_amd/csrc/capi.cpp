@@ -411,6 +411,7 @@ int dmgx_server_create(dmgx_nep* nep, const dmgx_server_config* cfg, dmgx_server
     c.max_requests = cfg->max_requests;
     c.flags = cfg->flags;
     c.margin = cfg->margin;
+    c.use_graph = cfg->use_graph != 0;
     if (!(c.margin >= 0.0)) throw std::invalid_argument("negative shell margin");
     *out = new dmgx_server(&nep->engine, c);
   });

@@ -208,6 +208,7 @@ Server::~Server()
   cv_.notify_all();
   if (thread_.joinable()) thread_.join();
   if (stream_) (void)hipStreamSynchronize(stream_);
+  drop_graph();
   for (DBuf<double>* b : {&pos_, &vel_, &force_, &pe_, &vir_, &mass_, &pos2_, &vel2_, &force2_, &mass2_, &req_dt_,
                           &g_shift_, &g_shift2_, &out_})
     b->release();
@@ -505,6 +506,7 @@ bool Server::admit()
 
 void Server::relayout()
 {
+  drop_graph();  // the captured launches take the old layout's arrays
   // placement: every request's extended region in its own cell of a cubic
   // grid of equal slots, separated by more than the Verlet radius
   const double gap = nep_->model().rcR_max + kSkin + 2.0 + 2.0 * cfg_.margin;
@@ -655,16 +657,32 @@ void Server::iterate()
     check(launch_serve_kick(1, n_owned_, n_total_, atom_req_.p, req_md_.p, req_dt_.p, mass_.p, pos_.p, vel_.p,
                             force_.p, stream_),
           "serve kick 1");
-  check(launch_serve_ghosts(n_owned_, n_ghost_, n_total_, owner_.p + n_owned_, g_shift_.p, pos_.p, stream_),
-        "serve ghosts");
-  unsigned flags = cfg_.flags & ~1u;
-  if (forced_rebuild_) flags |= 2u;
-  ImageMap map;
-  map.owner = owner_.p;
-  map.image = image_.p;
-  map.gstart = gstart_.p;
-  nep_->compute(batch_box_, n_total_, 0, n_owned_, 0, n_owned_, type_.p, pos_.p, nullptr, pe_.p, force_.p,
-                vir_.p, flags, stream_, &map);
+  const unsigned flags = cfg_.flags & ~3u;
+  if (cfg_.use_graph) {
+    // one graph launch per iteration; a Verlet overflow reallocates the
+    // engine's rows, so the graph is captured again and replayed
+    bool done = false;
+    for (int attempt = 0; attempt < 4 && !done; ++attempt) {
+      if (!graph_exec_) capture_force(flags);
+      if (forced_rebuild_) check(launch_request_rebuild(nep_->work(), stream_), "serve rebuild request");
+      check(hipGraphLaunch(graph_exec_, stream_), "serve force graph");
+      done = !nep_->settle(stream_);
+      if (!done) {
+        drop_graph();
+        forced_rebuild_ = true;
+      }
+    }
+    if (!done) throw std::runtime_error("serving: Verlet rows kept overflowing");
+  } else {
+    check(launch_serve_ghosts(n_owned_, n_ghost_, n_total_, owner_.p + n_owned_, g_shift_.p, pos_.p, stream_),
+          "serve ghosts");
+    ImageMap map;
+    map.owner = owner_.p;
+    map.image = image_.p;
+    map.gstart = gstart_.p;
+    nep_->compute(batch_box_, n_total_, 0, n_owned_, 0, n_owned_, type_.p, pos_.p, nullptr, pe_.p, force_.p,
+                  vir_.p, flags | (forced_rebuild_ ? 2u : 0u), stream_, &map);
+  }
   forced_rebuild_ = false;
   if (any_md)
     check(launch_serve_kick(0, n_owned_, n_total_, atom_req_.p, req_md_.p, req_dt_.p, mass_.p, pos_.p, vel_.p,
@@ -672,6 +690,53 @@ void Server::iterate()
           "serve kick 2");
   std::lock_guard<std::mutex> lk(mu_);
   st_.iterations++;
+}
+
+// Captures the force launches of the current layout: the plan (host checks,
+// workspace sizing) is made first, outside the capture; the pack never forces
+// a rebuild itself (a forced rebuild is requested on the device before a
+// replay).  The captured kernels take the layout's arrays, so the graph lives
+// until the next relayout or workspace reallocation.
+void Server::capture_force(unsigned flags)
+{
+  ImageMap map;
+  map.owner = owner_.p;
+  map.image = image_.p;
+  map.gstart = gstart_.p;
+  const NepEngine::Plan p = nep_->plan(batch_box_, n_total_, 0, n_owned_, 0, n_owned_, type_.p, pos_.p, nullptr,
+                                       pe_.p, force_.p, vir_.p, flags, &map);
+  if (p.forced) forced_rebuild_ = true;
+  check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "serve capture");
+  hipError_t e = launch_serve_ghosts(n_owned_, n_ghost_, n_total_, owner_.p + n_owned_, g_shift_.p, pos_.p, stream_);
+  if (e == hipSuccess)
+    e = launch_pack(p.box, n_total_, nep_->model().num_types, pos_.p, type_.p, false, nep_->work(), stream_);
+  if (e == hipSuccess) {
+    try {
+      nep_->launch_force(p, n_total_, 0, n_owned_, 0, n_owned_, type_.p, nullptr, pe_.p, force_.p, vir_.p, flags,
+                         stream_);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(stream_, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    check(e != hipSuccess ? e : e2, "serve capture");
+  }
+  graph_ = g;
+  check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "serve graph instantiate");
+}
+
+void Server::drop_graph()
+{
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
 }
 
 void Server::retire()

@@ -45,6 +45,7 @@ struct ServeConfig {
   unsigned flags = 0;             // force flags (ANN mode)
   double margin = 1.0;            // extra shell thickness (A) for motion during a request
   long long reserve_atoms = 1LL << 21;  // engine workspace sized up front (capped by max_atoms)
+  bool use_graph = true;          // replay the force launches of a layout as one hipGraph
 };
 
 struct ServeStats {
@@ -149,6 +150,12 @@ class Server {
   size_t h_stage_bytes_ = 0;
   void stage(size_t bytes);
   bool forced_rebuild_ = true;
+  // the force launches of the current layout (ghost refresh, pack, Verlet
+  // rebuild, filter, descriptor, gather) captured once and replayed
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  void capture_force(unsigned flags);
+  void drop_graph();
 };
 
 }  // namespace dmgx

@@ -257,10 +257,10 @@ class Server:
     the server while it lives."""
 
     def __init__(self, nep: NEP, max_atoms: int = 4_000_000, max_requests: int = 4096, ann_bf16=False,
-                 ann_valu=False, margin: float = 1.0):
+                 ann_valu=False, margin: float = 1.0, graph: bool = True):
         cfg = _lib.ServerConfig(int(max_atoms), int(max_requests),
                                 (_lib.ANN_BF16 if ann_bf16 else 0) | (_lib.ANN_VALU if ann_valu else 0),
-                                float(margin))
+                                float(margin), 1 if graph else 0)
         h = ctypes.c_void_p()
         check(lib().dmgx_server_create(nep.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

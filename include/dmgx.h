@@ -240,6 +240,8 @@ typedef struct dmgx_server_config {
   int max_requests;      /* requests in flight */
   unsigned flags;        /* DMGX_ANN_BF16 / DMGX_ANN_VALU */
   double margin;         /* extra shell thickness (A) for motion during a request */
+  int use_graph;         /* nonzero: the force launches of a batch layout are captured once
+                            into a hipGraph and replayed every iteration */
 } dmgx_server_config;
 
 typedef struct dmgx_server_stats {
