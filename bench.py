@@ -148,18 +148,18 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
         n = p.shape[1]
         return srv.submit(dmgx.Box(h), np.zeros(n, np.int32), np.full(n, 12.011), p, v, steps=stp, dt_fs=0.5)
 
-    def burst(count, clients=8):
+    def burst(count, nep, clients=8):
         # `clients` threads submit the stream at once (the submit call
         # prepares a request's image shell and upload on the calling thread)
         from concurrent.futures import ThreadPoolExecutor
-        srv = dmgx.Server(dmgx.NEP(C0_MODEL), max_atoms=6_000_000, ann_bf16=True)
+        srv = dmgx.Server(nep, max_atoms=6_000_000, ann_bf16=True)
         with ThreadPoolExecutor(clients) as pool:
             ids = list(pool.map(lambda k: submit(srv, k), range(count)))
         srv.drain()
         return srv, ids, srv.stats()
 
-    def poisson():
-        srv = dmgx.Server(dmgx.NEP(C0_MODEL), max_atoms=6_000_000, ann_bf16=True)
+    def poisson(nep):
+        srv = dmgx.Server(nep, max_atoms=6_000_000, ann_bf16=True)
         t_next = time.perf_counter()
         for k, g in enumerate(gaps):
             t_next += g
@@ -169,9 +169,13 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
         srv.drain()
         return srv.stats()
 
-    burst(min(requests, 8))  # first launches of the serving kernels
-    srv, ids, st = burst(requests)
-    lat = poisson()
+    # first launches of the serving kernels; the server's autotuner picks the
+    # tile shapes on the first batch of >= 64k atoms, here in this warm-up
+    nep_srv = dmgx.NEP(C0_MODEL)
+    burst(min(requests, 48), nep_srv)
+    tune = nep_srv.tuning()
+    srv, ids, st = burst(requests, nep_srv)
+    lat = poisson(nep_srv)
     # BF16 force error against FP32 on single points of the stream
     err = 0.0
     nep32 = dmgx.NEP(C0_MODEL)
@@ -222,6 +226,8 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
             "latency_run": f"Poisson arrivals {rate_hz:.0f}/s", "p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],
             "latency_run_frames_per_s": lat["frames_per_s"],
             "bf16_force_err_max_eV_per_A": err,
+            "tile_shapes": {"filter_lanes_per_centre": tune["filter_lanes"],
+                            "gather_lanes_per_atom": tune["gather_lanes"], "autotuned": tune["tuned"]},
             "plain_fp32_frames_per_s": plain, "plain_sample_requests": plain_requests,
             "speedup_vs_plain": st["frames_per_s"] / plain if plain > 0 else None}
 
@@ -235,6 +241,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default filter/gather tile shapes")
     ap.add_argument("--ann", choices=["fp32", "valu", "bf16"], default="fp32",
                     help="ANN arithmetic: fp32 on the matrix cores (default), fp32 on the vector ALUs, "
                          "or bfloat16 operands on the matrix cores")
@@ -270,6 +277,9 @@ def main():
     ens = dict(ensemble="nvt_bdp", t_init=300.0, t_final=300.0, t_coupling=100.0, seed=12345678,
                ann_bf16=args.ann == "bf16", ann_valu=args.ann == "valu")
     print(f"bench: Cu fcc {args.cells}^3 = {N} atoms, warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    # filter / gather tile shapes timed on this system before anything else
+    tune = nep.tuning() if args.no_autotune else md.autotune(ann_bf16=args.ann == "bf16")
+    print(f"bench: tile shapes {tune}", file=sys.stderr, flush=True)
     md.run(args.warmup, 2.0, **ens)
     torch.cuda.synchronize()
     # timed region: only the dominant kernel is timed, with HIP events
@@ -327,6 +337,11 @@ def main():
             "verlet_rebuilds_in_run": stats["rebuilds"],
             "kernel_avg_ms": kernel_ms,
             "kernel_avg_ms_from": "a separate profiled stretch after the timed steps",
+            "tile_shapes": {"filter_lanes_per_centre": tune["filter_lanes"],
+                            "gather_lanes_per_atom": tune["gather_lanes"],
+                            "autotuned": tune["tuned"],
+                            "filter_ms_by_lanes": {str(k): round(v, 4) for k, v in tune["filter_ms"].items()},
+                            "gather_ms_by_lanes": {str(k): round(v, 4) for k, v in tune["gather_ms"].items()}},
             "final_temperature_K": th[0],
         },
         "roofline": {

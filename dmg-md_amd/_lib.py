@@ -61,6 +61,11 @@ class ServerStats(ctypes.Structure):
         (n, ctypes.c_double) for n in ("relayout_s", "iterate_s", "retire_s")]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("filter_lanes", ctypes.c_int), ("gather_lanes", ctypes.c_int),
+                ("filter_ms", ctypes.c_double * 3), ("gather_ms", ctypes.c_double * 3), ("tuned", ctypes.c_int)]
+
+
 ACCUMULATE = 1
 FORCE_REBUILD = 2
 ANN_BF16 = 4
@@ -87,6 +92,9 @@ _SIGNATURES = {
     "dmgx_nep_set_profiling": (I, [VP, I]),
     "dmgx_nep_profile_read": (I, [VP, I, PI, PD]),
     "dmgx_nep_profile_reset": (I, [VP]),
+    "dmgx_nep_get_tuning": (I, [VP, ctypes.POINTER(Tuning)]),
+    "dmgx_nep_set_tiles": (I, [VP, I, I]),
+    "dmgx_nep_autotune": (I, [VP, ctypes.POINTER(Box), I, VP, VP, VP, VP, VP, U, VP, ctypes.POINTER(Tuning)]),
     "dmgx_nep_copy_verlet_rows": (I, [VP, I, I, I, PI, PI]),
     "dmgx_nep_copy_pair_rows": (I, [VP, I, I, I, I, PI, PI]),
     "dmgx_velocity_verlet": (I, [I, I, D, VP, VP, VP, VP, VP]),
@@ -108,6 +116,7 @@ _SIGNATURES = {
     "dmgx_md_run": (I, [VP, I, D, ctypes.POINTER(Ensemble), U, PD]),
     "dmgx_md_arrays": (I, [VP] + [ctypes.POINTER(VP)] * 7),
     "dmgx_md_box": (I, [VP, ctypes.POINTER(Box)]),
+    "dmgx_md_autotune": (I, [VP, U, ctypes.POINTER(Tuning)]),
     "dmgx_server_create": (I, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
     "dmgx_server_destroy": (None, [VP]),
     "dmgx_server_submit": (I, [VP, ctypes.POINTER(Box), I, PI, PD, PD, PD, I, D,

@@ -175,6 +175,50 @@ int dmgx_nep_set_profiling(dmgx_nep* nep, int enable)
   });
 }
 
+namespace {
+void fill_tuning(const dmgx::NepEngine::Tuning& t, dmgx_tuning* out)
+{
+  if (!out) return;
+  out->filter_lanes = t.filter_lanes;
+  out->gather_lanes = t.gather_lanes;
+  for (int k = 0; k < 3; ++k) {
+    out->filter_ms[k] = t.filter_ms[k];
+    out->gather_ms[k] = t.gather_ms[k];
+  }
+  out->tuned = t.tuned ? 1 : 0;
+}
+}  // namespace
+
+int dmgx_nep_get_tuning(const dmgx_nep* nep, dmgx_tuning* out)
+{
+  return guarded([&] {
+    if (!nep || !out) throw std::invalid_argument("null argument");
+    fill_tuning(nep->engine.tuning(), out);
+  });
+}
+
+int dmgx_nep_set_tiles(dmgx_nep* nep, int filter_lanes, int gather_lanes)
+{
+  return guarded([&] {
+    if (!nep) throw std::invalid_argument("null handle");
+    nep->engine.set_tiles(filter_lanes, gather_lanes);
+  });
+}
+
+int dmgx_nep_autotune(dmgx_nep* nep, const dmgx_box* box, int N, const int* type, const double* position,
+                      double* potential, double* force, double* virial, unsigned flags, void* stream,
+                      dmgx_tuning* out)
+{
+  return guarded([&] {
+    if (!nep || !box || !type || !position || !potential || !force || !virial)
+      throw std::invalid_argument("null argument");
+    if (N <= 0) throw std::invalid_argument("N must be positive");
+    fill_tuning(nep->engine.autotune(to_box(box), N, N, type, position, potential, force, virial, flags,
+                                     (hipStream_t)stream),
+                out);
+  });
+}
+
 int dmgx_nep_profile_read(dmgx_nep* nep, int kernel, int* launches, double* total_ms)
 {
   return guarded([&] {
@@ -398,6 +442,14 @@ int dmgx_md_box(dmgx_md* md, dmgx_box* out)
     if (!md || !out) throw std::invalid_argument("null argument");
     std::memcpy(out->h, md->state.box().h, sizeof(out->h));
     std::memcpy(out->pbc, md->state.box().pbc, sizeof(out->pbc));
+  });
+}
+
+int dmgx_md_autotune(dmgx_md* md, unsigned flags, dmgx_tuning* out)
+{
+  return guarded([&] {
+    if (!md) throw std::invalid_argument("null handle");
+    fill_tuning(md->state.autotune(flags), out);
   });
 }
 

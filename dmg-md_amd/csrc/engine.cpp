@@ -401,7 +401,7 @@ void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int 
   prof_.end(s);
   const bool acc = (flags & 1u) != 0;
   prof_.begin(kProfFilter, s);
-  check(launch_filter(p.box, dm_, nd1, nd2, w_, s), "filter");
+  check(launch_filter(p.box, dm_, nd1, nd2, w_, s, tune_.filter_lanes), "filter");
   prof_.end(s);
   hipEvent_t ev0, ev1;
   const bool attached = prof_.attach(kProfDescriptor, &ev0, &ev1);
@@ -411,7 +411,7 @@ void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int 
   check(launch_descriptor(dm_, N, nd1, nd2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
   if (!attached) prof_.end(s);
   prof_.begin(kProfGather, s);
-  check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s), "gather");
+  check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s, tune_.gather_lanes), "gather");
   prof_.end(s);
   last_N_stats_ = N;
   last_c1_ = nd1;
@@ -531,6 +531,67 @@ void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2,
           "Verlet row overflow: accumulated outputs are invalid (workspace grown; clear them and call again)");
   }
   throw std::runtime_error("Verlet rows kept overflowing");
+}
+
+const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, const int* type, const double* pos,
+                                              double* pe, double* force, double* virial, unsigned flags,
+                                              hipStream_t s, const ImageMap* map)
+{
+  int reps[3];
+  if (N <= 0 || nc <= 0 || nc > N || small_box(box, reps)) return tune_;  // small boxes keep the defaults
+  flags &= ~1u;
+  compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags, s, map);
+  // rows, edges, adjoints and partials are current: the filter rewrites its
+  // outputs identically and the gather overwrites the forces, so every
+  // candidate can be timed on the same inputs
+  const DevBox b = make_devbox(box);
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "tune event");
+  check(hipEventCreate(&e1), "tune event");
+  auto time3 = [&](const std::function<void()>& launch) {
+    launch();  // warm-up (and first-launch costs)
+    check(hipEventRecord(e0, s), "tune event");
+    for (int k = 0; k < 3; ++k) launch();
+    check(hipEventRecord(e1, s), "tune event");
+    check(hipEventSynchronize(e1), "tune event");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, e0, e1), "tune event");
+    return ms / 3.0;
+  };
+  const int fl[3] = {8, 16, 32}, gl[3] = {4, 8, 16};
+  int best = 1;
+  for (int k = 0; k < 3; ++k) {
+    tune_.filter_ms[k] = time3([&] { check(launch_filter(b, dm_, 0, nc, w_, s, fl[k]), "tune filter"); });
+    if (tune_.filter_ms[k] < tune_.filter_ms[best]) best = k;
+  }
+  tune_.filter_lanes = fl[best];
+  best = 1;
+  for (int k = 0; k < 3; ++k) {
+    tune_.gather_ms[k] = time3([&] {
+      check(launch_gather(dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, gl[k]), "tune gather");
+    });
+    if (tune_.gather_ms[k] < tune_.gather_ms[best]) best = k;
+  }
+  tune_.gather_lanes = gl[best];
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  tune_.tuned = true;
+  // the outputs as a step with the chosen shapes writes them (the list is
+  // current, so this is one filter, descriptor and gather)
+  compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags & ~2u, s, map);
+  check(hipStreamSynchronize(s), "tune");
+  return tune_;
+}
+
+void NepEngine::set_tiles(int filter_lanes, int gather_lanes)
+{
+  if (filter_lanes != 8 && filter_lanes != 16 && filter_lanes != 32)
+    throw std::invalid_argument("filter lanes per centre are 8, 16 or 32");
+  if (gather_lanes != 4 && gather_lanes != 8 && gather_lanes != 16)
+    throw std::invalid_argument("gather lanes per atom are 4, 8 or 16");
+  tune_.filter_lanes = filter_lanes;
+  tune_.gather_lanes = gather_lanes;
+  tune_.tuned = true;
 }
 
 void NepEngine::stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds)
@@ -735,6 +796,14 @@ void MdState::vv(int first_half, double dt)
   check(launch_velocity_verlet(first_half, N_, dt, mass_, pos_, vel_, force_, stream_, unwrapped_),
         "velocity Verlet");
   p.end(stream_);
+}
+
+const NepEngine::Tuning& MdState::autotune(unsigned flags)
+{
+  wrap();
+  const NepEngine::Tuning& t = nep_->autotune(box_, N_, N_, type_, pos_, pe_, force_, vir_, flags, stream_);
+  have_force_ = true;
+  return t;
 }
 
 void MdState::enable_unwrapped()

@@ -104,6 +104,25 @@ class NepEngine {
   void launch_force(const Plan& p, int N, int n1, int n2, int nd1, int nd2, const int* type, const uint64_t* gid,
                     double* pe, double* force, double* virial, unsigned flags, hipStream_t stream);
   DevWork& work() { return w_; }
+  // Tile shapes picked by autotune(): lanes per centre of the filter (8, 16,
+  // 32) and per atom of the gather (4, 8, 16).  autotune() evaluates the
+  // given system once (overwriting pe/force/virial), times every candidate
+  // of the two kernels on it (HIP events, one warm-up and three timed
+  // launches each), keeps the fastest and leaves the outputs as an
+  // evaluation with the chosen shapes writes them.
+  struct Tuning {
+    int filter_lanes = 16, gather_lanes = 8;
+    double filter_ms[3] = {0, 0, 0}, gather_ms[3] = {0, 0, 0};  // per candidate, in the order above
+    bool tuned = false;
+  };
+  const Tuning& tuning() const { return tune_; }
+  // Sets the shapes directly (rejects other values); marks them tuned, so
+  // a server does not re-tune them
+  void set_tiles(int filter_lanes, int gather_lanes);
+  // (centres [0, nc); `map` as for compute)
+  const Tuning& autotune(const Box& box, int N, int nc, const int* type, const double* pos, double* pe,
+                         double* force, double* virial, unsigned flags, hipStream_t stream,
+                         const ImageMap* map = nullptr);
   // A periodic direction thinner than 2.5 (rc + skin) is evaluated on a
   // replicated supercell; reps[d] is the replication per box vector.
   bool small_box(const Box& box, int reps[3]) const;
@@ -134,6 +153,7 @@ class NepEngine {
   double last_h_[9] = {};
   int last_pbc_[3] = {};
   ImageMap last_map_;
+  Tuning tune_;
   int last_N_stats_ = 0, last_c1_ = 0, last_c2_ = 0;
   Profiler prof_;
   // replicated small-box system (atoms x replicas)
@@ -179,6 +199,9 @@ class MdState {
   // added to them (AtomsState::enable_unwrapped, update_unwrapped,
   // runtime.cu:80-90,423-440).  unwrapped() is null until enabled.
   void enable_unwrapped();
+  // NepEngine::autotune on this system's current positions (the forces are
+  // those positions' forces afterwards)
+  const NepEngine::Tuning& autotune(unsigned flags);
   const double* unwrapped() const { return unwrapped_; }
   long long force_calls() const { return force_calls_; }
   void arrays(double** pos, double** vel, double** force, double** pe, double** vir, double** mass, int** type);

@@ -231,10 +231,10 @@ constexpr int kFilterBatch = DMGX_FILTER_BATCH;
 constexpr int kGatherLanes = DMGX_GATHER_LANES;
 constexpr int kGatherBatch = DMGX_GATHER_BATCH;  // radial edges per lane per batch (kR <= 9)
 
-template <bool ONE, bool MAP>
+template <bool ONE, bool MAP, int G, int RB>
 __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1, int c2, int nblocks, DevWork w)
 {
-  constexpr int G = kFilterLanes, PER = kBlock / G, RB = kFilterBatch;
+  constexpr int PER = kBlock / G;
   static_assert(G >= 1 && G <= 32 && (G & (G - 1)) == 0, "filter lanes: a power of two up to 32");
   constexpr unsigned long long gmask = (1ull << G) - 1ull;
   __shared__ double cut2[ONE ? 1 : 2 * kMaxTypes * kMaxTypes];  // [ti][tj] radial, then angular
@@ -1397,18 +1397,28 @@ __global__ void k_check_scaled(int n, double factor, const double* __restrict__ 
 
 static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 
-hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s)
+// lanes per centre (8, 16 or 32) and the Verlet entries each lane loads
+// per pass (7, 7 or 4)
+template <int G, int RB>
+static void run_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s)
+{
+  constexpr int per = kBlock / G;
+  const int nb = (c2 - c1 + per - 1) / per;
+  if (w.map.owner) {
+    if (m.T == 1) k_filter<true, true, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+    else k_filter<false, true, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+  } else {
+    if (m.T == 1) k_filter<true, false, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+    else k_filter<false, false, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+  }
+}
+
+hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s, int lanes)
 {
   if (c2 > c1) {
-    constexpr int per = kBlock / kFilterLanes;
-    const int nb = (c2 - c1 + per - 1) / per;
-    if (w.map.owner) {
-      if (m.T == 1) k_filter<true, true><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
-      else k_filter<false, true><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
-    } else {
-      if (m.T == 1) k_filter<true, false><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
-      else k_filter<false, false><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
-    }
+    if (lanes == 8) run_filter<8, kFilterBatch>(b, m, c1, c2, w, s);
+    else if (lanes == 32) run_filter<32, 4>(b, m, c1, c2, w, s);
+    else run_filter<kFilterLanes, kFilterBatch>(b, m, c1, c2, w, s);
   }
   return hipGetLastError();
 }
@@ -1470,14 +1480,13 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
-                         double* force, double* virial, bool acc, DevWork& w, hipStream_t s)
+template <int G>
+static void run_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type, double* force,
+                       double* virial, int a, DevWork& w, hipStream_t s)
 {
-  if (n2 <= n1) return hipSuccess;
-  constexpr int G = kGatherLanes, per = kBlock / G;
+  constexpr int per = kBlock / G;
   const int nb = (n2 - n1 + per - 1) / per;
   const int g = xcd_grid(nb);
-  const int a = acc ? 1 : 0;
   if (m.T == 1 && m.kR == 9)
     k_gather<9, true, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
   else if (m.T == 1)
@@ -1486,6 +1495,16 @@ hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int 
     k_gather<11, false, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
   else
     k_gather<kMaxK, false, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
+}
+
+hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                         double* force, double* virial, bool acc, DevWork& w, hipStream_t s, int lanes)
+{
+  if (n2 <= n1) return hipSuccess;
+  const int a = acc ? 1 : 0;
+  if (lanes == 4) run_gather<4>(m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
+  else if (lanes == 16) run_gather<16>(m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
+  else run_gather<kGatherLanes>(m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
   return hipGetLastError();
 }
 

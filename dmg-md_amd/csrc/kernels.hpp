@@ -128,7 +128,9 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
                           hipStream_t s);
 
 // Force evaluation (per step)
-hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s);
+// lanes: lanes per centre, 8, 16 (default) or 32
+hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s,
+                         int lanes = 16);
 // ann: 0 FP32 on the vector ALUs, 1 FP32 on the matrix cores, 2 BF16 operands
 // on the matrix cores; modes 1 and 2 apply to single-type models of the
 // exact configurations with at most 64 neurons, the vector path otherwise.
@@ -136,8 +138,10 @@ hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, Dev
 hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
                              bool accumulate, int ann, DevWork& w, hipStream_t s, hipEvent_t start = nullptr,
                              hipEvent_t stop = nullptr);
+// lanes: lanes per atom, 4, 8 (default) or 16
 hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
-                         double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s);
+                         double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s,
+                         int lanes = 8);
 
 // Integrator / thermo
 // first half: kick + drift (and the drift added to `unwrapped` when given)

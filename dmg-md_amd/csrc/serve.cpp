@@ -12,6 +12,8 @@ namespace dmgx {
 
 namespace {
 
+constexpr int kTuneMinAtoms = 65536;  // owned atoms of the batch that triggers the autotuner
+
 void check(hipError_t e, const char* what)
 {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -658,6 +660,19 @@ void Server::iterate()
                             force_.p, stream_),
           "serve kick 1");
   const unsigned flags = cfg_.flags & ~3u;
+  if (!nep_->tuning().tuned && n_owned_ >= kTuneMinAtoms) {
+    // tile shapes picked once, on the first batch large enough to tell them apart
+    ImageMap map;
+    map.owner = owner_.p;
+    map.image = image_.p;
+    map.gstart = gstart_.p;
+    check(launch_serve_ghosts(n_owned_, n_ghost_, n_total_, owner_.p + n_owned_, g_shift_.p, pos_.p, stream_),
+          "serve ghosts");
+    nep_->autotune(batch_box_, n_total_, n_owned_, type_.p, pos_.p, pe_.p, force_.p, vir_.p,
+                   flags | (forced_rebuild_ ? 2u : 0u), stream_, &map);
+    drop_graph();
+    forced_rebuild_ = false;
+  }
   if (cfg_.use_graph) {
     // one graph launch per iteration; a Verlet overflow reallocates the
     // engine's rows, so the graph is captured again and replayed

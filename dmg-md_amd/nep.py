@@ -173,6 +173,32 @@ class NEP:
             out[name] = (n.value, ms.value)
         return out
 
+    def tuning(self) -> dict:
+        """Tile shapes the per-step kernels use (and the autotuner's timings)."""
+        t = _lib.Tuning()
+        check(lib().dmgx_nep_get_tuning(self._h, ctypes.byref(t)))
+        return _tuning_dict(t)
+
+    def set_tiles(self, filter_lanes: int, gather_lanes: int) -> None:
+        """Sets the tile shapes directly (filter 8/16/32, gather 4/8/16)."""
+        check(lib().dmgx_nep_set_tiles(self._h, int(filter_lanes), int(gather_lanes)))
+
+    def autotune(self, box: Box, type, position, potential, force, virial, *, ann_bf16=False,
+                 stream=None) -> dict:
+        """Times the filter and gather tile shapes on this system and keeps the
+        fastest for every later evaluation; the outputs hold the system's
+        values afterwards."""
+        import torch
+        N = type.numel()
+        t = _lib.Tuning()
+        b = box.to_c()
+        check(lib().dmgx_nep_autotune(
+            self._h, ctypes.byref(b), N, _ptr(type, torch.int32, N, "type"),
+            _ptr(position, torch.float64, 3 * N, "position"), _ptr(potential, torch.float64, N, "potential"),
+            _ptr(force, torch.float64, 3 * N, "force"), _ptr(virial, torch.float64, 9 * N, "virial"),
+            _lib.ANN_BF16 if ann_bf16 else 0, _stream(stream), ctypes.byref(t)))
+        return _tuning_dict(t)
+
     def _rows(self, fn, args, c1, c2, max_nb):
         n = c2 - c1
         counts = np.zeros(max(n, 1), np.int32)
@@ -248,6 +274,12 @@ class MD:
         check(lib().dmgx_md_box(self._h, ctypes.byref(b)))
         return Box(np.array(b.h[:]).reshape(3, 3), tuple(b.pbc[:]))
 
+    def autotune(self, ann_bf16=False) -> dict:
+        """NEP.autotune on the state's current positions."""
+        t = _lib.Tuning()
+        check(lib().dmgx_md_autotune(self._h, _lib.ANN_BF16 if ann_bf16 else 0, ctypes.byref(t)))
+        return _tuning_dict(t)
+
 
 class Server:
     """Continuous-batching server of independent periodic systems (no
@@ -315,6 +347,11 @@ class Server:
 
 
 _hip = None
+
+
+def _tuning_dict(t) -> dict:
+    return {"filter_lanes": t.filter_lanes, "gather_lanes": t.gather_lanes, "tuned": bool(t.tuned),
+            "filter_ms": dict(zip((8, 16, 32), t.filter_ms[:])), "gather_ms": dict(zip((4, 8, 16), t.gather_ms[:]))}
 
 
 def _copy_d2h(dev_ptr, host: np.ndarray) -> None:

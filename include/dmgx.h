@@ -124,6 +124,30 @@ int dmgx_nep_set_profiling(dmgx_nep* nep, int enable);
 int dmgx_nep_profile_read(dmgx_nep* nep, int kernel, int* launches, double* total_ms);
 int dmgx_nep_profile_reset(dmgx_nep* nep);
 
+/* Tile shapes of the per-step kernels (no counterpart in the reference, whose
+ * launch shapes are fixed): lanes per centre of the radial/angular filter
+ * (8, 16 or 32) and lanes per atom of the force gather (4, 8 or 16), with the
+ * average milliseconds each candidate took when the autotuner timed it (0
+ * before).  Defaults 16 and 8; tuned = 1 once the autotuner has run.  Boxes
+ * small enough to be replicated keep the defaults. */
+typedef struct dmgx_tuning {
+  int filter_lanes, gather_lanes;
+  double filter_ms[3]; /* candidates 8, 16, 32 */
+  double gather_ms[3]; /* candidates 4, 8, 16 */
+  int tuned;
+} dmgx_tuning;
+int dmgx_nep_get_tuning(const dmgx_nep* nep, dmgx_tuning* out);
+/* Sets the shapes directly (DMGX_EINVAL for values other than the
+ * candidates); a server bound to the handle keeps them (no autotuning). */
+int dmgx_nep_set_tiles(dmgx_nep* nep, int filter_lanes, int gather_lanes);
+/* Times every candidate on this system (positions in device memory, as
+ * dmgx_nep_compute) and keeps the fastest for all later evaluations of this
+ * handle; afterwards pe/force/virial hold the system's values as the chosen
+ * shapes compute them.  Synchronises the stream. */
+int dmgx_nep_autotune(dmgx_nep* nep, const dmgx_box* box, int N, const int* type,
+                      const double* position, double* potential, double* force, double* virial,
+                      unsigned flags, void* stream, dmgx_tuning* out);
+
 /* Copies the current Verlet rows of centres [c1,c2) to host memory, sorted
  * ascending by slot (test access to the row contract of neighbor.cu:141-218):
  * counts_host[c2-c1], rows_host[(c2-c1)*max_nb]. */
@@ -227,6 +251,9 @@ int dmgx_md_run(dmgx_md* md, int steps, double dt_fs, const dmgx_ensemble* ens, 
 int dmgx_md_arrays(dmgx_md* md, double** position, double** velocity, double** force,
                    double** potential, double** virial, double** mass, int** type);
 int dmgx_md_box(dmgx_md* md, dmgx_box* box_out);
+/* dmgx_nep_autotune on the state's current positions (wrapped into the box
+ * first); the state's forces are those positions' forces afterwards. */
+int dmgx_md_autotune(dmgx_md* md, unsigned flags, dmgx_tuning* out);
 
 /* ---------------------------------------------------------------------- */
 /* Serving loop: continuous batching of independent periodic systems.      */

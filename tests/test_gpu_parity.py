@@ -400,3 +400,63 @@ def test_nep5_and_zbl_forms_against_oracle(dmgx, name, tiles):
     assert err["energy_atom_max"] <= max(TOL_EA, 2e-7 * emax), (err, emax)
     assert err["energy_per_atom"] <= max(TOL_E, 2e-7 * emax), (err, emax)
     assert err["virial"] <= max(TOL_V, 2e-6 * np.abs(ref["virial"]).max()), err
+
+
+def test_every_tile_shape_against_oracle(dmgx, workloads):
+    """Each filter / gather shape the autotuner may pick gives the oracle's
+    forces, energies and virials (Cu fcc 10^3, 4000 atoms)."""
+    s, path = _system(workloads, "c1")
+    ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
+    nep = dmgx.NEP(path)
+    for fl in (8, 16, 32):
+        for gl in (4, 8, 16):
+            nep.set_tiles(fl, gl)
+            t = nep.tuning()
+            assert (t["filter_lanes"], t["gather_lanes"]) == (fl, gl)
+            check(errors(gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"], force_rebuild=True), ref))
+    with pytest.raises(dmgx.DmgxError):
+        nep.set_tiles(12, 8)
+    with pytest.raises(dmgx.DmgxError):
+        nep.set_tiles(16, 32)
+
+
+def test_autotune_picks_a_candidate_and_keeps_parity(dmgx, workloads):
+    import torch
+    s, path = _system(workloads, "c0")
+    ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
+    nep = dmgx.NEP(path)
+    assert nep.tuning() == {"filter_lanes": 16, "gather_lanes": 8, "tuned": False,
+                            "filter_ms": {8: 0.0, 16: 0.0, 32: 0.0}, "gather_ms": {4: 0.0, 8: 0.0, 16: 0.0}}
+    N = len(s["types"])
+    t, x = to_dev(s["types"], torch.int32), to_dev(s["pos"], torch.float64)
+    pe = torch.zeros(N, dtype=torch.float64, device="cuda")
+    f = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+    v = torch.zeros(9 * N, dtype=torch.float64, device="cuda")
+    tune = nep.autotune(s["box"], t, x, pe, f, v)
+    assert tune["tuned"] and tune["filter_lanes"] in (8, 16, 32) and tune["gather_lanes"] in (4, 8, 16)
+    assert all(ms > 0 for ms in tune["filter_ms"].values()) and all(ms > 0 for ms in tune["gather_ms"].values())
+    assert tune["filter_ms"][tune["filter_lanes"]] == min(tune["filter_ms"].values())
+    assert tune["gather_ms"][tune["gather_lanes"]] == min(tune["gather_ms"].values())
+    # the autotuner's own outputs, then a later evaluation, against the oracle
+    got = (pe.cpu().numpy(), f.cpu().numpy().reshape(3, N), v.cpu().numpy().reshape(9, N))
+    check(errors(got, ref))
+    check(errors(gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"]), ref))
+    assert nep.tuning() == tune
+
+
+def test_md_autotune_leaves_the_state_forces(dmgx, workloads):
+    s, path = _system(workloads, "c1")
+    nep = dmgx.NEP(path)
+    md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    tune = md.autotune()
+    assert tune["tuned"]
+    a = md.arrays()
+    ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"],
+                                                             a["position"])
+    check(errors((a["potential"], a["force"], a["virial"]), ref))
+    # a run after tuning continues from those forces like an untuned state
+    plain = dmgx.MD(dmgx.NEP(path), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    th_t = md.run(20, 1.0, "nve")
+    th_p = plain.run(20, 1.0, "nve")
+    assert abs(th_t[1] - th_p[1]) <= 1e-6 * len(s["types"])
+    assert np.abs(md.arrays()["position"] - plain.arrays()["position"]).max() < 1e-9
