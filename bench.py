@@ -227,7 +227,8 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
             "latency_run_frames_per_s": lat["frames_per_s"],
             "bf16_force_err_max_eV_per_A": err,
             "tile_shapes": {"filter_lanes_per_centre": tune["filter_lanes"],
-                            "gather_lanes_per_atom": tune["gather_lanes"], "autotuned": tune["tuned"]},
+                            "gather_lanes_per_atom": tune["gather_lanes"], "autotuned": tune["tuned"],
+                            "row_capacity": tune["row_capacity"], "row_layout": tune["row_layout"]},
             "plain_fp32_frames_per_s": plain, "plain_sample_requests": plain_requests,
             "speedup_vs_plain": st["frames_per_s"] / plain if plain > 0 else None}
 
@@ -341,7 +342,10 @@ def main():
                             "gather_lanes_per_atom": tune["gather_lanes"],
                             "autotuned": tune["tuned"],
                             "filter_ms_by_lanes": {str(k): round(v, 4) for k, v in tune["filter_ms"].items()},
-                            "gather_ms_by_lanes": {str(k): round(v, 4) for k, v in tune["gather_ms"].items()}},
+                            "gather_ms_by_lanes": {str(k): round(v, 4) for k, v in tune["gather_ms"].items()},
+                            "row_capacity": tune["row_capacity"],
+                            "row_layout": {0: "multiple of 4", 1: "power of 2"}.get(tune["row_layout"], "default"),
+                            "force_ms_by_row_layout": {k: round(v, 4) for k, v in tune["force_ms"].items()}},
             "final_temperature_K": th[0],
         },
         "roofline": {

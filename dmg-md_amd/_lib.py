@@ -63,7 +63,8 @@ class ServerStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("filter_lanes", ctypes.c_int), ("gather_lanes", ctypes.c_int),
-                ("filter_ms", ctypes.c_double * 3), ("gather_ms", ctypes.c_double * 3), ("tuned", ctypes.c_int)]
+                ("filter_ms", ctypes.c_double * 3), ("gather_ms", ctypes.c_double * 3), ("tuned", ctypes.c_int),
+                ("row_layout", ctypes.c_int), ("row_capacity", ctypes.c_int), ("force_ms", ctypes.c_double * 2)]
 
 
 ACCUMULATE = 1

@@ -186,6 +186,10 @@ void fill_tuning(const dmgx::NepEngine::Tuning& t, dmgx_tuning* out)
     out->gather_ms[k] = t.gather_ms[k];
   }
   out->tuned = t.tuned ? 1 : 0;
+  out->row_layout = t.row_layout;
+  out->row_capacity = t.row_capacity;
+  out->force_ms[0] = t.force_ms[0];
+  out->force_ms[1] = t.force_ms[1];
 }
 }  // namespace
 

@@ -252,6 +252,7 @@ void NepEngine::free_all()
   dfree(&w_.cell_of);
   dfree(&w_.cell_rank);
   dfree(&w_.cell_atoms);
+  dfree(&w_.xs);
   dfree(&w_.vn);
   dfree(&w_.vl);
   dfree(&w_.vrev);
@@ -271,8 +272,26 @@ void NepEngine::free_all()
   have_list_ = false;
 }
 
+// Row capacity in a layout: a multiple of 4 (16-byte row chunks in the
+// Verlet build's sort), or a power of two; layout -1 takes the power of two
+// where it costs at most 1.25x.
+int NepEngine::padded_rows(int mv, int layout) const
+{
+  const int r4 = (mv + 3) & ~3;
+  int p2 = 4;
+  while (p2 < mv) p2 <<= 1;
+  if (layout < 0) layout = (p2 <= 1.25 * mv) ? 1 : 0;
+  return layout == 1 ? p2 : r4;
+}
+
+// device bytes per row entry: Verlet slot, reverse index, edge record,
+// edge reverse, Verlet->angular index, angular partial force
+constexpr double kRowEntryBytes = 4 + 4 + 16 + 4 + 4 + 16;
+
 void NepEngine::allocate(int capacity, int mv)
 {
+  mv_base_ = mv;
+  mv = padded_rows(mv, tune_.row_layout);
   const size_t cap = (size_t)std::max(capacity, 1);
   const size_t rows = cap * (size_t)mv;
   if (rows >= (size_t)INT32_MAX)
@@ -281,6 +300,7 @@ void NepEngine::allocate(int capacity, int mv)
   dalloc(&w_.cell_of, cap, "cell_of");
   dalloc(&w_.cell_rank, cap, "cell rank");
   dalloc(&w_.cell_atoms, cap, "cell_atoms");
+  dalloc(&w_.xs, cap, "cell-ordered positions");
   dalloc(&w_.vn, cap, "vn");
   dalloc(&w_.vl, rows, "verlet rows");
   dalloc(&w_.vrev, rows, "verlet reverse");
@@ -294,6 +314,7 @@ void NepEngine::allocate(int capacity, int mv)
   dalloc(&w_.f12, rows, "angular partials");
   w_.cap = (int)cap;
   w_.MV = mv;
+  tune_.row_capacity = mv;
   w_.MR = mv;
   w_.MA = mv;
   have_list_ = false;
@@ -303,7 +324,7 @@ void NepEngine::reserve(int capacity)
 {
   if (capacity < 0) throw std::invalid_argument("negative capacity");
   if (capacity <= w_.cap && w_.MV > 0) return;
-  const int mv = std::max(w_.MV, (int)std::ceil(model_.MN_R * std::pow((model_.rcR_max + kSkin) / model_.rcR_max, 3)));
+  const int mv = std::max(mv_base_, (int)std::ceil(model_.MN_R * std::pow((model_.rcR_max + kSkin) / model_.rcR_max, 3)));
   allocate(std::max(capacity, w_.cap), std::max(mv, 8));
 }
 
@@ -312,7 +333,7 @@ void NepEngine::ensure_cells(int ncell)
   if (ncell <= w_.ncell_cap) return;
   dalloc(&w_.cell_count, (size_t)ncell, "cell counts");
   dalloc(&w_.cell_start, (size_t)ncell + 1, "cell starts");
-  dalloc(&w_.cell_tiles, 2 * ((size_t)ncell / 4096 + 2), "cell scan tiles");
+  dalloc(&w_.cell_tiles, 2 * ((size_t)ncell / 4096 + 2) + 8, "cell scan tiles");
   check(hipMemset(w_.cell_count, 0, sizeof(int) * ncell), "clear cell counts");
   w_.ncell_cap = ncell;
 }
@@ -533,6 +554,15 @@ void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2,
   throw std::runtime_error("Verlet rows kept overflowing");
 }
 
+void NepEngine::launch_step_kernels(const DevBox& b, int N, int nc, const int* type, double* pe, double* force,
+                                    double* virial, unsigned flags, hipStream_t s)
+{
+  const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
+  check(launch_filter(b, dm_, 0, nc, w_, s, tune_.filter_lanes), "filter");
+  check(launch_descriptor(dm_, N, 0, nc, type, pe, false, ann, w_, s), "descriptor");
+  check(launch_gather(dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, tune_.gather_lanes), "gather");
+}
+
 const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, const int* type, const double* pos,
                                               double* pe, double* force, double* virial, unsigned flags,
                                               hipStream_t s, const ImageMap* map)
@@ -541,44 +571,82 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
   if (N <= 0 || nc <= 0 || nc > N || small_box(box, reps)) return tune_;  // small boxes keep the defaults
   flags &= ~1u;
   compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags, s, map);
+  flags &= ~2u;
   // rows, edges, adjoints and partials are current: the filter rewrites its
   // outputs identically and the gather overwrites the forces, so every
   // candidate can be timed on the same inputs
   const DevBox b = make_devbox(box);
-  hipEvent_t e0, e1;
-  check(hipEventCreate(&e0), "tune event");
-  check(hipEventCreate(&e1), "tune event");
-  auto time3 = [&](const std::function<void()>& launch) {
-    launch();  // warm-up (and first-launch costs)
-    check(hipEventRecord(e0, s), "tune event");
-    for (int k = 0; k < 3; ++k) launch();
-    check(hipEventRecord(e1, s), "tune event");
-    check(hipEventSynchronize(e1), "tune event");
-    float ms = 0.0f;
-    check(hipEventElapsedTime(&ms, e0, e1), "tune event");
-    return ms / 3.0;
+  // each candidate: one warm-up launch, then kReps launches, each between
+  // two consecutive events; its time is the median launch
+  constexpr int kReps = 7;
+  hipEvent_t ev[kReps + 1];
+  for (hipEvent_t& e : ev) check(hipEventCreate(&e), "tune event");
+  auto time_median = [&](const std::function<void()>& launch) {
+    launch();
+    check(hipEventRecord(ev[0], s), "tune event");
+    for (int k = 0; k < kReps; ++k) {
+      launch();
+      check(hipEventRecord(ev[k + 1], s), "tune event");
+    }
+    check(hipEventSynchronize(ev[kReps]), "tune event");
+    float ms[kReps];
+    for (int k = 0; k < kReps; ++k) check(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]), "tune event");
+    std::sort(ms, ms + kReps);
+    return (double)ms[kReps / 2];
   };
-  const int fl[3] = {8, 16, 32}, gl[3] = {4, 8, 16};
-  int best = 1;
-  for (int k = 0; k < 3; ++k) {
-    tune_.filter_ms[k] = time3([&] { check(launch_filter(b, dm_, 0, nc, w_, s, fl[k]), "tune filter"); });
-    if (tune_.filter_ms[k] < tune_.filter_ms[best]) best = k;
+  // lanes of the filter and gather in the current row layout, then the
+  // whole force sequence with them
+  auto tune_lanes = [&](int layout) {
+    const int fl[3] = {8, 16, 32}, gl[3] = {4, 8, 16};
+    int best = 1;
+    for (int k = 0; k < 3; ++k) {
+      tune_.filter_ms[k] = time_median([&] { check(launch_filter(b, dm_, 0, nc, w_, s, fl[k]), "tune filter"); });
+      if (tune_.filter_ms[k] < tune_.filter_ms[best]) best = k;
+    }
+    tune_.filter_lanes = fl[best];
+    best = 1;
+    for (int k = 0; k < 3; ++k) {
+      tune_.gather_ms[k] = time_median([&] {
+        check(launch_gather(dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, gl[k]), "tune gather");
+      });
+      if (tune_.gather_ms[k] < tune_.gather_ms[best]) best = k;
+    }
+    tune_.gather_lanes = gl[best];
+    tune_.force_ms[layout] = time_median([&] { launch_step_kernels(b, N, nc, type, pe, force, virial, flags, s); });
+  };
+  const int layout0 = (w_.MV == padded_rows(mv_base_, 1)) ? 1 : 0;
+  tune_.row_layout = layout0;
+  tune_lanes(layout0);
+  // the other row layout, where it differs and its rows fit comfortably
+  const int other = 1 - layout0;
+  const int mv_other = padded_rows(mv_base_, other);
+  size_t free_b = 0, total_b = 0;
+  check(hipMemGetInfo(&free_b, &total_b), "memory info");
+  const double extra = (double)w_.cap * std::max(0, mv_other - w_.MV) * kRowEntryBytes;
+  if (mv_other != w_.MV && (double)w_.cap * mv_other < (double)INT32_MAX && extra < 0.5 * (double)free_b) {
+    const Tuning first = tune_;
+    tune_.row_layout = other;
+    bool ok = true;
+    try {
+      allocate(w_.cap, mv_base_);
+      compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags | 2u, s, map);
+    } catch (const std::runtime_error&) {
+      ok = false;  // no room for the other layout's rows: keep the first
+      (void)hipGetLastError();
+    }
+    if (ok) tune_lanes(other);
+    if (!ok || !(tune_.force_ms[other] < first.force_ms[layout0])) {
+      const double t_other = ok ? tune_.force_ms[other] : 0.0;
+      tune_ = first;
+      tune_.force_ms[other] = t_other;
+      allocate(w_.cap, mv_base_);
+    }
   }
-  tune_.filter_lanes = fl[best];
-  best = 1;
-  for (int k = 0; k < 3; ++k) {
-    tune_.gather_ms[k] = time3([&] {
-      check(launch_gather(dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, gl[k]), "tune gather");
-    });
-    if (tune_.gather_ms[k] < tune_.gather_ms[best]) best = k;
-  }
-  tune_.gather_lanes = gl[best];
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
   tune_.tuned = true;
-  // the outputs as a step with the chosen shapes writes them (the list is
-  // current, so this is one filter, descriptor and gather)
-  compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags & ~2u, s, map);
+  // the outputs as a step with the chosen shapes writes them (a rebuild
+  // where the rows were re-allocated)
+  compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags, s, map);
   check(hipStreamSynchronize(s), "tune");
   return tune_;
 }

@@ -105,14 +105,23 @@ class NepEngine {
                     double* pe, double* force, double* virial, unsigned flags, hipStream_t stream);
   DevWork& work() { return w_; }
   // Tile shapes picked by autotune(): lanes per centre of the filter (8, 16,
-  // 32) and per atom of the gather (4, 8, 16).  autotune() evaluates the
-  // given system once (overwriting pe/force/virial), times every candidate
-  // of the two kernels on it (HIP events, one warm-up and three timed
-  // launches each), keeps the fastest and leaves the outputs as an
-  // evaluation with the chosen shapes writes them.
+  // 32) and per atom of the gather (4, 8, 16), and the row layout: the
+  // capacity of every per-centre row (Verlet, edge, partial-force rows)
+  // rounded up to a multiple of 4 (layout 0) or to a power of two (layout
+  // 1).  Before tuning the power of two is used where it costs at most 1.25x
+  // the rows.  autotune() evaluates the given system once (overwriting
+  // pe/force/virial), times every lane candidate of the two kernels on it
+  // (one warm-up launch, then the median of seven launches, each between two
+  // HIP events), then the filter + descriptor + gather sequence in each row
+  // layout (re-allocating and rebuilding the rows for the other one), keeps
+  // the fastest and leaves the outputs as an evaluation with the chosen
+  // shapes writes them.
   struct Tuning {
     int filter_lanes = 16, gather_lanes = 8;
     double filter_ms[3] = {0, 0, 0}, gather_ms[3] = {0, 0, 0};  // per candidate, in the order above
+    int row_layout = -1;                // -1 before tuning (the default rule), else 0 or 1
+    int row_capacity = 0;               // entries per row now allocated
+    double force_ms[2] = {0, 0};        // filter + descriptor + gather per row layout (0: not timed)
     bool tuned = false;
   };
   const Tuning& tuning() const { return tune_; }
@@ -154,6 +163,10 @@ class NepEngine {
   int last_pbc_[3] = {};
   ImageMap last_map_;
   Tuning tune_;
+  int mv_base_ = 0;  // row capacity asked for, before the layout's rounding
+  int padded_rows(int mv, int layout) const;
+  void launch_step_kernels(const DevBox& b, int N, int nc, const int* type, double* pe, double* force,
+                           double* virial, unsigned flags, hipStream_t s);
   int last_N_stats_ = 0, last_c1_ = 0, last_c2_ = 0;
   Profiler prof_;
   // replicated small-box system (atoms x replicas)

@@ -79,7 +79,8 @@ struct DevWork {
   int* cell_count = nullptr;   // [ncell] zero between rebuilds
   int* cell_start = nullptr;   // [ncell+1]
   int* cell_atoms = nullptr;   // [cap]
-  int* cell_tiles = nullptr;   // [ncell/4096 + 2] tile sums, then offsets, of the multi-block scan
+  double4* xs = nullptr;       // [cap] xq in cell_atoms order (rebuild only)
+  int* cell_tiles = nullptr;   // [2 (ncell/4096 + 2) + 8] tile sums, then (16-byte aligned) offsets, of the multi-block scan
   int* vn = nullptr;           // [cap] Verlet row length
   int* vl = nullptr;           // [cap*MV] Verlet rows
   int* vrev = nullptr;         // [cap*MV] position of the centre in the neighbour's row

@@ -74,47 +74,84 @@ __global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const 
   x0[2 * N + i] = p.z;
 }
 
-// Exclusive scan of the cell counts by one workgroup in tiles of 4096, which
-// also clears the counts for the next rebuild; start[ncell] = total.
+// Exclusive scan of the cell counts by one workgroup, which also clears the
+// counts for the next rebuild; start[ncell] = total.  Each pass covers 4
+// sub-tiles of 4096 cells, 4 consecutive cells per thread in each (one
+// 16-byte access per thread, contiguous across the wavefront), scanned
+// together; the next pass's counts are loaded before this pass is scanned.
 __global__ __launch_bounds__(1024) void k_scan_cells(const int* __restrict__ go, int ncell,
                                                      int* __restrict__ count, int* __restrict__ start)
 {
   if (!*go) return;
-  __shared__ int wsum[16];
+  constexpr int kSub = 4096, kChunks = 4, kTile = kSub * kChunks;
+  __shared__ int wsum[kChunks][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int carry = 0;
-  for (int base = 0; base < ncell; base += 4096) {
-    const int k0 = base + threadIdx.x * 4;
-    int v[4], s = 0;
+  auto load = [&](int base, int4* v) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      v[k] = k0 + k < ncell ? count[k0 + k] : 0;
-      s += v[k];
+    for (int c = 0; c < kChunks; ++c) {
+      const int k = base + c * kSub + threadIdx.x * 4;
+      if (k + 3 < ncell) {
+        v[c] = *reinterpret_cast<const int4*>(count + k);
+      } else {
+        v[c].x = k < ncell ? count[k] : 0;
+        v[c].y = k + 1 < ncell ? count[k + 1] : 0;
+        v[c].z = k + 2 < ncell ? count[k + 2] : 0;
+        v[c].w = 0;
+      }
     }
-    // inclusive scan of s over the wavefront
-    int x = s;
+  };
+  int4 v[kChunks], nv[kChunks];
+  load(0, v);
+  int carry = 0;
+  for (int base = 0; base < ncell; base += kTile) {
+    if (base + kTile < ncell) load(base + kTile, nv);
+    int s[kChunks], x[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+      s[c] = v[c].x + v[c].y + v[c].z + v[c].w;
+      x[c] = s[c];
+    }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(x, off, 64);
-      if (lane >= off) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    int before = carry, tile_total = 0;
-    for (int q = 0; q < 16; ++q) {
-      if (q < wave) before += wsum[q];
-      tile_total += wsum[q];
-    }
-    int run = before + x - s;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k0 + k < ncell) {
-        start[k0 + k] = run;
-        count[k0 + k] = 0;
+      for (int c = 0; c < kChunks; ++c) {
+        const int y = __shfl_up(x[c], off, 64);
+        if (lane >= off) x[c] += y;
       }
-      run += v[k];
     }
-    carry += tile_total;
+    if (lane == 63) {
+#pragma unroll
+      for (int c = 0; c < kChunks; ++c) wsum[c][wave] = x[c];
+    }
+    __syncthreads();
+    int run_base = carry;
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+      int before = 0, total = 0;
+      for (int q = 0; q < 16; ++q) {
+        const int t = wsum[c][q];
+        if (q < wave) before += t;
+        total += t;
+      }
+      const int k = base + c * kSub + threadIdx.x * 4;
+      int4 o;
+      o.x = run_base + before + x[c] - s[c];
+      o.y = o.x + v[c].x;
+      o.z = o.y + v[c].y;
+      o.w = o.z + v[c].z;
+      if (k + 3 < ncell) {
+        *reinterpret_cast<int4*>(start + k) = o;
+        *reinterpret_cast<int4*>(count + k) = make_int4(0, 0, 0, 0);
+      } else {
+        if (k < ncell) start[k] = o.x, count[k] = 0;
+        if (k + 1 < ncell) start[k + 1] = o.y, count[k + 1] = 0;
+        if (k + 2 < ncell) start[k + 2] = o.z, count[k + 2] = 0;
+      }
+      run_base += total;
+    }
+    carry = run_base;
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) v[c] = nv[c];
     __syncthreads();
   }
   if (threadIdx.x == 0) start[ncell] = carry;
@@ -172,12 +209,15 @@ __global__ void k_scan_offsets(const int* __restrict__ go, int ncell, const int*
 }
 
 __global__ void k_cell_fill(const int* __restrict__ go, int N, const int* __restrict__ cell_of,
-                            const int* __restrict__ rank, const int* __restrict__ start, int* __restrict__ atoms)
+                            const int* __restrict__ rank, const int* __restrict__ start,
+                            const double4* __restrict__ xq, int* __restrict__ atoms, double4* __restrict__ xs)
 {
   if (!*go) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  atoms[start[cell_of[i]] + rank[i]] = i;
+  const int at = start[cell_of[i]] + rank[i];
+  atoms[at] = i;
+  xs[at] = xq[i];
 }
 
 // Row key order: slot, or (global id, slot).
@@ -190,12 +230,29 @@ __device__ __forceinline__ bool key_less(const uint64_t* gid, int a, int b)
   return a < b;
 }
 
-// One kRowLanes group per centre: the 5x5 rows of cells around it (cell edge
-// >= (rc+skin)/2, so +-2 cells cover rc+skin) are scanned as contiguous atom
-// ranges (one per row, two where a periodic row wraps), accepted slots are
-// compacted into LDS with a ballot, rank-sorted there on (key, slot) and
-// written out as one contiguous row.  Rows past the capacity MV are counted
-// but not stored; the count is reported through the flags.
+// Candidate ranges of one centre: the 5x5 rows of cells around it (cell edge
+// >= (rc+skin)/2, so +-2 cells cover rc+skin), each a contiguous range of
+// cell_atoms, split in two where a periodic row wraps: at most 50 ranges.
+constexpr int kRanges = 64;
+constexpr int kRangesPerLane = kRanges / kRowLanes;
+constexpr size_t kRangeLds = 2 * kRanges * sizeof(int);  // starts + prefix, per centre
+
+// LDS bytes of one centre's row buffers: 64-bit keys (rows keyed by global
+// id only), the accepted slots and the sorted row; a multiple of 8
+__host__ __device__ inline size_t build_row_lds(int mv, bool keyed)
+{
+  return (keyed ? 16 : 8) * (size_t)mv;
+}
+
+// One kRowLanes group per centre.  The group first reads the bounds of all
+// its ranges at once and prefix-sums their lengths, then streams the
+// candidates as one list: candidate k is slot rs[q] + k - rp[q] of
+// cell_atoms / xs (the cell-ordered copy of the positions), so a candidate
+// costs two independent contiguous loads; each step tests two candidates
+// per lane whose loads were issued a step earlier.  Accepted slots are compacted into LDS
+// with a ballot, rank-sorted there on (key, slot) and written out as one
+// contiguous row.  Rows past the capacity MV are counted but not stored;
+// the count is reported through the flags.
 __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go, DevBox b, int c1, int c2,
                                                       int groups, const double4* __restrict__ xq, double rv2,
                                                       const uint64_t* __restrict__ gid, DevWork w)
@@ -204,9 +261,13 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
   extern __shared__ uint64_t shk[];
   const int g = threadIdx.x / kRowLanes, lg = threadIdx.x % kRowLanes;
   const int gw = g % (64 / kRowLanes);  // group inside the wavefront
-  uint64_t* key = shk + (size_t)g * 2 * w.MV;
-  int* row = reinterpret_cast<int*>(key + w.MV);
+  // per centre: [keys (with gid only) | row | sorted row], then the ranges
+  const size_t per = build_row_lds(w.MV, gid != nullptr) / sizeof(uint64_t);
+  uint64_t* key = shk + (size_t)g * per;
+  int* row = reinterpret_cast<int*>(gid ? key + w.MV : key);
   int* sorted = row + w.MV;
+  int* rs = reinterpret_cast<int*>(shk + (size_t)groups * per) + g * 2 * kRanges;
+  int* rp = rs + kRanges;
   const int i = c1 + blockIdx.x * groups + g;
   const bool active = i < c2;
   double4 pi = make_double4(0, 0, 0, 0);
@@ -215,16 +276,22 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
     pi = xq[i];
     cell_coords(b, pi.x, pi.y, pi.z, c);
   }
-  int cnt = 0;
-  for (int dz = -2; dz <= 2; ++dz) {
-    for (int dy = -2; dy <= 2; ++dy) {
-      int cz = c[2] + dz, cy = c[1] + dy;
-      bool ok = active;
+  // ranges q = 2 * (5 (dz+2) + dy+2) + part, lane lg holding q in
+  // [lg*kRangesPerLane, (lg+1)*kRangesPerLane)
+  int len[kRangesPerLane];
+  int tot = 0;
+#pragma unroll
+  for (int u = 0; u < kRangesPerLane; ++u) {
+    const int q = lg * kRangesPerLane + u;
+    int s = 0, e = 0;
+    if (active && q < 50) {
+      const int r = q >> 1, part = q & 1;
+      int cz = c[2] + r / 5 - 2, cy = c[1] + r % 5 - 2;
+      bool ok = true;
       if (b.pbc[2]) cz = (cz + b.nc[2]) % b.nc[2];
       else ok = ok && cz >= 0 && cz < b.nc[2];
       if (b.pbc[1]) cy = (cy + b.nc[1]) % b.nc[1];
       else ok = ok && cy >= 0 && cy < b.nc[1];
-      // cells [lo, hi] of this row, split in two ranges where the row wraps
       int lo0 = c[0] - 2, hi0 = c[0] + 2, lo1 = 0, hi1 = -1;
       if (b.pbc[0]) {
         if (lo0 < 0) {
@@ -240,51 +307,115 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
         lo0 = max(lo0, 0);
         hi0 = min(hi0, b.nc[0] - 1);
       }
-      const int base = b.nc[0] * (cy + b.nc[1] * cz);
-      for (int part = 0; part < 2; ++part) {
-        int s = 0, e = 0;
-        const int lo = part ? lo1 : lo0, hi = part ? hi1 : hi0;
-        if (ok && hi >= lo) {
-          s = w.cell_start[base + lo];
-          e = w.cell_start[base + hi + 1];
-        }
-        for (int k0 = s; __any(k0 < e); k0 += kRowLanes) {
-          const int k = k0 + lg;
-          bool take = false;
-          int j = 0;
-          if (k < e) {
-            j = w.cell_atoms[k];
-            if (j != i) {
-              const double4 pj = xq[j];
-              double x = pj.x - pi.x, y = pj.y - pi.y, z = pj.z - pi.z;
-              mic(b, x, y, z);
-              take = dist2(x, y, z) < rv2;
-            }
-          }
-          const unsigned gm = (unsigned)(__ballot(take) >> (gw * kRowLanes)) & ((1u << kRowLanes) - 1u);
-          if (take) {
-            const int at = cnt + __popc(gm & ((1u << lg) - 1u));
-            if (at < w.MV) {
-              row[at] = j;
-              key[at] = gid ? gid[j] : 0ull;
-            }
-          }
-          cnt += __popc(gm);
+      const int lo = part ? lo1 : lo0, hi = part ? hi1 : hi0;
+      if (ok && hi >= lo) {
+        const int base = b.nc[0] * (cy + b.nc[1] * cz);
+        s = w.cell_start[base + lo];
+        e = w.cell_start[base + hi + 1];
+      }
+    }
+    rs[q] = s;
+    len[u] = e - s;
+    tot += e - s;
+  }
+  int incl = tot;
+#pragma unroll
+  for (int d = 1; d < kRowLanes; d <<= 1) {
+    const int v = __shfl_up(incl, d, kRowLanes);
+    if (lg >= d) incl += v;
+  }
+  int off = incl - tot;
+#pragma unroll
+  for (int u = 0; u < kRangesPerLane; ++u) {
+    rp[lg * kRangesPerLane + u] = off;
+    off += len[u];
+  }
+  const int total = __shfl(incl, kRowLanes - 1, kRowLanes);
+  __syncthreads();
+  // k < total lies in range q with rp[q] <= k < rp[q+1] (q < 50, and
+  // rp[50] = total); a lane's k only grows, so q only moves forward
+  int q = 0;
+  auto slot_of = [&](int k) {
+    while (rp[q + 1] <= k) ++q;
+    return rs[q] + (k - rp[q]);
+  };
+  // each step tests candidates k and k + kRowLanes of every lane, whose
+  // loads were issued in the step before
+  constexpr int kStep = 2 * kRowLanes;
+  int jn[2] = {0, 0};
+  double4 pn[2] = {make_double4(0, 0, 0, 0), make_double4(0, 0, 0, 0)};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = lg + h * kRowLanes;
+    if (k < total) {
+      const int sl = slot_of(k);
+      jn[h] = w.cell_atoms[sl];
+      pn[h] = w.xs[sl];
+    }
+  }
+  int cnt = 0;
+  for (int k0 = 0; __any(k0 < total); k0 += kStep) {
+    int jc[2];
+    double4 pc[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      jc[h] = jn[h];
+      pc[h] = pn[h];
+      const int k = k0 + kStep + lg + h * kRowLanes;
+      if (k < total) {
+        const int sl = slot_of(k);
+        jn[h] = w.cell_atoms[sl];
+        pn[h] = w.xs[sl];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = k0 + lg + h * kRowLanes;
+      const int j = jc[h];
+      bool take = false;
+      if (k < total && j != i) {
+        double x = pc[h].x - pi.x, y = pc[h].y - pi.y, z = pc[h].z - pi.z;
+        mic(b, x, y, z);
+        take = dist2(x, y, z) < rv2;
+      }
+      const unsigned gm = (unsigned)(__ballot(take) >> (gw * kRowLanes)) & ((1u << kRowLanes) - 1u);
+      if (take) {
+        const int at = cnt + __popc(gm & ((1u << lg) - 1u));
+        if (at < w.MV) {
+          row[at] = j;
+          if (gid) key[at] = gid[j];
         }
       }
+      cnt += __popc(gm);
     }
   }
   __syncthreads();
   const int n = min(cnt, w.MV);
-  for (int k = lg; k < n; k += kRowLanes) {
-    const uint64_t kv = key[k];
-    const int v = row[k];
-    int r = 0;
-    for (int t = 0; t < n; ++t) {
-      const uint64_t kt = key[t];
-      r += (kt < kv || (kt == kv && row[t] < v)) ? 1 : 0;
+  if (gid) {
+    for (int k = lg; k < n; k += kRowLanes) {
+      const uint64_t kv = key[k];
+      const int v = row[k];
+      int r = 0;
+      for (int t = 0; t < n; ++t) {
+        const uint64_t kt = key[t];
+        r += (kt < kv || (kt == kv && row[t] < v)) ? 1 : 0;
+      }
+      sorted[r] = v;
     }
-    sorted[r] = v;
+  } else {
+    // four entries per LDS read (row is 16-byte aligned: MV is a multiple of 4)
+    const int4* row4 = reinterpret_cast<const int4*>(row);
+    const int n4 = n >> 2;
+    for (int k = lg; k < n; k += kRowLanes) {
+      const int v = row[k];
+      int r = 0;
+      for (int t = 0; t < n4; ++t) {
+        const int4 x = row4[t];
+        r += (x.x < v) + (x.y < v) + (x.z < v) + (x.w < v);
+      }
+      for (int t = n4 * 4; t < n; ++t) r += row[t] < v ? 1 : 0;
+      sorted[r] = v;
+    }
   }
   __syncthreads();
   if (active) {
@@ -298,6 +429,13 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
 
 // Position of the centre inside each neighbour's row (binary search on the
 // row key); -1 when the neighbour has no row (outside the centre range).
+// Rows are symmetric (the same FP64 test from both ends), so when both ends
+// are centres and no image map is given, only the lower slot searches: it
+// finds q (i in j's row) and writes the reverse entries of both directed
+// edges, i -> j and j -> i; every entry is still written exactly once.  A
+// lane searches kRevBatch edges of its centre in lockstep.
+constexpr int kRevBatch = 4;
+
 __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ go, int c1, int c2,
                                                         const uint64_t* __restrict__ gid, DevWork w)
 {
@@ -308,51 +446,88 @@ __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ 
   const int n = min(w.vn[i], w.MV);
   const int* ri = w.vl + (size_t)i * w.MV;
   const ImageMap& im = w.map;
-  for (int p = lg; p < n; p += kRowLanes) {
-    const int j = ri[p];
-    // the reverse of i -> j is found in the row of j; with an image map, j
-    // stands for its owner o = owner[j] shifted by S, and the reverse entry
-    // is o -> (the image of i shifted by -S), one of i's own ghost slots
-    int o = j, t = i;
-    if (im.owner) {
-      o = im.owner[j];
-      const int c = im.image[j];
-      if (c != image_code(0, 0, 0)) {
-        const int want = image_neg(c);
-        t = -1;
-        for (int g = im.gstart[i]; g < im.gstart[i + 1]; ++g)
-          if (im.image[g] == want) {
-            t = g;
-            break;
+  for (int p0 = lg; p0 < n; p0 += kRevBatch * kRowLanes) {
+    int t[kRevBatch], lo[kRevBatch], hi[kRevBatch], q[kRevBatch], ov[kRevBatch];
+    bool own[kRevBatch];
+    const int* ro[kRevBatch];
+#pragma unroll
+    for (int u = 0; u < kRevBatch; ++u) {
+      const int p = p0 + u * kRowLanes;
+      // the reverse of i -> j is found in the row of j; with an image map, j
+      // stands for its owner o = owner[j] shifted by S, and the reverse entry
+      // is o -> (the image of i shifted by -S), one of i's own ghost slots
+      int o = -1, tt = i;
+      if (p < n) {
+        const int j = ri[p];
+        o = j;
+        if (im.owner) {
+          o = im.owner[j];
+          const int c = im.image[j];
+          if (c != image_code(0, 0, 0)) {
+            const int want = image_neg(c);
+            tt = -1;
+            for (int g = im.gstart[i]; g < im.gstart[i + 1]; ++g)
+              if (im.image[g] == want) {
+                tt = g;
+                break;
+              }
           }
-      }
-    }
-    int q = -1;
-    if (t >= 0 && o >= c1 && o < c2) {
-      const int* ro = w.vl + (size_t)o * w.MV;
-      int lo = 0, hi = min(w.vn[o], w.MV) - 1;
-      while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const int v = ro[mid];
-        if (v == t) {
-          q = mid;
-          break;
         }
-        if (key_less(gid, v, t)) lo = mid + 1;
-        else hi = mid - 1;
+      }
+      const bool centre = o >= c1 && o < c2;
+      const bool mine = im.owner || !centre || o > i;
+      own[u] = p < n && mine;
+      const bool search = own[u] && tt >= 0 && centre;
+      t[u] = tt;
+      ov[u] = o;
+      q[u] = -1;
+      lo[u] = 0;
+      hi[u] = search ? min(w.vn[o], w.MV) - 1 : -1;
+      ro[u] = search ? w.vl + (size_t)o * w.MV : ri;  // an in-bounds row for idle probes
+    }
+    bool more = true;
+    while (more) {
+      int v[kRevBatch];
+#pragma unroll
+      for (int u = 0; u < kRevBatch; ++u) v[u] = ro[u][lo[u] <= hi[u] ? (lo[u] + hi[u]) >> 1 : 0];
+      more = false;
+#pragma unroll
+      for (int u = 0; u < kRevBatch; ++u) {
+        if (lo[u] <= hi[u]) {
+          const int mid = (lo[u] + hi[u]) >> 1;
+          if (v[u] == t[u]) {
+            q[u] = mid;
+            lo[u] = hi[u] + 1;
+          } else if (key_less(gid, v[u], t[u])) {
+            lo[u] = mid + 1;
+          } else {
+            hi[u] = mid - 1;
+          }
+        }
+        more = more || lo[u] <= hi[u];
       }
     }
-    w.vrev[(size_t)i * w.MV + p] = q;
+#pragma unroll
+    for (int u = 0; u < kRevBatch; ++u) {
+      const int p = p0 + u * kRowLanes;
+      if (own[u]) {
+        w.vrev[(size_t)i * w.MV + p] = q[u];
+        if (!im.owner && q[u] >= 0) w.vrev[(size_t)ov[u] * w.MV + q[u]] = p;
+      }
+    }
   }
 }
 
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 
+// LDS of one centre of the build: its row buffers and candidate ranges
+size_t build_lds_per_centre(int mv, bool keyed) { return build_row_lds(mv, keyed) + kRangeLds; }
+
 // centres per build workgroup: 16 groups of kRowLanes lanes unless the LDS
-// rows (16 bytes per entry) would exceed 64 KiB
-int build_groups(int mv)
+// would exceed 64 KiB
+int build_groups(int mv, bool keyed)
 {
-  const size_t per = 16 * (size_t)mv;
+  const size_t per = build_lds_per_centre(mv, keyed);
   int groups = 256 / kRowLanes;
   while (groups > 1 && per * groups > 64 * 1024) groups >>= 1;
   return groups;
@@ -389,16 +564,18 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
   } else {
     const int tiles = (ncell + 4095) / 4096;
     int* tile_sum = w.cell_tiles;
-    int* tile_off = w.cell_tiles + tiles + 1;
+    int* tile_off = w.cell_tiles + ((tiles + 4) & ~3);  // 16-byte aligned for the scan's vector stores
     k_scan_tiles<<<tiles, 1024, 0, s>>>(go, ncell, w.cell_count, w.cell_start, tile_sum);
     k_scan_cells<<<1, 1024, 0, s>>>(go, tiles, tile_sum, tile_off);
     k_scan_offsets<<<nblk(ncell), kBlock, 0, s>>>(go, ncell, tile_off, w.cell_start);
   }
-  if (N > 0) k_cell_fill<<<nblk(N), kBlock, 0, s>>>(go, N, w.cell_of, w.cell_rank, w.cell_start, w.cell_atoms);
+  if (N > 0)
+    k_cell_fill<<<nblk(N), kBlock, 0, s>>>(go, N, w.cell_of, w.cell_rank, w.cell_start, w.xq, w.cell_atoms, w.xs);
   if (c2 > c1) {
-    const int groups = build_groups(w.MV);
+    const bool keyed = gid != nullptr;
+    const int groups = build_groups(w.MV, keyed);
     const int nb = (c2 - c1 + groups - 1) / groups;
-    const size_t lds = 16 * (size_t)w.MV * groups;
+    const size_t lds = build_lds_per_centre(w.MV, keyed) * groups;
     k_verlet_build<<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
     constexpr int rows_per_block = 256 / kRowLanes;
     k_verlet_reverse<<<(c2 - c1 + rows_per_block - 1) / rows_per_block, 256, 0, s>>>(go, c1, c2, gid, w);

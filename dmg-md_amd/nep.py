@@ -351,7 +351,9 @@ _hip = None
 
 def _tuning_dict(t) -> dict:
     return {"filter_lanes": t.filter_lanes, "gather_lanes": t.gather_lanes, "tuned": bool(t.tuned),
-            "filter_ms": dict(zip((8, 16, 32), t.filter_ms[:])), "gather_ms": dict(zip((4, 8, 16), t.gather_ms[:]))}
+            "filter_ms": dict(zip((8, 16, 32), t.filter_ms[:])), "gather_ms": dict(zip((4, 8, 16), t.gather_ms[:])),
+            "row_layout": t.row_layout, "row_capacity": t.row_capacity,
+            "force_ms": dict(zip(("multiple_of_4", "power_of_2"), t.force_ms[:]))}
 
 
 def _copy_d2h(dev_ptr, host: np.ndarray) -> None:

@@ -127,14 +127,21 @@ int dmgx_nep_profile_reset(dmgx_nep* nep);
 /* Tile shapes of the per-step kernels (no counterpart in the reference, whose
  * launch shapes are fixed): lanes per centre of the radial/angular filter
  * (8, 16 or 32) and lanes per atom of the force gather (4, 8 or 16), with the
- * average milliseconds each candidate took when the autotuner timed it (0
- * before).  Defaults 16 and 8; tuned = 1 once the autotuner has run.  Boxes
- * small enough to be replicated keep the defaults. */
+ * median milliseconds each candidate took when the autotuner timed it (0
+ * before), and the row layout: every per-centre row (Verlet, edge,
+ * partial-force rows) holds row_capacity entries, the capacity needed
+ * rounded up to a multiple of 4 (layout 0) or to a power of two (layout 1;
+ * -1 before tuning, when the power of two is taken where it costs at most
+ * 1.25x), with the filter + descriptor + gather time of each layout the
+ * autotuner timed.  Defaults 16 and 8; tuned = 1 once the autotuner has run.
+ * Boxes small enough to be replicated keep the defaults. */
 typedef struct dmgx_tuning {
   int filter_lanes, gather_lanes;
   double filter_ms[3]; /* candidates 8, 16, 32 */
   double gather_ms[3]; /* candidates 4, 8, 16 */
   int tuned;
+  int row_layout, row_capacity;
+  double force_ms[2]; /* layouts 0, 1 */
 } dmgx_tuning;
 int dmgx_nep_get_tuning(const dmgx_nep* nep, dmgx_tuning* out);
 /* Sets the shapes directly (DMGX_EINVAL for values other than the

@@ -425,8 +425,9 @@ def test_autotune_picks_a_candidate_and_keeps_parity(dmgx, workloads):
     s, path = _system(workloads, "c0")
     ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
     nep = dmgx.NEP(path)
-    assert nep.tuning() == {"filter_lanes": 16, "gather_lanes": 8, "tuned": False,
-                            "filter_ms": {8: 0.0, 16: 0.0, 32: 0.0}, "gather_ms": {4: 0.0, 8: 0.0, 16: 0.0}}
+    t0 = nep.tuning()
+    assert (t0["filter_lanes"], t0["gather_lanes"], t0["tuned"], t0["row_layout"]) == (16, 8, False, -1)
+    assert set(t0["filter_ms"].values()) == {0.0} and set(t0["gather_ms"].values()) == {0.0}
     N = len(s["types"])
     t, x = to_dev(s["types"], torch.int32), to_dev(s["pos"], torch.float64)
     pe = torch.zeros(N, dtype=torch.float64, device="cuda")
@@ -437,6 +438,10 @@ def test_autotune_picks_a_candidate_and_keeps_parity(dmgx, workloads):
     assert all(ms > 0 for ms in tune["filter_ms"].values()) and all(ms > 0 for ms in tune["gather_ms"].values())
     assert tune["filter_ms"][tune["filter_lanes"]] == min(tune["filter_ms"].values())
     assert tune["gather_ms"][tune["gather_lanes"]] == min(tune["gather_ms"].values())
+    # carbon model: rows of 152 entries (multiple of 4) or 256 (power of two); both layouts timed
+    assert tune["row_layout"] in (0, 1) and tune["row_capacity"] == (152, 256)[tune["row_layout"]]
+    ms = list(tune["force_ms"].values())
+    assert all(m > 0 for m in ms) and ms[tune["row_layout"]] == min(ms)
     # the autotuner's own outputs, then a later evaluation, against the oracle
     got = (pe.cpu().numpy(), f.cpu().numpy().reshape(3, N), v.cpu().numpy().reshape(9, N))
     check(errors(got, ref))
@@ -460,3 +465,26 @@ def test_md_autotune_leaves_the_state_forces(dmgx, workloads):
     th_p = plain.run(20, 1.0, "nve")
     assert abs(th_t[1] - th_p[1]) <= 1e-6 * len(s["types"])
     assert np.abs(md.arrays()["position"] - plain.arrays()["position"]).max() < 1e-9
+
+
+def test_sparse_open_box_uses_the_multi_block_cell_scan(dmgx, workloads):
+    """Two diamond clusters at opposite corners of a 210 A open box: the
+    cell grid has more than 262,144 cells, so the cell counts go through the
+    multi-workgroup scan; rows and forces still match the oracle."""
+    pos, _ = workloads.diamond(2, 3.567)
+    pos = workloads.jitter(pos, 0.05, 11)
+    a = pos + 5.0
+    b = pos + 190.0
+    xyz = np.concatenate([a, b], axis=0)
+    N = xyz.shape[0]
+    box = dmgx.Box.orthogonal(210.0, 210.0, 210.0, pbc=(0, 0, 0))
+    types = np.zeros(N, np.int32)
+    p = np.ascontiguousarray(xyz.T).reshape(-1)
+    nep = dmgx.NEP(C0)
+    out = gpu_compute(dmgx, nep, box, types, p)
+    ref = OracleNEP(C0, round_params_to_float=True).compute(box.h, box.pbc, types, p)
+    check(errors(out, ref))
+    rv = nep.info.rc_radial_max + 1.0
+    rows = nep.verlet_rows(0, N)
+    rows_ref = neighbor_rows(box.h, box.pbc, p, rv, max_nb=1024)
+    assert all(np.array_equal(rows[i], rows_ref[i]) for i in range(N))
