@@ -45,16 +45,19 @@ void upload_angular_constants(const AngularConstants& a)
 // helpers
 // ---------------------------------------------------------------------------
 
-// cosine cutoff (nep_utilities.cuh:317-342)
+// cosine cutoff (nep_utilities.cuh:317-342).  cos and sin of pi d/rc come
+// from the hardware sine/cosine (v_sin_f32 / v_cos_f32), whose argument is in
+// revolutions: 0.5 d/rc lies in [0, 0.5), far inside their [-256, 256]
+// domain, with absolute errors near 1e-7.
 __device__ __forceinline__ void cutoff_fc(float rc, float rcinv, float d, float& fc)
 {
-  fc = d < rc ? 0.5f * cosf(3.14159265f * d * rcinv) + 0.5f : 0.0f;
+  fc = d < rc ? 0.5f * __builtin_amdgcn_cosf(0.5f * d * rcinv) + 0.5f : 0.0f;
 }
 __device__ __forceinline__ void cutoff_fc_fcp(float rc, float rcinv, float d, float& fc, float& fcp)
 {
   if (d < rc) {
-    float sn, cs;
-    sincosf(3.14159265f * d * rcinv, &sn, &cs);
+    const float rev = 0.5f * d * rcinv;
+    const float sn = __builtin_amdgcn_sinf(rev), cs = __builtin_amdgcn_cosf(rev);
     fc = 0.5f * cs + 0.5f;
     fcp = -1.57079633f * sn * rcinv;
   } else {
@@ -562,12 +565,24 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   float Fs[KR];
 #pragma unroll
   for (int k = 0; k < KR; ++k) Fs[k] = 0.0f;
-  // each loop below loads the next edge record before using the current one
-  float4 rn = erow[0 < na ? 0 : w.MR - 1];
-  for (int e = 0; e < nr; ++e) {
-    const float4 r = rn;
-    if (e + 1 < nr) rn = erow[e + 1 < na ? e + 1 : w.MR - 1 - (e + 1 - na)];
-    const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+  // the radial loop keeps kAhead edge records in flight (its work per edge
+  // is short); the angular loops below load the next record before using
+  // the current one and the first record of the next 128-byte line a line
+  // ahead
+  constexpr int kAhead = 8;
+  auto ridx = [&](int e) { return e < na ? e : w.MR - 1 - (e - na); };
+  float4 ring[kAhead];
+#pragma unroll
+  for (int u = 0; u < kAhead; ++u) ring[u] = u < nr ? erow[ridx(u)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int e0 = 0; e0 < nr; e0 += kAhead)
+#pragma unroll
+  for (int u = 0; u < kAhead; ++u) {
+    const int e = e0 + u;
+    if (e >= nr) break;
+    const float4 r = ring[u];
+    if (e + kAhead < nr) ring[u] = erow[ridx(e + kAhead)];
+    const float d2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    const float d = d2 * __builtin_amdgcn_rsqf(d2);  // edges have d > 0
     float rc;
     int tj = 0;
     if (C::one) {
@@ -619,12 +634,21 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
 #pragma unroll
     for (int a = 0; a < NABC; ++a) s[n][a] = 0.0f;
   float zpe = 0.0f;
-  rn = erow[0];
+  float4 rn = erow[0];
+  float4 ahead = erow[8 < na ? 8 : 0];
   for (int e = 0; e < na; ++e) {
     const float4 r = rn;
-    if (e + 1 < na) rn = erow[e + 1];
-    const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
-    const float dinv = 1.0f / d;
+    if (e + 1 < na) {
+      if (((e + 1) & 7) == 0) {
+        rn = ahead;
+        if (e + 9 < na) ahead = erow[e + 9];
+      } else {
+        rn = erow[e + 1];
+      }
+    }
+    const float d2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    const float dinv = __builtin_amdgcn_rsqf(d2);  // hardware 1/sqrt (1 ulp); edges have d > 0
+    const float d = d2 * dinv;
     float rc;
     int tj = 0;
     if (C::one) {
@@ -785,12 +809,21 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
 
   // ---- angular partial forces dE_i/dr_ij
   rn = erow[0];
+  ahead = erow[8 < na ? 8 : 0];
   for (int e = 0; e < na; ++e) {
     const float4 r = rn;
-    if (e + 1 < na) rn = erow[e + 1];
+    if (e + 1 < na) {
+      if (((e + 1) & 7) == 0) {
+        rn = ahead;
+        if (e + 9 < na) ahead = erow[e + 9];
+      } else {
+        rn = erow[e + 1];
+      }
+    }
     const int j = __float_as_int(r.w);
-    const float d = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
-    const float dinv = 1.0f / d;
+    const float d2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    const float dinv = __builtin_amdgcn_rsqf(d2);  // hardware 1/sqrt (1 ulp); edges have d > 0
+    const float d = d2 * dinv;
     const float ux = r.x * dinv, uy = r.y * dinv, uz = r.z * dinv;
     float rc;
     int tj = 0;
@@ -951,7 +984,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const float4 q = rr[r];
-      const float d = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+      // lanes past the row end carry r = 0: d = 0 and 1/d = 0 there
+      const float d2 = q.x * q.x + q.y * q.y + q.z * q.z;
+      const float dinv = jj[r] >= 0 ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+      const float d = d2 * dinv;
       const float rc = ONE ? rcR0 : (float)((m.rcR[ti] + m.rcR[tt[r]]) * 0.5);
       const float rcinv = 1.0f / rc;
       float fc, fcp;
@@ -964,9 +1000,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
         Gi += (ONE ? bi[k] : bo[ONE ? 0 : r][k]) * fp[k];
         Gj += bj[r][k] * fp[k];
       }
-      // lanes past the row end carry r = 0 and contribute nothing
-      const float dinv = jj[r] >= 0 ? 1.0f / d : 0.0f;
-      const float sc = (Gi + Gj) * dinv;
+      const float sc = (Gi + Gj) * dinv;  // zero past the row end
       fx += (double)(sc * q.x);
       fy += (double)(sc * q.y);
       fz += (double)(sc * q.z);
