@@ -227,6 +227,66 @@ NepEngine::NepEngine(const std::string& path)
   dm_.cA = params_ + im.off_cA;
   dm_.qs = params_ + im.off_qs;
   dm_.zblp = params_ + zoff;
+  // The descriptor runs on an exact instance: the model's own sizes, or the
+  // bounded sizes it is padded to.  Padding: Chebyshev coefficients beyond
+  // the model's orders are zero; the descriptor has 3-body order 4 and both
+  // many-body blocks, each model component keeps its ANN input weights and
+  // scaler at its place in that layout, and the added components get zero
+  // weights, so they change neither the energy nor, with zero adjoints, the
+  // forces.  The radial-adjoint pitch stays the model's (the gather reads it).
+  dmd_ = dm_;
+  int bd[4];
+  if (descriptor_bucket(dm_, bd)) {
+    const int nR = m.nR, kR = m.kR, nA = m.nA, kA = m.kA, D = m.dim, H = m.H;
+    const int NRb = bd[0], KRb = bd[1], NAb = bd[2], KAb = bd[3], Db = NRb + NAb * (kMaxL + 2);
+    std::vector<int> col(D);
+    for (int d = 0; d < nR; ++d) col[d] = d;
+    int b = 0;
+    for (; b < m.L_max; ++b)
+      for (int n = 0; n < nA; ++n) col[nR + b * nA + n] = NRb + b * NAb + n;
+    if (m.has222) {
+      for (int n = 0; n < nA; ++n) col[nR + b * nA + n] = NRb + kMaxL * NAb + n;
+      ++b;
+    }
+    if (m.has1111)
+      for (int n = 0; n < nA; ++n) col[nR + b * nA + n] = NRb + (kMaxL + 1) * NAb + n;
+    const size_t n_w0 = (size_t)T * H * Db, n_cR = (size_t)T * T * NRb * KRb, n_cA = (size_t)T * T * NAb * KAb;
+    std::vector<float> pad(n_w0 + Db + n_cR + n_cA, 0.0f);
+    float* pw0 = pad.data();
+    float* pqs = pw0 + n_w0;
+    float* pcR = pqs + Db;
+    float* pcA = pcR + n_cR;
+    for (int t = 0; t < T; ++t)
+      for (int h = 0; h < H; ++h)
+        for (int d = 0; d < D; ++d)
+          pw0[((size_t)t * H + h) * Db + col[d]] = im.data[im.off_w0 + ((size_t)t * H + h) * D + d];
+    for (int d = 0; d < Db; ++d) pqs[d] = 1.0f;
+    for (int d = 0; d < D; ++d) pqs[col[d]] = im.data[im.off_qs + d];
+    for (int pr = 0; pr < T * T; ++pr) {
+      for (int n = 0; n < nR; ++n)
+        for (int k = 0; k < kR; ++k)
+          pcR[((size_t)pr * NRb + n) * KRb + k] = im.data[im.off_cR + ((size_t)pr * nR + n) * kR + k];
+      for (int n = 0; n < nA; ++n)
+        for (int k = 0; k < kA; ++k)
+          pcA[((size_t)pr * NAb + n) * KAb + k] = im.data[im.off_cA + ((size_t)pr * nA + n) * kA + k];
+    }
+    dalloc(&params_pad_, pad.size(), "padded descriptor parameters");
+    check(hipMemcpy(params_pad_, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice),
+          "upload padded parameters");
+    dmd_.nR = NRb;
+    dmd_.kR = KRb;
+    dmd_.nA = NAb;
+    dmd_.kA = KAb;
+    dmd_.L = kMaxL;
+    dmd_.has222 = 1;
+    dmd_.has1111 = 1;
+    dmd_.numL = kMaxL + 2;
+    dmd_.D = Db;
+    dmd_.w0 = params_pad_;
+    dmd_.qs = params_pad_ + n_w0;
+    dmd_.cR = params_pad_ + n_w0 + Db;
+    dmd_.cA = params_pad_ + n_w0 + Db + n_cR;
+  }
   check(hipHostMalloc((void**)&hflags_, kNumFlags * sizeof(int), hipHostMallocDefault), "pinned flags");
   dalloc(&w_.flags, kNumFlags, "flags");
   check(hipMemset(w_.flags, 0, kNumFlags * sizeof(int)), "clear flags");
@@ -236,6 +296,7 @@ NepEngine::~NepEngine()
 {
   free_all();
   dfree(&params_);
+  dfree(&params_pad_);
   dfree(&sb_pos_);
   dfree(&sb_type_);
   dfree(&sb_gid_);
@@ -429,7 +490,7 @@ void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int 
   if (!attached) prof_.begin(kProfDescriptor, s);
   // ANN: FP32 on the matrix cores unless BF16 operands or the vector path are asked for
   const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
-  check(launch_descriptor(dm_, N, nd1, nd2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
+  check(launch_descriptor(dmd_, N, nd1, nd2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
   if (!attached) prof_.end(s);
   prof_.begin(kProfGather, s);
   check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s, tune_.gather_lanes), "gather");
@@ -559,7 +620,7 @@ void NepEngine::launch_step_kernels(const DevBox& b, int N, int nc, const int* t
 {
   const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
   check(launch_filter(b, dm_, 0, nc, w_, s, tune_.filter_lanes), "filter");
-  check(launch_descriptor(dm_, N, 0, nc, type, pe, false, ann, w_, s), "descriptor");
+  check(launch_descriptor(dmd_, N, 0, nc, type, pe, false, ann, w_, s), "descriptor");
   check(launch_gather(dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, tune_.gather_lanes), "gather");
 }
 

@@ -155,6 +155,8 @@ class NepEngine {
   NepModel model_;
   DevModel dm_{};
   float* params_ = nullptr;
+  float* params_pad_ = nullptr;  // descriptor parameters padded to a bounded instance (when used)
+  DevModel dmd_{};               // the model as the descriptor kernel sees it
   DevWork w_{};
   int* hflags_ = nullptr;  // pinned host mirror of w_.flags
   bool have_list_ = false, box_changed_ = true, last_gid_sort_ = false;

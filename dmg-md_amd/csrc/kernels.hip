@@ -542,6 +542,8 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
     __syncthreads();
   }
   const int nR = C::exact ? NR : m.nR, nA = C::exact ? NA : m.nA;
+  // 3-body order: components L <= Lm, then the 4-/5-body blocks
+  const int Lm = C::exact ? LM : m.L;
   const int kR = C::exact ? KR : m.kR, kA = C::exact ? KA : m.kA;
   const bool h222 = C::exact ? C::q222 : (m.has222 != 0);
   const bool h1111 = C::exact ? C::q1111 : (m.has1111 != 0);
@@ -697,9 +699,9 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
       float v = 0.0f;
 #pragma unroll
       for (int a = L * L - 1; a < (L + 1) * (L + 1) - 1; ++a) v += c_c3[a] * s[n][a] * s[n][a];
-      q[nR + (L - 1) * nA + n] = v;
+      if (L <= Lm) q[nR + (L - 1) * nA + n] = v;
     }
-    int li = LM;
+    int li = Lm;
     if (h222) {
       const float* x = &s[n][3];
       q[nR + li * nA + n] = c_c4[0] * x[0] * x[0] * x[0] + c_c4[1] * x[0] * (x[1] * x[1] + x[2] * x[2]) +
@@ -775,7 +777,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
     float extra[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) extra[a] = 0.0f;
-    int li = LM;
+    int li = Lm;
     if (h222) {
       const float F = Fp[nR + li * nA + n];
       const float* x = &s[n][3];
@@ -797,7 +799,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
     }
 #pragma unroll
     for (int L = 1; L <= LM; ++L) {
-      const float F2 = 2.0f * Fp[nR + (L - 1) * nA + n];
+      const float F2 = L <= Lm ? 2.0f * Fp[nR + (L - 1) * nA + n] : 0.0f;  // no adjoint above the order
 #pragma unroll
       for (int a = L * L - 1; a < (L + 1) * (L + 1) - 1; ++a) {
         float v = F2 * c_c3[a] * s[n][a];
@@ -1484,6 +1486,40 @@ static void run_single(const DevModel& m, int N, int c1, int c2, const int* type
     run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
 }
 
+namespace {
+bool model_is(const DevModel& m, int nr, int kr, int na, int ka, int L, bool a, bool b, bool one)
+{
+  return m.nR == nr && m.kR == kr && m.nA == na && m.kA == ka && m.L == L && (m.has222 != 0) == a &&
+         (m.has1111 != 0) == b && (m.T == 1) == one;
+}
+
+// sizes of the exactly instantiated configurations (nR, kR, nA, kA, q222,
+// q1111, single type); the bounded sizes below come after them
+struct ExactCfg {
+  int nr, kr, na, ka;
+  bool q2, q1, one;
+};
+constexpr ExactCfg kExact[] = {
+    {5, 9, 5, 9, true, true, true},     {7, 9, 5, 9, true, false, true},   {13, 17, 9, 13, true, true, true},
+    {7, 11, 7, 11, true, true, false},  {5, 9, 5, 9, true, false, false},  {9, 9, 7, 9, false, false, false},
+    {7, 9, 5, 9, true, true, true},     {7, 9, 5, 9, true, true, false},   {9, 13, 7, 13, true, true, true},
+    {9, 13, 7, 13, true, true, false}};
+constexpr int kBuckets[][4] = {{7, 9, 5, 9}, {9, 13, 7, 13}};
+}  // namespace
+
+bool descriptor_bucket(const DevModel& m, int dims[4])
+{
+  for (const ExactCfg& c : kExact)
+    if (model_is(m, c.nr, c.kr, c.na, c.ka, 4, c.q2, c.q1, c.one)) return false;
+  if (m.L > 4) return false;
+  for (const auto& b : kBuckets)
+    if (m.nR <= b[0] && m.kR <= b[1] && m.nA <= b[2] && m.kA <= b[3]) {
+      for (int k = 0; k < 4; ++k) dims[k] = b[k];
+      return true;
+    }
+  return false;
+}
+
 hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
                              bool acc, int ann, DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
@@ -1491,9 +1527,10 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
   const bool one = m.T == 1;
   const bool q2 = m.has222 != 0, q1 = m.has1111 != 0;
   auto is = [&](int nr, int kr, int na, int ka, int L, bool a, bool b, bool o) {
-    return m.nR == nr && m.kR == kr && m.nA == na && m.kA == ka && m.L == L && q2 == a && q1 == b && one == o;
+    return model_is(m, nr, kr, na, ka, L, a, b, o);
   };
-  // exact instantiations of the configurations in use; generic otherwise
+  // exact instantiations: the configurations in use, and the bounded sizes
+  // that other models are padded to (descriptor_bucket); generic otherwise
   if (is(5, 9, 5, 9, 4, true, true, true))
     run_single<5, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, false, true))
@@ -1506,6 +1543,14 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
     run_descriptor<Cfg<5, 9, 5, 9, 4, true, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(9, 9, 7, 9, 4, false, false, false))
     run_descriptor<Cfg<9, 9, 7, 9, 4, false, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+  else if (is(7, 9, 5, 9, 4, true, true, true))
+    run_single<7, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+  else if (is(7, 9, 5, 9, 4, true, true, false))
+    run_descriptor<Cfg<7, 9, 5, 9, 4, true, true, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+  else if (is(9, 13, 7, 13, 4, true, true, true))
+    run_single<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+  else if (is(9, 13, 7, 13, 4, true, true, false))
+    run_descriptor<Cfg<9, 13, 7, 13, 4, true, true, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (m.L <= 4 && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
     run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, 4, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
                                                                                   w, s, e0, e1);

@@ -132,6 +132,12 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
 // lanes: lanes per centre, 8, 16 (default) or 32
 hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s,
                          int lanes = 16);
+// Sizes (nR, kR, nA, kA) of the exactly instantiated descriptor that a model
+// without an exact instance of its own is padded to (zero coefficients and
+// ANN weights on the added entries, 3-body order 4 with both many-body
+// blocks); false where the model has an exact instance, or exceeds every
+// padded size and runs on the generic instance.
+bool descriptor_bucket(const DevModel& m, int dims[4]);
 // ann: 0 FP32 on the vector ALUs, 1 FP32 on the matrix cores, 2 BF16 operands
 // on the matrix cores; modes 1 and 2 apply to single-type models of the
 // exact configurations with at most 64 neurons, the vector path otherwise.

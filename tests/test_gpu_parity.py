@@ -373,14 +373,16 @@ def test_ann_modes(dmgx, workloads, which):
 
 
 @pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt",
-                                  "nep4_typecut_ch.txt"])
+                                  "nep4_typecut_ch.txt", "nep4_lmax2_ch.txt", "nep4_lmax3_ch.txt"])
 @pytest.mark.parametrize("tiles", [1, 3])
 def test_nep5_and_zbl_forms_against_oracle(dmgx, name, tiles):
-    """NEP5, the flexible per-pair ZBL table and the typewise ZBL cutoff on a
-    random 2-type (C, H) system with pairs down to 0.5 A: the 11 A cell
-    (replicated small-box path) and its 3x3x3 tiling (large-box path).  The
-    FP32 bars scale with the largest force and energy, which the ZBL pairs
-    make large."""
+    """NEP5, the flexible per-pair ZBL table, the typewise ZBL cutoff,
+    per-type cutoffs and 3-body orders 2 and 3 on a random 2-type (C, H)
+    system with pairs down to 0.5 A: the 11 A cell (replicated small-box
+    path) and its 3x3x3 tiling (large-box path).  None of these sizes has an
+    exact descriptor instance, so all run padded to a bounded one.  The FP32
+    bars scale with the largest force and energy, which the ZBL pairs make
+    large."""
     from test_oracle_golden import coverage_sample
     h, pos, t = coverage_sample()
     if tiles > 1:
@@ -488,3 +490,43 @@ def test_sparse_open_box_uses_the_multi_block_cell_scan(dmgx, workloads):
     rows = nep.verlet_rows(0, N)
     rows_ref = neighbor_rows(box.h, box.pbc, p, rv, max_nb=1024)
     assert all(np.array_equal(rows[i], rows_ref[i]) for i in range(N))
+
+
+def _drop_5body(path, out):
+    """Writes the single-type model at `path` (l_max 4 2 1) without its 5-body
+    block: the matching ANN input weights and scalers are removed."""
+    lines = open(path).read().split("\n")
+    head, vals = lines[:6], np.array([float(x) for x in lines[6:] if x.strip()])
+    nR, nA = [int(x) + 1 for x in head[2].split()[1:3]]
+    kR, kA = [int(x) + 1 for x in head[3].split()[1:3]]
+    assert head[4].split()[1:4] == ["4", "2", "1"] and head[0].split()[1] == "1"
+    H = int(head[5].split()[1])
+    D = nR + 6 * nA
+    keep = np.arange(D) < D - nA
+    w0 = vals[:H * D].reshape(H, D)[:, keep].reshape(-1)
+    rest = vals[H * D:H * D + 2 * H + 1]
+    nd = nR * kR + nA * kA
+    desc = vals[H * D + 2 * H + 1:H * D + 2 * H + 1 + nd]
+    qs = vals[H * D + 2 * H + 1 + nd:][:D][keep]
+    head[4] = "l_max 4 2 0"
+    with open(out, "w") as fh:
+        fh.write("\n".join(head) + "\n" + "\n".join(f"{v:.9e}" for v in np.concatenate([w0, rest, desc, qs])) + "\n")
+
+
+@pytest.mark.parametrize("ann", ["fp32", "bf16", "valu"])
+def test_single_type_model_padded_to_a_bounded_instance(dmgx, workloads, tmp_path, ann):
+    """A single-type model with no exact descriptor instance (the carbon
+    model without its 5-body block) runs padded to a bounded instance, with
+    its ANN on the matrix cores or the vector ALUs."""
+    path = str(tmp_path / "c0_no5.txt")
+    _drop_5body(C0, path)
+    s = workloads.carbon_diamond_4096(seed=1)
+    out = gpu_compute(dmgx, dmgx.NEP(path), s["box"], s["types"], s["pos"], ann_bf16=ann == "bf16",
+                      ann_valu=ann == "valu")
+    ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
+    if ann == "bf16":  # the bfloat16 bars of test_ann_modes
+        e = errors(out, ref)
+        assert e["force"] < 2e-2 * max(np.abs(ref["force"]).max(), 1.0), e
+        assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
+    else:
+        check(errors(out, ref))

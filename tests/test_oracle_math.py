@@ -204,3 +204,71 @@ def test_per_type_cutoffs_use_the_pair_mean():
     for d, same in ((4.2, True), (4.45, True), (3.95, False)):
         pe = nep.compute(h, [1, 1, 1], t, np.array([[5.0, 5.0 + d], [5.0, 5.0], [5.0, 5.0]]))["pe"]
         assert np.array_equal(pe, far) == same, (d, pe, far)
+
+
+def embed_in_lmax4(path, out_path):
+    """Writes the 3-body order 4 model with a 4-body block equal to the
+    model at `path` (order 2 or 3, with or without the 4-body block): every
+    descriptor component keeps its ANN weights and scaler in the order-4
+    layout, added components (L above the model's order, a missing 4-body
+    block) get zero input weights, and the 4-/5-body blocks sit behind L = 4
+    in that order."""
+    lines = open(path).read().split("\n")
+    head, vals = lines[:6], np.array([float(x) for x in lines[6:] if x.strip()])
+    T = int(head[0].split()[1])
+    nR, nA = [int(x) + 1 for x in head[2].split()[1:3]]
+    kR, kA = [int(x) + 1 for x in head[3].split()[1:3]]
+    L, f4, f5 = [int(x) for x in head[4].split()[1:4]]
+    H = int(head[5].split()[1])
+    q4, q5 = int(f4 != 0), int(f5 != 0)
+    D = nR + nA * (L + q4 + q5)
+    D4 = nR + nA * (4 + 1 + q5)
+    blocks = list(range(L)) + ([4] if q4 else []) + ([5] if q5 else [])
+    col = list(range(nR)) + [nR + b4 * nA + n for b4 in blocks for n in range(nA)]
+    out, p = [], 0
+    for _ in range(T):
+        w0 = vals[p:p + H * D].reshape(H, D)
+        p += H * D
+        w = np.zeros((H, D4))
+        w[:, col] = w0
+        out += [w.reshape(-1), vals[p:p + 2 * H]]
+        p += 2 * H
+    out.append(vals[p:p + 1])
+    p += 1
+    nd = T * T * (nR * kR + nA * kA)
+    out.append(vals[p:p + nd])
+    p += nd
+    qs = np.ones(D4)
+    qs[col] = vals[p:p + D]
+    out += [qs, vals[p + D:]]
+    head[4] = f"l_max 4 2 {1 if q5 else 0}"
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(head) + "\n" + "\n".join(f"{v:.9e}" for v in np.concatenate(out)) + "\n")
+
+
+@pytest.mark.parametrize("name", ["nep4_lmax2_ch.txt", "nep4_lmax3_ch.txt"])
+def test_low_3body_orders_equal_their_order4_embedding(name, tmp_path):
+    """3-body orders 2 and 3 (nep_utilities.cuh:1335-1420: components for
+    L <= l_max, then the 4-/5-body invariants that are switched on), and a
+    5-body block without the 4-body one: the reference's CPU evaluator fixes
+    the order at 4 and takes a single extra block for the 4-body one
+    (nep_cpu.cpp:883,1016-1022), so these are pinned by equivalence: the same
+    model written at order 4 with both blocks, zero weights on the added
+    components (checked against that evaluator itself where it is built),
+    has the same energies, forces and virials."""
+    from test_oracle_golden import coverage_sample
+    path = os.path.join(POTENTIALS, name)
+    emb = str(tmp_path / "emb.txt")
+    embed_in_lmax4(path, emb)
+    h, pos, t = coverage_sample()
+    a = OracleNEP(path).compute(h, [1, 1, 1], t, pos)
+    b = OracleNEP(emb).compute(h, [1, 1, 1], t, pos)
+    for k in ("pe", "force", "virial"):
+        scale = max(1.0, np.abs(b[k]).max())
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * scale, (k, np.abs(a[k] - b[k]).max())
+    ref = pytest.importorskip("ref")
+    if ref.available():
+        r = ref.RefNEP(emb).compute(h, [1, 1, 1], t, pos)
+        for k in ("pe", "force", "virial"):
+            scale = max(1.0, np.abs(r[k]).max())
+            assert np.abs(b[k] - r[k]).max() <= 1e-12 * scale, (k, np.abs(b[k] - r[k]).max())
