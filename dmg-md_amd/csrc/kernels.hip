@@ -346,10 +346,16 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
 // ANN modes of the descriptor kernel
 enum : int { kAnnValu = 0, kAnnMfma32 = 1, kAnnMfmaBf16 = 2 };
 
-template <int NR, int KR, int NA, int KA, int LM, bool Q222, bool Q1111, bool ONE, bool EXACT, int ANN = kAnnValu>
+// multi-type models with at most this many types keep their coefficient
+// blocks in LDS
+constexpr int kTypesLds = 4;
+
+template <int NR, int KR, int NA, int KA, int LM, bool Q222, bool Q1111, bool ONE, bool EXACT, int ANN = kAnnValu,
+          bool CL = false>
 struct Cfg {
   static constexpr int nr = NR, kr = KR, na = NA, ka = KA, lm = LM, ann = ANN;
   static constexpr bool q222 = Q222, q1111 = Q1111, one = ONE, exact = EXACT;
+  static constexpr bool cl = CL && EXACT && !ONE;  // coefficient blocks in LDS (T <= kTypesLds)
   static constexpr int nabc = (LM + 1) * (LM + 1) - 1;
   static constexpr int numl = LM + (Q222 ? 1 : 0) + (Q1111 ? 1 : 0);
   static constexpr int dim = NR + NA * numl;
@@ -533,8 +539,30 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   const bool active = i0 < c2;
   // the matrix-core ANN needs every lane of the wavefront: lanes past the
   // range carry an empty row of a valid atom and write nothing
-  if (C::ann == kAnnValu && !active) return;
   const int i = active ? i0 : c1;
+  // multi-type: per type pair the float cutoffs and their inverses (radial,
+  // angular; the FP64 pair mean rounded once) and, with C::cl, the
+  // coefficient blocks [pair][n][k] (radial, then angular), staged in LDS
+  constexpr int kPairs = kMaxTypes * kMaxTypes;
+  __shared__ float pcut[C::one ? 1 : 4 * kPairs];
+  __shared__ float ctab[C::cl ? kTypesLds * kTypesLds * (NR * KR + NA * KA) : 1];
+  if (!C::one) {
+    const int T = m.T;
+    for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+      const int a = k / T, b = k % T;
+      const float rr = (float)((m.rcR[a] + m.rcR[b]) * 0.5), ra = (float)((m.rcA[a] + m.rcA[b]) * 0.5);
+      pcut[k] = rr;
+      pcut[kPairs + k] = 1.0f / rr;
+      pcut[2 * kPairs + k] = ra;
+      pcut[3 * kPairs + k] = 1.0f / ra;
+    }
+    if (C::cl) {
+      const int nrad = T * T * NR * KR, nang = T * T * NA * KA;
+      for (int k = threadIdx.x; k < nrad + nang; k += blockDim.x) ctab[k] = k < nrad ? m.cR[k] : m.cA[k - nrad];
+    }
+    __syncthreads();
+  }
+  if (C::ann == kAnnValu && !active) return;
   __shared__ float ann_w[C::ann != kAnnValu ? ann_w_floats<DMAX>() : 1];
   if (C::ann != kAnnValu) {
     // single-type models only: the one type's weights, staged once per workgroup
@@ -591,9 +619,9 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
       rc = rcRf[0];
     } else {
       tj = safe_type(type[__float_as_int(r.w)], T);
-      rc = (float)((m.rcR[ti] + m.rcR[tj]) * 0.5);
+      rc = pcut[ti * T + tj];
     }
-    const float rcinv = 1.0f / rc;
+    const float rcinv = C::one ? 1.0f / rc : pcut[kPairs + ti * T + tj];
     float fc;
     cutoff_fc(rc, rcinv, d, fc);
     float f[KR];
@@ -603,7 +631,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
       for (int k = 0; k < KR; ++k)
         if (k < kR) Fs[k] += f[k];
     } else {
-      const float* c = m.cR + (size_t)(ti * T + tj) * nR * kR;
+      const float* c = C::cl ? ctab + (ti * T + tj) * NR * KR : m.cR + (size_t)(ti * T + tj) * nR * kR;
 #pragma unroll
       for (int n = 0; n < NR; ++n) {
         if (n < nR) {
@@ -657,7 +685,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
       rc = rcAf[0];
     } else {
       tj = safe_type(type[__float_as_int(r.w)], T);
-      rc = (float)((m.rcA[ti] + m.rcA[tj]) * 0.5);
+      rc = pcut[2 * kPairs + ti * T + tj];
     }
     if (m.zbl) {
       // half of each ZBL pair energy belongs to the centre; ZBL pairs are
@@ -666,12 +694,13 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
       zbl_pair(m.zblp + (size_t)(ti * m.T + tj) * 12, d, dinv, zf, zfp);
       zpe += 0.5f * zf;
     }
-    const float rcinv = 1.0f / rc;
+    const float rcinv = C::one ? 1.0f / rc : pcut[3 * kPairs + ti * T + tj];
     float fc;
     cutoff_fc(rc, rcinv, d, fc);
     float f[KA];
     cheb<KA>(kA, rcinv, d, fc, f);
-    const float* c = m.cA + (size_t)(ti * T + tj) * nA * kA;
+    const float* c = C::cl ? ctab + T * T * NR * KR + (ti * T + tj) * NA * KA
+                           : m.cA + (size_t)(ti * T + tj) * nA * kA;
     float g[NA];
 #pragma unroll
     for (int n = 0; n < NA; ++n) {
@@ -755,7 +784,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
 
   // ---- radial adjoint in basis space: bR[i][t][k] = sum_n Fp_n c[ti,t,n,k]
   for (int t = 0; t < T; ++t) {
-    const float* c = m.cR + (size_t)(ti * T + t) * nR * kR;
+    const float* c = C::cl ? ctab + (ti * T + t) * NR * KR : m.cR + (size_t)(ti * T + t) * nR * kR;
     float* out = w.bR + ((size_t)i * T + t) * m.kRp;
     if (!active) break;
 #pragma unroll
@@ -833,14 +862,15 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
       rc = rcAf[0];
     } else {
       tj = safe_type(type[j], T);
-      rc = (float)((m.rcA[ti] + m.rcA[tj]) * 0.5);
+      rc = pcut[2 * kPairs + ti * T + tj];
     }
-    const float rcinv = 1.0f / rc;
+    const float rcinv = C::one ? 1.0f / rc : pcut[3 * kPairs + ti * T + tj];
     float fc, fcp;
     cutoff_fc_fcp(rc, rcinv, d, fc, fcp);
     float f[KA], fp[KA];
     cheb_d<KA>(kA, rcinv, d, fc, fcp, f, fp);
-    const float* c = m.cA + (size_t)(ti * T + tj) * nA * kA;
+    const float* c = C::cl ? ctab + T * T * NR * KR + (ti * T + tj) * NA * KA
+                           : m.cA + (size_t)(ti * T + tj) * nA * kA;
     float g[NA], gp[NA];
 #pragma unroll
     for (int n = 0; n < NA; ++n) {
@@ -1486,6 +1516,17 @@ static void run_single(const DevModel& m, int N, int c1, int c2, const int* type
     run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
 }
 
+template <int NR, int KR, int NA, int KA, bool Q2, bool Q1>
+static void run_multi(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, DevWork& w,
+                      hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+{
+  if (m.T <= kTypesLds)
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnValu, true>>(m, N, c1, c2, type, pe, acc, w, s,
+                                                                               e0, e1);
+  else
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+}
+
 namespace {
 bool model_is(const DevModel& m, int nr, int kr, int na, int ka, int L, bool a, bool b, bool one)
 {
@@ -1538,19 +1579,19 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
   else if (is(13, 17, 9, 13, 4, true, true, true))
     run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(7, 11, 7, 11, 4, true, true, false))
-    run_descriptor<Cfg<7, 11, 7, 11, 4, true, true, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<7, 11, 7, 11, true, true>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(5, 9, 5, 9, 4, true, false, false))
-    run_descriptor<Cfg<5, 9, 5, 9, 4, true, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<5, 9, 5, 9, true, false>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(9, 9, 7, 9, 4, false, false, false))
-    run_descriptor<Cfg<9, 9, 7, 9, 4, false, false, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<9, 9, 7, 9, false, false>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, true, true))
     run_single<7, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, true, false))
-    run_descriptor<Cfg<7, 9, 5, 9, 4, true, true, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<7, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(9, 13, 7, 13, 4, true, true, true))
     run_single<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(9, 13, 7, 13, 4, true, true, false))
-    run_descriptor<Cfg<9, 13, 7, 13, 4, true, true, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (m.L <= 4 && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
     run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, 4, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
                                                                                   w, s, e0, e1);
