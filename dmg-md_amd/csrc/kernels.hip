@@ -349,6 +349,10 @@ enum : int { kAnnValu = 0, kAnnMfma32 = 1, kAnnMfmaBf16 = 2 };
 // multi-type models with at most this many types keep their coefficient
 // blocks in LDS
 constexpr int kTypesLds = 4;
+// multi-type models with at most this many types run their ANN on the
+// matrix cores: every type's weights staged in LDS, one GEMM pass per type
+// present in the wavefront, each atom taking the pass of its own type
+constexpr int kTypesMfma = 2;
 
 template <int NR, int KR, int NA, int KA, int LM, bool Q222, bool Q1111, bool ONE, bool EXACT, int ANN = kAnnValu,
           bool CL = false>
@@ -356,6 +360,10 @@ struct Cfg {
   static constexpr int nr = NR, kr = KR, na = NA, ka = KA, lm = LM, ann = ANN;
   static constexpr bool q222 = Q222, q1111 = Q1111, one = ONE, exact = EXACT;
   static constexpr bool cl = CL && EXACT && !ONE;  // coefficient blocks in LDS (T <= kTypesLds)
+  // type pairs of the LDS tables: every pair of up to kMaxTypes types, or
+  // the types an LDS-table instance serves (kTypesMfma with the matrix-core
+  // ANN, which stages every type's weights too)
+  static constexpr int ltypes = !cl ? kMaxTypes : (ANN != kAnnValu ? kTypesMfma : kTypesLds);
   static constexpr int nabc = (LM + 1) * (LM + 1) - 1;
   static constexpr int numl = LM + (Q222 ? 1 : 0) + (Q1111 ? 1 : 0);
   static constexpr int dim = NR + NA * numl;
@@ -375,7 +383,7 @@ __device__ __forceinline__ short to_bf16(float x)
 // LDS floats per wavefront of the matrix-core ANN: 64 rows of DMAX+1 (odd
 // stride), then 64 energies
 template <int DMAX>
-constexpr int ann_lds_floats() { return 64 * (DMAX + 1) + 64; }
+constexpr int ann_lds_floats() { return 64 * (DMAX + 1) + 128; }
 
 // ANN of the 64 atoms of a wavefront on the matrix cores (single-type
 // model, H <= 64).  Lane l holds atom l's scaled descriptor q[0..D).
@@ -398,19 +406,23 @@ template <int DMAX>
 constexpr int ann_w_floats() { return 64 * ann_w_stride<DMAX>() + 128; }
 
 template <int DMAX>
-__device__ __forceinline__ void stage_ann_weights(const DevModel& m, int D, float* __restrict__ ws)
+__device__ __forceinline__ void stage_ann_weights(const DevModel& m, int D, float* __restrict__ ws, int t = 0)
 {
   constexpr int WS = ann_w_stride<DMAX>();
   const int H = m.H;
+  const float* __restrict__ w0 = m.w0 + (size_t)t * H * D;
+  const float* __restrict__ b0 = m.b0 + (size_t)t * H;
+  const float* __restrict__ w1 = m.w1 + (size_t)t * H;
   for (int k = threadIdx.x; k < 64 * WS; k += blockDim.x) {
     const int h = k / WS, d = k % WS;
-    ws[k] = (h < H && d < D) ? m.w0[h * D + d] : 0.0f;
+    ws[k] = (h < H && d < D) ? w0[h * D + d] : 0.0f;
   }
   for (int h = threadIdx.x; h < 64; h += blockDim.x) {
-    ws[64 * WS + h] = h < H ? m.b0[h] : 0.0f;
-    ws[64 * WS + 64 + h] = h < H ? m.w1[h] : 0.0f;
+    ws[64 * WS + h] = h < H ? b0[h] : 0.0f;
+    ws[64 * WS + 64 + h] = h < H ? w1[h] : 0.0f;
   }
 }
+
 
 // orders this wavefront's LDS writes before its later LDS reads (each
 // wavefront works in its own LDS region, so no workgroup barrier is needed)
@@ -421,23 +433,33 @@ __device__ __forceinline__ void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int DMAX, bool BF16>
+// MT: multi-type (ti is the lane's type, T the number of types <=
+// kTypesMfma, ws holds the types' weight blocks one after another): per
+// type present in the wavefront one pass over the hidden tiles whose
+// activation gradients and energies count only for atoms of that type.
+template <int DMAX, bool BF16, bool MT = false>
 __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* q, float* Fp, float& E,
-                                         float* __restrict__ lds, const float* __restrict__ ws)
+                                         float* __restrict__ lds, const float* __restrict__ ws, int ti = 0,
+                                         int T = 1)
 {
   constexpr int DS = DMAX + 1;
   constexpr int DT = (DMAX + 15) / 16;
   const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
   float* Qs = lds;
   float* Es = lds + 64 * DS;
+  int* Ts = reinterpret_cast<int*>(lds + 64 * DS + 64);
 #pragma unroll
   for (int d = 0; d < DMAX; ++d) Qs[lane * DS + d] = d < D ? q[d] : 0.0f;
+  unsigned present = 1u;
+  if (MT) {
+    Ts[lane] = ti;
+    present = 0u;
+    for (int t = 0; t < T; ++t)
+      if (__any(ti == t)) present |= 1u << t;
+  }
   wave_lds_sync();
   const int H = m.H, HT = (H + 15) >> 4;
   constexpr int WS = ann_w_stride<DMAX>();
-  const float* __restrict__ W0 = ws;  // [64][WS], zero-padded
-  const float* __restrict__ b0 = ws + 64 * WS;
-  const float* __restrict__ w1 = ws + 64 * WS + 64;
 #pragma unroll 1
   for (int nt = 0; nt < 4; ++nt) {
     const float* qrow = Qs + (16 * nt + lr) * DS;
@@ -445,72 +467,84 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
 #pragma unroll
     for (int dt = 0; dt < DT; ++dt) facc[dt] = f32x4{0.f, 0.f, 0.f, 0.f};
     float ep = 0.0f;
-    // one tile of 16 hidden units at a time: GEMM 1, activation, and its
-    // share of GEMM 2, so that only one accumulator tile is live
+    const int at = MT ? Ts[16 * nt + lr] : 0;  // type of this lane's atom in the tile
+    // the type loop is unrolled over its compile-time bound: a runtime-bounded
+    // loop that skips absent types gave wrong accumulators in GEMM 2 here
+    // (forces off by 5% with exact energies), the unrolled form does not
+#pragma unroll
+    for (int t = 0; t < (MT ? kTypesMfma : 1); ++t) {
+      if (t >= T || !((present >> t) & 1u)) continue;
+      const bool mine = !MT || at == t;
+      const float* __restrict__ W0 = ws + (size_t)t * ann_w_floats<DMAX>();  // [64][WS], zero-padded
+      const float* __restrict__ b0 = W0 + 64 * WS;
+      const float* __restrict__ w1 = W0 + 64 * WS + 64;
+      // one tile of 16 hidden units at a time: GEMM 1, activation, and its
+      // share of GEMM 2, so that only one accumulator tile is live
 #pragma unroll 1
-    for (int mt = 0; mt < HT; ++mt) {
-      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-      const int ha = 16 * mt + lr;
-      if (!BF16) {
+      for (int mt = 0; mt < HT; ++mt) {
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int ha = 16 * mt + lr;
+        if (!BF16) {
 #pragma unroll
-        for (int ks = 0; ks < (DMAX + 3) / 4; ++ks) {
-          const int d = 4 * ks + lg;
-          const float bq = d < D ? qrow[d] : 0.0f;
-          const float a = W0[ha * WS + d];
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bq, acc, 0, 0, 0);
-        }
-      } else {
-#pragma unroll
-        for (int ks = 0; ks < (DMAX + 15) / 16; ++ks) {
-          s16x4 bq, a;
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const int d = 16 * ks + 4 * lg + c;
-            bq[c] = to_bf16(d < D ? qrow[d] : 0.0f);
-            a[c] = to_bf16(W0[ha * WS + d]);
+          for (int ks = 0; ks < (DMAX + 3) / 4; ++ks) {
+            const int d = 4 * ks + lg;
+            const float bq = d < D ? qrow[d] : 0.0f;
+            const float a = W0[ha * WS + d];
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bq, acc, 0, 0, 0);
           }
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
-        }
-      }
-      float G[4];
+        } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int h = 16 * mt + 4 * lg + r;
-        G[r] = 0.0f;
-        if (h < H) {
-          const float x = tanhf(acc[r] - b0[h]);
-          ep += w1[h] * x;
-          G[r] = w1[h] * (1.0f - x * x);
+          for (int ks = 0; ks < (DMAX + 15) / 16; ++ks) {
+            s16x4 bq, a;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const int d = 16 * ks + 4 * lg + c;
+              bq[c] = to_bf16(d < D ? qrow[d] : 0.0f);
+              a[c] = to_bf16(W0[ha * WS + d]);
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
+          }
         }
-      }
-      if (!BF16) {
+        float G[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int h = 16 * mt + 4 * lg + r;
+          G[r] = 0.0f;
+          if (h < H && mine) {
+            const float x = tanhf(acc[r] - b0[h]);
+            ep += w1[h] * x;
+            G[r] = w1[h] * (1.0f - x * x);
+          }
+        }
+        if (!BF16) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int h = 16 * mt + 4 * lg + r;
+#pragma unroll
+            for (int dt = 0; dt < DT; ++dt) {
+              const int d = 16 * dt + lr;
+              const float bw = W0[h * WS + d];
+              facc[dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(G[r], bw, facc[dt], 0, 0, 0);
+            }
+          }
+        } else {
+          s16x4 a;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[r] = to_bf16(G[r]);
 #pragma unroll
           for (int dt = 0; dt < DT; ++dt) {
             const int d = 16 * dt + lr;
-            const float bw = W0[h * WS + d];
-            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(G[r], bw, facc[dt], 0, 0, 0);
+            s16x4 bw;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const int h = 16 * mt + 4 * lg + c;
+              bw[c] = to_bf16(W0[h * WS + d]);
+            }
+            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, facc[dt], 0, 0, 0);
           }
-        }
-      } else {
-        s16x4 a;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) a[r] = to_bf16(G[r]);
-#pragma unroll
-        for (int dt = 0; dt < DT; ++dt) {
-          const int d = 16 * dt + lr;
-          s16x4 bw;
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const int h = 16 * mt + 4 * lg + c;
-            bw[c] = to_bf16(W0[h * WS + d]);
-          }
-          facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, facc[dt], 0, 0, 0);
         }
       }
-    }
+    }  // types
     ep += __shfl_xor(ep, 16);
     ep += __shfl_xor(ep, 32);
     if (lg == 0) Es[16 * nt + lr] = ep;
@@ -529,6 +563,27 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
   E = Es[lane];
 }
 
+// Static LDS of a descriptor instance: pair tables, coefficient blocks, ANN
+// weights and the per-wavefront ANN staging.  Instances stay within 64 KiB
+// per workgroup; where the matrix-core ANN would not, the vector ANN runs.
+template <class C>
+constexpr int desc_lds_bytes()
+{
+  const int pairs = C::ltypes * C::ltypes;
+  const int pcut = C::one ? 1 : 4 * pairs;
+  const int ctab = C::cl ? pairs * (C::nr * C::kr + C::na * C::ka) : 1;
+  const int mfma = C::ann != kAnnValu;
+  const int annw = mfma ? (C::one ? 1 : kTypesMfma) * ann_w_floats<C::dim>() : 1;
+  const int annl = mfma ? (kDescBlock / 64) * ann_lds_floats<C::dim>() : 0;
+  return 4 * (pcut + ctab + annw + annl);
+}
+
+template <int NR, int KR, int NA, int KA, bool Q2, bool Q1, bool ONE>
+constexpr bool mfma_ann_fits()
+{
+  return desc_lds_bytes<Cfg<NR, KR, NA, KA, 4, Q2, Q1, ONE, true, kAnnMfma32, !ONE>>() <= 65536;
+}
+
 template <class C>
 __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, int c1, int c2, int nblocks,
                                                        const int* __restrict__ type, double* __restrict__ pe,
@@ -543,9 +598,10 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   // multi-type: per type pair the float cutoffs and their inverses (radial,
   // angular; the FP64 pair mean rounded once) and, with C::cl, the
   // coefficient blocks [pair][n][k] (radial, then angular), staged in LDS
-  constexpr int kPairs = kMaxTypes * kMaxTypes;
+  constexpr int kPairs = C::ltypes * C::ltypes;
+  static_assert(desc_lds_bytes<C>() <= 65536, "descriptor LDS above 64 KiB per workgroup");
   __shared__ float pcut[C::one ? 1 : 4 * kPairs];
-  __shared__ float ctab[C::cl ? kTypesLds * kTypesLds * (NR * KR + NA * KA) : 1];
+  __shared__ float ctab[C::cl ? kPairs * (NR * KR + NA * KA) : 1];
   if (!C::one) {
     const int T = m.T;
     for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
@@ -563,10 +619,11 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
     __syncthreads();
   }
   if (C::ann == kAnnValu && !active) return;
-  __shared__ float ann_w[C::ann != kAnnValu ? ann_w_floats<DMAX>() : 1];
+  __shared__ float ann_w[C::ann != kAnnValu ? (C::one ? 1 : kTypesMfma) * ann_w_floats<DMAX>() : 1];
   if (C::ann != kAnnValu) {
-    // single-type models only: the one type's weights, staged once per workgroup
-    stage_ann_weights<DMAX>(m, C::exact ? DMAX : m.D, ann_w);
+    // every type's weights (at most kTypesMfma types), staged once per workgroup
+    for (int t = 0; t < (C::one ? 1 : m.T); ++t)
+      stage_ann_weights<DMAX>(m, C::exact ? DMAX : m.D, ann_w + t * ann_w_floats<DMAX>(), t);
     __syncthreads();
   }
   const int nR = C::exact ? NR : m.nR, nA = C::exact ? NA : m.nA;
@@ -752,7 +809,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   float E = 0.0f;
   if (C::ann != kAnnValu) {
     __shared__ float ann_lds[kDescBlock / 64][ann_lds_floats<DMAX>()];
-    ann_mfma<DMAX, C::ann == kAnnMfmaBf16>(m, D, q, Fp, E, ann_lds[threadIdx.x >> 6], ann_w);
+    ann_mfma<DMAX, C::ann == kAnnMfmaBf16, !C::one>(m, D, q, Fp, E, ann_lds[threadIdx.x >> 6], ann_w, ti, T);
   } else {
     const int H = m.H;
     const float* w0 = m.w0 + (size_t)ti * H * D;
@@ -1508,18 +1565,38 @@ static void run_single(const DevModel& m, int N, int c1, int c2, const int* type
                        DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
   const int mode = m.H <= 64 ? ann : kAnnValu;
-  if (mode == kAnnMfma32)
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfma32>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
-  else if (mode == kAnnMfmaBf16)
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaBf16>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
-  else
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+  if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, true>()) {
+    if (mode == kAnnMfma32) {
+      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfma32>>(m, N, c1, c2, type, pe, acc, w, s, e0,
+                                                                            e1);
+      return;
+    }
+    if (mode == kAnnMfmaBf16) {
+      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaBf16>>(m, N, c1, c2, type, pe, acc, w, s, e0,
+                                                                              e1);
+      return;
+    }
+  }
+  run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
 }
 
 template <int NR, int KR, int NA, int KA, bool Q2, bool Q1>
-static void run_multi(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, DevWork& w,
-                      hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+static void run_multi(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
+                      DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
+  const int mode = (m.H <= 64 && m.T <= kTypesMfma) ? ann : kAnnValu;
+  if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, false>()) {
+    if (mode == kAnnMfma32) {
+      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfma32, true>>(m, N, c1, c2, type, pe, acc, w,
+                                                                                   s, e0, e1);
+      return;
+    }
+    if (mode == kAnnMfmaBf16) {
+      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfmaBf16, true>>(m, N, c1, c2, type, pe, acc,
+                                                                                     w, s, e0, e1);
+      return;
+    }
+  }
   if (m.T <= kTypesLds)
     run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnValu, true>>(m, N, c1, c2, type, pe, acc, w, s,
                                                                                e0, e1);
@@ -1579,19 +1656,19 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
   else if (is(13, 17, 9, 13, 4, true, true, true))
     run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(7, 11, 7, 11, 4, true, true, false))
-    run_multi<7, 11, 7, 11, true, true>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<7, 11, 7, 11, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(5, 9, 5, 9, 4, true, false, false))
-    run_multi<5, 9, 5, 9, true, false>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<5, 9, 5, 9, true, false>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(9, 9, 7, 9, 4, false, false, false))
-    run_multi<9, 9, 7, 9, false, false>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<9, 9, 7, 9, false, false>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, true, true))
     run_single<7, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, true, false))
-    run_multi<7, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<7, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(9, 13, 7, 13, 4, true, true, true))
     run_single<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(9, 13, 7, 13, 4, true, true, false))
-    run_multi<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_multi<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (m.L <= 4 && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
     run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, 4, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
                                                                                   w, s, e0, e1);

@@ -139,8 +139,9 @@ hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, Dev
 // padded size and runs on the generic instance.
 bool descriptor_bucket(const DevModel& m, int dims[4]);
 // ann: 0 FP32 on the vector ALUs, 1 FP32 on the matrix cores, 2 BF16 operands
-// on the matrix cores; modes 1 and 2 apply to single-type models of the
-// exact configurations with at most 64 neurons, the vector path otherwise.
+// on the matrix cores; modes 1 and 2 apply to models of at most 2 types with
+// at most 64 neurons (every size but the generic fallback), the vector path
+// otherwise.
 // start/stop (optional) are recorded with the kernel's own dispatch
 hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
                              bool accumulate, int ann, DevWork& w, hipStream_t s, hipEvent_t start = nullptr,

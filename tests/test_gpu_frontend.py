@@ -25,14 +25,18 @@ TOL = {k: dict(v) for k, v in MANIFEST["tolerances"].items()}
 # single_small_static; this build's values are nearer the oracle (4.8e-6 eV
 # and 3.4e-6 eV).  The same holds for velocities: replaying the nvt_ber
 # segments of multi_nvt_restart in FP64 with the oracle puts GPUMD's
-# trajectory velocities up to 1.5e-8 A/fs away (5.6x the manifest's bound),
+# trajectory velocities up to 1.5e-8 A/fs away (5.6x the manifest's bound;
+# in restart.xyz after step 2 of the second segment, atom 6's v_y is
+# 0.0039015447 in GPUMD's file, 0.0039015293 in the FP64 replay and
+# 0.0039015202 here, so restart velocities get the same floor),
 # and the oracle's forces at GPUMD's own trajectory positions differ from
 # GPUMD's by up to 9.1e-6 eV/A (2.15x the bound: water forces cancel O-H
 # terms of tens of eV/A, whose FP32 rounding step is ~7.6e-6).  A second
 # FP32 implementation is therefore held to max(manifest floor, the FP32 floor
 # below); every relative tolerance is the manifest's, and the FP64-oracle bar
 # of the parity tests is unchanged.
-FP32_FLOOR = {"energy": 1e-5, "virial": 1e-5, "xyz_stress": 1e-7, "velocity": 3e-8, "force": 2e-5}
+FP32_FLOOR = {"energy": 1e-5, "virial": 1e-5, "xyz_stress": 1e-7, "velocity": 3e-8, "restart_velocity": 3e-8,
+              "force": 2e-5}
 for key, floor in FP32_FLOOR.items():
     TOL[key]["atol"] = max(TOL[key]["atol"], floor)
 

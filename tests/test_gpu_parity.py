@@ -530,3 +530,31 @@ def test_single_type_model_padded_to_a_bounded_instance(dmgx, workloads, tmp_pat
         assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
     else:
         check(errors(out, ref))
+
+
+@pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_typecut_ch.txt"])
+def test_two_type_ann_modes(dmgx, name):
+    """Two-type models run their ANN on the matrix cores, one pass per type
+    present in a wavefront: the FP32 result against the vector-ALU ANN (same
+    products, different summation order) and the BF16-operand result against
+    the oracle at the bfloat16 bars of test_ann_modes."""
+    from test_oracle_golden import coverage_sample
+    h, pos, t = coverage_sample()
+    L = h[0, 0]
+    shifts = np.array([[a, b, c] for a in range(3) for b in range(3) for c in range(3)]) * L
+    pos = np.ascontiguousarray(np.concatenate([pos + s[:, None] for s in shifts], axis=1))
+    t = np.tile(t, 27)
+    h = h * 3
+    box = dmgx.Box(h, (1, 1, 1))
+    path = os.path.join(POTENTIALS, name)
+    nep = dmgx.NEP(path)
+    mf = gpu_compute(dmgx, nep, box, t, pos)
+    va = gpu_compute(dmgx, nep, box, t, pos, ann_valu=True)
+    bf = gpu_compute(dmgx, nep, box, t, pos, ann_bf16=True)
+    fscale = max(np.abs(va[1]).max(), 1.0)
+    d = errors(mf, dict(pe=va[0], force=va[1], virial=va[2]))
+    assert d["force"] < 1e-5 * fscale and d["energy_atom_max"] < 1e-5, d
+    ref = OracleNEP(path, round_params_to_float=True).compute(h, (1, 1, 1), t, pos)
+    e = errors(bf, ref)
+    assert e["force"] < 2e-2 * max(np.abs(ref["force"]).max(), 1.0), e
+    assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
