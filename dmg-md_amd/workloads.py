@@ -77,3 +77,102 @@ def copper_fcc(n: int = 40, seed: int = 1):
     m = np.full(pos.shape[0], MASS["Cu"])
     return dict(box=box, pos=soa(pos), types=np.zeros(pos.shape[0], np.int32), mass=m,
                 vel=maxwell_boltzmann(m, 300.0, seed + 1))
+
+
+AVOGADRO = 6.02214076e23
+
+
+def cube_for_density(mass_amu: float, g_per_cm3: float) -> float:
+    """Edge (A) of the cube holding `mass_amu` of matter at the given density."""
+    return (mass_amu / (AVOGADRO * g_per_cm3) * 1e24) ** (1.0 / 3.0)
+
+
+def random_separated(n: int, L: float, dmin: float, rng, batch: int = 0) -> np.ndarray:
+    """n points uniform in the periodic cube [0, L)^3, every pair at least
+    dmin apart (minimum image): random sequential addition in batches, each
+    candidate tested against the accepted points and the earlier survivors
+    of its own batch."""
+    from scipy.spatial import cKDTree
+    batch = batch or max(1024, n // 8)
+    pts = np.zeros((0, 3))
+    while len(pts) < n:
+        cand = rng.uniform(0.0, L, (batch, 3))
+        if len(pts):
+            d, _ = cKDTree(pts, boxsize=L).query(cand, k=1, distance_upper_bound=dmin)
+            cand = cand[~(d < dmin)]
+        if len(cand) > 1:
+            pairs = cKDTree(cand, boxsize=L).query_pairs(dmin, output_type="ndarray")
+            drop = np.zeros(len(cand), bool)
+            # greedy in index order: a candidate is dropped when an earlier kept one is too close
+            for a, b in pairs[np.lexsort((pairs[:, 0], pairs[:, 1]))]:
+                if not drop[a]:
+                    drop[b] = True
+            cand = cand[~drop]
+        pts = np.concatenate([pts, cand[:n - len(pts)]])
+    return pts
+
+
+def _random_rotations(n: int, rng) -> np.ndarray:
+    """n rotation matrices uniform on SO(3) (normalised Gaussian quaternions)."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def liquid_water(n_molecules: int = 64000, seed: int = 1, dmin: float = 2.5):
+    """Config: liquid water, n_molecules H2O at 1.0 g/cm^3 in a periodic cube
+    (64,000 molecules: 192,000 atoms, L = 124.2 A).  O-H 0.96 A, H-O-H
+    104.5 degrees, uniformly random orientations; the oxygens are random
+    centres at least dmin apart.  Atoms per molecule O, H, H; types O = 0,
+    H = 1; Maxwell-Boltzmann velocities at 300 K."""
+    rng = np.random.default_rng(seed)
+    L = cube_for_density(n_molecules * (MASS["O"] + 2 * MASS["H"]), 1.0)
+    o = random_separated(n_molecules, L, dmin, rng)
+    half = np.radians(104.5) / 2.0
+    local = 0.96 * np.array([[np.sin(half), np.cos(half), 0.0], [-np.sin(half), np.cos(half), 0.0]])
+    R = _random_rotations(n_molecules, rng)
+    h = o[:, None, :] + np.einsum("mij,kj->mki", R, local)
+    pos = np.concatenate([o[:, None, :], h], axis=1).reshape(-1, 3)
+    box = Box.orthogonal(L, L, L)
+    pos = wrap(pos, box)
+    types = np.tile(np.array([0, 1, 1], np.int32), n_molecules)
+    m = np.where(types == 0, MASS["O"], MASS["H"])
+    return dict(box=box, pos=soa(pos), types=types, mass=m, vel=maxwell_boltzmann(m, 300.0, seed + 1))
+
+
+def amorphous_carbon_share(n: int = 2048000, seed: int = 1, dmin: float = 1.2, shell: float = 5.0,
+                           margin: float = 30.0):
+    """Config: one GPU's share of an amorphous carbon melt: n atoms at
+    2.0 g/cm^3 in the cube [0, L)^3 (2,048,000 atoms: L = 273.3 A), random
+    positions at least dmin apart, 5000 K velocities, followed by a static
+    ghost shell: the periodic images of the atoms within `shell` A outside
+    the cube.  The share is evaluated in an open box enclosing cube and
+    shell with `margin` A of room beyond the shell (owned atoms pushed apart
+    by close pairs may leave the cube during a run); `owned` = n and
+    `ghost_of[k]` is the owned atom that ghost k images; `origin` is where
+    the cube starts in the box."""
+    rng = np.random.default_rng(seed)
+    L = cube_for_density(n * MASS["C"], 2.0)
+    p = random_separated(n, L, dmin, rng)
+    ghosts, owner = [], []
+    for a in (-1, 0, 1):
+        for b in (-1, 0, 1):
+            for c in (-1, 0, 1):
+                if a == b == c == 0:
+                    continue
+                q = p + L * np.array([a, b, c])
+                keep = np.all((q > -shell) & (q < L + shell), axis=1)
+                ghosts.append(q[keep])
+                owner.append(np.nonzero(keep)[0])
+    g = np.concatenate(ghosts)
+    origin = shell + margin
+    pos = np.concatenate([p, g]) + origin  # open box [0, L + 2 origin)^3
+    E = L + 2 * origin
+    m = np.full(len(pos), MASS["C"])
+    vel = maxwell_boltzmann(m[:n], 5000.0, seed + 1)
+    return dict(box=Box.orthogonal(E, E, E, pbc=(0, 0, 0)), pos=soa(pos), types=np.zeros(len(pos), np.int32),
+                mass=m, vel=vel, owned=n, ghost_of=np.concatenate(owner), L=L, shell=shell, origin=origin)

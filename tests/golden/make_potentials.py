@@ -34,6 +34,8 @@ SPECS = {
                       lambda: W.copper_fcc(n=10, seed=12), 102),
     "nep_c4_carbon_zbl.txt": (["C"], 4.2, 3.7, (4, 4), (8, 8), "4 2 1", 30, "zbl 1.0 2.0",
                               lambda: W.carbon_diamond_4096(seed=13), 104),
+    "nep_c2_water.txt": (["O", "H"], 6.0, 4.0, (6, 6), (10, 10), "4 2 1", 40, None,
+                         lambda: W.liquid_water(64000, seed=23), 103),
 }
 
 

@@ -558,3 +558,94 @@ def test_two_type_ann_modes(dmgx, name):
     e = errors(bf, ref)
     assert e["force"] < 2e-2 * max(np.abs(ref["force"]).max(), 1.0), e
     assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
+
+
+C2 = os.path.join(POTENTIALS, "nep_c2_water.txt")
+TIME_UNIT = 10.18051   # fs per natural time unit
+PRESSURE_UNIT = 160.2177  # GPa per eV/A^3
+KB = 8.617343e-5
+
+
+def test_c2_water_against_oracle(dmgx, workloads):
+    """The liquid-water configuration's model (2 types, n_max 6/6, basis 10,
+    l_max 4/2/1, rc 6/4 A, 40 neurons) on 1000 molecules of the same
+    generator, against the FP64 oracle."""
+    s = workloads.liquid_water(1000, seed=3)
+    out = gpu_compute(dmgx, dmgx.NEP(C2), s["box"], s["types"], s["pos"])
+    ref = OracleNEP(C2, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
+    fscale = max(1.0, np.abs(ref["force"]).max() / 10.0)
+    check(errors(out, ref), scale_f=fscale)
+
+
+def _npt_replay(dmgx, s, ensemble, steps, dt_fs, T, P, B, tau_p, tau_t, seed, model=C2):
+    """FP64 replay of an NPT run with the oracle's forces, integrator and
+    thermo, and the ensemble step of the reference (ensemble_control.cpp:
+    Berendsen or Bussi velocity factor after the second kick, then the box
+    factor from the pressure before that rescale: Berendsen, or stochastic
+    cell rescaling with the next Gaussian of the same mt19937 stream)."""
+    import ctypes
+    from oracle import thermo, velocity_verlet
+    L = dmgx._lib.lib()
+    N = len(s["types"])
+    h = np.array(s["box"].h, float).reshape(3, 3).copy()
+    pos = np.ascontiguousarray(s["pos"], float).reshape(3, N).copy()
+    vel = np.ascontiguousarray(s["vel"], float).reshape(3, N).copy()
+    mass = np.asarray(s["mass"], float)
+    orc = OracleNEP(model, round_params_to_float=True)
+    dt = dt_fs / TIME_UNIT
+    rng = ctypes.c_void_p()
+    assert L.dmgx_rng_create(seed, ctypes.byref(rng)) == 0
+    r = orc.compute(h, (1, 1, 1), s["types"], pos)
+    f = np.ascontiguousarray(r["force"]).reshape(3, N)
+    p_target, p_coupling = P / PRESSURE_UNIT, PRESSURE_UNIT / (3.0 * tau_p * B)
+    try:
+        for _ in range(steps):
+            velocity_verlet(1, dt, mass, pos, vel, f)
+            r = orc.compute(h, (1, 1, 1), s["types"], pos)
+            f = np.ascontiguousarray(r["force"]).reshape(3, N)
+            velocity_verlet(0, dt, mass, pos, vel, f)
+            th = thermo(h, mass, r["pe"], vel, r["virial"])
+            if ensemble == "npt_ber":
+                vel *= np.sqrt(1.0 + (1.0 / tau_t) * (T / th[0] - 1.0))
+            else:
+                ndeg = 3 * N
+                old = th[0] * ndeg * KB * 0.5
+                new = L.dmgx_resamplekin(old, ndeg * KB * T * 0.5, ndeg, tau_t, rng)
+                vel *= np.sqrt(new / old)
+            p_inst = (th[2] + th[3] + th[4]) / 3.0
+            if ensemble == "npt_ber":
+                sc = 1.0 - p_coupling * (p_target - p_inst)
+            else:
+                sc = L.dmgx_npt_scr_scale(p_target, p_coupling, p_inst, T, abs(np.linalg.det(h)),
+                                          L.dmgx_gasdev(rng))
+            h *= sc
+            pos *= sc
+    finally:
+        L.dmgx_rng_destroy(rng)
+    return h, pos, vel
+
+
+@pytest.mark.parametrize("ensemble", ["npt_ber", "npt_scr"])
+def test_npt_run_matches_oracle_replay(dmgx, workloads, ensemble):
+    """Ten NPT steps of 1000 water molecules (the liquid-water configuration's
+    model and generator, 300 K, 1e-4 GPa, bulk modulus 2.2 GPa, a strong
+    tau_p of 50 steps so the box moves): box, positions and velocities
+    against the FP64 replay."""
+    s = workloads.liquid_water(1000, seed=4)
+    kw = dict(steps=10, dt_fs=0.5, T=300.0, P=1e-4, B=2.2, tau_p=50.0, tau_t=100.0, seed=12345)
+    md = dmgx.MD(dmgx.NEP(C2), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    md.run(kw["steps"], kw["dt_fs"], ensemble, t_init=kw["T"], t_final=kw["T"], t_coupling=kw["tau_t"],
+           p_target_gpa=kw["P"], bulk_modulus_gpa=kw["B"], p_coupling=kw["tau_p"], seed=kw["seed"])
+    a = md.arrays()
+    h_gpu = np.array(md.box().h, float).reshape(3, 3)
+    h, pos, vel = _npt_replay(dmgx, s, ensemble, **kw)
+    L0 = np.array(s["box"].h, float).reshape(3, 3)[0, 0]
+    Lg, Lr = h_gpu[0, 0], h[0, 0]
+    d = a["position"] - pos
+    d -= Lr * np.round(d / Lr)
+    dv = np.abs(a["velocity"] - vel).max()
+    print(ensemble, "box change", Lr / L0 - 1.0, "box diff", abs(Lg - Lr), "pos diff", np.abs(d).max(), "vel diff", dv)
+    assert abs(Lr / L0 - 1.0) > 1e-6  # the barostat moved the box
+    assert abs(Lg / Lr - 1.0) < 1e-8
+    assert np.abs(d).max() < 1e-6
+    assert dv < 1e-5 * max(1.0, np.abs(vel).max())
