@@ -233,6 +233,36 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
             "speedup_vs_plain": st["frames_per_s"] / plain if plain > 0 else None}
 
 
+C2_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_c2_water.txt")
+
+
+def water_bench(dmgx, steps=100):
+    """The liquid-water configuration (64,000 H2O, 192,000 atoms, 2-type
+    NEP4 n_max 6/6, basis 10, l_max 4/2/1, rc 6/4 A, 40 neurons), dt 0.5 fs
+    at 300 K: a Bussi NVT stretch (the sync-free chunked step) and the
+    configuration's SCR NPT (1e-4 GPa; the barostat reads the pressure on
+    the host every step), each after 20 untimed steps."""
+    import torch
+    W = importlib.import_module("dmg-md_amd.workloads")
+    s = W.liquid_water(64000, seed=1)
+    N = len(s["types"])
+    out = {"workload": "liquid water 64,000 H2O (192,000 atoms), 2-type NEP4 n_max 6/6 basis 10 l_max 4/2/1 "
+                       "rc 6.0/4.0 A 40 neurons, dt 0.5 fs, 300 K", "atoms": N}
+    for name, kw in (("nvt_bdp", dict(t_coupling=100.0)),
+                     ("npt_scr", dict(t_coupling=100.0, p_target_gpa=1e-4, bulk_modulus_gpa=2.2, p_coupling=1000.0))):
+        nep = dmgx.NEP(C2_MODEL)
+        md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+        md.autotune()
+        md.run(20, 0.5, name, t_init=300.0, t_final=300.0, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        md.run(steps, 0.5, name, t_init=300.0, t_final=300.0, **kw)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = {"atom_steps_per_s": N * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +273,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default filter/gather tile shapes")
+    ap.add_argument("--no-water", action="store_true", help="skip the liquid-water (2-type) sample")
     ap.add_argument("--ann", choices=["fp32", "valu", "bf16"], default="fp32",
                     help="ANN arithmetic: fp32 on the matrix cores (default), fp32 on the vector ALUs, "
                          "or bfloat16 operands on the matrix cores")
@@ -314,6 +345,7 @@ def main():
           f"per-kernel avg ms {kernel_ms}, mean rows {means}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     serving = None if args.no_serve else serve_bench(dmgx)
+    water = None if args.no_water else water_bench(dmgx)
     out = {
         "metric": METRIC,
         "value": value,
@@ -361,6 +393,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "serving": serving,
+        "water": water,
     }
     print(json.dumps(out))
 

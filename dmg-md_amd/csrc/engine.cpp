@@ -457,6 +457,7 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
              by_gid != last_gid_sort_ || (flags & 2u) || std::memcmp(box.h, last_h_, sizeof(last_h_)) != 0 ||
              std::memcmp(box.pbc, last_pbc_, sizeof(last_pbc_)) != 0 || im.owner != last_map_.owner ||
              im.image != last_map_.image || im.gstart != last_map_.gstart;
+  if (p.forced) scale_since_build_ = 1.0;  // this evaluation rebuilds from the current positions
   p.empty = false;
   w_.map = im;
   last_map_ = im;
@@ -541,11 +542,29 @@ void NepEngine::enqueue_small(const Box& box, const int reps[3], int N, const in
   const uint64_t* gid2 = by_gid ? sb_gid_ : nullptr;
   const unsigned f2 = flags & ~1u;  // the replicas' outputs are overwritten, then folded
   const Plan p = plan(big, M, 0, N, 0, M, sb_type_, sb_pos_, gid2, sb_pe_, sb_f_, sb_vir_, f2);
+  last_small_ = true;
   prof_.begin(kProfCheck, s);
   check(launch_pack(p.box, M, model_.num_types, sb_pos_, sb_type_, p.forced, w_, s), "pack / displacement check");
   prof_.end(s);
   launch_force(p, M, 0, N, 0, M, sb_type_, gid2, sb_pe_, sb_f_, sb_vir_, f2, s);
   check(launch_fold(N, n2, (flags & 1u) != 0, sb_pe_, sb_f_, sb_vir_, pe, force, virial, s), "fold replicas");
+}
+
+double NepEngine::half_skin() const
+{
+  return 0.5 * (kSkin - std::fabs(1.0 - scale_since_build_) * (model_.rcR_max + kSkin));
+}
+
+void NepEngine::notify_box_scaled(double scale, hipStream_t stream)
+{
+  if (!have_list_ || last_small_ || last_N_ <= 0) {
+    box_changed_ = true;
+    return;
+  }
+  for (double& v : last_h_) v *= scale;  // the same products the caller forms for its box
+  check(launch_scale(3 * last_N_, scale, w_.x0, stream), "rescale displacement references");
+  scale_since_build_ *= scale;
+  if (std::fabs(1.0 - scale_since_build_) * (model_.rcR_max + kSkin) > 0.5 * kSkin) box_changed_ = true;
 }
 
 void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2, const int* type,
@@ -564,9 +583,12 @@ void NepEngine::enqueue(const Box& box, int N, int n1, int n2, int nd1, int nd2,
   }
   const Plan p = plan(box, N, n1, n2, nd1, nd2, type, pos, gid, pe, force, virial, flags, map);
   if (p.empty) return;
-  // rebuild decision on the device: forced by the host, or any atom moved > skin/2
+  last_small_ = false;
+  // rebuild decision on the device: forced by the host, or any atom moved
+  // more than half the skin left
   prof_.begin(kProfCheck, s);
-  check(launch_pack(p.box, N, model_.num_types, pos, type, p.forced, w_, s), "pack / displacement check");
+  check(launch_pack(p.box, N, model_.num_types, pos, type, p.forced, w_, s, half_skin()),
+        "pack / displacement check");
   prof_.end(s);
   launch_force(p, N, n1, n2, nd1, nd2, type, gid, pe, force, virial, flags, s);
 }
@@ -581,6 +603,10 @@ const int* NepEngine::read_flags(hipStream_t s)
 bool NepEngine::settle(hipStream_t s)
 {
   read_flags(s);
+  if (hflags_[kFlagRebuildCount] != seen_rebuilds_) {  // a rebuild ran: fresh references
+    seen_rebuilds_ = hflags_[kFlagRebuildCount];
+    scale_since_build_ = 1.0;
+  }
   if (hflags_[kFlagBadType]) {
     check(hipMemsetAsync(w_.flags + kFlagBadType, 0, sizeof(int), s), "clear flag");
     throw std::invalid_argument("an atom type lies outside [0, num_types)");
@@ -1030,7 +1056,7 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
         if (sc != 1.0) {
           for (double& v : box_.h) v *= sc;
           check(launch_scale(3 * N_, sc, pos_, stream_), "scale positions");
-          nep_->notify_box_changed();
+          nep_->notify_box_scaled(sc, stream_);
         }
       }
       if (on_output && ((every > 0 && (step + 1) % every == 0) || step + 1 == steps)) {
@@ -1058,6 +1084,9 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
   constexpr int kChunk = 128;
   if (!rand_) dalloc(&rand_, 2 * kChunk, "thermostat draws");
   if (!factor_) dalloc(&factor_, 1, "thermostat factor");
+  // the chunks' skin check uses the whole skin: references rescaled by an
+  // earlier barostat run are replaced by a rebuild first
+  if (nep_->rescaled_since_build()) nep_->notify_box_changed();
   if (!hrand_) check(hipHostMalloc((void**)&hrand_, 2 * kChunk * sizeof(double), hipHostMallocDefault), "pinned");
   for (int c0 = 0; c0 < steps;) {
     int n = std::min(kChunk, steps - c0);

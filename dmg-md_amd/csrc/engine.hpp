@@ -137,6 +137,15 @@ class NepEngine {
   bool small_box(const Box& box, int reps[3]) const;
   const int* read_flags(hipStream_t stream);
   void notify_box_changed() { box_changed_ = true; }
+  // Box and positions were rescaled isotropically by `scale` (a barostat
+  // step; the caller scales both).  The Verlet list is kept: the skin
+  // check's references are rescaled on `stream` and its budget shrinks by
+  // |1 - accumulated scale| (rc + skin), so a pair beyond rc + skin at the
+  // build stays beyond rc while no atom's non-affine displacement exceeds
+  // half of what is left; once the scale has used half the skin (or for a
+  // replicated small box) the next evaluation rebuilds.
+  void notify_box_scaled(double scale, hipStream_t stream);
+  bool rescaled_since_build() const { return scale_since_build_ != 1.0; }
   void stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds);
   void totals(long long* sum_radial, long long* sum_angular, long long* sum_verlet, int* centres);
   void copy_verlet_rows(int c1, int c2, int max_nb, int* counts, int* rows);
@@ -166,6 +175,10 @@ class NepEngine {
   ImageMap last_map_;
   Tuning tune_;
   int mv_base_ = 0;  // row capacity asked for, before the layout's rounding
+  double scale_since_build_ = 1.0;  // isotropic box scale accumulated since the last Verlet build
+  int seen_rebuilds_ = 0;           // device rebuild count when last read
+  bool last_small_ = false;         // the last evaluation ran on a replicated small box
+  double half_skin() const;
   int padded_rows(int mv, int layout) const;
   void launch_step_kernels(const DevBox& b, int N, int nc, const int* type, double* pe, double* force,
                            double* virial, unsigned flags, hipStream_t s);

@@ -123,8 +123,10 @@ void upload_angular_constants(const AngularConstants& a);
 // it returns at once unless flags[kFlagRebuild] is set.  The request is
 // closed (and counted) by the filter kernel that follows.
 hipError_t launch_request_rebuild(DevWork& w, hipStream_t s);
+// half_skin: the displacement since the last build that requests a rebuild
+// (default half the skin)
 hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const int* type, bool forced, DevWork& w,
-                       hipStream_t s);
+                       hipStream_t s, double half_skin = -1.0);
 hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, const uint64_t* gid, DevWork& w,
                           hipStream_t s);
 

@@ -542,9 +542,9 @@ hipError_t launch_request_rebuild(DevWork& w, hipStream_t s)
 }
 
 hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const int* type, bool forced, DevWork& w,
-                       hipStream_t s)
+                       hipStream_t s, double half_skin)
 {
-  const double half = 0.5 * kSkin;
+  const double half = half_skin > 0.0 ? half_skin : 0.5 * kSkin;
   if (N > 0)
     k_pack<<<nblk(N), kBlock, 0, s>>>(b, N, T, pos, type, w.x0, half * half, forced ? 1 : 0, w.xq, w.flags);
   else if (forced)

@@ -625,15 +625,19 @@ def _npt_replay(dmgx, s, ensemble, steps, dt_fs, T, P, B, tau_p, tau_t, seed, mo
     return h, pos, vel
 
 
-@pytest.mark.parametrize("ensemble", ["npt_ber", "npt_scr"])
-def test_npt_run_matches_oracle_replay(dmgx, workloads, ensemble):
+@pytest.mark.parametrize("ensemble,tau_p", [("npt_ber", 50.0), ("npt_scr", 50.0), ("npt_ber", 1000.0),
+                                             ("npt_scr", 1000.0)])
+def test_npt_run_matches_oracle_replay(dmgx, workloads, ensemble, tau_p):
     """Ten NPT steps of 1000 water molecules (the liquid-water configuration's
-    model and generator, 300 K, 1e-4 GPa, bulk modulus 2.2 GPa, a strong
-    tau_p of 50 steps so the box moves): box, positions and velocities
-    against the FP64 replay."""
+    model and generator, 300 K, 1e-4 GPa, bulk modulus 2.2 GPa): box,
+    positions and velocities against the FP64 replay.  With tau_p 50 steps
+    the box moves 9%, past the skin budget, so the Verlet list is rebuilt;
+    with tau_p 1000 steps it moves 0.5% and the list is kept across the
+    rescales (fewer rebuilds than steps)."""
     s = workloads.liquid_water(1000, seed=4)
-    kw = dict(steps=10, dt_fs=0.5, T=300.0, P=1e-4, B=2.2, tau_p=50.0, tau_t=100.0, seed=12345)
-    md = dmgx.MD(dmgx.NEP(C2), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    kw = dict(steps=10, dt_fs=0.5, T=300.0, P=1e-4, B=2.2, tau_p=tau_p, tau_t=100.0, seed=12345)
+    nep = dmgx.NEP(C2)
+    md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
     md.run(kw["steps"], kw["dt_fs"], ensemble, t_init=kw["T"], t_final=kw["T"], t_coupling=kw["tau_t"],
            p_target_gpa=kw["P"], bulk_modulus_gpa=kw["B"], p_coupling=kw["tau_p"], seed=kw["seed"])
     a = md.arrays()
@@ -644,8 +648,12 @@ def test_npt_run_matches_oracle_replay(dmgx, workloads, ensemble):
     d = a["position"] - pos
     d -= Lr * np.round(d / Lr)
     dv = np.abs(a["velocity"] - vel).max()
-    print(ensemble, "box change", Lr / L0 - 1.0, "box diff", abs(Lg - Lr), "pos diff", np.abs(d).max(), "vel diff", dv)
+    rebuilds = nep.neighbor_stats()["rebuilds"]
+    print(ensemble, tau_p, "box change", Lr / L0 - 1.0, "box diff", abs(Lg - Lr), "pos diff", np.abs(d).max(),
+          "vel diff", dv, "rebuilds", rebuilds)
     assert abs(Lr / L0 - 1.0) > 1e-6  # the barostat moved the box
+    if tau_p >= 1000.0:
+        assert rebuilds < kw["steps"]
     assert abs(Lg / Lr - 1.0) < 1e-8
     assert np.abs(d).max() < 1e-6
     assert dv < 1e-5 * max(1.0, np.abs(vel).max())
