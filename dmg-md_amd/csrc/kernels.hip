@@ -649,9 +649,17 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   for (int d = 0; d < DMAX; ++d) q[d] = 0.0f;
 
   // ---- radial descriptor: q_n = sum_j sum_k c[ti,tj,n,k] f_k(r_ij)
+  // The basis values are summed per neighbour type and contracted with the
+  // coefficients once per atom (single type: one sum; up to kTypesLds types:
+  // one per type); more types contract per edge.
   float Fs[KR];
 #pragma unroll
   for (int k = 0; k < KR; ++k) Fs[k] = 0.0f;
+  float Ft[C::cl ? kTypesLds : 1][KR];
+#pragma unroll
+  for (int t = 0; t < (C::cl ? kTypesLds : 1); ++t)
+#pragma unroll
+    for (int k = 0; k < KR; ++k) Ft[t][k] = 0.0f;
   // the radial loop keeps kAhead edge records in flight (its work per edge
   // is short); the angular loops below load the next record before using
   // the current one and the first record of the next 128-byte line a line
@@ -687,8 +695,17 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
 #pragma unroll
       for (int k = 0; k < KR; ++k)
         if (k < kR) Fs[k] += f[k];
+    } else if (C::cl) {
+#pragma unroll
+      for (int t = 0; t < kTypesLds; ++t) {
+        if (t < T) {
+          const float wt = tj == t ? 1.0f : 0.0f;
+#pragma unroll
+          for (int k = 0; k < KR; ++k) Ft[t][k] = fmaf(wt, f[k], Ft[t][k]);
+        }
+      }
     } else {
-      const float* c = C::cl ? ctab + (ti * T + tj) * NR * KR : m.cR + (size_t)(ti * T + tj) * nR * kR;
+      const float* c = m.cR + (size_t)(ti * T + tj) * nR * kR;
 #pragma unroll
       for (int n = 0; n < NR; ++n) {
         if (n < nR) {
@@ -711,6 +728,20 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
           if (k < kR) g += m.cR[n * kR + k] * Fs[k];
         q[n] = g;
       }
+    }
+  } else if (C::cl) {
+#pragma unroll
+    for (int n = 0; n < NR; ++n) {
+      float g = 0.0f;
+#pragma unroll
+      for (int t = 0; t < kTypesLds; ++t) {
+        if (t < T) {
+          const float* c = ctab + (ti * T + t) * NR * KR + n * KR;
+#pragma unroll
+          for (int k = 0; k < KR; ++k) g += c[k] * Ft[t][k];
+        }
+      }
+      q[n] = g;
     }
   }
 
