@@ -564,8 +564,10 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
 }
 
 // Static LDS of a descriptor instance: pair tables, coefficient blocks, ANN
-// weights and the per-wavefront ANN staging.  Instances stay within 64 KiB
-// per workgroup; where the matrix-core ANN would not, the vector ANN runs.
+// weights and the per-wavefront ANN staging.  Instances stay within
+// kDescLdsMax per workgroup; where the matrix-core ANN would not, the vector
+// ANN runs.
+constexpr int kDescLdsMax = 96 * 1024;
 template <class C>
 constexpr int desc_lds_bytes()
 {
@@ -581,7 +583,10 @@ constexpr int desc_lds_bytes()
 template <int NR, int KR, int NA, int KA, bool Q2, bool Q1, bool ONE>
 constexpr bool mfma_ann_fits()
 {
-  return desc_lds_bytes<Cfg<NR, KR, NA, KA, 4, Q2, Q1, ONE, true, kAnnMfma32, !ONE>>() <= 65536;
+  // single-type instances reach two wavefronts per SIMD, so they keep two
+  // workgroups per CU (64 KiB each); multi-type ones run one either way
+  return desc_lds_bytes<Cfg<NR, KR, NA, KA, 4, Q2, Q1, ONE, true, kAnnMfma32, !ONE>>() <=
+         (ONE ? 64 * 1024 : kDescLdsMax);
 }
 
 template <class C>
@@ -599,7 +604,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   // angular; the FP64 pair mean rounded once) and, with C::cl, the
   // coefficient blocks [pair][n][k] (radial, then angular), staged in LDS
   constexpr int kPairs = C::ltypes * C::ltypes;
-  static_assert(desc_lds_bytes<C>() <= 65536, "descriptor LDS above 64 KiB per workgroup");
+  static_assert(desc_lds_bytes<C>() <= kDescLdsMax, "descriptor LDS above its bound per workgroup");
   __shared__ float pcut[C::one ? 1 : 4 * kPairs];
   __shared__ float ctab[C::cl ? kPairs * (NR * KR + NA * KA) : 1];
   if (!C::one) {

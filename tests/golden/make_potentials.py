@@ -99,7 +99,7 @@ def two_type_sample(n=120, L=12.0, dmin=1.0, seed=0):
 # and typewise ZBL forms, per-type cutoffs and 3-body orders below 4, each on
 # a 2-type (C, H) system.
 # name: (version, zbl line or None, flexible ZBL per-pair table or None, seed,
-#        per-type (rc_radial, rc_angular) or None[, l_max line])
+#        per-type (rc_radial, rc_angular) or None[, l_max line[, n_max, basis]])
 UNIVERSAL = [0.18175, 3.1998, 0.50986, 0.94229, 0.28022, 0.4029, 0.02817, 0.20162]
 COVERAGE = {
     "nep5_ch.txt": (5, None, None, 201, None),
@@ -109,13 +109,16 @@ COVERAGE = {
     "nep4_typecut_ch.txt": (4, None, None, 204, [(4.5, 3.5), (3.6, 2.9)]),
     "nep4_lmax2_ch.txt": (4, None, None, 205, None, "2 2 0"),
     "nep4_lmax3_ch.txt": (4, None, None, 206, None, "3 0 1"),
+    "nep4_big_ch.txt": (4, None, None, 207, None, "4 2 1", (8, 6), (12, 12)),
 }
 
 
 def make_coverage(name, spec):
     version, zbl, flex, seed, cut = spec[:5]
     lmax = spec[5] if len(spec) > 5 else "4 2 1"
-    syms, rcR, rcA, nmax, basis, H = ["C", "H"], 4.5, 3.5, (4, 3), (6, 6), 20
+    nmax = spec[6] if len(spec) > 6 else (4, 3)
+    basis = spec[7] if len(spec) > 7 else (6, 6)
+    syms, rcR, rcA, H = ["C", "H"], 4.5, 3.5, 20
     if cut:
         rcR, rcA = max(c[0] for c in cut), max(c[1] for c in cut)
     T = len(syms)

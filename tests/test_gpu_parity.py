@@ -373,7 +373,8 @@ def test_ann_modes(dmgx, workloads, which):
 
 
 @pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt",
-                                  "nep4_typecut_ch.txt", "nep4_lmax2_ch.txt", "nep4_lmax3_ch.txt"])
+                                  "nep4_typecut_ch.txt", "nep4_lmax2_ch.txt", "nep4_lmax3_ch.txt",
+                                  "nep4_big_ch.txt"])
 @pytest.mark.parametrize("tiles", [1, 3])
 def test_nep5_and_zbl_forms_against_oracle(dmgx, name, tiles):
     """NEP5, the flexible per-pair ZBL table, the typewise ZBL cutoff,
@@ -532,7 +533,7 @@ def test_single_type_model_padded_to_a_bounded_instance(dmgx, workloads, tmp_pat
         check(errors(out, ref))
 
 
-@pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_typecut_ch.txt"])
+@pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_typecut_ch.txt", "nep4_big_ch.txt"])
 def test_two_type_ann_modes(dmgx, name):
     """Two-type models run their ANN on the matrix cores, one pass per type
     present in a wavefront: the FP32 result against the vector-ALU ANN (same

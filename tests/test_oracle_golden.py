@@ -122,7 +122,7 @@ def test_oracle_replays_gpumd_nvt_ber_trajectory():
         assert within(p, g["pos"], TOL["position"]), (k, np.abs(p - g["pos"]).max())
 
 
-COVERAGE_MODELS = ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt"]
+COVERAGE_MODELS = ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt", "nep4_big_ch.txt"]
 
 
 def coverage_sample(seed=7, n=100, L=11.0, dmin=0.5):
