@@ -241,22 +241,31 @@ def water_bench(dmgx, steps=100):
     NEP4 n_max 6/6, basis 10, l_max 4/2/1, rc 6/4 A, 40 neurons), dt 0.5 fs
     at 300 K: a Bussi NVT stretch (the sync-free chunked step) and the
     configuration's SCR NPT (1e-4 GPa; the barostat reads the pressure on
-    the host every step), each after 20 untimed steps."""
+    the host every step), each after 20 untimed steps. The generator inserts
+    molecules at random, so their storage order is spatially random: both
+    runs store the atoms in cell order first (dmgx_md_reorder), and the NVT
+    run is also timed in the generator's order."""
     import torch
     W = importlib.import_module("dmg-md_amd.workloads")
     s = W.liquid_water(64000, seed=1)
     N = len(s["types"])
     out = {"workload": "liquid water 64,000 H2O (192,000 atoms), 2-type NEP4 n_max 6/6 basis 10 l_max 4/2/1 "
-                       "rc 6.0/4.0 A 40 neurons, dt 0.5 fs, 300 K", "atoms": N}
-    for name, kw in (("nvt_bdp", dict(t_coupling=100.0)),
-                     ("npt_scr", dict(t_coupling=100.0, p_target_gpa=1e-4, bulk_modulus_gpa=2.2, p_coupling=1000.0))):
+                       "rc 6.0/4.0 A 40 neurons, dt 0.5 fs, 300 K", "atoms": N,
+           "storage_order": "cell order (dmgx_md_reorder) unless the key says input order"}
+    for name, ens, kw, reorder in (
+            ("nvt_bdp", "nvt_bdp", dict(t_coupling=100.0), True),
+            ("npt_scr", "npt_scr", dict(t_coupling=100.0, p_target_gpa=1e-4, bulk_modulus_gpa=2.2, p_coupling=1000.0),
+             True),
+            ("nvt_bdp_input_order", "nvt_bdp", dict(t_coupling=100.0), False)):
         nep = dmgx.NEP(C2_MODEL)
         md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+        if reorder:
+            md.reorder()
         md.autotune()
-        md.run(20, 0.5, name, t_init=300.0, t_final=300.0, **kw)
+        md.run(20, 0.5, ens, t_init=300.0, t_final=300.0, **kw)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        md.run(steps, 0.5, name, t_init=300.0, t_final=300.0, **kw)
+        md.run(steps, 0.5, ens, t_init=300.0, t_final=300.0, **kw)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         out[name] = {"atom_steps_per_s": N * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps}

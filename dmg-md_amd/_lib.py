@@ -118,6 +118,7 @@ _SIGNATURES = {
     "dmgx_md_arrays": (I, [VP] + [ctypes.POINTER(VP)] * 7),
     "dmgx_md_box": (I, [VP, ctypes.POINTER(Box)]),
     "dmgx_md_autotune": (I, [VP, U, ctypes.POINTER(Tuning)]),
+    "dmgx_md_reorder": (I, [VP, PI]),
     "dmgx_server_create": (I, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
     "dmgx_server_destroy": (None, [VP]),
     "dmgx_server_submit": (I, [VP, ctypes.POINTER(Box), I, PI, PD, PD, PD, I, D,

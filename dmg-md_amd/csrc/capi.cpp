@@ -449,6 +449,15 @@ int dmgx_md_box(dmgx_md* md, dmgx_box* out)
   });
 }
 
+int dmgx_md_reorder(dmgx_md* md, int* order_host)
+{
+  return guarded([&] {
+    if (!md) throw std::invalid_argument("null handle");
+    md->state.reorder();
+    if (order_host) std::memcpy(order_host, md->state.order().data(), sizeof(int) * md->state.order().size());
+  });
+}
+
 int dmgx_md_autotune(dmgx_md* md, unsigned flags, dmgx_tuning* out)
 {
   return guarded([&] {

@@ -873,6 +873,79 @@ MdState::MdState(NepEngine* nep, const Box& box, int N, const int* type, const d
   check(hipMemset(pe_, 0, sizeof(double) * N), "clear");
   check(hipMemset(vir_, 0, 9 * sizeof(double) * N), "clear");
   nep_->reserve(N);
+  order_.resize(N);
+  for (int i = 0; i < N; ++i) order_[i] = i;
+}
+
+namespace {
+// a[c·N + i] <- a[c·N + perm[i]] for each of the K components of an SoA array
+template <class T>
+void permute_soa(T* dev, int N, int K, const std::vector<int>& perm, hipStream_t s)
+{
+  std::vector<T> src((size_t)K * N), dst((size_t)K * N);
+  check(hipMemcpyAsync(src.data(), dev, sizeof(T) * src.size(), hipMemcpyDeviceToHost, s), "reorder download");
+  check(hipStreamSynchronize(s), "reorder download");
+  for (int c = 0; c < K; ++c)
+    for (int i = 0; i < N; ++i) dst[(size_t)c * N + i] = src[(size_t)c * N + perm[i]];
+  check(hipMemcpyAsync(dev, dst.data(), sizeof(T) * dst.size(), hipMemcpyHostToDevice, s), "reorder upload");
+  check(hipStreamSynchronize(s), "reorder upload");
+}
+}  // namespace
+
+void MdState::reorder()
+{
+  wrap();
+  std::vector<double> x(3 * (size_t)N_);
+  check(hipMemcpyAsync(x.data(), pos_, sizeof(double) * x.size(), hipMemcpyDeviceToHost, stream_), "download");
+  check(hipStreamSynchronize(stream_), "download");
+  double hi[9];
+  box_inverse(box_.h, hi);
+  // cells per direction: thickness / (half the Verlet range), at least one
+  const double* h = box_.h;
+  const double a[3] = {h[0], h[3], h[6]}, b[3] = {h[1], h[4], h[7]}, c[3] = {h[2], h[5], h[8]};
+  auto cross = [](const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  double bc[3], ca[3], ab[3];
+  cross(b, c, bc);
+  cross(c, a, ca);
+  cross(a, b, ab);
+  const double vol = std::fabs(box_volume(h));
+  const double* nrm[3] = {bc, ca, ab};
+  const double edge = 0.5 * (nep_->model().rcR_max + kSkin);
+  long long nc[3];
+  for (int d = 0; d < 3; ++d) {
+    const double thick = vol / std::sqrt(nrm[d][0] * nrm[d][0] + nrm[d][1] * nrm[d][1] + nrm[d][2] * nrm[d][2]);
+    nc[d] = std::max(1LL, (long long)std::floor(thick / edge));
+  }
+  std::vector<long long> key(N_);
+  for (int i = 0; i < N_; ++i) {
+    long long cidx[3];
+    for (int d = 0; d < 3; ++d) {
+      double s = hi[3 * d] * x[i] + hi[3 * d + 1] * x[N_ + i] + hi[3 * d + 2] * x[2 * (size_t)N_ + i];
+      s -= std::floor(s);  // open directions: a fraction of the current extent is enough for locality
+      cidx[d] = std::min(nc[d] - 1, std::max(0LL, (long long)std::floor(s * nc[d])));
+    }
+    key[i] = (cidx[2] * nc[1] + cidx[1]) * nc[0] + cidx[0];
+  }
+  std::vector<int> perm(N_);
+  for (int i = 0; i < N_; ++i) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int p, int q) { return key[p] < key[q]; });
+  permute_soa(pos_, N_, 3, perm, stream_);
+  permute_soa(vel_, N_, 3, perm, stream_);
+  permute_soa(force_, N_, 3, perm, stream_);
+  permute_soa(pe_, N_, 1, perm, stream_);
+  permute_soa(vir_, N_, 9, perm, stream_);
+  permute_soa(mass_, N_, 1, perm, stream_);
+  permute_soa(type_, N_, 1, perm, stream_);
+  if (unwrapped_) permute_soa(unwrapped_, N_, 3, perm, stream_);
+  std::vector<int> o(N_);
+  for (int i = 0; i < N_; ++i) o[i] = order_[perm[i]];
+  order_.swap(o);
+  // the Verlet rows hold storage indices: rebuild them at the next force
+  nep_->notify_box_changed();
 }
 
 MdState::~MdState()

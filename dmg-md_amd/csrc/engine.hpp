@@ -231,6 +231,13 @@ class MdState {
   // those positions' forces afterwards)
   const NepEngine::Tuning& autotune(unsigned flags);
   const double* unwrapped() const { return unwrapped_; }
+  // Stores the atoms in cell order (cells of half the Verlet range, z-major,
+  // input order kept within a cell), once, so that neighbours sit near each
+  // other in memory. Every per-atom array is permuted alike, forces included;
+  // order()[i] is the index, in the arrays given at construction, of the atom
+  // now stored at i (composed over repeated calls).
+  void reorder();
+  const std::vector<int>& order() const { return order_; }
   long long force_calls() const { return force_calls_; }
   void arrays(double** pos, double** vel, double** force, double** pe, double** vir, double** mass, int** type);
   const Box& box() const { return box_; }
@@ -259,6 +266,7 @@ class MdState {
   double* unwrapped_ = nullptr;          // device [3N] when enabled
   double* ck_unwrapped_ = nullptr;
   double elapsed_ = 0.0;
+  std::vector<int> order_;  // stored index -> construction index
 };
 
 double box_volume(const double* h);

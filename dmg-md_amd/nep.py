@@ -238,6 +238,15 @@ class MD:
         self._h = h
         self.nep = nep
         self.N = N
+        self.order = None  # stored index -> construction index, once reordered
+
+    def reorder(self) -> np.ndarray:
+        """Stores the atoms in cell order (``dmgx_md_reorder``); returns, for
+        each stored atom, its index in the arrays given at construction."""
+        o = np.zeros(self.N, np.int32)
+        check(lib().dmgx_md_reorder(self._h, o.ctypes.data_as(_lib.PI)))
+        self.order = o
+        return o
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -257,8 +266,10 @@ class MD:
                                 th.ctypes.data_as(_lib.PD)))
         return th
 
-    def arrays(self) -> dict:
-        """Host copies of position, velocity, force, potential, virial."""
+    def arrays(self, input_order=False) -> dict:
+        """Host copies of position, velocity, force, potential, virial, in
+        storage order, or with ``input_order`` in the order given at
+        construction (the same unless ``reorder`` was called)."""
         ptrs = [ctypes.c_void_p() for _ in range(7)]
         check(lib().dmgx_md_arrays(self._h, *[ctypes.byref(p) for p in ptrs]))
         out = {}
@@ -267,6 +278,11 @@ class MD:
             a = np.zeros(n * self.N)
             _copy_d2h(p, a)
             out[name] = a.reshape(n, self.N) if n > 1 else a
+        if input_order and self.order is not None:
+            for name, a in out.items():
+                b = np.empty_like(a)
+                b[..., self.order] = a
+                out[name] = b
         return out
 
     def box(self) -> Box:

@@ -258,6 +258,15 @@ int dmgx_md_run(dmgx_md* md, int steps, double dt_fs, const dmgx_ensemble* ens, 
 int dmgx_md_arrays(dmgx_md* md, double** position, double** velocity, double** force,
                    double** potential, double** virial, double** mass, int** type);
 int dmgx_md_box(dmgx_md* md, dmgx_box* box_out);
+/* Stores the state's atoms in cell order (cells of half the Verlet range),
+ * once, so that neighbours sit near each other in memory; every per-atom
+ * array, including dmgx_md_arrays', is permuted alike. order_host (N ints,
+ * may be NULL) receives, for each stored atom, its index in the arrays given
+ * to dmgx_md_create (composed over repeated calls). No counterpart in the
+ * reference, which keeps the input order: it is for inputs whose order is
+ * spatially random (random insertion, shuffled files), where the neighbour
+ * gathers miss the caches. */
+int dmgx_md_reorder(dmgx_md* md, int* order_host);
 /* dmgx_nep_autotune on the state's current positions (wrapped into the box
  * first); the state's forces are those positions' forces afterwards. */
 int dmgx_md_autotune(dmgx_md* md, unsigned flags, dmgx_tuning* out);

@@ -658,3 +658,44 @@ def test_npt_run_matches_oracle_replay(dmgx, workloads, ensemble, tau_p):
     assert abs(Lg / Lr - 1.0) < 1e-8
     assert np.abs(d).max() < 1e-6
     assert dv < 1e-5 * max(1.0, np.abs(vel).max())
+
+
+@pytest.mark.gpu
+def test_md_reorder_keeps_the_trajectory(dmgx, workloads):
+    """dmgx_md_reorder stores 6,000 randomly inserted water atoms in cell order:
+    a permutation whose neighbours in storage are near in space, and an NVT
+    run after it gives the trajectory of the run without it, mapped back
+    through the returned order (forces summed in another order: FP32
+    rounding only)."""
+    s = workloads.liquid_water(2000, seed=5)
+    runs = []
+    for reorder in (False, True):
+        nep = dmgx.NEP(C2)
+        md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+        if reorder:
+            o = md.reorder()
+            assert sorted(o.tolist()) == list(range(s["types"].shape[0]))
+            a = md.arrays()
+            # reorder wraps into the box first (fractional round trip: 1 ulp)
+            np.testing.assert_allclose(a["position"], s["pos"][:, o], rtol=0, atol=1e-12)
+            np.testing.assert_array_equal(a["velocity"], s["vel"][:, o])
+            L = np.diag(s["box"].h)
+            def step_len(x, lag=48):  # a wavefront's span of stored atoms
+                d = x[:, lag:] - x[:, :-lag]
+                d -= L[:, None] * np.round(d / L[:, None])
+                return np.median(np.sqrt((d * d).sum(0)))
+            assert step_len(a["position"]) < 0.4 * step_len(s["pos"])
+        md.run(20, 0.5, "nvt_bdp", t_init=300.0, t_final=300.0, t_coupling=100.0)
+        runs.append(md.arrays(input_order=True))
+        if reorder:
+            before = md.arrays(input_order=True)
+            o2 = md.reorder()
+            assert sorted(o2.tolist()) == list(range(s["types"].shape[0]))
+            after = md.arrays(input_order=True)
+            for k in before:
+                np.testing.assert_allclose(after[k], before[k], rtol=0, atol=1e-12 if k == "position" else 0)
+    ref, got = runs
+    assert np.abs(got["position"] - ref["position"]).max() < 1e-6
+    assert np.abs(got["velocity"] - ref["velocity"]).max() < 1e-5
+    assert np.abs(got["force"] - ref["force"]).max() < 1e-4
+    assert abs(got["potential"].sum() - ref["potential"].sum()) < 1e-6 * s["types"].shape[0]
