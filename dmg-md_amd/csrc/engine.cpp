@@ -878,6 +878,18 @@ MdState::MdState(NepEngine* nep, const Box& box, int N, const int* type, const d
 }
 
 namespace {
+// the low 21 bits of v spread to every third bit (Morton / Z-order interleave)
+unsigned long long morton3(long long v)
+{
+  unsigned long long x = (unsigned long long)v & 0x1fffffULL;
+  x = (x | x << 32) & 0x1f00000000ffffULL;
+  x = (x | x << 16) & 0x1f0000ff0000ffULL;
+  x = (x | x << 8) & 0x100f00f00f00f00fULL;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+  x = (x | x << 2) & 0x1249249249249249ULL;
+  return x;
+}
+
 // a[c·N + i] <- a[c·N + perm[i]] for each of the K components of an SoA array
 template <class T>
 void permute_soa(T* dev, int N, int K, const std::vector<int>& perm, hipStream_t s)
@@ -920,7 +932,7 @@ void MdState::reorder()
     const double thick = vol / std::sqrt(nrm[d][0] * nrm[d][0] + nrm[d][1] * nrm[d][1] + nrm[d][2] * nrm[d][2]);
     nc[d] = std::max(1LL, (long long)std::floor(thick / edge));
   }
-  std::vector<long long> key(N_);
+  std::vector<unsigned long long> key(N_);
   for (int i = 0; i < N_; ++i) {
     long long cidx[3];
     for (int d = 0; d < 3; ++d) {
@@ -928,7 +940,7 @@ void MdState::reorder()
       s -= std::floor(s);  // open directions: a fraction of the current extent is enough for locality
       cidx[d] = std::min(nc[d] - 1, std::max(0LL, (long long)std::floor(s * nc[d])));
     }
-    key[i] = (cidx[2] * nc[1] + cidx[1]) * nc[0] + cidx[0];
+    key[i] = morton3(cidx[0]) | (morton3(cidx[1]) << 1) | (morton3(cidx[2]) << 2);
   }
   std::vector<int> perm(N_);
   for (int i = 0; i < N_; ++i) perm[i] = i;

@@ -231,8 +231,8 @@ class MdState {
   // those positions' forces afterwards)
   const NepEngine::Tuning& autotune(unsigned flags);
   const double* unwrapped() const { return unwrapped_; }
-  // Stores the atoms in cell order (cells of half the Verlet range, z-major,
-  // input order kept within a cell), once, so that neighbours sit near each
+  // Stores the atoms in cell order (cells of half the Verlet range in Morton
+  // order, input order kept within a cell), once, so that neighbours sit near each
   // other in memory. Every per-atom array is permuted alike, forces included;
   // order()[i] is the index, in the arrays given at construction, of the atom
   // now stored at i (composed over repeated calls).

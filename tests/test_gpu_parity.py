@@ -663,7 +663,7 @@ def test_npt_run_matches_oracle_replay(dmgx, workloads, ensemble, tau_p):
 @pytest.mark.gpu
 def test_md_reorder_keeps_the_trajectory(dmgx, workloads):
     """dmgx_md_reorder stores 6,000 randomly inserted water atoms in cell order:
-    a permutation whose neighbours in storage are near in space, and an NVT
+    a permutation whose wavefront-sized groups are compact in space, and an NVT
     run after it gives the trajectory of the run without it, mapped back
     through the returned order (forces summed in another order: FP32
     rounding only)."""
@@ -680,11 +680,14 @@ def test_md_reorder_keeps_the_trajectory(dmgx, workloads):
             np.testing.assert_allclose(a["position"], s["pos"][:, o], rtol=0, atol=1e-12)
             np.testing.assert_array_equal(a["velocity"], s["vel"][:, o])
             L = np.diag(s["box"].h)
-            def step_len(x, lag=48):  # a wavefront's span of stored atoms
-                d = x[:, lag:] - x[:, :-lag]
-                d -= L[:, None] * np.round(d / L[:, None])
-                return np.median(np.sqrt((d * d).sum(0)))
-            assert step_len(a["position"]) < 0.4 * step_len(s["pos"])
+            def group_rms(x, g=64):  # median RMS radius of aligned wavefront-sized groups
+                n = x.shape[1] // g * g
+                d = x[:, :n].reshape(3, -1, g)
+                d = d - d[:, :, :1]
+                d -= L[:, None, None] * np.round(d / L[:, None, None])
+                d -= d.mean(2, keepdims=True)
+                return np.median(np.sqrt((d * d).sum(0).mean(1)))
+            assert group_rms(a["position"]) < 0.5 * group_rms(s["pos"])
         md.run(20, 0.5, "nvt_bdp", t_init=300.0, t_final=300.0, t_coupling=100.0)
         runs.append(md.arrays(input_order=True))
         if reorder:
