@@ -283,6 +283,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default filter/gather tile shapes")
     ap.add_argument("--no-water", action="store_true", help="skip the liquid-water (2-type) sample")
+    ap.add_argument("--water-only", type=int, default=0, metavar="STEPS",
+                    help="run only the liquid-water sample with STEPS timed steps per run and print its "
+                         "JSON object (for kernel profiles of the 2-type path)")
     ap.add_argument("--ann", choices=["fp32", "valu", "bf16"], default="fp32",
                     help="ANN arithmetic: fp32 on the matrix cores (default), fp32 on the vector ALUs, "
                          "or bfloat16 operands on the matrix cores")
@@ -310,6 +313,9 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
     dmgx = importlib.import_module("dmg-md_amd")
+    if args.water_only > 0:
+        print(json.dumps(water_bench(dmgx, steps=args.water_only)))
+        return
     W = importlib.import_module("dmg-md_amd.workloads")
     s = W.copper_fcc(n=args.cells, seed=1)
     N = len(s["types"])
