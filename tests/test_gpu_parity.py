@@ -702,3 +702,42 @@ def test_md_reorder_keeps_the_trajectory(dmgx, workloads):
     assert np.abs(got["velocity"] - ref["velocity"]).max() < 1e-5
     assert np.abs(got["force"] - ref["force"]).max() < 1e-4
     assert abs(got["potential"].sum() - ref["potential"].sum()) < 1e-6 * s["types"].shape[0]
+
+
+@pytest.mark.gpu
+def test_md_reorder_in_a_triclinic_box(dmgx):
+    """The reorder's cell key uses fractional coordinates: a shuffled
+    5,488-atom diamond supercell of the triclinic primitive cell (14 x 14 x
+    14 cells) runs the same NVE trajectory with and without it."""
+    rng = np.random.default_rng(3)
+    a, n = 3.567, 14
+    h = n * 0.5 * a * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]]).T  # columns a, b, c
+    ijk = np.array([[i, j, k] for i in range(n) for j in range(n) for k in range(n)], float)
+    frac = np.concatenate([ijk, ijk + 0.25]) / n
+    pos = h @ frac.T + rng.normal(0, 0.05, (3, frac.shape[0]))
+    N = pos.shape[1]
+    shuffle = rng.permutation(N)
+    pos = np.ascontiguousarray(pos[:, shuffle])
+    vel = rng.normal(0, 0.01, (3, N))
+    vel -= vel.mean(1, keepdims=True)
+    types, mass = np.zeros(N, np.int32), np.full(N, 12.011)
+    runs = []
+    for reorder in (False, True):
+        md = dmgx.MD(dmgx.NEP(C0), dmgx.Box(h), types, mass, pos, vel)
+        if reorder:
+            o = md.reorder()
+            assert sorted(o.tolist()) == list(range(N))
+            def group_rms(x, g=64):  # median RMS radius (fractional, minimum image) of wavefront groups
+                f = np.linalg.solve(h, x)[:, : N // g * g].reshape(3, -1, g)
+                f = f - f[:, :, :1]
+                f -= np.round(f)
+                d = np.einsum("ij,jkl->ikl", h, f)
+                d -= d.mean(2, keepdims=True)
+                return np.median(np.sqrt((d * d).sum(0).mean(1)))
+            assert group_rms(pos[:, o]) < 0.5 * group_rms(pos)
+        md.run(20, 0.5, "nve")
+        runs.append(md.arrays(input_order=True))
+    ref, got = runs
+    assert np.abs(got["position"] - ref["position"]).max() < 1e-6
+    assert np.abs(got["force"] - ref["force"]).max() < 1e-4
+    assert abs(got["potential"].sum() - ref["potential"].sum()) < 1e-6 * N
