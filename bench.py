@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "atom-steps/s (1 GPU, NEP4 Cu 256k); served frames/s + p99 latency at fixed force err"
-MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_c1_cu.txt")
+MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_cu.txt")
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (vector and MFMA), AMD public figure
 PEAK_HBM_GBS = 8000.0
 
@@ -102,7 +102,7 @@ def cpu_baseline(budget_s=10.0):
                       f"{el:.1f} s on 1 host core, FP64"}
 
 
-C0_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_c0_carbon.txt")
+CARBON_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_carbon.txt")
 
 
 def _carbon_request(rng, n_min=64, n_max=20000):
@@ -171,14 +171,14 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
 
     # first launches of the serving kernels; the server's autotuner picks the
     # tile shapes on the first batch of >= 64k atoms, here in this warm-up
-    nep_srv = dmgx.NEP(C0_MODEL)
+    nep_srv = dmgx.NEP(CARBON_MODEL)
     burst(min(requests, 48), nep_srv)
     tune = nep_srv.tuning()
     srv, ids, st = burst(requests, nep_srv)
     lat = poisson(nep_srv)
     # BF16 force error against FP32 on single points of the stream
     err = 0.0
-    nep32 = dmgx.NEP(C0_MODEL)
+    nep32 = dmgx.NEP(CARBON_MODEL)
     for k in range(0, min(requests, 16), 2):
         h, p, v, _ = reqs[k]
         n = p.shape[1]
@@ -216,7 +216,7 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
         frames += plain_one(h, p, v, stp)
     plain = frames / (time.perf_counter() - tp)
     return {"workload": f"{requests} carbon requests, 64..20000 atoms (log-uniform, 30% vacancies), "
-                        "half single points / half 50-step NVE at 0.5 fs, c0 NEP4 model, bf16 ANN",
+                        "half single points / half 50-step NVE at 0.5 fs, carbon NEP4 model, bf16 ANN",
             "frames_per_s": st["frames_per_s"], "atom_frames_per_s": st["atom_frames"] / max(st["elapsed_s"], 1e-9),
             "capacity_run": "all requests submitted at once by 8 client threads",
             "requests": st["completed"], "relayouts": st["relayouts"], "iterations": st["iterations"],
@@ -233,7 +233,7 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
             "speedup_vs_plain": st["frames_per_s"] / plain if plain > 0 else None}
 
 
-C2_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_c2_water.txt")
+WATER_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_water.txt")
 
 
 def water_bench(dmgx, steps=100):
@@ -257,7 +257,7 @@ def water_bench(dmgx, steps=100):
             ("npt_scr", "npt_scr", dict(t_coupling=100.0, p_target_gpa=1e-4, bulk_modulus_gpa=2.2, p_coupling=1000.0),
              True),
             ("nvt_bdp_input_order", "nvt_bdp", dict(t_coupling=100.0), False)):
-        nep = dmgx.NEP(C2_MODEL)
+        nep = dmgx.NEP(WATER_MODEL)
         md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
         if reorder:
             md.reorder()

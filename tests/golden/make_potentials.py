@@ -28,13 +28,13 @@ W = importlib.import_module("dmg-md_amd.workloads")
 
 SPECS = {
     # name: (symbols, rcR, rcA, n_max, basis, l_max line, neurons, zbl line, sample, seed)
-    "nep_c0_carbon.txt": (["C"], 4.2, 3.7, (4, 4), (8, 8), "4 2 1", 30, None,
+    "nep_carbon.txt": (["C"], 4.2, 3.7, (4, 4), (8, 8), "4 2 1", 30, None,
                           lambda: W.carbon_diamond_4096(seed=11), 101),
-    "nep_c1_cu.txt": (["Cu"], 5.0, 4.0, (6, 4), (8, 8), "4 2 0", 50, None,
+    "nep_cu.txt": (["Cu"], 5.0, 4.0, (6, 4), (8, 8), "4 2 0", 50, None,
                       lambda: W.copper_fcc(n=10, seed=12), 102),
-    "nep_c4_carbon_zbl.txt": (["C"], 4.2, 3.7, (4, 4), (8, 8), "4 2 1", 30, "zbl 1.0 2.0",
+    "nep_carbon_zbl.txt": (["C"], 4.2, 3.7, (4, 4), (8, 8), "4 2 1", 30, "zbl 1.0 2.0",
                               lambda: W.carbon_diamond_4096(seed=13), 104),
-    "nep_c2_water.txt": (["O", "H"], 6.0, 4.0, (6, 6), (10, 10), "4 2 1", 40, None,
+    "nep_water.txt": (["O", "H"], 6.0, 4.0, (6, 6), (10, 10), "4 2 1", 40, None,
                          lambda: W.liquid_water(64000, seed=23), 103),
 }
 

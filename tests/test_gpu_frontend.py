@@ -160,11 +160,11 @@ def test_golden_case_replays_end_to_end(dmgx, tmp_path, case):
 # velocity / correct_velocity / adaptive time_step through the front end
 # ---------------------------------------------------------------------------
 
-def _c0_dir(tmp_path, run_in, n_cells=2, vel=None, groups=None, seed=3):
-    """A jittered carbon diamond model.xyz (c0 model) with optional velocities
+def _carbon_dir(tmp_path, run_in, n_cells=2, vel=None, groups=None, seed=3):
+    """A jittered carbon diamond model.xyz (carbon model) with optional velocities
     (A/fs) and one grouping method, plus run.in."""
     from conftest import POTENTIALS
-    shutil.copy(os.path.join(POTENTIALS, "nep_c0_carbon.txt"), tmp_path / "nep.txt")
+    shutil.copy(os.path.join(POTENTIALS, "nep_carbon.txt"), tmp_path / "nep.txt")
     a = 3.567
     frac = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
                      [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
@@ -223,7 +223,7 @@ def test_velocity_command_draw_correction_and_temperature(dmgx, tmp_path):
     import ctypes
     import importlib
     cli = importlib.import_module("dmg-md_amd.cli")
-    pos = _c0_dir(tmp_path, "potential nep.txt\nvelocity 300 seed 7\ntime_step 0\nensemble nve\n"
+    pos = _carbon_dir(tmp_path, "potential nep.txt\nvelocity 300 seed 7\ntime_step 0\nensemble nve\n"
                             "dump_restart 1\nrun 1\n")
     cli.run_directory(str(tmp_path), "run.in", "model.xyz")
     from xyz_io import read_xyz, TIME_UNIT_CONVERSION as TU
@@ -256,7 +256,7 @@ def test_correct_velocity_removes_momentum_per_group(dmgx, tmp_path):
     n = 64
     vel = rng.normal(0, 0.01, (n, 3)) + np.array([0.004, -0.002, 0.003])
     groups = (np.arange(n) % 3 == 0).astype(int)
-    pos = _c0_dir(tmp_path, "potential nep.txt\ntime_step 0\nensemble nve\ncorrect_velocity 10 0\n"
+    pos = _carbon_dir(tmp_path, "potential nep.txt\ntime_step 0\nensemble nve\ncorrect_velocity 10 0\n"
                             "dump_restart 1\nrun 1\n", vel=vel, groups=groups)
     cli.run_directory(str(tmp_path), "run.in", "model.xyz")
     from xyz_io import read_xyz
@@ -288,10 +288,10 @@ def test_adaptive_time_step_matches_oracle_replay(dmgx, tmp_path):
     vel = rng.normal(0, 0.012, (n, 3))
     vel -= vel.mean(0)
     steps = 6
-    pos = _c0_dir(tmp_path, "potential nep.txt\ntime_step 1 0.005\nensemble nve\n"
+    pos = _carbon_dir(tmp_path, "potential nep.txt\ntime_step 1 0.005\nensemble nve\n"
                             f"dump_xyz 1 traj.xyz precision double velocity\nrun {steps}\n", vel=vel)
     cli.run_directory(str(tmp_path), "run.in", "model.xyz")
-    orc = OracleNEP(os.path.join(POTENTIALS, "nep_c0_carbon.txt"), round_params_to_float=True)
+    orc = OracleNEP(os.path.join(POTENTIALS, "nep_carbon.txt"), round_params_to_float=True)
     L = 3.567 * 2
     h = np.diag([L, L, L])
     mass = np.full(n, 12.011)
@@ -335,7 +335,7 @@ def test_dump_xyz_group_filter_and_unwrapped_positions(dmgx, tmp_path):
     vel -= vel.mean(0)
     groups = (np.arange(n) % 4 == 1).astype(int)
     steps = 8
-    pos = _c0_dir(tmp_path, "potential nep.txt\ntime_step 1\nensemble nve\n"
+    pos = _carbon_dir(tmp_path, "potential nep.txt\ntime_step 1\nensemble nve\n"
                             "dump_xyz 1 g.xyz precision double group 0 1 unwrapped_position velocity\n"
                             f"run {steps}\n", vel=vel, groups=groups)
     cli.run_directory(str(tmp_path), "run.in", "model.xyz")

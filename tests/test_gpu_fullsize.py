@@ -1,5 +1,5 @@
 """The benchmark configuration at its full size: 256,000-atom Cu fcc with the
-c1 NEP4 model.  Forces, energies and virials of every atom against the FP64
+Cu NEP4 model.  Forces, energies and virials of every atom against the FP64
 oracle (FP32 and BF16-operand ANN), and an NVE run whose total energy per
 atom stays within 1e-5 eV of its start (the reference's long-NVE metrics:
 energy per atom, its largest excursion and its linear drift,
@@ -15,7 +15,7 @@ from oracle import OracleNEP
 
 pytestmark = pytest.mark.gpu
 
-C1 = os.path.join(POTENTIALS, "nep_c1_cu.txt")
+CU = os.path.join(POTENTIALS, "nep_cu.txt")
 K_B = 8.617343e-5
 
 
@@ -27,14 +27,14 @@ def cu256k(workloads):
 @pytest.fixture(scope="module")
 def oracle_cu256k(cu256k):
     s = cu256k
-    return OracleNEP(C1, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
+    return OracleNEP(CU, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
 
 
 def test_fullsize_against_oracle(dmgx, cu256k, oracle_cu256k):
     s, ref = cu256k, oracle_cu256k
     N = len(s["types"])
     assert N == 256000
-    out = gpu_compute(dmgx, dmgx.NEP(C1), s["box"], s["types"], s["pos"])
+    out = gpu_compute(dmgx, dmgx.NEP(CU), s["box"], s["types"], s["pos"])
     err = errors(out, ref)
     assert err["force"] <= 1e-4, err
     assert err["energy_per_atom"] <= 1e-6, err
@@ -48,7 +48,7 @@ def test_fullsize_bf16_ann_error(dmgx, cu256k, oracle_cu256k):
     """BF16 operands on the matrix cores: the error against FP64 is that of
     bfloat16 rounding of the ANN inputs and weights, reported and bounded."""
     s, ref = cu256k, oracle_cu256k
-    out = gpu_compute(dmgx, dmgx.NEP(C1), s["box"], s["types"], s["pos"], ann_bf16=True)
+    out = gpu_compute(dmgx, dmgx.NEP(CU), s["box"], s["types"], s["pos"], ann_bf16=True)
     err = errors(out, ref)
     print("bf16 ANN vs FP64:", err)
     assert err["force"] <= 0.05, err
@@ -57,7 +57,7 @@ def test_fullsize_bf16_ann_error(dmgx, cu256k, oracle_cu256k):
 
 def _nve_energies(dmgx, s, dt, chunk, chunks=20):
     N = len(s["types"])
-    md = dmgx.MD(dmgx.NEP(C1), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    md = dmgx.MD(dmgx.NEP(CU), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
     th = md.run(0, dt, "nve")
     e = [(1.5 * N * K_B * th[0] + th[1]) / N]
     for _ in range(chunks):
@@ -91,7 +91,7 @@ def test_fullsize_nve_energy_drift(dmgx, cu256k):
 
 # ---- liquid water: 64,000 H2O (192,000 atoms), 2-type NEP4 ----------------
 
-C2 = os.path.join(POTENTIALS, "nep_c2_water.txt")
+WATER = os.path.join(POTENTIALS, "nep_water.txt")
 
 
 @pytest.fixture(scope="module")
@@ -106,8 +106,8 @@ def test_water_fullsize_against_oracle(dmgx, water192k):
     s = water192k
     N = len(s["types"])
     assert N == 192000
-    ref = OracleNEP(C2, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
-    out = gpu_compute(dmgx, dmgx.NEP(C2), s["box"], s["types"], s["pos"])
+    ref = OracleNEP(WATER, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
+    out = gpu_compute(dmgx, dmgx.NEP(WATER), s["box"], s["types"], s["pos"])
     err = errors(out, ref)
     fmax = np.abs(ref["force"]).max()
     print("water 192k vs FP64:", err, "max |F|", fmax)
@@ -130,13 +130,13 @@ def test_water_fullsize_scr_npt_run(dmgx, water192k):
     s = water192k
     N = len(s["types"])
     kw = dict(T=300.0, P=1e-4, B=2.2, tau_p=1000.0, tau_t=100.0, seed=12345678)
-    md = dmgx.MD(dmgx.NEP(C2), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    md = dmgx.MD(dmgx.NEP(WATER), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
     run = lambda k: md.run(k, 0.5, "npt_scr", t_init=kw["T"], t_final=kw["T"], t_coupling=kw["tau_t"],
                            p_target_gpa=kw["P"], bulk_modulus_gpa=kw["B"], p_coupling=kw["tau_p"], seed=kw["seed"])
     run(2)
     a = md.arrays()
     hg = np.array(md.box().h, float).reshape(3, 3)
-    h, pos, vel = _npt_replay(dmgx, s, "npt_scr", steps=2, dt_fs=0.5, model=C2, **kw)
+    h, pos, vel = _npt_replay(dmgx, s, "npt_scr", steps=2, dt_fs=0.5, model=WATER, **kw)
     L = h[0, 0]
     d = a["position"] - pos
     d -= L * np.round(d / L)
@@ -145,7 +145,7 @@ def test_water_fullsize_scr_npt_run(dmgx, water192k):
     assert abs(hg[0, 0] / L - 1.0) < 1e-9
     assert np.abs(d).max() < 1e-6
     assert np.abs(a["velocity"] - vel).max() < 1e-5 * max(1.0, np.abs(vel).max())
-    md = dmgx.MD(dmgx.NEP(C2), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    md = dmgx.MD(dmgx.NEP(WATER), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
     run(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -163,7 +163,7 @@ def test_water_fullsize_scr_npt_run(dmgx, water192k):
 
 # ---- one GPU's share: amorphous carbon, 2,048,000 atoms + 5 A ghost shell ----
 
-C4 = os.path.join(POTENTIALS, "nep_c4_carbon_zbl.txt")
+CARBON_ZBL = os.path.join(POTENTIALS, "nep_carbon_zbl.txt")
 
 
 @pytest.fixture(scope="module")
@@ -187,7 +187,7 @@ def _crop_oracle(s, centre, core=10.0, rc=4.2):
     rest_i = np.nonzero((d >= core + rc + 0.5) & (d < core + 2.0 * rc + 1.0))[0]
     idx = np.concatenate([core_i, dep_i, rest_i])
     sub = np.ascontiguousarray(x[idx].T)
-    ref = OracleNEP(C4, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"][idx], sub,
+    ref = OracleNEP(CARBON_ZBL, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"][idx], sub,
                                                             n1=0, n2=len(core_i), nd1=0,
                                                             nd2=len(core_i) + len(dep_i))
     return core_i, ref
@@ -230,7 +230,7 @@ def test_carbon_share_against_oracle(dmgx, carbon_share):
     n = s["owned"]
     assert n == 2048000
     crops = [_crop_oracle(s, c) for c in _crop_centres(s)]
-    nep = dmgx.NEP(C4)
+    nep = dmgx.NEP(CARBON_ZBL)
     for bf16 in (False, True):
         pe, f, v = _share_compute(dmgx, nep, s, bf16=bf16)
         pe, f, v = pe.cpu().numpy(), f.cpu().numpy(), v.cpu().numpy()
@@ -265,7 +265,7 @@ def _share_nve(dmgx, s, dt_fs, steps, bf16):
     N, n = len(s["types"]), s["owned"]
     dt = dt_fs / 10.18051
     m = torch.tensor(s["mass"][:n], dtype=torch.float64, device="cuda")
-    nep = dmgx.NEP(C4)
+    nep = dmgx.NEP(CARBON_ZBL)
     x = torch.tensor(np.ascontiguousarray(s["pos"]).reshape(-1), dtype=torch.float64, device="cuda")
     xv = x.view(3, N)
     v = torch.tensor(np.ascontiguousarray(s["vel"]), dtype=torch.float64, device="cuda").view(3, n)

@@ -17,9 +17,9 @@ from xyz_io import XYZ_VIRIAL_TO_SOA, read_xyz, types_of
 
 pytestmark = pytest.mark.gpu
 
-C0 = os.path.join(POTENTIALS, "nep_c0_carbon.txt")
-C1 = os.path.join(POTENTIALS, "nep_c1_cu.txt")
-C4 = os.path.join(POTENTIALS, "nep_c4_carbon_zbl.txt")
+CARBON = os.path.join(POTENTIALS, "nep_carbon.txt")
+CU = os.path.join(POTENTIALS, "nep_cu.txt")
+CARBON_ZBL = os.path.join(POTENTIALS, "nep_carbon_zbl.txt")
 
 TOL_F, TOL_E, TOL_EA, TOL_V = 1e-4, 1e-6, 2e-5, 2e-4
 
@@ -63,11 +63,11 @@ def test_golden_systems_against_oracle_and_gpumd(dmgx, case, stage, pot, fscale)
 
 def _system(workloads, which):
     if which == "c0":
-        return workloads.carbon_diamond_4096(seed=1), C0
+        return workloads.carbon_diamond_4096(seed=1), CARBON
     if which == "c1":
-        return workloads.copper_fcc(n=10, seed=3), C1
+        return workloads.copper_fcc(n=10, seed=3), CU
     if which == "c4":
-        return workloads.carbon_diamond_4096(seed=5), C4
+        return workloads.carbon_diamond_4096(seed=5), CARBON_ZBL
     raise KeyError(which)
 
 
@@ -311,8 +311,8 @@ def _small_cells(workloads):
 def test_small_boxes_against_oracle(dmgx, workloads):
     """Boxes thinner than 2.5 (rc + skin) go through periodic replicas; the
     oracle sums the periodic images directly."""
-    nep = dmgx.NEP(C0)
-    orc = OracleNEP(C0, round_params_to_float=True)
+    nep = dmgx.NEP(CARBON)
+    orc = OracleNEP(CARBON, round_params_to_float=True)
     for h, pos in _small_cells(workloads):
         n = pos.shape[1]
         t = np.zeros(n, np.int32)
@@ -333,11 +333,11 @@ def test_small_box_nve_matches_oracle(dmgx, workloads):
     vel = np.random.default_rng(3).normal(0, 0.01, (3, n))
     vel -= vel.mean(1, keepdims=True)
     box = dmgx.Box(h, (1, 1, 1))
-    md = dmgx.MD(dmgx.NEP(C0), box, t, mass, pos, vel)
+    md = dmgx.MD(dmgx.NEP(CARBON), box, t, mass, pos, vel)
     steps, dt = 5, 1.0
     md.run(steps, dt, "nve")
     got = md.arrays()
-    orc = OracleNEP(C0, round_params_to_float=True)
+    orc = OracleNEP(CARBON, round_params_to_float=True)
     x, v = pos.copy(), vel.copy()
     O.wrap_positions(h, (1, 1, 1), x)
     o = orc.compute(h, (1, 1, 1), t, x)
@@ -483,9 +483,9 @@ def test_sparse_open_box_uses_the_multi_block_cell_scan(dmgx, workloads):
     box = dmgx.Box.orthogonal(210.0, 210.0, 210.0, pbc=(0, 0, 0))
     types = np.zeros(N, np.int32)
     p = np.ascontiguousarray(xyz.T).reshape(-1)
-    nep = dmgx.NEP(C0)
+    nep = dmgx.NEP(CARBON)
     out = gpu_compute(dmgx, nep, box, types, p)
-    ref = OracleNEP(C0, round_params_to_float=True).compute(box.h, box.pbc, types, p)
+    ref = OracleNEP(CARBON, round_params_to_float=True).compute(box.h, box.pbc, types, p)
     check(errors(out, ref))
     rv = nep.info.rc_radial_max + 1.0
     rows = nep.verlet_rows(0, N)
@@ -520,7 +520,7 @@ def test_single_type_model_padded_to_a_bounded_instance(dmgx, workloads, tmp_pat
     model without its 5-body block) runs padded to a bounded instance, with
     its ANN on the matrix cores or the vector ALUs."""
     path = str(tmp_path / "c0_no5.txt")
-    _drop_5body(C0, path)
+    _drop_5body(CARBON, path)
     s = workloads.carbon_diamond_4096(seed=1)
     out = gpu_compute(dmgx, dmgx.NEP(path), s["box"], s["types"], s["pos"], ann_bf16=ann == "bf16",
                       ann_valu=ann == "valu")
@@ -561,24 +561,24 @@ def test_two_type_ann_modes(dmgx, name):
     assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
 
 
-C2 = os.path.join(POTENTIALS, "nep_c2_water.txt")
+WATER = os.path.join(POTENTIALS, "nep_water.txt")
 TIME_UNIT = 10.18051   # fs per natural time unit
 PRESSURE_UNIT = 160.2177  # GPa per eV/A^3
 KB = 8.617343e-5
 
 
-def test_c2_water_against_oracle(dmgx, workloads):
+def test_water_model_against_oracle(dmgx, workloads):
     """The liquid-water configuration's model (2 types, n_max 6/6, basis 10,
     l_max 4/2/1, rc 6/4 A, 40 neurons) on 1000 molecules of the same
     generator, against the FP64 oracle."""
     s = workloads.liquid_water(1000, seed=3)
-    out = gpu_compute(dmgx, dmgx.NEP(C2), s["box"], s["types"], s["pos"])
-    ref = OracleNEP(C2, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
+    out = gpu_compute(dmgx, dmgx.NEP(WATER), s["box"], s["types"], s["pos"])
+    ref = OracleNEP(WATER, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
     fscale = max(1.0, np.abs(ref["force"]).max() / 10.0)
     check(errors(out, ref), scale_f=fscale)
 
 
-def _npt_replay(dmgx, s, ensemble, steps, dt_fs, T, P, B, tau_p, tau_t, seed, model=C2):
+def _npt_replay(dmgx, s, ensemble, steps, dt_fs, T, P, B, tau_p, tau_t, seed, model=WATER):
     """FP64 replay of an NPT run with the oracle's forces, integrator and
     thermo, and the ensemble step of the reference (ensemble_control.cpp:
     Berendsen or Bussi velocity factor after the second kick, then the box
@@ -637,7 +637,7 @@ def test_npt_run_matches_oracle_replay(dmgx, workloads, ensemble, tau_p):
     rescales (fewer rebuilds than steps)."""
     s = workloads.liquid_water(1000, seed=4)
     kw = dict(steps=10, dt_fs=0.5, T=300.0, P=1e-4, B=2.2, tau_p=tau_p, tau_t=100.0, seed=12345)
-    nep = dmgx.NEP(C2)
+    nep = dmgx.NEP(WATER)
     md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
     md.run(kw["steps"], kw["dt_fs"], ensemble, t_init=kw["T"], t_final=kw["T"], t_coupling=kw["tau_t"],
            p_target_gpa=kw["P"], bulk_modulus_gpa=kw["B"], p_coupling=kw["tau_p"], seed=kw["seed"])
@@ -670,7 +670,7 @@ def test_md_reorder_keeps_the_trajectory(dmgx, workloads):
     s = workloads.liquid_water(2000, seed=5)
     runs = []
     for reorder in (False, True):
-        nep = dmgx.NEP(C2)
+        nep = dmgx.NEP(WATER)
         md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
         if reorder:
             o = md.reorder()
@@ -723,7 +723,7 @@ def test_md_reorder_in_a_triclinic_box(dmgx):
     types, mass = np.zeros(N, np.int32), np.full(N, 12.011)
     runs = []
     for reorder in (False, True):
-        md = dmgx.MD(dmgx.NEP(C0), dmgx.Box(h), types, mass, pos, vel)
+        md = dmgx.MD(dmgx.NEP(CARBON), dmgx.Box(h), types, mass, pos, vel)
         if reorder:
             o = md.reorder()
             assert sorted(o.tolist()) == list(range(N))

@@ -13,7 +13,7 @@ from oracle import OracleNEP
 
 pytestmark = pytest.mark.gpu
 
-C0 = os.path.join(POTENTIALS, "nep_c0_carbon.txt")
+CARBON = os.path.join(POTENTIALS, "nep_carbon.txt")
 TOL_F, TOL_EA, TOL_V = 1e-4, 2e-5, 2e-4
 
 
@@ -39,8 +39,8 @@ def _check_single(out, h, pos, orc):
 
 
 def test_server_requests_match_standalone(dmgx, workloads):
-    orc = OracleNEP(C0, round_params_to_float=True)
-    srv = dmgx.Server(dmgx.NEP(C0), max_atoms=2_000_000)
+    orc = OracleNEP(CARBON, round_params_to_float=True)
+    srv = dmgx.Server(dmgx.NEP(CARBON), max_atoms=2_000_000)
     # a 64-atom NVE request runs first; a large single point joins mid-run
     h64, p64 = _diamond(2, 0.05, 1)
     v64 = np.random.default_rng(2).normal(0, 0.01, p64.shape)
@@ -64,7 +64,7 @@ def test_server_requests_match_standalone(dmgx, workloads):
     _check_single(srv.poll(r_216), h216, p216, orc)
     # the NVE request against the standalone MD of the same system
     got = srv.poll(r_md)
-    md = dmgx.MD(dmgx.NEP(C0), box64, np.zeros(64, np.int32), m64, p64, v64)
+    md = dmgx.MD(dmgx.NEP(CARBON), box64, np.zeros(64, np.int32), m64, p64, v64)
     md.run(steps, 1.0, "nve")
     ref = md.arrays()
     L = h64[0, 0]
@@ -78,8 +78,8 @@ def test_server_requests_match_standalone(dmgx, workloads):
 def test_server_many_small_requests(dmgx, workloads):
     """Many concurrent single points of different sizes: all correct, and the
     loop reports its frames and latency percentiles."""
-    orc = OracleNEP(C0, round_params_to_float=True)
-    srv = dmgx.Server(dmgx.NEP(C0), max_atoms=400_000)
+    orc = OracleNEP(CARBON, round_params_to_float=True)
+    srv = dmgx.Server(dmgx.NEP(CARBON), max_atoms=400_000)
     cases = []
     for k in range(24):
         reps = 1 + k % 3
@@ -117,8 +117,8 @@ def test_server_image_map_edge_cases(dmgx):
     tw = np.array([sym.index(l.split()[0]) for l in wat], np.int32)
     pw = np.array([[float(x) for x in l.split()[1:4]] for l in wat]).T
     hw = np.diag([8.0, 8.5, 9.0])
-    cases = [(C0, h_prim, np.zeros(2, np.int32), p_prim), (C0, h_sh, np.zeros(64, np.int32), p_sh),
-             (os.path.join(POTENTIALS, "nep_c4_carbon_zbl.txt"), hz, np.zeros(64, np.int32), pz),
+    cases = [(CARBON, h_prim, np.zeros(2, np.int32), p_prim), (CARBON, h_sh, np.zeros(64, np.int32), p_sh),
+             (os.path.join(POTENTIALS, "nep_carbon_zbl.txt"), hz, np.zeros(64, np.int32), pz),
              (water, hw, tw, pw)]
     by_model = {}
     for path, h, t, p in cases:
@@ -148,7 +148,7 @@ def test_graph_replays_match_eager_iterations(dmgx):
     the launches eagerly (placement translations may round differently when
     admission timing differs, hence 1e-9 rather than bitwise)."""
     def serve(graph):
-        srv = dmgx.Server(dmgx.NEP(C0), max_atoms=400_000, graph=graph)
+        srv = dmgx.Server(dmgx.NEP(CARBON), max_atoms=400_000, graph=graph)
         out = []
         rng = np.random.default_rng(31)
         for k in range(6):

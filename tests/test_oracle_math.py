@@ -185,7 +185,7 @@ def test_zbl_pair_is_universal_screened_coulomb():
 
 
 def test_seeded_models_parse_and_are_finite():
-    for name in ("nep_c0_carbon.txt", "nep_c1_cu.txt", "nep_c4_carbon_zbl.txt"):
+    for name in ("nep_carbon.txt", "nep_cu.txt", "nep_carbon_zbl.txt"):
         nep = OracleNEP(os.path.join(POTENTIALS, name))
         assert nep.version == 4
 
