@@ -62,16 +62,16 @@ def test_golden_systems_against_oracle_and_gpumd(dmgx, case, stage, pot, fscale)
 
 
 def _system(workloads, which):
-    if which == "c0":
+    if which == "carbon":
         return workloads.carbon_diamond_4096(seed=1), CARBON
-    if which == "c1":
+    if which == "cu":
         return workloads.copper_fcc(n=10, seed=3), CU
-    if which == "c4":
+    if which == "carbon_zbl":
         return workloads.carbon_diamond_4096(seed=5), CARBON_ZBL
     raise KeyError(which)
 
 
-@pytest.mark.parametrize("which", ["c0", "c1", "c4"])
+@pytest.mark.parametrize("which", ["carbon", "cu", "carbon_zbl"])
 def test_config_force_energy_virial(dmgx, workloads, which):
     s, path = _system(workloads, which)
     out = gpu_compute(dmgx, dmgx.NEP(path), s["box"], s["types"], s["pos"])
@@ -80,7 +80,7 @@ def test_config_force_energy_virial(dmgx, workloads, which):
     assert abs(out[1].sum(1)).max() < 1e-6 * len(s["types"])  # momentum conservation
 
 
-@pytest.mark.parametrize("which", ["c0", "c1"])
+@pytest.mark.parametrize("which", ["carbon", "cu"])
 def test_neighbor_rows_bit_exact(dmgx, workloads, which):
     s, path = _system(workloads, which)
     nep = dmgx.NEP(path)
@@ -98,7 +98,7 @@ def test_neighbor_rows_bit_exact(dmgx, workloads, which):
 
 def test_deterministic_and_accumulate(dmgx, workloads):
     import torch
-    s, path = _system(workloads, "c0")
+    s, path = _system(workloads, "carbon")
     nep = dmgx.NEP(path)
     a = gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"], force_rebuild=True)
     b = gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"])
@@ -117,7 +117,7 @@ def test_deterministic_and_accumulate(dmgx, workloads):
 
 
 def test_verlet_reuse_within_skin_and_rebuild(dmgx, workloads):
-    s, path = _system(workloads, "c1")
+    s, path = _system(workloads, "cu")
     nep = dmgx.NEP(path)
     gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"])
     r0 = nep.neighbor_stats()["rebuilds"]
@@ -136,7 +136,7 @@ def test_verlet_reuse_within_skin_and_rebuild(dmgx, workloads):
 def test_domain_ranges(dmgx, workloads):
     """Force centres [0,n2) of a full-dependency evaluation equal the plain
     call; dependency centres restricted to what the force centres need."""
-    s, path = _system(workloads, "c0")
+    s, path = _system(workloads, "carbon")
     nep = dmgx.NEP(path)
     N = len(s["types"])
     full = gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"])
@@ -154,7 +154,7 @@ def test_domain_ranges(dmgx, workloads):
 def test_thermo_and_integrator_match_oracle(dmgx, workloads):
     import torch
     import oracle as O
-    s, path = _system(workloads, "c0")
+    s, path = _system(workloads, "carbon")
     N = len(s["types"])
     rng = np.random.default_rng(2)
     f = rng.normal(size=(3, N))
@@ -187,7 +187,7 @@ def test_nve_trajectory_matches_oracle(dmgx, workloads):
     """20 NVE steps at 1 fs of the 4096-atom carbon config: positions,
     velocities and energies against an FP64 oracle trajectory."""
     import oracle as O
-    s, path = _system(workloads, "c0")
+    s, path = _system(workloads, "carbon")
     md = dmgx.MD(dmgx.NEP(path), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
     steps, dt = 20, 1.0
     th = md.run(steps, dt, "nve")
@@ -213,7 +213,7 @@ def test_nve_trajectory_matches_oracle(dmgx, workloads):
 
 def test_rejects_host_tensors_and_bad_ranges(dmgx, workloads):
     import torch
-    s, path = _system(workloads, "c0")
+    s, path = _system(workloads, "carbon")
     nep = dmgx.NEP(path)
     N = len(s["types"])
     t = torch.tensor(s["types"], dtype=torch.int32)
@@ -235,7 +235,7 @@ def test_thermostat_chunks_match_stepwise_calls(dmgx, workloads, ensemble):
     import math
     import torch
     from ctypes import c_void_p as P
-    s, path = _system(workloads, "c0")
+    s, path = _system(workloads, "carbon")
     N = len(s["types"])
     steps, dt_fs, t_target, tau, seed = 12, 1.0, 500.0, 10.0, 777
     md = dmgx.MD(dmgx.NEP(path), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
@@ -351,7 +351,7 @@ def test_small_box_nve_matches_oracle(dmgx, workloads):
     assert np.abs(got["velocity"] - v).max() < 1e-7
 
 
-@pytest.mark.parametrize("which", ["c0", "c1"])
+@pytest.mark.parametrize("which", ["carbon", "cu"])
 def test_ann_modes(dmgx, workloads, which):
     """The FP32 ANN on the matrix cores (default) against the FP32 ANN on the
     vector ALUs (same products in FP32, different summation order), and the
@@ -408,7 +408,7 @@ def test_nep5_and_zbl_forms_against_oracle(dmgx, name, tiles):
 def test_every_tile_shape_against_oracle(dmgx, workloads):
     """Each filter / gather shape the autotuner may pick gives the oracle's
     forces, energies and virials (Cu fcc 10^3, 4000 atoms)."""
-    s, path = _system(workloads, "c1")
+    s, path = _system(workloads, "cu")
     ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
     nep = dmgx.NEP(path)
     for fl in (8, 16, 32):
@@ -425,7 +425,7 @@ def test_every_tile_shape_against_oracle(dmgx, workloads):
 
 def test_autotune_picks_a_candidate_and_keeps_parity(dmgx, workloads):
     import torch
-    s, path = _system(workloads, "c0")
+    s, path = _system(workloads, "carbon")
     ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
     nep = dmgx.NEP(path)
     t0 = nep.tuning()
@@ -453,7 +453,7 @@ def test_autotune_picks_a_candidate_and_keeps_parity(dmgx, workloads):
 
 
 def test_md_autotune_leaves_the_state_forces(dmgx, workloads):
-    s, path = _system(workloads, "c1")
+    s, path = _system(workloads, "cu")
     nep = dmgx.NEP(path)
     md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
     tune = md.autotune()
