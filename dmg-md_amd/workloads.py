@@ -144,6 +144,18 @@ def liquid_water(n_molecules: int = 64000, seed: int = 1, dmin: float = 2.5):
     return dict(box=box, pos=soa(pos), types=types, mass=m, vel=maxwell_boltzmann(m, 300.0, seed + 1))
 
 
+def amorphous_carbon(n: int, seed: int = 1, dmin: float = 1.2):
+    """A periodic amorphous-carbon melt of n atoms at 2.0 g/cm^3, random
+    positions at least dmin apart, 5000 K velocities: the bulk of the domain
+    share below, as a small periodic sample."""
+    rng = np.random.default_rng(seed)
+    L = cube_for_density(n * MASS["C"], 2.0)
+    p = random_separated(n, L, dmin, rng)
+    m = np.full(n, MASS["C"])
+    return dict(box=Box.orthogonal(L, L, L), pos=soa(p), types=np.zeros(n, np.int32), mass=m,
+                vel=maxwell_boltzmann(m, 5000.0, seed + 1))
+
+
 def amorphous_carbon_share(n: int = 2048000, seed: int = 1, dmin: float = 1.2, shell: float = 5.0,
                            margin: float = 30.0):
     """Config: one GPU's share of an amorphous carbon melt: n atoms at

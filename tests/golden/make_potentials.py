@@ -32,8 +32,9 @@ SPECS = {
                           lambda: W.carbon_diamond_4096(seed=11), 101),
     "nep_cu.txt": (["Cu"], 5.0, 4.0, (6, 4), (8, 8), "4 2 0", 50, None,
                       lambda: W.copper_fcc(n=10, seed=12), 102),
+    # the model of the amorphous-carbon melt: its scaler is fit on a melt sample
     "nep_carbon_zbl.txt": (["C"], 4.2, 3.7, (4, 4), (8, 8), "4 2 1", 30, "zbl 1.0 2.0",
-                              lambda: W.carbon_diamond_4096(seed=13), 104),
+                              lambda: W.amorphous_carbon(8000, seed=13), 104),
     "nep_water.txt": (["O", "H"], 6.0, 4.0, (6, 6), (10, 10), "4 2 1", 40, None,
                          lambda: W.liquid_water(64000, seed=23), 103),
 }
@@ -155,8 +156,12 @@ def make_coverage(name, spec):
 
 
 if __name__ == "__main__":
+    # optional arguments: the file names to (re)write; default all
     os.makedirs(OUT, exist_ok=True)
+    only = set(sys.argv[1:])
     for n, s in SPECS.items():
-        make(n, s)
+        if not only or n in only:
+            make(n, s)
     for n, s in COVERAGE.items():
-        make_coverage(n, s)
+        if not only or n in only:
+            make_coverage(n, s)

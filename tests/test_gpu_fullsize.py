@@ -27,6 +27,7 @@ def cu256k(workloads):
 @pytest.fixture(scope="module")
 def oracle_cu256k(cu256k):
     s = cu256k
+    print("Cu 256k: FP64 oracle on every atom", flush=True)
     return OracleNEP(CU, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
 
 
@@ -106,6 +107,7 @@ def test_water_fullsize_against_oracle(dmgx, water192k):
     s = water192k
     N = len(s["types"])
     assert N == 192000
+    print("water 192k: FP64 oracle on every atom", flush=True)
     ref = OracleNEP(WATER, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"])
     out = gpu_compute(dmgx, dmgx.NEP(WATER), s["box"], s["types"], s["pos"])
     err = errors(out, ref)
@@ -168,6 +170,7 @@ CARBON_ZBL = os.path.join(POTENTIALS, "nep_carbon_zbl.txt")
 
 @pytest.fixture(scope="module")
 def carbon_share(workloads):
+    print("a-C share: generating 2,048,000 atoms and the ghost shell", flush=True)
     return workloads.amorphous_carbon_share(2048000, seed=1)
 
 
@@ -218,20 +221,22 @@ def test_carbon_share_against_oracle(dmgx, carbon_share):
     owned atoms, dependency centres = owned + ghosts), FP32 and BF16 ANN:
     owned forces, energies and virials against the oracle on three crops
     with the same centre definitions.  Random positions 1.2 A apart put
-    ZBL-range pairs everywhere; the typical atom's FP32 force is within
-    1.4e-6 of its magnitude, a few atoms at 1.2-1.4 A pairs (large opposing
-    terms) within 1.3e-4, whether the cutoff cosine comes from the hardware
-    or libm.  Bars: the largest error within 1e-4 of the crop's largest
-    force, the median relative error within 1e-5.  BF16 ANN operands: 4.3%
-    of the largest force here (the melt's descriptors lie far outside the
-    diamond sample the model's scaler was fit on, so bfloat16 rounding of
-    the ANN inputs weighs more than on the lattices), bounded at 5%."""
+    ZBL-range pairs everywhere.  FP32: every crop atom within 1e-4 eV/A,
+    2e-4 eV per virial component and 2e-5 eV per atom energy (the model's
+    scaler is fit on a melt sample; a host FP32 emulation of this melt
+    puts the largest force error near 1.6e-5).  BF16 ANN operands: the
+    error is that of bfloat16 rounding of the ANN inputs, reported and
+    bounded relative to the largest force."""
     s = carbon_share
     n = s["owned"]
     assert n == 2048000
-    crops = [_crop_oracle(s, c) for c in _crop_centres(s)]
+    crops = []
+    for c in _crop_centres(s):
+        print("a-C share: oracle on the crop around", tuple(round(x, 1) for x in c), flush=True)
+        crops.append(_crop_oracle(s, c))
     nep = dmgx.NEP(CARBON_ZBL)
     for bf16 in (False, True):
+        print("a-C share: device evaluation", "bf16" if bf16 else "fp32", flush=True)
         pe, f, v = _share_compute(dmgx, nep, s, bf16=bf16)
         pe, f, v = pe.cpu().numpy(), f.cpu().numpy(), v.cpu().numpy()
         for core_i, ref in crops:
@@ -249,11 +254,9 @@ def test_carbon_share_against_oracle(dmgx, carbon_share):
                 assert err["force"] <= 5e-2 * max(fmax, 1.0), err
                 assert err["energy_atom_max"] <= 2e-2 * max(np.abs(ref["pe"][:m]).max(), 1.0), err
             else:
-                rel = ferr / np.maximum(np.abs(ref["force"][:, :m]).max(0), 1.0)
-                assert err["force"] <= max(1e-4, 1e-4 * fmax), err
-                assert np.median(rel) <= 1e-5, np.median(rel)
+                assert err["force"] <= 1e-4, err
                 assert err["energy_atom_max"] <= 2e-5, err
-                assert err["virial"] <= max(2e-4, 1e-4 * np.abs(ref["virial"][:, :m]).max()), err
+                assert err["virial"] <= 2e-4, err
 
 
 def _share_nve(dmgx, s, dt_fs, steps, bf16):

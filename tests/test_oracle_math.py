@@ -190,6 +190,23 @@ def test_seeded_models_parse_and_are_finite():
         assert nep.version == 4
 
 
+def test_seeded_models_are_scaled_for_their_configuration(workloads):
+    """Each configuration's model has q_scaler = 1/range of its descriptor
+    over a sample of that configuration, so on an independent sample of the
+    same configuration every scaled component spans a range of order one.  The amorphous-carbon
+    model fit on diamond instead saw 4-/5-body invariants up to ~2,400x their
+    range on the melt and dE/dq up to 6e6; FP32 rounding of the angular
+    moments, amplified by those gradients, then moved forces by ~1e-3 eV/A."""
+    cases = [("nep_carbon.txt", workloads.carbon_diamond_4096(seed=3)),
+             ("nep_cu.txt", workloads.copper_fcc(n=6, seed=3)),
+             ("nep_carbon_zbl.txt", workloads.amorphous_carbon(3000, seed=3))]
+    for name, s in cases:
+        o = OracleNEP(os.path.join(POTENTIALS, name)).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"],
+                                                              want_descriptor=True)
+        spread = o["q"].max(0) - o["q"].min(0)
+        assert spread.max() < 3.0, (name, spread)
+
+
 def test_per_type_cutoffs_use_the_pair_mean():
     """`cutoff rcR_C rcA_C rcR_H rcA_H ...`: a C-H pair interacts within the
     mean of the two types' radii (nep.cu find_neighbor_list_large_box: pair
