@@ -87,6 +87,7 @@ _SIGNATURES = {
     "dmgx_nep_get_info": (I, [VP, ctypes.POINTER(NepInfo)]),
     "dmgx_nep_reserve": (I, [VP, I]),
     "dmgx_nep_compute": (I, [VP, ctypes.POINTER(Box), I, I, I, I, I, VP, VP, VP, VP, VP, VP, U, VP]),
+    "dmgx_nep_compute_noalloc": (I, [VP, ctypes.POINTER(Box), I, I, I, I, I, VP, VP, VP, VP, VP, VP, U, VP]),
     "dmgx_nep_notify_box_changed": (I, [VP]),
     "dmgx_nep_neighbor_stats": (I, [VP, PI, PI, PI, PI]),
     "dmgx_nep_neighbor_totals": (I, [VP] + [ctypes.POINTER(ctypes.c_longlong)] * 3 + [PI]),

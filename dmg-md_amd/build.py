@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdmgx.so")
-SOURCES = ["kernels.hip", "neighbor.hip", "nep_model.cpp", "ensemble.cpp", "engine.cpp", "serve.cpp", "frontend.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "neighbor.hip", "nep_model.cpp", "ensemble.cpp", "engine.cpp", "serve.cpp", "frontend.cpp", "noalloc.hip", "capi.cpp"]
 ARCH = os.environ.get("DMGX_OFFLOAD_ARCH", "gfx950")
 
 

@@ -14,6 +14,7 @@
 #include "engine.hpp"
 #include "ensemble.hpp"
 #include "frontend.hpp"
+#include "noalloc.hpp"
 #include "serve.hpp"
 
 struct dmgx_nep {
@@ -138,6 +139,21 @@ int dmgx_nep_compute(dmgx_nep* nep, const dmgx_box* box, int N, int n1, int n2, 
     if (!nep) throw std::invalid_argument("null handle");
     nep->engine.compute(to_box(box), N, n1, n2, nd1, nd2, type, position, global_id, potential, force, virial,
                         flags, (hipStream_t)stream);
+  });
+}
+
+int dmgx_nep_compute_noalloc(dmgx_nep* nep, const dmgx_box* box, int N, int n1, int n2, int nd1, int nd2,
+                             const int* type, const double* position, const uint64_t* global_id,
+                             double* potential, double* force, double* virial, unsigned flags, void* stream)
+{
+  return guarded([&] {
+    if (!nep) throw std::invalid_argument("null handle");
+    void* ws = nullptr;
+    size_t bytes = 0;
+    const hipError_t e = dmgx::fixed_workspace(&ws, &bytes);
+    if (e != hipSuccess) throw std::runtime_error(std::string("fixed workspace: ") + hipGetErrorString(e));
+    nep->engine.compute_in(ws, bytes, to_box(box), N, n1, n2, nd1, nd2, type, position, global_id, potential, force,
+                           virial, flags, (hipStream_t)stream);
   });
 }
 

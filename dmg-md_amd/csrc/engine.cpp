@@ -749,6 +749,121 @@ void NepEngine::set_tiles(int filter_lanes, int gather_lanes)
   tune_.tuned = true;
 }
 
+namespace {
+
+// Hands out 256-byte aligned pieces of one buffer; with a null base it only
+// counts the bytes the pieces need.
+struct Carver {
+  char* base;
+  size_t used = 0;
+  template <class T>
+  T* take(size_t n)
+  {
+    used = (used + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+    used += std::max<size_t>(n, 1) * sizeof(T);
+    return p;
+  }
+};
+
+// The workspace of compute_in for cap slots, rows of mv entries and ncell
+// cells, carved from `base` (or only measured when base is null).
+DevWork carve_work(char* base, int cap, int mv, long long ncell, int T, int kRp, size_t* bytes)
+{
+  Carver c{base};
+  DevWork w{};
+  const size_t n = (size_t)cap, rows = n * (size_t)mv;
+  w.flags = c.take<int>(kNumFlags);
+  w.cell_count = c.take<int>((size_t)ncell);
+  w.cell_start = c.take<int>((size_t)ncell + 1);
+  w.cell_tiles = c.take<int>(2 * ((size_t)ncell / 4096 + 2) + 8);
+  w.xq = c.take<double4>(n);
+  w.xs = c.take<double4>(n);
+  w.cell_of = c.take<int>(n);
+  w.cell_rank = c.take<int>(n);
+  w.cell_atoms = c.take<int>(n);
+  w.vn = c.take<int>(n);
+  w.nr = c.take<int>(n);
+  w.na = c.take<int>(n);
+  w.x0 = c.take<double>(3 * n);
+  w.bR = c.take<float>(n * (size_t)T * (size_t)kRp);
+  w.vl = c.take<int>(rows);
+  w.vrev = c.take<int>(rows);
+  w.eq = c.take<int>(rows);
+  w.vsa = c.take<int>(rows);
+  w.edge = c.take<float4>(rows);
+  w.f12 = c.take<float4>(rows);
+  w.cap = cap;
+  w.MV = w.MR = w.MA = mv;
+  w.ncell_cap = (int)ncell;
+  *bytes = c.used;
+  return w;
+}
+
+}  // namespace
+
+void NepEngine::compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int n1, int n2, int nd1, int nd2,
+                           const int* type, const double* pos, const uint64_t* gid, double* pe, double* force,
+                           double* virial, unsigned flags, hipStream_t s)
+{
+  if (N < 0 || n1 < 0 || n2 < n1 || n2 > N || nd1 < 0 || nd2 < nd1 || nd2 > N)
+    throw std::invalid_argument("centre range is outside [0, N)");
+  if (n1 < nd1 || n2 > nd2) throw std::invalid_argument("force centres must be dependency centres");
+  if (N > 0 && (!type || !pos || !pe || !force || !virial)) throw std::invalid_argument("null array");
+  const bool by_gid = (flags & 8u) != 0;
+  if (by_gid && !gid) throw std::invalid_argument("DMGX_SORT_BY_GID needs global_id");
+  if (!ws || (reinterpret_cast<uintptr_t>(ws) & 255)) throw std::invalid_argument("workspace not 256-byte aligned");
+  if (N == 0 || nd2 == nd1) return;
+  int reps[3];
+  if (small_box(box, reps))
+    throw std::invalid_argument(
+        "a periodic box thinner than 2.5 (rc + skin) is evaluated on a replicated supercell, which the fixed "
+        "workspace does not hold; use dmgx_nep_compute");
+  const DevBox b = make_devbox(box);
+  const long long ncell = (long long)b.nc[0] * b.nc[1] * b.nc[2];
+  if (ncell > (1LL << 28)) throw std::invalid_argument("cell grid too large");
+  // rows as the allocating path sizes them before any overflow has grown them
+  const int mv_need = std::max(8, (int)std::ceil(model_.MN_R * std::pow((model_.rcR_max + kSkin) / model_.rcR_max, 3)));
+  const int mv = padded_rows(mv_need, tune_.row_layout);
+  const int kRp = (model_.kR + 3) & ~3;
+  size_t need = 0;
+  carve_work(nullptr, N, mv, ncell, model_.num_types, kRp, &need);
+  if (need > ws_bytes) {
+    size_t one = 0, none = 0;
+    carve_work(nullptr, 1, mv, ncell, model_.num_types, kRp, &one);
+    carve_work(nullptr, 0, mv, ncell, model_.num_types, kRp, &none);
+    const size_t per_atom = std::max<size_t>(one - none, 1);
+    throw std::invalid_argument("the system needs " + std::to_string(need) + " bytes of workspace (" +
+                                std::to_string(N) + " slots, rows of " + std::to_string(mv) + ", " +
+                                std::to_string(ncell) + " cells); the fixed workspace holds " +
+                                std::to_string(ws_bytes) + " (about " +
+                                std::to_string(ws_bytes > none ? (ws_bytes - none) / per_atom : 0) +
+                                " slots of this model in this box)");
+  }
+  DevWork w = carve_work(static_cast<char*>(ws), N, mv, ncell, model_.num_types, kRp, &need);
+  // the workspace keeps the previous call's contents: clear the status words
+  // and the cell counts (the cell scan expects them at zero) with a kernel
+  check(launch_clear_ints(w.flags, kNumFlags, s), "clear status words");
+  check(launch_clear_ints(w.cell_count, ncell, s), "clear cell counts");
+  check(launch_pack(b, N, model_.num_types, pos, type, true, w, s), "pack");
+  const double rv = model_.rcR_max + kSkin;
+  check(launch_rebuild(b, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w, s), "rebuild");
+  check(launch_filter(b, dm_, nd1, nd2, w, s, tune_.filter_lanes), "filter");
+  const bool acc = (flags & 1u) != 0;
+  const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
+  check(launch_descriptor(dmd_, N, nd1, nd2, type, pe, acc, ann, w, s), "descriptor");
+  check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w, s, tune_.gather_lanes), "gather");
+  check(hipMemcpyAsync(hflags_, w.flags, kNumFlags * sizeof(int), hipMemcpyDeviceToHost, s), "read status words");
+  check(hipStreamSynchronize(s), "synchronise");
+  if (hflags_[kFlagBadType]) throw std::invalid_argument("an atom type lies outside [0, num_types)");
+  if (hflags_[kFlagOutOfBox])
+    throw std::invalid_argument("a position lies outside the box in a non-periodic direction");
+  if (hflags_[kFlagVerletOverflow] > mv)
+    throw std::runtime_error("Verlet row overflow: " + std::to_string(hflags_[kFlagVerletOverflow]) +
+                             " neighbours within rc + skin, the fixed workspace's rows hold " + std::to_string(mv) +
+                             " (MN_radial of the model); outputs are invalid");
+}
+
 void NepEngine::stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds)
 {
   const int n = last_c2_ - last_c1_;

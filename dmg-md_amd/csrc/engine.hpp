@@ -146,6 +146,19 @@ class NepEngine {
   // replicated small box) the next evaluation rebuilds.
   void notify_box_scaled(double scale, hipStream_t stream);
   bool rescaled_since_build() const { return scale_since_build_ != 1.0; }
+  // One evaluation whose per-atom workspace (packed positions, cell grid,
+  // Verlet rows, edge rows, adjoints, partial forces, status words) is carved
+  // out of `ws` (ws_bytes long, 256-byte aligned, device memory) instead of
+  // the handle's own allocations: nothing is allocated or freed.  The Verlet
+  // list is built afresh (the workspace keeps no list between calls), the
+  // handle's own list and workspace are left untouched, and the stream is
+  // synchronised to read the status words.  Throws std::invalid_argument if
+  // the system needs more than ws_bytes or is a replicated small box, and
+  // std::runtime_error ("overflow") if a Verlet row exceeds the capacity the
+  // model's MN_radial implies.
+  void compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int n1, int n2, int nd1, int nd2,
+                  const int* type, const double* pos, const uint64_t* gid, double* pe, double* force,
+                  double* virial, unsigned flags, hipStream_t stream);
   void stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds);
   void totals(long long* sum_radial, long long* sum_angular, long long* sum_verlet, int* centres);
   void copy_verlet_rows(int c1, int c2, int max_nb, int* counts, int* rows);

@@ -123,6 +123,8 @@ void upload_angular_constants(const AngularConstants& a);
 // it returns at once unless flags[kFlagRebuild] is set.  The request is
 // closed (and counted) by the filter kernel that follows.
 hipError_t launch_request_rebuild(DevWork& w, hipStream_t s);
+// p[0..n) = 0 with a kernel (no memset node, so callers may capture it)
+hipError_t launch_clear_ints(int* p, long long n, hipStream_t s);
 // half_skin: the displacement since the last build that requests a rebuild
 // (default half the skin)
 hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const int* type, bool forced, DevWork& w,

@@ -25,6 +25,12 @@ constexpr int kScanOneBlock = 262144;  // larger cell grids are scanned by many 
 
 __global__ void k_set_flag(int* __restrict__ flag, int value) { *flag = value; }
 
+__global__ void k_clear_ints(int* __restrict__ p, long long n)
+{
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    p[k] = 0;
+}
+
 // Per-step packing of positions and types into one 32-byte record per atom
 // (x, y, z, type bits) so that every neighbour gather is a single 32-byte
 // access; also validates the types and, unless a rebuild is forced, tests
@@ -538,6 +544,12 @@ int build_groups(int mv, bool keyed)
 hipError_t launch_request_rebuild(DevWork& w, hipStream_t s)
 {
   k_set_flag<<<1, 1, 0, s>>>(w.flags + kFlagRebuild, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_ints(int* p, long long n, hipStream_t s)
+{
+  if (n > 0) k_clear_ints<<<(int)std::min<long long>((n + kBlock - 1) / kBlock, 1024), kBlock, 0, s>>>(p, n);
   return hipGetLastError();
 }
 

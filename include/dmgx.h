@@ -97,6 +97,28 @@ int dmgx_nep_compute(dmgx_nep* nep, const dmgx_box* box, int N, int n1, int n2, 
                      double* potential, double* force, double* virial, unsigned flags,
                      void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * evaluation as dmgx_nep_compute (same arguments, flags and results), with
+ * every per-atom workspace carved from a fixed 128 MiB device array that the
+ * library holds on each GPU until the process ends.  It calls no hipMalloc,
+ * hipFree or allocating library routine.  Limits and differences:
+ *   - the system must fit the fixed workspace (about 17,000 slots of a model
+ *     with rows of 152 entries); larger systems get DMGX_EINVAL with the
+ *     sizes in the message;
+ *   - Verlet rows hold the capacity MN_radial implies (no growth): a longer
+ *     row gives DMGX_EOVERFLOW and invalid outputs;
+ *   - periodic boxes thinner than 2.5 (rc + skin) give DMGX_EINVAL;
+ *   - the Verlet list is rebuilt on every call, and the handle's own list
+ *     and workspace are left as they were;
+ *   - the stream is synchronised (the status words are read back);
+ *   - every call on one GPU shares that GPU's array: calls that overlap on
+ *     one GPU (other threads or streams) race. */
+int dmgx_nep_compute_noalloc(dmgx_nep* nep, const dmgx_box* box, int N, int n1, int n2, int nd1, int nd2,
+                             const int* type, const double* position, const uint64_t* global_id,
+                             double* potential, double* force, double* virial, unsigned flags,
+                             void* stream);
+
 /* NEP::notify_box_changed (nep.cuh:170, neighbor.cuh:224): the next compute
  * rebuilds the Verlet list. */
 int dmgx_nep_notify_box_changed(dmgx_nep* nep);
