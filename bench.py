@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s (1 GPU, NEP4 Cu 256k); served frames/s + p99 latency at fixed force err"
 MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_cu.txt")
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (vector and MFMA), AMD public figure
+PEAK_FP32_SCALAR_TFLOPS = 78.6  # the same without packed FP32 (v_pk_*_f32), which this build keeps off
 PEAK_HBM_GBS = 8000.0
 
 
@@ -105,61 +106,46 @@ def cpu_baseline(budget_s=10.0):
 CARBON_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_carbon.txt")
 
 
-def _carbon_request(rng, n_min=64, n_max=20000):
-    """A carbon diamond supercell of log-uniform size in [n_min, n_max]
-    atoms (cubic n^3 cells), 30% vacancies, 0.05 A jitter."""
-    a = 3.567
-    frac = [[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
-            [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]]
-    import numpy as np
-    target = float(np.exp(rng.uniform(np.log(n_min / 0.7), np.log(n_max / 0.7))))
-    n = max(2, int(round((target / 8) ** (1 / 3))))
-    cells = np.array([[i, j, k] for i in range(n) for j in range(n) for k in range(n)])
-    pos = ((cells[:, None, :] + np.array(frac)[None]).reshape(-1, 3) * a)
-    keep = rng.random(len(pos)) >= 0.3
-    pos = pos[keep].T + rng.normal(0, 0.05, (3, int(keep.sum())))
-    N = pos.shape[1]
-    vel = rng.normal(0, 0.01, (3, N))
-    vel -= vel.mean(1, keepdims=True)
-    return np.diag([a * n] * 3), pos, vel
-
-
-def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
-    """Served frames/s and latency percentiles of the continuous-batching
-    server on a bounded sample of the serving workload (carbon requests of
-    64..20000 atoms with 30% vacancies, half single points and half 50-step
-    NVE, BF16 ANN).  Capacity: the whole stream submitted at once (closed
-    loop, like the plain loop it is compared with).  Latency: the same
-    stream again with Poisson arrivals at rate_hz.  Beside them a plain
-    sequential FP32 loop (one request at a time through NEP.compute /
-    MD.run) on the first requests of the stream, and the BF16 force error
-    against FP32."""
+def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
+    """The serving configuration: `requests` carbon and Cu requests (each
+    with its own model: the 4096-atom carbon and the 256k Cu configurations'
+    NEP4 models, one server), 64..20000 sites log-uniform, 30% vacancies on a
+    random subset, half single points (energy, forces, virial) and half
+    50-step NVE segments.  The served mode is the FP32 ANN, the force error
+    the plain path has; the bfloat16-operand ANN is measured beside it with
+    its force error against FP32.
+      capacity: the whole stream submitted at once by 8 client threads
+        (closed loop, like the plain loop), frames/s from first submit to
+        last completion;
+      latency: the same stream with Poisson arrivals at rate_hz, p50/p99
+        submit-to-result;
+      plain FP32: the same requests one at a time through NEP.compute /
+        MD.run (FP32 ANN), after one untimed request per model."""
     import numpy as np
     import torch
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for k in range(requests):
-        h, p, v = _carbon_request(rng)
-        reqs.append((h, p, v, 50 if k % 2 else 0))
-    gaps = rng.exponential(1.0 / rate_hz, requests)
+    from concurrent.futures import ThreadPoolExecutor
+    W = importlib.import_module("dmg-md_amd.workloads")
+    t_gen = time.perf_counter()
+    stream = W.serving_stream(requests, seed)
+    print(f"bench: serving stream of {requests} requests generated in {time.perf_counter() - t_gen:.1f} s",
+          file=sys.stderr, flush=True)
+    paths = [CARBON_MODEL, MODEL]
+    gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate_hz, requests)
 
     def submit(srv, k):
-        h, p, v, stp = reqs[k]
-        n = p.shape[1]
-        return srv.submit(dmgx.Box(h), np.zeros(n, np.int32), np.full(n, 12.011), p, v, steps=stp, dt_fs=0.5)
+        r, model, steps = stream[k]
+        return srv.submit(r["box"], r["types"], r["mass"], r["pos"], r["vel"], steps=steps, dt_fs=r["dt_fs"],
+                          model=model)
 
-    def burst(count, nep, clients=8):
-        # `clients` threads submit the stream at once (the submit call
-        # prepares a request's image shell and upload on the calling thread)
-        from concurrent.futures import ThreadPoolExecutor
-        srv = dmgx.Server(nep, max_atoms=6_000_000, ann_bf16=True)
+    def burst(neps, bf16, count=requests, clients=8):
+        srv = dmgx.Server(neps, max_atoms=20_000_000, ann_bf16=bf16)
         with ThreadPoolExecutor(clients) as pool:
             ids = list(pool.map(lambda k: submit(srv, k), range(count)))
         srv.drain()
         return srv, ids, srv.stats()
 
-    def poisson(nep):
-        srv = dmgx.Server(nep, max_atoms=6_000_000, ann_bf16=True)
+    def poisson(neps, bf16):
+        srv = dmgx.Server(neps, max_atoms=20_000_000, ann_bf16=bf16)
         t_next = time.perf_counter()
         for k, g in enumerate(gaps):
             t_next += g
@@ -169,68 +155,243 @@ def serve_bench(dmgx, requests=256, rate_hz=2000.0, plain_requests=24, seed=7):
         srv.drain()
         return srv.stats()
 
-    # first launches of the serving kernels; the server's autotuner picks the
-    # tile shapes on the first batch of >= 64k atoms, here in this warm-up
-    nep_srv = dmgx.NEP(CARBON_MODEL)
-    burst(min(requests, 48), nep_srv)
-    tune = nep_srv.tuning()
-    srv, ids, st = burst(requests, nep_srv)
-    lat = poisson(nep_srv)
-    # BF16 force error against FP32 on single points of the stream
-    err = 0.0
-    nep32 = dmgx.NEP(CARBON_MODEL)
-    for k in range(0, min(requests, 16), 2):
-        h, p, v, _ = reqs[k]
-        n = p.shape[1]
-        got = srv.poll(ids[k])
-        t = torch.zeros(n, dtype=torch.int32, device="cuda")
-        x = torch.tensor(p.reshape(-1), dtype=torch.float64, device="cuda")
-        pe = torch.zeros(n, dtype=torch.float64, device="cuda")
-        f = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
-        vir = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
-        nep32.compute(dmgx.Box(h), t, x, pe, f, vir)
-        torch.cuda.synchronize()
-        err = max(err, float(np.abs(got["force"] - f.cpu().numpy().reshape(3, n)).max()))
-    # the plain path: one request at a time, FP32, after one untimed request
-    # (first launches of its kernels)
-    def plain_one(h, p, v, stp):
-        n = p.shape[1]
-        if stp == 0:
-            t = torch.zeros(n, dtype=torch.int32, device="cuda")
-            x = torch.tensor(p.reshape(-1), dtype=torch.float64, device="cuda")
+    neps = [dmgx.NEP(p) for p in paths]
+    # first launches of the serving kernels; each model's autotuner picks
+    # its tile shapes on its first batch of >= 64k atoms, here in this warm-up
+    burst(neps, False, count=min(requests, 256))
+    srv, ids, st = burst(neps, False)
+    print(f"bench: served FP32 capacity {st['frames_per_s']:.0f} frames/s", file=sys.stderr, flush=True)
+    lat = poisson(neps, False)
+    print(f"bench: served FP32 Poisson p50/p99 {lat['p50_ms']:.2f}/{lat['p99_ms']:.2f} ms", file=sys.stderr,
+          flush=True)
+    srv_bf, ids_bf, st_bf = burst(neps, True)
+    lat_bf = poisson(neps, True)
+    tunes = [n.tuning() for n in neps]
+    # the plain path: one request at a time, FP32; single points go through
+    # NEP.compute on device arrays, NVE segments through MD.run
+    plain = [dmgx.NEP(p) for p in paths]
+    errs = {"fp32": 0.0, "bf16": 0.0}
+
+    def plain_one(k, check=False):
+        r, model, steps = stream[k]
+        n = len(r["types"])
+        if steps == 0:
+            t = torch.tensor(r["types"], dtype=torch.int32, device="cuda")
+            x = torch.tensor(r["pos"].reshape(-1), dtype=torch.float64, device="cuda")
             pe = torch.zeros(n, dtype=torch.float64, device="cuda")
             f = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
             vir = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
-            nep32.compute(dmgx.Box(h), t, x, pe, f, vir)
+            plain[model].compute(r["box"], t, x, pe, f, vir)
             torch.cuda.synchronize()
+            if check:
+                fh = f.cpu().numpy().reshape(3, n)
+                errs["fp32"] = max(errs["fp32"], float(np.abs(srv.poll(ids[k])["force"] - fh).max()))
+                errs["bf16"] = max(errs["bf16"], float(np.abs(srv_bf.poll(ids_bf[k])["force"] - fh).max()))
             return 1
-        md = dmgx.MD(nep32, dmgx.Box(h), np.zeros(n, np.int32), np.full(n, 12.011), p, v)
-        md.run(stp, 0.5, "nve")
-        return stp
+        md = dmgx.MD(plain[model], r["box"], r["types"], r["mass"], r["pos"], r["vel"])
+        md.run(steps, r["dt_fs"], "nve")
+        return steps
 
-    plain_one(*reqs[1])
+    for k in range(4):  # first launches of both models, single point and MD
+        plain_one(k)
     frames = 0
     torch.cuda.synchronize()
     tp = time.perf_counter()
-    for h, p, v, stp in reqs[:plain_requests]:
-        frames += plain_one(h, p, v, stp)
-    plain = frames / (time.perf_counter() - tp)
-    return {"workload": f"{requests} carbon requests, 64..20000 atoms (log-uniform, 30% vacancies), "
-                        "half single points / half 50-step NVE at 0.5 fs, carbon NEP4 model, bf16 ANN",
+    for k in range(requests):
+        frames += plain_one(k)
+    plain_fps = frames / (time.perf_counter() - tp)
+    print(f"bench: plain FP32 {plain_fps:.0f} frames/s", file=sys.stderr, flush=True)
+    for k in range(0, min(requests, 512), 2):  # single points: served against plain
+        plain_one(k, check=True)
+    n_carbon = sum(1 for _, m, _ in stream if m == 0)
+    return {"workload": f"{requests} requests ({n_carbon} carbon diamond, {requests - n_carbon} Cu fcc, each with "
+                        "its own NEP4 model), 64..20000 sites log-uniform, 30% vacancies on half of them, half "
+                        "single points (energy, forces, virial) and half 50-step NVE (carbon 1 fs, Cu 2 fs)",
+            "served_ann": "fp32 (matrix cores): the plain path's arithmetic, so equal force error",
             "frames_per_s": st["frames_per_s"], "atom_frames_per_s": st["atom_frames"] / max(st["elapsed_s"], 1e-9),
             "capacity_run": "all requests submitted at once by 8 client threads",
             "requests": st["completed"], "relayouts": st["relayouts"], "iterations": st["iterations"],
             "max_batch_slots": st["max_slots"],
+            "hbm_budget_bytes": st["hbm_budget"], "max_bytes_in_flight": st["max_bytes_in_flight"],
             "worker_s": {"relayout": round(st["relayout_s"], 4), "iterate": round(st["iterate_s"], 4),
                          "retire": round(st["retire_s"], 4)},
             "latency_run": f"Poisson arrivals {rate_hz:.0f}/s", "p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"],
             "latency_run_frames_per_s": lat["frames_per_s"],
-            "bf16_force_err_max_eV_per_A": err,
-            "tile_shapes": {"filter_lanes_per_centre": tune["filter_lanes"],
-                            "gather_lanes_per_atom": tune["gather_lanes"], "autotuned": tune["tuned"],
-                            "row_capacity": tune["row_capacity"], "row_layout": tune["row_layout"]},
-            "plain_fp32_frames_per_s": plain, "plain_sample_requests": plain_requests,
-            "speedup_vs_plain": st["frames_per_s"] / plain if plain > 0 else None}
+            "plain_fp32_frames_per_s": plain_fps, "plain_requests": requests,
+            "speedup_vs_plain": st["frames_per_s"] / plain_fps if plain_fps > 0 else None,
+            "target_speedup": 8.0,
+            "served_vs_plain_force_err_max_eV_per_A": errs["fp32"],
+            "force_err_note": "served single points against the plain FP32 path on the stream's first 256 single "
+                              "points; the plain path and the served FP32 mode are each held to the FP64 oracle "
+                              "within 1e-4 eV/A by the GPU tests",
+            "bf16": {"frames_per_s": st_bf["frames_per_s"], "p50_ms": lat_bf["p50_ms"], "p99_ms": lat_bf["p99_ms"],
+                     "speedup_vs_plain": st_bf["frames_per_s"] / plain_fps if plain_fps > 0 else None,
+                     "force_err_vs_plain_fp32_max_eV_per_A": errs["bf16"]},
+            "tile_shapes": [{"model": os.path.basename(p), "filter_lanes_per_centre": t["filter_lanes"],
+                             "gather_lanes_per_atom": t["gather_lanes"], "autotuned": t["tuned"],
+                             "row_capacity": t["row_capacity"], "row_layout": t["row_layout"]}
+                            for p, t in zip(paths, tunes)]}
+
+
+def counter_flops(kernel: str, atoms: int):
+    """FP32 VALU and MFMA FLOP per atom of a kernel from the committed
+    rocprofv3 counter summary, when it covers this workload; None otherwise."""
+    try:
+        d = json.load(open(os.path.join(ROOT, "profiles", "k_descriptor_cu256k_counters.json")))
+    except (OSError, ValueError):
+        return None
+    if d.get("atoms") != atoms:
+        return None
+    return d["kernels"].get(kernel)
+
+
+def small_bench(dmgx, steps=100, warmup=20):
+    """The 4,096-atom carbon diamond configuration (NEP4 carbon model): NVE
+    at dt 1 fs, `steps` timed steps after `warmup`."""
+    import torch
+    W = importlib.import_module("dmg-md_amd.workloads")
+    s = W.carbon_diamond_4096(seed=1)
+    N = len(s["types"])
+    md = dmgx.MD(dmgx.NEP(CARBON_MODEL), s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+    md.run(warmup, 1.0, "nve")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    md.run(steps, 1.0, "nve")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": "carbon diamond 8^3 cells (4096 atoms, a=3.567 A, 0.05 A jitter), NEP4 n_max 4/4 basis 8/8 "
+                        "l_max 4/2/1 rc 4.2/3.7 A, 30 neurons; NVE 300 K, dt 1 fs", "atoms": N, "steps": steps,
+            "atom_steps_per_s": N * steps / el, "ms_per_step": el / steps * 1e3}
+
+
+CARBON_ZBL_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_carbon_zbl.txt")
+
+
+def share_bench(dmgx, steps=200):
+    """One GPU's share of an amorphous-carbon melt: 2,048,000 owned atoms at
+    2.0 g/cm^3 plus a static 5 A ghost shell, NEP4-ZBL carbon model, NVE at
+    dt 0.5 fs for the owned atoms (forces through the domain call: owned
+    atoms as force centres, owned + ghosts as dependency centres; half kicks
+    and drift as FP64 device array updates), FP32 and BF16 ANN operands."""
+    import numpy as np
+    import torch
+    W = importlib.import_module("dmg-md_amd.workloads")
+    t_gen = time.perf_counter()
+    s = W.amorphous_carbon_share(2048000, seed=1)
+    print(f"bench: a-C share generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    N, n = len(s["types"]), s["owned"]
+    dt = 0.5 / 10.18051
+    t = torch.tensor(s["types"], dtype=torch.int32, device="cuda")
+    m = torch.tensor(s["mass"][:n], dtype=torch.float64, device="cuda")
+    out = {"workload": f"amorphous carbon melt share: {n} owned atoms (2.0 g/cm^3, 1.2 A minimum separation, "
+                       f"5000 K) + {N - n} static ghost-shell atoms (5 A), NEP4-ZBL carbon model (ZBL 1.0/2.0 A), "
+                       "NVE dt 0.5 fs of the owned atoms", "owned_atoms": n, "slots": N, "steps": steps}
+    for mode in ("fp32", "bf16"):
+        nep = dmgx.NEP(CARBON_ZBL_MODEL)
+        x = torch.tensor(np.ascontiguousarray(s["pos"]).reshape(-1), dtype=torch.float64, device="cuda")
+        xv = x.view(3, N)
+        v = torch.tensor(np.ascontiguousarray(s["vel"]), dtype=torch.float64, device="cuda").view(3, n)
+        pe = torch.zeros(N, dtype=torch.float64, device="cuda")
+        f = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+        vir = torch.zeros(9 * N, dtype=torch.float64, device="cuda")
+        fv = f.view(3, N)
+
+        def force():
+            nep.compute_domain(s["box"], N, t, x, pe, f, vir, n1=0, n2=n, nd1=0, nd2=N, ann_bf16=mode == "bf16")
+
+        def step():
+            v.add_(fv[:, :n] / m, alpha=0.5 * dt)
+            xv[:, :n].add_(v, alpha=dt)
+            force()
+            v.add_(fv[:, :n] / m, alpha=0.5 * dt)
+
+        force()
+        for _ in range(5):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[mode] = {"atom_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3,
+                     "verlet_rebuilds": nep.neighbor_stats()["rebuilds"]}
+        del nep
+    return out
+
+
+def cpu_baseline_parallel(procs=8, budget_s=10.0):
+    """The same CPU sample in `procs` independent processes at once (one per
+    core): the host's aggregate FP64 throughput."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_worker, [budget_s] * procs)
+    return {"value": sum(r["value"] for r in res), "unit": "atom-steps/s", "cores": procs, "kind": res[0]["kind"],
+            "sample": f"{procs} processes, each running the 1-core sample independently; aggregate throughput"}
+
+
+def _cpu_worker(budget_s):
+    return cpu_baseline(budget_s)
+
+
+def host_cpu():
+    """CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "cores_available": avail, "nproc": os.cpu_count()}
+
+
+def context_bench(dmgx):
+    """The reference's only published timing job, replayed through the
+    run.in front end (dmgx_run_directory): 5,760-atom diamond carbon (20x6x6
+    cells), the reference's nep_C.txt (NEP4, rc 7/4 A, n_max 12/8, basis
+    16/12, 100 neurons), npt_ber 300 K / 0 GPa (Tdamp 40 steps, B 425 GPa,
+    tau_p 1000 steps), dt 0.1 fs, thermo every 100 steps, a restart at the
+    end.  The steady-state step time is the difference of a 2,000-step and
+    a 1,000-step job's wall times over 1,000 steps (the published figure is
+    the mean over steps 1,000-2,000 of a 2,000-step job)."""
+    import shutil
+    import tempfile
+    W = importlib.import_module("dmg-md_amd.workloads")
+    cli = importlib.import_module("dmg-md_amd.cli")
+    model = W.diamond_npt_model_xyz((20, 6, 6), 301)
+    pot = os.path.join(ROOT, "tests", "golden", "baseline", "inputs", "potentials", "nep_C.txt")
+    walls = {}
+    for steps in (1000, 2000):
+        d = tempfile.mkdtemp(prefix="dmgx_ctx_")
+        try:
+            open(os.path.join(d, "model.xyz"), "w").write(model)
+            shutil.copy(pot, os.path.join(d, "nep.txt"))
+            open(os.path.join(d, "run.in"), "w").write(
+                "potential nep.txt\ntime_step 0.1\nensemble npt_ber 300 300 40 0 425 1000\n"
+                f"dump_thermo 100\ndump_restart {steps}\nrun {steps}\n")
+            t0 = time.perf_counter()
+            cli.run_directory(d)
+            walls[steps] = time.perf_counter() - t0
+            rows = open(os.path.join(d, "thermo.out")).read().split("\n")
+            last = [float(x) for x in rows[-2].split()] if len(rows) > 1 else []
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    ms = (walls[2000] - walls[1000]) / 1000 * 1e3
+    return {"workload": "5,760-atom diamond carbon (20x6x6 cells, a = 3.57 x 1.0023 A), the reference's nep_C.txt, "
+                        "npt_ber 300 K / 0 GPa, Tdamp 40, B 425 GPa, tau_p 1000, dt 0.1 fs, thermo every 100 steps, "
+                        "restart at the end, through dmgx_run_directory",
+            "wall_s_1000_steps": walls[1000], "wall_s_2000_steps": walls[2000],
+            "steady_ms_per_step": ms, "atom_steps_per_s": 5760 / (ms * 1e-3),
+            "final_thermo_row": last,
+            "published_L20_ms_per_step": {"DMG-MD (1 rank, HostStaged)": 5.1768, "GPUMD 5.9.1": 3.5262},
+            "published_source": "docs/status/performance/npt-benchmarks.md:52-57 (NVIDIA L20)"}
 
 
 WATER_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_water.txt")
@@ -269,6 +430,17 @@ def water_bench(dmgx, steps=100):
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         out[name] = {"atom_steps_per_s": N * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps}
+        if ens == "npt_scr":
+            # averages over a further stretch, read every 5 steps
+            vols, press = [], []
+            for _ in range(20):
+                th = md.run(5, 0.5, ens, t_init=300.0, t_final=300.0, **kw)
+                vols.append(md.box().volume())
+                press.append(float(th[2] + th[3] + th[4]) / 3.0 * dmgx.PRESSURE_UNIT_CONVERSION)
+            out[name].update({"mean_volume_A3": float(sum(vols) / len(vols)),
+                              "initial_volume_A3": s["box"].volume(),
+                              "mean_pressure_GPa": float(sum(press) / len(press)),
+                              "averaged_over": "100 further steps, sampled every 5"})
     return out
 
 
@@ -283,6 +455,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default filter/gather tile shapes")
     ap.add_argument("--no-water", action="store_true", help="skip the liquid-water (2-type) sample")
+    ap.add_argument("--no-small", action="store_true", help="skip the 4,096-atom carbon NVE line")
+    ap.add_argument("--no-context", action="store_true",
+                    help="skip the replay of the reference's published 5,760-atom NPT timing job")
+    ap.add_argument("--no-share", action="store_true", help="skip the 2,048,000-atom amorphous-carbon share line")
+    ap.add_argument("--amortised-steps", type=int, default=300,
+                    help="steps of the stretch timed with its Verlet rebuilds")
     ap.add_argument("--water-only", type=int, default=0, metavar="STEPS",
                     help="run only the liquid-water sample with STEPS timed steps per run and print its "
                          "JSON object (for kernel profiles of the 2-type path)")
@@ -347,7 +525,10 @@ def main():
     fl = flops_per_atom(nep.info, means["radial"], means["angular"])
     n_fused, ms_fused = prof["descriptor"]
     avg_fused = ms_fused / max(n_fused, 1)
-    achieved = fl["fused"] * N / (avg_fused * 1e-3) / 1e12
+    # the fused kernel's time per step (one launch over all atoms, or two
+    # over half of them each with the overlapped filter schedule)
+    per_step_fused = ms_fused / args.steps
+    achieved = fl["fused"] * N / (per_step_fused * 1e-3) / 1e12
     # per-kernel breakdown from a separate, untimed profiled stretch (events
     # around every kernel group separate the kernels by a few microseconds)
     nep.profile_reset()
@@ -358,9 +539,79 @@ def main():
     kernel_ms = {k: round(v[1] / max(v[0], 1), 4) for k, v in nep.profile().items()}
     print(f"bench: {args.steps} steps in {elapsed:.3f} s, T={th[0]:.1f} K, rebuilds={stats['rebuilds']}, "
           f"per-kernel avg ms {kernel_ms}, mean rows {means}", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    # a longer stretch that holds Verlet rebuilds at their natural rate
+    # (the timed steps above may hold none when K is small)
+    r0 = nep.neighbor_stats()["rebuilds"]
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    md.run(args.amortised_steps, 2.0, **ens)
+    torch.cuda.synchronize()
+    el_a = time.perf_counter() - ta
+    rebuilds_a = nep.neighbor_stats()["rebuilds"] - r0
+    amortised = {"steps": args.amortised_steps, "verlet_rebuilds": rebuilds_a,
+                 "ms_per_step": el_a / args.amortised_steps * 1e3,
+                 "atom_steps_per_s": N * args.amortised_steps / el_a}
+    # the same configuration with bfloat16 ANN operands, fresh state
+    bf16 = None
+    if args.ann == "fp32":
+        nep_b = dmgx.NEP(MODEL)
+        md_b = dmgx.MD(nep_b, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+        md_b.autotune(ann_bf16=True)
+        ens_b = dict(ens, ann_bf16=True)
+        md_b.run(args.warmup, 2.0, **ens_b)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        md_b.run(args.steps, 2.0, **ens_b)
+        torch.cuda.synchronize()
+        el_b = time.perf_counter() - tb
+        bf16 = {"ann": "bf16 operands, fp32 accumulation (matrix cores)", "steps": args.steps,
+                "atom_steps_per_s": N * args.steps / el_b, "ms_per_step": el_b / args.steps * 1e3}
+        del md_b, nep_b
+    small = None if args.no_small else small_bench(dmgx)
+    share = None if args.no_share else share_bench(dmgx)
+    context = None if args.no_context else context_bench(dmgx)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_seconds)
+        cpu.update(host_cpu())
+        cpu["parallel"] = cpu_baseline_parallel(8, args.cpu_seconds)
     serving = None if args.no_serve else serve_bench(dmgx)
     water = None if args.no_water else water_bench(dmgx)
+    # the fused kernel's measured arithmetic (committed counter summary of
+    # this workload) over its live launch time
+    cnt = counter_flops("k_descriptor", N)
+    valu_measured = None
+    if cnt:
+        valu_measured = {"valu_fp32_flop_per_atom": cnt["valu_fp32_flop_per_atom"],
+                         "mfma_flop_per_atom": cnt["mfma_flop_per_atom"],
+                         "valu_tflops": cnt["valu_fp32_flop_per_atom"] * N / (per_step_fused * 1e-3) / 1e12,
+                         "scalar_fp32_peak_tflops": PEAK_FP32_SCALAR_TFLOPS,
+                         "frac_of_scalar_peak": cnt["valu_fp32_flop_per_atom"] * N / (per_step_fused * 1e-3) / 1e12
+                         / PEAK_FP32_SCALAR_TFLOPS,
+                         "valu_issue_share": cnt["valu_issue_share"],
+                         "source": "profiles/k_descriptor_cu256k_counters.md"}
+    # the memory-bound per-step kernels against HBM: algorithmic bytes per
+    # atom (the filter's from the measurement plan's formula, the gather's
+    # minimal streams) and the counters' bytes, over the profiled stretch's
+    # average launch time
+    n_r, n_a, n_v = means["radial"], means["angular"], means["verlet"]
+    kRp = (nep.info.basis_size_radial + 1 + 3) & ~3
+    alg = {"k_filter": 4 * n_v + 4 * (n_r + n_a) + 8,
+           "k_gather": 16 * n_r + 24 * n_a + 4 * kRp * nep.info.num_types + 96}
+    hbm = {}
+    for k, key in (("k_filter", "filter"), ("k_gather", "gather")):
+        t_ms = kernel_ms.get(key, 0.0)
+        if t_ms <= 0:
+            continue
+        tr = measured_traffic("dmgx::" + k, N)
+        e = {"algorithmic_bytes_per_atom": round(alg[k], 1), "avg_launch_ms": t_ms,
+             "algorithmic_GBs": alg[k] * N / (t_ms * 1e-3) / 1e9}
+        e["frac_algorithmic"] = e["algorithmic_GBs"] / PEAK_HBM_GBS
+        if tr:
+            e["counter_bytes_per_atom"] = round(tr / N, 1)
+            e["counter_GBs"] = tr / (t_ms * 1e-3) / 1e9
+            e["frac_counter"] = e["counter_GBs"] / PEAK_HBM_GBS
+        hbm[k] = e
     out = {
         "metric": METRIC,
         "value": value,
@@ -392,22 +643,34 @@ def main():
                             "gather_ms_by_lanes": {str(k): round(v, 4) for k, v in tune["gather_ms"].items()},
                             "row_capacity": tune["row_capacity"],
                             "row_layout": {0: "multiple of 4", 1: "power of 2"}.get(tune["row_layout"], "default"),
-                            "force_ms_by_row_layout": {k: round(v, 4) for k, v in tune["force_ms"].items()}},
+                            "force_ms_by_row_layout": {k: round(v, 4) for k, v in tune["force_ms"].items()},
+                            "filter_descriptor_overlap": tune["overlap"],
+                            "force_ms_by_schedule": {k: round(v, 4) for k, v in tune["overlap_ms"].items()}},
             "final_temperature_K": th[0],
         },
+        "ms_per_step_with_rebuilds": amortised,
+        "bf16_ann": bf16,
         "roofline": {
             "kernel": "k_descriptor (fused descriptor + ANN + adjoint + angular partial forces)",
-            "bound": "mfma",
+            "bound": "valu",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": measured_traffic("dmgx::k_descriptor", N),
             "flop_per_atom": fl["fused"],
+            "flop_count": "algorithmic: the reference algorithm's FLOP per atom (measurement-plan formulas)",
             "avg_launch_ms": avg_fused,
+            "launches_per_step": n_fused / args.steps,
+            "ms_per_step_in_kernel": per_step_fused,
+            "measured_valu": valu_measured,
+            "hbm_kernels": hbm,
         },
         "cpu_baseline": cpu,
         "serving": serving,
+        "carbon_4096": small,
+        "carbon_share": share,
+        "published_job": context,
         "water": water,
     }
     print(json.dumps(out))

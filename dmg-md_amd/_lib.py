@@ -51,20 +51,22 @@ class Nhc(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_atoms", ctypes.c_longlong), ("max_requests", ctypes.c_int), ("flags", ctypes.c_uint),
-                ("margin", ctypes.c_double), ("use_graph", ctypes.c_int)]
+                ("margin", ctypes.c_double), ("use_graph", ctypes.c_int), ("hbm_bytes", ctypes.c_longlong)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_longlong) for n in ("submitted", "completed", "frames", "atom_frames")] + [
         (n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "elapsed_s", "frames_per_s")] + [
         (n, ctypes.c_longlong) for n in ("iterations", "relayouts", "max_slots")] + [
-        (n, ctypes.c_double) for n in ("relayout_s", "iterate_s", "retire_s")]
+        (n, ctypes.c_double) for n in ("relayout_s", "iterate_s", "retire_s")] + [
+        (n, ctypes.c_longlong) for n in ("hbm_budget", "max_bytes_in_flight")]
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("filter_lanes", ctypes.c_int), ("gather_lanes", ctypes.c_int),
                 ("filter_ms", ctypes.c_double * 3), ("gather_ms", ctypes.c_double * 3), ("tuned", ctypes.c_int),
-                ("row_layout", ctypes.c_int), ("row_capacity", ctypes.c_int), ("force_ms", ctypes.c_double * 2)]
+                ("row_layout", ctypes.c_int), ("row_capacity", ctypes.c_int), ("force_ms", ctypes.c_double * 2),
+                ("overlap", ctypes.c_int), ("overlap_ms", ctypes.c_double * 2)]
 
 
 ACCUMULATE = 1
@@ -72,6 +74,7 @@ FORCE_REBUILD = 2
 ANN_BF16 = 4
 SORT_BY_GID = 8
 ANN_VALU = 16
+ANN_BF16_SPLIT = 32
 
 VP = ctypes.c_void_p
 I = ctypes.c_int
@@ -120,11 +123,11 @@ _SIGNATURES = {
     "dmgx_md_box": (I, [VP, ctypes.POINTER(Box)]),
     "dmgx_md_autotune": (I, [VP, U, ctypes.POINTER(Tuning)]),
     "dmgx_md_reorder": (I, [VP, PI]),
-    "dmgx_server_create": (I, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
+    "dmgx_server_create": (I, [ctypes.POINTER(VP), I, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
     "dmgx_server_destroy": (None, [VP]),
-    "dmgx_server_submit": (I, [VP, ctypes.POINTER(Box), I, PI, PD, PD, PD, I, D,
+    "dmgx_server_submit": (I, [VP, I, ctypes.POINTER(Box), I, PI, PD, PD, PD, I, D,
                                ctypes.POINTER(ctypes.c_longlong)]),
-    "dmgx_server_poll": (I, [VP, ctypes.c_longlong, I, PD, PD, PD, PD, PD, PI]),
+    "dmgx_server_poll": (I, [VP, ctypes.c_longlong, I, PD, PD, PD, PD, PD, PD, PI]),
     "dmgx_server_drain": (I, [VP]),
     "dmgx_server_get_stats": (I, [VP, ctypes.POINTER(ServerStats)]),
     "dmgx_run_directory": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),

@@ -10,6 +10,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 #include "ensemble.hpp"
@@ -30,7 +31,7 @@ struct dmgx_md {
 
 struct dmgx_server {
   dmgx::Server server;
-  dmgx_server(dmgx::NepEngine* nep, const dmgx::ServeConfig& cfg) : server(nep, cfg) {}
+  dmgx_server(const std::vector<dmgx::NepEngine*>& models, const dmgx::ServeConfig& cfg) : server(models, cfg) {}
 };
 
 struct dmgx_rng {
@@ -206,6 +207,9 @@ void fill_tuning(const dmgx::NepEngine::Tuning& t, dmgx_tuning* out)
   out->row_capacity = t.row_capacity;
   out->force_ms[0] = t.force_ms[0];
   out->force_ms[1] = t.force_ms[1];
+  out->overlap = t.overlap;
+  out->overlap_ms[0] = t.overlap_ms[0];
+  out->overlap_ms[1] = t.overlap_ms[1];
 }
 }  // namespace
 
@@ -482,40 +486,51 @@ int dmgx_md_autotune(dmgx_md* md, unsigned flags, dmgx_tuning* out)
   });
 }
 
-int dmgx_server_create(dmgx_nep* nep, const dmgx_server_config* cfg, dmgx_server** out)
+int dmgx_server_create(dmgx_nep* const* models, int num_models, const dmgx_server_config* cfg,
+                       dmgx_server** out)
 {
   return guarded([&] {
-    if (!nep || !cfg || !out) throw std::invalid_argument("null argument");
+    if (!models || !cfg || !out) throw std::invalid_argument("null argument");
+    if (num_models <= 0 || num_models > 64) throw std::invalid_argument("num_models must lie in [1, 64]");
     *out = nullptr;
+    std::vector<dmgx::NepEngine*> engines;
+    for (int m = 0; m < num_models; ++m) {
+      if (!models[m]) throw std::invalid_argument("null potential handle");
+      engines.push_back(&models[m]->engine);
+    }
     dmgx::ServeConfig c;
     c.max_atoms = cfg->max_atoms;
     c.max_requests = cfg->max_requests;
     c.flags = cfg->flags;
     c.margin = cfg->margin;
     c.use_graph = cfg->use_graph != 0;
+    c.hbm_bytes = cfg->hbm_bytes;
     if (!(c.margin >= 0.0)) throw std::invalid_argument("negative shell margin");
-    *out = new dmgx_server(&nep->engine, c);
+    *out = new dmgx_server(engines, c);
   });
 }
 
 void dmgx_server_destroy(dmgx_server* server) { delete server; }
 
-int dmgx_server_submit(dmgx_server* server, const dmgx_box* box, int N, const int* type_host,
+int dmgx_server_submit(dmgx_server* server, int model, const dmgx_box* box, int N, const int* type_host,
                        const double* mass_host, const double* position_host, const double* velocity_host, int steps,
                        double dt_fs, long long* id_out)
 {
   return guarded([&] {
     if (!server || !box || !id_out) throw std::invalid_argument("null argument");
-    *id_out = server->server.submit(to_box(box), N, type_host, mass_host, position_host, velocity_host, steps, dt_fs);
+    *id_out = server->server.submit(model, to_box(box), N, type_host, mass_host, position_host, velocity_host, steps,
+                                    dt_fs);
   });
 }
 
 int dmgx_server_poll(dmgx_server* server, long long id, int wait, double* potential_host, double* force_host,
-                     double* position_host, double* velocity_host, double* latency_ms, int* done)
+                     double* position_host, double* velocity_host, double* virial_host, double* latency_ms,
+                     int* done)
 {
   return guarded([&] {
     if (!server || !done) throw std::invalid_argument("null argument");
-    *done = server->server.poll(id, wait != 0, potential_host, force_host, position_host, velocity_host, latency_ms)
+    *done = server->server.poll(id, wait != 0, potential_host, force_host, position_host, velocity_host, virial_host,
+                                latency_ms)
                 ? 1
                 : 0;
   });
@@ -550,6 +565,8 @@ int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out)
     out->relayout_s = s.relayout_s;
     out->iterate_s = s.iterate_s;
     out->retire_s = s.retire_s;
+    out->hbm_budget = s.hbm_budget;
+    out->max_bytes_in_flight = s.max_bytes_in_flight;
   });
 }
 

@@ -33,6 +33,13 @@ void check(hipError_t e, const char* what)
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// ANN arithmetic of a call: FP32 on the matrix cores unless split-bfloat16
+// (32), bfloat16 (4) operands or the vector ALUs (16) are asked for
+int ann_mode(unsigned flags)
+{
+  return (flags & 32u) ? 3 : (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
+}
+
 template <class T>
 void dalloc(T** p, size_t n, const char* what)
 {
@@ -288,12 +295,18 @@ NepEngine::NepEngine(const std::string& path)
     dmd_.cA = params_pad_ + n_w0 + Db + n_cR;
   }
   check(hipHostMalloc((void**)&hflags_, kNumFlags * sizeof(int), hipHostMallocDefault), "pinned flags");
+  // the overlapped filter's stream and its fork/join events (made here, not
+  // at first use, which may fall inside a caller's graph capture)
+  check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "side stream");
+  check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "fork event");
+  check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "join event");
   dalloc(&w_.flags, kNumFlags, "flags");
   check(hipMemset(w_.flags, 0, kNumFlags * sizeof(int)), "clear flags");
 }
 
 NepEngine::~NepEngine()
 {
+  if (side_) (void)hipStreamSynchronize(side_);  // an overlapped filter may still use the workspace
   free_all();
   dfree(&params_);
   dfree(&params_pad_);
@@ -305,6 +318,9 @@ NepEngine::~NepEngine()
   dfree(&sb_vir_);
   dfree(&w_.flags);
   if (hflags_) (void)hipHostFree(hflags_);
+  if (side_) (void)hipStreamDestroy(side_);
+  if (fork_) (void)hipEventDestroy(fork_);
+  if (join_) (void)hipEventDestroy(join_);
 }
 
 void NepEngine::free_all()
@@ -483,16 +499,9 @@ void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int 
   check(launch_rebuild(p.box, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w_, s), "rebuild");
   prof_.end(s);
   const bool acc = (flags & 1u) != 0;
-  prof_.begin(kProfFilter, s);
-  check(launch_filter(p.box, dm_, nd1, nd2, w_, s, tune_.filter_lanes), "filter");
-  prof_.end(s);
-  hipEvent_t ev0, ev1;
-  const bool attached = prof_.attach(kProfDescriptor, &ev0, &ev1);
-  if (!attached) prof_.begin(kProfDescriptor, s);
   // ANN: FP32 on the matrix cores unless BF16 operands or the vector path are asked for
-  const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
-  check(launch_descriptor(dmd_, N, nd1, nd2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
-  if (!attached) prof_.end(s);
+  const int ann = ann_mode(flags);
+  launch_filter_descriptor(p.box, N, nd1, nd2, type, pe, acc, ann, true, s, true);
   prof_.begin(kProfGather, s);
   check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s, tune_.gather_lanes), "gather");
   prof_.end(s);
@@ -641,12 +650,51 @@ void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2,
   throw std::runtime_error("Verlet rows kept overflowing");
 }
 
+// Smallest centre range split for the overlapped schedule: each half keeps
+// at least this many centres (below it the launches are too short to gain)
+constexpr int kOverlapMinCentres = 16384;
+
+void NepEngine::launch_filter_descriptor(const DevBox& b, int N, int c1, int c2, const int* type, double* pe,
+                                         bool acc, int ann, bool close, hipStream_t s, bool profile)
+{
+  const int n = c2 - c1;
+  // per-kernel event timing (profiling mode 1) keeps the sequential order,
+  // so that each kernel group is timed alone
+  const bool split = tune_.overlap && n >= 2 * kOverlapMinCentres && !(profile && prof_.on() && !prof_.attach_mode());
+  if (!split) {
+    if (profile) prof_.begin(kProfFilter, s);
+    check(launch_filter(b, dm_, c1, c2, w_, s, tune_.filter_lanes, close), "filter");
+    if (profile) prof_.end(s);
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    const bool attached = profile && prof_.attach(kProfDescriptor, &ev0, &ev1);
+    if (profile && !attached) prof_.begin(kProfDescriptor, s);
+    check(launch_descriptor(dmd_, N, c1, c2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
+    if (profile && !attached) prof_.end(s);
+    return;
+  }
+  // halves on workgroup boundaries of the descriptor (256 centres)
+  const int mid = c1 + ((n / 2 + 255) & ~255);
+  check(launch_filter(b, dm_, c1, mid, w_, s, tune_.filter_lanes, close), "filter");
+  // the side stream starts after the first filter (and the rebuild before
+  // it): the second filter overlaps the first descriptor
+  check(hipEventRecord(fork_, s), "fork");
+  check(hipStreamWaitEvent(side_, fork_, 0), "fork wait");
+  check(launch_filter(b, dm_, mid, c2, w_, side_, tune_.filter_lanes, false), "filter");
+  check(hipEventRecord(join_, side_), "join");
+  for (int h = 0; h < 2; ++h) {
+    if (h == 1) check(hipStreamWaitEvent(s, join_, 0), "join wait");
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (profile) prof_.attach(kProfDescriptor, &ev0, &ev1);
+    check(launch_descriptor(dmd_, N, h ? mid : c1, h ? c2 : mid, type, pe, acc, ann, w_, s, ev0, ev1),
+          "descriptor");
+  }
+}
+
 void NepEngine::launch_step_kernels(const DevBox& b, int N, int nc, const int* type, double* pe, double* force,
                                     double* virial, unsigned flags, hipStream_t s)
 {
-  const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
-  check(launch_filter(b, dm_, 0, nc, w_, s, tune_.filter_lanes), "filter");
-  check(launch_descriptor(dmd_, N, 0, nc, type, pe, false, ann, w_, s), "descriptor");
+  const int ann = ann_mode(flags);
+  launch_filter_descriptor(b, N, 0, nc, type, pe, false, ann, true, s, false);
   check(launch_gather(dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, tune_.gather_lanes), "gather");
 }
 
@@ -728,6 +776,16 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
       tune_.force_ms[other] = t_other;
       allocate(w_.cap, mv_base_);
     }
+  }
+  // the overlapped filter/descriptor schedule, where the centres split
+  if (nc >= 2 * kOverlapMinCentres) {
+    double t[2];
+    for (int o = 0; o < 2; ++o) {
+      tune_.overlap = o;
+      t[o] = time_median([&] { launch_step_kernels(b, N, nc, type, pe, force, virial, flags, s); });
+      tune_.overlap_ms[o] = t[o];
+    }
+    tune_.overlap = t[1] < t[0] ? 1 : 0;
   }
   for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
   tune_.tuned = true;
@@ -850,7 +908,7 @@ void NepEngine::compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int
   check(launch_rebuild(b, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w, s), "rebuild");
   check(launch_filter(b, dm_, nd1, nd2, w, s, tune_.filter_lanes), "filter");
   const bool acc = (flags & 1u) != 0;
-  const int ann = (flags & 4u) ? 2 : (flags & 16u) ? 0 : 1;
+  const int ann = ann_mode(flags);
   check(launch_descriptor(dmd_, N, nd1, nd2, type, pe, acc, ann, w, s), "descriptor");
   check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w, s, tune_.gather_lanes), "gather");
   check(hipMemcpyAsync(hflags_, w.flags, kNumFlags * sizeof(int), hipMemcpyDeviceToHost, s), "read status words");
@@ -862,6 +920,17 @@ void NepEngine::compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int
     throw std::runtime_error("Verlet row overflow: " + std::to_string(hflags_[kFlagVerletOverflow]) +
                              " neighbours within rc + skin, the fixed workspace's rows hold " + std::to_string(mv) +
                              " (MN_radial of the model); outputs are invalid");
+}
+
+double NepEngine::bytes_per_slot() const
+{
+  int mv = w_.MV;
+  if (mv <= 0)
+    mv = padded_rows(std::max(8, (int)std::ceil(model_.MN_R * std::pow((model_.rcR_max + kSkin) / model_.rcR_max, 3))),
+                     tune_.row_layout);
+  const double per = 2 * sizeof(double4) + 6 * sizeof(int) + 3 * sizeof(double) +
+                     (double)model_.num_types * ((model_.kR + 3) & ~3) * sizeof(float);
+  return per + kRowEntryBytes * mv;
 }
 
 void NepEngine::stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds)
@@ -921,10 +990,7 @@ void NepEngine::copy_verlet_rows(int c1, int c2, int max_nb, int* counts, int* r
     const int n = std::min(vn[i], w_.MV);
     counts[i - c1] = vn[i];
     if (n > max_nb) throw std::invalid_argument("max_nb too small");
-    std::vector<int> r(n);
-    for (int p = 0; p < n; ++p) r[p] = vl[(size_t)i * w_.MV + p];
-    std::sort(r.begin(), r.end());
-    std::copy(r.begin(), r.end(), rows + (size_t)(i - c1) * max_nb);
+    std::copy(vl.begin() + (size_t)i * w_.MV, vl.begin() + (size_t)i * w_.MV + n, rows + (size_t)(i - c1) * max_nb);
   }
 }
 
@@ -1215,10 +1281,13 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
     force(flags);
     have_force_ = true;
   }
-  // NHC needs the pre-force kinetic energy and NPT the pressure on the host
-  // every step (scalar chain/box state): those stay on the synchronous path.
+  // NHC needs the pre-force kinetic energy on the host before every force
+  // (scalar chain state): it stays on the synchronous path, as do replicated
+  // small boxes, the adaptive step and velocity corrections.  NPT runs the
+  // device-resident step below one step per chunk, with a single
+  // synchronisation per step for the barostat's pressure.
   int reps[3];
-  if (ctl.needs_pre_force_ke() || ctl.has_barostat() || nep_->small_box(box_, reps) || adaptive || correct) {
+  if (ctl.needs_pre_force_ke() || nep_->small_box(box_, reps) || adaptive || correct) {
     for (int step = 0; step < steps; ++step) {
       if (correct && step % opt.correct_every == 0) {
         check(hipStreamSynchronize(stream_), "velocity correction");
@@ -1274,22 +1343,24 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
   // overflow rolls the chunk back to its checkpoint.
   int kind = 0;
   double coupling = 0.0;
-  if (ens.kind == Ensemble::nvt_ber) {
+  if (ens.kind == Ensemble::nvt_ber || ens.kind == Ensemble::npt_ber) {
     coupling = 1.0 / ens.t_coupling;
     kind = coupling > 1.0e-5 ? 1 : 0;
-  } else if (ens.kind == Ensemble::nvt_bdp) {
+  } else if (ens.kind == Ensemble::nvt_bdp || ens.kind == Ensemble::npt_scr) {
     kind = 2;
     coupling = ens.t_coupling > 0.1 ? std::exp(-1.0 / ens.t_coupling) : 0.0;
   }
+  const bool baro = ctl.has_barostat();
   constexpr int kChunk = 128;
   if (!rand_) dalloc(&rand_, 2 * kChunk, "thermostat draws");
   if (!factor_) dalloc(&factor_, 1, "thermostat factor");
-  // the chunks' skin check uses the whole skin: references rescaled by an
-  // earlier barostat run are replaced by a rebuild first
-  if (nep_->rescaled_since_build()) nep_->notify_box_changed();
-  if (!hrand_) check(hipHostMalloc((void**)&hrand_, 2 * kChunk * sizeof(double), hipHostMallocDefault), "pinned");
+  // pinned: the thermostat draws of a chunk, then one step's thermo
+  if (!hrand_)
+    check(hipHostMalloc((void**)&hrand_, (2 * kChunk + 8) * sizeof(double), hipHostMallocDefault), "pinned");
+  double* hth = hrand_ + 2 * kChunk;
   for (int c0 = 0; c0 < steps;) {
-    int n = std::min(kChunk, steps - c0);
+    // a barostat step needs the step's pressure on the host: chunks of one
+    int n = baro ? 1 : std::min(kChunk, steps - c0);
     if (on_output && every > 0) n = std::min(n, every - c0 % every);
     const Rng saved = ctl.rng();
     checkpoint(true);
@@ -1311,44 +1382,69 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
             nep_->plan(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u);
         Profiler& p = nep_->profiler();
         p.begin(kProfIntegrate, stream_);
+        // the displacement test honours what box rescales have used of the skin
         check(launch_md_first(plan.box, N_, nep_->model().num_types, dt, mass_, pos_, vel_, force_,
-                              pending ? factor_ : nullptr, type_, plan.forced, nep_->work(), stream_, unwrapped_),
+                              pending ? factor_ : nullptr, type_, plan.forced, nep_->work(), stream_, unwrapped_,
+                              nep_->half_skin()),
               "velocity Verlet (first half)");
         p.end(stream_);
         nep_->launch_force(plan, N_, 0, N_, 0, N_, type_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
         p.begin(kProfIntegrate, stream_);
-        check(launch_md_second(N_, dt, mass_, vel_, force_, pe_, vir_, kind != 0, tw_, stream_),
+        check(launch_md_second(N_, dt, mass_, vel_, force_, pe_, vir_, kind != 0 || baro, tw_, stream_),
               "velocity Verlet (second half)");
         p.end(stream_);
-        if (kind != 0) {
+        if (kind != 0 || baro) {
           p.begin(kProfThermo, stream_);
           check(launch_thermo_final(N_, volume, kind, coupling, ctl.target_temperature(step, steps), rand_ + 2 * k,
                                     factor_, tw_, stream_),
                 "thermostat");
           p.end(stream_);
-          pending = true;
+          pending = kind != 0;
         }
       }
       if (pending) check(launch_scale_by(3 * N_, factor_, vel_, stream_), "thermostat scale");
+      // a barostat step's thermo (before the thermostat rescale) comes back
+      // with the status words, in the same synchronisation
+      if (baro)
+        check(hipMemcpyAsync(hth, tw_.thermo, 8 * sizeof(double), hipMemcpyDeviceToHost, stream_), "thermo copy");
       redo = nep_->settle(stream_);
       if (!redo) break;
       checkpoint(false);
       ctl.set_rng(saved);
     }
     if (redo) throw std::runtime_error("Verlet rows kept overflowing");
+    if (baro) {
+      // the isotropic box step from the pressure before the thermostat
+      // rescale (ensemble_control.cpp:262-304): box and positions scaled,
+      // the Verlet list kept while the scale leaves room in the skin
+      std::memcpy(th, hth, sizeof(th));
+      const double sc = ctl.barostat_scale(th, box_volume(box_.h), c0, steps);
+      if (sc != 1.0) {
+        for (double& v : box_.h) v *= sc;
+        check(launch_scale(3 * N_, sc, pos_, stream_), "scale positions");
+        nep_->notify_box_scaled(sc, stream_);
+      }
+    }
     force_calls_ += n;
     c0 += n;
     elapsed_ += dt * n;
     if (on_output && ((every > 0 && c0 % every == 0) || c0 == steps)) {
       // the last step's thermo before its rescale (NVE: nothing rescales)
       double tho[8];
-      if (kind == 0) enqueue_thermo();
-      check(hipMemcpyAsync(tho, tw_.thermo, sizeof(tho), hipMemcpyDeviceToHost, stream_), "thermo copy");
-      check(hipStreamSynchronize(stream_), "output");
+      if (baro) {
+        std::memcpy(tho, th, sizeof(tho));
+        check(hipStreamSynchronize(stream_), "output");
+      } else {
+        if (kind == 0) enqueue_thermo();
+        check(hipMemcpyAsync(tho, tw_.thermo, sizeof(tho), hipMemcpyDeviceToHost, stream_), "thermo copy");
+        check(hipStreamSynchronize(stream_), "output");
+      }
       on_output(c0, tho);
     }
   }
-  thermo(th);
+  // a barostat run reports its last step's thermo before the rescales, as
+  // the per-step path does; the others the final state's
+  if (!baro || steps == 0) thermo(th);
   if (thermo_out) std::memcpy(thermo_out, th, sizeof(th));
 }
 

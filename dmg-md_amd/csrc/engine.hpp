@@ -39,6 +39,7 @@ class Profiler {
   // descriptor kernel only, with events attached to its dispatch
   void enable(int mode) { mode_ = mode; }
   bool on() const { return mode_ != 0; }
+  bool attach_mode() const { return mode_ == 2; }
   // mode 1: event pair recorded around a launch (group)
   void begin(int kernel, hipStream_t s);
   void end(hipStream_t s);
@@ -122,6 +123,11 @@ class NepEngine {
     int row_layout = -1;                // -1 before tuning (the default rule), else 0 or 1
     int row_capacity = 0;               // entries per row now allocated
     double force_ms[2] = {0, 0};        // filter + descriptor + gather per row layout (0: not timed)
+    // 1: the filter of the second half of the centres runs on a side stream
+    // while the fused descriptor kernel works on the first half (the
+    // memory-bound filter overlaps the VALU-bound descriptor)
+    int overlap = 0;
+    double overlap_ms[2] = {0, 0};      // filter + descriptor + gather without / with the overlap
     bool tuned = false;
   };
   const Tuning& tuning() const { return tune_; }
@@ -146,6 +152,9 @@ class NepEngine {
   // replicated small box) the next evaluation rebuilds.
   void notify_box_scaled(double scale, hipStream_t stream);
   bool rescaled_since_build() const { return scale_since_build_ != 1.0; }
+  // the displacement since the last Verlet build that requests a rebuild:
+  // half the skin, less what isotropic box rescales have used of it
+  double half_skin() const;
   // One evaluation whose per-atom workspace (packed positions, cell grid,
   // Verlet rows, edge rows, adjoints, partial forces, status words) is carved
   // out of `ws` (ws_bytes long, 256-byte aligned, device memory) instead of
@@ -159,6 +168,10 @@ class NepEngine {
   void compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int n1, int n2, int nd1, int nd2,
                   const int* type, const double* pos, const uint64_t* gid, double* pe, double* force,
                   double* virial, unsigned flags, hipStream_t stream);
+  // Device bytes of the workspace per local slot at the current row
+  // capacity (before the first allocation: the capacity the model's
+  // MN_radial implies)
+  double bytes_per_slot() const;
   void stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds);
   void totals(long long* sum_radial, long long* sum_angular, long long* sum_verlet, int* centres);
   void copy_verlet_rows(int c1, int c2, int max_nb, int* counts, int* rows);
@@ -191,10 +204,16 @@ class NepEngine {
   double scale_since_build_ = 1.0;  // isotropic box scale accumulated since the last Verlet build
   int seen_rebuilds_ = 0;           // device rebuild count when last read
   bool last_small_ = false;         // the last evaluation ran on a replicated small box
-  double half_skin() const;
   int padded_rows(int mv, int layout) const;
   void launch_step_kernels(const DevBox& b, int N, int nc, const int* type, double* pe, double* force,
                            double* virial, unsigned flags, hipStream_t s);
+  // filter + fused descriptor of centres [c1, c2), sequential or overlapped
+  // (tune_.overlap); `close` as launch_filter's; ev0/ev1 attached to the
+  // (first) descriptor dispatch when given
+  void launch_filter_descriptor(const DevBox& b, int N, int c1, int c2, const int* type, double* pe, bool acc,
+                                int ann, bool close, hipStream_t s, bool profile);
+  hipStream_t side_ = nullptr;  // the overlapped filter's stream
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
   int last_N_stats_ = 0, last_c1_ = 0, last_c2_ = 0;
   Profiler prof_;
   // replicated small-box system (atoms x replicas)

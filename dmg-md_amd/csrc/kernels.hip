@@ -235,7 +235,8 @@ constexpr int kGatherLanes = DMGX_GATHER_LANES;
 constexpr int kGatherBatch = DMGX_GATHER_BATCH;  // radial edges per lane per batch (kR <= 9)
 
 template <bool ONE, bool MAP, int G, int RB>
-__global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1, int c2, int nblocks, DevWork w)
+__global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1, int c2, int nblocks, int close,
+                                                 DevWork w)
 {
   constexpr int PER = kBlock / G;
   static_assert(G >= 1 && G <= 32 && (G & (G - 1)) == 0, "filter lanes: a power of two up to 32");
@@ -257,7 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
     __syncthreads();
   }
   // the rebuild kernels of this step ran before this one: close the request
-  if (blockIdx.x == 0 && threadIdx.x == 0 && w.flags[kFlagRebuild]) {
+  // (one filter launch per evaluation does; the others of a split one skip)
+  if (close && blockIdx.x == 0 && threadIdx.x == 0 && w.flags[kFlagRebuild]) {
     w.flags[kFlagRebuild] = 0;
     w.flags[kFlagRebuildCount] += 1;
   }
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
 // ---------------------------------------------------------------------------
 
 // ANN modes of the descriptor kernel
-enum : int { kAnnValu = 0, kAnnMfma32 = 1, kAnnMfmaBf16 = 2 };
+enum : int { kAnnValu = 0, kAnnMfma32 = 1, kAnnMfmaBf16 = 2, kAnnMfmaSplit = 3 };
 
 // multi-type models with at most this many types keep their coefficient
 // blocks in LDS
@@ -402,10 +404,30 @@ constexpr int ann_lds_floats() { return 64 * (DMAX + 1) + 128; }
 // b0 and w1 (zero past H)
 template <int DMAX>
 constexpr int ann_w_stride() { return DMAX + 1; }
-template <int DMAX>
-constexpr int ann_w_floats() { return 64 * ann_w_stride<DMAX>() + 128; }
+// floats of W0 rows: 64 x WS floats, or with SPLIT three bfloat16 parts of
+// 64 x WS shorts each (96 x WS floats); then b0 and w1
+template <int DMAX, bool SPLIT = false>
+constexpr int ann_w_floats() { return (SPLIT ? 96 : 64) * ann_w_stride<DMAX>() + 128; }
 
-template <int DMAX>
+// bfloat16 -> float (exact)
+__device__ __forceinline__ float from_bf16(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
+
+// x = hi + mid + lo + r with hi = bf16(x), mid = bf16(x - hi),
+// lo = bf16(x - hi - mid) (each difference exact in FP32), |r| <= 2^-27 |x|
+struct Bf16Split {
+  short hi, mid, lo;
+};
+__device__ __forceinline__ Bf16Split split_bf16(float x)
+{
+  const short hi = to_bf16(x);
+  const float r1 = x - from_bf16(hi);
+  const short mid = to_bf16(r1);
+  return Bf16Split{hi, mid, to_bf16(r1 - from_bf16(mid))};
+}
+
+// SPLIT: W0 is stored as its three bfloat16 parts, shorts [64][WS] each
+// (hi, then mid, then lo), in place of the float rows
+template <int DMAX, bool SPLIT = false>
 __device__ __forceinline__ void stage_ann_weights(const DevModel& m, int D, float* __restrict__ ws, int t = 0)
 {
   constexpr int WS = ann_w_stride<DMAX>();
@@ -413,13 +435,23 @@ __device__ __forceinline__ void stage_ann_weights(const DevModel& m, int D, floa
   const float* __restrict__ w0 = m.w0 + (size_t)t * H * D;
   const float* __restrict__ b0 = m.b0 + (size_t)t * H;
   const float* __restrict__ w1 = m.w1 + (size_t)t * H;
+  short* __restrict__ whl = reinterpret_cast<short*>(ws);
   for (int k = threadIdx.x; k < 64 * WS; k += blockDim.x) {
     const int h = k / WS, d = k % WS;
-    ws[k] = (h < H && d < D) ? w0[h * D + d] : 0.0f;
+    const float x = (h < H && d < D) ? w0[h * D + d] : 0.0f;
+    if (SPLIT) {
+      const Bf16Split p = split_bf16(x);
+      whl[k] = p.hi;
+      whl[64 * WS + k] = p.mid;
+      whl[128 * WS + k] = p.lo;
+    } else {
+      ws[k] = x;
+    }
   }
   for (int h = threadIdx.x; h < 64; h += blockDim.x) {
-    ws[64 * WS + h] = h < H ? b0[h] : 0.0f;
-    ws[64 * WS + 64 + h] = h < H ? w1[h] : 0.0f;
+    constexpr int bo = (SPLIT ? 96 : 64) * WS;
+    ws[bo + h] = h < H ? b0[h] : 0.0f;
+    ws[bo + 64 + h] = h < H ? w1[h] : 0.0f;
   }
 }
 
@@ -437,11 +469,20 @@ __device__ __forceinline__ void wave_lds_sync()
 // kTypesMfma, ws holds the types' weight blocks one after another): per
 // type present in the wavefront one pass over the hidden tiles whose
 // activation gradients and energies count only for atoms of that type.
-template <int DMAX, bool BF16, bool MT = false>
+template <int DMAX, int MODE, bool MT = false>
 __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* q, float* Fp, float& E,
                                          float* __restrict__ lds, const float* __restrict__ ws, int ti = 0,
                                          int T = 1)
 {
+  // MODE kAnnMfma32: FP32 operands (16x16x4 FP32 MFMA, exact products);
+  // kAnnMfmaBf16: bfloat16 operands; kAnnMfmaSplit: every operand split
+  // into three bfloat16 parts (hi + mid + lo) and each product formed from
+  // the six part products of relative size above 2^-24 (hi*hi, hi*mid,
+  // mid*hi, mid*mid, hi*lo, lo*hi; 16x16x16 bf16 MFMAs, FP32 accumulation,
+  // smallest first): FP32 accuracy from bfloat16 products, the "bf16x6"
+  // method of Henry, Tang and Heinecke (ARITH 2019)
+  constexpr bool BF16 = MODE == kAnnMfmaBf16;
+  constexpr bool X3 = MODE == kAnnMfmaSplit;
   constexpr int DS = DMAX + 1;
   constexpr int DT = (DMAX + 15) / 16;
   const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
@@ -463,6 +504,20 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
 #pragma unroll 1
   for (int nt = 0; nt < 4; ++nt) {
     const float* qrow = Qs + (16 * nt + lr) * DS;
+    // split descriptor operands of this atom tile, reused by every hidden tile
+    s16x4 qh[X3 ? DT : 1], qm[X3 ? DT : 1], ql[X3 ? DT : 1];
+    if (X3) {
+#pragma unroll
+      for (int ks = 0; ks < DT; ++ks)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int d = 16 * ks + 4 * lg + c;
+          const Bf16Split p = split_bf16(d < D ? qrow[d] : 0.0f);
+          qh[ks][c] = p.hi;
+          qm[ks][c] = p.mid;
+          ql[ks][c] = p.lo;
+        }
+    }
     f32x4 facc[DT];
 #pragma unroll
     for (int dt = 0; dt < DT; ++dt) facc[dt] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -475,16 +530,41 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
     for (int t = 0; t < (MT ? kTypesMfma : 1); ++t) {
       if (t >= T || !((present >> t) & 1u)) continue;
       const bool mine = !MT || at == t;
-      const float* __restrict__ W0 = ws + (size_t)t * ann_w_floats<DMAX>();  // [64][WS], zero-padded
-      const float* __restrict__ b0 = W0 + 64 * WS;
-      const float* __restrict__ w1 = W0 + 64 * WS + 64;
+      // [64][WS] floats (or three [64][WS] bfloat16 parts), zero-padded
+      const float* __restrict__ W0 = ws + (size_t)t * ann_w_floats<DMAX, X3>();
+      const float* __restrict__ b0 = W0 + (X3 ? 96 : 64) * WS;
+      const float* __restrict__ w1 = b0 + 64;
       // one tile of 16 hidden units at a time: GEMM 1, activation, and its
       // share of GEMM 2, so that only one accumulator tile is live
 #pragma unroll 1
       for (int mt = 0; mt < HT; ++mt) {
         f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
         const int ha = 16 * mt + lr;
-        if (!BF16) {
+        const short* __restrict__ Wh = reinterpret_cast<const short*>(W0);
+        const short* __restrict__ Wm = Wh + 64 * WS;
+        const short* __restrict__ Wl = Wh + 128 * WS;
+        if (X3) {
+#pragma unroll
+          for (int ks = 0; ks < DT; ++ks) {
+            s16x4 ah, am, al;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              // columns past the row stride would read the next row or, past
+              // the last row, the float biases as bfloat16 pairs (possibly
+              // NaN patterns): they hold zeros
+              const int d = 16 * ks + 4 * lg + c;
+              ah[c] = d < WS ? Wh[ha * WS + d] : (short)0;
+              am[c] = d < WS ? Wm[ha * WS + d] : (short)0;
+              al[c] = d < WS ? Wl[ha * WS + d] : (short)0;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, qh[ks], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, ql[ks], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, qm[ks], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, qh[ks], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, qm[ks], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, qh[ks], acc, 0, 0, 0);
+          }
+        } else if (!BF16) {
 #pragma unroll
           for (int ks = 0; ks < (DMAX + 3) / 4; ++ks) {
             const int d = 4 * ks + lg;
@@ -516,7 +596,34 @@ __device__ __forceinline__ void ann_mfma(const DevModel& m, int D, const float* 
             G[r] = w1[h] * (1.0f - x * x);
           }
         }
-        if (!BF16) {
+        if (X3) {
+          s16x4 gh, gm, gl;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const Bf16Split p = split_bf16(G[r]);
+            gh[r] = p.hi;
+            gm[r] = p.mid;
+            gl[r] = p.lo;
+          }
+#pragma unroll
+          for (int dt = 0; dt < DT; ++dt) {
+            const int d = 16 * dt + lr;
+            s16x4 bh, bm, bl;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const int h = 16 * mt + 4 * lg + c;
+              bh[c] = d < WS ? Wh[h * WS + d] : (short)0;
+              bm[c] = d < WS ? Wm[h * WS + d] : (short)0;
+              bl[c] = d < WS ? Wl[h * WS + d] : (short)0;
+            }
+            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gl, bh, facc[dt], 0, 0, 0);
+            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gh, bl, facc[dt], 0, 0, 0);
+            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gm, bm, facc[dt], 0, 0, 0);
+            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gm, bh, facc[dt], 0, 0, 0);
+            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gh, bm, facc[dt], 0, 0, 0);
+            facc[dt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gh, bh, facc[dt], 0, 0, 0);
+          }
+        } else if (!BF16) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int h = 16 * mt + 4 * lg + r;
@@ -575,17 +682,17 @@ constexpr int desc_lds_bytes()
   const int pcut = C::one ? 1 : 4 * pairs;
   const int ctab = C::cl ? pairs * (C::nr * C::kr + C::na * C::ka) : 1;
   const int mfma = C::ann != kAnnValu;
-  const int annw = mfma ? (C::one ? 1 : kTypesMfma) * ann_w_floats<C::dim>() : 1;
+  const int annw = mfma ? (C::one ? 1 : kTypesMfma) * ann_w_floats<C::dim, C::ann == kAnnMfmaSplit>() : 1;
   const int annl = mfma ? (kDescBlock / 64) * ann_lds_floats<C::dim>() : 0;
   return 4 * (pcut + ctab + annw + annl);
 }
 
-template <int NR, int KR, int NA, int KA, bool Q2, bool Q1, bool ONE>
+template <int NR, int KR, int NA, int KA, bool Q2, bool Q1, bool ONE, int ANN = kAnnMfma32>
 constexpr bool mfma_ann_fits()
 {
   // single-type instances reach two wavefronts per SIMD, so they keep two
   // workgroups per CU (64 KiB each); multi-type ones run one either way
-  return desc_lds_bytes<Cfg<NR, KR, NA, KA, 4, Q2, Q1, ONE, true, kAnnMfma32, !ONE>>() <=
+  return desc_lds_bytes<Cfg<NR, KR, NA, KA, 4, Q2, Q1, ONE, true, ANN, !ONE>>() <=
          (ONE ? 64 * 1024 : kDescLdsMax);
 }
 
@@ -624,11 +731,13 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
     __syncthreads();
   }
   if (C::ann == kAnnValu && !active) return;
-  __shared__ float ann_w[C::ann != kAnnValu ? (C::one ? 1 : kTypesMfma) * ann_w_floats<DMAX>() : 1];
+  __shared__ float ann_w[C::ann != kAnnValu ? (C::one ? 1 : kTypesMfma) * ann_w_floats<DMAX, C::ann == kAnnMfmaSplit>()
+                                           : 1];
   if (C::ann != kAnnValu) {
     // every type's weights (at most kTypesMfma types), staged once per workgroup
     for (int t = 0; t < (C::one ? 1 : m.T); ++t)
-      stage_ann_weights<DMAX>(m, C::exact ? DMAX : m.D, ann_w + t * ann_w_floats<DMAX>(), t);
+      stage_ann_weights<DMAX, C::ann == kAnnMfmaSplit>(
+          m, C::exact ? DMAX : m.D, ann_w + t * ann_w_floats<DMAX, C::ann == kAnnMfmaSplit>(), t);
     __syncthreads();
   }
   const int nR = C::exact ? NR : m.nR, nA = C::exact ? NA : m.nA;
@@ -655,14 +764,14 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
 
   // ---- radial descriptor: q_n = sum_j sum_k c[ti,tj,n,k] f_k(r_ij)
   // The basis values are summed per neighbour type and contracted with the
-  // coefficients once per atom (single type: one sum; up to kTypesLds types:
+  // coefficients once per atom (single type: one sum; up to C::ltypes types:
   // one per type); more types contract per edge.
   float Fs[KR];
 #pragma unroll
   for (int k = 0; k < KR; ++k) Fs[k] = 0.0f;
-  float Ft[C::cl ? kTypesLds : 1][KR];
+  float Ft[C::cl ? C::ltypes : 1][KR];
 #pragma unroll
-  for (int t = 0; t < (C::cl ? kTypesLds : 1); ++t)
+  for (int t = 0; t < (C::cl ? C::ltypes : 1); ++t)
 #pragma unroll
     for (int k = 0; k < KR; ++k) Ft[t][k] = 0.0f;
   // the radial loop keeps kAhead edge records in flight (its work per edge
@@ -702,7 +811,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
         if (k < kR) Fs[k] += f[k];
     } else if (C::cl) {
 #pragma unroll
-      for (int t = 0; t < kTypesLds; ++t) {
+      for (int t = 0; t < C::ltypes; ++t) {
         if (t < T) {
           const float wt = tj == t ? 1.0f : 0.0f;
 #pragma unroll
@@ -739,7 +848,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
     for (int n = 0; n < NR; ++n) {
       float g = 0.0f;
 #pragma unroll
-      for (int t = 0; t < kTypesLds; ++t) {
+      for (int t = 0; t < C::ltypes; ++t) {
         if (t < T) {
           const float* c = ctab + (ti * T + t) * NR * KR + n * KR;
 #pragma unroll
@@ -845,7 +954,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   float E = 0.0f;
   if (C::ann != kAnnValu) {
     __shared__ float ann_lds[kDescBlock / 64][ann_lds_floats<DMAX>()];
-    ann_mfma<DMAX, C::ann == kAnnMfmaBf16, !C::one>(m, D, q, Fp, E, ann_lds[threadIdx.x >> 6], ann_w, ti, T);
+    ann_mfma<DMAX, C::ann, !C::one>(m, D, q, Fp, E, ann_lds[threadIdx.x >> 6], ann_w, ti, T);
   } else {
     const int H = m.H;
     const float* w0 = m.w0 + (size_t)ti * H * D;
@@ -1558,26 +1667,40 @@ static inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 
 // lanes per centre (8, 16 or 32) and the Verlet entries each lane loads
 // per pass (7, 7 or 4)
+// closes a rebuild request as the filter does, for an evaluation whose
+// first filter launch has no centres
+__global__ void k_close_request(int* __restrict__ flags)
+{
+  if (flags[kFlagRebuild]) {
+    flags[kFlagRebuild] = 0;
+    flags[kFlagRebuildCount] += 1;
+  }
+}
+
 template <int G, int RB>
-static void run_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s)
+static void run_filter(const DevBox& b, const DevModel& m, int c1, int c2, int close, DevWork& w, hipStream_t s)
 {
   constexpr int per = kBlock / G;
   const int nb = (c2 - c1 + per - 1) / per;
   if (w.map.owner) {
-    if (m.T == 1) k_filter<true, true, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
-    else k_filter<false, true, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+    if (m.T == 1) k_filter<true, true, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
+    else k_filter<false, true, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
   } else {
-    if (m.T == 1) k_filter<true, false, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
-    else k_filter<false, false, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, w);
+    if (m.T == 1) k_filter<true, false, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
+    else k_filter<false, false, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
   }
 }
 
-hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s, int lanes)
+hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s, int lanes,
+                         bool close)
 {
   if (c2 > c1) {
-    if (lanes == 8) run_filter<8, kFilterBatch>(b, m, c1, c2, w, s);
-    else if (lanes == 32) run_filter<32, 4>(b, m, c1, c2, w, s);
-    else run_filter<kFilterLanes, kFilterBatch>(b, m, c1, c2, w, s);
+    const int cl = close ? 1 : 0;
+    if (lanes == 8) run_filter<8, kFilterBatch>(b, m, c1, c2, cl, w, s);
+    else if (lanes == 32) run_filter<32, 4>(b, m, c1, c2, cl, w, s);
+    else run_filter<kFilterLanes, kFilterBatch>(b, m, c1, c2, cl, w, s);
+  } else if (close) {
+    k_close_request<<<1, 1, 0, s>>>(w.flags);
   }
   return hipGetLastError();
 }
@@ -1612,6 +1735,13 @@ static void run_single(const DevModel& m, int N, int c1, int c2, const int* type
                                                                               e1);
       return;
     }
+    if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, true, kAnnMfmaSplit>()) {
+      if (mode == kAnnMfmaSplit) {
+        run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaSplit>>(m, N, c1, c2, type, pe, acc, w, s,
+                                                                                  e0, e1);
+        return;
+      }
+    }
   }
   run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
 }
@@ -1631,6 +1761,13 @@ static void run_multi(const DevModel& m, int N, int c1, int c2, const int* type,
       run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfmaBf16, true>>(m, N, c1, c2, type, pe, acc,
                                                                                      w, s, e0, e1);
       return;
+    }
+    if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, false, kAnnMfmaSplit>()) {
+      if (mode == kAnnMfmaSplit) {
+        run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfmaSplit, true>>(m, N, c1, c2, type, pe,
+                                                                                         acc, w, s, e0, e1);
+        return;
+      }
     }
   }
   if (m.T <= kTypesLds)
@@ -1779,9 +1916,9 @@ hipError_t launch_ensemble_scale(int kind, int N, double coupling, double t_targ
 
 hipError_t launch_md_first(const DevBox& b, int N, int T, double dt, const double* mass, double* pos, double* vel,
                            const double* force, const double* factor, const int* type, bool forced, DevWork& w,
-                           hipStream_t s, double* unwrapped)
+                           hipStream_t s, double* unwrapped, double half_skin)
 {
-  const double half = 0.5 * kSkin;
+  const double half = half_skin > 0.0 ? half_skin : 0.5 * kSkin;
   if (N > 0)
     k_md_first<<<nblk(N), kBlock, 0, s>>>(b, N, T, dt, mass, pos, vel, force, factor, type, w.x0, half * half,
                                           forced ? 1 : 0, w.xq, w.flags, unwrapped);
@@ -1990,18 +2127,18 @@ __global__ void k_serve_assemble(int R, const ServeSlots* __restrict__ req, Serv
 __global__ void k_serve_extract(int R, const ServeOut* __restrict__ req, long long no, int Nt,
                                 const double* __restrict__ pos, const double* __restrict__ vel,
                                 const double* __restrict__ force, const double* __restrict__ pe,
-                                double* __restrict__ out)
+                                const double* __restrict__ vir, double* __restrict__ out)
 {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= no) return;
   int lo = 0, hi = R - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (req[mid].out / 10 <= k) lo = mid;
+    if (req[mid].out / kServeOutDoubles <= k) lo = mid;
     else hi = mid - 1;
   }
   const ServeOut& q = req[lo];
-  const int a = (int)(k - q.out / 10), n = q.n, i = q.slot0 + a;
+  const int a = (int)(k - q.out / kServeOutDoubles), n = q.n, i = q.slot0 + a;
   double* blk = out + q.out;
   blk[a] = pe[i];
   double x[3], f[3];
@@ -2011,6 +2148,8 @@ __global__ void k_serve_extract(int R, const ServeOut* __restrict__ req, long lo
     blk[n + c * n + a] = force[c * Nt + i];
     blk[7 * n + c * n + a] = vel[c * Nt + i];
   }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) blk[10 * n + c * n + a] = vir[(size_t)c * Nt + i];
   // back into the request's box: fractional coordinates, wrapped where periodic
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -2044,12 +2183,12 @@ hipError_t launch_serve_assemble(int R, const ServeSlots* req, const ServeLayout
 }
 
 hipError_t launch_serve_extract(int R, const ServeOut* req, long long no, int Nt, const double* pos,
-                                const double* vel, const double* force, const double* pe, double* out,
-                                hipStream_t s)
+                                const double* vel, const double* force, const double* pe, const double* vir,
+                                double* out, hipStream_t s)
 {
   if (R > 0 && no > 0)
     k_serve_extract<<<(unsigned)((no + kBlock - 1) / kBlock), kBlock, 0, s>>>(R, req, no, Nt, pos, vel, force, pe,
-                                                                             out);
+                                                                             vir, out);
   return hipGetLastError();
 }
 

@@ -134,15 +134,18 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
 
 // Force evaluation (per step)
 // lanes: lanes per centre, 8, 16 (default) or 32
+// close: this launch closes the step's rebuild request (exactly one filter
+// launch of an evaluation does)
 hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s,
-                         int lanes = 16);
+                         int lanes = 16, bool close = true);
 // Sizes (nR, kR, nA, kA) of the exactly instantiated descriptor that a model
 // without an exact instance of its own is padded to (zero coefficients and
 // ANN weights on the added entries, 3-body order 4 with both many-body
 // blocks); false where the model has an exact instance, or exceeds every
 // padded size and runs on the generic instance.
 bool descriptor_bucket(const DevModel& m, int dims[4]);
-// ann: 0 FP32 on the vector ALUs, 1 FP32 on the matrix cores, 2 BF16 operands
+// ann: 0 FP32 on the vector ALUs, 1 FP32 on the matrix cores, 2 BF16 operands,
+// 3 split-BF16 operands (hi + lo, three MFMAs per product)
 // on the matrix cores; modes 1 and 2 apply to models of at most 2 types with
 // at most 64 neurons (every size but the generic fallback), the vector path
 // otherwise.
@@ -181,7 +184,7 @@ hipError_t launch_check_scaled(int n, double factor, const double* data, int* fl
 // thermostat into factor[0] (see launch_ensemble_scale for kind).
 hipError_t launch_md_first(const DevBox& b, int N, int T, double dt, const double* mass, double* pos, double* vel,
                            const double* force, const double* factor, const int* type, bool forced, DevWork& w,
-                           hipStream_t s, double* unwrapped = nullptr);
+                           hipStream_t s, double* unwrapped = nullptr, double half_skin = -1.0);
 hipError_t launch_md_second(int N, double dt, const double* mass, double* vel, const double* force, const double* pe,
                             const double* virial, bool thermo, DevWork& tw, hipStream_t s);
 hipError_t launch_thermo_final(int N, double volume, int kind, double coupling, double t_target,
@@ -241,18 +244,20 @@ hipError_t launch_serve_assemble(int R, const ServeSlots* req, const ServeLayout
                                  int* atom_req, hipStream_t s);
 // One finished request for launch_serve_extract: its owned slots, the
 // offset of its block in the output, its placement translation and its box.
+constexpr int kServeOutDoubles = 19;  // result doubles per atom
 struct ServeOut {
   int slot0, n;
-  long long out;  // first double of its block of 10 n
+  long long out;  // first double of its block of kServeOutDoubles n
   int pbc[3];
   double trans[3];
   double h[9], hi[9];
 };
 // Writes, per finished request, the block [potential n | force 3n | position
-// 3n | velocity 3n] (SoA, position wrapped into the request's own box).
+// 3n | velocity 3n | virial 9n] (SoA, position wrapped into the request's own
+// box, virial in the order xx yy zz xy xz yz yx zx zy).
 hipError_t launch_serve_extract(int R, const ServeOut* req, long long no, int Nt, const double* pos,
-                                const double* vel, const double* force, const double* pe, double* out,
-                                hipStream_t s);
+                                const double* vel, const double* force, const double* pe, const double* vir,
+                                double* out, hipStream_t s);
 
 // Number of thermo block partials launch_thermo uses for N atoms.
 int thermo_blocks(int N);

@@ -73,16 +73,6 @@ void Server::DBuf<T>::release()
   cap = 0;
 }
 
-void Server::stage(size_t bytes)
-{
-  if (bytes <= h_stage_bytes_) return;
-  if (h_stage_) (void)hipHostFree(h_stage_);
-  h_stage_ = nullptr;
-  h_stage_bytes_ = 0;
-  check(hipHostMalloc((void**)&h_stage_, bytes, hipHostMallocDefault), "pinned staging");
-  h_stage_bytes_ = bytes;
-}
-
 // Device buffers for request packs, reused by size class (powers of two from
 // 4 KiB): freeing device memory would synchronise the device under the
 // serving loop, so buffers are only freed with the server.
@@ -137,12 +127,14 @@ void* Server::pin_get(size_t bytes, size_t* cap)
 
 struct Server::Request {
   long long id = 0;
+  int model = 0;
   Box box{};
   double hi[9] = {};
   int n = 0;
   int ng = 0;  // periodic images within rc + skin + margin of the box
   int steps = 0;
   double dt = 0;
+  long long bytes = 0;  // device bytes while in flight (slots x the model's bytes per slot)
   // the request's atoms and images on the device until its first layout
   // (ServeSlots pack layout), from the server's pack pool
   void* pack = nullptr;
@@ -157,16 +149,94 @@ struct Server::Request {
   bool live = true;
   double t_submit = 0, t_done = 0;
   std::string error;
-  // placement in the current layout
+  // placement in the current layout of its model's lane
   double trans[3] = {0, 0, 0};
   int slot0 = -1, gslot0 = -1;  // first owned / ghost slot in the layout
   bool placed = false;
-  // results: block [potential n | force 3n | position 3n | velocity 3n]
+  // results: block of kServeOutDoubles n doubles (kernels.hpp)
   std::shared_ptr<ResultBlock> res;
   long long res_off = 0;
-  long long ghosts() const { return ng; }
-  long long footprint() const { return n + ghosts(); }
+  long long footprint() const { return (long long)n + ng; }
 };
+
+// One model's batch: its engine, stream, layout arrays and captured force
+// launches.  Worker thread only, except `active` membership (under mu_).
+struct Server::Lane {
+  NepEngine* nep = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<std::shared_ptr<Request>> active;
+  bool dirty = false;  // membership changed since the last layout
+  bool any_md = false;  // requests taking an MD step in the frame being evaluated
+  int n_owned = 0, n_ghost = 0, n_total = 0;
+  Box batch_box{};
+  DBuf<double> pos, vel, force, pe, vir, mass;
+  DBuf<int> type;
+  DBuf<int> owner;     // slot -> owned slot it is an image of (itself for owned slots)
+  DBuf<int> image;     // slot -> image code (image_code(0, 0, 0) for owned slots)
+  DBuf<int> gstart;    // owned slot -> first ghost slot of its images [n_owned + 1]
+  DBuf<double> g_shift;  // ghost k -> box shift [3]
+  // the next layout's per-slot arrays, assembled on the device and swapped in
+  DBuf<double> pos2, vel2, force2, mass2, g_shift2;
+  DBuf<int> type2, owner2, image2, gstart2;
+  DBuf<int> atom_req;    // owned slot -> active request index
+  DBuf<int> req_md;      // active request -> 1 when it takes an MD step this iteration
+  DBuf<double> req_dt;   // active request -> dt (natural units)
+  DBuf<ServeSlots> slots;  // layout table (one row per active request)
+  DBuf<ServeOut> outq;     // finished requests (extract table)
+  DBuf<double> out;        // finished requests' result blocks
+  char* h_stage = nullptr;  // pinned staging for this lane's uploads
+  size_t h_stage_bytes = 0;
+  bool forced_rebuild = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+
+  void stage(size_t bytes)
+  {
+    if (bytes <= h_stage_bytes) return;
+    if (h_stage) (void)hipHostFree(h_stage);
+    h_stage = nullptr;
+    h_stage_bytes = 0;
+    check(hipHostMalloc((void**)&h_stage, bytes, hipHostMallocDefault), "pinned staging");
+    h_stage_bytes = bytes;
+  }
+  bool live() const
+  {
+    for (auto& r : active)
+      if (r->live) return true;
+    return false;
+  }
+  ImageMap map() const
+  {
+    ImageMap m;
+    m.owner = owner.p;
+    m.image = image.p;
+    m.gstart = gstart.p;
+    return m;
+  }
+  void release()
+  {
+    for (DBuf<double>* b : {&pos, &vel, &force, &pe, &vir, &mass, &pos2, &vel2, &force2, &mass2, &req_dt, &g_shift,
+                            &g_shift2, &out})
+      b->release();
+    for (DBuf<int>* b : {&type, &type2, &atom_req, &req_md, &owner, &owner2, &image, &image2, &gstart, &gstart2})
+      b->release();
+    slots.release();
+    outq.release();
+    if (h_stage) (void)hipHostFree(h_stage);
+    h_stage = nullptr;
+  }
+};
+
+// device bytes per slot of a lane's layout: position, velocity and force
+// (3 doubles each) and mass, type, owner, image and ghost shift, all
+// double-buffered; potential, virial, the atom->request index and the
+// result block of a finished atom
+constexpr double kLayoutSlotBytes = 2 * (9 * 8 + 8 + 3 * 4 + 3 * 8) + 8 + 9 * 8 + 4 + kServeOutDoubles * 8;
+
+double Server::slot_bytes(int model) const
+{
+  return lanes_[(size_t)model]->nep->bytes_per_slot() + kLayoutSlotBytes;
+}
 
 void Server::pin_put(void* p, size_t cap)
 {
@@ -187,17 +257,47 @@ void Server::release_upload(Request& r)
   r.hpack = nullptr;
 }
 
-Server::Server(NepEngine* nep, const ServeConfig& cfg) : nep_(nep), cfg_(cfg)
+Server::Server(const std::vector<NepEngine*>& models, const ServeConfig& cfg) : cfg_(cfg)
 {
-  if (!nep) throw std::invalid_argument("null potential");
+  if (models.empty()) throw std::invalid_argument("a server needs at least one potential");
+  for (size_t a = 0; a < models.size(); ++a) {
+    if (!models[a]) throw std::invalid_argument("null potential");
+    for (size_t b = 0; b < a; ++b)
+      if (models[a] == models[b]) throw std::invalid_argument("a potential handle is given twice");
+  }
   if (cfg_.max_atoms <= 0 || cfg_.max_requests <= 0) throw std::invalid_argument("bad serving limits");
   if (cfg_.max_atoms >= (1LL << 30)) throw std::invalid_argument("max_atoms must stay below 2^30");
-  // the batch grows request by request: size the engine once, up front,
-  // instead of reallocating (and synchronising the device) as it grows
-  nep_->reserve((int)std::min(cfg_.max_atoms, cfg_.reserve_atoms));
-  nep_->reserve_cells(1LL << 23);
-  check(hipStreamCreate(&stream_), "serving stream");
-  check(hipStreamCreate(&upload_), "serving upload stream");
+  if (cfg_.hbm_bytes < 0) throw std::invalid_argument("negative HBM budget");
+  budget_ = cfg_.hbm_bytes;
+  if (budget_ == 0) {
+    size_t free_b = 0, total_b = 0;
+    check(hipMemGetInfo(&free_b, &total_b), "device memory info");
+    budget_ = (long long)(free_b / 2);
+  }
+  st_.hbm_budget = budget_;
+  try {
+    for (NepEngine* m : models) {
+      auto L = std::make_unique<Lane>();
+      L->nep = m;
+      lanes_.push_back(std::move(L));
+      Lane& l = *lanes_.back();
+      // each batch grows request by request: size its engine once, up front,
+      // for what the budget allows, instead of reallocating (and
+      // synchronising the device) as it grows
+      const long long fit = (long long)(budget_ / slot_bytes((int)lanes_.size() - 1));
+      const long long cap = std::min({cfg_.max_atoms, cfg_.reserve_atoms, fit});
+      m->reserve((int)std::max<long long>(cap, 1));
+      m->reserve_cells(1LL << 23);
+      check(hipStreamCreate(&l.stream), "serving stream");
+    }
+    check(hipStreamCreate(&upload_), "serving upload stream");
+  } catch (...) {
+    for (auto& L : lanes_) {
+      if (L->stream) (void)hipStreamDestroy(L->stream);
+      L->release();
+    }
+    throw;
+  }
   thread_ = std::thread([this] { worker(); });
 }
 
@@ -209,37 +309,35 @@ Server::~Server()
   }
   cv_.notify_all();
   if (thread_.joinable()) thread_.join();
-  if (stream_) (void)hipStreamSynchronize(stream_);
-  drop_graph();
-  for (DBuf<double>* b : {&pos_, &vel_, &force_, &pe_, &vir_, &mass_, &pos2_, &vel2_, &force2_, &mass2_, &req_dt_,
-                          &g_shift_, &g_shift2_, &out_})
-    b->release();
-  for (DBuf<int>* b : {&type_, &type2_, &atom_req_, &req_md_, &owner_, &owner2_, &image_, &image2_, &gstart_,
-                       &gstart2_})
-    b->release();
-  slots_.release();
-  outq_.release();
+  for (auto& L : lanes_) {
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    drop_graph(*L);
+    L->release();
+  }
   if (upload_) (void)hipStreamSynchronize(upload_);
   for (auto& kv : all_) release_upload(*kv.second);  // every request, waiting ones included
   // dropping the requests returns their result blocks to the pinned pool
   waiting_.clear();
-  active_.clear();
+  for (auto& L : lanes_) L->active.clear();
   all_.clear();
   for (auto& kv : pack_free_) (void)hipFree(kv.second);
   for (auto& kv : pin_free_) (void)hipHostFree(kv.second);
   if (upload_) (void)hipStreamDestroy(upload_);
-  if (h_stage_) (void)hipHostFree(h_stage_);
-  if (stream_) (void)hipStreamDestroy(stream_);
+  for (auto& L : lanes_)
+    if (L->stream) (void)hipStreamDestroy(L->stream);
 }
 
-long long Server::submit(const Box& box, int N, const int* type, const double* mass, const double* pos,
+long long Server::submit(int model, const Box& box, int N, const int* type, const double* mass, const double* pos,
                          const double* vel, int steps, double dt_fs)
 {
+  if (model < 0 || model >= (int)lanes_.size()) throw std::invalid_argument("model index out of range");
   if (N <= 0 || !type || !mass || !pos) throw std::invalid_argument("a request needs atoms, types, masses, positions");
   if (steps < 0) throw std::invalid_argument("negative step count");
   if (steps > 0 && !(dt_fs > 0.0)) throw std::invalid_argument("MD requests need dt_fs > 0");
   const double t_call = now_s();
+  NepEngine* nep = lanes_[(size_t)model]->nep;
   auto r = std::make_shared<Request>();
+  r->model = model;
   r->box = box;
   const double vol = box_volume(box.h);
   if (!(std::fabs(vol) > 0.0) || !std::isfinite(vol)) throw std::invalid_argument("box has no volume");
@@ -247,14 +345,14 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
   r->n = N;
   r->steps = steps;
   r->dt = dt_fs / kTimeUnit;
-  const int T = nep_->model().num_types;
+  const int T = nep->model().num_types;
   for (int i = 0; i < N; ++i) {
     if (type[i] < 0 || type[i] >= T) throw std::invalid_argument("atom type out of range");
     if (!(mass[i] > 0.0)) throw std::invalid_argument("masses must be positive");
   }
   std::vector<double> wpos(3 * (size_t)N);  // wrapped into the box, SoA
   // wrap into the box and collect the periodic images of the shell
-  const double rc = nep_->model().rcR_max;
+  const double rc = nep->model().rcR_max;
   const double d1 = rc + kSkin + cfg_.margin;
   double th[3];
   thickness(box.h, th);
@@ -324,11 +422,15 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
                          wpos[2 * (size_t)N + o] + g_shift[3 * g + 2]};
     extend(x);
   }
+  const size_t doff = serve_pack_double_offset(N, NG), bytes = serve_pack_bytes(N, NG);
+  r->bytes = (long long)std::ceil(r->footprint() * slot_bytes(model)) + (long long)bytes;
   if (r->footprint() > cfg_.max_atoms)
     throw std::invalid_argument("request (with its periodic shell) exceeds the serving atom budget");
+  if (r->bytes > budget_)
+    throw std::invalid_argument("request (with its periodic shell) needs " + std::to_string(r->bytes) +
+                                " device bytes, more than the serving HBM budget of " + std::to_string(budget_));
   // the pack: built here and uploaded on the submitting thread, so the
   // serving loop only copies it device-to-device when it places the request
-  const size_t doff = serve_pack_double_offset(N, NG), bytes = serve_pack_bytes(N, NG);
   r->hpack = pin_get(bytes, &r->hpack_cap);
   char* host = static_cast<char*>(r->hpack);
   int* pi = reinterpret_cast<int*>(host);
@@ -347,7 +449,7 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
       pd[4 * (size_t)N + 3 * (size_t)a + d] = vel ? vel[(size_t)d * N + a] : 0.0;
     }
   std::copy(g_shift.begin(), g_shift.end(), pd + 7 * (size_t)N);
-  // asynchronous upload: the serving stream waits for `ready` before it
+  // asynchronous upload: the lane's stream waits for `ready` before it
   // places the request
   r->pack = pack_get(bytes, &r->pack_cap);
   hipError_t e = hipEventCreateWithFlags(&r->ready, hipEventDisableTiming);
@@ -373,7 +475,7 @@ long long Server::submit(const Box& box, int N, const int* type, const double* m
 }
 
 bool Server::poll(long long id, bool wait, double* potential, double* force, double* position, double* velocity,
-                  double* latency_ms)
+                  double* virial, double* latency_ms)
 {
   std::unique_lock<std::mutex> lk(mu_);
   auto it = all_.find(id);
@@ -391,6 +493,7 @@ bool Server::poll(long long id, bool wait, double* potential, double* force, dou
   if (force) std::copy(blk + n, blk + 4 * n, force);
   if (position) std::copy(blk + 4 * n, blk + 7 * n, position);
   if (velocity) std::copy(blk + 7 * n, blk + 10 * n, velocity);
+  if (virial) std::copy(blk + 10 * n, blk + 19 * n, virial);
   if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
   return true;
 }
@@ -436,8 +539,8 @@ void Server::worker()
       cv_.wait(lk, [&] {
         if (stop_ || !error_.empty()) return true;
         if (!waiting_.empty()) return true;
-        for (auto& r : active_)
-          if (r->live) return true;
+        for (auto& L : lanes_)
+          if (L->live()) return true;
         return false;
       });
       if (stop_ || !error_.empty()) return;
@@ -445,19 +548,28 @@ void Server::worker()
     }
     try {
       const double t0 = now_s();
-      if (dirty_) relayout();
-      bool any = false;
-      for (auto& r : active_) any = any || r->live;
+      for (auto& L : lanes_)
+        if (L->dirty) relayout(*L);
+      std::vector<Lane*> run;
+      long long slots = 0;
+      for (auto& L : lanes_) {
+        if (L->live()) run.push_back(L.get());
+        slots += L->n_total;
+      }
       const double t1 = now_s();
-      if (any) iterate();
+      // every lane's frame is enqueued on its own stream before any is
+      // waited for, so the lanes' kernels overlap on the device
+      for (Lane* L : run) enqueue_frame(*L);
+      for (Lane* L : run) finish_frame(*L);
       const double t2 = now_s();
-      if (any) retire();
+      for (Lane* L : run) retire(*L);
       const double t3 = now_s();
       std::lock_guard<std::mutex> lk(mu_);
       st_.relayout_s += t1 - t0;
       st_.iterate_s += t2 - t1;
       st_.retire_s += t3 - t2;
-      st_.max_slots = std::max<long long>(st_.max_slots, n_total_);
+      if (!run.empty()) st_.iterations++;
+      st_.max_slots = std::max<long long>(st_.max_slots, slots);
     } catch (const std::exception& e) {
       std::lock_guard<std::mutex> lk(mu_);
       error_ = e.what();
@@ -467,87 +579,124 @@ void Server::worker()
   }
 }
 
-// Called with mu_ held.  Admits waiting requests in FIFO order while the
-// batch (live requests + new ones) fits the budget; a new layout is made when
-// the batch is empty, or when the atoms that would be admitted plus those of
-// finished requests still in the layout reach a quarter of the layout, so
-// that the layout (and the Verlet rebuild it forces) is not redone for every
-// single arrival.
-bool Server::admit()
+// Called with mu_ held.  Admits waiting requests in submission order while
+// the device bytes of the live requests plus the new ones stay within the
+// HBM budget (and the slot and request limits); the first request that does
+// not fit stops admission, so large requests are not starved.  A lane takes
+// its admitted requests into a new layout when it is empty, or when those
+// requests' slots plus the slots of its finished requests still in the
+// layout reach a quarter of the layout, so that a layout (and the Verlet
+// rebuild it forces) is not redone for every single arrival; requests a lane
+// declines stay waiting without holding budget.
+void Server::admit()
 {
-  long long live_atoms = 0, dead_atoms = 0;
+  long long live_bytes = 0, live_slots = 0;
   int live = 0;
-  for (auto& r : active_) {
-    if (r->live) {
-      live_atoms += r->footprint();
-      ++live;
-    } else {
-      dead_atoms += r->footprint();
+  const size_t nl = lanes_.size();
+  std::vector<long long> lane_live(nl, 0), lane_dead(nl, 0);
+  for (size_t l = 0; l < nl; ++l)
+    for (auto& r : lanes_[l]->active) {
+      if (r->live) {
+        live_bytes += r->bytes;
+        live_slots += r->footprint();
+        lane_live[l] += r->footprint();
+        ++live;
+      } else {
+        lane_dead[l] += r->footprint();
+      }
     }
+  std::vector<std::vector<size_t>> cand(nl);
+  std::vector<long long> add(nl, 0);
+  long long add_bytes = 0, add_slots = 0;
+  int added = 0;
+  for (size_t k = 0; k < waiting_.size(); ++k) {
+    const Request& r = *waiting_[k];
+    if (live_bytes + add_bytes + r.bytes > budget_ || live_slots + add_slots + r.footprint() > cfg_.max_atoms ||
+        live + added + 1 > cfg_.max_requests)
+      break;
+    add_bytes += r.bytes;
+    add_slots += r.footprint();
+    ++added;
+    cand[(size_t)r.model].push_back(k);
+    add[(size_t)r.model] += r.footprint();
   }
-  long long add_atoms = 0;
-  size_t k = 0;
-  for (; k < waiting_.size(); ++k) {
-    const long long f = waiting_[k]->footprint();
-    if (live_atoms + add_atoms + f > cfg_.max_atoms || live + (int)k + 1 > cfg_.max_requests) break;
-    add_atoms += f;
+  std::vector<char> take(waiting_.size(), 0);
+  bool any = false;
+  for (size_t l = 0; l < nl; ++l) {
+    if (cand[l].empty()) continue;
+    Lane& L = *lanes_[l];
+    const bool empty = lane_live[l] == 0;
+    if (!empty && 4 * (add[l] + lane_dead[l]) < lane_live[l] + lane_dead[l]) continue;
+    std::vector<std::shared_ptr<Request>> next;
+    for (auto& r : L.active)
+      if (r->live) next.push_back(r);
+    for (size_t k : cand[l]) {
+      next.push_back(waiting_[k]);
+      take[k] = 1;
+    }
+    L.active.swap(next);
+    L.dirty = true;
+    any = true;
   }
-  if (k == 0) return false;
-  const long long layout_atoms = live_atoms + dead_atoms;
-  const bool empty = live == 0;
-  if (!empty && 4 * (add_atoms + dead_atoms) < layout_atoms) return false;
-  std::vector<std::shared_ptr<Request>> next;
-  for (auto& r : active_)
-    if (r->live) next.push_back(r);
-  for (size_t j = 0; j < k; ++j) next.push_back(waiting_[j]);
-  waiting_.erase(waiting_.begin(), waiting_.begin() + (long)k);
-  active_.swap(next);
-  dirty_ = true;
-  return true;
+  if (!any) return;
+  std::deque<std::shared_ptr<Request>> rest;
+  for (size_t k = 0; k < waiting_.size(); ++k)
+    if (!take[k]) rest.push_back(waiting_[k]);
+  waiting_.swap(rest);
+  long long in_flight = 0;
+  for (auto& L : lanes_)
+    for (auto& r : L->active)
+      if (r->live) in_flight += r->bytes;
+  st_.max_bytes_in_flight = std::max(st_.max_bytes_in_flight, in_flight);
 }
 
-void Server::relayout()
+void Server::relayout(Lane& L)
 {
-  drop_graph();  // the captured launches take the old layout's arrays
+  drop_graph(L);  // the captured launches take the old layout's arrays
+  std::vector<std::shared_ptr<Request>> act;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    act = L.active;
+  }
   // placement: every request's extended region in its own cell of a cubic
   // grid of equal slots, separated by more than the Verlet radius
-  const double gap = nep_->model().rcR_max + kSkin + 2.0 + 2.0 * cfg_.margin;
+  const double gap = L.nep->model().rcR_max + kSkin + 2.0 + 2.0 * cfg_.margin;
   double slot[3] = {0, 0, 0};
-  for (auto& r : active_)
+  for (auto& r : act)
     for (int d = 0; d < 3; ++d) slot[d] = std::max(slot[d], r->hi_ext[d] - r->lo[d] + gap);
-  const int R = (int)active_.size();
+  const int R = (int)act.size();
   int side = 1;
   while ((long long)side * side * side < R) ++side;
   long long n_owned = 0, n_ghost = 0;
-  for (auto& r : active_) {
+  for (auto& r : act) {
     n_owned += r->n;
-    n_ghost += r->ghosts();
+    n_ghost += r->ng;
   }
   const long long Nt = n_owned + n_ghost;
   if (Nt >= (1LL << 30)) throw std::runtime_error("serving batch too large");
   // the new layout's arrays (double-buffered: the previous layout is the source)
-  pos2_.need(3 * Nt, "serve positions");
-  vel2_.need(3 * Nt, "serve velocities");
-  force2_.need(3 * Nt, "serve forces");
-  mass2_.need(Nt, "serve masses");
-  type2_.need(Nt, "serve types");
-  owner2_.need(Nt, "serve slot owners");
-  image2_.need(Nt, "serve slot images");
-  gstart2_.need(n_owned + 1, "serve ghost ranges");
-  g_shift2_.need(3 * n_ghost, "serve ghost shifts");
-  pe_.need(Nt, "serve potential");
-  vir_.need(9 * Nt, "serve virial");
-  atom_req_.need(n_owned, "serve atom->request");
-  req_md_.need(R, "serve request flags");
-  req_dt_.need(R, "serve request dt");
-  slots_.need(R, "serve layout table");
+  L.pos2.need(3 * Nt, "serve positions");
+  L.vel2.need(3 * Nt, "serve velocities");
+  L.force2.need(3 * Nt, "serve forces");
+  L.mass2.need(Nt, "serve masses");
+  L.type2.need(Nt, "serve types");
+  L.owner2.need(Nt, "serve slot owners");
+  L.image2.need(Nt, "serve slot images");
+  L.gstart2.need(n_owned + 1, "serve ghost ranges");
+  L.g_shift2.need(3 * n_ghost, "serve ghost shifts");
+  L.pe.need(Nt, "serve potential");
+  L.vir.need(9 * Nt, "serve virial");
+  L.atom_req.need(n_owned, "serve atom->request");
+  L.req_md.need(R, "serve request flags");
+  L.req_dt.need(R, "serve request dt");
+  L.slots.need(R, "serve layout table");
   // host side: one table row per request; new requests bring their packs
   std::vector<ServeSlots> tab((size_t)R);
   std::vector<double> h_dt((size_t)std::max(R, 1));
   std::vector<Request*> consumed;
   long long o = 0, gk = n_owned;
   for (int q = 0; q < R; ++q) {
-    Request& r = *active_[q];
+    Request& r = *act[q];
     const int g3[3] = {q % side, (q / side) % side, q / (side * side)};
     double tnew[3];
     for (int d = 0; d < 3; ++d) tnew[d] = g3[d] * slot[d] + 0.5 * gap - r.lo[d];
@@ -566,7 +715,7 @@ void Server::relayout()
     } else {
       if (!r.pack) throw std::logic_error("serving: a new request has no pack");
       t.old_o = t.old_g = -1;
-      check(hipStreamWaitEvent(stream_, r.ready, 0), "serve wait for upload");
+      check(hipStreamWaitEvent(L.stream, r.ready, 0), "serve wait for upload");
       t.pi = static_cast<const int*>(r.pack);
       t.pd = reinterpret_cast<const double*>(static_cast<const char*>(r.pack) + serve_pack_double_offset(r.n, r.ng));
       for (int d = 0; d < 3; ++d) t.dtr[d] = tnew[d];
@@ -579,188 +728,193 @@ void Server::relayout()
     o += r.n;
     gk += r.ng;
   }
-  // uploads through one pinned staging buffer (the stream is idle here)
+  // uploads through the lane's pinned staging buffer (its stream is idle here)
   const size_t bytes = tab.size() * sizeof(ServeSlots) + h_dt.size() * 8 + 4 * 16;
-  stage(bytes);
-  char* hs = h_stage_;
+  L.stage(bytes);
+  char* hs = L.h_stage;
   size_t off = 0;
   auto up = [&](void* dev, const void* src, size_t n) {
     if (n == 0) return;
     off = (off + 15) & ~size_t(15);
     std::memcpy(hs + off, src, n);
-    check(hipMemcpyAsync(dev, hs + off, n, hipMemcpyHostToDevice, stream_), "serve upload");
+    check(hipMemcpyAsync(dev, hs + off, n, hipMemcpyHostToDevice, L.stream), "serve upload");
     off += n;
   };
-  up(slots_.p, tab.data(), tab.size() * sizeof(ServeSlots));
-  up(req_dt_.p, h_dt.data(), h_dt.size() * 8);
+  up(L.slots.p, tab.data(), tab.size() * sizeof(ServeSlots));
+  up(L.req_dt.p, h_dt.data(), h_dt.size() * 8);
   ServeLayout from, to;
-  from.Nt = n_total_;
-  from.n_owned = n_owned_;
-  from.pos = pos_.p;
-  from.vel = vel_.p;
-  from.force = force_.p;
-  from.mass = mass_.p;
-  from.gshift = g_shift_.p;
-  from.type = type_.p;
-  from.owner = owner_.p;
-  from.image = image_.p;
-  from.gstart = gstart_.p;
+  from.Nt = L.n_total;
+  from.n_owned = L.n_owned;
+  from.pos = L.pos.p;
+  from.vel = L.vel.p;
+  from.force = L.force.p;
+  from.mass = L.mass.p;
+  from.gshift = L.g_shift.p;
+  from.type = L.type.p;
+  from.owner = L.owner.p;
+  from.image = L.image.p;
+  from.gstart = L.gstart.p;
   to.Nt = (int)Nt;
   to.n_owned = (int)n_owned;
-  to.pos = pos2_.p;
-  to.vel = vel2_.p;
-  to.force = force2_.p;
-  to.mass = mass2_.p;
-  to.gshift = g_shift2_.p;
-  to.type = type2_.p;
-  to.owner = owner2_.p;
-  to.image = image2_.p;
-  to.gstart = gstart2_.p;
-  check(launch_serve_assemble(R, slots_.p, from, to, atom_req_.p, stream_), "serve assemble");
-  check(hipStreamSynchronize(stream_), "serve relayout");
+  to.pos = L.pos2.p;
+  to.vel = L.vel2.p;
+  to.force = L.force2.p;
+  to.mass = L.mass2.p;
+  to.gshift = L.g_shift2.p;
+  to.type = L.type2.p;
+  to.owner = L.owner2.p;
+  to.image = L.image2.p;
+  to.gstart = L.gstart2.p;
+  check(launch_serve_assemble(R, L.slots.p, from, to, L.atom_req.p, L.stream), "serve assemble");
+  check(hipStreamSynchronize(L.stream), "serve relayout");
   for (Request* r : consumed) release_upload(*r);
-  std::swap(pos_, pos2_);
-  std::swap(vel_, vel2_);
-  std::swap(force_, force2_);
-  std::swap(mass_, mass2_);
-  std::swap(type_, type2_);
-  std::swap(owner_, owner2_);
-  std::swap(image_, image2_);
-  std::swap(gstart_, gstart2_);
-  std::swap(g_shift_, g_shift2_);
-  n_owned_ = (int)n_owned;
-  n_ghost_ = (int)n_ghost;
-  n_total_ = (int)Nt;
-  std::memset(&batch_box_, 0, sizeof(batch_box_));
-  for (int d = 0; d < 3; ++d) batch_box_.h[4 * d] = side * slot[d];
-  batch_box_.pbc[0] = batch_box_.pbc[1] = batch_box_.pbc[2] = 0;
-  forced_rebuild_ = true;
-  dirty_ = false;
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    st_.relayouts++;
-  }
+  std::swap(L.pos, L.pos2);
+  std::swap(L.vel, L.vel2);
+  std::swap(L.force, L.force2);
+  std::swap(L.mass, L.mass2);
+  std::swap(L.type, L.type2);
+  std::swap(L.owner, L.owner2);
+  std::swap(L.image, L.image2);
+  std::swap(L.gstart, L.gstart2);
+  std::swap(L.g_shift, L.g_shift2);
+  L.n_owned = (int)n_owned;
+  L.n_ghost = (int)n_ghost;
+  L.n_total = (int)Nt;
+  std::memset(&L.batch_box, 0, sizeof(L.batch_box));
+  for (int d = 0; d < 3; ++d) L.batch_box.h[4 * d] = side * slot[d];
+  L.batch_box.pbc[0] = L.batch_box.pbc[1] = L.batch_box.pbc[2] = 0;
+  L.forced_rebuild = true;
+  L.dirty = false;
+  std::lock_guard<std::mutex> lk(mu_);
+  st_.relayouts++;
 }
 
-void Server::iterate()
+// Enqueues one frame of a lane on its stream, without waiting: the first
+// half-kick of the requests that take an MD step, then the force launches
+// (the captured graph, or the eager launches).
+void Server::enqueue_frame(Lane& L)
 {
-  const int R = (int)active_.size();
+  const int R = (int)L.active.size();
   // which requests take an MD step this iteration
   std::vector<int> md((size_t)std::max(R, 1), 0);
-  bool any_md = false;
-  for (int q = 0; q < R; ++q) {
-    md[q] = (active_[q]->live && active_[q]->state == 1) ? 1 : 0;
-    any_md = any_md || md[q];
+  L.any_md = false;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (int q = 0; q < R; ++q) {
+      md[q] = (L.active[q]->live && L.active[q]->state == 1) ? 1 : 0;
+      L.any_md = L.any_md || md[q];
+    }
   }
-  stage(md.size() * 4);
-  std::memcpy(h_stage_, md.data(), md.size() * 4);
-  check(hipMemcpyAsync(req_md_.p, h_stage_, md.size() * 4, hipMemcpyHostToDevice, stream_), "serve flags");
-  if (any_md)
-    check(launch_serve_kick(1, n_owned_, n_total_, atom_req_.p, req_md_.p, req_dt_.p, mass_.p, pos_.p, vel_.p,
-                            force_.p, stream_),
+  L.stage(md.size() * 4);
+  std::memcpy(L.h_stage, md.data(), md.size() * 4);
+  check(hipMemcpyAsync(L.req_md.p, L.h_stage, md.size() * 4, hipMemcpyHostToDevice, L.stream), "serve flags");
+  if (L.any_md)
+    check(launch_serve_kick(1, L.n_owned, L.n_total, L.atom_req.p, L.req_md.p, L.req_dt.p, L.mass.p, L.pos.p, L.vel.p,
+                            L.force.p, L.stream),
           "serve kick 1");
   const unsigned flags = cfg_.flags & ~3u;
-  if (!nep_->tuning().tuned && n_owned_ >= kTuneMinAtoms) {
+  const ImageMap map = L.map();
+  if (!L.nep->tuning().tuned && L.n_owned >= kTuneMinAtoms) {
     // tile shapes picked once, on the first batch large enough to tell them apart
-    ImageMap map;
-    map.owner = owner_.p;
-    map.image = image_.p;
-    map.gstart = gstart_.p;
-    check(launch_serve_ghosts(n_owned_, n_ghost_, n_total_, owner_.p + n_owned_, g_shift_.p, pos_.p, stream_),
+    check(launch_serve_ghosts(L.n_owned, L.n_ghost, L.n_total, L.owner.p + L.n_owned, L.g_shift.p, L.pos.p, L.stream),
           "serve ghosts");
-    nep_->autotune(batch_box_, n_total_, n_owned_, type_.p, pos_.p, pe_.p, force_.p, vir_.p,
-                   flags | (forced_rebuild_ ? 2u : 0u), stream_, &map);
-    drop_graph();
-    forced_rebuild_ = false;
+    L.nep->autotune(L.batch_box, L.n_total, L.n_owned, L.type.p, L.pos.p, L.pe.p, L.force.p, L.vir.p,
+                    flags | (L.forced_rebuild ? 2u : 0u), L.stream, &map);
+    drop_graph(L);
+    L.forced_rebuild = false;
   }
   if (cfg_.use_graph) {
-    // one graph launch per iteration; a Verlet overflow reallocates the
-    // engine's rows, so the graph is captured again and replayed
-    bool done = false;
-    for (int attempt = 0; attempt < 4 && !done; ++attempt) {
-      if (!graph_exec_) capture_force(flags);
-      if (forced_rebuild_) check(launch_request_rebuild(nep_->work(), stream_), "serve rebuild request");
-      check(hipGraphLaunch(graph_exec_, stream_), "serve force graph");
-      done = !nep_->settle(stream_);
-      if (!done) {
-        drop_graph();
-        forced_rebuild_ = true;
-      }
-    }
-    if (!done) throw std::runtime_error("serving: Verlet rows kept overflowing");
+    if (!L.graph_exec) capture_force(L, flags);
+    if (L.forced_rebuild) check(launch_request_rebuild(L.nep->work(), L.stream), "serve rebuild request");
+    check(hipGraphLaunch(L.graph_exec, L.stream), "serve force graph");
   } else {
-    check(launch_serve_ghosts(n_owned_, n_ghost_, n_total_, owner_.p + n_owned_, g_shift_.p, pos_.p, stream_),
+    check(launch_serve_ghosts(L.n_owned, L.n_ghost, L.n_total, L.owner.p + L.n_owned, L.g_shift.p, L.pos.p, L.stream),
           "serve ghosts");
-    ImageMap map;
-    map.owner = owner_.p;
-    map.image = image_.p;
-    map.gstart = gstart_.p;
-    nep_->compute(batch_box_, n_total_, 0, n_owned_, 0, n_owned_, type_.p, pos_.p, nullptr, pe_.p, force_.p,
-                  vir_.p, flags | (forced_rebuild_ ? 2u : 0u), stream_, &map);
+    L.nep->enqueue(L.batch_box, L.n_total, 0, L.n_owned, 0, L.n_owned, L.type.p, L.pos.p, nullptr, L.pe.p, L.force.p,
+                   L.vir.p, flags | (L.forced_rebuild ? 2u : 0u), L.stream, &map);
   }
-  forced_rebuild_ = false;
-  if (any_md)
-    check(launch_serve_kick(0, n_owned_, n_total_, atom_req_.p, req_md_.p, req_dt_.p, mass_.p, pos_.p, vel_.p,
-                            force_.p, stream_),
-          "serve kick 2");
-  std::lock_guard<std::mutex> lk(mu_);
-  st_.iterations++;
 }
 
-// Captures the force launches of the current layout: the plan (host checks,
+// Waits for a lane's frame; a Verlet overflow (the engine has grown its rows)
+// redoes the force evaluation (the graph captured again); then the second
+// half-kick is enqueued.
+void Server::finish_frame(Lane& L)
+{
+  const unsigned flags = cfg_.flags & ~3u;
+  const ImageMap map = L.map();
+  bool done = !L.nep->settle(L.stream);
+  for (int attempt = 0; attempt < 4 && !done; ++attempt) {
+    if (cfg_.use_graph) {
+      drop_graph(L);
+      capture_force(L, flags);
+      check(launch_request_rebuild(L.nep->work(), L.stream), "serve rebuild request");
+      check(hipGraphLaunch(L.graph_exec, L.stream), "serve force graph");
+    } else {
+      L.nep->enqueue(L.batch_box, L.n_total, 0, L.n_owned, 0, L.n_owned, L.type.p, L.pos.p, nullptr, L.pe.p,
+                     L.force.p, L.vir.p, flags | 2u, L.stream, &map);
+    }
+    done = !L.nep->settle(L.stream);
+  }
+  if (!done) throw std::runtime_error("serving: Verlet rows kept overflowing");
+  L.forced_rebuild = false;
+  if (L.any_md)
+    check(launch_serve_kick(0, L.n_owned, L.n_total, L.atom_req.p, L.req_md.p, L.req_dt.p, L.mass.p, L.pos.p, L.vel.p,
+                            L.force.p, L.stream),
+          "serve kick 2");
+}
+
+// Captures the force launches of a lane's layout: the plan (host checks,
 // workspace sizing) is made first, outside the capture; the pack never forces
 // a rebuild itself (a forced rebuild is requested on the device before a
 // replay).  The captured kernels take the layout's arrays, so the graph lives
 // until the next relayout or workspace reallocation.
-void Server::capture_force(unsigned flags)
+void Server::capture_force(Lane& L, unsigned flags)
 {
-  ImageMap map;
-  map.owner = owner_.p;
-  map.image = image_.p;
-  map.gstart = gstart_.p;
-  const NepEngine::Plan p = nep_->plan(batch_box_, n_total_, 0, n_owned_, 0, n_owned_, type_.p, pos_.p, nullptr,
-                                       pe_.p, force_.p, vir_.p, flags, &map);
-  if (p.forced) forced_rebuild_ = true;
-  check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "serve capture");
-  hipError_t e = launch_serve_ghosts(n_owned_, n_ghost_, n_total_, owner_.p + n_owned_, g_shift_.p, pos_.p, stream_);
+  const ImageMap map = L.map();
+  const NepEngine::Plan p = L.nep->plan(L.batch_box, L.n_total, 0, L.n_owned, 0, L.n_owned, L.type.p, L.pos.p,
+                                        nullptr, L.pe.p, L.force.p, L.vir.p, flags, &map);
+  if (p.forced) L.forced_rebuild = true;
+  check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal), "serve capture");
+  hipError_t e =
+      launch_serve_ghosts(L.n_owned, L.n_ghost, L.n_total, L.owner.p + L.n_owned, L.g_shift.p, L.pos.p, L.stream);
   if (e == hipSuccess)
-    e = launch_pack(p.box, n_total_, nep_->model().num_types, pos_.p, type_.p, false, nep_->work(), stream_);
+    e = launch_pack(p.box, L.n_total, L.nep->model().num_types, L.pos.p, L.type.p, false, L.nep->work(), L.stream);
   if (e == hipSuccess) {
     try {
-      nep_->launch_force(p, n_total_, 0, n_owned_, 0, n_owned_, type_.p, nullptr, pe_.p, force_.p, vir_.p, flags,
-                         stream_);
+      L.nep->launch_force(p, L.n_total, 0, L.n_owned, 0, L.n_owned, L.type.p, nullptr, L.pe.p, L.force.p, L.vir.p,
+                          flags, L.stream);
     } catch (...) {
       hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(stream_, &g);
+      (void)hipStreamEndCapture(L.stream, &g);
       if (g) (void)hipGraphDestroy(g);
       throw;
     }
   }
   hipGraph_t g = nullptr;
-  const hipError_t e2 = hipStreamEndCapture(stream_, &g);
+  const hipError_t e2 = hipStreamEndCapture(L.stream, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
     check(e != hipSuccess ? e : e2, "serve capture");
   }
-  graph_ = g;
-  check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "serve graph instantiate");
+  L.graph = g;
+  check(hipGraphInstantiate(&L.graph_exec, L.graph, nullptr, nullptr, 0), "serve graph instantiate");
 }
 
-void Server::drop_graph()
+void Server::drop_graph(Lane& L)
 {
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
-  graph_exec_ = nullptr;
-  graph_ = nullptr;
+  if (L.graph_exec) (void)hipGraphExecDestroy(L.graph_exec);
+  if (L.graph) (void)hipGraphDestroy(L.graph);
+  L.graph_exec = nullptr;
+  L.graph = nullptr;
 }
 
-void Server::retire()
+void Server::retire(Lane& L)
 {
   // advance the request states; collect the ones that finished
   std::vector<Request*> fin;
   long long frames = 0, atom_frames = 0;
   std::unique_lock<std::mutex> slk(mu_);
-  for (auto& rp : active_) {
+  for (auto& rp : L.active) {
     Request& r = *rp;
     if (!r.live) continue;
     if (r.state == 0) {
@@ -787,7 +941,7 @@ void Server::retire()
       ServeOut& t = tab[q];
       t.slot0 = r.slot0;
       t.n = r.n;
-      t.out = 10 * no;
+      t.out = (long long)kServeOutDoubles * no;
       for (int d = 0; d < 3; ++d) {
         t.pbc[d] = r.box.pbc[d] ? 1 : 0;
         t.trans[d] = r.trans[d];
@@ -796,22 +950,23 @@ void Server::retire()
       std::memcpy(t.hi, r.hi, sizeof(t.hi));
       no += r.n;
     }
-    outq_.need((long long)tab.size(), "serve out table");
-    out_.need(10 * no, "serve results");
-    check(hipStreamSynchronize(stream_), "serve step");  // the staging buffer is free again
-    stage(tab.size() * sizeof(ServeOut));
-    std::memcpy(h_stage_, tab.data(), tab.size() * sizeof(ServeOut));
-    check(hipMemcpyAsync(outq_.p, h_stage_, tab.size() * sizeof(ServeOut), hipMemcpyHostToDevice, stream_),
+    const size_t out_doubles = (size_t)kServeOutDoubles * (size_t)no;
+    L.outq.need((long long)tab.size(), "serve out table");
+    L.out.need((long long)out_doubles, "serve results");
+    check(hipStreamSynchronize(L.stream), "serve step");  // the staging buffer is free again
+    L.stage(tab.size() * sizeof(ServeOut));
+    std::memcpy(L.h_stage, tab.data(), tab.size() * sizeof(ServeOut));
+    check(hipMemcpyAsync(L.outq.p, L.h_stage, tab.size() * sizeof(ServeOut), hipMemcpyHostToDevice, L.stream),
           "serve out upload");
-    check(launch_serve_extract((int)tab.size(), outq_.p, no, n_total_, pos_.p, vel_.p, force_.p, pe_.p, out_.p,
-                               stream_),
+    check(launch_serve_extract((int)tab.size(), L.outq.p, no, L.n_total, L.pos.p, L.vel.p, L.force.p, L.pe.p,
+                               L.vir.p, L.out.p, L.stream),
           "serve extract");
     // results go straight into a pinned block that the requests keep
     auto blk = std::make_shared<ResultBlock>();
     blk->srv = this;
-    blk->p = pin_get(10 * (size_t)no * 8, &blk->cap);
-    check(hipMemcpyAsync(blk->p, out_.p, 10 * (size_t)no * 8, hipMemcpyDeviceToHost, stream_), "serve results");
-    check(hipStreamSynchronize(stream_), "serve results");
+    blk->p = pin_get(out_doubles * 8, &blk->cap);
+    check(hipMemcpyAsync(blk->p, L.out.p, out_doubles * 8, hipMemcpyDeviceToHost, L.stream), "serve results");
+    check(hipStreamSynchronize(L.stream), "serve results");
     const double t = now_s();
     std::lock_guard<std::mutex> lk(mu_);
     for (size_t q = 0; q < fin.size(); ++q) {
@@ -828,12 +983,12 @@ void Server::retire()
     st_.atom_frames += atom_frames;
     t_last_ = t;
     done_cv_.notify_all();
-    // a finished request leaves the layout at the next relayout
-    bool any_live = false;
-    for (auto& r : active_) any_live = any_live || r->live;
-    if (!any_live) {
-      active_.clear();
-      dirty_ = true;
+    // a finished request leaves the layout at the lane's next relayout; an
+    // empty lane drops its layout now
+    if (!L.live()) {
+      L.active.clear();
+      L.n_owned = L.n_ghost = L.n_total = 0;
+      drop_graph(L);
     }
     return;
   }

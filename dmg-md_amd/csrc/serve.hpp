@@ -2,12 +2,13 @@
 // evaluated together on one GPU with continuous batching.
 //
 // No reference counterpart: the reference runs one system per process.  A
-// request is a periodic box with N atoms and either a single-point force
-// evaluation (steps = 0) or `steps` NVE velocity-Verlet steps.  At submit
-// the request gets a shell of the periodic images of its atoms that lie
-// within rc + skin + margin of its box.  Active requests are placed in
-// disjoint, translated regions of one non-periodic batch box (gaps wider than
-// the Verlet radius), laid out as
+// request is a periodic box with N atoms, evaluated with one of the server's
+// models, either as a single point (steps = 0: energy, forces, virial) or as
+// `steps` NVE velocity-Verlet steps.  At submit the request gets a shell of
+// the periodic images of its atoms that lie within rc + skin + margin of its
+// box.  The active requests of one model are placed in disjoint, translated
+// regions of one non-periodic batch box (gaps wider than the Verlet radius),
+// laid out as
 //     [owned atoms of every request | their image ghosts, grouped by owner]
 // and the batch is evaluated with an ImageMap: owned atoms are the only
 // force and descriptor centres, and a ghost in a row stands for its owner
@@ -16,11 +17,14 @@
 // while ghosts cost positions only.  Ghost positions are refreshed from their
 // owners (+ box shift) every step.
 //
-// A worker thread owns the GPU work: each iteration admits waiting requests
-// (FIFO) while the batch stays within the HBM budget (atoms incl. ghosts),
-// re-lays the batch out when its membership changed, advances every active
-// request by one frame (its first force evaluation, or one MD step), and
-// retires the finished ones, recording submit-to-result latency.
+// Each model has its own batch ("lane": layout, engine workspace, stream and
+// captured force graph).  A worker thread owns the GPU work: each iteration
+// admits waiting requests in submission order while the device bytes of the
+// requests in flight stay within the HBM budget, re-lays out the lanes whose
+// membership changed, enqueues every lane's frame (one MD step, or the first
+// force evaluation) on its own stream so the lanes overlap on the device,
+// then settles each lane and retires the finished requests, recording
+// submit-to-result latency.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,11 +44,12 @@
 namespace dmgx {
 
 struct ServeConfig {
-  long long max_atoms = 4000000;  // owned + ghost atoms in flight
+  long long max_atoms = 4000000;  // owned + ghost slots in flight, all models
   int max_requests = 4096;
   unsigned flags = 0;             // force flags (ANN mode)
   double margin = 1.0;            // extra shell thickness (A) for motion during a request
-  long long reserve_atoms = 1LL << 21;  // engine workspace sized up front (capped by max_atoms)
+  long long hbm_bytes = 0;        // device bytes of the requests in flight; 0: half the free memory at start
+  long long reserve_atoms = 1LL << 21;  // engine workspace sized up front per model (capped by the budget)
   bool use_graph = true;          // replay the force launches of a layout as one hipGraph
 };
 
@@ -54,50 +59,56 @@ struct ServeStats {
   long long iterations = 0, relayouts = 0;
   long long max_slots = 0;
   double relayout_s = 0, iterate_s = 0, retire_s = 0;
+  long long hbm_budget = 0, max_bytes_in_flight = 0;
 };
 
-// The Server owns the NepEngine's GPU work while it lives: the engine must not
-// be used by other threads meanwhile.  Requests overlapping on the GPU are the
-// point; the results of each are those of its periodic system alone.
+// The Server owns its NepEngines' GPU work while it lives: the engines must
+// not be used by other threads meanwhile.  Requests overlapping on the GPU
+// are the point; the results of each are those of its periodic system alone.
 class Server {
  public:
-  Server(NepEngine* nep, const ServeConfig& cfg);
+  Server(const std::vector<NepEngine*>& models, const ServeConfig& cfg);
   ~Server();
   Server(const Server&) = delete;
   Server& operator=(const Server&) = delete;
 
   // host arrays are copied; position/velocity SoA [3*N]; velocity may be null
-  long long submit(const Box& box, int N, const int* type, const double* mass, const double* pos,
+  long long submit(int model, const Box& box, int N, const int* type, const double* mass, const double* pos,
                    const double* vel, int steps, double dt_fs);
   // true when done; then copies results (each may be null): potential[N],
-  // force[3N], position[3N] (wrapped into the request's box), velocity[3N]
+  // force[3N], position[3N] (wrapped into the request's box), velocity[3N],
+  // virial[9N]
   bool poll(long long id, bool wait, double* potential, double* force, double* position, double* velocity,
-            double* latency_ms);
+            double* virial, double* latency_ms);
   ServeStats stats();
   // blocks until every submitted request is done
   void drain();
+  // device bytes one slot (owned atom or image ghost) of a model's batch
+  // takes: engine workspace (rows of the model's capacity) and the layout's
+  // double-buffered per-slot arrays
+  double slot_bytes(int model) const;
 
  private:
   struct Request;
+  struct Lane;
   void worker();
-  bool admit();
-  void relayout();
-  void iterate();
-  void retire();
+  void admit();
+  void relayout(Lane& L);
+  void enqueue_frame(Lane& L);
+  void finish_frame(Lane& L);
+  void retire(Lane& L);
 
-  NepEngine* nep_;
   ServeConfig cfg_;
-  hipStream_t stream_ = nullptr;
+  std::vector<std::unique_ptr<Lane>> lanes_;
   std::thread thread_;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
   bool stop_ = false;
   std::string error_;  // first worker failure, reported by poll/drain/stats
   long long next_id_ = 1;
+  long long budget_ = 0;  // HBM bytes of requests in flight
   std::deque<std::shared_ptr<Request>> waiting_;
   std::map<long long, std::shared_ptr<Request>> all_;
-  std::vector<std::shared_ptr<Request>> active_;  // worker only
-  bool dirty_ = true;                               // membership changed since the last layout
   double t_first_ = -1, t_last_ = 0;
   std::vector<double> latencies_;
   ServeStats st_;
@@ -111,24 +122,6 @@ class Server {
     void release();
   };
 
-  // batch layout (worker only)
-  int n_owned_ = 0, n_ghost_ = 0, n_total_ = 0;
-  Box batch_box_{};
-  DBuf<double> pos_, vel_, force_, pe_, vir_, mass_;
-  DBuf<int> type_;
-  DBuf<int> owner_;        // slot -> owned slot it is an image of (itself for owned slots)
-  DBuf<int> image_;        // slot -> image code (image_code(0, 0, 0) for owned slots)
-  DBuf<int> gstart_;       // owned slot -> first ghost slot of its images [n_owned + 1]
-  DBuf<double> g_shift_;   // ghost k -> box shift [3]
-  // the next layout's per-slot arrays, assembled on the device and swapped in
-  DBuf<double> pos2_, vel2_, force2_, mass2_, g_shift2_;
-  DBuf<int> type2_, owner2_, image2_, gstart2_;
-  DBuf<int> atom_req_;     // owned slot -> active request index
-  DBuf<int> req_md_;       // active request -> 1 when it takes an MD step this iteration
-  DBuf<double> req_dt_;    // active request -> dt (natural units)
-  DBuf<ServeSlots> slots_; // layout table (one row per active request)
-  DBuf<ServeOut> outq_;    // finished requests (extract table)
-  DBuf<double> out_;       // finished requests' result blocks (10 doubles per atom)
   hipStream_t upload_ = nullptr;  // request uploads (submitting threads)
   std::mutex pack_mu_;
   std::multimap<size_t, void*> pack_free_;  // reusable device pack buffers by capacity
@@ -146,16 +139,8 @@ class Server {
     size_t cap = 0;
     ~ResultBlock() { srv->pin_put(p, cap); }
   };
-  char* h_stage_ = nullptr;  // pinned staging for uploads and results
-  size_t h_stage_bytes_ = 0;
-  void stage(size_t bytes);
-  bool forced_rebuild_ = true;
-  // the force launches of the current layout (ghost refresh, pack, Verlet
-  // rebuild, filter, descriptor, gather) captured once and replayed
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t graph_exec_ = nullptr;
-  void capture_force(unsigned flags);
-  void drop_graph();
+  void capture_force(Lane& L, unsigned flags);
+  void drop_graph(Lane& L);
 };
 
 }  // namespace dmgx

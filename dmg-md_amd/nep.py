@@ -105,27 +105,30 @@ class NEP:
         check(lib().dmgx_nep_reserve(self._h, int(capacity)))
 
     def compute(self, box: Box, type, position, potential, force, virial, *, accumulate=False,
-                ann_bf16=False, ann_valu=False, force_rebuild=False, stream=None) -> None:
+                ann_bf16=False, ann_valu=False, ann_bf16_split=False, force_rebuild=False, stream=None) -> None:
         """Potential::compute: every atom is a force and dependency centre."""
         N = type.numel()
         self.compute_domain(box, N, type, position, potential, force, virial, n1=0, n2=N, nd1=0,
-                            nd2=N, accumulate=accumulate, ann_bf16=ann_bf16, ann_valu=ann_valu,
+                            nd2=N, accumulate=accumulate, ann_bf16=ann_bf16, ann_valu=ann_valu, ann_bf16_split=ann_bf16_split,
                             force_rebuild=force_rebuild, stream=stream)
 
     def compute_domain(self, box: Box, num_atoms: int, type, position, potential, force, virial,
                        global_id=None, *, n1=0, n2=None, nd1=0, nd2=None, sort_by_gid=False,
-                       accumulate=False, ann_bf16=False, ann_valu=False, force_rebuild=False,
+                       accumulate=False, ann_bf16=False, ann_valu=False, ann_bf16_split=False, force_rebuild=False,
                        stream=None) -> None:
         """NEP::compute_domain: [n1,n2) force centres, [nd1,nd2) dependency
         centres, every slot of [0,num_atoms) a neighbour candidate.  The ANN
         runs in FP32 on the matrix cores where the model allows (single type,
-        <= 64 neurons); ann_valu keeps it on the vector ALUs, ann_bf16 uses
+        <= 64 neurons); ann_valu keeps it on the vector ALUs, ann_bf16_split splits
+        every operand into three bfloat16 parts (six part products, FP32
+        accuracy), ann_bf16 uses
         bfloat16 operands."""
         import torch
         N = int(num_atoms)
         n2 = N if n2 is None else n2
         nd2 = N if nd2 is None else nd2
         flags = ((_lib.ACCUMULATE if accumulate else 0) | (_lib.ANN_BF16 if ann_bf16 else 0) |
+                 (_lib.ANN_BF16_SPLIT if ann_bf16_split else 0) |
                  (_lib.FORCE_REBUILD if force_rebuild else 0) |
                  (_lib.SORT_BY_GID if sort_by_gid else 0) | (_lib.ANN_VALU if ann_valu else 0))
         b = box.to_c()
@@ -256,14 +259,14 @@ class MD:
 
     def run(self, steps: int, dt_fs: float, ensemble: str = "nve", t_init=0.0, t_final=None,
             t_coupling=100.0, p_target_gpa=0.0, bulk_modulus_gpa=0.0, p_coupling=1000.0,
-            seed=12345678, ann_bf16=False, ann_valu=False) -> np.ndarray:
+            seed=12345678, ann_bf16=False, ann_valu=False, ann_bf16_split=False) -> np.ndarray:
         e = _lib.Ensemble(ENSEMBLES[ensemble], float(t_init), float(t_init if t_final is None else t_final),
                           float(t_coupling), float(p_target_gpa), float(bulk_modulus_gpa),
                           float(p_coupling), int(seed))
         th = np.zeros(8)
         check(lib().dmgx_md_run(self._h, int(steps), float(dt_fs), ctypes.byref(e),
-                                (_lib.ANN_BF16 if ann_bf16 else 0) | (_lib.ANN_VALU if ann_valu else 0),
-                                th.ctypes.data_as(_lib.PD)))
+                                (_lib.ANN_BF16 if ann_bf16 else 0) | (_lib.ANN_VALU if ann_valu else 0) |
+                                (_lib.ANN_BF16_SPLIT if ann_bf16_split else 0), th.ctypes.data_as(_lib.PD)))
         return th
 
     def arrays(self, input_order=False) -> dict:
@@ -290,29 +293,36 @@ class MD:
         check(lib().dmgx_md_box(self._h, ctypes.byref(b)))
         return Box(np.array(b.h[:]).reshape(3, 3), tuple(b.pbc[:]))
 
-    def autotune(self, ann_bf16=False) -> dict:
+    def autotune(self, ann_bf16=False, ann_bf16_split=False) -> dict:
         """NEP.autotune on the state's current positions."""
         t = _lib.Tuning()
-        check(lib().dmgx_md_autotune(self._h, _lib.ANN_BF16 if ann_bf16 else 0, ctypes.byref(t)))
+        check(lib().dmgx_md_autotune(self._h, (_lib.ANN_BF16 if ann_bf16 else 0) |
+                                     (_lib.ANN_BF16_SPLIT if ann_bf16_split else 0), ctypes.byref(t)))
         return _tuning_dict(t)
 
 
 class Server:
     """Continuous-batching server of independent periodic systems (no
     reference counterpart): each request is a single point (steps=0) or an
-    NVE run of `steps` steps; requests share the GPU in one batch and each
-    gets the result of its own periodic system.  The NEP handle is owned by
-    the server while it lives."""
+    NVE run of `steps` steps, evaluated with one of the server's models
+    (`model`, an index into the NEP handles given); requests share the GPU in
+    one batch per model and each gets the result of its own periodic system.
+    Requests are admitted while their device bytes fit `hbm_bytes` (0: half
+    the device memory free at start).  The NEP handles are owned by the
+    server while it lives."""
 
-    def __init__(self, nep: NEP, max_atoms: int = 4_000_000, max_requests: int = 4096, ann_bf16=False,
-                 ann_valu=False, margin: float = 1.0, graph: bool = True):
+    def __init__(self, nep, max_atoms: int = 4_000_000, max_requests: int = 4096, ann_bf16=False,
+                 ann_valu=False, margin: float = 1.0, graph: bool = True, hbm_bytes: int = 0, ann_bf16_split=False):
+        neps = list(nep) if isinstance(nep, (list, tuple)) else [nep]
         cfg = _lib.ServerConfig(int(max_atoms), int(max_requests),
-                                (_lib.ANN_BF16 if ann_bf16 else 0) | (_lib.ANN_VALU if ann_valu else 0),
-                                float(margin), 1 if graph else 0)
+                                (_lib.ANN_BF16 if ann_bf16 else 0) | (_lib.ANN_VALU if ann_valu else 0) |
+                                (_lib.ANN_BF16_SPLIT if ann_bf16_split else 0),
+                                float(margin), 1 if graph else 0, int(hbm_bytes))
+        arr = (ctypes.c_void_p * len(neps))(*[n.handle for n in neps])
         h = ctypes.c_void_p()
-        check(lib().dmgx_server_create(nep.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        check(lib().dmgx_server_create(arr, len(neps), ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
-        self.nep = nep
+        self.neps = neps
         self._n = {}
 
     def __del__(self):
@@ -322,7 +332,7 @@ class Server:
             self._h = None
 
     def submit(self, box: Box, types, masses, positions, velocities=None, steps: int = 0,
-               dt_fs: float = 1.0) -> int:
+               dt_fs: float = 1.0, model: int = 0) -> int:
         t = np.ascontiguousarray(types, np.int32)
         N = t.shape[0]
         m = np.ascontiguousarray(masses, np.float64)
@@ -330,7 +340,7 @@ class Server:
         v = None if velocities is None else np.ascontiguousarray(velocities, np.float64).reshape(3, N)
         b = box.to_c()
         rid = ctypes.c_longlong()
-        check(lib().dmgx_server_submit(self._h, ctypes.byref(b), N, t.ctypes.data_as(_lib.PI),
+        check(lib().dmgx_server_submit(self._h, int(model), ctypes.byref(b), N, t.ctypes.data_as(_lib.PI),
                                        m.ctypes.data_as(_lib.PD), x.ctypes.data_as(_lib.PD),
                                        None if v is None else v.ctypes.data_as(_lib.PD), int(steps),
                                        float(dt_fs), ctypes.byref(rid)))
@@ -339,19 +349,19 @@ class Server:
 
     def poll(self, rid: int, wait: bool = False):
         """None while pending; else potential[N], force[3,N], position[3,N],
-        velocity[3,N] and latency_ms."""
+        velocity[3,N], virial[9,N] and latency_ms."""
         N = self._n[rid]
-        pe, f, x, v = np.zeros(N), np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N)
+        pe, f, x, v, w = np.zeros(N), np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N), np.zeros(9 * N)
         lat = ctypes.c_double()
         done = ctypes.c_int()
         check(lib().dmgx_server_poll(self._h, ctypes.c_longlong(rid), 1 if wait else 0,
                                      pe.ctypes.data_as(_lib.PD), f.ctypes.data_as(_lib.PD),
-                                     x.ctypes.data_as(_lib.PD), v.ctypes.data_as(_lib.PD), ctypes.byref(lat),
-                                     ctypes.byref(done)))
+                                     x.ctypes.data_as(_lib.PD), v.ctypes.data_as(_lib.PD),
+                                     w.ctypes.data_as(_lib.PD), ctypes.byref(lat), ctypes.byref(done)))
         if not done.value:
             return None
         return {"potential": pe, "force": f.reshape(3, N), "position": x.reshape(3, N),
-                "velocity": v.reshape(3, N), "latency_ms": lat.value}
+                "velocity": v.reshape(3, N), "virial": w.reshape(9, N), "latency_ms": lat.value}
 
     def drain(self) -> None:
         check(lib().dmgx_server_drain(self._h))
@@ -369,7 +379,8 @@ def _tuning_dict(t) -> dict:
     return {"filter_lanes": t.filter_lanes, "gather_lanes": t.gather_lanes, "tuned": bool(t.tuned),
             "filter_ms": dict(zip((8, 16, 32), t.filter_ms[:])), "gather_ms": dict(zip((4, 8, 16), t.gather_ms[:])),
             "row_layout": t.row_layout, "row_capacity": t.row_capacity,
-            "force_ms": dict(zip(("multiple_of_4", "power_of_2"), t.force_ms[:]))}
+            "force_ms": dict(zip(("multiple_of_4", "power_of_2"), t.force_ms[:])),
+            "overlap": t.overlap, "overlap_ms": dict(zip(("sequential", "overlapped"), t.overlap_ms[:]))}
 
 
 def _copy_d2h(dev_ptr, host: np.ndarray) -> None:

@@ -188,3 +188,87 @@ def amorphous_carbon_share(n: int = 2048000, seed: int = 1, dmin: float = 1.2, s
     vel = maxwell_boltzmann(m[:n], 5000.0, seed + 1)
     return dict(box=Box.orthogonal(E, E, E, pbc=(0, 0, 0)), pos=soa(pos), types=np.zeros(len(pos), np.int32),
                 mass=m, vel=vel, owned=n, ghost_of=np.concatenate(owner), L=L, shell=shell, origin=origin)
+
+
+# the serving stream's materials: lattice, constant (A), atoms per cubic
+# cell, mass, and the NVE step (fs) of the material's MD configuration
+SERVING_MATERIALS = {"carbon": (diamond, 3.567, 8, MASS["C"], 1.0),
+                     "cu": (fcc, 3.615, 4, MASS["Cu"], 2.0)}
+
+
+def vacancy_lattice(material: str, n: int, rng, vacancies: bool = True):
+    """n^3 cubic cells of the material's lattice (SERVING_MATERIALS), 30% of
+    the sites left vacant when `vacancies`, Gaussian jitter 0.05 A, 300 K
+    Maxwell-Boltzmann velocities; dict(box, pos (3,N), types, mass, vel)."""
+    lattice, a, per_cell, mass, dt = SERVING_MATERIALS[material]
+    pos, box = lattice(n, a)
+    if vacancies:
+        keep = rng.random(pos.shape[0]) >= 0.3
+        if keep.sum() < 2:
+            keep[:2] = True
+        pos = pos[keep]
+    pos = wrap(pos + rng.normal(0.0, 0.05, pos.shape), box)
+    N = pos.shape[0]
+    m = np.full(N, mass)
+    return dict(box=box, pos=soa(pos), types=np.zeros(N, np.int32), mass=m,
+                vel=maxwell_boltzmann(m, 300.0, int(rng.integers(1 << 30))))
+
+
+def serving_request(rng, material: str, n_min: int = 64, n_max: int = 20000, vacancy_share: float = 0.5):
+    """One request of the serving stream: a cubic replica of the material's
+    lattice whose site count is drawn log-uniform in [n_min, n_max] (the
+    nearest cube of cells, at least n_min sites), Gaussian jitter 0.05 A,
+    30% of the sites left vacant with probability vacancy_share, 300 K
+    Maxwell-Boltzmann velocities.  Returns dict(box, pos (3,N), types, mass,
+    vel, dt_fs)."""
+    lattice, a, per_cell, mass, dt = SERVING_MATERIALS[material]
+    target = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+    n = max(1, int(round((target / per_cell) ** (1.0 / 3.0))))
+    while per_cell * n ** 3 < n_min:
+        n += 1
+    r = vacancy_lattice(material, n, rng, vacancies=rng.random() < vacancy_share)
+    r.update(dt_fs=dt, material=material)
+    return r
+
+
+def serving_stream(count: int = 4096, seed: int = 7, n_min: int = 64, n_max: int = 20000, md_steps: int = 50):
+    """The serving configuration: `count` requests, each a carbon or Cu
+    request (serving_request) with equal odds; every second one a single
+    point (steps 0), the others md_steps-step NVE segments at the material's
+    step.  Returns a list of (request dict, model index 0 carbon / 1 Cu,
+    steps)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        mat = "carbon" if rng.random() < 0.5 else "cu"
+        r = serving_request(rng, mat, n_min, n_max)
+        out.append((r, 0 if mat == "carbon" else 1, md_steps if k % 2 else 0))
+    return out
+
+
+def diamond_npt_model_xyz(cells=(20, 6, 6), seed: int = 301, scale: float = 1.0023) -> str:
+    """model.xyz text of the reference's published NPT timing job (5,760-atom
+    diamond carbon for cells 20x6x6): conventional cells of edge
+    3.57 * scale A offset by 1 A, the diamond basis in the reference
+    generator's order, and velocities drawn per atom as three Gaussians of
+    sigma sqrt(k_B 300 K / m) / 10.18051 from Python's random.Random(seed)
+    (tests/npt/generate_inputs.py:28-48 describes the job's input)."""
+    import math
+    import random
+    basis = ((0, 0, 0), (.25, .25, .25), (.5, .5, 0), (.75, .75, .25),
+             (.5, 0, .5), (.75, .25, .75), (0, .5, .5), (.25, .75, .75))
+    nx, ny, nz = cells
+    a = 3.57 * scale
+    sigma = math.sqrt(K_B * 300.0 / MASS["C"]) / 10.18051
+    rng = random.Random(seed)
+    out = [str(8 * nx * ny * nz),
+           'pbc="T T T" Lattice="%r 0 0 0 %r 0 0 0 %r" Properties=species:S:1:pos:R:3:mass:R:1:vel:R:3'
+           % (3.57 * nx * scale, 3.57 * ny * scale, 3.57 * nz * scale)]
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                for x, y, z in basis:
+                    p = (1 + a * (i + x), 1 + a * (j + y), 1 + a * (k + z))
+                    v = [rng.gauss(0.0, sigma) for _ in range(3)]
+                    out.append("C " + " ".join(format(c, ".17g") for c in (*p, MASS["C"], *v)))
+    return "\n".join(out) + "\n"

@@ -68,6 +68,11 @@ typedef struct dmgx_nep_info {
                                   neighbor.cuh:164-196); needs global_id */
 #define DMGX_ANN_VALU 16u      /* fp32 ANN on the vector ALUs; the default fp32 ANN
                                   runs on the matrix cores where the model allows */
+#define DMGX_ANN_BF16_SPLIT 32u /* ANN on the matrix cores with every operand split into
+                                  three bf16 parts (hi + mid + lo) and each product formed
+                                  from the six part products above 2^-24 relative, fp32
+                                  accumulation: fp32 accuracy from bf16 matrix-core
+                                  products (same models as DMGX_ANN_BF16) */
 
 /* NEP::NEP(const char*) -- parse nep.txt once and upload the parameters
  * (nep.cu:135-415).  Supports nep4, nep4_zbl, nep5, nep5_zbl with l_max <= 4
@@ -164,6 +169,10 @@ typedef struct dmgx_tuning {
   int tuned;
   int row_layout, row_capacity;
   double force_ms[2]; /* layouts 0, 1 */
+  int overlap;          /* 1: the filter of the second half of the centres runs on a side
+                           stream beside the descriptor of the first half (timed only where
+                           each half holds at least 16,384 centres) */
+  double overlap_ms[2]; /* filter + descriptor + gather without / with the overlap */
 } dmgx_tuning;
 int dmgx_nep_get_tuning(const dmgx_nep* nep, dmgx_tuning* out);
 /* Sets the shapes directly (DMGX_EINVAL for values other than the
@@ -177,8 +186,10 @@ int dmgx_nep_autotune(dmgx_nep* nep, const dmgx_box* box, int N, const int* type
                       const double* position, double* potential, double* force, double* virial,
                       unsigned flags, void* stream, dmgx_tuning* out);
 
-/* Copies the current Verlet rows of centres [c1,c2) to host memory, sorted
- * ascending by slot (test access to the row contract of neighbor.cu:141-218):
+/* Copies the current Verlet rows of centres [c1,c2) to host memory in the
+ * order the device stores them: ascending by slot, or by (global id, slot)
+ * when the last compute set DMGX_SORT_BY_GID (test access to the row
+ * contract of neighbor.cu:141-218 and neighbor.cuh:133-196):
  * counts_host[c2-c1], rows_host[(c2-c1)*max_nb]. */
 int dmgx_nep_copy_verlet_rows(dmgx_nep* nep, int c1, int c2, int max_nb, int* counts_host,
                               int* rows_host);
@@ -301,12 +312,17 @@ int dmgx_md_autotune(dmgx_md* md, unsigned flags, dmgx_tuning* out);
 typedef struct dmgx_server dmgx_server;
 
 typedef struct dmgx_server_config {
-  long long max_atoms;   /* atoms in flight, periodic-image shells included */
+  long long max_atoms;   /* slots in flight over all models: owned atoms + periodic-image shells */
   int max_requests;      /* requests in flight */
   unsigned flags;        /* DMGX_ANN_BF16 / DMGX_ANN_VALU */
   double margin;         /* extra shell thickness (A) for motion during a request */
   int use_graph;         /* nonzero: the force launches of a batch layout are captured once
                             into a hipGraph and replayed every iteration */
+  long long hbm_bytes;   /* admission budget: device bytes of the requests in flight, each
+                            request counting (atoms + image shell) x its model's bytes per
+                            slot (engine workspace at the model's row capacity and the batch
+                            layout's arrays) plus its upload; 0 = half the device memory
+                            free when the server starts */
 } dmgx_server_config;
 
 typedef struct dmgx_server_stats {
@@ -317,28 +333,36 @@ typedef struct dmgx_server_stats {
   double elapsed_s;      /* first submit to last completion */
   double frames_per_s;
   long long iterations, relayouts;
-  long long max_slots;   /* largest batch: owned atoms + image ghosts */
+  long long max_slots;   /* largest batch over all models: owned atoms + image ghosts */
   double relayout_s, iterate_s, retire_s; /* worker wall time per phase */
+  long long hbm_budget;          /* the admission budget in bytes */
+  long long max_bytes_in_flight; /* the most device bytes admitted at once */
 } dmgx_server_stats;
 
-/* Starts a server (one worker thread, one stream) on a NEP handle; the
- * handle must not be used elsewhere while the server lives. */
-int dmgx_server_create(dmgx_nep* nep, const dmgx_server_config* cfg, dmgx_server** out);
+/* Starts a server (one worker thread; one batch and one stream per model) on
+ * num_models NEP handles; a request names its model by index into `models`.
+ * The handles must not be used elsewhere while the server lives.  Every
+ * model's batch is evaluated each iteration, the batches' launches
+ * overlapping on the device. */
+int dmgx_server_create(dmgx_nep* const* models, int num_models, const dmgx_server_config* cfg,
+                       dmgx_server** out);
 /* Stops the worker (pending requests are dropped) and frees the server. */
 void dmgx_server_destroy(dmgx_server* server);
-/* Submits a periodic system (host arrays, copied): steps = 0 is a single
- * point, steps > 0 that many NVE velocity-Verlet steps of dt_fs. velocity may
- * be NULL (zero). */
-int dmgx_server_submit(dmgx_server* server, const dmgx_box* box, int N, const int* type_host,
+/* Submits a periodic system (host arrays, copied) for model `model`: steps = 0
+ * is a single point (energy, forces, virial), steps > 0 that many NVE
+ * velocity-Verlet steps of dt_fs.  velocity may be NULL (zero).  A request
+ * whose device bytes exceed the whole budget is refused with DMGX_EINVAL. */
+int dmgx_server_submit(dmgx_server* server, int model, const dmgx_box* box, int N, const int* type_host,
                        const double* mass_host, const double* position_host,
                        const double* velocity_host, int steps, double dt_fs, long long* id_out);
 /* *done = 1 once the request finished; then its results are copied to the
  * host arrays given (each may be NULL): potential[N], force[3N],
- * position[3N] (wrapped into its box), velocity[3N], latency in ms.  wait != 0
- * blocks until it finishes. */
+ * position[3N] (wrapped into its box), velocity[3N], virial[9N] (per atom,
+ * xx yy zz xy xz yz yx zx zy, as dmgx_nep_compute), latency in ms.
+ * wait != 0 blocks until it finishes. */
 int dmgx_server_poll(dmgx_server* server, long long id, int wait, double* potential_host,
                      double* force_host, double* position_host, double* velocity_host,
-                     double* latency_ms, int* done);
+                     double* virial_host, double* latency_ms, int* done);
 /* Blocks until every submitted request has finished. */
 int dmgx_server_drain(dmgx_server* server);
 int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out);

@@ -3,9 +3,12 @@
 Each model is GPUMD nep.txt text with weights drawn from N(0, 0.1) (numpy
 PCG64, fixed seed), every value stored as an exactly representable float32 so
 that FP32 device storage and FP64 parsing see the same numbers.  q_scaler is
-1/(max-min) of each descriptor component over a sample of the configuration's
-initial frame, computed with the FP64 oracle.  MN_radial / MN_angular are the
-largest row lengths of that sample plus 25%.
+1/(max-min) of each descriptor component over samples of the configurations
+the model serves, computed with the FP64 oracle: the carbon and Cu models
+serve their lattice configurations and the serving stream's lattices with
+30% vacancies, whose under-coordinated atoms have descriptors far outside a
+perfect lattice's range.  MN_radial / MN_angular are the largest row lengths
+of the first sample plus 25%.
 
 Usage: python tests/golden/make_potentials.py   (writes tests/golden/potentials)
 """
@@ -28,10 +31,13 @@ W = importlib.import_module("dmg-md_amd.workloads")
 
 SPECS = {
     # name: (symbols, rcR, rcA, n_max, basis, l_max line, neurons, zbl line, sample, seed)
+    # the lattice models also serve the serving stream's vacancy lattices
     "nep_carbon.txt": (["C"], 4.2, 3.7, (4, 4), (8, 8), "4 2 1", 30, None,
-                          lambda: W.carbon_diamond_4096(seed=11), 101),
+                          lambda: [W.carbon_diamond_4096(seed=11),
+                                   W.vacancy_lattice("carbon", 8, np.random.default_rng(21))], 101),
     "nep_cu.txt": (["Cu"], 5.0, 4.0, (6, 4), (8, 8), "4 2 0", 50, None,
-                      lambda: W.copper_fcc(n=10, seed=12), 102),
+                      lambda: [W.copper_fcc(n=10, seed=12),
+                               W.vacancy_lattice("cu", 10, np.random.default_rng(22))], 102),
     # the model of the amorphous-carbon melt: its scaler is fit on a melt sample
     "nep_carbon_zbl.txt": (["C"], 4.2, 3.7, (4, 4), (8, 8), "4 2 1", 30, "zbl 1.0 2.0",
                               lambda: W.amorphous_carbon(8000, seed=13), 104),
@@ -53,7 +59,9 @@ def write(path, header, params):
 
 def make(name, spec):
     syms, rcR, rcA, nmax, basis, lmax, H, zbl, sample, seed = spec
-    sysd = sample()
+    samples = sample()
+    samples = samples if isinstance(samples, list) else [samples]
+    sysd = samples[0]
     rng = np.random.default_rng(seed)
     T = len(syms)
     L, q222, q1111 = [int(x) for x in lmax.split()]
@@ -74,9 +82,8 @@ def make(name, spec):
              f"basis_size {basis[0]} {basis[1]}", f"l_max {lmax}", f"ANN {H} 0"]
     path = os.path.join(OUT, name)
     write(path, head, np.concatenate([ann, desc, np.ones(dim)]))
-    o = OracleNEP(path).compute(sysd["box"].h, sysd["box"].pbc, sysd["types"], sysd["pos"],
-                                want_descriptor=True)
-    q = o["q"]
+    q = np.concatenate([OracleNEP(path).compute(x["box"].h, x["box"].pbc, x["types"], x["pos"],
+                                                want_descriptor=True)["q"] for x in samples])
     rngq = q.max(0) - q.min(0)
     scaler = f32(np.where(rngq > 0, 1.0 / np.where(rngq > 0, rngq, 1.0), 1.0))
     write(path, head, np.concatenate([ann, desc, scaler]))

@@ -56,6 +56,20 @@ def test_fullsize_bf16_ann_error(dmgx, cu256k, oracle_cu256k):
     assert err["energy_per_atom"] <= 1e-3, err
 
 
+def test_fullsize_split_bf16_ann_meets_the_fp32_bars(dmgx, cu256k, oracle_cu256k):
+    """Split bfloat16 operands on the matrix cores (every operand as hi + lo
+    bfloat16 parts, three products per term, FP32 accumulation): every atom
+    within the FP32 bars (1e-4 eV/A, 1e-6 eV/atom)."""
+    s, ref = cu256k, oracle_cu256k
+    out = gpu_compute(dmgx, dmgx.NEP(CU), s["box"], s["types"], s["pos"], ann_bf16_split=True)
+    err = errors(out, ref)
+    print("split-bf16 ANN vs FP64:", err)
+    assert err["force"] <= 1e-4, err
+    assert err["energy_per_atom"] <= 1e-6, err
+    assert err["energy_atom_max"] <= 2e-5, err
+    assert err["virial"] <= 2e-4, err
+
+
 def _nve_energies(dmgx, s, dt, chunk, chunks=20):
     N = len(s["types"])
     md = dmgx.MD(dmgx.NEP(CU), s["box"], s["types"], s["mass"], s["pos"], s["vel"])

@@ -370,6 +370,11 @@ def test_ann_modes(dmgx, workloads, which):
     fmax = np.abs(ref["force"]).max()
     assert e["force"] < 2e-2 * max(fmax, 1.0), e
     assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
+    # split bfloat16 operands (hi + lo, three products): the FP32 parity bars
+    x3 = gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"], ann_bf16_split=True)
+    e3 = errors(x3, ref)
+    print(which, "split-bf16 vs oracle", e3)
+    check(e3)
 
 
 @pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt",
@@ -559,6 +564,12 @@ def test_two_type_ann_modes(dmgx, name):
     e = errors(bf, ref)
     assert e["force"] < 2e-2 * max(np.abs(ref["force"]).max(), 1.0), e
     assert e["energy_atom_max"] < 2e-2 * max(np.abs(ref["pe"]).max(), 1.0), e
+    # split bfloat16 operands: against the oracle at the FP32 bars (forces
+    # scaled by the system's largest force, as the FP32 comparison above)
+    x3 = gpu_compute(dmgx, nep, box, t, pos, ann_bf16_split=True)
+    e3 = errors(x3, ref)
+    print(name, "split-bf16 vs oracle", e3)
+    assert e3["force"] < 1e-4 * fscale and e3["energy_atom_max"] < TOL_EA, e3
 
 
 WATER = os.path.join(POTENTIALS, "nep_water.txt")

@@ -28,14 +28,15 @@ def _diamond(reps, jitter, seed):
     return np.diag([a * reps] * 3), pos
 
 
-def _check_single(out, h, pos, orc):
+def _check_single(out, h, pos, orc, types=None):
     n = pos.shape[1]
-    ref = orc.compute(h, (1, 1, 1), np.zeros(n, np.int32), pos)
-    # the server returns no virial; compare energy and force
-    err = errors((out["potential"], out["force"], np.zeros((9, n))),
-                 dict(pe=ref["pe"], force=ref["force"], virial=np.zeros((9, n))))
+    t = np.zeros(n, np.int32) if types is None else types
+    ref = orc.compute(h, (1, 1, 1), t, pos)
+    err = errors((out["potential"], out["force"], out["virial"]), ref)
     assert err["force"] <= TOL_F, err
     assert err["energy_atom_max"] <= TOL_EA, err
+    assert err["virial"] <= TOL_V, err
+    return err
 
 
 def test_server_requests_match_standalone(dmgx, workloads):
@@ -88,7 +89,7 @@ def test_server_many_small_requests(dmgx, workloads):
         rid = srv.submit(dmgx.Box(h, (1, 1, 1)), np.zeros(n, np.int32), np.full(n, 12.011), p)
         cases.append((rid, h, p))
     srv.drain()
-    for rid, h, p in cases[::5]:
+    for rid, h, p in cases:
         _check_single(srv.poll(rid), h, p, orc)
     st = srv.stats()
     assert st["completed"] == 24 and st["frames"] == 24
@@ -132,12 +133,13 @@ def test_server_image_map_edge_cases(dmgx):
             out = srv.poll(rid)
             ref = orc.compute(h, (1, 1, 1), t, p)
             n = len(t)
-            err = errors((out["potential"], out["force"], np.zeros((9, n))),
-                         dict(pe=ref["pe"], force=ref["force"], virial=np.zeros((9, n))))
+            err = errors((out["potential"], out["force"], out["virial"]), ref)
             # the ZBL pair carries 40 eV/A: FP32 rounding scales with the force
             tol_f = max(TOL_F, 2e-6 * np.abs(ref["force"]).max())
+            tol_v = max(TOL_V, 2e-6 * np.abs(ref["virial"]).max())
             assert err["force"] <= tol_f, (os.path.basename(path), n, err)
             assert err["energy_atom_max"] <= TOL_EA, (os.path.basename(path), n, err)
+            assert err["virial"] <= tol_v, (os.path.basename(path), n, err)
 
 
 def test_graph_replays_match_eager_iterations(dmgx):
@@ -166,6 +168,65 @@ def test_graph_replays_match_eager_iterations(dmgx):
     e, se = serve(False)
     assert sg["completed"] == se["completed"] == 6
     for a, b in zip(g, e):
-        for key in ("potential", "force", "velocity"):
+        for key in ("potential", "force", "velocity", "virial"):
             assert np.abs(a[key] - b[key]).max() < 1e-9, key
         assert np.abs(a["position"] - b["position"]).max() < 1e-9  # both wrapped into the request's box
+
+
+CU = os.path.join(POTENTIALS, "nep_cu.txt")
+
+
+def test_serving_stream_mix_against_oracle(dmgx, workloads):
+    """A cut-down serving stream as the serving configuration shapes it:
+    carbon and Cu requests (each with its own model, one server), 30%
+    vacancies on a random subset, half single points and half NVE segments,
+    arriving while others run.  Every single point against the FP64 oracle
+    (energy, forces and per-atom virial), every NVE request against the
+    standalone FP32 MD path of the same system (positions, velocities,
+    forces, virial), in the served FP32 mode; then the same stream with
+    bfloat16 ANN operands, whose error against FP32 is reported."""
+    stream = workloads.serving_stream(count=40, seed=11, n_min=64, n_max=3000, md_steps=20)
+    paths = [CARBON, CU]
+    orcs = [OracleNEP(p, round_params_to_float=True) for p in paths]
+
+    def serve(bf16):
+        srv = dmgx.Server([dmgx.NEP(p) for p in paths], max_atoms=2_000_000, ann_bf16=bf16)
+        ids = []
+        for k, (r, model, steps) in enumerate(stream):
+            ids.append(srv.submit(r["box"], r["types"], r["mass"], r["pos"], r["vel"], steps=steps,
+                                  dt_fs=r["dt_fs"], model=model))
+            if k % 8 == 7:
+                time.sleep(0.003)  # later requests join a running batch
+        srv.drain()
+        st = srv.stats()
+        assert st["completed"] == len(stream), st
+        assert 0 < st["max_bytes_in_flight"] <= st["hbm_budget"], st
+        return [srv.poll(i) for i in ids]
+
+    got = serve(False)
+    worst = {"force": 0.0, "virial": 0.0, "energy_atom_max": 0.0}
+    n_md = 0
+    for (r, model, steps), out in zip(stream, got):
+        n = len(r["types"])
+        if steps == 0:
+            err = _check_single(out, r["box"].h, r["pos"], orcs[model])
+            for key in worst:
+                worst[key] = max(worst[key], err[key])
+            continue
+        n_md += 1
+        md = dmgx.MD(dmgx.NEP(paths[model]), r["box"], r["types"], r["mass"], r["pos"], r["vel"])
+        md.run(steps, r["dt_fs"], "nve")
+        ref = md.arrays()
+        L = r["box"].h[0, 0]
+        d = out["position"] - ref["position"]
+        d -= np.round(d / L) * L
+        assert np.abs(d).max() < 1e-6, (r["material"], n)
+        assert np.abs(out["velocity"] - ref["velocity"]).max() < 1e-6, (r["material"], n)
+        assert np.abs(out["force"] - ref["force"].reshape(3, n)).max() < TOL_F, (r["material"], n)
+        assert np.abs(out["virial"] - ref["virial"].reshape(9, n)).max() < TOL_V, (r["material"], n)
+    print("served FP32 single points vs FP64:", worst, "; NVE requests checked:", n_md, flush=True)
+    assert n_md > 0 and len(stream) - n_md > 0
+    bf = serve(True)
+    ferr = max(float(np.abs(a["force"] - b["force"]).max()) for (r, m, s), a, b in zip(stream, bf, got) if s == 0)
+    print(f"served BF16 vs FP32 single-point forces: {ferr:.3e} eV/A", flush=True)
+    assert ferr < 0.1

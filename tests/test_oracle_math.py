@@ -193,12 +193,17 @@ def test_seeded_models_parse_and_are_finite():
 def test_seeded_models_are_scaled_for_their_configuration(workloads):
     """Each configuration's model has q_scaler = 1/range of its descriptor
     over a sample of that configuration, so on an independent sample of the
-    same configuration every scaled component spans a range of order one.  The amorphous-carbon
+    same configuration every scaled component spans a range of order one
+    (the carbon and Cu models serve their lattices and the serving stream's
+    lattices with 30% vacancies; fit on the perfect lattices alone, the
+    vacancy lattices' scaled components spanned up to ~290).  The amorphous-carbon
     model fit on diamond instead saw 4-/5-body invariants up to ~2,400x their
     range on the melt and dE/dq up to 6e6; FP32 rounding of the angular
     moments, amplified by those gradients, then moved forces by ~1e-3 eV/A."""
     cases = [("nep_carbon.txt", workloads.carbon_diamond_4096(seed=3)),
              ("nep_cu.txt", workloads.copper_fcc(n=6, seed=3)),
+             ("nep_carbon.txt", workloads.vacancy_lattice("carbon", 5, np.random.default_rng(3))),
+             ("nep_cu.txt", workloads.vacancy_lattice("cu", 6, np.random.default_rng(4))),
              ("nep_carbon_zbl.txt", workloads.amorphous_carbon(3000, seed=3))]
     for name, s in cases:
         o = OracleNEP(os.path.join(POTENTIALS, name)).compute(s["box"].h, s["box"].pbc, s["types"], s["pos"],
