@@ -435,10 +435,10 @@ def water_bench(dmgx, steps=100):
             vols, press = [], []
             for _ in range(20):
                 th = md.run(5, 0.5, ens, t_init=300.0, t_final=300.0, **kw)
-                vols.append(md.box().volume())
+                vols.append(md.box().volume)
                 press.append(float(th[2] + th[3] + th[4]) / 3.0 * dmgx.PRESSURE_UNIT_CONVERSION)
             out[name].update({"mean_volume_A3": float(sum(vols) / len(vols)),
-                              "initial_volume_A3": s["box"].volume(),
+                              "initial_volume_A3": s["box"].volume,
                               "mean_pressure_GPa": float(sum(press) / len(press)),
                               "averaged_over": "100 further steps, sampled every 5"})
     return out

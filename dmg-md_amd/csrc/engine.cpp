@@ -245,18 +245,18 @@ NepEngine::NepEngine(const std::string& path)
   int bd[4];
   if (descriptor_bucket(dm_, bd)) {
     const int nR = m.nR, kR = m.kR, nA = m.nA, kA = m.kA, D = m.dim, H = m.H;
-    const int NRb = bd[0], KRb = bd[1], NAb = bd[2], KAb = bd[3], Db = NRb + NAb * (kMaxL + 2);
+    const int NRb = bd[0], KRb = bd[1], NAb = bd[2], KAb = bd[3], Db = NRb + NAb * (kPadL + 2);
     std::vector<int> col(D);
     for (int d = 0; d < nR; ++d) col[d] = d;
     int b = 0;
     for (; b < m.L_max; ++b)
       for (int n = 0; n < nA; ++n) col[nR + b * nA + n] = NRb + b * NAb + n;
     if (m.has222) {
-      for (int n = 0; n < nA; ++n) col[nR + b * nA + n] = NRb + kMaxL * NAb + n;
+      for (int n = 0; n < nA; ++n) col[nR + b * nA + n] = NRb + kPadL * NAb + n;
       ++b;
     }
     if (m.has1111)
-      for (int n = 0; n < nA; ++n) col[nR + b * nA + n] = NRb + (kMaxL + 1) * NAb + n;
+      for (int n = 0; n < nA; ++n) col[nR + b * nA + n] = NRb + (kPadL + 1) * NAb + n;
     const size_t n_w0 = (size_t)T * H * Db, n_cR = (size_t)T * T * NRb * KRb, n_cA = (size_t)T * T * NAb * KAb;
     std::vector<float> pad(n_w0 + Db + n_cR + n_cA, 0.0f);
     float* pw0 = pad.data();
@@ -284,10 +284,10 @@ NepEngine::NepEngine(const std::string& path)
     dmd_.kR = KRb;
     dmd_.nA = NAb;
     dmd_.kA = KAb;
-    dmd_.L = kMaxL;
+    dmd_.L = kPadL;
     dmd_.has222 = 1;
     dmd_.has1111 = 1;
-    dmd_.numL = kMaxL + 2;
+    dmd_.numL = kPadL + 2;
     dmd_.D = Db;
     dmd_.w0 = params_pad_;
     dmd_.qs = params_pad_ + n_w0;

@@ -28,7 +28,7 @@
 
 namespace dmgx {
 
-__constant__ float c_c3[24];
+__constant__ float c_c3[(kMaxL + 1) * (kMaxL + 1) - 1];
 __constant__ float c_zc[kMaxL + 1][kMaxL + 1][kMaxL + 1];
 __constant__ float c_c4[5];
 __constant__ float c_c5;
@@ -1842,9 +1842,14 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
     run_single<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(9, 13, 7, 13, 4, true, true, false))
     run_multi<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
-  else if (m.L <= 4 && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
-    run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, 4, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
-                                                                                  w, s, e0, e1);
+  else if (m.L <= kPadL && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
+    run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, kPadL, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
+                                                                                      w, s, e0, e1);
+  else if (m.L <= kMaxL && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
+    // 3-body orders 5..8 (80 angular moments per radial function): the
+    // generic instance at order 8, its per-atom arrays partly in scratch
+    run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, kMaxL, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
+                                                                                      w, s, e0, e1);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

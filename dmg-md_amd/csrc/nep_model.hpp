@@ -13,11 +13,12 @@
 namespace dmgx {
 
 constexpr int kMaxTypes = 16;   // per-type device tables
-constexpr int kMaxL = 4;        // 3-body angular order supported on device
+constexpr int kMaxL = 8;        // 3-body angular order supported (nep_utilities.cuh:50, 80 moments)
+constexpr int kPadL = 4;        // 3-body order of the bounded instances models are padded to
 constexpr int kMaxNA = 10;      // n_max_angular + 1
 constexpr int kMaxNR = 16;      // n_max_radial + 1
 constexpr int kMaxK = 17;       // basis_size + 1 (nep_utilities.cuh MAX_NUM_N)
-constexpr int kMaxDim = 96;
+constexpr int kMaxDim = 128;
 constexpr int kMaxNeurons = 128;
 
 struct NepModel {
@@ -54,7 +55,7 @@ struct NepModel {
 // folded in), integer coefficients of Z_{L,m}(z) = primitive part of
 // d^m P_L/dz^m, and the q_222 / q_1111 weights (Wigner-3j couplings).
 struct AngularConstants {
-  float c3[24];
+  float c3[(kMaxL + 1) * (kMaxL + 1) - 1];
   float zc[kMaxL + 1][kMaxL + 1][kMaxL + 1];
   float c4[5];
   float c5;

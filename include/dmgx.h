@@ -75,8 +75,8 @@ typedef struct dmgx_nep_info {
                                   products (same models as DMGX_ANN_BF16) */
 
 /* NEP::NEP(const char*) -- parse nep.txt once and upload the parameters
- * (nep.cu:135-415).  Supports nep4, nep4_zbl, nep5, nep5_zbl with l_max <= 4
- * and the q_222 / q_1111 many-body invariants. */
+ * (nep.cu:135-415).  Supports nep4, nep4_zbl, nep5, nep5_zbl with 3-body
+ * orders l_max 1..8 and the q_222 / q_1111 many-body invariants. */
 int dmgx_nep_create(const char* nep_txt_path, dmgx_nep** out);
 
 /* ~NEP (nep.cu:455) */

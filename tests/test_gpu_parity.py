@@ -379,14 +379,15 @@ def test_ann_modes(dmgx, workloads, which):
 
 @pytest.mark.parametrize("name", ["nep5_ch.txt", "nep4_zbl_flexible_ch.txt", "nep5_zbl_typewise_ch.txt",
                                   "nep4_typecut_ch.txt", "nep4_lmax2_ch.txt", "nep4_lmax3_ch.txt",
-                                  "nep4_big_ch.txt"])
+                                  "nep4_big_ch.txt", "nep4_lmax6_ch.txt"])
 @pytest.mark.parametrize("tiles", [1, 3])
 def test_nep5_and_zbl_forms_against_oracle(dmgx, name, tiles):
     """NEP5, the flexible per-pair ZBL table, the typewise ZBL cutoff,
-    per-type cutoffs and 3-body orders 2 and 3 on a random 2-type (C, H)
+    per-type cutoffs and 3-body orders 2, 3 and 6 on a random 2-type (C, H)
     system with pairs down to 0.5 A: the 11 A cell (replicated small-box
     path) and its 3x3x3 tiling (large-box path).  None of these sizes has an
-    exact descriptor instance, so all run padded to a bounded one.  The FP32
+    exact descriptor instance, so all run padded to a bounded one, except
+    order 6, which runs on the generic order-8 instance.  The FP32
     bars scale with the largest force and energy, which the ZBL pairs make
     large."""
     from test_oracle_golden import coverage_sample
