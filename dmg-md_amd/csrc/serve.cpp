@@ -570,6 +570,21 @@ void Server::worker()
       st_.retire_s += t3 - t2;
       if (!run.empty()) st_.iterations++;
       st_.max_slots = std::max<long long>(st_.max_slots, slots);
+      // a lane whose finished requests hold a quarter of its slots drops
+      // them at once (a new layout of its live requests), whether or not
+      // requests are waiting: every frame evaluates the whole layout
+      for (auto& L : lanes_) {
+        if (L->dirty) continue;
+        long long live = 0, dead = 0;
+        for (auto& r : L->active) (r->live ? live : dead) += r->footprint();
+        if (live > 0 && 4 * dead >= live + dead) {
+          std::vector<std::shared_ptr<Request>> keep;
+          for (auto& r : L->active)
+            if (r->live) keep.push_back(r);
+          L->active.swap(keep);
+          L->dirty = true;
+        }
+      }
     } catch (const std::exception& e) {
       std::lock_guard<std::mutex> lk(mu_);
       error_ = e.what();
