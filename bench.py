@@ -229,7 +229,8 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
                      "force_err_vs_plain_fp32_max_eV_per_A": errs["bf16"]},
             "tile_shapes": [{"model": os.path.basename(p), "filter_lanes_per_centre": t["filter_lanes"],
                              "gather_lanes_per_atom": t["gather_lanes"], "autotuned": t["tuned"],
-                             "row_capacity": t["row_capacity"], "row_layout": t["row_layout"]}
+                             "row_capacity": t["row_capacity"], "row_layout": t["row_layout"],
+                             "rows_split_at_rebuild": t["split"]}
                             for p, t in zip(paths, tunes)]}
 
 
@@ -464,6 +465,8 @@ def main():
     ap.add_argument("--water-only", type=int, default=0, metavar="STEPS",
                     help="run only the liquid-water sample with STEPS timed steps per run and print its "
                          "JSON object (for kernel profiles of the 2-type path)")
+    ap.add_argument("--rows", choices=["auto", "filter", "split"], default="auto",
+                    help="typewise row split every step (filter), once per rebuild (split), or the autotuner's")
     ap.add_argument("--ann", choices=["fp32", "valu", "bf16"], default="fp32",
                     help="ANN arithmetic: fp32 on the matrix cores (default), fp32 on the vector ALUs, "
                          "or bfloat16 operands on the matrix cores")
@@ -504,6 +507,9 @@ def main():
     print(f"bench: Cu fcc {args.cells}^3 = {N} atoms, warmup {args.warmup} steps", file=sys.stderr, flush=True)
     # filter / gather tile shapes timed on this system before anything else
     tune = nep.tuning() if args.no_autotune else md.autotune(ann_bf16=args.ann == "bf16")
+    if args.rows != "auto":
+        nep.set_split(args.rows == "split")
+        tune = nep.tuning()
     print(f"bench: tile shapes {tune}", file=sys.stderr, flush=True)
     md.run(args.warmup, 2.0, **ens)
     torch.cuda.synchronize()
@@ -645,7 +651,9 @@ def main():
                             "row_layout": {0: "multiple of 4", 1: "power of 2"}.get(tune["row_layout"], "default"),
                             "force_ms_by_row_layout": {k: round(v, 4) for k, v in tune["force_ms"].items()},
                             "filter_descriptor_overlap": tune["overlap"],
-                            "force_ms_by_schedule": {k: round(v, 4) for k, v in tune["overlap_ms"].items()}},
+                            "force_ms_by_schedule": {k: round(v, 4) for k, v in tune["overlap_ms"].items()},
+                            "rows_split_at_rebuild": tune["split"],
+                            "force_ms_by_row_split": {k: round(v, 4) for k, v in tune["split_ms"].items()}},
             "final_temperature_K": th[0],
         },
         "ms_per_step_with_rebuilds": amortised,

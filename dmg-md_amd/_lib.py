@@ -66,7 +66,8 @@ class Tuning(ctypes.Structure):
     _fields_ = [("filter_lanes", ctypes.c_int), ("gather_lanes", ctypes.c_int),
                 ("filter_ms", ctypes.c_double * 3), ("gather_ms", ctypes.c_double * 3), ("tuned", ctypes.c_int),
                 ("row_layout", ctypes.c_int), ("row_capacity", ctypes.c_int), ("force_ms", ctypes.c_double * 2),
-                ("overlap", ctypes.c_int), ("overlap_ms", ctypes.c_double * 2)]
+                ("overlap", ctypes.c_int), ("overlap_ms", ctypes.c_double * 2),
+                ("split", ctypes.c_int), ("split_ms", ctypes.c_double * 2)]
 
 
 ACCUMULATE = 1
@@ -98,6 +99,7 @@ _SIGNATURES = {
     "dmgx_nep_profile_read": (I, [VP, I, PI, PD]),
     "dmgx_nep_profile_reset": (I, [VP]),
     "dmgx_nep_get_tuning": (I, [VP, ctypes.POINTER(Tuning)]),
+    "dmgx_nep_set_split": (I, [VP, I]),
     "dmgx_nep_set_tiles": (I, [VP, I, I]),
     "dmgx_nep_autotune": (I, [VP, ctypes.POINTER(Box), I, VP, VP, VP, VP, VP, U, VP, ctypes.POINTER(Tuning)]),
     "dmgx_nep_copy_verlet_rows": (I, [VP, I, I, I, PI, PI]),

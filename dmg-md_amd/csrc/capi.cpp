@@ -210,6 +210,9 @@ void fill_tuning(const dmgx::NepEngine::Tuning& t, dmgx_tuning* out)
   out->overlap = t.overlap;
   out->overlap_ms[0] = t.overlap_ms[0];
   out->overlap_ms[1] = t.overlap_ms[1];
+  out->split = t.split;
+  out->split_ms[0] = t.split_ms[0];
+  out->split_ms[1] = t.split_ms[1];
 }
 }  // namespace
 
@@ -226,6 +229,14 @@ int dmgx_nep_set_tiles(dmgx_nep* nep, int filter_lanes, int gather_lanes)
   return guarded([&] {
     if (!nep) throw std::invalid_argument("null handle");
     nep->engine.set_tiles(filter_lanes, gather_lanes);
+  });
+}
+
+int dmgx_nep_set_split(dmgx_nep* nep, int split)
+{
+  return guarded([&] {
+    if (!nep) throw std::invalid_argument("null handle");
+    nep->engine.set_split(split);
   });
 }
 

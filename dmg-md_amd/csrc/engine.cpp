@@ -341,6 +341,11 @@ void NepEngine::free_all()
   dfree(&w_.vsa);
   dfree(&w_.bR);
   dfree(&w_.f12);
+  dfree(&w_.cl);
+  dfree(&w_.cpos);
+  dfree(&w_.cq);
+  dfree(&w_.nrc);
+  dfree(&w_.nac);
   dfree(&w_.cell_count);
   dfree(&w_.cell_start);
   dfree(&w_.cell_tiles);
@@ -362,8 +367,10 @@ int NepEngine::padded_rows(int mv, int layout) const
 }
 
 // device bytes per row entry: Verlet slot, reverse index, edge record,
-// edge reverse, Verlet->angular index, angular partial force
+// edge reverse, Verlet->angular index, angular partial force; and with
+// candidate rows the candidate slot, its angular slot and reverse index
 constexpr double kRowEntryBytes = 4 + 4 + 16 + 4 + 4 + 16;
+constexpr double kSplitEntryBytes = 4 + 4 + 4;
 
 void NepEngine::allocate(int capacity, int mv)
 {
@@ -395,6 +402,37 @@ void NepEngine::allocate(int capacity, int mv)
   w_.MR = mv;
   w_.MA = mv;
   have_list_ = false;
+  if (tune_.split) allocate_split();
+}
+
+void NepEngine::allocate_split()
+{
+  const size_t cap = (size_t)std::max(w_.cap, 1), rows = cap * (size_t)w_.MV;
+  dalloc(&w_.cl, rows, "candidate rows");
+  dalloc(&w_.cpos, rows, "verlet->angular candidate");
+  dalloc(&w_.cq, rows, "candidate reverse");
+  dalloc(&w_.nrc, cap, "candidates");
+  dalloc(&w_.nac, cap, "angular candidates");
+  w_.split = 1;
+  have_list_ = false;  // the candidate rows are made by the next rebuild
+}
+
+void NepEngine::set_split(int split)
+{
+  if (split != 0 && split != 1) throw std::invalid_argument("split is 0 (per-step filter) or 1 (rebuild-time rows)");
+  tune_.split = split;
+  if (split) {
+    if (w_.cap > 0 && w_.MV > 0) allocate_split();
+  } else {
+    w_.split = 0;
+    dfree(&w_.cl);
+    dfree(&w_.cpos);
+    dfree(&w_.cq);
+    dfree(&w_.nrc);
+    dfree(&w_.nac);
+  }
+  box_changed_ = true;
+  tune_.tuned = true;
 }
 
 void NepEngine::reserve(int capacity)
@@ -497,14 +535,17 @@ void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int 
   const double rv = model_.rcR_max + kSkin;
   prof_.begin(kProfRebuild, s);
   check(launch_rebuild(p.box, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w_, s), "rebuild");
+  if (w_.split) check(launch_split(p.box, dm_, nd1, nd2, w_, s, tune_.filter_lanes), "candidate rows");
   prof_.end(s);
   const bool acc = (flags & 1u) != 0;
   // ANN: FP32 on the matrix cores unless BF16 operands or the vector path are asked for
   const int ann = ann_mode(flags);
   launch_filter_descriptor(p.box, N, nd1, nd2, type, pe, acc, ann, true, s, true);
   prof_.begin(kProfGather, s);
-  check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s, tune_.gather_lanes), "gather");
+  check(launch_gather(p.box, dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w_, s, tune_.gather_lanes),
+        "gather");
   prof_.end(s);
+  last_box_ = p.box;
   last_N_stats_ = N;
   last_c1_ = nd1;
   last_c2_ = nd2;
@@ -659,16 +700,20 @@ void NepEngine::launch_filter_descriptor(const DevBox& b, int N, int c1, int c2,
 {
   const int n = c2 - c1;
   // per-kernel event timing (profiling mode 1) keeps the sequential order,
-  // so that each kernel group is timed alone
-  const bool split = tune_.overlap && n >= 2 * kOverlapMinCentres && !(profile && prof_.on() && !prof_.attach_mode());
+  // so that each kernel group is timed alone; candidate rows have no filter
+  // pass (the descriptor, a single launch, closes the rebuild request)
+  const bool split = !w_.split && tune_.overlap && n >= 2 * kOverlapMinCentres &&
+                     !(profile && prof_.on() && !prof_.attach_mode());
   if (!split) {
-    if (profile) prof_.begin(kProfFilter, s);
-    check(launch_filter(b, dm_, c1, c2, w_, s, tune_.filter_lanes, close), "filter");
-    if (profile) prof_.end(s);
+    if (!w_.split) {
+      if (profile) prof_.begin(kProfFilter, s);
+      check(launch_filter(b, dm_, c1, c2, w_, s, tune_.filter_lanes, close), "filter");
+      if (profile) prof_.end(s);
+    }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const bool attached = profile && prof_.attach(kProfDescriptor, &ev0, &ev1);
     if (profile && !attached) prof_.begin(kProfDescriptor, s);
-    check(launch_descriptor(dmd_, N, c1, c2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
+    check(launch_descriptor(b, dmd_, N, c1, c2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
     if (profile && !attached) prof_.end(s);
     return;
   }
@@ -685,7 +730,7 @@ void NepEngine::launch_filter_descriptor(const DevBox& b, int N, int c1, int c2,
     if (h == 1) check(hipStreamWaitEvent(s, join_, 0), "join wait");
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (profile) prof_.attach(kProfDescriptor, &ev0, &ev1);
-    check(launch_descriptor(dmd_, N, h ? mid : c1, h ? c2 : mid, type, pe, acc, ann, w_, s, ev0, ev1),
+    check(launch_descriptor(b, dmd_, N, h ? mid : c1, h ? c2 : mid, type, pe, acc, ann, w_, s, ev0, ev1),
           "descriptor");
   }
 }
@@ -695,7 +740,7 @@ void NepEngine::launch_step_kernels(const DevBox& b, int N, int nc, const int* t
 {
   const int ann = ann_mode(flags);
   launch_filter_descriptor(b, N, 0, nc, type, pe, false, ann, true, s, false);
-  check(launch_gather(dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, tune_.gather_lanes), "gather");
+  check(launch_gather(b, dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, tune_.gather_lanes), "gather");
 }
 
 const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, const int* type, const double* pos,
@@ -705,6 +750,8 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
   int reps[3];
   if (N <= 0 || nc <= 0 || nc > N || small_box(box, reps)) return tune_;  // small boxes keep the defaults
   flags &= ~1u;
+  // the filter's shapes first, then the candidate rows against them
+  if (w_.split) set_split(0);
   compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags, s, map);
   flags &= ~2u;
   // rows, edges, adjoints and partials are current: the filter rewrites its
@@ -742,7 +789,7 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
     best = 1;
     for (int k = 0; k < 3; ++k) {
       tune_.gather_ms[k] = time_median([&] {
-        check(launch_gather(dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, gl[k]), "tune gather");
+        check(launch_gather(b, dm_, N, 0, nc, 0, nc, type, force, virial, false, w_, s, gl[k]), "tune gather");
       });
       if (tune_.gather_ms[k] < tune_.gather_ms[best]) best = k;
     }
@@ -787,6 +834,49 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
     }
     tune_.overlap = t[1] < t[0] ? 1 : 0;
   }
+  // candidate rows split at rebuild time, with every gather shape, where
+  // their rows fit comfortably; kept if the step's force kernels are faster
+  {
+    double t_filter = tune_.force_ms[tune_.row_layout];
+    if (tune_.overlap_ms[tune_.overlap] > 0.0) t_filter = std::min(t_filter, tune_.overlap_ms[tune_.overlap]);
+    tune_.split_ms[0] = t_filter;
+    tune_.split_ms[1] = 0.0;
+    size_t free_s = 0, total_s = 0;
+    check(hipMemGetInfo(&free_s, &total_s), "memory info");
+    const double need = (double)w_.cap * (w_.MV * kSplitEntryBytes + 8.0);
+    if (need < 0.5 * (double)free_s) {
+      const Tuning first = tune_;
+      bool ok = true;
+      try {
+        set_split(1);
+        compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags | 2u, s, map);
+      } catch (const std::runtime_error&) {
+        ok = false;  // no room for the candidate rows
+        (void)hipGetLastError();
+      }
+      double best = 0.0;
+      int best_g = tune_.gather_lanes;
+      if (ok) {
+        const int gl[3] = {4, 8, 16};
+        for (int g : gl) {
+          tune_.gather_lanes = g;
+          const double t = time_median([&] { launch_step_kernels(b, N, nc, type, pe, force, virial, flags, s); });
+          if (best == 0.0 || t < best) {
+            best = t;
+            best_g = g;
+          }
+        }
+      }
+      if (ok && best < t_filter) {
+        tune_.gather_lanes = best_g;
+        tune_.split_ms[1] = best;
+      } else {
+        set_split(0);
+        tune_ = first;
+        tune_.split_ms[1] = best;
+      }
+    }
+  }
   for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
   tune_.tuned = true;
   // the outputs as a step with the chosen shapes writes them (a rebuild
@@ -826,7 +916,7 @@ struct Carver {
 
 // The workspace of compute_in for cap slots, rows of mv entries and ncell
 // cells, carved from `base` (or only measured when base is null).
-DevWork carve_work(char* base, int cap, int mv, long long ncell, int T, int kRp, size_t* bytes)
+DevWork carve_work(char* base, int cap, int mv, long long ncell, int T, int kRp, bool split, size_t* bytes)
 {
   Carver c{base};
   DevWork w{};
@@ -851,6 +941,14 @@ DevWork carve_work(char* base, int cap, int mv, long long ncell, int T, int kRp,
   w.vsa = c.take<int>(rows);
   w.edge = c.take<float4>(rows);
   w.f12 = c.take<float4>(rows);
+  if (split) {
+    w.split = 1;
+    w.cl = c.take<int>(rows);
+    w.cpos = c.take<int>(rows);
+    w.cq = c.take<int>(rows);
+    w.nrc = c.take<int>(n);
+    w.nac = c.take<int>(n);
+  }
   w.cap = cap;
   w.MV = w.MR = w.MA = mv;
   w.ncell_cap = (int)ncell;
@@ -885,11 +983,12 @@ void NepEngine::compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int
   const int mv = padded_rows(mv_need, tune_.row_layout);
   const int kRp = (model_.kR + 3) & ~3;
   size_t need = 0;
-  carve_work(nullptr, N, mv, ncell, model_.num_types, kRp, &need);
+  const bool split = tune_.split != 0;
+  carve_work(nullptr, N, mv, ncell, model_.num_types, kRp, split, &need);
   if (need > ws_bytes) {
     size_t one = 0, none = 0;
-    carve_work(nullptr, 1, mv, ncell, model_.num_types, kRp, &one);
-    carve_work(nullptr, 0, mv, ncell, model_.num_types, kRp, &none);
+    carve_work(nullptr, 1, mv, ncell, model_.num_types, kRp, split, &one);
+    carve_work(nullptr, 0, mv, ncell, model_.num_types, kRp, split, &none);
     const size_t per_atom = std::max<size_t>(one - none, 1);
     throw std::invalid_argument("the system needs " + std::to_string(need) + " bytes of workspace (" +
                                 std::to_string(N) + " slots, rows of " + std::to_string(mv) + ", " +
@@ -898,7 +997,7 @@ void NepEngine::compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int
                                 std::to_string(ws_bytes > none ? (ws_bytes - none) / per_atom : 0) +
                                 " slots of this model in this box)");
   }
-  DevWork w = carve_work(static_cast<char*>(ws), N, mv, ncell, model_.num_types, kRp, &need);
+  DevWork w = carve_work(static_cast<char*>(ws), N, mv, ncell, model_.num_types, kRp, split, &need);
   // the workspace keeps the previous call's contents: clear the status words
   // and the cell counts (the cell scan expects them at zero) with a kernel
   check(launch_clear_ints(w.flags, kNumFlags, s), "clear status words");
@@ -906,11 +1005,12 @@ void NepEngine::compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int
   check(launch_pack(b, N, model_.num_types, pos, type, true, w, s), "pack");
   const double rv = model_.rcR_max + kSkin;
   check(launch_rebuild(b, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w, s), "rebuild");
-  check(launch_filter(b, dm_, nd1, nd2, w, s, tune_.filter_lanes), "filter");
+  if (split) check(launch_split(b, dm_, nd1, nd2, w, s, tune_.filter_lanes), "candidate rows");
+  else check(launch_filter(b, dm_, nd1, nd2, w, s, tune_.filter_lanes), "filter");
   const bool acc = (flags & 1u) != 0;
   const int ann = ann_mode(flags);
-  check(launch_descriptor(dmd_, N, nd1, nd2, type, pe, acc, ann, w, s), "descriptor");
-  check(launch_gather(dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w, s, tune_.gather_lanes), "gather");
+  check(launch_descriptor(b, dmd_, N, nd1, nd2, type, pe, acc, ann, w, s), "descriptor");
+  check(launch_gather(b, dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w, s, tune_.gather_lanes), "gather");
   check(hipMemcpyAsync(hflags_, w.flags, kNumFlags * sizeof(int), hipMemcpyDeviceToHost, s), "read status words");
   check(hipStreamSynchronize(s), "synchronise");
   if (hflags_[kFlagBadType]) throw std::invalid_argument("an atom type lies outside [0, num_types)");
@@ -928,9 +1028,14 @@ double NepEngine::bytes_per_slot() const
   if (mv <= 0)
     mv = padded_rows(std::max(8, (int)std::ceil(model_.MN_R * std::pow((model_.rcR_max + kSkin) / model_.rcR_max, 3))),
                      tune_.row_layout);
-  const double per = 2 * sizeof(double4) + 6 * sizeof(int) + 3 * sizeof(double) +
-                     (double)model_.num_types * ((model_.kR + 3) & ~3) * sizeof(float);
-  return per + kRowEntryBytes * mv;
+  double per = 2 * sizeof(double4) + 6 * sizeof(int) + 3 * sizeof(double) +
+               (double)model_.num_types * ((model_.kR + 3) & ~3) * sizeof(float);
+  double entry = kRowEntryBytes;
+  if (tune_.split) {
+    per += 2 * sizeof(int);
+    entry += kSplitEntryBytes;
+  }
+  return per + entry * mv;
 }
 
 void NepEngine::stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds)
@@ -999,6 +1104,11 @@ void NepEngine::copy_pair_rows(int which, int c1, int c2, int max_nb, int* count
   if (!have_list_) throw std::runtime_error("no neighbour list yet");
   if (c1 < last_c1_ || c2 > last_c2_ || c2 < c1) throw std::invalid_argument("rows outside the centre range");
   const int N = last_N_stats_;
+  // candidate rows keep the step's angular rows (written by the descriptor)
+  // but no radial rows: for those the filter pass runs once on the last
+  // evaluation's positions
+  if (w_.split && which == 0)
+    check(launch_filter(last_box_, dm_, last_c1_, last_c2_, w_, nullptr, tune_.filter_lanes, false), "filter");
   check(hipDeviceSynchronize(), "sync");
   std::vector<int> nr(N), na(N);
   std::vector<float4> e((size_t)w_.MR * N);

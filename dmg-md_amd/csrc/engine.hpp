@@ -128,12 +128,21 @@ class NepEngine {
     // memory-bound filter overlaps the VALU-bound descriptor)
     int overlap = 0;
     double overlap_ms[2] = {0, 0};      // filter + descriptor + gather without / with the overlap
+    // 1: the rows are split by type pair at rebuild time into candidate rows
+    // (every Verlet entry within its pair's rc + skin) and the step has no
+    // filter pass: the descriptor and the gather form the distances from the
+    // packed positions, and the descriptor writes the step's angular edges
+    int split = 0;
+    double split_ms[2] = {0, 0};        // the step's force kernels with the filter / with candidate rows
     bool tuned = false;
   };
   const Tuning& tuning() const { return tune_; }
   // Sets the shapes directly (rejects other values); marks them tuned, so
   // a server does not re-tune them
   void set_tiles(int filter_lanes, int gather_lanes);
+  // Selects the per-step filter (0) or the rebuild-time split (1); the next
+  // evaluation rebuilds the rows
+  void set_split(int split);
   // (centres [0, nc); `map` as for compute)
   const Tuning& autotune(const Box& box, int N, int nc, const int* type, const double* pos, double* pe,
                          double* force, double* virial, unsigned flags, hipStream_t stream,
@@ -185,6 +194,7 @@ class NepEngine {
                      hipStream_t stream);
   void ensure_cells(int ncell);
   void allocate(int capacity, int mv);
+  void allocate_split();
   void free_all();
 
   NepModel model_;
@@ -212,6 +222,7 @@ class NepEngine {
   // (first) descriptor dispatch when given
   void launch_filter_descriptor(const DevBox& b, int N, int c1, int c2, const int* type, double* pe, bool acc,
                                 int ann, bool close, hipStream_t s, bool profile);
+  DevBox last_box_{};  // box of the last evaluation (the filter of copy_pair_rows)
   hipStream_t side_ = nullptr;  // the overlapped filter's stream
   hipEvent_t fork_ = nullptr, join_ = nullptr;
   int last_N_stats_ = 0, last_c1_ = 0, last_c2_ = 0;

@@ -234,24 +234,29 @@ constexpr int kFilterBatch = DMGX_FILTER_BATCH;
 constexpr int kGatherLanes = DMGX_GATHER_LANES;
 constexpr int kGatherBatch = DMGX_GATHER_BATCH;  // radial edges per lane per batch (kR <= 9)
 
-template <bool ONE, bool MAP, int G, int RB>
+// CAND: the rebuild-time split of the rows into candidate rows instead (the
+// same pass with every cutoff widened by the skin, writing slot indices
+// instead of edge records); it runs only when this step rebuilt the rows.
+template <bool ONE, bool MAP, int G, int RB, bool CAND = false>
 __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1, int c2, int nblocks, int close,
                                                  DevWork w)
 {
   constexpr int PER = kBlock / G;
   static_assert(G >= 1 && G <= 32 && (G & (G - 1)) == 0, "filter lanes: a power of two up to 32");
   constexpr unsigned long long gmask = (1ull << G) - 1ull;
+  if (CAND && !w.flags[kFlagRebuild]) return;
   __shared__ double cut2[ONE ? 1 : 2 * kMaxTypes * kMaxTypes];  // [ti][tj] radial, then angular
   const int T = m.T;
+  const double widen = CAND ? kSkin : 0.0;
   double rr2 = 0.0, ra2 = 0.0;
   if (ONE) {
-    const double rr = (m.rcR[0] + m.rcR[0]) * 0.5, ra = (m.rcA[0] + m.rcA[0]) * 0.5;
+    const double rr = (m.rcR[0] + m.rcR[0]) * 0.5 + widen, ra = (m.rcA[0] + m.rcA[0]) * 0.5 + widen;
     rr2 = rr * rr;
     ra2 = ra * ra;
   } else {
     for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
       const int p = k / T, q = k % T;
-      const double rr = (m.rcR[p] + m.rcR[q]) * 0.5, ra = (m.rcA[p] + m.rcA[q]) * 0.5;
+      const double rr = (m.rcR[p] + m.rcR[q]) * 0.5 + widen, ra = (m.rcA[p] + m.rcA[q]) * 0.5 + widen;
       cut2[k] = rr * rr;
       cut2[T * T + k] = ra * ra;
     }
@@ -259,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
   }
   // the rebuild kernels of this step ran before this one: close the request
   // (one filter launch per evaluation does; the others of a split one skip)
-  if (close && blockIdx.x == 0 && threadIdx.x == 0 && w.flags[kFlagRebuild]) {
+  if (!CAND && close && blockIdx.x == 0 && threadIdx.x == 0 && w.flags[kFlagRebuild]) {
     w.flags[kFlagRebuild] = 0;
     w.flags[kFlagRebuildCount] += 1;
   }
@@ -321,14 +326,28 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
       const unsigned gR = (unsigned)((__ballot(isR) >> (gw * G)) & gmask);
       // every row entry gets its angular slot or -1: one contiguous store
       const int a = na + __popc(gA & below);
-      if (jj[r] >= 0) w.vsa[vrow + p] = (isA && a < w.MA) ? a : -1;
-      if (isA && a < w.MA) {
-        w.edge[erow + a] = rec;
-        w.eq[arow + a] = qq[r] >= 0 ? jo[r] * w.MV + qq[r] : -1;
-      }
-      if (isR) {
-        const int slot = w.MR - 1 - (nro + __popc(gR & below));
-        if (slot >= 0) w.edge[erow + slot] = rec;
+      if (CAND) {
+        // candidate rows hold the slot itself (its position is read every
+        // step); the reverse index becomes a candidate slot in k_split_reverse
+        if (jj[r] >= 0) w.cpos[vrow + p] = (isA && a < w.MA) ? a : -1;
+        if (isA && a < w.MA) {
+          w.cl[vrow + a] = jj[r];
+          w.cq[arow + a] = qq[r] >= 0 ? jo[r] * w.MV + qq[r] : -1;
+        }
+        if (isR) {
+          const int slot = w.MV - 1 - (nro + __popc(gR & below));
+          if (slot >= 0) w.cl[vrow + slot] = jj[r];
+        }
+      } else {
+        if (jj[r] >= 0) w.vsa[vrow + p] = (isA && a < w.MA) ? a : -1;
+        if (isA && a < w.MA) {
+          w.edge[erow + a] = rec;
+          w.eq[arow + a] = qq[r] >= 0 ? jo[r] * w.MV + qq[r] : -1;
+        }
+        if (isR) {
+          const int slot = w.MR - 1 - (nro + __popc(gR & below));
+          if (slot >= 0) w.edge[erow + slot] = rec;
+        }
       }
       na += __popc(gA);
       nro += __popc(gR);
@@ -336,8 +355,41 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
   }
   // na + nro <= nv <= MV = MR = MA: the two ends of a row never meet
   if (active && lg == 0) {
-    w.nr[i] = min(na + nro, w.MR);
-    w.na[i] = min(na, w.MA);
+    if (CAND) {
+      w.nrc[i] = min(na + nro, w.MV);
+      w.nac[i] = min(na, w.MA);
+    } else {
+      w.nr[i] = min(na + nro, w.MR);
+      w.na[i] = min(na, w.MA);
+    }
+  }
+}
+
+// Rebuild-time reverse of the candidate rows: an angular candidate i->j's
+// Verlet reverse index j*MV + q becomes j*MA + (i's angular candidate slot
+// in j's row), or -1 where j has no row in [c1, c2) or i is not one of its
+// angular candidates.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_split_reverse(int c1, int c2, int nblocks, DevWork w)
+{
+  if (!w.flags[kFlagRebuild]) return;
+  constexpr int PER = kBlock / G;
+  const int g = threadIdx.x / G, lg = threadIdx.x % G;
+  const int i = c1 + xcd_tile(nblocks) * PER + g;
+  if (i >= c2) return;
+  const int n = w.nac[i];
+  const size_t arow = (size_t)i * w.MA;
+  for (int a = lg; a < n; a += G) {
+    const int q = w.cq[arow + a];
+    int v = -1;
+    if (q >= 0) {
+      const int j = q / w.MV;
+      if (j >= c1 && j < c2) {
+        const int c = w.cpos[q];
+        if (c >= 0) v = j * w.MA + c;
+      }
+    }
+    w.cq[arow + a] = v;
   }
 }
 
@@ -679,7 +731,7 @@ template <class C>
 constexpr int desc_lds_bytes()
 {
   const int pairs = C::ltypes * C::ltypes;
-  const int pcut = C::one ? 1 : 4 * pairs;
+  const int pcut = C::one ? 1 + 2 : 4 * pairs + 4 * pairs;  // float cutoffs, then the FP64 squares
   const int ctab = C::cl ? pairs * (C::nr * C::kr + C::na * C::ka) : 1;
   const int mfma = C::ann != kAnnValu;
   const int annw = mfma ? (C::one ? 1 : kTypesMfma) * ann_w_floats<C::dim, C::ann == kAnnMfmaSplit>() : 1;
@@ -696,10 +748,15 @@ constexpr bool mfma_ann_fits()
          (ONE ? 64 * 1024 : kDescLdsMax);
 }
 
-template <class C>
-__global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, int c1, int c2, int nblocks,
+// CAND: the edges come from the candidate rows (DevWork::split).  edge, vsa
+// and eq are w.edge, w.vsa and w.eq, passed as restrict parameters: with
+// candidate rows this kernel writes them, and only through them, so the
+// model's tables stay provably unmodified (scalar loads).
+template <class C, bool CAND>
+__global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m, int N, int c1, int c2, int nblocks,
                                                        const int* __restrict__ type, double* __restrict__ pe,
-                                                       int accumulate, DevWork w)
+                                                       float4* __restrict__ edge, int* __restrict__ vsa,
+                                                       int* __restrict__ eq, int accumulate, DevWork w)
 {
   constexpr int NR = C::nr, KR = C::kr, NA = C::na, KA = C::ka, LM = C::lm, NABC = C::nabc, DMAX = C::dim;
   const int i0 = c1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
@@ -713,22 +770,32 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   constexpr int kPairs = C::ltypes * C::ltypes;
   static_assert(desc_lds_bytes<C>() <= kDescLdsMax, "descriptor LDS above its bound per workgroup");
   __shared__ float pcut[C::one ? 1 : 4 * kPairs];
+  __shared__ double pcut2[C::one ? 1 : 2 * kPairs];  // FP64 squared pair cutoffs (candidate rows)
   __shared__ float ctab[C::cl ? kPairs * (NR * KR + NA * KA) : 1];
   if (!C::one) {
     const int T = m.T;
     for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
-      const int a = k / T, b = k % T;
-      const float rr = (float)((m.rcR[a] + m.rcR[b]) * 0.5), ra = (float)((m.rcA[a] + m.rcA[b]) * 0.5);
+      const int p = k / T, q = k % T;
+      const double rrd = (m.rcR[p] + m.rcR[q]) * 0.5, rad = (m.rcA[p] + m.rcA[q]) * 0.5;
+      const float rr = (float)rrd, ra = (float)rad;
       pcut[k] = rr;
       pcut[kPairs + k] = 1.0f / rr;
       pcut[2 * kPairs + k] = ra;
       pcut[3 * kPairs + k] = 1.0f / ra;
+      pcut2[k] = rrd * rrd;
+      pcut2[kPairs + k] = rad * rad;
     }
     if (C::cl) {
       const int nrad = T * T * NR * KR, nang = T * T * NA * KA;
       for (int k = threadIdx.x; k < nrad + nang; k += blockDim.x) ctab[k] = k < nrad ? m.cR[k] : m.cA[k - nrad];
     }
     __syncthreads();
+  }
+  // candidate rows: the rebuild kernels of this step ran before this launch
+  // (the only descriptor launch of the evaluation), which closes the request
+  if (CAND && blockIdx.x == 0 && threadIdx.x == 0 && w.flags[kFlagRebuild]) {
+    w.flags[kFlagRebuild] = 0;
+    w.flags[kFlagRebuildCount] += 1;
   }
   if (C::ann == kAnnValu && !active) return;
   __shared__ float ann_w[C::ann != kAnnValu ? (C::one ? 1 : kTypesMfma) * ann_w_floats<DMAX, C::ann == kAnnMfmaSplit>()
@@ -755,8 +822,13 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
     rcRf[0] = (float)m.rcR[0];
     rcAf[0] = (float)m.rcA[0];
   }
-  const int nr = active ? w.nr[i] : 0, na = active ? w.na[i] : 0;
-  const float4* __restrict__ erow = w.edge + (size_t)i * w.MR;
+  int nr = 0, na = 0;
+  if (active && !CAND) {
+    nr = w.nr[i];
+    na = w.na[i];
+  }
+  // with candidate rows this kernel writes the angular edges it reads back
+  const float4* erow = edge + (size_t)i * w.MR;
   float4* __restrict__ frow = w.f12 + (size_t)i * w.MA;
   float q[DMAX];
 #pragma unroll
@@ -779,27 +851,10 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
   // the current one and the first record of the next 128-byte line a line
   // ahead
   constexpr int kAhead = 8;
-  auto ridx = [&](int e) { return e < na ? e : w.MR - 1 - (e - na); };
-  float4 ring[kAhead];
-#pragma unroll
-  for (int u = 0; u < kAhead; ++u) ring[u] = u < nr ? erow[ridx(u)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  for (int e0 = 0; e0 < nr; e0 += kAhead)
-#pragma unroll
-  for (int u = 0; u < kAhead; ++u) {
-    const int e = e0 + u;
-    if (e >= nr) break;
-    const float4 r = ring[u];
-    if (e + kAhead < nr) ring[u] = erow[ridx(e + kAhead)];
+  auto radial_edge = [&](const float4 r, const int tj) {
     const float d2 = r.x * r.x + r.y * r.y + r.z * r.z;
     const float d = d2 * __builtin_amdgcn_rsqf(d2);  // edges have d > 0
-    float rc;
-    int tj = 0;
-    if (C::one) {
-      rc = rcRf[0];
-    } else {
-      tj = safe_type(type[__float_as_int(r.w)], T);
-      rc = pcut[ti * T + tj];
-    }
+    const float rc = C::one ? rcRf[0] : pcut[ti * T + tj];
     const float rcinv = C::one ? 1.0f / rc : pcut[kPairs + ti * T + tj];
     float fc;
     cutoff_fc(rc, rcinv, d, fc);
@@ -830,6 +885,88 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
           q[n] += g;
         }
       }
+    }
+  };
+  if (!CAND) {
+    auto ridx = [&](int e) { return e < na ? e : w.MR - 1 - (e - na); };
+    float4 ring[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) ring[u] = u < nr ? erow[ridx(u)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int e0 = 0; e0 < nr; e0 += kAhead)
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u) {
+        const int e = e0 + u;
+        if (e >= nr) break;
+        const float4 r = ring[u];
+        if (e + kAhead < nr) ring[u] = erow[ridx(e + kAhead)];
+        radial_edge(r, C::one ? 0 : safe_type(type[__float_as_int(r.w)], T));
+      }
+  } else {
+    // candidate rows: the FP64 minimum-image distance of every candidate
+    // (the filter's expressions, so the same edges pass), the radial sum over
+    // those within the pair's radial cutoff, and the step's angular edges
+    // (in candidate order) with their reverse indices and the candidates'
+    // slot status for the gather.  Slot indices are read eight candidates
+    // ahead, the packed positions four.
+    const int nrc = active ? w.nrc[i] : 0, nac = active ? w.nac[i] : 0;
+    const size_t vrow = (size_t)i * w.MV, arow = (size_t)i * w.MA;
+    const int* __restrict__ crow = w.cl + vrow;
+    float4* ew = edge + (size_t)i * w.MR;
+    const double4 pi = w.xq[i];
+    double cr1 = 0.0, ca1 = 0.0;
+    if (C::one) {
+      const double rr = (m.rcR[0] + m.rcR[0]) * 0.5, ra = (m.rcA[0] + m.rcA[0]) * 0.5;
+      cr1 = rr * rr;
+      ca1 = ra * ra;
+    }
+    auto cidx = [&](int e) { return e < nac ? e : w.MV - 1 - (e - nac); };
+    constexpr int kSplitAhead = 4;
+    int jr[2 * kSplitAhead];
+    double4 pr[kSplitAhead];
+#pragma unroll
+    for (int u = 0; u < 2 * kSplitAhead; ++u) jr[u] = u < nrc ? crow[cidx(u)] : 0;
+#pragma unroll
+    for (int u = 0; u < kSplitAhead; ++u) pr[u] = u < nrc ? w.xq[jr[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
+    int nin = 0;
+    for (int e0 = 0; e0 < nrc; e0 += 2 * kSplitAhead)
+#pragma unroll
+      for (int u = 0; u < 2 * kSplitAhead; ++u) {
+        const int e = e0 + u;
+        if (e >= nrc) break;
+        const int j = jr[u];
+        const double4 pj = pr[u % kSplitAhead];
+        if (e + kSplitAhead < nrc) pr[u % kSplitAhead] = w.xq[jr[(u + kSplitAhead) % (2 * kSplitAhead)]];
+        if (e + 2 * kSplitAhead < nrc) jr[u] = crow[cidx(e + 2 * kSplitAhead)];
+        double x = pj.x - pi.x, y = pj.y - pi.y, z = pj.z - pi.z;
+        mic(b, x, y, z);
+        const double d2 = dist2(x, y, z);
+        int tj = 0;
+        double cr = cr1, ca = ca1;
+        if (!C::one) {
+          tj = safe_type((int)__double_as_longlong(pj.w), T);
+          cr = pcut2[ti * T + tj];
+          ca = pcut2[kPairs + ti * T + tj];
+        }
+        const bool in_r = d2 < cr;
+        const int jo = w.map.owner ? w.map.owner[j] : j;
+        const float4 r = make_float4((float)x, (float)y, (float)z, __int_as_float(jo));
+        if (e < nac) {
+          const bool in_a = in_r && d2 < ca;
+          vsa[arow + e] = in_a ? na : -1;
+          if (in_a) {
+            ew[na] = r;
+            eq[arow + na] = w.cq[arow + e];
+            ++na;
+          }
+        }
+        if (in_r) {
+          ++nin;
+          radial_edge(r, tj);
+        }
+      }
+    if (active) {
+      w.nr[i] = nin;
+      w.na[i] = na;
     }
   }
   if (C::one) {
@@ -1151,10 +1288,12 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevModel m, int N, in
 // gather: radial pair force, angular partials, ZBL -> FP64 force/virial
 // ---------------------------------------------------------------------------
 
-template <int KR, bool ONE, int G>
-__global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, int nd1, int nd2, int nblocks,
-                                                   const int* __restrict__ type, int N, double* __restrict__ force,
-                                                   double* __restrict__ virial, int accumulate, DevWork w)
+// CAND: the radial edges come from the candidate rows (DevWork::split)
+template <int KR, bool ONE, int G, bool CAND>
+__global__ __launch_bounds__(kBlock) void k_gather(DevBox b, DevModel m, int n1, int n2, int nd1, int nd2,
+                                                   int nblocks, const int* __restrict__ type, int N,
+                                                   double* __restrict__ force, double* __restrict__ virial,
+                                                   int accumulate, DevWork w)
 {
   // G lanes per atom share its edges; the per-lane FP64 sums are combined
   // with a butterfly over the group at the end.  Each lane takes its edges
@@ -1168,12 +1307,22 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
   const bool active = i < n2;
   const int T = ONE ? 1 : m.T;
   const int kR = m.kR;
-  int ti = 0, nr = 0, na = 0;
+  int ti = 0, nr = 0, na = 0, nac = 0;
+  double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
   if (active) {
     ti = ONE ? 0 : safe_type(type[i], T);
-    nr = w.nr[i];
     na = w.na[i];
+    if (CAND) {
+      // candidate rows: the radial loop runs over the candidates and keeps
+      // those within the pair's radial cutoff (the descriptor's test)
+      nr = w.nrc[i];
+      nac = w.nac[i];
+      pi = w.xq[i];
+    } else {
+      nr = w.nr[i];
+    }
   }
+  const int* __restrict__ crow = w.cl + (size_t)i * w.MV;
   constexpr int KP = (KR + 3) & ~3;  // padded row: KP/4 float4 loads
   const int kRp = m.kRp;
   const float4* __restrict__ erow = w.edge + (size_t)i * w.MR;
@@ -1191,14 +1340,42 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevModel m, int n1, int n2, i
   for (int e0 = lg; e0 < nr; e0 += G * RB) {
     float4 rr[RB];
     int jj[RB], tt[RB];
+    if (!CAND) {
 #pragma unroll
-    for (int r = 0; r < RB; ++r) {
-      const int e = e0 + r * G;
-      rr[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-      jj[r] = -1;
-      if (e < nr) {
-        rr[r] = erow[e < na ? e : w.MR - 1 - (e - na)];
-        jj[r] = __float_as_int(rr[r].w);
+      for (int r = 0; r < RB; ++r) {
+        const int e = e0 + r * G;
+        rr[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        jj[r] = -1;
+        if (e < nr) {
+          rr[r] = erow[e < na ? e : w.MR - 1 - (e - na)];
+          jj[r] = __float_as_int(rr[r].w);
+        }
+      }
+    } else {
+      int jc[RB];
+      double4 pj[RB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int e = e0 + r * G;
+        jc[r] = e < nr ? crow[e < nac ? e : w.MV - 1 - (e - nac)] : -1;
+      }
+#pragma unroll
+      for (int r = 0; r < RB; ++r) pj[r] = jc[r] >= 0 ? w.xq[jc[r]] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        rr[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        jj[r] = -1;
+        if (jc[r] >= 0) {
+          double x = pj[r].x - pi.x, y = pj[r].y - pi.y, z = pj[r].z - pi.z;
+          mic(b, x, y, z);
+          const double d2 = dist2(x, y, z);
+          double rc = (m.rcR[0] + m.rcR[0]) * 0.5;
+          if (!ONE) rc = (m.rcR[ti] + m.rcR[safe_type((int)__double_as_longlong(pj[r].w), T)]) * 0.5;
+          if (d2 < rc * rc) {
+            jj[r] = w.map.owner ? w.map.owner[jc[r]] : jc[r];
+            rr[r] = make_float4((float)x, (float)y, (float)z, __int_as_float(jj[r]));
+          }
+        }
       }
     }
 #pragma unroll
@@ -1677,18 +1854,32 @@ __global__ void k_close_request(int* __restrict__ flags)
   }
 }
 
-template <int G, int RB>
+template <int G, int RB, bool CAND = false>
 static void run_filter(const DevBox& b, const DevModel& m, int c1, int c2, int close, DevWork& w, hipStream_t s)
 {
   constexpr int per = kBlock / G;
   const int nb = (c2 - c1 + per - 1) / per;
   if (w.map.owner) {
-    if (m.T == 1) k_filter<true, true, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
-    else k_filter<false, true, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
+    if (m.T == 1) k_filter<true, true, G, RB, CAND><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
+    else k_filter<false, true, G, RB, CAND><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
   } else {
-    if (m.T == 1) k_filter<true, false, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
-    else k_filter<false, false, G, RB><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
+    if (m.T == 1) k_filter<true, false, G, RB, CAND><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
+    else k_filter<false, false, G, RB, CAND><<<xcd_grid(nb), kBlock, 0, s>>>(b, m, c1, c2, nb, close, w);
   }
+  if (CAND) k_split_reverse<G><<<xcd_grid(nb), kBlock, 0, s>>>(c1, c2, nb, w);
+}
+
+hipError_t launch_split(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s, int lanes)
+{
+  if (!w.split || !w.cl) return hipErrorInvalidValue;
+  if (c2 > c1) {
+    if (lanes == 8) run_filter<8, kFilterBatch, true>(b, m, c1, c2, 0, w, s);
+    else if (lanes == 32) run_filter<32, 4, true>(b, m, c1, c2, 0, w, s);
+    else run_filter<kFilterLanes, kFilterBatch, true>(b, m, c1, c2, 0, w, s);
+  } else {
+    k_close_request<<<1, 1, 0, s>>>(w.flags);  // no descriptor launch closes it
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s, int lanes,
@@ -1706,75 +1897,76 @@ hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, Dev
 }
 
 template <class C>
-static void run_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc,
-                           DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+static void run_descriptor(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
+                           bool acc, DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
   const int nb = (c2 - c1 + kDescBlock - 1) / kDescBlock;
+  auto k = w.split ? k_descriptor<C, true> : k_descriptor<C, false>;
   if (e0 || e1)
-    hipExtLaunchKernelGGL(k_descriptor<C>, dim3(xcd_grid(nb)), dim3(kDescBlock), 0, s, e0, e1, 0, m, N, c1, c2, nb,
-                          type, pe, acc ? 1 : 0, w);
+    hipExtLaunchKernelGGL(k, dim3(xcd_grid(nb)), dim3(kDescBlock), 0, s, e0, e1, 0, b, m, N, c1, c2, nb, type, pe,
+                          w.edge, w.vsa, w.eq, acc ? 1 : 0, w);
   else
-    k_descriptor<C><<<xcd_grid(nb), kDescBlock, 0, s>>>(m, N, c1, c2, nb, type, pe, acc ? 1 : 0, w);
+    k<<<xcd_grid(nb), kDescBlock, 0, s>>>(b, m, N, c1, c2, nb, type, pe, w.edge, w.vsa, w.eq, acc ? 1 : 0, w);
 }
 
 // single-type exact configuration with the ANN mode picked at run time:
 // matrix cores for H <= 64 (FP32 or BF16 operands), vector ALUs otherwise
 template <int NR, int KR, int NA, int KA, bool Q2, bool Q1>
-static void run_single(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
+static void run_single(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
                        DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
   const int mode = m.H <= 64 ? ann : kAnnValu;
   if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, true>()) {
     if (mode == kAnnMfma32) {
-      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfma32>>(m, N, c1, c2, type, pe, acc, w, s, e0,
+      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfma32>>(b, m, N, c1, c2, type, pe, acc, w, s, e0,
                                                                             e1);
       return;
     }
     if (mode == kAnnMfmaBf16) {
-      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaBf16>>(m, N, c1, c2, type, pe, acc, w, s, e0,
+      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaBf16>>(b, m, N, c1, c2, type, pe, acc, w, s, e0,
                                                                               e1);
       return;
     }
     if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, true, kAnnMfmaSplit>()) {
       if (mode == kAnnMfmaSplit) {
-        run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaSplit>>(m, N, c1, c2, type, pe, acc, w, s,
+        run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfmaSplit>>(b, m, N, c1, c2, type, pe, acc, w, s,
                                                                                   e0, e1);
         return;
       }
     }
   }
-  run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+  run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1);
 }
 
 template <int NR, int KR, int NA, int KA, bool Q2, bool Q1>
-static void run_multi(const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
+static void run_multi(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
                       DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
   const int mode = (m.H <= 64 && m.T <= kTypesMfma) ? ann : kAnnValu;
   if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, false>()) {
     if (mode == kAnnMfma32) {
-      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfma32, true>>(m, N, c1, c2, type, pe, acc, w,
+      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfma32, true>>(b, m, N, c1, c2, type, pe, acc, w,
                                                                                    s, e0, e1);
       return;
     }
     if (mode == kAnnMfmaBf16) {
-      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfmaBf16, true>>(m, N, c1, c2, type, pe, acc,
+      run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfmaBf16, true>>(b, m, N, c1, c2, type, pe, acc,
                                                                                      w, s, e0, e1);
       return;
     }
     if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, false, kAnnMfmaSplit>()) {
       if (mode == kAnnMfmaSplit) {
-        run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfmaSplit, true>>(m, N, c1, c2, type, pe,
+        run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfmaSplit, true>>(b, m, N, c1, c2, type, pe,
                                                                                          acc, w, s, e0, e1);
         return;
       }
     }
   }
   if (m.T <= kTypesLds)
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnValu, true>>(m, N, c1, c2, type, pe, acc, w, s,
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnValu, true>>(b, m, N, c1, c2, type, pe, acc, w, s,
                                                                                e0, e1);
   else
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1);
 }
 
 namespace {
@@ -1811,7 +2003,7 @@ bool descriptor_bucket(const DevModel& m, int dims[4])
   return false;
 }
 
-hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
+hipError_t launch_descriptor(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
                              bool acc, int ann, DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
   if (c2 <= c1) return hipSuccess;
@@ -1823,32 +2015,32 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
   // exact instantiations: the configurations in use, and the bounded sizes
   // that other models are padded to (descriptor_bucket); generic otherwise
   if (is(5, 9, 5, 9, 4, true, true, true))
-    run_single<5, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_single<5, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, false, true))
-    run_single<7, 9, 5, 9, true, false>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_single<7, 9, 5, 9, true, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(13, 17, 9, 13, 4, true, true, true))
-    run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1);
   else if (is(7, 11, 7, 11, 4, true, true, false))
-    run_multi<7, 11, 7, 11, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<7, 11, 7, 11, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(5, 9, 5, 9, 4, true, false, false))
-    run_multi<5, 9, 5, 9, true, false>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<5, 9, 5, 9, true, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(9, 9, 7, 9, 4, false, false, false))
-    run_multi<9, 9, 7, 9, false, false>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<9, 9, 7, 9, false, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, true, true))
-    run_single<7, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_single<7, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(7, 9, 5, 9, 4, true, true, false))
-    run_multi<7, 9, 5, 9, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<7, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(9, 13, 7, 13, 4, true, true, true))
-    run_single<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_single<9, 13, 7, 13, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (is(9, 13, 7, 13, 4, true, true, false))
-    run_multi<9, 13, 7, 13, true, true>(m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<9, 13, 7, 13, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
   else if (m.L <= kPadL && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
-    run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, kPadL, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
+    run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, kPadL, true, true, false, false>>(b, m, N, c1, c2, type, pe, acc,
                                                                                       w, s, e0, e1);
   else if (m.L <= kMaxL && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
     // 3-body orders 5..8 (80 angular moments per radial function): the
     // generic instance at order 8, its per-atom arrays partly in scratch
-    run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, kMaxL, true, true, false, false>>(m, N, c1, c2, type, pe, acc,
+    run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, kMaxL, true, true, false, false>>(b, m, N, c1, c2, type, pe, acc,
                                                                                       w, s, e0, e1);
   else
     return hipErrorInvalidValue;
@@ -1856,30 +2048,30 @@ hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int
 }
 
 template <int G>
-static void run_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type, double* force,
-                       double* virial, int a, DevWork& w, hipStream_t s)
+static void run_gather(const DevBox& b, const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
+                       double* force, double* virial, int a, DevWork& w, hipStream_t s)
 {
   constexpr int per = kBlock / G;
   const int nb = (n2 - n1 + per - 1) / per;
   const int g = xcd_grid(nb);
-  if (m.T == 1 && m.kR == 9)
-    k_gather<9, true, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
-  else if (m.T == 1)
-    k_gather<kMaxK, true, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
-  else if (m.kR <= 11)
-    k_gather<11, false, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
-  else
-    k_gather<kMaxK, false, G><<<g, kBlock, 0, s>>>(m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
+  auto pick = [&](auto k_rows, auto k_cand) { return w.split ? k_cand : k_rows; };
+  decltype(&k_gather<9, true, G, false>) k;
+  if (m.T == 1 && m.kR == 9) k = pick(k_gather<9, true, G, false>, k_gather<9, true, G, true>);
+  else if (m.T == 1) k = pick(k_gather<kMaxK, true, G, false>, k_gather<kMaxK, true, G, true>);
+  else if (m.kR <= 11) k = pick(k_gather<11, false, G, false>, k_gather<11, false, G, true>);
+  else k = pick(k_gather<kMaxK, false, G, false>, k_gather<kMaxK, false, G, true>);
+  k<<<g, kBlock, 0, s>>>(b, m, n1, n2, nd1, nd2, nb, type, N, force, virial, a, w);
 }
 
-hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
-                         double* force, double* virial, bool acc, DevWork& w, hipStream_t s, int lanes)
+hipError_t launch_gather(const DevBox& b, const DevModel& m, int N, int n1, int n2, int nd1, int nd2,
+                         const int* type, double* force, double* virial, bool acc, DevWork& w, hipStream_t s,
+                         int lanes)
 {
   if (n2 <= n1) return hipSuccess;
   const int a = acc ? 1 : 0;
-  if (lanes == 4) run_gather<4>(m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
-  else if (lanes == 16) run_gather<16>(m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
-  else run_gather<kGatherLanes>(m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
+  if (lanes == 4) run_gather<4>(b, m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
+  else if (lanes == 16) run_gather<16>(b, m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
+  else run_gather<kGatherLanes>(b, m, N, n1, n2, nd1, nd2, type, force, virial, a, w, s);
   return hipGetLastError();
 }
 

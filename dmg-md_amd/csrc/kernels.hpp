@@ -94,6 +94,20 @@ struct DevWork {
   int* vsa = nullptr;          // [cap*MV] Verlet position -> angular slot, -1 if not angular
   float* bR = nullptr;         // [cap*T*kRp] radial adjoint in basis space
   float4* f12 = nullptr;       // [cap*MA] dE_i/dr_ij of angular edges
+  // Rows split by type pair at rebuild time (split != 0): the step has no
+  // filter pass; the descriptor and the gather read candidate rows, form the
+  // FP64 distances themselves, and the descriptor writes the step's angular
+  // edges (edge, eq, na), its slot status (vsa, indexed by candidate slot)
+  // and the radial count nr.
+  int split = 0;
+  int* cl = nullptr;           // [cap*MV] candidate slots: every Verlet entry within its pair's
+                               // rc + skin, angular candidates (rcA + skin) at 0..nac-1,
+                               // radial-only ones at MV-1 downward
+  int* cpos = nullptr;         // [cap*MV] Verlet position -> angular candidate slot, -1 if none
+  int* cq = nullptr;           // [cap*MA] of an angular candidate i->j: j*MA + (i's angular
+                               // candidate slot in j's row), an index into vsa; -1 if none
+  int* nrc = nullptr;          // [cap] candidates
+  int* nac = nullptr;          // [cap] angular candidates
   ImageMap map;                // periodic-image map of the current call (empty: none)
   int* flags = nullptr;        // [kNumFlags] device status words
   double* partial = nullptr;   // thermo block partials
@@ -138,6 +152,11 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
 // launch of an evaluation does)
 hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s,
                          int lanes = 16, bool close = true);
+// Rebuild-time split of the Verlet rows of centres [c1, c2) into candidate
+// rows (w.split mode): both kernels return at once unless a rebuild was
+// requested this step; the descriptor launch closes the request.
+hipError_t launch_split(const DevBox& b, const DevModel& m, int c1, int c2, DevWork& w, hipStream_t s,
+                        int lanes = 16);
 // Sizes (nR, kR, nA, kA) of the exactly instantiated descriptor that a model
 // without an exact instance of its own is padded to (zero coefficients and
 // ANN weights on the added entries, 3-body order 4 with both many-body
@@ -150,12 +169,14 @@ bool descriptor_bucket(const DevModel& m, int dims[4]);
 // at most 64 neurons (every size but the generic fallback), the vector path
 // otherwise.
 // start/stop (optional) are recorded with the kernel's own dispatch
-hipError_t launch_descriptor(const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
+// With w.split the descriptor forms its edges from the candidate rows (b is
+// the box of the step) and closes the step's rebuild request.
+hipError_t launch_descriptor(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
                              bool accumulate, int ann, DevWork& w, hipStream_t s, hipEvent_t start = nullptr,
                              hipEvent_t stop = nullptr);
 // lanes: lanes per atom, 4, 8 (default) or 16
-hipError_t launch_gather(const DevModel& m, int N, int n1, int n2, int nd1, int nd2, const int* type,
-                         double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s,
+hipError_t launch_gather(const DevBox& b, const DevModel& m, int N, int n1, int n2, int nd1, int nd2,
+                         const int* type, double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s,
                          int lanes = 8);
 
 // Integrator / thermo

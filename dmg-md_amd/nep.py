@@ -186,6 +186,11 @@ class NEP:
         """Sets the tile shapes directly (filter 8/16/32, gather 4/8/16)."""
         check(lib().dmgx_nep_set_tiles(self._h, int(filter_lanes), int(gather_lanes)))
 
+    def set_split(self, split: bool) -> None:
+        """Typewise row split once per Verlet rebuild (True) or every step
+        (False, the reference's filter pass); the next evaluation rebuilds."""
+        check(lib().dmgx_nep_set_split(self._h, 1 if split else 0))
+
     def autotune(self, box: Box, type, position, potential, force, virial, *, ann_bf16=False,
                  stream=None) -> dict:
         """Times the filter and gather tile shapes on this system and keeps the
@@ -380,7 +385,8 @@ def _tuning_dict(t) -> dict:
             "filter_ms": dict(zip((8, 16, 32), t.filter_ms[:])), "gather_ms": dict(zip((4, 8, 16), t.gather_ms[:])),
             "row_layout": t.row_layout, "row_capacity": t.row_capacity,
             "force_ms": dict(zip(("multiple_of_4", "power_of_2"), t.force_ms[:])),
-            "overlap": t.overlap, "overlap_ms": dict(zip(("sequential", "overlapped"), t.overlap_ms[:]))}
+            "overlap": t.overlap, "overlap_ms": dict(zip(("sequential", "overlapped"), t.overlap_ms[:])),
+            "split": t.split, "split_ms": dict(zip(("filter", "split_rows"), t.split_ms[:]))}
 
 
 def _copy_d2h(dev_ptr, host: np.ndarray) -> None:

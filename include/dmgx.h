@@ -173,11 +173,20 @@ typedef struct dmgx_tuning {
                            stream beside the descriptor of the first half (timed only where
                            each half holds at least 16,384 centres) */
   double overlap_ms[2]; /* filter + descriptor + gather without / with the overlap */
+  int split;            /* 1: the rows are split by type pair at rebuild time (every Verlet
+                           entry within its pair's rc + skin) and the step has no filter
+                           pass: the descriptor and gather form the distances themselves */
+  double split_ms[2];   /* the step's force kernels with the filter / with the split rows */
 } dmgx_tuning;
 int dmgx_nep_get_tuning(const dmgx_nep* nep, dmgx_tuning* out);
 /* Sets the shapes directly (DMGX_EINVAL for values other than the
  * candidates); a server bound to the handle keeps them (no autotuning). */
 int dmgx_nep_set_tiles(dmgx_nep* nep, int filter_lanes, int gather_lanes);
+/* Selects where the typewise radial/angular split of the Verlet rows happens:
+ * 0 every step (the filter pass of nep.cu:503-561), 1 once per Verlet
+ * rebuild, with the distance test at the point of use.  The next evaluation
+ * rebuilds the rows; a server bound to the handle keeps the choice. */
+int dmgx_nep_set_split(dmgx_nep* nep, int split);
 /* Times every candidate on this system (positions in device memory, as
  * dmgx_nep_compute) and keeps the fastest for all later evaluations of this
  * handle; afterwards pe/force/virial hold the system's values as the chosen

@@ -96,3 +96,23 @@ def test_noalloc_rejects_what_does_not_fit(dmgx, workloads):
     with pytest.raises(L.DmgxError) as e:
         _call(dmgx, "dmgx_nep_compute_noalloc", nep, s)
     assert e.value.code == -1 and "replicated" in str(e.value)
+
+
+def test_noalloc_with_candidate_rows(dmgx, workloads):
+    """The same two calls on a handle whose rows are split at rebuild time
+    (the fixed array then also holds the candidate rows)."""
+    nep = dmgx.NEP(CARBON)
+    nep.set_split(True)
+    big = workloads.carbon_diamond_4096(seed=6)
+    print("no-alloc, candidate rows: carbon diamond 4096 atoms", flush=True)
+    a = _call(dmgx, "dmgx_nep_compute_noalloc", nep, big)
+    b = _call(dmgx, "dmgx_nep_compute", nep, big)
+    _agree(a, b, "4096 atoms, candidate rows")
+    pos, box = workloads.diamond(6, 3.567)
+    pos = workloads.wrap(workloads.jitter(pos, 0.05, 10), box)
+    small = dict(box=box, pos=workloads.soa(pos), types=np.zeros(pos.shape[0], np.int32))
+    ranges = (100, 1500, 0, 1728)
+    print(f"no-alloc, candidate rows: carbon diamond 1728 atoms, centres {ranges}", flush=True)
+    a = _call(dmgx, "dmgx_nep_compute_noalloc", nep, small, ranges)
+    b = _call(dmgx, "dmgx_nep_compute", nep, small, ranges)
+    _agree(a, b, "1728 atoms, candidate rows")
