@@ -558,21 +558,26 @@ def main():
                  "ms_per_step": el_a / args.amortised_steps * 1e3,
                  "atom_steps_per_s": N * args.amortised_steps / el_a}
     # the same configuration with bfloat16 ANN operands, fresh state
-    bf16 = None
+    # the same configuration with bfloat16 ANN operands, and with split
+    # bfloat16 operands (three parts each, FP32 accuracy), fresh states
+    bf16 = bf16_split = None
     if args.ann == "fp32":
-        nep_b = dmgx.NEP(MODEL)
-        md_b = dmgx.MD(nep_b, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
-        md_b.autotune(ann_bf16=True)
-        ens_b = dict(ens, ann_bf16=True)
-        md_b.run(args.warmup, 2.0, **ens_b)
-        torch.cuda.synchronize()
-        tb = time.perf_counter()
-        md_b.run(args.steps, 2.0, **ens_b)
-        torch.cuda.synchronize()
-        el_b = time.perf_counter() - tb
-        bf16 = {"ann": "bf16 operands, fp32 accumulation (matrix cores)", "steps": args.steps,
-                "atom_steps_per_s": N * args.steps / el_b, "ms_per_step": el_b / args.steps * 1e3}
-        del md_b, nep_b
+        def ann_line(kw, label):
+            nep_b = dmgx.NEP(MODEL)
+            md_b = dmgx.MD(nep_b, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
+            md_b.autotune(**kw)
+            ens_b = dict(ens, **kw)
+            md_b.run(args.warmup, 2.0, **ens_b)
+            torch.cuda.synchronize()
+            tb = time.perf_counter()
+            md_b.run(args.steps, 2.0, **ens_b)
+            torch.cuda.synchronize()
+            el_b = time.perf_counter() - tb
+            return {"ann": label, "steps": args.steps, "atom_steps_per_s": N * args.steps / el_b,
+                    "ms_per_step": el_b / args.steps * 1e3}
+        bf16 = ann_line({"ann_bf16": True}, "bf16 operands, fp32 accumulation (matrix cores)")
+        bf16_split = ann_line({"ann_bf16_split": True},
+                              "split bf16 operands (hi + mid + lo, six products), fp32 accumulation (matrix cores)")
     small = None if args.no_small else small_bench(dmgx)
     share = None if args.no_share else share_bench(dmgx)
     context = None if args.no_context else context_bench(dmgx)
@@ -658,6 +663,7 @@ def main():
         },
         "ms_per_step_with_rebuilds": amortised,
         "bf16_ann": bf16,
+        "bf16_split_ann": bf16_split,
         "roofline": {
             "kernel": "k_descriptor (fused descriptor + ANN + adjoint + angular partial forces)",
             "bound": "valu",

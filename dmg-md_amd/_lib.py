@@ -67,7 +67,8 @@ class Tuning(ctypes.Structure):
                 ("filter_ms", ctypes.c_double * 3), ("gather_ms", ctypes.c_double * 3), ("tuned", ctypes.c_int),
                 ("row_layout", ctypes.c_int), ("row_capacity", ctypes.c_int), ("force_ms", ctypes.c_double * 2),
                 ("overlap", ctypes.c_int), ("overlap_ms", ctypes.c_double * 2),
-                ("split", ctypes.c_int), ("split_ms", ctypes.c_double * 2)]
+                ("split", ctypes.c_int), ("split_ms", ctypes.c_double * 2),
+                ("desc_lanes", ctypes.c_int), ("desc_ms", ctypes.c_double * 2)]
 
 
 ACCUMULATE = 1
@@ -100,6 +101,7 @@ _SIGNATURES = {
     "dmgx_nep_profile_reset": (I, [VP]),
     "dmgx_nep_get_tuning": (I, [VP, ctypes.POINTER(Tuning)]),
     "dmgx_nep_set_split": (I, [VP, I]),
+    "dmgx_nep_set_desc_lanes": (I, [VP, I]),
     "dmgx_nep_set_tiles": (I, [VP, I, I]),
     "dmgx_nep_autotune": (I, [VP, ctypes.POINTER(Box), I, VP, VP, VP, VP, VP, U, VP, ctypes.POINTER(Tuning)]),
     "dmgx_nep_copy_verlet_rows": (I, [VP, I, I, I, PI, PI]),

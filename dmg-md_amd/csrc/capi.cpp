@@ -213,6 +213,9 @@ void fill_tuning(const dmgx::NepEngine::Tuning& t, dmgx_tuning* out)
   out->split = t.split;
   out->split_ms[0] = t.split_ms[0];
   out->split_ms[1] = t.split_ms[1];
+  out->desc_lanes = t.desc_lanes;
+  out->desc_ms[0] = t.desc_ms[0];
+  out->desc_ms[1] = t.desc_ms[1];
 }
 }  // namespace
 
@@ -237,6 +240,14 @@ int dmgx_nep_set_split(dmgx_nep* nep, int split)
   return guarded([&] {
     if (!nep) throw std::invalid_argument("null handle");
     nep->engine.set_split(split);
+  });
+}
+
+int dmgx_nep_set_desc_lanes(dmgx_nep* nep, int lanes)
+{
+  return guarded([&] {
+    if (!nep) throw std::invalid_argument("null handle");
+    nep->engine.set_desc_lanes(lanes);
   });
 }
 

@@ -417,6 +417,13 @@ void NepEngine::allocate_split()
   have_list_ = false;  // the candidate rows are made by the next rebuild
 }
 
+void NepEngine::set_desc_lanes(int lanes)
+{
+  if (lanes != 0 && lanes != 1 && lanes != 4) throw std::invalid_argument("descriptor lanes per atom are 0, 1 or 4");
+  tune_.desc_lanes = lanes;
+  tune_.tuned = true;
+}
+
 void NepEngine::set_split(int split)
 {
   if (split != 0 && split != 1) throw std::invalid_argument("split is 0 (per-step filter) or 1 (rebuild-time rows)");
@@ -694,6 +701,16 @@ void NepEngine::compute(const Box& box, int N, int n1, int n2, int nd1, int nd2,
 // Smallest centre range split for the overlapped schedule: each half keeps
 // at least this many centres (below it the launches are too short to gain)
 constexpr int kOverlapMinCentres = 16384;
+// Below this many centres one lane per atom leaves most SIMDs idle (256 CUs x
+// 4 SIMDs x 64 lanes): the untuned descriptor then takes four lanes per atom
+constexpr int kLaneGroupCentres = 65536;
+
+int NepEngine::desc_lanes(int n, int ann) const
+{
+  if (w_.split || (ann != 1 && ann != 0)) return 1;  // candidate rows, or BF16 operands asked for
+  if (tune_.desc_lanes) return tune_.desc_lanes;
+  return n < kLaneGroupCentres ? 4 : 1;
+}
 
 void NepEngine::launch_filter_descriptor(const DevBox& b, int N, int c1, int c2, const int* type, double* pe,
                                          bool acc, int ann, bool close, hipStream_t s, bool profile)
@@ -713,7 +730,8 @@ void NepEngine::launch_filter_descriptor(const DevBox& b, int N, int c1, int c2,
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const bool attached = profile && prof_.attach(kProfDescriptor, &ev0, &ev1);
     if (profile && !attached) prof_.begin(kProfDescriptor, s);
-    check(launch_descriptor(b, dmd_, N, c1, c2, type, pe, acc, ann, w_, s, ev0, ev1), "descriptor");
+    check(launch_descriptor(b, dmd_, N, c1, c2, type, pe, acc, ann, w_, s, ev0, ev1, desc_lanes(n, ann)),
+          "descriptor");
     if (profile && !attached) prof_.end(s);
     return;
   }
@@ -730,7 +748,8 @@ void NepEngine::launch_filter_descriptor(const DevBox& b, int N, int c1, int c2,
     if (h == 1) check(hipStreamWaitEvent(s, join_, 0), "join wait");
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (profile) prof_.attach(kProfDescriptor, &ev0, &ev1);
-    check(launch_descriptor(b, dmd_, N, h ? mid : c1, h ? c2 : mid, type, pe, acc, ann, w_, s, ev0, ev1),
+    check(launch_descriptor(b, dmd_, N, h ? mid : c1, h ? c2 : mid, type, pe, acc, ann, w_, s, ev0, ev1,
+                            desc_lanes(n, ann)),
           "descriptor");
   }
 }
@@ -834,11 +853,26 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
     }
     tune_.overlap = t[1] < t[0] ? 1 : 0;
   }
+  // one or four lanes per atom in the descriptor (FP32 requests only)
+  {
+    const int ann = ann_mode(flags);
+    if (ann == 0 || ann == 1) {
+      double t[2];
+      for (int k = 0; k < 2; ++k) {
+        tune_.desc_lanes = k ? 4 : 1;
+        t[k] = time_median([&] { launch_step_kernels(b, N, nc, type, pe, force, virial, flags, s); });
+        tune_.desc_ms[k] = t[k];
+      }
+      tune_.desc_lanes = t[1] < t[0] ? 4 : 1;
+    }
+  }
   // candidate rows split at rebuild time, with every gather shape, where
   // their rows fit comfortably; kept if the step's force kernels are faster
   {
     double t_filter = tune_.force_ms[tune_.row_layout];
     if (tune_.overlap_ms[tune_.overlap] > 0.0) t_filter = std::min(t_filter, tune_.overlap_ms[tune_.overlap]);
+    const int dl = tune_.desc_lanes == 4 ? 1 : 0;
+    if (tune_.desc_ms[dl] > 0.0) t_filter = std::min(t_filter, tune_.desc_ms[dl]);
     tune_.split_ms[0] = t_filter;
     tune_.split_ms[1] = 0.0;
     size_t free_s = 0, total_s = 0;
@@ -1009,7 +1043,9 @@ void NepEngine::compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int
   else check(launch_filter(b, dm_, nd1, nd2, w, s, tune_.filter_lanes), "filter");
   const bool acc = (flags & 1u) != 0;
   const int ann = ann_mode(flags);
-  check(launch_descriptor(b, dmd_, N, nd1, nd2, type, pe, acc, ann, w, s), "descriptor");
+  const int lanes = (split || (ann != 1 && ann != 0)) ? 1
+                    : tune_.desc_lanes ? tune_.desc_lanes : (nd2 - nd1 < kLaneGroupCentres ? 4 : 1);
+  check(launch_descriptor(b, dmd_, N, nd1, nd2, type, pe, acc, ann, w, s, nullptr, nullptr, lanes), "descriptor");
   check(launch_gather(b, dm_, N, n1, n2, nd1, nd2, type, force, virial, acc, w, s, tune_.gather_lanes), "gather");
   check(hipMemcpyAsync(hflags_, w.flags, kNumFlags * sizeof(int), hipMemcpyDeviceToHost, s), "read status words");
   check(hipStreamSynchronize(s), "synchronise");

@@ -134,6 +134,12 @@ class NepEngine {
     // packed positions, and the descriptor writes the step's angular edges
     int split = 0;
     double split_ms[2] = {0, 0};        // the step's force kernels with the filter / with candidate rows
+    // lanes per atom of the descriptor: 1, 4 (a group of four lanes shares an
+    // atom's edges and hidden neurons, the ANN on the vector ALUs in FP32), or
+    // 0 before tuning: four below kLaneGroupCentres centres, else one.  BF16
+    // and split-BF16 ANN requests keep one lane per atom.
+    int desc_lanes = 0;
+    double desc_ms[2] = {0, 0};         // the step's force kernels with 1 / 4 lanes per atom
     bool tuned = false;
   };
   const Tuning& tuning() const { return tune_; }
@@ -143,6 +149,8 @@ class NepEngine {
   // Selects the per-step filter (0) or the rebuild-time split (1); the next
   // evaluation rebuilds the rows
   void set_split(int split);
+  // Sets the descriptor's lanes per atom: 0 (by system size), 1 or 4
+  void set_desc_lanes(int lanes);
   // (centres [0, nc); `map` as for compute)
   const Tuning& autotune(const Box& box, int N, int nc, const int* type, const double* pos, double* pe,
                          double* force, double* virial, unsigned flags, hipStream_t stream,
@@ -222,6 +230,8 @@ class NepEngine {
   // (first) descriptor dispatch when given
   void launch_filter_descriptor(const DevBox& b, int N, int c1, int c2, const int* type, double* pe, bool acc,
                                 int ann, bool close, hipStream_t s, bool profile);
+  // lanes per atom of the descriptor for n centres in ANN mode ann
+  int desc_lanes(int n, int ann) const;
   DevBox last_box_{};  // box of the last evaluation (the filter of copy_pair_rows)
   hipStream_t side_ = nullptr;  // the overlapped filter's stream
   hipEvent_t fork_ = nullptr, join_ = nullptr;

@@ -748,18 +748,34 @@ constexpr bool mfma_ann_fits()
          (ONE ? 64 * 1024 : kDescLdsMax);
 }
 
+// sum of v over the P consecutive lanes of a lane group (every lane gets it)
+template <int P>
+__device__ __forceinline__ float group_sum(float v)
+{
+#pragma unroll
+  for (int off = 1; off < P; off <<= 1) v += __shfl_xor(v, off, P);
+  return v;
+}
+
 // CAND: the edges come from the candidate rows (DevWork::split).  edge, vsa
 // and eq are w.edge, w.vsa and w.eq, passed as restrict parameters: with
 // candidate rows this kernel writes them, and only through them, so the
 // model's tables stay provably unmodified (scalar loads).
-template <class C, bool CAND>
+// P: lanes per atom.  With P > 1 (systems too small to fill the GPU with one
+// lane per atom) a group of P lanes shares an atom: each takes every P-th
+// edge and every P-th hidden neuron of the (vector-ALU) ANN, and the partial
+// sums are combined across the group, so every lane holds the atom's
+// descriptor, moments and derivatives.
+template <class C, bool CAND, int P = 1>
 __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m, int N, int c1, int c2, int nblocks,
                                                        const int* __restrict__ type, double* __restrict__ pe,
                                                        float4* __restrict__ edge, int* __restrict__ vsa,
                                                        int* __restrict__ eq, int accumulate, DevWork w)
 {
   constexpr int NR = C::nr, KR = C::kr, NA = C::na, KA = C::ka, LM = C::lm, NABC = C::nabc, DMAX = C::dim;
-  const int i0 = c1 + xcd_tile(nblocks) * blockDim.x + threadIdx.x;
+  static_assert(P == 1 || (P == 4 && !CAND && C::ann == kAnnValu), "lane groups: the filter's rows, vector ANN");
+  const int lr = P == 1 ? 0 : (int)(threadIdx.x & (P - 1));  // lane within the atom's group
+  const int i0 = c1 + xcd_tile(nblocks) * (int)(blockDim.x / P) + (int)(threadIdx.x / P);
   const bool active = i0 < c2;
   // the matrix-core ANN needs every lane of the wavefront: lanes past the
   // range carry an empty row of a valid atom and write nothing
@@ -891,16 +907,32 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
     auto ridx = [&](int e) { return e < na ? e : w.MR - 1 - (e - na); };
     float4 ring[kAhead];
 #pragma unroll
-    for (int u = 0; u < kAhead; ++u) ring[u] = u < nr ? erow[ridx(u)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int e0 = 0; e0 < nr; e0 += kAhead)
+    for (int u = 0; u < kAhead; ++u)
+      ring[u] = u * P + lr < nr ? erow[ridx(u * P + lr)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int e0 = lr; e0 < nr; e0 += kAhead * P)
 #pragma unroll
       for (int u = 0; u < kAhead; ++u) {
-        const int e = e0 + u;
+        const int e = e0 + u * P;
         if (e >= nr) break;
         const float4 r = ring[u];
-        if (e + kAhead < nr) ring[u] = erow[ridx(e + kAhead)];
+        if (e + kAhead * P < nr) ring[u] = erow[ridx(e + kAhead * P)];
         radial_edge(r, C::one ? 0 : safe_type(type[__float_as_int(r.w)], T));
       }
+    if (P > 1) {
+      // the group's partial radial sums
+      if (C::one) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) Fs[k] = group_sum<P>(Fs[k]);
+      } else if (C::cl) {
+#pragma unroll
+        for (int t = 0; t < (C::cl ? C::ltypes : 1); ++t)
+#pragma unroll
+          for (int k = 0; k < KR; ++k) Ft[t][k] = group_sum<P>(Ft[t][k]);
+      } else {
+#pragma unroll
+        for (int n = 0; n < NR; ++n) q[n] = group_sum<P>(q[n]);
+      }
+    }
   } else {
     // candidate rows: the FP64 minimum-image distance of every candidate
     // (the filter's expressions, so the same edges pass), the radial sum over
@@ -1003,16 +1035,20 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
 #pragma unroll
     for (int a = 0; a < NABC; ++a) s[n][a] = 0.0f;
   float zpe = 0.0f;
-  float4 rn = erow[0];
-  float4 ahead = erow[8 < na ? 8 : 0];
-  for (int e = 0; e < na; ++e) {
+  // each lane takes every P-th edge; the next record is loaded before the
+  // current one is used, and the first record of the next 128-byte line
+  // (eight of the lane's records) a line ahead
+  float4 rn = erow[lr < na ? lr : 0];
+  float4 ahead = erow[lr + 8 * P < na ? lr + 8 * P : 0];
+  for (int e = lr; e < na; e += P) {
     const float4 r = rn;
-    if (e + 1 < na) {
-      if (((e + 1) & 7) == 0) {
+    const int en = e + P;
+    if (en < na) {
+      if ((((en - lr) / P) & 7) == 0) {
         rn = ahead;
-        if (e + 9 < na) ahead = erow[e + 9];
+        if (en + 8 * P < na) ahead = erow[en + 8 * P];
       } else {
-        rn = erow[e + 1];
+        rn = erow[en];
       }
     }
     const float d2 = r.x * r.x + r.y * r.y + r.z * r.z;
@@ -1057,6 +1093,14 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
 #pragma unroll
       for (int a = 0; a < NABC; ++a) s[n][a] += g[n] * Y[a];
   }
+  if (P > 1) {
+    // the group's partial moments and ZBL energies
+#pragma unroll
+    for (int n = 0; n < NA; ++n)
+#pragma unroll
+      for (int a = 0; a < NABC; ++a) s[n][a] = group_sum<P>(s[n][a]);
+    zpe = group_sum<P>(zpe);
+  }
 
   // ---- invariants
 #pragma unroll
@@ -1099,7 +1143,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
     const float* w1 = m.w1 + (size_t)ti * H;
 #pragma unroll
     for (int d = 0; d < DMAX; ++d) Fp[d] = 0.0f;
-    for (int h = 0; h < H; ++h) {
+    for (int h = lr; h < H; h += P) {
       const float* wr = w0 + (size_t)h * D;
       float a = 0.0f;
 #pragma unroll
@@ -1112,6 +1156,12 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
       for (int d = 0; d < DMAX; ++d)
         if (d < D) Fp[d] += gq * wr[d];
     }
+    if (P > 1) {
+      // the group's hidden neurons
+      E = group_sum<P>(E);
+#pragma unroll
+      for (int d = 0; d < DMAX; ++d) Fp[d] = group_sum<P>(Fp[d]);
+    }
   }
   E -= m.b1[0];
   if (m.version == 5) E -= m.w1x[ti];
@@ -1119,7 +1169,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
   for (int d = 0; d < DMAX; ++d)
     if (d < D) Fp[d] *= m.qs[d];
   const double etot = (double)E + (double)zpe;
-  if (active) pe[i] = accumulate ? pe[i] + etot : etot;
+  if (active && lr == 0) pe[i] = accumulate ? pe[i] + etot : etot;
 
   // ---- radial adjoint in basis space: bR[i][t][k] = sum_n Fp_n c[ti,t,n,k]
   for (int t = 0; t < T; ++t) {
@@ -1134,7 +1184,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
         for (int n = 0; n < NR; ++n)
           if (n < nR) v += Fp[n] * c[n * kR + k];
       }
-      if (k < m.kRp) out[k] = v;
+      if (k < m.kRp && (P == 1 || (k & (P - 1)) == lr)) out[k] = v;
     }
   }
 
@@ -1177,17 +1227,18 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
     }
   }
 
-  // ---- angular partial forces dE_i/dr_ij
-  rn = erow[0];
-  ahead = erow[8 < na ? 8 : 0];
-  for (int e = 0; e < na; ++e) {
+  // ---- angular partial forces dE_i/dr_ij (the edges of the moments loop)
+  rn = erow[lr < na ? lr : 0];
+  ahead = erow[lr + 8 * P < na ? lr + 8 * P : 0];
+  for (int e = lr; e < na; e += P) {
     const float4 r = rn;
-    if (e + 1 < na) {
-      if (((e + 1) & 7) == 0) {
+    const int en = e + P;
+    if (en < na) {
+      if ((((en - lr) / P) & 7) == 0) {
         rn = ahead;
-        if (e + 9 < na) ahead = erow[e + 9];
+        if (en + 8 * P < na) ahead = erow[en + 8 * P];
       } else {
-        rn = erow[e + 1];
+        rn = erow[en];
       }
     }
     const int j = __float_as_int(r.w);
@@ -1225,7 +1276,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
       }
     }
     // contract the adjoint with the basis: A = sum_n a g_n, B = sum_n a g'_n,
-    // then P = sum B Y and G = sum A dY/du (Y = Z(uz) Re/Im (ux+i uy)^m)
+    // then Pr = sum B Y and G = sum A dY/du (Y = Z(uz) Re/Im (ux+i uy)^m)
     float zp[LM + 1];
     zp[0] = 1.0f;
 #pragma unroll
@@ -1238,7 +1289,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
       pr[k] = pr[k - 1] * ux - pim[k - 1] * uy;
       pim[k] = pr[k - 1] * uy + pim[k - 1] * ux;
     }
-    float P = 0.0f, Gx = 0.0f, Gy = 0.0f, Gz = 0.0f;
+    float Pr = 0.0f, Gx = 0.0f, Gy = 0.0f, Gz = 0.0f;
 #pragma unroll
     for (int L = 1; L <= LM; ++L) {
 #pragma unroll
@@ -1257,7 +1308,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
             A += s[n][b] * g[n];
             B += s[n][b] * gp[n];
           }
-          P += B * Z;
+          Pr += B * Z;
           Gz += A * dZ;
         } else {
           const int ir = b + 2 * mm - 1, ii = b + 2 * mm;
@@ -1269,7 +1320,7 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
             Br += s[n][ir] * gp[n];
             Bi += s[n][ii] * gp[n];
           }
-          P += Z * (Br * pr[mm] + Bi * pim[mm]);
+          Pr += Z * (Br * pr[mm] + Bi * pim[mm]);
           const float mz = (float)mm * Z;
           Gx += mz * (Ar * pr[mm - 1] + Ai * pim[mm - 1]);
           Gy += mz * (Ai * pr[mm - 1] - Ar * pim[mm - 1]);
@@ -1278,8 +1329,8 @@ __global__ __launch_bounds__(kDescBlock) void k_descriptor(DevBox b, DevModel m,
       }
     }
     const float gu = Gx * ux + Gy * uy + Gz * uz;
-    const float4 fo = make_float4(P * ux + (Gx - gu * ux) * dinv, P * uy + (Gy - gu * uy) * dinv,
-                                  P * uz + (Gz - gu * uz) * dinv, 0.0f);
+    const float4 fo = make_float4(Pr * ux + (Gx - gu * ux) * dinv, Pr * uy + (Gy - gu * uy) * dinv,
+                                  Pr * uz + (Gz - gu * uz) * dinv, 0.0f);
     frow[e] = fo;
   }
 }
@@ -1898,8 +1949,21 @@ hipError_t launch_filter(const DevBox& b, const DevModel& m, int c1, int c2, Dev
 
 template <class C>
 static void run_descriptor(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
-                           bool acc, DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+                           bool acc, DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int lanes = 1)
 {
+  if constexpr (C::ann == kAnnValu) {
+    if (lanes == 4 && !w.split) {
+      constexpr int per = kDescBlock / 4;
+      const int nb = (c2 - c1 + per - 1) / per;
+      auto k = k_descriptor<C, false, 4>;
+      if (e0 || e1)
+        hipExtLaunchKernelGGL(k, dim3(xcd_grid(nb)), dim3(kDescBlock), 0, s, e0, e1, 0, b, m, N, c1, c2, nb, type,
+                              pe, w.edge, w.vsa, w.eq, acc ? 1 : 0, w);
+      else
+        k<<<xcd_grid(nb), kDescBlock, 0, s>>>(b, m, N, c1, c2, nb, type, pe, w.edge, w.vsa, w.eq, acc ? 1 : 0, w);
+      return;
+    }
+  }
   const int nb = (c2 - c1 + kDescBlock - 1) / kDescBlock;
   auto k = w.split ? k_descriptor<C, true> : k_descriptor<C, false>;
   if (e0 || e1)
@@ -1913,9 +1977,9 @@ static void run_descriptor(const DevBox& b, const DevModel& m, int N, int c1, in
 // matrix cores for H <= 64 (FP32 or BF16 operands), vector ALUs otherwise
 template <int NR, int KR, int NA, int KA, bool Q2, bool Q1>
 static void run_single(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
-                       DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+                       DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int lanes)
 {
-  const int mode = m.H <= 64 ? ann : kAnnValu;
+  const int mode = (m.H <= 64 && !(lanes == 4 && !w.split)) ? ann : kAnnValu;
   if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, true>()) {
     if (mode == kAnnMfma32) {
       run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnMfma32>>(b, m, N, c1, c2, type, pe, acc, w, s, e0,
@@ -1935,14 +1999,15 @@ static void run_single(const DevBox& b, const DevModel& m, int N, int c1, int c2
       }
     }
   }
-  run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+  run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, true, true, kAnnValu>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1,
+                                                                      lanes);
 }
 
 template <int NR, int KR, int NA, int KA, bool Q2, bool Q1>
 static void run_multi(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe, bool acc, int ann,
-                      DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+                      DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int lanes)
 {
-  const int mode = (m.H <= 64 && m.T <= kTypesMfma) ? ann : kAnnValu;
+  const int mode = (m.H <= 64 && m.T <= kTypesMfma && !(lanes == 4 && !w.split)) ? ann : kAnnValu;
   if constexpr (mfma_ann_fits<NR, KR, NA, KA, Q2, Q1, false>()) {
     if (mode == kAnnMfma32) {
       run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnMfma32, true>>(b, m, N, c1, c2, type, pe, acc, w,
@@ -1964,9 +2029,9 @@ static void run_multi(const DevBox& b, const DevModel& m, int N, int c1, int c2,
   }
   if (m.T <= kTypesLds)
     run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true, kAnnValu, true>>(b, m, N, c1, c2, type, pe, acc, w, s,
-                                                                               e0, e1);
+                                                                               e0, e1, lanes);
   else
-    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_descriptor<Cfg<NR, KR, NA, KA, 4, Q2, Q1, false, true>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1, lanes);
 }
 
 namespace {
@@ -2004,8 +2069,9 @@ bool descriptor_bucket(const DevModel& m, int dims[4])
 }
 
 hipError_t launch_descriptor(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
-                             bool acc, int ann, DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+                             bool acc, int ann, DevWork& w, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int lanes)
 {
+  if (lanes != 1 && lanes != 4) return hipErrorInvalidValue;
   if (c2 <= c1) return hipSuccess;
   const bool one = m.T == 1;
   const bool q2 = m.has222 != 0, q1 = m.has1111 != 0;
@@ -2015,33 +2081,33 @@ hipError_t launch_descriptor(const DevBox& b, const DevModel& m, int N, int c1, 
   // exact instantiations: the configurations in use, and the bounded sizes
   // that other models are padded to (descriptor_bucket); generic otherwise
   if (is(5, 9, 5, 9, 4, true, true, true))
-    run_single<5, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_single<5, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (is(7, 9, 5, 9, 4, true, false, true))
-    run_single<7, 9, 5, 9, true, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_single<7, 9, 5, 9, true, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (is(13, 17, 9, 13, 4, true, true, true))
-    run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1);
+    run_descriptor<Cfg<13, 17, 9, 13, 4, true, true, true, true>>(b, m, N, c1, c2, type, pe, acc, w, s, e0, e1, lanes);
   else if (is(7, 11, 7, 11, 4, true, true, false))
-    run_multi<7, 11, 7, 11, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<7, 11, 7, 11, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (is(5, 9, 5, 9, 4, true, false, false))
-    run_multi<5, 9, 5, 9, true, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<5, 9, 5, 9, true, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (is(9, 9, 7, 9, 4, false, false, false))
-    run_multi<9, 9, 7, 9, false, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<9, 9, 7, 9, false, false>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (is(7, 9, 5, 9, 4, true, true, true))
-    run_single<7, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_single<7, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (is(7, 9, 5, 9, 4, true, true, false))
-    run_multi<7, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<7, 9, 5, 9, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (is(9, 13, 7, 13, 4, true, true, true))
-    run_single<9, 13, 7, 13, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_single<9, 13, 7, 13, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (is(9, 13, 7, 13, 4, true, true, false))
-    run_multi<9, 13, 7, 13, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1);
+    run_multi<9, 13, 7, 13, true, true>(b, m, N, c1, c2, type, pe, acc, ann, w, s, e0, e1, lanes);
   else if (m.L <= kPadL && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
     run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, kPadL, true, true, false, false>>(b, m, N, c1, c2, type, pe, acc,
-                                                                                      w, s, e0, e1);
+                                                                                      w, s, e0, e1, lanes);
   else if (m.L <= kMaxL && m.nR <= kMaxNR && m.kR <= kMaxK && m.nA <= kMaxNA && m.kA <= kMaxK)
     // 3-body orders 5..8 (80 angular moments per radial function): the
     // generic instance at order 8, its per-atom arrays partly in scratch
     run_descriptor<Cfg<kMaxNR, kMaxK, kMaxNA, kMaxK, kMaxL, true, true, false, false>>(b, m, N, c1, c2, type, pe, acc,
-                                                                                      w, s, e0, e1);
+                                                                                      w, s, e0, e1, lanes);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -170,10 +170,12 @@ bool descriptor_bucket(const DevModel& m, int dims[4]);
 // otherwise.
 // start/stop (optional) are recorded with the kernel's own dispatch
 // With w.split the descriptor forms its edges from the candidate rows (b is
-// the box of the step) and closes the step's rebuild request.
+// the box of the step) and closes the step's rebuild request.  lanes: lanes
+// per atom, 1 or 4 (four: every model on the vector-ALU ANN in FP32, with the
+// filter's rows; for systems too small to fill the GPU one lane per atom).
 hipError_t launch_descriptor(const DevBox& b, const DevModel& m, int N, int c1, int c2, const int* type, double* pe,
                              bool accumulate, int ann, DevWork& w, hipStream_t s, hipEvent_t start = nullptr,
-                             hipEvent_t stop = nullptr);
+                             hipEvent_t stop = nullptr, int lanes = 1);
 // lanes: lanes per atom, 4, 8 (default) or 16
 hipError_t launch_gather(const DevBox& b, const DevModel& m, int N, int n1, int n2, int nd1, int nd2,
                          const int* type, double* force, double* virial, bool accumulate, DevWork& w, hipStream_t s,

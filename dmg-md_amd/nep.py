@@ -191,6 +191,11 @@ class NEP:
         (False, the reference's filter pass); the next evaluation rebuilds."""
         check(lib().dmgx_nep_set_split(self._h, 1 if split else 0))
 
+    def set_desc_lanes(self, lanes: int) -> None:
+        """Lanes per atom of the descriptor kernel: 1, 4, or 0 (by system
+        size: four below 65,536 centres)."""
+        check(lib().dmgx_nep_set_desc_lanes(self._h, int(lanes)))
+
     def autotune(self, box: Box, type, position, potential, force, virial, *, ann_bf16=False,
                  stream=None) -> dict:
         """Times the filter and gather tile shapes on this system and keeps the
@@ -386,7 +391,8 @@ def _tuning_dict(t) -> dict:
             "row_layout": t.row_layout, "row_capacity": t.row_capacity,
             "force_ms": dict(zip(("multiple_of_4", "power_of_2"), t.force_ms[:])),
             "overlap": t.overlap, "overlap_ms": dict(zip(("sequential", "overlapped"), t.overlap_ms[:])),
-            "split": t.split, "split_ms": dict(zip(("filter", "split_rows"), t.split_ms[:]))}
+            "split": t.split, "split_ms": dict(zip(("filter", "split_rows"), t.split_ms[:])),
+            "desc_lanes": t.desc_lanes, "desc_ms": dict(zip((1, 4), t.desc_ms[:]))}
 
 
 def _copy_d2h(dev_ptr, host: np.ndarray) -> None:

@@ -177,6 +177,11 @@ typedef struct dmgx_tuning {
                            entry within its pair's rc + skin) and the step has no filter
                            pass: the descriptor and gather form the distances themselves */
   double split_ms[2];   /* the step's force kernels with the filter / with the split rows */
+  int desc_lanes;       /* lanes per atom of the descriptor: 1, 4 (a group of four lanes shares
+                           an atom's edges and hidden neurons; FP32 ANN on the vector ALUs),
+                           0 before tuning (four below 65,536 centres, else one); BF16 ANN
+                           requests keep one */
+  double desc_ms[2];    /* the step's force kernels with 1 / 4 lanes per atom */
 } dmgx_tuning;
 int dmgx_nep_get_tuning(const dmgx_nep* nep, dmgx_tuning* out);
 /* Sets the shapes directly (DMGX_EINVAL for values other than the
@@ -187,6 +192,10 @@ int dmgx_nep_set_tiles(dmgx_nep* nep, int filter_lanes, int gather_lanes);
  * rebuild, with the distance test at the point of use.  The next evaluation
  * rebuilds the rows; a server bound to the handle keeps the choice. */
 int dmgx_nep_set_split(dmgx_nep* nep, int split);
+/* Sets the descriptor's lanes per atom (0: by system size, 1 or 4; no
+ * counterpart in the reference, whose launch shape is fixed); a server bound
+ * to the handle keeps the choice. */
+int dmgx_nep_set_desc_lanes(dmgx_nep* nep, int lanes);
 /* Times every candidate on this system (positions in device memory, as
  * dmgx_nep_compute) and keeps the fastest for all later evaluations of this
  * handle; afterwards pe/force/virial hold the system's values as the chosen

@@ -204,8 +204,15 @@ def test_serving_stream_mix_against_oracle(dmgx, workloads):
     paths = [CARBON, CU]
     orcs = [OracleNEP(p, round_params_to_float=True) for p in paths]
 
+    def one_lane(path):
+        # the batch and the standalone runs compare step by step: the same
+        # descriptor arithmetic (one lane per atom) whatever their sizes
+        nep = dmgx.NEP(path)
+        nep.set_desc_lanes(1)
+        return nep
+
     def serve(bf16):
-        srv = dmgx.Server([dmgx.NEP(p) for p in paths], max_atoms=2_000_000, ann_bf16=bf16)
+        srv = dmgx.Server([one_lane(p) for p in paths], max_atoms=2_000_000, ann_bf16=bf16)
         ids = []
         for k, (r, model, steps) in enumerate(stream):
             ids.append(srv.submit(r["box"], r["types"], r["mass"], r["pos"], r["vel"], steps=steps,
@@ -219,7 +226,7 @@ def test_serving_stream_mix_against_oracle(dmgx, workloads):
         return [srv.poll(i) for i in ids]
 
     got = serve(False)
-    plain_nep = [dmgx.NEP(p) for p in paths]
+    plain_nep = [one_lane(p) for p in paths]
     rows = []
     n_md = 0
     for (r, model, steps), out in zip(stream, got):
@@ -235,7 +242,7 @@ def test_serving_stream_mix_against_oracle(dmgx, workloads):
             rows.append((r["material"], n, spread, e_srv, e_plain, d_plain))
             continue
         n_md += 1
-        md = dmgx.MD(dmgx.NEP(paths[model]), r["box"], r["types"], r["mass"], r["pos"], r["vel"])
+        md = dmgx.MD(one_lane(paths[model]), r["box"], r["types"], r["mass"], r["pos"], r["vel"])
         md.run(steps, r["dt_fs"], "nve")
         std = md.arrays()
         L = r["box"].h[0, 0]
