@@ -462,6 +462,7 @@ def main():
     ap.add_argument("--no-share", action="store_true", help="skip the 2,048,000-atom amorphous-carbon share line")
     ap.add_argument("--amortised-steps", type=int, default=300,
                     help="steps of the stretch timed with its Verlet rebuilds")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving sample (its JSON object)")
     ap.add_argument("--water-only", type=int, default=0, metavar="STEPS",
                     help="run only the liquid-water sample with STEPS timed steps per run and print its "
                          "JSON object (for kernel profiles of the 2-type path)")
@@ -496,6 +497,9 @@ def main():
     dmgx = importlib.import_module("dmg-md_amd")
     if args.water_only > 0:
         print(json.dumps(water_bench(dmgx, steps=args.water_only)))
+        return
+    if args.serve_only:
+        print(json.dumps(serve_bench(dmgx)))
         return
     W = importlib.import_module("dmg-md_amd.workloads")
     s = W.copper_fcc(n=args.cells, seed=1)

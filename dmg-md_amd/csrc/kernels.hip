@@ -2350,6 +2350,7 @@ __global__ void k_serve_assemble(int R, const ServeSlots* __restrict__ req, Serv
         to.pos[c * Nt + s] = from.pos[c * from.Nt + src] + q.dtr[c];
         to.vel[c * Nt + s] = from.vel[c * from.Nt + src];
         to.force[c * Nt + s] = from.force[c * from.Nt + src];
+        to.home[c * no + s] = from.home[c * from.n_owned + src] + q.dtr[c];
       }
     } else {
       to.type[s] = q.pi[k];
@@ -2357,9 +2358,11 @@ __global__ void k_serve_assemble(int R, const ServeSlots* __restrict__ req, Serv
       to.mass[s] = q.pd[k];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        to.pos[c * Nt + s] = q.pd[q.n + 3 * k + c] + q.dtr[c];
+        const double x = q.pd[q.n + 3 * k + c] + q.dtr[c];
+        to.pos[c * Nt + s] = x;
+        to.home[c * no + s] = x;
         to.vel[c * Nt + s] = q.pd[4 * q.n + 3 * k + c];
-        to.force[c * Nt + s] = 0.0;
+        to.force[c * Nt + s] = q.pd[7 * q.n + 3 * k + c];
       }
     }
   } else {
@@ -2380,9 +2383,23 @@ __global__ void k_serve_assemble(int R, const ServeSlots* __restrict__ req, Serv
       to.type[s] = q.pi[a];
       to.mass[s] = q.pd[a];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) to.gshift[3 * g + c] = q.pd[7 * q.n + 3 * k + c];
+      for (int c = 0; c < 3; ++c) to.gshift[3 * g + c] = q.pd[10 * q.n + 3 * k + c];
     }
   }
+}
+
+__global__ void k_serve_drift(int n_owned, int Nt, const int* __restrict__ atom_req, const double* __restrict__ pos,
+                              const double* __restrict__ home, double soft2, double hard2, int* __restrict__ req_flag)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_owned) return;
+  double d2 = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double d = pos[c * Nt + s] - home[c * n_owned + s];
+    d2 += d * d;
+  }
+  if (!(d2 <= soft2)) atomicMax(&req_flag[atom_req[s]], (d2 <= hard2) ? 1 : 2);
 }
 
 // finished requests -> their result blocks, one thread per atom; the
@@ -2442,6 +2459,14 @@ hipError_t launch_serve_assemble(int R, const ServeSlots* req, const ServeLayout
                                  int* atom_req, hipStream_t s)
 {
   if (R > 0 && to.Nt > 0) k_serve_assemble<<<nblk(to.Nt), kBlock, 0, s>>>(R, req, from, to, atom_req);
+  return hipGetLastError();
+}
+
+hipError_t launch_serve_drift(int n_owned, int Nt, const int* atom_req, const double* pos, const double* home,
+                              double soft2, double hard2, int* req_flag, hipStream_t s)
+{
+  if (n_owned > 0)
+    k_serve_drift<<<nblk(n_owned), kBlock, 0, s>>>(n_owned, Nt, atom_req, pos, home, soft2, hard2, req_flag);
   return hipGetLastError();
 }
 

@@ -247,9 +247,12 @@ struct ServeSlots {
   const double* pd;   // a new request's pack: doubles
   double dtr[3];      // translation added to the positions
 };
-// pack layout: ints then doubles, the doubles at an 8-byte boundary
+// pack layout: ints (types [n], first ghost of each atom [n], ghost owners
+// [ng], ghost image codes [ng]) then, at an 8-byte boundary, doubles (masses
+// [n], positions [3n], velocities [3n], forces [3n], ghost shifts [3ng], the
+// per-atom ones atom-major)
 inline size_t serve_pack_ints(int n, int ng) { return 2 * (size_t)n + 2 * (size_t)ng; }
-inline size_t serve_pack_doubles(int n, int ng) { return 7 * (size_t)n + 3 * (size_t)ng; }
+inline size_t serve_pack_doubles(int n, int ng) { return 10 * (size_t)n + 3 * (size_t)ng; }
 inline size_t serve_pack_double_offset(int n, int ng) { return (serve_pack_ints(n, ng) * 4 + 7) & ~size_t(7); }
 inline size_t serve_pack_bytes(int n, int ng)
 {
@@ -259,12 +262,18 @@ inline size_t serve_pack_bytes(int n, int ng)
 struct ServeLayout {
   int Nt = 0, n_owned = 0;
   double *pos = nullptr, *vel = nullptr, *force = nullptr, *mass = nullptr, *gshift = nullptr;
+  double* home = nullptr;  // owned slots' positions where their request's image shell was built [3*n_owned]
   int *type = nullptr, *owner = nullptr, *image = nullptr, *gstart = nullptr;
 };
 // Writes every slot of `to` (R requests in slot order; ghost positions are
 // left to launch_serve_ghosts) and atom_req[owned slot] = request index.
 hipError_t launch_serve_assemble(int R, const ServeSlots* req, const ServeLayout& from, const ServeLayout& to,
                                  int* atom_req, hipStream_t s);
+// req_flag[request] raised to 1 for every request with an owned atom farther
+// than sqrt(soft2) from its home position, to 2 beyond sqrt(hard2) (flags are
+// only ever raised here)
+hipError_t launch_serve_drift(int n_owned, int Nt, const int* atom_req, const double* pos, const double* home,
+                              double soft2, double hard2, int* req_flag, hipStream_t s);
 // One finished request for launch_serve_extract: its owned slots, the
 // offset of its block in the output, its placement translation and its box.
 constexpr int kServeOutDoubles = 19;  // result doubles per atom

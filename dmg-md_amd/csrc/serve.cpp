@@ -2,9 +2,11 @@
 #include "serve.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <stdexcept>
 #include <string>
 
@@ -153,6 +155,9 @@ struct Server::Request {
   double trans[3] = {0, 0, 0};
   int slot0 = -1, gslot0 = -1;  // first owned / ghost slot in the layout
   bool placed = false;
+  // types and masses as submitted (a new image shell is packed from them)
+  std::vector<int> types;
+  std::vector<double> masses;
   // results: block of kServeOutDoubles n doubles (kernels.hpp)
   std::shared_ptr<ResultBlock> res;
   long long res_off = 0;
@@ -177,6 +182,10 @@ struct Server::Lane {
   DBuf<double> g_shift;  // ghost k -> box shift [3]
   // the next layout's per-slot arrays, assembled on the device and swapped in
   DBuf<double> pos2, vel2, force2, mass2, g_shift2;
+  DBuf<double> home, home2;  // owned slots' positions where their request's shell was built
+  DBuf<int> req_flag;        // active request -> 1 once an atom drifted past the shell's budget
+  int* h_drift = nullptr;    // pinned copy of req_flag, read after each frame
+  int h_drift_cap = 0;
   DBuf<int> type2, owner2, image2, gstart2;
   DBuf<int> atom_req;    // owned slot -> active request index
   DBuf<int> req_md;      // active request -> 1 when it takes an MD step this iteration
@@ -216,10 +225,14 @@ struct Server::Lane {
   void release()
   {
     for (DBuf<double>* b : {&pos, &vel, &force, &pe, &vir, &mass, &pos2, &vel2, &force2, &mass2, &req_dt, &g_shift,
-                            &g_shift2, &out})
+                            &g_shift2, &out, &home, &home2})
       b->release();
-    for (DBuf<int>* b : {&type, &type2, &atom_req, &req_md, &owner, &owner2, &image, &image2, &gstart, &gstart2})
+    for (DBuf<int>* b : {&type, &type2, &atom_req, &req_md, &owner, &owner2, &image, &image2, &gstart, &gstart2,
+                         &req_flag})
       b->release();
+    if (h_drift) (void)hipHostFree(h_drift);
+    h_drift = nullptr;
+    h_drift_cap = 0;
     slots.release();
     outq.release();
     if (h_stage) (void)hipHostFree(h_stage);
@@ -228,10 +241,10 @@ struct Server::Lane {
 };
 
 // device bytes per slot of a lane's layout: position, velocity and force
-// (3 doubles each) and mass, type, owner, image and ghost shift, all
-// double-buffered; potential, virial, the atom->request index and the
-// result block of a finished atom
-constexpr double kLayoutSlotBytes = 2 * (9 * 8 + 8 + 3 * 4 + 3 * 8) + 8 + 9 * 8 + 4 + kServeOutDoubles * 8;
+// (3 doubles each) and mass, type, owner, image, ghost shift and home
+// position, all double-buffered; potential, virial, the atom->request index
+// and the result block of a finished atom
+constexpr double kLayoutSlotBytes = 2 * (9 * 8 + 8 + 3 * 4 + 3 * 8 + 3 * 8) + 8 + 9 * 8 + 4 + kServeOutDoubles * 8;
 
 double Server::slot_bytes(int model) const
 {
@@ -327,33 +340,19 @@ Server::~Server()
     if (L->stream) (void)hipStreamDestroy(L->stream);
 }
 
-long long Server::submit(int model, const Box& box, int N, const int* type, const double* mass, const double* pos,
-                         const double* vel, int steps, double dt_fs)
+void Server::make_pack(Request& rq, const int* type, const double* mass, const double* pos, const double* vel,
+                       const double* force, long long limit)
 {
-  if (model < 0 || model >= (int)lanes_.size()) throw std::invalid_argument("model index out of range");
-  if (N <= 0 || !type || !mass || !pos) throw std::invalid_argument("a request needs atoms, types, masses, positions");
-  if (steps < 0) throw std::invalid_argument("negative step count");
-  if (steps > 0 && !(dt_fs > 0.0)) throw std::invalid_argument("MD requests need dt_fs > 0");
-  const double t_call = now_s();
+  Request* r = &rq;
+  const int N = r->n;
+  const int model = r->model;
   NepEngine* nep = lanes_[(size_t)model]->nep;
-  auto r = std::make_shared<Request>();
-  r->model = model;
-  r->box = box;
-  const double vol = box_volume(box.h);
-  if (!(std::fabs(vol) > 0.0) || !std::isfinite(vol)) throw std::invalid_argument("box has no volume");
-  box_inverse(box.h, r->hi);
-  r->n = N;
-  r->steps = steps;
-  r->dt = dt_fs / kTimeUnit;
-  const int T = nep->model().num_types;
-  for (int i = 0; i < N; ++i) {
-    if (type[i] < 0 || type[i] >= T) throw std::invalid_argument("atom type out of range");
-    if (!(mass[i] > 0.0)) throw std::invalid_argument("masses must be positive");
-  }
+  const Box& box = r->box;
   std::vector<double> wpos(3 * (size_t)N);  // wrapped into the box, SoA
   // wrap into the box and collect the periodic images of the shell
   const double rc = nep->model().rcR_max;
-  const double d1 = rc + kSkin + cfg_.margin;
+  // a single point never moves: its shell needs no margin for motion
+  const double d1 = rc + kSkin + (r->steps > 0 ? cfg_.margin : 0.0);
   double th[3];
   thickness(box.h, th);
   int K[3];
@@ -426,9 +425,9 @@ long long Server::submit(int model, const Box& box, int N, const int* type, cons
   r->bytes = (long long)std::ceil(r->footprint() * slot_bytes(model)) + (long long)bytes;
   if (r->footprint() > cfg_.max_atoms)
     throw std::invalid_argument("request (with its periodic shell) exceeds the serving atom budget");
-  if (r->bytes > budget_)
+  if (r->bytes > limit)
     throw std::invalid_argument("request (with its periodic shell) needs " + std::to_string(r->bytes) +
-                                " device bytes, more than the serving HBM budget of " + std::to_string(budget_));
+                                " device bytes, more than the serving HBM budget of " + std::to_string(limit));
   // the pack: built here and uploaded on the submitting thread, so the
   // serving loop only copies it device-to-device when it places the request
   r->hpack = pin_get(bytes, &r->hpack_cap);
@@ -447,8 +446,9 @@ long long Server::submit(int model, const Box& box, int N, const int* type, cons
     for (int d = 0; d < 3; ++d) {
       pd[N + 3 * (size_t)a + d] = wpos[(size_t)d * N + a];
       pd[4 * (size_t)N + 3 * (size_t)a + d] = vel ? vel[(size_t)d * N + a] : 0.0;
+      pd[7 * (size_t)N + 3 * (size_t)a + d] = force ? force[(size_t)d * N + a] : 0.0;
     }
-  std::copy(g_shift.begin(), g_shift.end(), pd + 7 * (size_t)N);
+  std::copy(g_shift.begin(), g_shift.end(), pd + 10 * (size_t)N);
   // asynchronous upload: the lane's stream waits for `ready` before it
   // places the request
   r->pack = pack_get(bytes, &r->pack_cap);
@@ -459,6 +459,34 @@ long long Server::submit(int model, const Box& box, int N, const int* type, cons
     release_upload(*r);
     throw std::runtime_error(std::string("request upload: ") + hipGetErrorString(e));
   }
+}
+
+long long Server::submit(int model, const Box& box, int N, const int* type, const double* mass, const double* pos,
+                         const double* vel, int steps, double dt_fs)
+{
+  if (model < 0 || model >= (int)lanes_.size()) throw std::invalid_argument("model index out of range");
+  if (N <= 0 || !type || !mass || !pos) throw std::invalid_argument("a request needs atoms, types, masses, positions");
+  if (steps < 0) throw std::invalid_argument("negative step count");
+  if (steps > 0 && !(dt_fs > 0.0)) throw std::invalid_argument("MD requests need dt_fs > 0");
+  const double t_call = now_s();
+  NepEngine* nep = lanes_[(size_t)model]->nep;
+  auto r = std::make_shared<Request>();
+  r->model = model;
+  r->box = box;
+  const double vol = box_volume(box.h);
+  if (!(std::fabs(vol) > 0.0) || !std::isfinite(vol)) throw std::invalid_argument("box has no volume");
+  box_inverse(box.h, r->hi);
+  r->n = N;
+  r->steps = steps;
+  r->dt = dt_fs / kTimeUnit;
+  const int T = nep->model().num_types;
+  for (int i = 0; i < N; ++i) {
+    if (type[i] < 0 || type[i] >= T) throw std::invalid_argument("atom type out of range");
+    if (!(mass[i] > 0.0)) throw std::invalid_argument("masses must be positive");
+  }
+  make_pack(*r, type, mass, pos, vel, nullptr, budget_);
+  r->types.assign(type, type + N);
+  r->masses.assign(mass, mass + N);
   std::lock_guard<std::mutex> lk(mu_);
   if (!error_.empty()) {
     release_upload(*r);
@@ -558,8 +586,19 @@ void Server::worker()
       }
       const double t1 = now_s();
       // every lane's frame is enqueued on its own stream before any is
-      // waited for, so the lanes' kernels overlap on the device
-      for (Lane* L : run) enqueue_frame(*L);
+      // waited for, so the lanes' kernels overlap on the device: first the
+      // half-kick and the image-shell check, then (after new shells where an
+      // atom left the shell's budget) the force evaluation
+      for (Lane* L : run) enqueue_kick(*L);
+      for (Lane* L : run) {
+        if (!L->any_md) continue;
+        check(hipStreamSynchronize(L->stream), "serve shell check");
+        if (reshell(*L)) {
+          relayout(*L);
+          upload_md_flags(*L);
+        }
+      }
+      for (Lane* L : run) enqueue_force(*L);
       for (Lane* L : run) finish_frame(*L);
       const double t2 = now_s();
       for (Lane* L : run) retire(*L);
@@ -598,7 +637,8 @@ void Server::worker()
 // the device bytes of the live requests plus the new ones stay within the
 // HBM budget (and the slot and request limits); the first request that does
 // not fit stops admission, so large requests are not starved.  A lane takes
-// its admitted requests into a new layout when it is empty, or when those
+// its admitted requests into a new layout when it is empty, when it is
+// already due for one (it drops its finished requests), or when those
 // requests' slots plus the slots of its finished requests still in the
 // layout reach a quarter of the layout, so that a layout (and the Verlet
 // rebuild it forces) is not redone for every single arrival; requests a lane
@@ -641,7 +681,9 @@ void Server::admit()
     if (cand[l].empty()) continue;
     Lane& L = *lanes_[l];
     const bool empty = lane_live[l] == 0;
-    if (!empty && 4 * (add[l] + lane_dead[l]) < lane_live[l] + lane_dead[l]) continue;
+    // a lane already due for a new layout (its finished requests are being
+    // dropped) takes whatever was admitted for it at no extra rebuild
+    if (!empty && !L.dirty && 4 * (add[l] + lane_dead[l]) < lane_live[l] + lane_dead[l]) continue;
     std::vector<std::shared_ptr<Request>> next;
     for (auto& r : L.active)
       if (r->live) next.push_back(r);
@@ -670,7 +712,12 @@ void Server::relayout(Lane& L)
   drop_graph(L);  // the captured launches take the old layout's arrays
   std::vector<std::shared_ptr<Request>> act;
   {
+    // finished requests (their results already extracted) leave the layout
     std::lock_guard<std::mutex> lk(mu_);
+    std::vector<std::shared_ptr<Request>> keep;
+    for (auto& r : L.active)
+      if (r->live) keep.push_back(r);
+    L.active.swap(keep);
     act = L.active;
   }
   // placement: every request's extended region in its own cell of a cubic
@@ -705,6 +752,15 @@ void Server::relayout(Lane& L)
   L.req_md.need(R, "serve request flags");
   L.req_dt.need(R, "serve request dt");
   L.slots.need(R, "serve layout table");
+  L.home2.need(3 * n_owned, "serve home positions");
+  L.req_flag.need(R, "serve drift flags");
+  if (R > L.h_drift_cap) {
+    if (L.h_drift) (void)hipHostFree(L.h_drift);
+    L.h_drift = nullptr;
+    L.h_drift_cap = 0;
+    check(hipHostMalloc((void**)&L.h_drift, (size_t)R * sizeof(int), hipHostMallocDefault), "pinned drift flags");
+    L.h_drift_cap = R;
+  }
   // host side: one table row per request; new requests bring their packs
   std::vector<ServeSlots> tab((size_t)R);
   std::vector<double> h_dt((size_t)std::max(R, 1));
@@ -769,6 +825,7 @@ void Server::relayout(Lane& L)
   from.owner = L.owner.p;
   from.image = L.image.p;
   from.gstart = L.gstart.p;
+  from.home = L.home.p;
   to.Nt = (int)Nt;
   to.n_owned = (int)n_owned;
   to.pos = L.pos2.p;
@@ -780,7 +837,9 @@ void Server::relayout(Lane& L)
   to.owner = L.owner2.p;
   to.image = L.image2.p;
   to.gstart = L.gstart2.p;
+  to.home = L.home2.p;
   check(launch_serve_assemble(R, L.slots.p, from, to, L.atom_req.p, L.stream), "serve assemble");
+  check(launch_clear_ints(L.req_flag.p, R, L.stream), "serve drift flags");
   check(hipStreamSynchronize(L.stream), "serve relayout");
   for (Request* r : consumed) release_upload(*r);
   std::swap(L.pos, L.pos2);
@@ -792,6 +851,8 @@ void Server::relayout(Lane& L)
   std::swap(L.image, L.image2);
   std::swap(L.gstart, L.gstart2);
   std::swap(L.g_shift, L.g_shift2);
+  std::swap(L.home, L.home2);
+  if (R > 0) std::memset(L.h_drift, 0, (size_t)R * sizeof(int));
   L.n_owned = (int)n_owned;
   L.n_ghost = (int)n_ghost;
   L.n_total = (int)Nt;
@@ -804,13 +865,10 @@ void Server::relayout(Lane& L)
   st_.relayouts++;
 }
 
-// Enqueues one frame of a lane on its stream, without waiting: the first
-// half-kick of the requests that take an MD step, then the force launches
-// (the captured graph, or the eager launches).
-void Server::enqueue_frame(Lane& L)
+// Uploads which of the lane's requests take an MD step this iteration.
+void Server::upload_md_flags(Lane& L)
 {
   const int R = (int)L.active.size();
-  // which requests take an MD step this iteration
   std::vector<int> md((size_t)std::max(R, 1), 0);
   L.any_md = false;
   {
@@ -823,10 +881,37 @@ void Server::enqueue_frame(Lane& L)
   L.stage(md.size() * 4);
   std::memcpy(L.h_stage, md.data(), md.size() * 4);
   check(hipMemcpyAsync(L.req_md.p, L.h_stage, md.size() * 4, hipMemcpyHostToDevice, L.stream), "serve flags");
-  if (L.any_md)
+}
+
+// Enqueues the first part of a lane's frame on its stream, without waiting:
+// the first half-kick of the requests that take an MD step and the check of
+// their drift against the image shells (flags copied back).
+void Server::enqueue_kick(Lane& L)
+{
+  const int R = (int)L.active.size();
+  upload_md_flags(L);
+  if (L.any_md) {
     check(launch_serve_kick(1, L.n_owned, L.n_total, L.atom_req.p, L.req_md.p, L.req_dt.p, L.mass.p, L.pos.p, L.vel.p,
                             L.force.p, L.stream),
           "serve kick 1");
+    // the image shell holds every image within rc + skin + margin of the
+    // box at its build: exact while no atom is more than margin / 2 from
+    // where it was then.  A request past that bound at this step's positions
+    // gets a new shell before the force evaluation, with every MD request
+    // past a quarter of the margin (one relayout for the wave).
+    const double soft = 0.25 * cfg_.margin, hard = 0.5 * cfg_.margin;
+    check(launch_serve_drift(L.n_owned, L.n_total, L.atom_req.p, L.pos.p, L.home.p, soft * soft, hard * hard,
+                             L.req_flag.p, L.stream),
+          "serve drift");
+    check(hipMemcpyAsync(L.h_drift, L.req_flag.p, (size_t)R * sizeof(int), hipMemcpyDeviceToHost, L.stream),
+          "serve drift flags");
+  }
+}
+
+// Enqueues the force launches of a lane's frame (the captured graph, or the
+// eager launches), without waiting.
+void Server::enqueue_force(Lane& L)
+{
   const unsigned flags = cfg_.flags & ~3u;
   const ImageMap map = L.map();
   if (!L.nep->tuning().tuned && L.n_owned >= kTuneMinAtoms) {
@@ -921,6 +1006,91 @@ void Server::drop_graph(Lane& L)
   if (L.graph) (void)hipGraphDestroy(L.graph);
   L.graph_exec = nullptr;
   L.graph = nullptr;
+}
+
+// host threads that pack new image shells
+constexpr size_t kReshellThreads = 8;
+
+bool Server::reshell(Lane& L)
+{
+  if (!L.any_md || L.n_owned == 0) return false;
+  std::vector<Request*> todo;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    bool hard = false;
+    for (size_t q = 0; q < L.active.size(); ++q) {
+      Request* r = L.active[q].get();
+      if (L.h_drift[q] == 2 && r->live && r->state == 1 && r->placed) hard = true;
+    }
+    if (!hard) return false;
+    for (size_t q = 0; q < L.active.size(); ++q) {
+      Request* r = L.active[q].get();
+      if (L.h_drift[q] && r->live && r->state == 1 && r->placed) todo.push_back(r);
+    }
+  }
+  // the requests' current state as a finished request's result block
+  // (positions wrapped into their boxes, velocities, forces), one extraction
+  // for all of them; then new shells from it
+  long long no = 0;
+  std::vector<ServeOut> tab(todo.size());
+  for (size_t q = 0; q < todo.size(); ++q) {
+    const Request& r = *todo[q];
+    ServeOut& t = tab[q];
+    t.slot0 = r.slot0;
+    t.n = r.n;
+    t.out = (long long)kServeOutDoubles * no;
+    for (int d = 0; d < 3; ++d) {
+      t.pbc[d] = r.box.pbc[d] ? 1 : 0;
+      t.trans[d] = r.trans[d];
+    }
+    std::memcpy(t.h, r.box.h, sizeof(t.h));
+    std::memcpy(t.hi, r.hi, sizeof(t.hi));
+    no += r.n;
+  }
+  const size_t out_doubles = (size_t)kServeOutDoubles * (size_t)no;
+  L.outq.need((long long)tab.size(), "serve out table");
+  L.out.need((long long)out_doubles, "serve results");
+  L.stage(tab.size() * sizeof(ServeOut));
+  std::memcpy(L.h_stage, tab.data(), tab.size() * sizeof(ServeOut));
+  check(hipMemcpyAsync(L.outq.p, L.h_stage, tab.size() * sizeof(ServeOut), hipMemcpyHostToDevice, L.stream),
+        "serve reshell table");
+  check(launch_serve_extract((int)tab.size(), L.outq.p, no, L.n_total, L.pos.p, L.vel.p, L.force.p, L.pe.p, L.vir.p,
+                             L.out.p, L.stream),
+        "serve reshell extract");
+  size_t cap = 0;
+  double* blk = static_cast<double*>(pin_get(out_doubles * 8, &cap));
+  check(hipMemcpyAsync(blk, L.out.p, out_doubles * 8, hipMemcpyDeviceToHost, L.stream), "serve reshell state");
+  check(hipStreamSynchronize(L.stream), "serve reshell state");
+  // the packs on up to kReshellThreads host threads (each request's own)
+  std::atomic<size_t> next{0};
+  std::string failure;
+  std::mutex fail_mu;
+  auto work = [&] {
+    for (size_t q; (q = next.fetch_add(1)) < todo.size();) {
+      Request* r = todo[q];
+      const size_t n = (size_t)r->n;
+      const double* b = blk + tab[q].out;
+      try {
+        // block: potential [n], force [3n], position [3n], velocity [3n], virial
+        make_pack(*r, r->types.data(), r->masses.data(), b + 4 * n, b + 7 * n, b + n,
+                  std::numeric_limits<long long>::max());
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(fail_mu);
+        if (failure.empty()) failure = e.what();
+      }
+      r->placed = false;
+    }
+  };
+  const size_t nt = std::min<size_t>(kReshellThreads, todo.size());
+  std::vector<std::thread> pool;
+  for (size_t k = 1; k < nt; ++k) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  pin_put(blk, cap);
+  if (!failure.empty()) throw std::runtime_error("serving: new image shell: " + failure);
+  std::lock_guard<std::mutex> lk(mu_);
+  st_.reshells += (long long)todo.size();
+  return true;
 }
 
 void Server::retire(Lane& L)

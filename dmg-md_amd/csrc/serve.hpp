@@ -15,7 +15,12 @@
 // (the owner's descriptor, radial adjoint and angular partials are the
 // image's by periodicity), so each request gets exactly its periodic result
 // while ghosts cost positions only.  Ghost positions are refreshed from their
-// owners (+ box shift) every step.
+// owners (+ box shift) every step.  The shell stays exact while no atom is
+// more than margin / 2 from where it was when the shell was built: after each
+// half-kick the device checks that, and an MD request past it gets a new
+// shell (its atoms wrapped into its box, its images recomputed, velocities
+// and forces carried) before that step's force evaluation, together with the
+// MD requests past a quarter of the margin.
 //
 // Each model has its own batch ("lane": layout, engine workspace, stream and
 // captured force graph).  A worker thread owns the GPU work: each iteration
@@ -47,7 +52,7 @@ struct ServeConfig {
   long long max_atoms = 4000000;  // owned + ghost slots in flight, all models
   int max_requests = 4096;
   unsigned flags = 0;             // force flags (ANN mode)
-  double margin = 1.0;            // extra shell thickness (A) for motion during a request
+  double margin = 1.0;            // extra shell thickness (A) for an MD request's motion between new shells
   long long hbm_bytes = 0;        // device bytes of the requests in flight; 0: half the free memory at start
   long long reserve_atoms = 1LL << 21;  // engine workspace sized up front per model (capped by the budget)
   bool use_graph = true;          // replay the force launches of a layout as one hipGraph
@@ -60,6 +65,7 @@ struct ServeStats {
   long long max_slots = 0;
   double relayout_s = 0, iterate_s = 0, retire_s = 0;
   long long hbm_budget = 0, max_bytes_in_flight = 0;
+  long long reshells = 0;  // MD requests given a new image shell after drifting
 };
 
 // The Server owns its NepEngines' GPU work while it lives: the engines must
@@ -94,9 +100,21 @@ class Server {
   void worker();
   void admit();
   void relayout(Lane& L);
-  void enqueue_frame(Lane& L);
+  void upload_md_flags(Lane& L);
+  void enqueue_kick(Lane& L);
+  void enqueue_force(Lane& L);
   void finish_frame(Lane& L);
   void retire(Lane& L);
+  // the request's image shell and upload pack from its positions (wrapped
+  // into its box here), velocities and forces, SoA [3n]; sets its ghost
+  // count, extent and device bytes (device bytes above `limit` throw)
+  void make_pack(Request& r, const int* type, const double* mass, const double* pos, const double* vel,
+                 const double* force, long long limit);
+  // after the frame's half-kick (the lane's stream idle): when an MD request
+  // of the lane has an atom past the image shell's budget, new shells for it
+  // and the other MD requests past a quarter of the margin, from their
+  // current state; true when the lane needs a new layout for them
+  bool reshell(Lane& L);
 
   ServeConfig cfg_;
   std::vector<std::unique_ptr<Lane>> lanes_;

@@ -333,7 +333,8 @@ typedef struct dmgx_server_config {
   long long max_atoms;   /* slots in flight over all models: owned atoms + periodic-image shells */
   int max_requests;      /* requests in flight */
   unsigned flags;        /* DMGX_ANN_BF16 / DMGX_ANN_VALU */
-  double margin;         /* extra shell thickness (A) for motion during a request */
+  double margin;         /* extra shell thickness (A) for an MD request's motion: a request
+                            whose atoms move margin / 2 gets a new image shell */
   int use_graph;         /* nonzero: the force launches of a batch layout are captured once
                             into a hipGraph and replayed every iteration */
   long long hbm_bytes;   /* admission budget: device bytes of the requests in flight, each
@@ -355,6 +356,8 @@ typedef struct dmgx_server_stats {
   double relayout_s, iterate_s, retire_s; /* worker wall time per phase */
   long long hbm_budget;          /* the admission budget in bytes */
   long long max_bytes_in_flight; /* the most device bytes admitted at once */
+  long long reshells;            /* MD requests given a new image shell after their atoms drifted
+                                    a quarter of the margin from where the shell was built */
 } dmgx_server_stats;
 
 /* Starts a server (one worker thread; one batch and one stream per model) on

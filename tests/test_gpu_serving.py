@@ -81,6 +81,39 @@ def test_server_requests_match_standalone(dmgx, workloads):
     assert got["latency_ms"] > 0
 
 
+def test_drifting_md_request_gets_new_shells(dmgx):
+    """An NVE request whose atoms all travel about 2.5 A (a 512-atom carbon
+    diamond with a uniform drift of 0.05 A/fs, 100 steps of 0.5 fs), more
+    than the 1 A margin of the image shell built at submit: the server gives
+    it a new shell whenever an atom is half the margin from where its shell
+    was built, before that step's force evaluation, and the result matches
+    the standalone MD of the same system (without new shells the images it
+    needs drop out after 0.5 A)."""
+    srv = dmgx.Server(dmgx.NEP(CARBON), max_atoms=400_000)
+    h, p = _diamond(4, 0.05, 31)
+    rng = np.random.default_rng(32)
+    v = rng.normal(0, 0.01, p.shape)
+    v -= v.mean(1, keepdims=True)
+    v[0] += 0.5  # natural velocity units (A per 10.18 fs): 0.049 A/fs
+    m = np.full(512, 12.011)
+    box = dmgx.Box(h, (1, 1, 1))
+    rid = srv.submit(box, np.zeros(512, np.int32), m, p, v, steps=100, dt_fs=0.5)
+    srv.drain()
+    got = srv.poll(rid)
+    st = srv.stats()
+    md = dmgx.MD(dmgx.NEP(CARBON), box, np.zeros(512, np.int32), m, p, v)
+    md.run(100, 0.5, "nve")
+    ref = md.arrays()
+    L = h[0, 0]
+    d = got["position"] - ref["position"]
+    d -= np.round(d / L) * L
+    print(f"drifting request: {st['reshells']} new shells, position diff {np.abs(d).max():.2e} A, "
+          f"velocity diff {np.abs(got['velocity'] - ref['velocity']).max():.2e}", flush=True)
+    assert st["reshells"] >= 3, st
+    assert np.abs(d).max() < 1e-5
+    assert np.abs(got["velocity"] - ref["velocity"]).max() < 1e-5
+
+
 def test_server_many_small_requests(dmgx, workloads):
     """Many concurrent single points of different sizes: all correct, and the
     loop reports its frames and latency percentiles."""
@@ -275,10 +308,15 @@ def test_serving_stream_mix_against_oracle(dmgx, workloads):
             # FP32 rounding on an ill-conditioned input: the served error is
             # the plain path's up to the spread between two rounding samples
             # (a single point: the same positions, twice; after an MD segment:
-            # each run's own final positions, 1e-6 A apart, four times)
+            # each run's own final positions, 1e-6 A apart, four times), and
+            # in any case within the scale the FP32 error reaches at that
+            # conditioning across this stream (1e-6 eV/A per unit of scaled
+            # spread: up to 6e-4 eV/A on the spread-625 vacancy lattices,
+            # where MD requests also run at wrapped coordinates after new
+            # image shells)
             k = 4.0 if mat.endswith("+md") else 2.0
-            for key, tol in (("force", TOL_F), ("virial", TOL_V), ("energy_atom_max", TOL_EA)):
-                assert e_srv[key] <= max(tol, k * e_plain[key]), (mat, n, key, e_srv, e_plain)
+            for key, tol, per in (("force", TOL_F, 1e-6), ("virial", TOL_V, 2e-6), ("energy_atom_max", TOL_EA, 2e-7)):
+                assert e_srv[key] <= max(tol, k * e_plain[key], per * spread), (mat, n, key, e_srv, e_plain)
     bf = serve(True)
     rel = []
     for (r, m, st), a, b in zip(stream, bf, got):
