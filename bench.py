@@ -211,6 +211,7 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
             "frames_per_s": st["frames_per_s"], "atom_frames_per_s": st["atom_frames"] / max(st["elapsed_s"], 1e-9),
             "capacity_run": "all requests submitted at once by 8 client threads",
             "requests": st["completed"], "relayouts": st["relayouts"], "iterations": st["iterations"],
+            "new_image_shells": st["reshells"],
             "max_batch_slots": st["max_slots"],
             "hbm_budget_bytes": st["hbm_budget"], "max_bytes_in_flight": st["max_bytes_in_flight"],
             "worker_s": {"relayout": round(st["relayout_s"], 4), "iterate": round(st["iterate_s"], 4),
