@@ -59,7 +59,8 @@ class ServerStats(ctypes.Structure):
         (n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "elapsed_s", "frames_per_s")] + [
         (n, ctypes.c_longlong) for n in ("iterations", "relayouts", "max_slots")] + [
         (n, ctypes.c_double) for n in ("relayout_s", "iterate_s", "retire_s")] + [
-        (n, ctypes.c_longlong) for n in ("hbm_budget", "max_bytes_in_flight", "reshells")]
+        (n, ctypes.c_longlong) for n in ("hbm_budget", "max_bytes_in_flight", "reshells",
+                                            "rows_carried")]
 
 
 class Tuning(ctypes.Structure):

@@ -590,6 +590,7 @@ int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out)
     out->hbm_budget = s.hbm_budget;
     out->max_bytes_in_flight = s.max_bytes_in_flight;
     out->reshells = s.reshells;
+    out->rows_carried = s.rows_carried;
   });
 }
 

@@ -558,6 +558,28 @@ void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int 
   last_c2_ = nd2;
 }
 
+void NepEngine::adopt_carried(const Box& box, int N, int n_owned, int c_keep, int g_keep, const int* type,
+                              const double* pos, double* pe, double* force, double* virial, const ImageMap& map,
+                              hipStream_t s)
+{
+  if (N > w_.cap) throw std::logic_error("adopt_carried: workspace smaller than the batch");
+  if (c_keep < 0 || c_keep > n_owned || g_keep < n_owned || g_keep > N)
+    throw std::invalid_argument("adopt_carried: carried ranges outside the batch");
+  const Plan p = plan(box, N, 0, n_owned, 0, n_owned, type, pos, nullptr, pe, force, virial, 0u, &map);
+  if (p.empty) return;
+  const double rv = model_.rcR_max + kSkin;
+  const int ranges[4] = {c_keep, n_owned, g_keep, N};
+  check(launch_pack(p.box, N, model_.num_types, pos, type, true, w_, s, half_skin()), "pack");
+  check(launch_rebuild(p.box, N, c_keep, n_owned, rv * rv, nullptr, w_, s, ranges), "rebuild new centres");
+  check(launch_filter(p.box, dm_, 0, 0, w_, s, tune_.filter_lanes, true), "close rebuild request");
+  scale_since_build_ = 1.0;
+  last_small_ = false;
+  last_box_ = p.box;
+  last_N_stats_ = N;
+  last_c1_ = 0;
+  last_c2_ = n_owned;
+}
+
 bool NepEngine::small_box(const Box& box, int reps[3]) const
 {
   const DevBox b = make_devbox(box);

@@ -105,6 +105,17 @@ class NepEngine {
   void launch_force(const Plan& p, int N, int n1, int n2, int nd1, int nd2, const int* type, const uint64_t* gid,
                     double* pe, double* force, double* virial, unsigned flags, hipStream_t stream);
   DevWork& work() { return w_; }
+  // Serving relayouts: adopts a batch of N slots (centres [0, n_owned)) whose
+  // first centres [0, c_keep) had their Verlet rows, and their slots their
+  // displacement references, carried over from the previous batch by the
+  // caller (continuing requests, translated).  Packs the positions and
+  // builds the rows of centres [c_keep, n_owned) only, resetting the
+  // references of the slots [c_keep, n_owned) and [g_keep, N); the list is
+  // then valid for this box, N and map (the device's skin check still asks
+  // for a full rebuild once a carried atom has moved half the skin).  The
+  // workspace must already hold N slots (reserve) with unchanged rows.
+  void adopt_carried(const Box& box, int N, int n_owned, int c_keep, int g_keep, const int* type, const double* pos,
+                     double* pe, double* force, double* virial, const ImageMap& map, hipStream_t stream);
   // Tile shapes picked by autotune(): lanes per centre of the filter (8, 16,
   // 32) and per atom of the gather (4, 8, 16), and the row layout: the
   // capacity of every per-centre row (Verlet, edge, partial-force rows)

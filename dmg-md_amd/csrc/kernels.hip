@@ -2388,6 +2388,57 @@ __global__ void k_serve_assemble(int R, const ServeSlots* __restrict__ req, Serv
   }
 }
 
+// the request of slot s among the first R table rows (last whose range starts
+// at or before s), by owned or ghost range
+__device__ __forceinline__ int serve_request_of(int R, const ServeSlots* __restrict__ req, int s, bool owned)
+{
+  int lo = 0, hi = R - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((owned ? req[mid].new_o : req[mid].new_g) <= s) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+constexpr int kCarryLanes = 16;  // lanes per row in k_serve_carry_rows
+
+__global__ void k_serve_carry_rows(int R, const ServeSlots* __restrict__ req, int old_no, int o_keep,
+                                   const int* __restrict__ vl0, const int* __restrict__ vrev0,
+                                   const int* __restrict__ vn0, DevWork w)
+{
+  const int s = blockIdx.x * (blockDim.x / kCarryLanes) + threadIdx.x / kCarryLanes;
+  const int lane = threadIdx.x % kCarryLanes;
+  if (s >= o_keep) return;
+  const ServeSlots t = req[serve_request_of(R, req, s, true)];
+  const int so = t.old_o + (s - t.new_o);
+  const int n = vn0[so];
+  if (lane == 0) w.vn[s] = n;
+  const int m = min(n, w.MV);
+  const size_t ro = (size_t)so * w.MV, rn = (size_t)s * w.MV;
+  // every entry of a request's row is one of its own slots (requests are
+  // disjoint in space): owned slots and ghosts each keep their offset
+  for (int p = lane; p < m; p += kCarryLanes) {
+    const int j = vl0[ro + p];
+    w.vl[rn + p] = j < old_no ? t.new_o + (j - t.old_o) : t.new_g + (j - t.old_g);
+    w.vrev[rn + p] = vrev0[ro + p];
+  }
+}
+
+__global__ void k_serve_carry_x0(int R, const ServeSlots* __restrict__ req, int old_nt, int new_no, int new_nt,
+                                 int o_keep, int g_keep, const double* __restrict__ x0_0, double* __restrict__ x0)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = o_keep + (g_keep - new_no);
+  if (k >= total) return;
+  const bool owned = k < o_keep;
+  const int s = owned ? k : new_no + (k - o_keep);
+  const ServeSlots t = req[serve_request_of(R, req, s, owned)];
+  const int so = owned ? t.old_o + (s - t.new_o) : t.old_g + (s - t.new_g);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) x0[(size_t)c * new_nt + s] = x0_0[(size_t)c * old_nt + so] + t.dtr[c];
+}
+
 __global__ void k_serve_drift(int n_owned, int Nt, const int* __restrict__ atom_req, const double* __restrict__ pos,
                               const double* __restrict__ home, double soft2, double hard2, int* __restrict__ req_flag)
 {
@@ -2459,6 +2510,21 @@ hipError_t launch_serve_assemble(int R, const ServeSlots* req, const ServeLayout
                                  int* atom_req, hipStream_t s)
 {
   if (R > 0 && to.Nt > 0) k_serve_assemble<<<nblk(to.Nt), kBlock, 0, s>>>(R, req, from, to, atom_req);
+  return hipGetLastError();
+}
+
+hipError_t launch_serve_carry_rows(int R_keep, const ServeSlots* req, int old_no, int old_nt, int new_no, int new_nt,
+                                   int o_keep, int g_keep, const int* vl0, const int* vrev0, const int* vn0,
+                                   const double* x0_0, DevWork& w, hipStream_t s)
+{
+  if (R_keep <= 0) return hipSuccess;
+  if (o_keep > 0) {
+    constexpr int per = kBlock / kCarryLanes;
+    k_serve_carry_rows<<<(o_keep + per - 1) / per, kBlock, 0, s>>>(R_keep, req, old_no, o_keep, vl0, vrev0, vn0, w);
+  }
+  const int total = o_keep + (g_keep - new_no);
+  if (total > 0)
+    k_serve_carry_x0<<<nblk(total), kBlock, 0, s>>>(R_keep, req, old_nt, new_no, new_nt, o_keep, g_keep, x0_0, w.x0);
   return hipGetLastError();
 }
 

@@ -143,8 +143,11 @@ hipError_t launch_clear_ints(int* p, long long n, hipStream_t s);
 // (default half the skin)
 hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const int* type, bool forced, DevWork& w,
                        hipStream_t s, double half_skin = -1.0);
+// x0_ranges (optional): {a1, b1, a2, b2}, the slots whose displacement
+// references this rebuild resets (default every slot); the others keep
+// theirs (rows carried over from an earlier build)
 hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, const uint64_t* gid, DevWork& w,
-                          hipStream_t s);
+                          hipStream_t s, const int* x0_ranges = nullptr);
 
 // Force evaluation (per step)
 // lanes: lanes per centre, 8, 16 (default) or 32
@@ -274,6 +277,16 @@ hipError_t launch_serve_assemble(int R, const ServeSlots* req, const ServeLayout
 // only ever raised here)
 hipError_t launch_serve_drift(int n_owned, int Nt, const int* atom_req, const double* pos, const double* home,
                               double soft2, double hard2, int* req_flag, hipStream_t s);
+// The Verlet rows of a relayout's continuing requests (the first R_keep
+// rows of the layout table) moved to their new slots: vl, vrev and vn of
+// their owned centres from the copies vl0, vrev0, vn0 (the previous
+// layout's, old_no owned slots), every entry mapped through its request's
+// owned and ghost slot offsets; x0 of all their slots from x0_0 (stride
+// old_nt) plus the translation, in w.x0 (stride new_nt).
+// Their owned slots are [0, o_keep), their ghosts [new_no, g_keep).
+hipError_t launch_serve_carry_rows(int R_keep, const ServeSlots* req, int old_no, int old_nt, int new_no, int new_nt,
+                                   int o_keep, int g_keep, const int* vl0, const int* vrev0, const int* vn0,
+                                   const double* x0_0, DevWork& w, hipStream_t s);
 // One finished request for launch_serve_extract: its owned slots, the
 // offset of its block in the output, its placement translation and its box.
 constexpr int kServeOutDoubles = 19;  // result doubles per atom

@@ -61,10 +61,11 @@ __global__ __launch_bounds__(kBlock) void k_pack(DevBox b, int N, int T, const d
 }
 
 // cell of every atom, its rank inside the cell, and the positions the next
-// displacement checks compare against
+// displacement checks compare against (for the slots in [a1, b1) and
+// [a2, b2): the slots whose rows this rebuild makes)
 __global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const double4* __restrict__ xq,
                              int* __restrict__ cell_of, int* __restrict__ rank, int* __restrict__ count,
-                             double* __restrict__ x0, int* __restrict__ flags)
+                             double* __restrict__ x0, int a1, int b1, int a2, int b2, int* __restrict__ flags)
 {
   if (!*go) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -75,9 +76,11 @@ __global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const 
   const int id = c[0] + b.nc[0] * (c[1] + b.nc[1] * c[2]);
   cell_of[i] = id;
   rank[i] = atomicAdd(&count[id], 1);
-  x0[i] = p.x;
-  x0[N + i] = p.y;
-  x0[2 * N + i] = p.z;
+  if ((i >= a1 && i < b1) || (i >= a2 && i < b2)) {
+    x0[i] = p.x;
+    x0[N + i] = p.y;
+    x0[2 * N + i] = p.z;
+  }
 }
 
 // Exclusive scan of the cell counts by one workgroup, which also clears the
@@ -565,12 +568,15 @@ hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const i
 }
 
 hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, const uint64_t* gid, DevWork& w,
-                          hipStream_t s)
+                          hipStream_t s, const int* x0_ranges)
 {
   const int* go = w.flags + kFlagRebuild;
   const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
+  const int r[4] = {x0_ranges ? x0_ranges[0] : 0, x0_ranges ? x0_ranges[1] : N, x0_ranges ? x0_ranges[2] : 0,
+                    x0_ranges ? x0_ranges[3] : 0};
   if (N > 0)
-    k_cell_count<<<nblk(N), kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, w.flags);
+    k_cell_count<<<nblk(N), kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, r[0], r[1],
+                                            r[2], r[3], w.flags);
   if (ncell <= kScanOneBlock) {
     k_scan_cells<<<1, 1024, 0, s>>>(go, ncell, w.cell_count, w.cell_start);
   } else {

@@ -183,6 +183,10 @@ struct Server::Lane {
   // the next layout's per-slot arrays, assembled on the device and swapped in
   DBuf<double> pos2, vel2, force2, mass2, g_shift2;
   DBuf<double> home, home2;  // owned slots' positions where their request's shell was built
+  // the previous layout's Verlet rows (and the displacement references of its
+  // slots), copied out of the engine at a relayout for the continuing requests
+  DBuf<int> vl0, vrev0, vn0;
+  DBuf<double> x00;
   DBuf<int> req_flag;        // active request -> 1 once an atom drifted past the shell's budget
   int* h_drift = nullptr;    // pinned copy of req_flag, read after each frame
   int h_drift_cap = 0;
@@ -225,10 +229,10 @@ struct Server::Lane {
   void release()
   {
     for (DBuf<double>* b : {&pos, &vel, &force, &pe, &vir, &mass, &pos2, &vel2, &force2, &mass2, &req_dt, &g_shift,
-                            &g_shift2, &out, &home, &home2})
+                            &g_shift2, &out, &home, &home2, &x00})
       b->release();
     for (DBuf<int>* b : {&type, &type2, &atom_req, &req_md, &owner, &owner2, &image, &image2, &gstart, &gstart2,
-                         &req_flag})
+                         &req_flag, &vl0, &vrev0, &vn0})
       b->release();
     if (h_drift) (void)hipHostFree(h_drift);
     h_drift = nullptr;
@@ -713,12 +717,34 @@ void Server::relayout(Lane& L)
   std::vector<std::shared_ptr<Request>> act;
   {
     // finished requests (their results already extracted) leave the layout
+    // and the continuing requests come first, so that their Verlet rows
+    // can be carried over (the others' rows are built)
     std::lock_guard<std::mutex> lk(mu_);
     std::vector<std::shared_ptr<Request>> keep;
     for (auto& r : L.active)
-      if (r->live) keep.push_back(r);
+      if (r->live && r->placed) keep.push_back(r);
+    for (auto& r : L.active)
+      if (r->live && !r->placed) keep.push_back(r);
     L.active.swap(keep);
     act = L.active;
+  }
+  // the previous layout's rows, copied out before the engine may grow (only
+  // when they are that layout's: a layout that has not been evaluated since
+  // an uncarried relayout has none)
+  DevWork& ew = L.nep->work();
+  const int old_no = L.n_owned, old_nt = L.n_total, mv = ew.MV;
+  bool carry = !L.forced_rebuild && old_no > 0 && !ew.split && ew.vl && ew.cap >= old_nt;
+  if (carry) {
+    const size_t rows = (size_t)old_no * (size_t)mv;
+    L.vl0.need((long long)rows, "serve carried rows");
+    L.vrev0.need((long long)rows, "serve carried reverse");
+    L.vn0.need(old_no, "serve carried row lengths");
+    L.x00.need(3LL * old_nt, "serve carried references");
+    check(hipMemcpyAsync(L.vl0.p, ew.vl, rows * 4, hipMemcpyDeviceToDevice, L.stream), "serve carry rows");
+    check(hipMemcpyAsync(L.vrev0.p, ew.vrev, rows * 4, hipMemcpyDeviceToDevice, L.stream), "serve carry reverse");
+    check(hipMemcpyAsync(L.vn0.p, ew.vn, (size_t)old_no * 4, hipMemcpyDeviceToDevice, L.stream), "serve carry lengths");
+    check(hipMemcpyAsync(L.x00.p, ew.x0, (size_t)old_nt * 24, hipMemcpyDeviceToDevice, L.stream),
+          "serve carry references");
   }
   // placement: every request's extended region in its own cell of a cubic
   // grid of equal slots, separated by more than the Verlet radius
@@ -736,6 +762,13 @@ void Server::relayout(Lane& L)
   }
   const long long Nt = n_owned + n_ghost;
   if (Nt >= (1LL << 30)) throw std::runtime_error("serving batch too large");
+  if (Nt > L.nep->work().cap) {
+    // geometric growth (the rows of the previous layout are already copied)
+    check(hipStreamSynchronize(L.stream), "serve carry");
+    const long long lim = (long long)(INT32_MAX - 1) / std::max(L.nep->work().MV, 128);
+    L.nep->reserve((int)std::max<long long>(Nt, std::min<long long>(2LL * L.nep->work().cap, lim)));
+  }
+  carry = carry && L.nep->work().MV == mv;
   // the new layout's arrays (double-buffered: the previous layout is the source)
   L.pos2.need(3 * Nt, "serve positions");
   L.vel2.need(3 * Nt, "serve velocities");
@@ -766,6 +799,8 @@ void Server::relayout(Lane& L)
   std::vector<double> h_dt((size_t)std::max(R, 1));
   std::vector<Request*> consumed;
   long long o = 0, gk = n_owned;
+  int r_keep = 0;
+  long long o_keep = 0, g_keep = n_owned;
   for (int q = 0; q < R; ++q) {
     Request& r = *act[q];
     const int g3[3] = {q % side, (q / side) % side, q / (side * side)};
@@ -783,6 +818,9 @@ void Server::relayout(Lane& L)
       t.old_o = r.slot0;
       t.old_g = r.gslot0;
       for (int d = 0; d < 3; ++d) t.dtr[d] = tnew[d] - r.trans[d];
+      ++r_keep;
+      o_keep += r.n;
+      g_keep += r.ng;
     } else {
       if (!r.pack) throw std::logic_error("serving: a new request has no pack");
       t.old_o = t.old_g = -1;
@@ -840,6 +878,10 @@ void Server::relayout(Lane& L)
   to.home = L.home2.p;
   check(launch_serve_assemble(R, L.slots.p, from, to, L.atom_req.p, L.stream), "serve assemble");
   check(launch_clear_ints(L.req_flag.p, R, L.stream), "serve drift flags");
+  if (carry)
+    check(launch_serve_carry_rows(r_keep, L.slots.p, old_no, old_nt, (int)n_owned, (int)Nt, (int)o_keep, (int)g_keep,
+                                  L.vl0.p, L.vrev0.p, L.vn0.p, L.x00.p, L.nep->work(), L.stream),
+          "serve carry rows");
   check(hipStreamSynchronize(L.stream), "serve relayout");
   for (Request* r : consumed) release_upload(*r);
   std::swap(L.pos, L.pos2);
@@ -860,9 +902,20 @@ void Server::relayout(Lane& L)
   for (int d = 0; d < 3; ++d) L.batch_box.h[4 * d] = side * slot[d];
   L.batch_box.pbc[0] = L.batch_box.pbc[1] = L.batch_box.pbc[2] = 0;
   L.forced_rebuild = true;
+  if (carry && Nt > 0) {
+    // the continuing requests keep their rows; only the others' are built
+    const ImageMap map = L.map();
+    check(launch_serve_ghosts(L.n_owned, L.n_ghost, L.n_total, L.owner.p + L.n_owned, L.g_shift.p, L.pos.p,
+                              L.stream),
+          "serve ghosts");
+    L.nep->adopt_carried(L.batch_box, L.n_total, L.n_owned, (int)o_keep, (int)g_keep, L.type.p, L.pos.p, L.pe.p,
+                         L.force.p, L.vir.p, map, L.stream);
+    L.forced_rebuild = false;
+  }
   L.dirty = false;
   std::lock_guard<std::mutex> lk(mu_);
   st_.relayouts++;
+  if (!L.forced_rebuild) st_.rows_carried++;
 }
 
 // Uploads which of the lane's requests take an MD step this iteration.

@@ -66,6 +66,7 @@ struct ServeStats {
   double relayout_s = 0, iterate_s = 0, retire_s = 0;
   long long hbm_budget = 0, max_bytes_in_flight = 0;
   long long reshells = 0;  // MD requests given a new image shell after drifting
+  long long rows_carried = 0;  // relayouts whose continuing requests kept their Verlet rows
 };
 
 // The Server owns its NepEngines' GPU work while it lives: the engines must

@@ -358,6 +358,8 @@ typedef struct dmgx_server_stats {
   long long max_bytes_in_flight; /* the most device bytes admitted at once */
   long long reshells;            /* MD requests given a new image shell after their atoms drifted
                                     a quarter of the margin from where the shell was built */
+  long long rows_carried;        /* relayouts whose continuing requests kept their Verlet rows
+                                    (only the new and re-shelled requests' rows were built) */
 } dmgx_server_stats;
 
 /* Starts a server (one worker thread; one batch and one stream per model) on

@@ -114,6 +114,67 @@ def test_drifting_md_request_gets_new_shells(dmgx):
     assert np.abs(got["velocity"] - ref["velocity"]).max() < 1e-5
 
 
+def test_continuing_requests_keep_their_rows(dmgx):
+    """Relayouts while MD requests run (arrivals of a quarter of the layout,
+    finished single points dropped, a drifting request re-shelled): the
+    continuing requests keep their Verlet rows, renumbered to their new
+    slots, and only the others' rows are built.  Every MD request matches
+    its standalone run and every single point the FP64 oracle."""
+    orc = OracleNEP(CARBON, round_params_to_float=True)
+    srv = dmgx.Server(dmgx.NEP(CARBON), max_atoms=2_000_000)
+    rng = np.random.default_rng(51)
+    md_cases, sp_cases = [], []
+
+    def md_request(reps, seed, steps, drift=0.0):
+        h, p = _diamond(reps, 0.05, seed)
+        n = p.shape[1]
+        v = rng.normal(0, 0.01, p.shape)
+        v -= v.mean(1, keepdims=True)
+        v[0] += drift
+        m = np.full(n, 12.011)
+        box = dmgx.Box(h, (1, 1, 1))
+        md_cases.append((srv.submit(box, np.zeros(n, np.int32), m, p, v, steps=steps, dt_fs=0.5), box, h, p, v, m,
+                         steps))
+
+    def single_points(count, seed):
+        for k in range(count):
+            h, p = _diamond(1 + k % 3, 0.05, seed + k)
+            n = p.shape[1]
+            sp_cases.append((srv.submit(dmgx.Box(h, (1, 1, 1)), np.zeros(n, np.int32), np.full(n, 12.011), p), h, p))
+
+    # at most 100 steps of 0.5 fs: FP32 summation-order differences between
+    # the batch and the standalone run grow along the trajectory (1.7e-5 A
+    # after 160 steps), while a wrong row shows within a step
+    md_request(4, 61, 100)
+    time.sleep(0.02)
+    md_request(3, 62, 90, drift=0.5)
+    single_points(6, 70)
+    time.sleep(0.02)
+    md_request(4, 63, 80)
+    md_request(2, 64, 60)
+    time.sleep(0.02)
+    single_points(6, 80)
+    srv.drain()
+    st = srv.stats()
+    print(f"relayouts {st['relayouts']}, rows carried {st['rows_carried']}, new shells {st['reshells']}", flush=True)
+    for rid, h, p in sp_cases:
+        _check_single(srv.poll(rid), h, p, orc)
+    worst = 0.0
+    for rid, box, h, p, v, m, steps in md_cases:
+        got = srv.poll(rid)
+        md = dmgx.MD(dmgx.NEP(CARBON), box, np.zeros(len(m), np.int32), m, p, v)
+        md.run(steps, 0.5, "nve")
+        ref = md.arrays()
+        L = h[0, 0]
+        d = got["position"] - ref["position"]
+        d -= np.round(d / L) * L
+        worst = max(worst, float(np.abs(d).max()))
+        assert np.abs(d).max() < 1e-5, (len(m), steps, np.abs(d).max())
+        assert np.abs(got["velocity"] - ref["velocity"]).max() < 1e-5
+    print(f"worst position diff against standalone MD {worst:.2e} A", flush=True)
+    assert st["rows_carried"] >= 1, st
+
+
 def test_server_many_small_requests(dmgx, workloads):
     """Many concurrent single points of different sizes: all correct, and the
     loop reports its frames and latency percentiles."""
