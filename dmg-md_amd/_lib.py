@@ -60,7 +60,7 @@ class ServerStats(ctypes.Structure):
         (n, ctypes.c_longlong) for n in ("iterations", "relayouts", "max_slots")] + [
         (n, ctypes.c_double) for n in ("relayout_s", "iterate_s", "retire_s")] + [
         (n, ctypes.c_longlong) for n in ("hbm_budget", "max_bytes_in_flight", "reshells",
-                                            "rows_carried")]
+                                            "rows_carried")] + [("reshell_s", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):

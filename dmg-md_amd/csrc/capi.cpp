@@ -591,6 +591,7 @@ int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out)
     out->max_bytes_in_flight = s.max_bytes_in_flight;
     out->reshells = s.reshells;
     out->rows_carried = s.rows_carried;
+    out->reshell_s = s.reshell_s;
   });
 }
 

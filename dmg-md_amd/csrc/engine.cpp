@@ -349,7 +349,7 @@ void NepEngine::free_all()
   dfree(&w_.cell_count);
   dfree(&w_.cell_start);
   dfree(&w_.cell_tiles);
-  w_.cap = 0;
+  w_.cap = w_.rcap = 0;
   w_.ncell_cap = 0;
   have_list_ = false;
 }
@@ -372,31 +372,34 @@ int NepEngine::padded_rows(int mv, int layout) const
 constexpr double kRowEntryBytes = 4 + 4 + 16 + 4 + 4 + 16;
 constexpr double kSplitEntryBytes = 4 + 4 + 4;
 
-void NepEngine::allocate(int capacity, int mv)
+void NepEngine::allocate(int capacity, int mv, int centres)
 {
   mv_base_ = mv;
   mv = padded_rows(mv, tune_.row_layout);
   const size_t cap = (size_t)std::max(capacity, 1);
-  const size_t rows = cap * (size_t)mv;
+  // centres with rows: by default every slot
+  const size_t rc = (size_t)std::max(1, std::min(capacity, centres < 0 ? capacity : centres));
+  const size_t rows = rc * (size_t)mv;
   if (rows >= (size_t)INT32_MAX)
-    throw std::invalid_argument("atoms x Verlet row capacity must stay below 2^31");
+    throw std::invalid_argument("centres x Verlet row capacity must stay below 2^31");
   dalloc(&w_.xq, cap, "packed positions");
   dalloc(&w_.cell_of, cap, "cell_of");
   dalloc(&w_.cell_rank, cap, "cell rank");
   dalloc(&w_.cell_atoms, cap, "cell_atoms");
   dalloc(&w_.xs, cap, "cell-ordered positions");
-  dalloc(&w_.vn, cap, "vn");
+  dalloc(&w_.x0, 3 * cap, "x0");
+  dalloc(&w_.vn, rc, "vn");
   dalloc(&w_.vl, rows, "verlet rows");
   dalloc(&w_.vrev, rows, "verlet reverse");
-  dalloc(&w_.x0, 3 * cap, "x0");
-  dalloc(&w_.nr, cap, "nr");
-  dalloc(&w_.na, cap, "na");
+  dalloc(&w_.nr, rc, "nr");
+  dalloc(&w_.na, rc, "na");
   dalloc(&w_.edge, rows, "edges");
   dalloc(&w_.eq, rows, "edge reverse");
   dalloc(&w_.vsa, rows, "verlet->angular");
-  dalloc(&w_.bR, cap * model_.num_types * (size_t)((model_.kR + 3) & ~3), "radial adjoint");
+  dalloc(&w_.bR, rc * model_.num_types * (size_t)((model_.kR + 3) & ~3), "radial adjoint");
   dalloc(&w_.f12, rows, "angular partials");
   w_.cap = (int)cap;
+  w_.rcap = (int)rc;
   w_.MV = mv;
   tune_.row_capacity = mv;
   w_.MR = mv;
@@ -407,7 +410,7 @@ void NepEngine::allocate(int capacity, int mv)
 
 void NepEngine::allocate_split()
 {
-  const size_t cap = (size_t)std::max(w_.cap, 1), rows = cap * (size_t)w_.MV;
+  const size_t cap = (size_t)std::max(w_.rcap, 1), rows = cap * (size_t)w_.MV;
   dalloc(&w_.cl, rows, "candidate rows");
   dalloc(&w_.cpos, rows, "verlet->angular candidate");
   dalloc(&w_.cq, rows, "candidate reverse");
@@ -442,12 +445,15 @@ void NepEngine::set_split(int split)
   tune_.tuned = true;
 }
 
-void NepEngine::reserve(int capacity)
+void NepEngine::reserve(int capacity, int centres)
 {
   if (capacity < 0) throw std::invalid_argument("negative capacity");
-  if (capacity <= w_.cap && w_.MV > 0) return;
+  if (centres < 0) centres = capacity;
+  centres = std::min(centres, capacity);
+  if (capacity <= w_.cap && centres <= w_.rcap && w_.MV > 0) return;
   const int mv = std::max(mv_base_, (int)std::ceil(model_.MN_R * std::pow((model_.rcR_max + kSkin) / model_.rcR_max, 3)));
-  allocate(std::max(capacity, w_.cap), std::max(mv, 8));
+  const int cap = std::max(capacity, w_.cap);
+  allocate(cap, std::max(mv, 8), std::min(cap, std::max(centres, w_.rcap)));
 }
 
 void NepEngine::ensure_cells(int ncell)
@@ -499,6 +505,8 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
   if (im.owner && (!im.image || !im.gstart)) throw std::invalid_argument("incomplete image map");
   if (im.owner && (n1 != 0 || nd1 != 0 || gid))
     throw std::invalid_argument("an image map needs the owned slots first and rows ordered by slot");
+  if (im.slot_req && (!im.owner || !im.req_moved || im.n_owned != nd2 || im.n_req <= 0))
+    throw std::invalid_argument("a request map needs an image map, its moved words and the owned slots as centres");
   p.box = make_devbox(box);
   for (int d = 0; d < 3; ++d)
     if (p.box.pbc[d] && p.box.thick[d] <= 2.5 * (model_.rcR_max + kSkin))
@@ -506,11 +514,14 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
           "box thinner than 2.5 (rc + skin) in a periodic direction");
   const long long ncell = (long long)p.box.nc[0] * p.box.nc[1] * p.box.nc[2];
   if (ncell > (1LL << 28)) throw std::invalid_argument("cell grid too large");
-  // geometric growth (a serving batch grows request by request); rows stay
-  // addressable by int (cap * MV < 2^31)
-  if (N > w_.cap || w_.MV == 0) {
+  // geometric growth (a serving batch grows request by request); rows (for
+  // the slots below nd2, the ones that can be centres) stay addressable by
+  // int (rcap * MV < 2^31)
+  if (N > w_.cap || nd2 > w_.rcap || w_.MV == 0) {
     const long long lim = (long long)(INT32_MAX - 1) / std::max(w_.MV, 128);
-    reserve((int)std::max<long long>(N, std::min<long long>(2LL * w_.cap, lim)));
+    const long long cap = N > w_.cap ? std::max<long long>(N, 2LL * w_.cap) : w_.cap;
+    const long long rc = nd2 > w_.rcap ? std::max<long long>(nd2, std::min<long long>(2LL * w_.rcap, lim)) : w_.rcap;
+    reserve((int)std::min<long long>(cap, INT32_MAX - 1), (int)rc);
   }
   if (ncell > w_.ncell_cap)
     ensure_cells((int)std::max<long long>(ncell, std::min<long long>(2LL * w_.ncell_cap, 1LL << 28)));
@@ -521,6 +532,12 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
   if (p.forced) scale_since_build_ = 1.0;  // this evaluation rebuilds from the current positions
   p.empty = false;
   w_.map = im;
+  if (w_.split) {
+    // candidate rows are remade for every centre at a rebuild
+    w_.map.slot_req = nullptr;
+    w_.map.req_moved = nullptr;
+    w_.map.n_req = 0;
+  }
   last_map_ = im;
   have_list_ = true;
   box_changed_ = false;
@@ -562,7 +579,7 @@ void NepEngine::adopt_carried(const Box& box, int N, int n_owned, int c_keep, in
                               const double* pos, double* pe, double* force, double* virial, const ImageMap& map,
                               hipStream_t s)
 {
-  if (N > w_.cap) throw std::logic_error("adopt_carried: workspace smaller than the batch");
+  if (N > w_.cap || n_owned > w_.rcap) throw std::logic_error("adopt_carried: workspace smaller than the batch");
   if (c_keep < 0 || c_keep > n_owned || g_keep < n_owned || g_keep > N)
     throw std::invalid_argument("adopt_carried: carried ranges outside the batch");
   const Plan p = plan(box, N, 0, n_owned, 0, n_owned, type, pos, nullptr, pe, force, virial, 0u, &map);
@@ -700,7 +717,7 @@ bool NepEngine::settle(hipStream_t s)
     if (mv > 4096) throw std::runtime_error("Verlet row overflow: more than 4096 neighbours within rc + skin");
     check(hipMemsetAsync(w_.flags + kFlagVerletOverflow, 0, sizeof(int), s), "clear flag");
     check(hipStreamSynchronize(s), "synchronise");
-    allocate(w_.cap, mv);  // drops the list: the next enqueue rebuilds
+    allocate(w_.cap, mv, w_.rcap);  // drops the list: the next enqueue rebuilds
     return true;
   }
   return false;
@@ -845,13 +862,13 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
   const int mv_other = padded_rows(mv_base_, other);
   size_t free_b = 0, total_b = 0;
   check(hipMemGetInfo(&free_b, &total_b), "memory info");
-  const double extra = (double)w_.cap * std::max(0, mv_other - w_.MV) * kRowEntryBytes;
-  if (mv_other != w_.MV && (double)w_.cap * mv_other < (double)INT32_MAX && extra < 0.5 * (double)free_b) {
+  const double extra = (double)w_.rcap * std::max(0, mv_other - w_.MV) * kRowEntryBytes;
+  if (mv_other != w_.MV && (double)w_.rcap * mv_other < (double)INT32_MAX && extra < 0.5 * (double)free_b) {
     const Tuning first = tune_;
     tune_.row_layout = other;
     bool ok = true;
     try {
-      allocate(w_.cap, mv_base_);
+      allocate(w_.cap, mv_base_, w_.rcap);
       compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags | 2u, s, map);
     } catch (const std::runtime_error&) {
       ok = false;  // no room for the other layout's rows: keep the first
@@ -862,7 +879,7 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
       const double t_other = ok ? tune_.force_ms[other] : 0.0;
       tune_ = first;
       tune_.force_ms[other] = t_other;
-      allocate(w_.cap, mv_base_);
+      allocate(w_.cap, mv_base_, w_.rcap);
     }
   }
   // the overlapped filter/descriptor schedule, where the centres split
@@ -899,7 +916,7 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
     tune_.split_ms[1] = 0.0;
     size_t free_s = 0, total_s = 0;
     check(hipMemGetInfo(&free_s, &total_s), "memory info");
-    const double need = (double)w_.cap * (w_.MV * kSplitEntryBytes + 8.0);
+    const double need = (double)w_.rcap * (w_.MV * kSplitEntryBytes + 8.0);
     if (need < 0.5 * (double)free_s) {
       const Tuning first = tune_;
       bool ok = true;
@@ -1005,7 +1022,7 @@ DevWork carve_work(char* base, int cap, int mv, long long ncell, int T, int kRp,
     w.nrc = c.take<int>(n);
     w.nac = c.take<int>(n);
   }
-  w.cap = cap;
+  w.cap = w.rcap = cap;
   w.MV = w.MR = w.MA = mv;
   w.ncell_cap = (int)ncell;
   *bytes = c.used;
@@ -1080,20 +1097,23 @@ void NepEngine::compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int
                              " (MN_radial of the model); outputs are invalid");
 }
 
-double NepEngine::bytes_per_slot() const
+double NepEngine::bytes_per_slot(bool centre) const
 {
+  // packed and cell-ordered positions, cell index, rank and entry, reference
+  const double slot = 2 * sizeof(double4) + 3 * sizeof(int) + 3 * sizeof(double);
+  if (!centre) return slot;
   int mv = w_.MV;
   if (mv <= 0)
     mv = padded_rows(std::max(8, (int)std::ceil(model_.MN_R * std::pow((model_.rcR_max + kSkin) / model_.rcR_max, 3))),
                      tune_.row_layout);
-  double per = 2 * sizeof(double4) + 6 * sizeof(int) + 3 * sizeof(double) +
-               (double)model_.num_types * ((model_.kR + 3) & ~3) * sizeof(float);
+  // row lengths, radial adjoints and the rows
+  double per = 3 * sizeof(int) + (double)model_.num_types * ((model_.kR + 3) & ~3) * sizeof(float);
   double entry = kRowEntryBytes;
   if (tune_.split) {
     per += 2 * sizeof(int);
     entry += kSplitEntryBytes;
   }
-  return per + entry * mv;
+  return slot + per + entry * mv;
 }
 
 void NepEngine::stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds)

@@ -73,7 +73,9 @@ class NepEngine {
   NepEngine& operator=(const NepEngine&) = delete;
 
   const NepModel& model() const { return model_; }
-  void reserve(int capacity);
+  // workspace for `capacity` slots, of which the first `centres` (default:
+  // all) can be centres with rows
+  void reserve(int capacity, int centres = -1);
   // sizes the cell grid arrays for at least ncell cells
   void reserve_cells(long long ncell) { ensure_cells((int)std::min<long long>(ncell, 1LL << 28)); }
   // Throws std::invalid_argument / std::runtime_error; see dmgx_nep_compute.
@@ -105,6 +107,7 @@ class NepEngine {
   void launch_force(const Plan& p, int N, int n1, int n2, int nd1, int nd2, const int* type, const uint64_t* gid,
                     double* pe, double* force, double* virial, unsigned flags, hipStream_t stream);
   DevWork& work() { return w_; }
+  const DevWork& work() const { return w_; }
   // Serving relayouts: adopts a batch of N slots (centres [0, n_owned)) whose
   // first centres [0, c_keep) had their Verlet rows, and their slots their
   // displacement references, carried over from the previous batch by the
@@ -196,10 +199,10 @@ class NepEngine {
   void compute_in(void* ws, size_t ws_bytes, const Box& box, int N, int n1, int n2, int nd1, int nd2,
                   const int* type, const double* pos, const uint64_t* gid, double* pe, double* force,
                   double* virial, unsigned flags, hipStream_t stream);
-  // Device bytes of the workspace per local slot at the current row
-  // capacity (before the first allocation: the capacity the model's
-  // MN_radial implies)
-  double bytes_per_slot() const;
+  // Device bytes of the workspace per slot that can be a centre (with its
+  // rows at the current row capacity; before the first allocation, the
+  // capacity the model's MN_radial implies), or per slot that cannot
+  double bytes_per_slot(bool centre = true) const;
   void stats(int* max_radial, int* max_angular, int* max_verlet, int* rebuilds);
   void totals(long long* sum_radial, long long* sum_angular, long long* sum_verlet, int* centres);
   void copy_verlet_rows(int c1, int c2, int max_nb, int* counts, int* rows);
@@ -212,7 +215,7 @@ class NepEngine {
                      const uint64_t* gid, double* pe, double* force, double* virial, unsigned flags,
                      hipStream_t stream);
   void ensure_cells(int ncell);
-  void allocate(int capacity, int mv);
+  void allocate(int capacity, int mv, int centres = -1);
   void allocate_split();
   void free_all();
 

@@ -1498,7 +1498,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevBox b, DevModel m, int n1,
         jj[r] = __float_as_int(rr[r].w);
       }
     }
-    const int lim = w.cap * w.MV;
+    const int lim = w.rcap * w.MV;
 #pragma unroll
     for (int r = 0; r < AB; ++r) {
       e2[r] = -1;
@@ -1635,7 +1635,7 @@ __global__ __launch_bounds__(kBlock) void k_md_first(DevBox b, int N, int T, dou
 {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (forced && i == 0) flags[kFlagRebuild] = 1;
+  if (forced && i == 0) flags[kFlagRebuild] = kRebuildAsked | kRebuildAll;
   bool moved = false, bad = false;
   if (i < N) {
     const double half = dt * 0.5;
@@ -1675,7 +1675,7 @@ __global__ __launch_bounds__(kBlock) void k_md_first(DevBox b, int N, int T, dou
     }
   }
   const bool lead = (threadIdx.x & 63) == 0;
-  if (__any(moved) && lead) atomicOr(&flags[kFlagRebuild], 1);
+  if (__any(moved) && lead) atomicOr(&flags[kFlagRebuild], kRebuildAsked);
   if (__any(bad) && lead) atomicOr(&flags[kFlagBadType], 1);
 }
 

@@ -55,7 +55,24 @@ struct ImageMap {
   const int* owner = nullptr;   // [N]
   const int* image = nullptr;   // [N]
   const int* gstart = nullptr;  // [owned + 1], absolute slots
+  // Optional request map of a batch of disjoint systems (every row holds
+  // only slots of its own system): the request of each owned slot and one
+  // "moved" word per request.  With it, a rebuild not requested for every
+  // centre remakes only the rows (and displacement references) of the
+  // requests with a slot that moved more than skin/2.
+  const int* slot_req = nullptr;  // [n_owned]
+  int* req_moved = nullptr;       // [n_req]
+  int n_owned = 0, n_req = 0;
 };
+// flags[kFlagRebuild] bits: a rebuild is requested; it covers every centre
+constexpr int kRebuildAsked = 1, kRebuildAll = 2;
+// whether slot i's rows (or reference position) are remade by the rebuild
+// whose request word is go
+__device__ __forceinline__ bool slot_rebuilt(const ImageMap& m, int go, int i)
+{
+  if (!m.slot_req || (go & kRebuildAll)) return true;
+  return m.req_moved[m.slot_req[i < m.n_owned ? i : m.owner[i]]] != 0;
+}
 constexpr int kImageMax = 15;  // |a|, |b|, |c| <= kImageMax
 __host__ __device__ inline int image_code(int a, int b, int c)
 {
@@ -67,10 +84,13 @@ __host__ __device__ inline int image_neg(int code)
   return (32 - (code & 31)) | ((32 - ((code >> 5) & 31)) << 5) | ((32 - ((code >> 10) & 31)) << 10);
 }
 
-// Per-atom / per-edge workspaces (device pointers), sized for `cap` slots.
-// Row arrays are row-major: entry p of centre i lives at [i*M + p].
+// Per-atom / per-edge workspaces (device pointers), sized for `cap` slots;
+// the per-centre arrays (rows, row lengths, radial adjoints) for the first
+// `rcap` slots, the ones that can be centres.  Row arrays are row-major:
+// entry p of centre i lives at [i*M + p].
 struct DevWork {
   int cap = 0;
+  int rcap = 0;
   int MV = 0, MR = 0, MA = 0;  // row capacities: Verlet, radial, angular
   int ncell_cap = 0;
   double4* xq = nullptr;       // [cap] per-step (x, y, z, type bits)
@@ -132,7 +152,8 @@ void upload_angular_constants(const AngularConstants& a);
 
 // Neighbour build (neighbor.hip).  launch_pack packs (x, y, z, type) for
 // the step, validates types and, unless `forced`, sets flags[kFlagRebuild]
-// when an atom moved more than skin/2; `forced` sets it unconditionally.
+// when an atom moved more than skin/2 (and, with a request map, marks the
+// atom's request); `forced` asks for every centre unconditionally.
 // launch_rebuild enqueues the whole cell list + Verlet build; every kernel of
 // it returns at once unless flags[kFlagRebuild] is set.  The request is
 // closed (and counted) by the filter kernel that follows.

@@ -25,6 +25,13 @@ constexpr int kScanOneBlock = 262144;  // larger cell grids are scanned by many 
 
 __global__ void k_set_flag(int* __restrict__ flag, int value) { *flag = value; }
 
+__global__ void k_clear_moved(const int* __restrict__ go, ImageMap m)
+{
+  if (!*go) return;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < m.n_req) m.req_moved[r] = 0;
+}
+
 __global__ void k_clear_ints(int* __restrict__ p, long long n)
 {
   for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
@@ -39,10 +46,10 @@ __global__ void k_clear_ints(int* __restrict__ p, long long n)
 __global__ __launch_bounds__(kBlock) void k_pack(DevBox b, int N, int T, const double* __restrict__ pos,
                                                  const int* __restrict__ type, const double* __restrict__ x0,
                                                  double lim2, int forced, double4* __restrict__ xq,
-                                                 int* __restrict__ flags)
+                                                 int* __restrict__ flags, ImageMap m)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (forced && i == 0) flags[kFlagRebuild] = 1;
+  if (forced && i == 0) flags[kFlagRebuild] = kRebuildAsked | kRebuildAll;
   bool moved = false, bad = false;
   if (i < N) {
     const double x = pos[i], y = pos[N + i], z = pos[2 * N + i];
@@ -53,21 +60,24 @@ __global__ __launch_bounds__(kBlock) void k_pack(DevBox b, int N, int T, const d
       double dx = x - x0[i], dy = y - x0[N + i], dz = z - x0[2 * N + i];
       mic(b, dx, dy, dz);
       moved = !(dist2(dx, dy, dz) <= lim2);  // NaN displacements request a rebuild too
+      if (moved && m.slot_req) m.req_moved[m.slot_req[i < m.n_owned ? i : m.owner[i]]] = 1;
     }
   }
   const bool lead = (threadIdx.x & 63) == 0;
-  if (__any(moved) && lead) atomicOr(&flags[kFlagRebuild], 1);
+  if (__any(moved) && lead) atomicOr(&flags[kFlagRebuild], kRebuildAsked);
   if (__any(bad) && lead) atomicOr(&flags[kFlagBadType], 1);
 }
 
 // cell of every atom, its rank inside the cell, and the positions the next
 // displacement checks compare against (for the slots in [a1, b1) and
-// [a2, b2): the slots whose rows this rebuild makes)
+// [a2, b2) whose rows this rebuild makes)
 __global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const double4* __restrict__ xq,
                              int* __restrict__ cell_of, int* __restrict__ rank, int* __restrict__ count,
-                             double* __restrict__ x0, int a1, int b1, int a2, int b2, int* __restrict__ flags)
+                             double* __restrict__ x0, int a1, int b1, int a2, int b2, int* __restrict__ flags,
+                             ImageMap m)
 {
-  if (!*go) return;
+  const int g = *go;
+  if (!g) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double4 p = xq[i];
@@ -76,7 +86,7 @@ __global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const 
   const int id = c[0] + b.nc[0] * (c[1] + b.nc[1] * c[2]);
   cell_of[i] = id;
   rank[i] = atomicAdd(&count[id], 1);
-  if ((i >= a1 && i < b1) || (i >= a2 && i < b2)) {
+  if (((i >= a1 && i < b1) || (i >= a2 && i < b2)) && slot_rebuilt(m, g, i)) {
     x0[i] = p.x;
     x0[N + i] = p.y;
     x0[2 * N + i] = p.z;
@@ -266,7 +276,8 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
                                                       int groups, const double4* __restrict__ xq, double rv2,
                                                       const uint64_t* __restrict__ gid, DevWork w)
 {
-  if (!*go) return;
+  const int go_word = *go;
+  if (!go_word) return;
   extern __shared__ uint64_t shk[];
   const int g = threadIdx.x / kRowLanes, lg = threadIdx.x % kRowLanes;
   const int gw = g % (64 / kRowLanes);  // group inside the wavefront
@@ -278,7 +289,8 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
   int* rs = reinterpret_cast<int*>(shk + (size_t)groups * per) + g * 2 * kRanges;
   int* rp = rs + kRanges;
   const int i = c1 + blockIdx.x * groups + g;
-  const bool active = i < c2;
+  const bool active = i < c2 && slot_rebuilt(w.map, go_word, i);
+  if (!__syncthreads_or(active)) return;  // no centre of this block is rebuilt
   double4 pi = make_double4(0, 0, 0, 0);
   int c[3] = {0, 0, 0};
   if (active) {
@@ -448,10 +460,11 @@ constexpr int kRevBatch = 4;
 __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ go, int c1, int c2,
                                                         const uint64_t* __restrict__ gid, DevWork w)
 {
-  if (!*go) return;
+  const int go_word = *go;
+  if (!go_word) return;
   const int i = c1 + (blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes;
   const int lg = threadIdx.x % kRowLanes;
-  if (i >= c2) return;
+  if (i >= c2 || !slot_rebuilt(w.map, go_word, i)) return;
   const int n = min(w.vn[i], w.MV);
   const int* ri = w.vl + (size_t)i * w.MV;
   const ImageMap& im = w.map;
@@ -546,7 +559,7 @@ int build_groups(int mv, bool keyed)
 
 hipError_t launch_request_rebuild(DevWork& w, hipStream_t s)
 {
-  k_set_flag<<<1, 1, 0, s>>>(w.flags + kFlagRebuild, 1);
+  k_set_flag<<<1, 1, 0, s>>>(w.flags + kFlagRebuild, kRebuildAsked | kRebuildAll);
   return hipGetLastError();
 }
 
@@ -561,9 +574,9 @@ hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const i
 {
   const double half = half_skin > 0.0 ? half_skin : 0.5 * kSkin;
   if (N > 0)
-    k_pack<<<nblk(N), kBlock, 0, s>>>(b, N, T, pos, type, w.x0, half * half, forced ? 1 : 0, w.xq, w.flags);
+    k_pack<<<nblk(N), kBlock, 0, s>>>(b, N, T, pos, type, w.x0, half * half, forced ? 1 : 0, w.xq, w.flags, w.map);
   else if (forced)
-    k_set_flag<<<1, 1, 0, s>>>(w.flags + kFlagRebuild, 1);
+    k_set_flag<<<1, 1, 0, s>>>(w.flags + kFlagRebuild, kRebuildAsked | kRebuildAll);
   return hipGetLastError();
 }
 
@@ -576,7 +589,7 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
                     x0_ranges ? x0_ranges[3] : 0};
   if (N > 0)
     k_cell_count<<<nblk(N), kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, r[0], r[1],
-                                            r[2], r[3], w.flags);
+                                            r[2], r[3], w.flags, w.map);
   if (ncell <= kScanOneBlock) {
     k_scan_cells<<<1, 1024, 0, s>>>(go, ncell, w.cell_count, w.cell_start);
   } else {
@@ -598,6 +611,8 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
     constexpr int rows_per_block = 256 / kRowLanes;
     k_verlet_reverse<<<(c2 - c1 + rows_per_block - 1) / rows_per_block, 256, 0, s>>>(go, c1, c2, gid, w);
   }
+  // the request marks of this rebuild are spent
+  if (w.map.slot_req && w.map.n_req > 0) k_clear_moved<<<nblk(w.map.n_req), kBlock, 0, s>>>(go, w.map);
   return hipGetLastError();
 }
 

@@ -59,8 +59,16 @@ void Server::DBuf<T>::need(long long n, const char* what)
 {
   n = std::max<long long>(n, 1);
   if (n <= cap && p) return;
+  // grows by half again at least: a freed block synchronises the device,
+  // and a growing batch would otherwise reallocate at every relayout
+  const long long want = std::max<long long>(n, cap + cap / 2);
   release();
-  if (hipMalloc((void**)&p, (size_t)n * sizeof(T)) != hipSuccess) {
+  if (hipMalloc((void**)&p, (size_t)want * sizeof(T)) == hipSuccess) {
+    cap = want;
+    return;
+  }
+  (void)hipGetLastError();
+  if (want == n || hipMalloc((void**)&p, (size_t)n * sizeof(T)) != hipSuccess) {
     p = nullptr;
     throw std::runtime_error(std::string("device allocation failed: ") + what);
   }
@@ -78,25 +86,41 @@ void Server::DBuf<T>::release()
 // Device buffers for request packs, reused by size class (powers of two from
 // 4 KiB): freeing device memory would synchronise the device under the
 // serving loop, so buffers are only freed with the server.
-void* Server::pack_get(size_t bytes, size_t* cap)
+// A pool miss allocates a slab of kSlabBytes (or one block, if larger) and
+// splits it into blocks of the class: allocation calls serialise with the
+// serving loop's, so they are kept few.
+constexpr size_t kSlabBytes = size_t(32) << 20;
+
+void* Server::pool_get(std::multimap<size_t, void*>& pool, std::vector<void*>& slabs, bool pinned, size_t bytes,
+                       size_t* cap)
 {
   size_t c = 4096;
   while (c < bytes) c <<= 1;
+  *cap = c;
   {
     std::lock_guard<std::mutex> lk(pack_mu_);
-    auto it = pack_free_.find(c);
-    if (it != pack_free_.end()) {
+    auto it = pool.find(c);
+    if (it != pool.end()) {
       void* p = it->second;
-      pack_free_.erase(it);
-      *cap = c;
+      pool.erase(it);
       return p;
     }
   }
+  const size_t slab = std::max(c, kSlabBytes);
   void* p = nullptr;
-  if (hipMalloc(&p, c) != hipSuccess) throw std::runtime_error("out of device memory for a request upload");
-  *cap = c;
+  if (pinned) {
+    if (hipHostMalloc(&p, slab, hipHostMallocDefault) != hipSuccess)
+      throw std::runtime_error("out of pinned host memory for a request upload");
+  } else if (hipMalloc(&p, slab) != hipSuccess) {
+    throw std::runtime_error("out of device memory for a request upload");
+  }
+  std::lock_guard<std::mutex> lk(pack_mu_);
+  slabs.push_back(p);
+  for (size_t off = c; off + c <= slab; off += c) pool.emplace(c, static_cast<char*>(p) + off);
   return p;
 }
+
+void* Server::pack_get(size_t bytes, size_t* cap) { return pool_get(pack_free_, pack_slabs_, false, bytes, cap); }
 
 void Server::pack_put(void* p, size_t cap)
 {
@@ -106,26 +130,7 @@ void Server::pack_put(void* p, size_t cap)
 }
 
 // pinned host buffers for building packs, reused the same way
-void* Server::pin_get(size_t bytes, size_t* cap)
-{
-  size_t c = 4096;
-  while (c < bytes) c <<= 1;
-  {
-    std::lock_guard<std::mutex> lk(pack_mu_);
-    auto it = pin_free_.find(c);
-    if (it != pin_free_.end()) {
-      void* p = it->second;
-      pin_free_.erase(it);
-      *cap = c;
-      return p;
-    }
-  }
-  void* p = nullptr;
-  if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess)
-    throw std::runtime_error("out of pinned host memory for a request upload");
-  *cap = c;
-  return p;
-}
+void* Server::pin_get(size_t bytes, size_t* cap) { return pool_get(pin_free_, pin_slabs_, true, bytes, cap); }
 
 struct Server::Request {
   long long id = 0;
@@ -188,6 +193,8 @@ struct Server::Lane {
   DBuf<int> vl0, vrev0, vn0;
   DBuf<double> x00;
   DBuf<int> req_flag;        // active request -> 1 once an atom drifted past the shell's budget
+  DBuf<int> req_moved;       // layout request -> 1 once an atom moved skin/2 since its rows were built
+  int n_req = 0;             // requests of the current layout
   int* h_drift = nullptr;    // pinned copy of req_flag, read after each frame
   int h_drift_cap = 0;
   DBuf<int> type2, owner2, image2, gstart2;
@@ -208,6 +215,7 @@ struct Server::Lane {
     if (bytes <= h_stage_bytes) return;
     if (h_stage) (void)hipHostFree(h_stage);
     h_stage = nullptr;
+    bytes = std::max(bytes, h_stage_bytes * 2);
     h_stage_bytes = 0;
     check(hipHostMalloc((void**)&h_stage, bytes, hipHostMallocDefault), "pinned staging");
     h_stage_bytes = bytes;
@@ -224,6 +232,13 @@ struct Server::Lane {
     m.owner = owner.p;
     m.image = image.p;
     m.gstart = gstart.p;
+    if (n_req > 0) {
+      // requests are disjoint: a rebuild remakes only the moved ones' rows
+      m.slot_req = atom_req.p;
+      m.req_moved = req_moved.p;
+      m.n_owned = n_owned;
+      m.n_req = n_req;
+    }
     return m;
   }
   void release()
@@ -232,7 +247,7 @@ struct Server::Lane {
                             &g_shift2, &out, &home, &home2, &x00})
       b->release();
     for (DBuf<int>* b : {&type, &type2, &atom_req, &req_md, &owner, &owner2, &image, &image2, &gstart, &gstart2,
-                         &req_flag, &vl0, &vrev0, &vn0})
+                         &req_flag, &req_moved, &vl0, &vrev0, &vn0})
       b->release();
     if (h_drift) (void)hipHostFree(h_drift);
     h_drift = nullptr;
@@ -246,13 +261,20 @@ struct Server::Lane {
 
 // device bytes per slot of a lane's layout: position, velocity and force
 // (3 doubles each) and mass, type, owner, image, ghost shift and home
-// position, all double-buffered; potential, virial, the atom->request index
-// and the result block of a finished atom
-constexpr double kLayoutSlotBytes = 2 * (9 * 8 + 8 + 3 * 4 + 3 * 8 + 3 * 8) + 8 + 9 * 8 + 4 + kServeOutDoubles * 8;
+// position, all double-buffered; potential, virial, the atom->request index,
+// the result block of a finished atom and the carried displacement
+// reference.  (Layout buffers grow by half again when they grow; the default
+// budget, half the free memory, leaves the other half for that slack.)
+constexpr double kLayoutSlotBytes =
+    2 * (9 * 8 + 8 + 3 * 4 + 3 * 8 + 3 * 8) + 8 + 9 * 8 + 4 + kServeOutDoubles * 8 + 3 * 8;
 
-double Server::slot_bytes(int model) const
+double Server::slot_bytes(int model, bool owned) const
 {
-  return lanes_[(size_t)model]->nep->bytes_per_slot() + kLayoutSlotBytes;
+  // owned slots are centres: their rows, plus the previous layout's rows and
+  // reverse slots copied at a relayout for the continuing requests
+  const NepEngine& e = *lanes_[(size_t)model]->nep;
+  if (!owned) return e.bytes_per_slot(false) + kLayoutSlotBytes;
+  return e.bytes_per_slot(true) + kLayoutSlotBytes + 2.0 * 4.0 * std::max(e.work().MV, 1);
 }
 
 void Server::pin_put(void* p, size_t cap)
@@ -301,9 +323,11 @@ Server::Server(const std::vector<NepEngine*>& models, const ServeConfig& cfg) : 
       // each batch grows request by request: size its engine once, up front,
       // for what the budget allows, instead of reallocating (and
       // synchronising the device) as it grows
-      const long long fit = (long long)(budget_ / slot_bytes((int)lanes_.size() - 1));
-      const long long cap = std::min({cfg_.max_atoms, cfg_.reserve_atoms, fit});
-      m->reserve((int)std::max<long long>(cap, 1));
+      // (with a ghost per owned slot and rows for the owned ones)
+      const int k = (int)lanes_.size() - 1;
+      const long long fit = (long long)(budget_ / (slot_bytes(k, true) + slot_bytes(k, false)));
+      const long long cap = std::min({cfg_.max_atoms, cfg_.reserve_atoms, 2 * fit});
+      m->reserve((int)std::max<long long>(cap, 1), (int)std::max<long long>(cap / 2, 1));
       m->reserve_cells(1LL << 23);
       check(hipStreamCreate(&l.stream), "serving stream");
     }
@@ -337,8 +361,8 @@ Server::~Server()
   waiting_.clear();
   for (auto& L : lanes_) L->active.clear();
   all_.clear();
-  for (auto& kv : pack_free_) (void)hipFree(kv.second);
-  for (auto& kv : pin_free_) (void)hipHostFree(kv.second);
+  for (void* p : pack_slabs_) (void)hipFree(p);
+  for (void* p : pin_slabs_) (void)hipHostFree(p);
   if (upload_) (void)hipStreamDestroy(upload_);
   for (auto& L : lanes_)
     if (L->stream) (void)hipStreamDestroy(L->stream);
@@ -426,7 +450,7 @@ void Server::make_pack(Request& rq, const int* type, const double* mass, const d
     extend(x);
   }
   const size_t doff = serve_pack_double_offset(N, NG), bytes = serve_pack_bytes(N, NG);
-  r->bytes = (long long)std::ceil(r->footprint() * slot_bytes(model)) + (long long)bytes;
+  r->bytes = (long long)std::ceil(N * slot_bytes(model, true) + NG * slot_bytes(model, false)) + (long long)bytes;
   if (r->footprint() > cfg_.max_atoms)
     throw std::invalid_argument("request (with its periodic shell) exceeds the serving atom budget");
   if (r->bytes > limit)
@@ -591,18 +615,30 @@ void Server::worker()
       const double t1 = now_s();
       // every lane's frame is enqueued on its own stream before any is
       // waited for, so the lanes' kernels overlap on the device: first the
-      // half-kick and the image-shell check, then (after new shells where an
-      // atom left the shell's budget) the force evaluation
+      // half-kick and the image-shell check, then the force evaluation of
+      // every lane that needs no new shells, then new shells (host work that
+      // overlaps those lanes' forces) and the other lanes' forces
       for (Lane* L : run) enqueue_kick(*L);
+      std::vector<Lane*> shells;
       for (Lane* L : run) {
-        if (!L->any_md) continue;
-        check(hipStreamSynchronize(L->stream), "serve shell check");
-        if (reshell(*L)) {
-          relayout(*L);
-          upload_md_flags(*L);
+        if (L->any_md) {
+          check(hipStreamSynchronize(L->stream), "serve shell check");
+          if (needs_shells(*L)) {
+            shells.push_back(L);
+            continue;
+          }
         }
+        enqueue_force(*L);
       }
-      for (Lane* L : run) enqueue_force(*L);
+      for (Lane* L : shells) {
+        const double ts = now_s();
+        reshell(*L);
+        relayout(*L);
+        upload_md_flags(*L);
+        enqueue_force(*L);
+        std::lock_guard<std::mutex> lk(mu_);
+        st_.reshell_s += now_s() - ts;
+      }
       for (Lane* L : run) finish_frame(*L);
       const double t2 = now_s();
       for (Lane* L : run) retire(*L);
@@ -733,7 +769,7 @@ void Server::relayout(Lane& L)
   // an uncarried relayout has none)
   DevWork& ew = L.nep->work();
   const int old_no = L.n_owned, old_nt = L.n_total, mv = ew.MV;
-  bool carry = !L.forced_rebuild && old_no > 0 && !ew.split && ew.vl && ew.cap >= old_nt;
+  bool carry = !L.forced_rebuild && old_no > 0 && !ew.split && ew.vl && ew.cap >= old_nt && ew.rcap >= old_no;
   if (carry) {
     const size_t rows = (size_t)old_no * (size_t)mv;
     L.vl0.need((long long)rows, "serve carried rows");
@@ -762,11 +798,15 @@ void Server::relayout(Lane& L)
   }
   const long long Nt = n_owned + n_ghost;
   if (Nt >= (1LL << 30)) throw std::runtime_error("serving batch too large");
-  if (Nt > L.nep->work().cap) {
-    // geometric growth (the rows of the previous layout are already copied)
+  if (Nt > L.nep->work().cap || n_owned > L.nep->work().rcap) {
+    // geometric growth (the rows of the previous layout are already copied);
+    // rows for the owned slots only
     check(hipStreamSynchronize(L.stream), "serve carry");
-    const long long lim = (long long)(INT32_MAX - 1) / std::max(L.nep->work().MV, 128);
-    L.nep->reserve((int)std::max<long long>(Nt, std::min<long long>(2LL * L.nep->work().cap, lim)));
+    const DevWork& w = L.nep->work();
+    const long long lim = (long long)(INT32_MAX - 1) / std::max(w.MV, 128);
+    const long long cap = Nt > w.cap ? std::max<long long>(Nt, 2LL * w.cap) : w.cap;
+    const long long rc = n_owned > w.rcap ? std::max<long long>(n_owned, std::min<long long>(2LL * w.rcap, lim)) : w.rcap;
+    L.nep->reserve((int)std::min<long long>(cap, INT32_MAX - 1), (int)rc);
   }
   carry = carry && L.nep->work().MV == mv;
   // the new layout's arrays (double-buffered: the previous layout is the source)
@@ -787,12 +827,14 @@ void Server::relayout(Lane& L)
   L.slots.need(R, "serve layout table");
   L.home2.need(3 * n_owned, "serve home positions");
   L.req_flag.need(R, "serve drift flags");
+  L.req_moved.need(R, "serve moved flags");
   if (R > L.h_drift_cap) {
+    const int cap = std::max(R, 2 * L.h_drift_cap);
     if (L.h_drift) (void)hipHostFree(L.h_drift);
     L.h_drift = nullptr;
     L.h_drift_cap = 0;
-    check(hipHostMalloc((void**)&L.h_drift, (size_t)R * sizeof(int), hipHostMallocDefault), "pinned drift flags");
-    L.h_drift_cap = R;
+    check(hipHostMalloc((void**)&L.h_drift, (size_t)cap * sizeof(int), hipHostMallocDefault), "pinned drift flags");
+    L.h_drift_cap = cap;
   }
   // host side: one table row per request; new requests bring their packs
   std::vector<ServeSlots> tab((size_t)R);
@@ -878,6 +920,7 @@ void Server::relayout(Lane& L)
   to.home = L.home2.p;
   check(launch_serve_assemble(R, L.slots.p, from, to, L.atom_req.p, L.stream), "serve assemble");
   check(launch_clear_ints(L.req_flag.p, R, L.stream), "serve drift flags");
+  check(launch_clear_ints(L.req_moved.p, R, L.stream), "serve moved flags");
   if (carry)
     check(launch_serve_carry_rows(r_keep, L.slots.p, old_no, old_nt, (int)n_owned, (int)Nt, (int)o_keep, (int)g_keep,
                                   L.vl0.p, L.vrev0.p, L.vn0.p, L.x00.p, L.nep->work(), L.stream),
@@ -898,6 +941,7 @@ void Server::relayout(Lane& L)
   L.n_owned = (int)n_owned;
   L.n_ghost = (int)n_ghost;
   L.n_total = (int)Nt;
+  L.n_req = R;
   std::memset(&L.batch_box, 0, sizeof(L.batch_box));
   for (int d = 0; d < 3; ++d) L.batch_box.h[4 * d] = side * slot[d];
   L.batch_box.pbc[0] = L.batch_box.pbc[1] = L.batch_box.pbc[2] = 0;
@@ -1064,18 +1108,22 @@ void Server::drop_graph(Lane& L)
 // host threads that pack new image shells
 constexpr size_t kReshellThreads = 8;
 
-bool Server::reshell(Lane& L)
+bool Server::needs_shells(Lane& L)
 {
   if (!L.any_md || L.n_owned == 0) return false;
+  std::lock_guard<std::mutex> lk(mu_);
+  for (size_t q = 0; q < L.active.size(); ++q) {
+    const Request* r = L.active[q].get();
+    if (L.h_drift[q] == 2 && r->live && r->state == 1 && r->placed) return true;
+  }
+  return false;
+}
+
+void Server::reshell(Lane& L)
+{
   std::vector<Request*> todo;
   {
     std::lock_guard<std::mutex> lk(mu_);
-    bool hard = false;
-    for (size_t q = 0; q < L.active.size(); ++q) {
-      Request* r = L.active[q].get();
-      if (L.h_drift[q] == 2 && r->live && r->state == 1 && r->placed) hard = true;
-    }
-    if (!hard) return false;
     for (size_t q = 0; q < L.active.size(); ++q) {
       Request* r = L.active[q].get();
       if (L.h_drift[q] && r->live && r->state == 1 && r->placed) todo.push_back(r);
@@ -1143,7 +1191,6 @@ bool Server::reshell(Lane& L)
   if (!failure.empty()) throw std::runtime_error("serving: new image shell: " + failure);
   std::lock_guard<std::mutex> lk(mu_);
   st_.reshells += (long long)todo.size();
-  return true;
 }
 
 void Server::retire(Lane& L)
@@ -1225,7 +1272,7 @@ void Server::retire(Lane& L)
     // empty lane drops its layout now
     if (!L.live()) {
       L.active.clear();
-      L.n_owned = L.n_ghost = L.n_total = 0;
+      L.n_owned = L.n_ghost = L.n_total = L.n_req = 0;
       drop_graph(L);
     }
     return;

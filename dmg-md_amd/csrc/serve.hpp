@@ -52,7 +52,7 @@ struct ServeConfig {
   long long max_atoms = 4000000;  // owned + ghost slots in flight, all models
   int max_requests = 4096;
   unsigned flags = 0;             // force flags (ANN mode)
-  double margin = 1.0;            // extra shell thickness (A) for an MD request's motion between new shells
+  double margin = 2.0;            // extra shell thickness (A) for an MD request's motion between new shells
   long long hbm_bytes = 0;        // device bytes of the requests in flight; 0: half the free memory at start
   long long reserve_atoms = 1LL << 21;  // engine workspace sized up front per model (capped by the budget)
   bool use_graph = true;          // replay the force launches of a layout as one hipGraph
@@ -67,6 +67,7 @@ struct ServeStats {
   long long hbm_budget = 0, max_bytes_in_flight = 0;
   long long reshells = 0;  // MD requests given a new image shell after drifting
   long long rows_carried = 0;  // relayouts whose continuing requests kept their Verlet rows
+  double reshell_s = 0;        // worker wall time building new image shells
 };
 
 // The Server owns its NepEngines' GPU work while it lives: the engines must
@@ -93,7 +94,7 @@ class Server {
   // device bytes one slot (owned atom or image ghost) of a model's batch
   // takes: engine workspace (rows of the model's capacity) and the layout's
   // double-buffered per-slot arrays
-  double slot_bytes(int model) const;
+  double slot_bytes(int model, bool owned) const;
 
  private:
   struct Request;
@@ -111,11 +112,12 @@ class Server {
   // count, extent and device bytes (device bytes above `limit` throw)
   void make_pack(Request& r, const int* type, const double* mass, const double* pos, const double* vel,
                  const double* force, long long limit);
-  // after the frame's half-kick (the lane's stream idle): when an MD request
-  // of the lane has an atom past the image shell's budget, new shells for it
-  // and the other MD requests past a quarter of the margin, from their
-  // current state; true when the lane needs a new layout for them
-  bool reshell(Lane& L);
+  // after the frame's half-kick (the lane's stream idle): whether an MD
+  // request of the lane has an atom past the image shell's budget
+  bool needs_shells(Lane& L);
+  // new shells for those requests and the other MD requests past a quarter
+  // of the margin, from their current state (a new layout places them)
+  void reshell(Lane& L);
 
   ServeConfig cfg_;
   std::vector<std::unique_ptr<Lane>> lanes_;
@@ -145,6 +147,9 @@ class Server {
   std::mutex pack_mu_;
   std::multimap<size_t, void*> pack_free_;  // reusable device pack buffers by capacity
   std::multimap<size_t, void*> pin_free_;   // reusable pinned host buffers by capacity
+  std::vector<void*> pack_slabs_, pin_slabs_;  // the allocations the buffers are cut from
+  void* pool_get(std::multimap<size_t, void*>& pool, std::vector<void*>& slabs, bool pinned, size_t bytes,
+                 size_t* cap);
   void* pack_get(size_t bytes, size_t* cap);
   void pack_put(void* p, size_t cap);
   void* pin_get(size_t bytes, size_t* cap);

@@ -322,7 +322,7 @@ class Server:
     server while it lives."""
 
     def __init__(self, nep, max_atoms: int = 4_000_000, max_requests: int = 4096, ann_bf16=False,
-                 ann_valu=False, margin: float = 1.0, graph: bool = True, hbm_bytes: int = 0, ann_bf16_split=False):
+                 ann_valu=False, margin: float = 2.0, graph: bool = True, hbm_bytes: int = 0, ann_bf16_split=False):
         neps = list(nep) if isinstance(nep, (list, tuple)) else [nep]
         cfg = _lib.ServerConfig(int(max_atoms), int(max_requests),
                                 (_lib.ANN_BF16 if ann_bf16 else 0) | (_lib.ANN_VALU if ann_valu else 0) |

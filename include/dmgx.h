@@ -360,6 +360,7 @@ typedef struct dmgx_server_stats {
                                     a quarter of the margin from where the shell was built */
   long long rows_carried;        /* relayouts whose continuing requests kept their Verlet rows
                                     (only the new and re-shelled requests' rows were built) */
+  double reshell_s;              /* worker wall time spent building new image shells */
 } dmgx_server_stats;
 
 /* Starts a server (one worker thread; one batch and one stream per model) on

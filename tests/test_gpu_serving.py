@@ -89,7 +89,7 @@ def test_drifting_md_request_gets_new_shells(dmgx):
     was built, before that step's force evaluation, and the result matches
     the standalone MD of the same system (without new shells the images it
     needs drop out after 0.5 A)."""
-    srv = dmgx.Server(dmgx.NEP(CARBON), max_atoms=400_000)
+    srv = dmgx.Server(dmgx.NEP(CARBON), max_atoms=400_000, margin=1.0)
     h, p = _diamond(4, 0.05, 31)
     rng = np.random.default_rng(32)
     v = rng.normal(0, 0.01, p.shape)
