@@ -116,9 +116,10 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
     its force error against FP32.
       capacity: the whole stream submitted at once by 8 client threads
         (closed loop, like the plain loop), frames/s from first submit to
-        last completion;
-      latency: the same stream with Poisson arrivals at rate_hz, p50/p99
-        submit-to-result;
+        last completion, on a server warmed by one untimed pass of the same
+        stream (its buffers, pools and the engines' workspaces sized);
+      latency: the same stream with Poisson arrivals at rate_hz on the same
+        server, p50/p99 submit-to-result;
       plain FP32: the same requests one at a time through NEP.compute /
         MD.run (FP32 ANN), after one untimed request per model."""
     import numpy as np
@@ -137,15 +138,15 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
         return srv.submit(r["box"], r["types"], r["mass"], r["pos"], r["vel"], steps=steps, dt_fs=r["dt_fs"],
                           model=model)
 
-    def burst(neps, bf16, count=requests, clients=8):
-        srv = dmgx.Server(neps, max_atoms=20_000_000, ann_bf16=bf16)
+    def burst(srv, count=requests, clients=8):
+        srv.reset_stats()
         with ThreadPoolExecutor(clients) as pool:
             ids = list(pool.map(lambda k: submit(srv, k), range(count)))
         srv.drain()
-        return srv, ids, srv.stats()
+        return ids, srv.stats()
 
-    def poisson(neps, bf16):
-        srv = dmgx.Server(neps, max_atoms=20_000_000, ann_bf16=bf16)
+    def poisson(srv):
+        srv.reset_stats()
         t_next = time.perf_counter()
         for k, g in enumerate(gaps):
             t_next += g
@@ -155,17 +156,24 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
         srv.drain()
         return srv.stats()
 
+    def warmed(bf16):
+        # the whole stream once, untimed: the server's layout buffers, pools
+        # and the engines' workspaces reach their sizes, and each model's
+        # autotuner picks its tile shapes on its first batch of >= 64k atoms
+        srv = dmgx.Server(neps, max_atoms=20_000_000, ann_bf16=bf16)
+        burst(srv)
+        return srv
+
     neps = [dmgx.NEP(p) for p in paths]
-    # first launches of the serving kernels; each model's autotuner picks
-    # its tile shapes on its first batch of >= 64k atoms, here in this warm-up
-    burst(neps, False, count=min(requests, 256))
-    srv, ids, st = burst(neps, False)
+    srv = warmed(False)
+    ids, st = burst(srv)
     print(f"bench: served FP32 capacity {st['frames_per_s']:.0f} frames/s", file=sys.stderr, flush=True)
-    lat = poisson(neps, False)
+    lat = poisson(srv)
     print(f"bench: served FP32 Poisson p50/p99 {lat['p50_ms']:.2f}/{lat['p99_ms']:.2f} ms", file=sys.stderr,
           flush=True)
-    srv_bf, ids_bf, st_bf = burst(neps, True)
-    lat_bf = poisson(neps, True)
+    srv_bf = warmed(True)
+    ids_bf, st_bf = burst(srv_bf)
+    lat_bf = poisson(srv_bf)
     tunes = [n.tuning() for n in neps]
     # the plain path: one request at a time, FP32; single points go through
     # NEP.compute on device arrays, NVE segments through MD.run
@@ -209,7 +217,8 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
                         "single points (energy, forces, virial) and half 50-step NVE (carbon 1 fs, Cu 2 fs)",
             "served_ann": "fp32 (matrix cores): the plain path's arithmetic, so equal force error",
             "frames_per_s": st["frames_per_s"], "atom_frames_per_s": st["atom_frames"] / max(st["elapsed_s"], 1e-9),
-            "capacity_run": "all requests submitted at once by 8 client threads",
+            "capacity_run": "all requests submitted at once by 8 client threads, on a server warmed by one "
+                            "untimed pass of the stream",
             "requests": st["completed"], "relayouts": st["relayouts"], "iterations": st["iterations"],
             "new_image_shells": st["reshells"],
             "max_batch_slots": st["max_slots"],

@@ -135,6 +135,7 @@ _SIGNATURES = {
     "dmgx_server_poll": (I, [VP, ctypes.c_longlong, I, PD, PD, PD, PD, PD, PD, PI]),
     "dmgx_server_drain": (I, [VP]),
     "dmgx_server_get_stats": (I, [VP, ctypes.POINTER(ServerStats)]),
+    "dmgx_server_reset_stats": (I, [VP]),
     "dmgx_run_directory": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
 }
 

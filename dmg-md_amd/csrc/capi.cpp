@@ -566,6 +566,14 @@ int dmgx_server_drain(dmgx_server* server)
   });
 }
 
+int dmgx_server_reset_stats(dmgx_server* server)
+{
+  return guarded([&] {
+    if (!server) throw std::invalid_argument("null handle");
+    server->server.reset_stats();
+  });
+}
+
 int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out)
 {
   return guarded([&] {

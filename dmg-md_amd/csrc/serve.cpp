@@ -170,12 +170,18 @@ struct Server::Request {
 };
 
 // One model's batch: its engine, stream, layout arrays and captured force
-// launches.  Worker thread only, except `active` membership (under mu_).
+// launches, driven by its own thread.  `active` (the layout's requests, in
+// layout order) changes only on that thread and under mu_; `incoming`,
+// `dirty` and `slots_now` are read and written under mu_.
 struct Server::Lane {
   NepEngine* nep = nullptr;
   hipStream_t stream = nullptr;
+  std::thread thread;
   std::vector<std::shared_ptr<Request>> active;
+  std::vector<std::shared_ptr<Request>> incoming;  // admitted, placed by the next layout
   bool dirty = false;  // membership changed since the last layout
+  long long seen_gen = -1;  // admit_gen_ when this lane last ran admission
+  long long slots_now = 0;  // slots of the current layout
   bool any_md = false;  // requests taking an MD step in the frame being evaluated
   int n_owned = 0, n_ghost = 0, n_total = 0;
   Box batch_box{};
@@ -339,7 +345,10 @@ Server::Server(const std::vector<NepEngine*>& models, const ServeConfig& cfg) : 
     }
     throw;
   }
-  thread_ = std::thread([this] { worker(); });
+  for (auto& L : lanes_) {
+    Lane* l = L.get();
+    l->thread = std::thread([this, l] { lane_worker(*l); });
+  }
 }
 
 Server::~Server()
@@ -349,7 +358,8 @@ Server::~Server()
     stop_ = true;
   }
   cv_.notify_all();
-  if (thread_.joinable()) thread_.join();
+  for (auto& L : lanes_)
+    if (L->thread.joinable()) L->thread.join();
   for (auto& L : lanes_) {
     if (L->stream) (void)hipStreamSynchronize(L->stream);
     drop_graph(*L);
@@ -359,7 +369,10 @@ Server::~Server()
   for (auto& kv : all_) release_upload(*kv.second);  // every request, waiting ones included
   // dropping the requests returns their result blocks to the pinned pool
   waiting_.clear();
-  for (auto& L : lanes_) L->active.clear();
+  for (auto& L : lanes_) {
+    L->active.clear();
+    L->incoming.clear();
+  }
   all_.clear();
   for (void* p : pack_slabs_) (void)hipFree(p);
   for (void* p : pin_slabs_) (void)hipHostFree(p);
@@ -526,6 +539,7 @@ long long Server::submit(int model, const Box& box, int N, const int* type, cons
   waiting_.push_back(r);
   all_[r->id] = r;
   st_.submitted++;
+  admit_gen_++;
   cv_.notify_all();
   return r->id;
 }
@@ -561,6 +575,19 @@ void Server::drain()
   if (!error_.empty()) throw std::runtime_error("serving loop failed: " + error_);
 }
 
+void Server::reset_stats()
+{
+  std::lock_guard<std::mutex> lk(mu_);
+  if (!error_.empty()) throw std::runtime_error("serving loop failed: " + error_);
+  if (st_.completed != st_.submitted) throw std::invalid_argument("statistics are reset on a drained server only");
+  const long long budget = st_.hbm_budget;
+  st_ = ServeStats{};
+  st_.hbm_budget = budget;
+  latencies_.clear();
+  t_first_ = -1;
+  t_last_ = 0;
+}
+
 ServeStats Server::stats()
 {
   std::lock_guard<std::mutex> lk(mu_);
@@ -587,90 +614,97 @@ ServeStats Server::stats()
 // worker
 // ---------------------------------------------------------------------------
 
-void Server::worker()
+void Server::lane_worker(Lane& L)
 {
   for (;;) {
+    bool relay = false;
     {
       std::unique_lock<std::mutex> lk(mu_);
-      cv_.wait(lk, [&] {
-        if (stop_ || !error_.empty()) return true;
-        if (!waiting_.empty()) return true;
-        for (auto& L : lanes_)
-          if (L->live()) return true;
-        return false;
-      });
+      cv_.wait(lk, [&] { return stop_ || !error_.empty() || L.dirty || L.live() || admit_gen_ != L.seen_gen; });
       if (stop_ || !error_.empty()) return;
+      L.seen_gen = admit_gen_;
       admit();
+      relay = L.dirty;
+      if (!relay && !L.live()) continue;
     }
     try {
       const double t0 = now_s();
-      for (auto& L : lanes_)
-        if (L->dirty) relayout(*L);
-      std::vector<Lane*> run;
-      long long slots = 0;
-      for (auto& L : lanes_) {
-        if (L->live()) run.push_back(L.get());
-        slots += L->n_total;
+      if (relay) {
+        std::shared_lock<std::shared_mutex> g(gpu_mu_);
+        relayout(L);
       }
+      if (!L.live()) continue;
+      if (needs_tuning(L)) {
+        std::unique_lock<std::shared_mutex> g(gpu_mu_);
+        tune(L);
+      }
+      // a stream capture may not overlap another thread's device-wide
+      // synchronisation (a buffer freed while a layout grows): the graph of
+      // a new layout is captured with the device to this lane
+      if (cfg_.use_graph && !L.graph_exec) {
+        std::unique_lock<std::shared_mutex> g(gpu_mu_);
+        capture_force(L, cfg_.flags & ~3u);
+      }
+      std::shared_lock<std::shared_mutex> g(gpu_mu_);
       const double t1 = now_s();
-      // every lane's frame is enqueued on its own stream before any is
-      // waited for, so the lanes' kernels overlap on the device: first the
-      // half-kick and the image-shell check, then the force evaluation of
-      // every lane that needs no new shells, then new shells (host work that
-      // overlaps those lanes' forces) and the other lanes' forces
-      for (Lane* L : run) enqueue_kick(*L);
-      std::vector<Lane*> shells;
-      for (Lane* L : run) {
-        if (L->any_md) {
-          check(hipStreamSynchronize(L->stream), "serve shell check");
-          if (needs_shells(*L)) {
-            shells.push_back(L);
-            continue;
-          }
+      // the half-kick and the image-shell check; new shells (and a new
+      // layout) before the force evaluation where an atom left a shell's
+      // budget
+      enqueue_kick(L);
+      if (L.any_md) {
+        check(hipStreamSynchronize(L.stream), "serve shell check");
+        if (needs_shells(L)) {
+          const double ts = now_s();
+          reshell(L);
+          relayout(L);
+          upload_md_flags(L);
+          std::lock_guard<std::mutex> lk(mu_);
+          st_.reshell_s += now_s() - ts;
         }
-        enqueue_force(*L);
       }
-      for (Lane* L : shells) {
-        const double ts = now_s();
-        reshell(*L);
-        relayout(*L);
-        upload_md_flags(*L);
-        enqueue_force(*L);
-        std::lock_guard<std::mutex> lk(mu_);
-        st_.reshell_s += now_s() - ts;
-      }
-      for (Lane* L : run) finish_frame(*L);
+      enqueue_force(L);
+      finish_frame(L);
       const double t2 = now_s();
-      for (Lane* L : run) retire(*L);
+      retire(L);
       const double t3 = now_s();
       std::lock_guard<std::mutex> lk(mu_);
       st_.relayout_s += t1 - t0;
       st_.iterate_s += t2 - t1;
       st_.retire_s += t3 - t2;
-      if (!run.empty()) st_.iterations++;
-      st_.max_slots = std::max<long long>(st_.max_slots, slots);
+      st_.iterations++;
+      long long slots = 0;
+      for (auto& l : lanes_) slots += l->slots_now;
+      st_.max_slots = std::max(st_.max_slots, slots);
       // a lane whose finished requests hold a quarter of its slots drops
       // them at once (a new layout of its live requests), whether or not
       // requests are waiting: every frame evaluates the whole layout
-      for (auto& L : lanes_) {
-        if (L->dirty) continue;
+      if (!L.dirty) {
         long long live = 0, dead = 0;
-        for (auto& r : L->active) (r->live ? live : dead) += r->footprint();
-        if (live > 0 && 4 * dead >= live + dead) {
-          std::vector<std::shared_ptr<Request>> keep;
-          for (auto& r : L->active)
-            if (r->live) keep.push_back(r);
-          L->active.swap(keep);
-          L->dirty = true;
-        }
+        for (auto& r : L.active) (r->live ? live : dead) += r->footprint();
+        if (live > 0 && 4 * dead >= live + dead) L.dirty = true;
       }
     } catch (const std::exception& e) {
       std::lock_guard<std::mutex> lk(mu_);
-      error_ = e.what();
+      if (error_.empty()) error_ = e.what();
+      cv_.notify_all();
       done_cv_.notify_all();
       return;
     }
   }
+}
+
+bool Server::needs_tuning(Lane& L) const { return !L.nep->tuning().tuned && L.n_owned >= kTuneMinAtoms; }
+
+void Server::tune(Lane& L)
+{
+  const unsigned flags = cfg_.flags & ~3u;
+  const ImageMap map = L.map();
+  check(launch_serve_ghosts(L.n_owned, L.n_ghost, L.n_total, L.owner.p + L.n_owned, L.g_shift.p, L.pos.p, L.stream),
+        "serve ghosts");
+  L.nep->autotune(L.batch_box, L.n_total, L.n_owned, L.type.p, L.pos.p, L.pe.p, L.force.p, L.vir.p,
+                  flags | (L.forced_rebuild ? 2u : 0u), L.stream, &map);
+  drop_graph(L);
+  L.forced_rebuild = false;
 }
 
 // Called with mu_ held.  Admits waiting requests in submission order while
@@ -689,17 +723,19 @@ void Server::admit()
   int live = 0;
   const size_t nl = lanes_.size();
   std::vector<long long> lane_live(nl, 0), lane_dead(nl, 0);
-  for (size_t l = 0; l < nl; ++l)
-    for (auto& r : lanes_[l]->active) {
-      if (r->live) {
-        live_bytes += r->bytes;
-        live_slots += r->footprint();
-        lane_live[l] += r->footprint();
-        ++live;
-      } else {
-        lane_dead[l] += r->footprint();
+  for (size_t l = 0; l < nl; ++l) {
+    for (auto* list : {&lanes_[l]->active, &lanes_[l]->incoming})
+      for (auto& r : *list) {
+        if (r->live) {
+          live_bytes += r->bytes;
+          live_slots += r->footprint();
+          lane_live[l] += r->footprint();
+          ++live;
+        } else {
+          lane_dead[l] += r->footprint();
+        }
       }
-    }
+  }
   std::vector<std::vector<size_t>> cand(nl);
   std::vector<long long> add(nl, 0);
   long long add_bytes = 0, add_slots = 0;
@@ -724,14 +760,10 @@ void Server::admit()
     // a lane already due for a new layout (its finished requests are being
     // dropped) takes whatever was admitted for it at no extra rebuild
     if (!empty && !L.dirty && 4 * (add[l] + lane_dead[l]) < lane_live[l] + lane_dead[l]) continue;
-    std::vector<std::shared_ptr<Request>> next;
-    for (auto& r : L.active)
-      if (r->live) next.push_back(r);
     for (size_t k : cand[l]) {
-      next.push_back(waiting_[k]);
+      L.incoming.push_back(waiting_[k]);
       take[k] = 1;
     }
-    L.active.swap(next);
     L.dirty = true;
     any = true;
   }
@@ -742,9 +774,11 @@ void Server::admit()
   waiting_.swap(rest);
   long long in_flight = 0;
   for (auto& L : lanes_)
-    for (auto& r : L->active)
-      if (r->live) in_flight += r->bytes;
+    for (auto* list : {&L->active, &L->incoming})
+      for (auto& r : *list)
+        if (r->live) in_flight += r->bytes;
   st_.max_bytes_in_flight = std::max(st_.max_bytes_in_flight, in_flight);
+  cv_.notify_all();  // the lanes given requests
 }
 
 void Server::relayout(Lane& L)
@@ -761,6 +795,9 @@ void Server::relayout(Lane& L)
       if (r->live && r->placed) keep.push_back(r);
     for (auto& r : L.active)
       if (r->live && !r->placed) keep.push_back(r);
+    for (auto& r : L.incoming) keep.push_back(r);
+    L.incoming.clear();
+    L.dirty = false;
     L.active.swap(keep);
     act = L.active;
   }
@@ -799,13 +836,14 @@ void Server::relayout(Lane& L)
   const long long Nt = n_owned + n_ghost;
   if (Nt >= (1LL << 30)) throw std::runtime_error("serving batch too large");
   if (Nt > L.nep->work().cap || n_owned > L.nep->work().rcap) {
-    // geometric growth (the rows of the previous layout are already copied);
-    // rows for the owned slots only
+    // geometric growth by a quarter (the rows of the previous layout are
+    // already copied); rows for the owned slots only
     check(hipStreamSynchronize(L.stream), "serve carry");
     const DevWork& w = L.nep->work();
     const long long lim = (long long)(INT32_MAX - 1) / std::max(w.MV, 128);
-    const long long cap = Nt > w.cap ? std::max<long long>(Nt, 2LL * w.cap) : w.cap;
-    const long long rc = n_owned > w.rcap ? std::max<long long>(n_owned, std::min<long long>(2LL * w.rcap, lim)) : w.rcap;
+    const long long cap = Nt > w.cap ? std::max<long long>(Nt, w.cap + w.cap / 4) : w.cap;
+    const long long rc =
+        n_owned > w.rcap ? std::max<long long>(n_owned, std::min<long long>(w.rcap + w.rcap / 4, lim)) : w.rcap;
     L.nep->reserve((int)std::min<long long>(cap, INT32_MAX - 1), (int)rc);
   }
   carry = carry && L.nep->work().MV == mv;
@@ -956,8 +994,8 @@ void Server::relayout(Lane& L)
                          L.force.p, L.vir.p, map, L.stream);
     L.forced_rebuild = false;
   }
-  L.dirty = false;
   std::lock_guard<std::mutex> lk(mu_);
+  L.slots_now = Nt;
   st_.relayouts++;
   if (!L.forced_rebuild) st_.rows_carried++;
 }
@@ -1011,20 +1049,12 @@ void Server::enqueue_force(Lane& L)
 {
   const unsigned flags = cfg_.flags & ~3u;
   const ImageMap map = L.map();
-  if (!L.nep->tuning().tuned && L.n_owned >= kTuneMinAtoms) {
-    // tile shapes picked once, on the first batch large enough to tell them apart
-    check(launch_serve_ghosts(L.n_owned, L.n_ghost, L.n_total, L.owner.p + L.n_owned, L.g_shift.p, L.pos.p, L.stream),
-          "serve ghosts");
-    L.nep->autotune(L.batch_box, L.n_total, L.n_owned, L.type.p, L.pos.p, L.pe.p, L.force.p, L.vir.p,
-                    flags | (L.forced_rebuild ? 2u : 0u), L.stream, &map);
-    drop_graph(L);
-    L.forced_rebuild = false;
-  }
-  if (cfg_.use_graph) {
-    if (!L.graph_exec) capture_force(L, flags);
+  if (L.graph_exec) {
     if (L.forced_rebuild) check(launch_request_rebuild(L.nep->work(), L.stream), "serve rebuild request");
     check(hipGraphLaunch(L.graph_exec, L.stream), "serve force graph");
   } else {
+    // no graph for this layout yet (it is captured at the start of the
+    // lane's next iteration): the same launches, enqueued directly
     check(launch_serve_ghosts(L.n_owned, L.n_ghost, L.n_total, L.owner.p + L.n_owned, L.g_shift.p, L.pos.p, L.stream),
           "serve ghosts");
     L.nep->enqueue(L.batch_box, L.n_total, 0, L.n_owned, 0, L.n_owned, L.type.p, L.pos.p, nullptr, L.pe.p, L.force.p,
@@ -1033,23 +1063,17 @@ void Server::enqueue_force(Lane& L)
 }
 
 // Waits for a lane's frame; a Verlet overflow (the engine has grown its rows)
-// redoes the force evaluation (the graph captured again); then the second
-// half-kick is enqueued.
+// redoes the force evaluation (enqueued directly; the graph is captured again
+// at the next iteration); then the second half-kick is enqueued.
 void Server::finish_frame(Lane& L)
 {
   const unsigned flags = cfg_.flags & ~3u;
   const ImageMap map = L.map();
   bool done = !L.nep->settle(L.stream);
   for (int attempt = 0; attempt < 4 && !done; ++attempt) {
-    if (cfg_.use_graph) {
-      drop_graph(L);
-      capture_force(L, flags);
-      check(launch_request_rebuild(L.nep->work(), L.stream), "serve rebuild request");
-      check(hipGraphLaunch(L.graph_exec, L.stream), "serve force graph");
-    } else {
-      L.nep->enqueue(L.batch_box, L.n_total, 0, L.n_owned, 0, L.n_owned, L.type.p, L.pos.p, nullptr, L.pe.p,
-                     L.force.p, L.vir.p, flags | 2u, L.stream, &map);
-    }
+    drop_graph(L);
+    L.nep->enqueue(L.batch_box, L.n_total, 0, L.n_owned, 0, L.n_owned, L.type.p, L.pos.p, nullptr, L.pe.p, L.force.p,
+                   L.vir.p, flags | 2u, L.stream, &map);
     done = !L.nep->settle(L.stream);
   }
   if (!done) throw std::runtime_error("serving: Verlet rows kept overflowing");
@@ -1267,12 +1291,15 @@ void Server::retire(Lane& L)
     st_.frames += frames;
     st_.atom_frames += atom_frames;
     t_last_ = t;
+    admit_gen_++;  // their budget is free
+    cv_.notify_all();
     done_cv_.notify_all();
     // a finished request leaves the layout at the lane's next relayout; an
     // empty lane drops its layout now
     if (!L.live()) {
       L.active.clear();
       L.n_owned = L.n_ghost = L.n_total = L.n_req = 0;
+      L.slots_now = 0;
       drop_graph(L);
     }
     return;

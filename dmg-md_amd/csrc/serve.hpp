@@ -40,6 +40,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -89,6 +90,9 @@ class Server {
   bool poll(long long id, bool wait, double* potential, double* force, double* position, double* velocity,
             double* virial, double* latency_ms);
   ServeStats stats();
+  // zeroes the counters, latencies and time window (a drained server only):
+  // a warmed server's buffers and pools are kept for the next measurement
+  void reset_stats();
   // blocks until every submitted request is done
   void drain();
   // device bytes one slot (owned atom or image ghost) of a model's batch
@@ -99,8 +103,14 @@ class Server {
  private:
   struct Request;
   struct Lane;
-  void worker();
+  // one thread per lane: its frames, new shells, relayouts and retirements,
+  // so that one lane's host work overlaps the others' device work
+  void lane_worker(Lane& L);
   void admit();
+  // the lane's tile shapes, once, on its first batch large enough to tell
+  // them apart, with the device to itself
+  bool needs_tuning(Lane& L) const;
+  void tune(Lane& L);
   void relayout(Lane& L);
   void upload_md_flags(Lane& L);
   void enqueue_kick(Lane& L);
@@ -121,9 +131,11 @@ class Server {
 
   ServeConfig cfg_;
   std::vector<std::unique_ptr<Lane>> lanes_;
-  std::thread thread_;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
+  // lanes' iterations hold it shared; tuning holds it exclusively
+  std::shared_mutex gpu_mu_;
+  long long admit_gen_ = 0;  // bumped when admission may make progress (a submit, a completion)
   bool stop_ = false;
   std::string error_;  // first worker failure, reported by poll/drain/stats
   long long next_id_ = 1;

@@ -376,6 +376,10 @@ class Server:
     def drain(self) -> None:
         check(lib().dmgx_server_drain(self._h))
 
+    def reset_stats(self) -> None:
+        """Zeroes the statistics of a drained server (its buffers stay)."""
+        check(lib().dmgx_server_reset_stats(self._h))
+
     def stats(self) -> dict:
         s = _lib.ServerStats()
         check(lib().dmgx_server_get_stats(self._h, ctypes.byref(s)))

@@ -390,6 +390,10 @@ int dmgx_server_poll(dmgx_server* server, long long id, int wait, double* potent
 /* Blocks until every submitted request has finished. */
 int dmgx_server_drain(dmgx_server* server);
 int dmgx_server_get_stats(dmgx_server* server, dmgx_server_stats* out);
+/* Zeroes the statistics (counters, latencies, time window) of a drained
+ * server, keeping its device buffers and pools: the next stream is measured
+ * on a warmed server.  No counterpart in the reference (serving loop). */
+int dmgx_server_reset_stats(dmgx_server* server);
 
 /* ---------------------------------------------------------------------- */
 /* Input front end (dmgmd_main.cpp + run_m1_replicated, runtime.cu)         */
