@@ -122,10 +122,15 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
         server, p50/p99 submit-to-result;
       plain FP32: the same requests one at a time through NEP.compute /
         MD.run (FP32 ANN), after one untimed request per model."""
+    import gc
     import numpy as np
     import torch
     from concurrent.futures import ThreadPoolExecutor
     W = importlib.import_module("dmg-md_amd.workloads")
+    # device memory the earlier samples left in torch's cache goes back to
+    # the device (the server's budget is half the free memory)
+    gc.collect()
+    torch.cuda.empty_cache()
     t_gen = time.perf_counter()
     stream = W.serving_stream(requests, seed)
     print(f"bench: serving stream of {requests} requests generated in {time.perf_counter() - t_gen:.1f} s",

@@ -507,6 +507,8 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
     throw std::invalid_argument("an image map needs the owned slots first and rows ordered by slot");
   if (im.slot_req && (!im.owner || !im.req_moved || im.n_owned != nd2 || im.n_req <= 0))
     throw std::invalid_argument("a request map needs an image map, its moved words and the owned slots as centres");
+  if ((im.req_range || im.req_off || im.lists) && !(im.slot_req && im.req_range && im.req_off && im.lists))
+    throw std::invalid_argument("compacted rebuild lists need a request map, its ranges, offsets and lists");
   p.box = make_devbox(box);
   for (int d = 0; d < 3; ++d)
     if (p.box.pbc[d] && p.box.thick[d] <= 2.5 * (model_.rcR_max + kSkin))
@@ -533,9 +535,12 @@ NepEngine::Plan NepEngine::plan(const Box& box, int N, int n1, int n2, int nd1, 
   p.empty = false;
   w_.map = im;
   if (w_.split) {
-    // candidate rows are remade for every centre at a rebuild
+    // candidate rows are remade for every centre at a rebuild: no request map
     w_.map.slot_req = nullptr;
     w_.map.req_moved = nullptr;
+    w_.map.req_range = nullptr;
+    w_.map.req_off = nullptr;
+    w_.map.lists = nullptr;
     w_.map.n_req = 0;
   }
   last_map_ = im;

@@ -63,7 +63,21 @@ struct ImageMap {
   const int* slot_req = nullptr;  // [n_owned]
   int* req_moved = nullptr;       // [n_req]
   int n_owned = 0, n_req = 0;
+  // Optional with the request map: each request's slots (first owned slot,
+  // owned count, first ghost slot, ghost count) and the buffers a rebuild
+  // compacts its work into, so that it visits only the requests it remakes:
+  // req_off [2 n_req + 2] (each request's offset in the centre and slot
+  // lists, then the two totals) and lists [n_owned + N] (centres, then slots)
+  const int4* req_range = nullptr;
+  int* req_off = nullptr;
+  int* lists = nullptr;
 };
+// whether a rebuild works from compacted lists (host launches and kernels
+// decide alike)
+__host__ __device__ inline bool rebuild_listed(const ImageMap& m)
+{
+  return m.slot_req && m.req_moved && m.req_range && m.req_off && m.lists && m.n_req > 0;
+}
 // flags[kFlagRebuild] bits: a rebuild is requested; it covers every centre
 constexpr int kRebuildAsked = 1, kRebuildAll = 2;
 // whether slot i's rows (or reference position) are remade by the rebuild

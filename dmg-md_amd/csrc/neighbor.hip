@@ -14,6 +14,8 @@
 // Layout: rows are row-major (vl[i*MV + p]) so that a group of lanes that
 // works on one centre reads and writes contiguous memory.  Build, sort and
 // reverse search run with kRowLanes lanes per centre (4 centres per wave).
+#include <algorithm>
+#include <climits>
 #include "device_common.hpp"
 
 namespace dmgx {
@@ -71,6 +73,9 @@ __global__ __launch_bounds__(kBlock) void k_pack(DevBox b, int N, int T, const d
 // cell of every atom, its rank inside the cell, and the positions the next
 // displacement checks compare against (for the slots in [a1, b1) and
 // [a2, b2) whose rows this rebuild makes)
+// With compacted lists (a rebuild of some requests of a batch) only the
+// listed slots enter the cell list: every row it makes holds slots of its
+// own request only.
 __global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const double4* __restrict__ xq,
                              int* __restrict__ cell_of, int* __restrict__ rank, int* __restrict__ count,
                              double* __restrict__ x0, int a1, int b1, int a2, int b2, int* __restrict__ flags,
@@ -78,18 +83,21 @@ __global__ void k_cell_count(const int* __restrict__ go, DevBox b, int N, const 
 {
   const int g = *go;
   if (!g) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const double4 p = xq[i];
-  int c[3];
-  if (!cell_coords(b, p.x, p.y, p.z, c)) atomicOr(&flags[kFlagOutOfBox], 1);
-  const int id = c[0] + b.nc[0] * (c[1] + b.nc[1] * c[2]);
-  cell_of[i] = id;
-  rank[i] = atomicAdd(&count[id], 1);
-  if (((i >= a1 && i < b1) || (i >= a2 && i < b2)) && slot_rebuilt(m, g, i)) {
-    x0[i] = p.x;
-    x0[N + i] = p.y;
-    x0[2 * N + i] = p.z;
+  const bool listed = rebuild_listed(m);
+  const int n = listed ? m.req_off[2 * m.n_req + 1] : N;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int i = listed ? m.lists[m.n_owned + k] : k;
+    const double4 p = xq[i];
+    int c[3];
+    if (!cell_coords(b, p.x, p.y, p.z, c)) atomicOr(&flags[kFlagOutOfBox], 1);
+    const int id = c[0] + b.nc[0] * (c[1] + b.nc[1] * c[2]);
+    cell_of[i] = id;
+    rank[i] = atomicAdd(&count[id], 1);
+    if (((i >= a1 && i < b1) || (i >= a2 && i < b2)) && (listed || slot_rebuilt(m, g, i))) {
+      x0[i] = p.x;
+      x0[N + i] = p.y;
+      x0[2 * N + i] = p.z;
+    }
   }
 }
 
@@ -229,14 +237,93 @@ __global__ void k_scan_offsets(const int* __restrict__ go, int ncell, const int*
 
 __global__ void k_cell_fill(const int* __restrict__ go, int N, const int* __restrict__ cell_of,
                             const int* __restrict__ rank, const int* __restrict__ start,
-                            const double4* __restrict__ xq, int* __restrict__ atoms, double4* __restrict__ xs)
+                            const double4* __restrict__ xq, int* __restrict__ atoms, double4* __restrict__ xs,
+                            ImageMap m)
 {
   if (!*go) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int at = start[cell_of[i]] + rank[i];
-  atoms[at] = i;
-  xs[at] = xq[i];
+  const bool listed = rebuild_listed(m);
+  const int n = listed ? m.req_off[2 * m.n_req + 1] : N;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int i = listed ? m.lists[m.n_owned + k] : k;
+    const int at = start[cell_of[i]] + rank[i];
+    atoms[at] = i;
+    xs[at] = xq[i];
+  }
+}
+
+// Offsets of each request's centres and slots in the compacted lists of the
+// requests this rebuild remakes (exclusive scans over the requests, one
+// workgroup), then the totals.
+__global__ __launch_bounds__(1024) void k_list_scan(const int* __restrict__ go, ImageMap m)
+{
+  const int g = *go;
+  if (!g) return;
+  __shared__ int wsum[2][16];
+  __shared__ int carry[2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+  __syncthreads();
+  for (int base = 0; base < m.n_req; base += blockDim.x) {
+    const int r = base + threadIdx.x;
+    int c = 0, s = 0;
+    if (r < m.n_req && ((g & kRebuildAll) || m.req_moved[r])) {
+      const int4 q = m.req_range[r];
+      c = q.y;
+      s = q.y + q.w;
+    }
+    int ic = c, is = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int vc = __shfl_up(ic, d), vs = __shfl_up(is, d);
+      if (lane >= d) {
+        ic += vc;
+        is += vs;
+      }
+    }
+    if (lane == 63) {
+      wsum[0][wave] = ic;
+      wsum[1][wave] = is;
+    }
+    __syncthreads();
+    int pc = carry[0], ps = carry[1];
+    for (int v = 0; v < wave; ++v) {
+      pc += wsum[0][v];
+      ps += wsum[1][v];
+    }
+    if (r < m.n_req) {
+      m.req_off[2 * r] = pc + ic - c;
+      m.req_off[2 * r + 1] = ps + is - s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int v = 0; v < nw; ++v) {
+        carry[0] += wsum[0][v];
+        carry[1] += wsum[1][v];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    m.req_off[2 * m.n_req] = carry[0];
+    m.req_off[2 * m.n_req + 1] = carry[1];
+  }
+}
+
+// The compacted lists: one workgroup per request
+__global__ void k_list_fill(const int* __restrict__ go, ImageMap m)
+{
+  const int g = *go;
+  if (!g) return;
+  const int r = blockIdx.x;
+  if (!((g & kRebuildAll) || m.req_moved[r])) return;
+  const int4 q = m.req_range[r];
+  int* cl = m.lists + m.req_off[2 * r];
+  int* sl = m.lists + m.n_owned + m.req_off[2 * r + 1];
+  for (int k = threadIdx.x; k < q.y; k += blockDim.x) {
+    cl[k] = q.x + k;
+    sl[k] = q.x + k;
+  }
+  for (int k = threadIdx.x; k < q.w; k += blockDim.x) sl[q.y + k] = q.z + k;
 }
 
 // Row key order: slot, or (global id, slot).
@@ -272,6 +359,7 @@ __host__ __device__ inline size_t build_row_lds(int mv, bool keyed)
 // with a ballot, rank-sorted there on (key, slot) and written out as one
 // contiguous row.  Rows past the capacity MV are counted but not stored;
 // the count is reported through the flags.
+template <bool LISTED>
 __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go, DevBox b, int c1, int c2,
                                                       int groups, const double4* __restrict__ xq, double rv2,
                                                       const uint64_t* __restrict__ gid, DevWork w)
@@ -288,9 +376,17 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
   int* sorted = row + w.MV;
   int* rs = reinterpret_cast<int*>(shk + (size_t)groups * per) + g * 2 * kRanges;
   int* rp = rs + kRanges;
-  const int i = c1 + blockIdx.x * groups + g;
-  const bool active = i < c2 && slot_rebuilt(w.map, go_word, i);
-  if (!__syncthreads_or(active)) return;  // no centre of this block is rebuilt
+  // centres c1 + k (one pass), or with compacted lists the listed centres
+  // (of the requests this rebuild remakes) inside [c1, c2), a pass per
+  // group of `groups` centres
+  const int ncent = LISTED ? w.map.req_off[2 * w.map.n_req] : c2 - c1;
+  for (int pass = 0; pass < (LISTED ? INT_MAX : 1); ++pass) {
+  const int base = (blockIdx.x + pass * gridDim.x) * groups;
+  if (base >= ncent) break;
+  const int kc = base + g;
+  const int i = LISTED ? (kc < ncent ? w.map.lists[kc] : -1) : c1 + kc;
+  const bool active = LISTED ? (i >= c1 && i < c2) : (i < c2 && slot_rebuilt(w.map, go_word, i));
+  if (!__syncthreads_or(active)) continue;  // no centre of this pass is rebuilt
   double4 pi = make_double4(0, 0, 0, 0);
   int c[3] = {0, 0, 0};
   if (active) {
@@ -374,7 +470,7 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
       pn[h] = w.xs[sl];
     }
   }
-  int cnt = 0;
+  int found = 0;
   for (int k0 = 0; __any(k0 < total); k0 += kStep) {
     int jc[2];
     double4 pc[2];
@@ -401,17 +497,17 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
       }
       const unsigned gm = (unsigned)(__ballot(take) >> (gw * kRowLanes)) & ((1u << kRowLanes) - 1u);
       if (take) {
-        const int at = cnt + __popc(gm & ((1u << lg) - 1u));
+        const int at = found + __popc(gm & ((1u << lg) - 1u));
         if (at < w.MV) {
           row[at] = j;
           if (gid) key[at] = gid[j];
         }
       }
-      cnt += __popc(gm);
+      found += __popc(gm);
     }
   }
   __syncthreads();
-  const int n = min(cnt, w.MV);
+  const int n = min(found, w.MV);
   if (gid) {
     for (int k = lg; k < n; k += kRowLanes) {
       const uint64_t kv = key[k];
@@ -442,9 +538,11 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
   if (active) {
     for (int k = lg; k < n; k += kRowLanes) w.vl[(size_t)i * w.MV + k] = sorted[k];
     if (lg == 0) {
-      w.vn[i] = cnt;
-      if (cnt > w.MV) atomicMax(&w.flags[kFlagVerletOverflow], cnt);
+      w.vn[i] = found;
+      if (found > w.MV) atomicMax(&w.flags[kFlagVerletOverflow], found);
     }
+  }
+  __syncthreads();  // the next pass reuses the LDS
   }
 }
 
@@ -462,9 +560,13 @@ __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ 
 {
   const int go_word = *go;
   if (!go_word) return;
-  const int i = c1 + (blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes;
+  const bool listed = rebuild_listed(w.map);
+  const int ncent = listed ? w.map.req_off[2 * w.map.n_req] : c2 - c1;
   const int lg = threadIdx.x % kRowLanes;
-  if (i >= c2 || !slot_rebuilt(w.map, go_word, i)) return;
+  for (int kc = (blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes; kc < ncent;
+       kc += gridDim.x * (blockDim.x / kRowLanes)) {
+  const int i = listed ? w.map.lists[kc] : c1 + kc;
+  if (i < c1 || i >= c2 || (!listed && !slot_rebuilt(w.map, go_word, i))) continue;
   const int n = min(w.vn[i], w.MV);
   const int* ri = w.vl + (size_t)i * w.MV;
   const ImageMap& im = w.map;
@@ -538,9 +640,12 @@ __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ 
       }
     }
   }
+  }
 }
 
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
+// workgroups of a rebuild kernel that strides over compacted lists
+constexpr int kListBlocks = 2048;
 
 // LDS of one centre of the build: its row buffers and candidate ranges
 size_t build_lds_per_centre(int mv, bool keyed) { return build_row_lds(mv, keyed) + kRangeLds; }
@@ -587,9 +692,17 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
   const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
   const int r[4] = {x0_ranges ? x0_ranges[0] : 0, x0_ranges ? x0_ranges[1] : N, x0_ranges ? x0_ranges[2] : 0,
                     x0_ranges ? x0_ranges[3] : 0};
+  // with compacted lists the work grids are capped (their kernels stride
+  // over the listed centres and slots, counted on the device)
+  const bool listed = rebuild_listed(w.map);
+  if (listed) {
+    k_list_scan<<<1, 1024, 0, s>>>(go, w.map);
+    k_list_fill<<<w.map.n_req, kBlock, 0, s>>>(go, w.map);
+  }
+  const int slot_blocks = listed ? std::min(nblk(N), kListBlocks) : nblk(N);
   if (N > 0)
-    k_cell_count<<<nblk(N), kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, r[0], r[1],
-                                            r[2], r[3], w.flags, w.map);
+    k_cell_count<<<slot_blocks, kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, r[0],
+                                                r[1], r[2], r[3], w.flags, w.map);
   if (ncell <= kScanOneBlock) {
     k_scan_cells<<<1, 1024, 0, s>>>(go, ncell, w.cell_count, w.cell_start);
   } else {
@@ -601,15 +714,22 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
     k_scan_offsets<<<nblk(ncell), kBlock, 0, s>>>(go, ncell, tile_off, w.cell_start);
   }
   if (N > 0)
-    k_cell_fill<<<nblk(N), kBlock, 0, s>>>(go, N, w.cell_of, w.cell_rank, w.cell_start, w.xq, w.cell_atoms, w.xs);
+    k_cell_fill<<<slot_blocks, kBlock, 0, s>>>(go, N, w.cell_of, w.cell_rank, w.cell_start, w.xq, w.cell_atoms, w.xs,
+                                               w.map);
   if (c2 > c1) {
     const bool keyed = gid != nullptr;
     const int groups = build_groups(w.MV, keyed);
-    const int nb = (c2 - c1 + groups - 1) / groups;
-    const size_t lds = build_lds_per_centre(w.MV, keyed) * groups;
-    k_verlet_build<<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
+    int nb = (c2 - c1 + groups - 1) / groups;
     constexpr int rows_per_block = 256 / kRowLanes;
-    k_verlet_reverse<<<(c2 - c1 + rows_per_block - 1) / rows_per_block, 256, 0, s>>>(go, c1, c2, gid, w);
+    int nr = (c2 - c1 + rows_per_block - 1) / rows_per_block;
+    if (listed) {
+      nb = std::min(nb, kListBlocks);
+      nr = std::min(nr, kListBlocks);
+    }
+    const size_t lds = build_lds_per_centre(w.MV, keyed) * groups;
+    if (listed) k_verlet_build<true><<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
+    else k_verlet_build<false><<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
+    k_verlet_reverse<<<nr, 256, 0, s>>>(go, c1, c2, gid, w);
   }
   // the request marks of this rebuild are spent
   if (w.map.slot_req && w.map.n_req > 0) k_clear_moved<<<nblk(w.map.n_req), kBlock, 0, s>>>(go, w.map);

@@ -200,6 +200,8 @@ struct Server::Lane {
   DBuf<double> x00;
   DBuf<int> req_flag;        // active request -> 1 once an atom drifted past the shell's budget
   DBuf<int> req_moved;       // layout request -> 1 once an atom moved skin/2 since its rows were built
+  DBuf<int4> req_range;      // layout request -> first owned slot, owned count, first ghost slot, ghost count
+  DBuf<int> req_off, lists;  // a rebuild's compacted centre and slot lists of the requests it remakes
   int n_req = 0;             // requests of the current layout
   int* h_drift = nullptr;    // pinned copy of req_flag, read after each frame
   int h_drift_cap = 0;
@@ -244,6 +246,9 @@ struct Server::Lane {
       m.req_moved = req_moved.p;
       m.n_owned = n_owned;
       m.n_req = n_req;
+      m.req_range = req_range.p;
+      m.req_off = req_off.p;
+      m.lists = lists.p;
     }
     return m;
   }
@@ -253,13 +258,14 @@ struct Server::Lane {
                             &g_shift2, &out, &home, &home2, &x00})
       b->release();
     for (DBuf<int>* b : {&type, &type2, &atom_req, &req_md, &owner, &owner2, &image, &image2, &gstart, &gstart2,
-                         &req_flag, &req_moved, &vl0, &vrev0, &vn0})
+                         &req_flag, &req_moved, &req_off, &lists, &vl0, &vrev0, &vn0})
       b->release();
     if (h_drift) (void)hipHostFree(h_drift);
     h_drift = nullptr;
     h_drift_cap = 0;
     slots.release();
     outq.release();
+    req_range.release();
     if (h_stage) (void)hipHostFree(h_stage);
     h_stage = nullptr;
   }
@@ -866,6 +872,9 @@ void Server::relayout(Lane& L)
   L.home2.need(3 * n_owned, "serve home positions");
   L.req_flag.need(R, "serve drift flags");
   L.req_moved.need(R, "serve moved flags");
+  L.req_range.need(R, "serve request ranges");
+  L.req_off.need(2LL * R + 2, "serve rebuild offsets");
+  L.lists.need(n_owned + Nt, "serve rebuild lists");
   if (R > L.h_drift_cap) {
     const int cap = std::max(R, 2 * L.h_drift_cap);
     if (L.h_drift) (void)hipHostFree(L.h_drift);
@@ -876,6 +885,7 @@ void Server::relayout(Lane& L)
   }
   // host side: one table row per request; new requests bring their packs
   std::vector<ServeSlots> tab((size_t)R);
+  std::vector<int4> ranges((size_t)std::max(R, 1));
   std::vector<double> h_dt((size_t)std::max(R, 1));
   std::vector<Request*> consumed;
   long long o = 0, gk = n_owned;
@@ -890,6 +900,7 @@ void Server::relayout(Lane& L)
     ServeSlots& t = tab[q];
     t.new_o = (int)o;
     t.new_g = (int)gk;
+    ranges[q] = make_int4((int)o, r.n, (int)gk, r.ng);
     t.n = r.n;
     t.ng = r.ng;
     t.pi = nullptr;
@@ -918,7 +929,7 @@ void Server::relayout(Lane& L)
     gk += r.ng;
   }
   // uploads through the lane's pinned staging buffer (its stream is idle here)
-  const size_t bytes = tab.size() * sizeof(ServeSlots) + h_dt.size() * 8 + 4 * 16;
+  const size_t bytes = tab.size() * sizeof(ServeSlots) + h_dt.size() * 8 + ranges.size() * sizeof(int4) + 4 * 16;
   L.stage(bytes);
   char* hs = L.h_stage;
   size_t off = 0;
@@ -930,6 +941,7 @@ void Server::relayout(Lane& L)
     off += n;
   };
   up(L.slots.p, tab.data(), tab.size() * sizeof(ServeSlots));
+  up(L.req_range.p, ranges.data(), (size_t)R * sizeof(int4));
   up(L.req_dt.p, h_dt.data(), h_dt.size() * 8);
   ServeLayout from, to;
   from.Nt = L.n_total;
