@@ -165,7 +165,9 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
         # the whole stream once, untimed: the server's layout buffers, pools
         # and the engines' workspaces reach their sizes, and each model's
         # autotuner picks its tile shapes on its first batch of >= 64k atoms
-        srv = dmgx.Server(neps, max_atoms=20_000_000, ann_bf16=bf16)
+        # the slot cap is set above what the budget admits: requests are
+        # admitted by their HBM bytes (half the free device memory)
+        srv = dmgx.Server(neps, max_atoms=40_000_000, ann_bf16=bf16)
         burst(srv)
         return srv
 

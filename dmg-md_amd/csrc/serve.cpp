@@ -636,22 +636,22 @@ void Server::lane_worker(Lane& L)
     try {
       const double t0 = now_s();
       if (relay) {
-        std::shared_lock<std::shared_mutex> g(gpu_mu_);
+        std::shared_lock<std::shared_mutex> g = gpu_shared();
         relayout(L);
       }
       if (!L.live()) continue;
       if (needs_tuning(L)) {
-        std::unique_lock<std::shared_mutex> g(gpu_mu_);
+        std::unique_lock<std::shared_mutex> g = gpu_exclusive();
         tune(L);
       }
       // a stream capture may not overlap another thread's device-wide
       // synchronisation (a buffer freed while a layout grows): the graph of
       // a new layout is captured with the device to this lane
       if (cfg_.use_graph && !L.graph_exec) {
-        std::unique_lock<std::shared_mutex> g(gpu_mu_);
+        std::unique_lock<std::shared_mutex> g = gpu_exclusive();
         capture_force(L, cfg_.flags & ~3u);
       }
-      std::shared_lock<std::shared_mutex> g(gpu_mu_);
+      std::shared_lock<std::shared_mutex> g = gpu_shared();
       const double t1 = now_s();
       // the half-kick and the image-shell check; new shells (and a new
       // layout) before the force evaluation where an atom left a shell's
@@ -697,6 +697,20 @@ void Server::lane_worker(Lane& L)
       return;
     }
   }
+}
+
+// Turnstile: a lane waiting for the device to itself holds gpu_turn_, so no
+// lane starts another shared stretch before it gets its turn.
+std::shared_lock<std::shared_mutex> Server::gpu_shared()
+{
+  { std::lock_guard<std::mutex> t(gpu_turn_); }
+  return std::shared_lock<std::shared_mutex>(gpu_mu_);
+}
+
+std::unique_lock<std::shared_mutex> Server::gpu_exclusive()
+{
+  std::lock_guard<std::mutex> t(gpu_turn_);
+  return std::unique_lock<std::shared_mutex>(gpu_mu_);
 }
 
 bool Server::needs_tuning(Lane& L) const { return !L.nep->tuning().tuned && L.n_owned >= kTuneMinAtoms; }

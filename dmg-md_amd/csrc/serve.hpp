@@ -133,8 +133,12 @@ class Server {
   std::vector<std::unique_ptr<Lane>> lanes_;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
-  // lanes' iterations hold it shared; tuning holds it exclusively
+  // lanes' iterations hold it shared; tuning and captures hold it
+  // exclusively, entering through the turnstile gpu_turn_
   std::shared_mutex gpu_mu_;
+  std::mutex gpu_turn_;
+  std::shared_lock<std::shared_mutex> gpu_shared();
+  std::unique_lock<std::shared_mutex> gpu_exclusive();
   long long admit_gen_ = 0;  // bumped when admission may make progress (a submit, a completion)
   bool stop_ = false;
   std::string error_;  // first worker failure, reported by poll/drain/stats
