@@ -164,9 +164,9 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
     def warmed(bf16):
         # the whole stream once, untimed: the server's layout buffers, pools
         # and the engines' workspaces reach their sizes, and each model's
-        # autotuner picks its tile shapes on its first batch of >= 64k atoms
-        # the slot cap is set above what the budget admits: requests are
-        # admitted by their HBM bytes (half the free device memory)
+        # autotuner picks its tile shapes on its first batch of >= 64k atoms.
+        # The slot cap is set above what the budget admits: requests are
+        # admitted by their HBM bytes (half the free device memory).
         srv = dmgx.Server(neps, max_atoms=40_000_000, ann_bf16=bf16)
         burst(srv)
         return srv
@@ -277,9 +277,18 @@ def small_bench(dmgx, steps=100, warmup=20):
     md.run(steps, 1.0, "nve")
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    # the same loop replaying one captured step (bit-identical results)
+    md.set_step_graph(True)
+    md.run(warmup, 1.0, "nve")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    md.run(steps, 1.0, "nve")
+    torch.cuda.synchronize()
+    el_g = time.perf_counter() - t0
     return {"workload": "carbon diamond 8^3 cells (4096 atoms, a=3.567 A, 0.05 A jitter), NEP4 n_max 4/4 basis 8/8 "
                         "l_max 4/2/1 rc 4.2/3.7 A, 30 neurons; NVE 300 K, dt 1 fs", "atoms": N, "steps": steps,
-            "atom_steps_per_s": N * steps / el, "ms_per_step": el / steps * 1e3}
+            "atom_steps_per_s": N * steps / el, "ms_per_step": el / steps * 1e3,
+            "ms_per_step_replayed_graph": el_g / steps * 1e3}
 
 
 CARBON_ZBL_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_carbon_zbl.txt")

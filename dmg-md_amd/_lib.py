@@ -128,6 +128,7 @@ _SIGNATURES = {
     "dmgx_md_box": (I, [VP, ctypes.POINTER(Box)]),
     "dmgx_md_autotune": (I, [VP, U, ctypes.POINTER(Tuning)]),
     "dmgx_md_reorder": (I, [VP, PI]),
+    "dmgx_md_set_step_graph": (I, [VP, I]),
     "dmgx_server_create": (I, [ctypes.POINTER(VP), I, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
     "dmgx_server_destroy": (None, [VP]),
     "dmgx_server_submit": (I, [VP, I, ctypes.POINTER(Box), I, PI, PD, PD, PD, I, D,

@@ -508,6 +508,14 @@ int dmgx_md_autotune(dmgx_md* md, unsigned flags, dmgx_tuning* out)
   });
 }
 
+int dmgx_md_set_step_graph(dmgx_md* md, int on)
+{
+  return guarded([&] {
+    if (!md) throw std::invalid_argument("null handle");
+    md->state.set_step_graph(on != 0);
+  });
+}
+
 int dmgx_server_create(dmgx_nep* const* models, int num_models, const dmgx_server_config* cfg,
                        dmgx_server** out)
 {

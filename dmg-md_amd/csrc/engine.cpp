@@ -12,6 +12,7 @@ namespace dmgx {
 namespace {
 
 constexpr double kCoulomb = 14.399645;  // K_C_SP, eV*A (common.cuh:57)
+constexpr int kMdChunk = 128;                // steps enqueued between two reads of the status words
 
 // covalent radii for the typewise ZBL cutoff (nep_utilities.cuh:175-186),
 // a data table that has to be identical to the reference's
@@ -600,6 +601,29 @@ void NepEngine::adopt_carried(const Box& box, int N, int n_owned, int c_keep, in
   last_N_stats_ = N;
   last_c1_ = 0;
   last_c2_ = n_owned;
+}
+
+void NepEngine::launch_key(std::vector<long long>& k) const
+{
+  const DevWork& w = w_;
+  auto p = [&](const void* q) { k.push_back((long long)(uintptr_t)q); };
+  for (long long v : {(long long)w.cap, (long long)w.rcap, (long long)w.MV, (long long)w.MR, (long long)w.MA,
+                      (long long)w.ncell_cap, (long long)w.split, (long long)w.map.n_owned, (long long)w.map.n_req})
+    k.push_back(v);
+  for (const void* q : {(const void*)w.xq, (const void*)w.cell_of, (const void*)w.cell_rank,
+                        (const void*)w.cell_count, (const void*)w.cell_start, (const void*)w.cell_atoms,
+                        (const void*)w.xs, (const void*)w.cell_tiles, (const void*)w.vn, (const void*)w.vl,
+                        (const void*)w.vrev, (const void*)w.x0, (const void*)w.nr, (const void*)w.na,
+                        (const void*)w.edge, (const void*)w.eq, (const void*)w.vsa, (const void*)w.bR,
+                        (const void*)w.f12, (const void*)w.cl, (const void*)w.cpos, (const void*)w.cq,
+                        (const void*)w.nrc, (const void*)w.nac, (const void*)w.map.owner, (const void*)w.map.image,
+                        (const void*)w.map.gstart, (const void*)w.map.slot_req, (const void*)w.map.req_moved,
+                        (const void*)w.map.req_range, (const void*)w.map.req_off, (const void*)w.map.lists,
+                        (const void*)w.flags, (const void*)w.partial, (const void*)w.thermo})
+    p(q);
+  for (int v : {tune_.filter_lanes, tune_.gather_lanes, tune_.row_layout, tune_.overlap, tune_.split,
+                tune_.desc_lanes})
+    k.push_back(v);
 }
 
 bool NepEngine::small_box(const Box& box, int reps[3]) const
@@ -1354,6 +1378,8 @@ MdState::~MdState()
   dfree(&ck_force_);
   dfree(&ck_pe_);
   dfree(&ck_vir_);
+  drop_step_graph();
+  dfree(&kstep_);
   dfree(&rand_);
   dfree(&factor_);
   if (hrand_) (void)hipHostFree(hrand_);
@@ -1418,6 +1444,66 @@ const NepEngine::Tuning& MdState::autotune(unsigned flags)
   const NepEngine::Tuning& t = nep_->autotune(box_, N_, N_, type_, pos_, pe_, force_, vir_, flags, stream_);
   have_force_ = true;
   return t;
+}
+
+void MdState::drop_step_graph()
+{
+  if (step_exec_) (void)hipGraphExecDestroy(step_exec_);
+  if (step_graph_) (void)hipGraphDestroy(step_graph_);
+  step_exec_ = nullptr;
+  step_graph_ = nullptr;
+  step_key_.clear();
+}
+
+void MdState::step_graph(const NepEngine::Plan& plan, double dt, int kind, double coupling, double volume,
+                         unsigned flags, bool thermo_on)
+{
+  std::vector<long long> key;
+  nep_->launch_key(key);
+  auto bits = [](double v) {
+    long long b;
+    std::memcpy(&b, &v, sizeof(b));
+    return b;
+  };
+  for (long long v : {(long long)N_, (long long)flags, (long long)kind, (long long)thermo_on, bits(dt), bits(coupling),
+                      bits(volume), bits(nep_->half_skin())})
+    key.push_back(v);
+  for (double v : box_.h) key.push_back(bits(v));
+  for (int v : box_.pbc) key.push_back(v);
+  for (const void* q : {(const void*)pos_, (const void*)vel_, (const void*)force_, (const void*)pe_,
+                        (const void*)vir_, (const void*)mass_, (const void*)type_, (const void*)unwrapped_,
+                        (const void*)factor_, (const void*)rand_, (const void*)kstep_, (const void*)tw_.partial,
+                        (const void*)tw_.thermo})
+    key.push_back((long long)(uintptr_t)q);
+  if (step_exec_ && key == step_key_) return;
+  drop_step_graph();
+  check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "step capture");
+  hipError_t e = launch_md_first(plan.box, N_, nep_->model().num_types, dt, mass_, pos_, vel_, force_,
+                                 kind != 0 ? factor_ : nullptr, type_, false, nep_->work(), stream_, unwrapped_,
+                                 nep_->half_skin());
+  if (e == hipSuccess) {
+    try {
+      nep_->launch_force(plan, N_, 0, N_, 0, N_, type_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    e = launch_md_second(N_, dt, mass_, vel_, force_, pe_, vir_, thermo_on, tw_, stream_);
+  }
+  if (e == hipSuccess && thermo_on)
+    e = launch_thermo_final(N_, volume, kind, coupling, 0.0, rand_, factor_, tw_, stream_, rand_ + 2 * kMdChunk,
+                            kind != 0 ? kstep_ : nullptr);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(stream_, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    check(e != hipSuccess ? e : e2, "step capture");
+  }
+  step_graph_ = g;
+  check(hipGraphInstantiate(&step_exec_, step_graph_, nullptr, nullptr, 0), "step graph instantiate");
+  step_key_ = std::move(key);
 }
 
 void MdState::enable_unwrapped()
@@ -1544,13 +1630,21 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
     coupling = ens.t_coupling > 0.1 ? std::exp(-1.0 / ens.t_coupling) : 0.0;
   }
   const bool baro = ctl.has_barostat();
-  constexpr int kChunk = 128;
-  if (!rand_) dalloc(&rand_, 2 * kChunk, "thermostat draws");
+  const bool thermo_on = kind != 0 || baro;
+  constexpr int kChunk = kMdChunk;
+  // device and pinned: the thermostat draws of a chunk, then its target
+  // temperatures (pinned: then one step's thermo)
+  if (!rand_) dalloc(&rand_, 3 * kChunk, "thermostat draws");
   if (!factor_) dalloc(&factor_, 1, "thermostat factor");
-  // pinned: the thermostat draws of a chunk, then one step's thermo
+  if (!kstep_) dalloc(&kstep_, 1, "chunk step");
   if (!hrand_)
-    check(hipHostMalloc((void**)&hrand_, (2 * kChunk + 8) * sizeof(double), hipHostMallocDefault), "pinned");
-  double* hth = hrand_ + 2 * kChunk;
+    check(hipHostMalloc((void**)&hrand_, (3 * kChunk + 8) * sizeof(double), hipHostMallocDefault), "pinned");
+  double* hth = hrand_ + 3 * kChunk;
+  double* targets = rand_ + 2 * kChunk;
+  // one captured step, replayed, when asked for (not under the per-kernel
+  // profiler, whose events are host work per launch, nor with the filter on
+  // a side stream)
+  const bool graphs = step_graph_on_ && !baro && !nep_->profiler().on() && nep_->tuning().overlap == 0;
   for (int c0 = 0; c0 < steps;) {
     // a barostat step needs the step's pressure on the host: chunks of one
     int n = baro ? 1 : std::min(kChunk, steps - c0);
@@ -1559,42 +1653,53 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
     checkpoint(true);
     bool redo = false;
     for (int attempt = 0; attempt < 4; ++attempt) {
-      if (kind == 2) {
+      if (kind != 0) {
         check(hipStreamSynchronize(stream_), "staging");  // hrand_ reuse
-        for (int k = 0; k < n; ++k) ctl.draw_bdp(&hrand_[2 * k], &hrand_[2 * k + 1]);
-        check(hipMemcpyAsync(rand_, hrand_, 2 * n * sizeof(double), hipMemcpyHostToDevice, stream_), "draws");
+        for (int k = 0; k < n; ++k) {
+          if (kind == 2) ctl.draw_bdp(&hrand_[2 * k], &hrand_[2 * k + 1]);
+          hrand_[2 * kChunk + k] = ctl.target_temperature(c0 + k, steps);
+        }
+        check(hipMemcpyAsync(rand_, hrand_, 3 * kChunk * sizeof(double), hipMemcpyHostToDevice, stream_),
+              "draws");
+        // no pending factor at the chunk's first step; its first draws
+        check(launch_chunk_start(factor_, kstep_, stream_), "chunk start");
       }
       // per step: fused first half (pending factor, kick, drift, wrap, pack,
       // displacement test), force, fused second half (+ thermo partials),
       // and the thermo reduction that also forms the next factor
       const double volume = box_volume(box_.h);
-      bool pending = false;
-      for (int k = 0; k < n; ++k) {
-        const int step = c0 + k;
+      double* const fac = kind != 0 ? factor_ : nullptr;
+      if (graphs && n > 0) {
+        const NepEngine::Plan plan =
+            nep_->plan(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u);
+        if (plan.forced) check(launch_request_rebuild(nep_->work(), stream_), "rebuild request");
+        step_graph(plan, dt, kind, coupling, volume, flags, thermo_on);
+        for (int k = 0; k < n; ++k) check(hipGraphLaunch(step_exec_, stream_), "step graph");
+      }
+      for (int k = 0; k < n && !graphs; ++k) {
         const NepEngine::Plan plan =
             nep_->plan(box_, N_, 0, N_, 0, N_, type_, pos_, nullptr, pe_, force_, vir_, flags & ~1u);
         Profiler& p = nep_->profiler();
         p.begin(kProfIntegrate, stream_);
         // the displacement test honours what box rescales have used of the skin
-        check(launch_md_first(plan.box, N_, nep_->model().num_types, dt, mass_, pos_, vel_, force_,
-                              pending ? factor_ : nullptr, type_, plan.forced, nep_->work(), stream_, unwrapped_,
-                              nep_->half_skin()),
+        check(launch_md_first(plan.box, N_, nep_->model().num_types, dt, mass_, pos_, vel_, force_, fac, type_,
+                              plan.forced, nep_->work(), stream_, unwrapped_, nep_->half_skin()),
               "velocity Verlet (first half)");
         p.end(stream_);
         nep_->launch_force(plan, N_, 0, N_, 0, N_, type_, nullptr, pe_, force_, vir_, flags & ~1u, stream_);
         p.begin(kProfIntegrate, stream_);
-        check(launch_md_second(N_, dt, mass_, vel_, force_, pe_, vir_, kind != 0 || baro, tw_, stream_),
+        check(launch_md_second(N_, dt, mass_, vel_, force_, pe_, vir_, thermo_on, tw_, stream_),
               "velocity Verlet (second half)");
         p.end(stream_);
-        if (kind != 0 || baro) {
+        if (thermo_on) {
           p.begin(kProfThermo, stream_);
-          check(launch_thermo_final(N_, volume, kind, coupling, ctl.target_temperature(step, steps), rand_ + 2 * k,
-                                    factor_, tw_, stream_),
+          check(launch_thermo_final(N_, volume, kind, coupling, 0.0, rand_, factor_, tw_, stream_, targets,
+                                    kind != 0 ? kstep_ : nullptr),
                 "thermostat");
           p.end(stream_);
-          pending = kind != 0;
         }
       }
+      const bool pending = kind != 0 && n > 0;
       if (pending) check(launch_scale_by(3 * N_, factor_, vel_, stream_), "thermostat scale");
       // a barostat step's thermo (before the thermostat rescale) comes back
       // with the status words, in the same synchronisation

@@ -108,6 +108,9 @@ class NepEngine {
                     double* pe, double* force, double* virial, unsigned flags, hipStream_t stream);
   DevWork& work() { return w_; }
   const DevWork& work() const { return w_; }
+  // What captured launches of this engine depend on: every workspace
+  // pointer and size and the tile shapes (a graph is valid while they stay)
+  void launch_key(std::vector<long long>& key) const;
   // Serving relayouts: adopts a batch of N slots (centres [0, n_owned)) whose
   // first centres [0, c_keep) had their Verlet rows, and their slots their
   // displacement references, carried over from the previous batch by the
@@ -305,6 +308,12 @@ class MdState {
   // now stored at i (composed over repeated calls).
   void reorder();
   const std::vector<int>& order() const { return order_; }
+  // the device-resident loop replays one captured step (off by default)
+  void set_step_graph(bool on)
+  {
+    step_graph_on_ = on;
+    if (!on) drop_step_graph();
+  }
   long long force_calls() const { return force_calls_; }
   void arrays(double** pos, double** vel, double** force, double** pe, double** vir, double** mass, int** type);
   const Box& box() const { return box_; }
@@ -325,8 +334,19 @@ class MdState {
   DevWork tw_{};  // thermo scratch only
   bool have_force_ = false;
   double *ck_pos_ = nullptr, *ck_vel_ = nullptr, *ck_force_ = nullptr, *ck_pe_ = nullptr, *ck_vir_ = nullptr;
-  double* rand_ = nullptr;    // per-step thermostat draws of a chunk (device)
-  double* hrand_ = nullptr;   // pinned staging of the draws
+  double* rand_ = nullptr;    // per-step thermostat draws, then target temperatures, of a chunk (device)
+  double* hrand_ = nullptr;   // pinned staging of the draws and targets
+  int* kstep_ = nullptr;      // device: the step of the chunk the thermostat launch takes
+  // small systems: one step's launches captured and replayed per step
+  bool step_graph_on_ = false;
+  hipGraph_t step_graph_ = nullptr;
+  hipGraphExec_t step_exec_ = nullptr;
+  std::vector<long long> step_key_;
+  void drop_step_graph();
+  // captures (or keeps, while nothing it depends on changed) one step of the
+  // device-resident loop: first half, force, second half, thermostat
+  void step_graph(const NepEngine::Plan& plan, double dt, int kind, double coupling, double volume, unsigned flags,
+                  bool thermo_on);
   double* factor_ = nullptr;  // device velocity factor
   long long force_calls_ = 0;
   unsigned long long* vmax2_ = nullptr;  // device: bits of the largest |v|^2 (adaptive step)

@@ -1836,12 +1836,17 @@ __global__ __launch_bounds__(kThermoThreads) void k_md_second(int N, double dt, 
 }
 
 // normalize_global_thermo (runtime_internal.hpp:704-716): T = sum m v^2/(3 N kB), stress/V
+// With step_index, the step's target temperature and draws are
+// targets[k] and rand2[2k .. 2k+1] for k = *step_index, which the kernel then
+// advances: the same launch serves every step of a chunk (a replayed graph).
 __global__ __launch_bounds__(kThermoThreads) void k_thermo_final(int nb, int N, double volume,
                                                                  const double* __restrict__ partial,
                                                                  double* __restrict__ thermo, int kind,
                                                                  double coupling, double t_target,
                                                                  const double* __restrict__ rand2,
-                                                                 double* __restrict__ factor)
+                                                                 double* __restrict__ factor,
+                                                                 const double* __restrict__ targets,
+                                                                 int* __restrict__ step_index)
 {
   KSum acc[8];
 #pragma unroll
@@ -1851,6 +1856,12 @@ __global__ __launch_bounds__(kThermoThreads) void k_thermo_final(int nb, int N, 
     for (int q = 0; q < 8; ++q) ksum_merge(acc[q], KSum{partial[(b * 8 + q) * 2], partial[(b * 8 + q) * 2 + 1]});
   block_reduce8(acc);
   if (threadIdx.x == 0) {
+    if (step_index) {
+      const int k = *step_index;
+      t_target = targets[k];
+      rand2 += 2 * k;
+      *step_index = k + 1;
+    }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       double v = acc[q].s + acc[q].c;
@@ -1860,6 +1871,12 @@ __global__ __launch_bounds__(kThermoThreads) void k_thermo_final(int nb, int N, 
       if (q == 0 && factor) factor[0] = ensemble_factor(kind, N, coupling, t_target, rand2, v);
     }
   }
+}
+
+__global__ void k_chunk_start(double* __restrict__ factor, int* __restrict__ step_index)
+{
+  factor[0] = 1.0;
+  *step_index = 0;
 }
 
 __global__ void k_ensemble_factor(int kind, int N, double coupling, double t_target,
@@ -2159,7 +2176,8 @@ hipError_t launch_thermo(int N, double volume, const double* mass, const double*
 {
   const int nb = thermo_blocks(N);
   k_thermo_partial<<<nb, kThermoThreads, 0, s>>>(N, mass, pe, vel, virial, w.partial);
-  k_thermo_final<<<1, kThermoThreads, 0, s>>>(nb, N, volume, w.partial, w.thermo, 0, 0.0, 0.0, nullptr, nullptr);
+  k_thermo_final<<<1, kThermoThreads, 0, s>>>(nb, N, volume, w.partial, w.thermo, 0, 0.0, 0.0, nullptr, nullptr,
+                                              nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -2201,10 +2219,17 @@ hipError_t launch_md_second(int N, double dt, const double* mass, double* vel, c
 }
 
 hipError_t launch_thermo_final(int N, double volume, int kind, double coupling, double t_target,
-                               const double* rand2, double* factor, DevWork& tw, hipStream_t s)
+                               const double* rand2, double* factor, DevWork& tw, hipStream_t s,
+                               const double* targets, int* step_index)
 {
   k_thermo_final<<<1, kThermoThreads, 0, s>>>(thermo_blocks(N), N, volume, tw.partial, tw.thermo, kind, coupling,
-                                                t_target, rand2, kind ? factor : nullptr);
+                                                t_target, rand2, kind ? factor : nullptr, targets, step_index);
+  return hipGetLastError();
+}
+
+hipError_t launch_chunk_start(double* factor, int* step_index, hipStream_t s)
+{
+  k_chunk_start<<<1, 1, 0, s>>>(factor, step_index);
   return hipGetLastError();
 }
 

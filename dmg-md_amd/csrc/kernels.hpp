@@ -102,6 +102,8 @@ __host__ __device__ inline int image_neg(int code)
 // the per-centre arrays (rows, row lengths, radial adjoints) for the first
 // `rcap` slots, the ones that can be centres.  Row arrays are row-major:
 // entry p of centre i lives at [i*M + p].
+// (Every field is part of NepEngine::launch_key: a captured launch takes
+// the struct by value.)
 struct DevWork {
   int cap = 0;
   int rcap = 0;
@@ -248,8 +250,14 @@ hipError_t launch_md_first(const DevBox& b, int N, int T, double dt, const doubl
                            hipStream_t s, double* unwrapped = nullptr, double half_skin = -1.0);
 hipError_t launch_md_second(int N, double dt, const double* mass, double* vel, const double* force, const double* pe,
                             const double* virial, bool thermo, DevWork& tw, hipStream_t s);
+// With step_index (device), the step's target and draws are targets[k] and
+// rand2[2k..] for k = *step_index, advanced by the kernel (t_target unused).
 hipError_t launch_thermo_final(int N, double volume, int kind, double coupling, double t_target,
-                               const double* rand2, double* factor, DevWork& tw, hipStream_t s);
+                               const double* rand2, double* factor, DevWork& tw, hipStream_t s,
+                               const double* targets = nullptr, int* step_index = nullptr);
+// factor[0] = 1 and *step_index = 0: a chunk's first step applies no
+// pending factor and takes the chunk's first target and draws
+hipError_t launch_chunk_start(double* factor, int* step_index, hipStream_t s);
 hipError_t launch_scale_by(int n, const double* factor, double* data, hipStream_t s);
 
 // Small boxes: replicate N atoms into reps[0]*reps[1]*reps[2] periodic copies

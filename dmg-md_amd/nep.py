@@ -253,6 +253,10 @@ class MD:
         self.N = N
         self.order = None  # stored index -> construction index, once reordered
 
+    def set_step_graph(self, on: bool) -> None:
+        """Replays one captured step per step (``dmgx_md_set_step_graph``)."""
+        check(lib().dmgx_md_set_step_graph(self._h, 1 if on else 0))
+
     def reorder(self) -> np.ndarray:
         """Stores the atoms in cell order (``dmgx_md_reorder``); returns, for
         each stored atom, its index in the arrays given at construction."""

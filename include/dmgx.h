@@ -321,6 +321,13 @@ int dmgx_md_reorder(dmgx_md* md, int* order_host);
 /* dmgx_nep_autotune on the state's current positions (wrapped into the box
  * first); the state's forces are those positions' forces afterwards. */
 int dmgx_md_autotune(dmgx_md* md, unsigned flags, dmgx_tuning* out);
+/* on != 0: the device-resident MD loop (NVE and the NVT thermostats; not a
+ * barostat, not under the per-kernel profiler) replays one captured step,
+ * a hipGraph of the step's launches, instead of launching each kernel;
+ * bit-identical results.  Off by default: on a 4,096-atom system the step
+ * is bound by its kernels, not their launches, and the replay measured
+ * slower.  No counterpart in the reference (a per-kernel launch loop). */
+int dmgx_md_set_step_graph(dmgx_md* md, int on);
 
 /* ---------------------------------------------------------------------- */
 /* Serving loop: continuous batching of independent periodic systems.      */
