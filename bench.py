@@ -449,14 +449,18 @@ def water_bench(dmgx, steps=100):
         md = dmgx.MD(nep, s["box"], s["types"], s["mass"], s["pos"], s["vel"])
         if reorder:
             md.reorder()
-        md.autotune()
+        tune = md.autotune()
         md.run(20, 0.5, ens, t_init=300.0, t_final=300.0, **kw)
         torch.cuda.synchronize()
+        r0 = nep.neighbor_stats()["rebuilds"]
         t0 = time.perf_counter()
         md.run(steps, 0.5, ens, t_init=300.0, t_final=300.0, **kw)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        out[name] = {"atom_steps_per_s": N * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps}
+        out[name] = {"atom_steps_per_s": N * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps,
+                     "verlet_rebuilds": nep.neighbor_stats()["rebuilds"] - r0,
+                     "tile_shapes": {k: tune[k] for k in ("filter_lanes", "gather_lanes", "row_layout", "overlap",
+                                                           "split", "desc_lanes")}}
         if ens == "npt_scr":
             # averages over a further stretch, read every 5 steps
             vols, press = [], []

@@ -863,6 +863,12 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
     std::sort(ms, ms + kReps);
     return (double)ms[kReps / 2];
   };
+  // the device busy before the first candidate is timed (its clocks up, and
+  // freshly allocated rows touched)
+  auto warm = [&] {
+    for (int k = 0; k < 10; ++k) launch_step_kernels(b, N, nc, type, pe, force, virial, flags, s);
+  };
+  warm();
   // lanes of the filter and gather in the current row layout, then the
   // whole force sequence with them
   auto tune_lanes = [&](int layout) {
@@ -903,12 +909,17 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
       ok = false;  // no room for the other layout's rows: keep the first
       (void)hipGetLastError();
     }
-    if (ok) tune_lanes(other);
+    if (ok) {
+      warm();
+      tune_lanes(other);
+    }
     if (!ok || !(tune_.force_ms[other] < first.force_ms[layout0])) {
       const double t_other = ok ? tune_.force_ms[other] : 0.0;
       tune_ = first;
       tune_.force_ms[other] = t_other;
       allocate(w_.cap, mv_base_, w_.rcap);
+      // fresh rows: the remaining candidates are timed on this layout's rows
+      compute(box, N, 0, nc, 0, nc, type, pos, nullptr, pe, force, virial, flags | 2u, s, map);
     }
   }
   // the overlapped filter/descriptor schedule, where the centres split
@@ -1654,7 +1665,8 @@ void MdState::run(int steps, double dt_fs, const EnsembleConfig& ens, unsigned f
     bool redo = false;
     for (int attempt = 0; attempt < 4; ++attempt) {
       if (kind != 0) {
-        check(hipStreamSynchronize(stream_), "staging");  // hrand_ reuse
+        // the staging buffer is free: every chunk (and every attempt) ends
+        // with the status read's synchronisation, after its upload
         for (int k = 0; k < n; ++k) {
           if (kind == 2) ctl.draw_bdp(&hrand_[2 * k], &hrand_[2 * k + 1]);
           hrand_[2 * kChunk + k] = ctl.target_temperature(c0 + k, steps);
