@@ -106,7 +106,7 @@ def cpu_baseline(budget_s=10.0):
 CARBON_MODEL = os.path.join(ROOT, "tests", "golden", "potentials", "nep_carbon.txt")
 
 
-def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
+def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7, capacity_only=False):
     """The serving configuration: `requests` carbon and Cu requests (each
     with its own model: the 4096-atom carbon and the 256k Cu configurations'
     NEP4 models, one server), 64..20000 sites log-uniform, 30% vacancies on a
@@ -175,6 +175,9 @@ def serve_bench(dmgx, requests=4096, rate_hz=1000.0, seed=7):
     srv = warmed(False)
     ids, st = burst(srv)
     print(f"bench: served FP32 capacity {st['frames_per_s']:.0f} frames/s", file=sys.stderr, flush=True)
+    if capacity_only:  # the warmed capacity pass alone (for kernel traces)
+        return {"frames_per_s": st["frames_per_s"], "requests": st["completed"], "relayouts": st["relayouts"],
+                "new_image_shells": st["reshells"], "max_batch_slots": st["max_slots"]}
     lat = poisson(srv)
     print(f"bench: served FP32 Poisson p50/p99 {lat['p50_ms']:.2f}/{lat['p99_ms']:.2f} ms", file=sys.stderr,
           flush=True)
@@ -493,6 +496,8 @@ def main():
     ap.add_argument("--amortised-steps", type=int, default=300,
                     help="steps of the stretch timed with its Verlet rebuilds")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving sample (its JSON object)")
+    ap.add_argument("--serve-capacity-only", action="store_true",
+                    help="with --serve-only: only the warm-up and the timed FP32 capacity pass")
     ap.add_argument("--water-only", type=int, default=0, metavar="STEPS",
                     help="run only the liquid-water sample with STEPS timed steps per run and print its "
                          "JSON object (for kernel profiles of the 2-type path)")
@@ -529,7 +534,7 @@ def main():
         print(json.dumps(water_bench(dmgx, steps=args.water_only)))
         return
     if args.serve_only:
-        print(json.dumps(serve_bench(dmgx)))
+        print(json.dumps(serve_bench(dmgx, capacity_only=args.serve_capacity_only)))
         return
     W = importlib.import_module("dmg-md_amd.workloads")
     s = W.copper_fcc(n=args.cells, seed=1)
