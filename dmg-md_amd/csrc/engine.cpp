@@ -913,7 +913,10 @@ const NepEngine::Tuning& NepEngine::autotune(const Box& box, int N, int nc, cons
       warm();
       tune_lanes(other);
     }
-    if (!ok || !(tune_.force_ms[other] < first.force_ms[layout0])) {
+    // wider rows cost memory and every rebuild more, which the force
+    // kernels' timing does not see: they must win by 3% to be kept
+    const double bar = mv_other > first.row_capacity ? 0.97 : 1.0;
+    if (!ok || !(tune_.force_ms[other] < bar * first.force_ms[layout0])) {
       const double t_other = ok ? tune_.force_ms[other] : 0.0;
       tune_ = first;
       tune_.force_ms[other] = t_other;
