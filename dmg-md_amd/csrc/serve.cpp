@@ -648,8 +648,13 @@ void Server::lane_worker(Lane& L)
       // synchronisation (a buffer freed while a layout grows): the graph of
       // a new layout is captured with the device to this lane
       if (cfg_.use_graph && !L.graph_exec) {
-        std::unique_lock<std::shared_mutex> g = gpu_exclusive();
-        capture_force(L, cfg_.flags & ~3u);
+        drop_graph(L);
+        {
+          std::unique_lock<std::shared_mutex> g = gpu_exclusive();
+          capture_force(L, cfg_.flags & ~3u);
+        }
+        // the instantiation touches no stream: the other lanes go on meanwhile
+        check(hipGraphInstantiate(&L.graph_exec, L.graph, nullptr, nullptr, 0), "serve graph instantiate");
       }
       std::shared_lock<std::shared_mutex> g = gpu_shared();
       const double t1 = now_s();
@@ -1114,7 +1119,8 @@ void Server::finish_frame(Lane& L)
 // workspace sizing) is made first, outside the capture; the pack never forces
 // a rebuild itself (a forced rebuild is requested on the device before a
 // replay).  The captured kernels take the layout's arrays, so the graph lives
-// until the next relayout or workspace reallocation.
+// until the next relayout or workspace reallocation.  Records L.graph; the
+// caller instantiates it.
 void Server::capture_force(Lane& L, unsigned flags)
 {
   const ImageMap map = L.map();
@@ -1144,7 +1150,6 @@ void Server::capture_force(Lane& L, unsigned flags)
     check(e != hipSuccess ? e : e2, "serve capture");
   }
   L.graph = g;
-  check(hipGraphInstantiate(&L.graph_exec, L.graph, nullptr, nullptr, 0), "serve graph instantiate");
 }
 
 void Server::drop_graph(Lane& L)
