@@ -646,8 +646,11 @@ void Server::lane_worker(Lane& L)
       }
       // a stream capture may not overlap another thread's device-wide
       // synchronisation (a buffer freed while a layout grows): the graph of
-      // a new layout is captured with the device to this lane
-      if (cfg_.use_graph && !L.graph_exec) {
+      // a new layout is captured with the device to this lane.  With several
+      // lanes those stalls cost more than the launches a replay saves
+      // (measured: 82-84 k frames/s launched directly against 77-80 k with
+      // graphs on the two-model stream), so only a single lane captures.
+      if (cfg_.use_graph && lanes_.size() == 1 && !L.graph_exec) {
         drop_graph(L);
         {
           std::unique_lock<std::shared_mutex> g = gpu_exclusive();

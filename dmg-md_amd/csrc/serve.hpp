@@ -56,7 +56,7 @@ struct ServeConfig {
   double margin = 2.0;            // extra shell thickness (A) for an MD request's motion between new shells
   long long hbm_bytes = 0;        // device bytes of the requests in flight; 0: half the free memory at start
   long long reserve_atoms = 1LL << 21;  // engine workspace sized up front per model (capped by the budget)
-  bool use_graph = true;          // replay the force launches of a layout as one hipGraph
+  bool use_graph = true;          // replay the force launches of a layout as one hipGraph (one model only)
 };
 
 struct ServeStats {

@@ -343,7 +343,9 @@ typedef struct dmgx_server_config {
   double margin;         /* extra shell thickness (A) for an MD request's motion: a request
                             whose atoms move margin / 2 gets a new image shell */
   int use_graph;         /* nonzero: the force launches of a batch layout are captured once
-                            into a hipGraph and replayed every iteration */
+                            into a hipGraph and replayed every iteration (a server of one
+                            model; with several, frames are launched directly: a capture
+                            needs the device to itself, which stalls the other models) */
   long long hbm_bytes;   /* admission budget: device bytes of the requests in flight, each
                             request counting (atoms + image shell) x its model's bytes per
                             slot (engine workspace at the model's row capacity and the batch
