@@ -340,6 +340,7 @@ void NepEngine::free_all()
   dfree(&w_.edge);
   dfree(&w_.eq);
   dfree(&w_.vsa);
+  dfree(&w_.amask);
   dfree(&w_.bR);
   dfree(&w_.f12);
   dfree(&w_.cl);
@@ -368,10 +369,11 @@ int NepEngine::padded_rows(int mv, int layout) const
 }
 
 // device bytes per row entry: Verlet slot, reverse index, edge record,
-// edge reverse, Verlet->angular index, angular partial force; and with
-// candidate rows the candidate slot, its angular slot and reverse index
-constexpr double kRowEntryBytes = 4 + 4 + 16 + 4 + 4 + 16;
-constexpr double kSplitEntryBytes = 4 + 4 + 4;
+// edge reverse, angular partial force and an angular bit; and with
+// candidate rows the candidate slot, its angular slot, its position and
+// reverse index
+constexpr double kRowEntryBytes = 4 + 4 + 16 + 4 + 16 + 4.0 / 32.0;
+constexpr double kSplitEntryBytes = 4 + 4 + 4 + 4;
 
 void NepEngine::allocate(int capacity, int mv, int centres)
 {
@@ -396,7 +398,8 @@ void NepEngine::allocate(int capacity, int mv, int centres)
   dalloc(&w_.na, rc, "na");
   dalloc(&w_.edge, rows, "edges");
   dalloc(&w_.eq, rows, "edge reverse");
-  dalloc(&w_.vsa, rows, "verlet->angular");
+  dalloc(&w_.amask, rc * (size_t)((mv + 31) / 32), "angular bits");
+  dfree(&w_.vsa);  // candidate rows only
   dalloc(&w_.bR, rc * model_.num_types * (size_t)((model_.kR + 3) & ~3), "radial adjoint");
   dalloc(&w_.f12, rows, "angular partials");
   w_.cap = (int)cap;
@@ -413,6 +416,7 @@ void NepEngine::allocate_split()
 {
   const size_t cap = (size_t)std::max(w_.rcap, 1), rows = cap * (size_t)w_.MV;
   dalloc(&w_.cl, rows, "candidate rows");
+  dalloc(&w_.vsa, rows, "candidate->angular");
   dalloc(&w_.cpos, rows, "verlet->angular candidate");
   dalloc(&w_.cq, rows, "candidate reverse");
   dalloc(&w_.nrc, cap, "candidates");
@@ -436,6 +440,7 @@ void NepEngine::set_split(int split)
     if (w_.cap > 0 && w_.MV > 0) allocate_split();
   } else {
     w_.split = 0;
+    dfree(&w_.vsa);
     dfree(&w_.cl);
     dfree(&w_.cpos);
     dfree(&w_.cq);
@@ -614,7 +619,8 @@ void NepEngine::launch_key(std::vector<long long>& k) const
                         (const void*)w.cell_count, (const void*)w.cell_start, (const void*)w.cell_atoms,
                         (const void*)w.xs, (const void*)w.cell_tiles, (const void*)w.vn, (const void*)w.vl,
                         (const void*)w.vrev, (const void*)w.x0, (const void*)w.nr, (const void*)w.na,
-                        (const void*)w.edge, (const void*)w.eq, (const void*)w.vsa, (const void*)w.bR,
+                        (const void*)w.edge, (const void*)w.eq, (const void*)w.vsa, (const void*)w.amask,
+                        (const void*)w.bR,
                         (const void*)w.f12, (const void*)w.cl, (const void*)w.cpos, (const void*)w.cq,
                         (const void*)w.nrc, (const void*)w.nac, (const void*)w.map.owner, (const void*)w.map.image,
                         (const void*)w.map.gstart, (const void*)w.map.slot_req, (const void*)w.map.req_moved,
@@ -1054,12 +1060,13 @@ DevWork carve_work(char* base, int cap, int mv, long long ncell, int T, int kRp,
   w.vl = c.take<int>(rows);
   w.vrev = c.take<int>(rows);
   w.eq = c.take<int>(rows);
-  w.vsa = c.take<int>(rows);
+  w.amask = c.take<unsigned>(n * (size_t)((mv + 31) / 32));
   w.edge = c.take<float4>(rows);
   w.f12 = c.take<float4>(rows);
   if (split) {
     w.split = 1;
     w.cl = c.take<int>(rows);
+    w.vsa = c.take<int>(rows);
     w.cpos = c.take<int>(rows);
     w.cq = c.take<int>(rows);
     w.nrc = c.take<int>(n);

@@ -281,6 +281,12 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
   const size_t vrow = (size_t)i * w.MV, erow = (size_t)i * w.MR, arow = (size_t)i * w.MA;
   const unsigned below = (1u << lg) - 1u;
   int na = 0, nro = 0;
+  // the row's angular bits, assembled a 32-bit word at a time by the
+  // group's first lane (a chunk of G consecutive entries never straddles
+  // a word) and stored once the word is complete
+  unsigned* const mrow = CAND ? nullptr : w.amask + (size_t)i * ((w.MV + 31) >> 5);
+  unsigned mword = 0u;
+  int mcur = 0;
   for (int p0 = 0; __any(p0 < nv); p0 += G * RB) {
     int jj[RB], qq[RB], jo[RB];
     double4 pj[RB];
@@ -339,7 +345,15 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
           if (slot >= 0) w.cl[vrow + slot] = jj[r];
         }
       } else {
-        if (jj[r] >= 0) w.vsa[vrow + p] = (isA && a < w.MA) ? a : -1;
+        const int base = p0 + r * G;
+        if (lg == 0 && base < nv) {
+          if ((base >> 5) != mcur) {
+            mrow[mcur] = mword;
+            mword = 0u;
+            mcur = base >> 5;
+          }
+          mword |= gA << (base & 31);
+        }
         if (isA && a < w.MA) {
           w.edge[erow + a] = rec;
           w.eq[arow + a] = qq[r] >= 0 ? jo[r] * w.MV + qq[r] : -1;
@@ -354,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(DevBox b, DevModel m, int c1,
     }
   }
   // na + nro <= nv <= MV = MR = MA: the two ends of a row never meet
+  if (!CAND && active && lg == 0 && nv > 0) mrow[mcur] = mword;
   if (active && lg == 0) {
     if (CAND) {
       w.nrc[i] = min(na + nro, w.MV);
@@ -1502,7 +1517,22 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevBox b, DevModel m, int n1,
 #pragma unroll
     for (int r = 0; r < AB; ++r) {
       e2[r] = -1;
-      if (jj[r] >= nd1 && jj[r] < nd2 && qv[r] >= 0 && qv[r] < lim) e2[r] = w.vsa[qv[r]];
+      if (jj[r] >= nd1 && jj[r] < nd2 && qv[r] >= 0 && qv[r] < lim) {
+        if (CAND) {
+          e2[r] = w.vsa[qv[r]];
+        } else {
+          // i's Verlet position q in j's row (qv = j MV + q); its angular
+          // slot is the number of angular entries of j's row before it
+          const int q = qv[r] - jj[r] * w.MV;
+          const unsigned* mk = w.amask + (size_t)jj[r] * ((w.MV + 31) >> 5);
+          const unsigned word = mk[q >> 5];
+          if ((word >> (q & 31)) & 1u) {
+            int c = __popc(word & ((1u << (q & 31)) - 1u));
+            for (int k = 0; k < (q >> 5); ++k) c += __popc(mk[k]);
+            e2[r] = c;
+          }
+        }
+      }
     }
 #pragma unroll
     for (int r = 0; r < AB; ++r) {

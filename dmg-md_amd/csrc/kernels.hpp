@@ -126,8 +126,11 @@ struct DevWork {
   float4* edge = nullptr;      // [cap*MR] (rx, ry, rz, bits(j)): angular edges at
                                // 0..na-1, radial-only edges at MR-1 downward
   int* eq = nullptr;           // [cap*MA] of an angular edge i->j: j*MV + (position of i in
-                               // j's Verlet row), an index into vsa; -1 if j has no row
-  int* vsa = nullptr;          // [cap*MV] Verlet position -> angular slot, -1 if not angular
+                               // j's Verlet row), whose angular bit gives i's slot in j's
+                               // angular row; -1 if j has no row
+  int* vsa = nullptr;          // [cap*MV] with candidate rows only: candidate slot -> angular slot, -1 if none
+  unsigned* amask = nullptr;   // [cap*ceil(MV/32)] per-step filter: bit p of row i set when Verlet
+                               // entry p is angular (its angular slot: the set bits below p)
   float* bR = nullptr;         // [cap*T*kRp] radial adjoint in basis space
   float4* f12 = nullptr;       // [cap*MA] dE_i/dr_ij of angular edges
   // Rows split by type pair at rebuild time (split != 0): the step has no

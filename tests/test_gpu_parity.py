@@ -447,10 +447,12 @@ def test_autotune_picks_a_candidate_and_keeps_parity(dmgx, workloads):
     assert all(ms > 0 for ms in tune["filter_ms"].values()) and all(ms > 0 for ms in tune["gather_ms"].values())
     assert tune["filter_ms"][tune["filter_lanes"]] == min(tune["filter_ms"].values())
     assert tune["gather_ms"][tune["gather_lanes"]] == min(tune["gather_ms"].values())
-    # carbon model: rows of 152 entries (multiple of 4) or 256 (power of two); both layouts timed
+    # carbon model: rows of 152 entries (multiple of 4) or 256 (power of two); both layouts timed,
+    # and the wider one kept only when its force kernels win by 3%
     assert tune["row_layout"] in (0, 1) and tune["row_capacity"] == (152, 256)[tune["row_layout"]]
     ms = list(tune["force_ms"].values())
-    assert all(m > 0 for m in ms) and ms[tune["row_layout"]] == min(ms)
+    assert all(m > 0 for m in ms)
+    assert (ms[1] < 0.97 * ms[0]) == (tune["row_layout"] == 1), ms
     # the autotuner's own outputs, then a later evaluation, against the oracle
     got = (pe.cpu().numpy(), f.cpu().numpy().reshape(3, N), v.cpu().numpy().reshape(9, N))
     check(errors(got, ref))
