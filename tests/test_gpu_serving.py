@@ -193,6 +193,20 @@ def test_server_many_small_requests(dmgx, workloads):
     st = srv.stats()
     assert st["completed"] == 24 and st["frames"] == 24
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    # a drained server's statistics restart from zero; its results and
+    # buffers stay, and the next stream is counted alone
+    srv.reset_stats()
+    z = srv.stats()
+    assert z["submitted"] == z["completed"] == z["frames"] == 0 and z["p50_ms"] == 0.0
+    assert z["hbm_budget"] == st["hbm_budget"]
+    _check_single(srv.poll(cases[0][0]), cases[0][1], cases[0][2], orc)
+    again = [(srv.submit(dmgx.Box(h, (1, 1, 1)), np.zeros(p.shape[1], np.int32), np.full(p.shape[1], 12.011), p),
+              h, p) for _, h, p in cases[:6]]
+    srv.drain()
+    st2 = srv.stats()
+    assert st2["submitted"] == st2["completed"] == st2["frames"] == 6, st2
+    for rid, h, p in again:
+        _check_single(srv.poll(rid), h, p, orc)
 
 
 def test_server_image_map_edge_cases(dmgx):
