@@ -47,7 +47,8 @@ struct DevModel {
 // (the serving loop's periodic requests).  owner[j] is the owned slot that j
 // is an image of (owner[j] == j for an owned slot), image[j] the lattice shift
 // (a, b, c) of that image as image_code(a, b, c), and the ghosts of owned slot
-// i occupy slots [gstart[i], gstart[i+1]).  With a map, a row entry j stands
+// i occupy slots [gstart[i], gstart[i+1]), in increasing image code.  With a
+// map, a row entry j stands
 // for its owner everywhere except for its position: the owner's descriptor,
 // radial adjoint and angular partials are the image's, so ghosts need no rows
 // and no descriptors of their own.

@@ -588,13 +588,20 @@ __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ 
           o = im.owner[j];
           const int c = im.image[j];
           if (c != image_code(0, 0, 0)) {
+            // i's ghosts are stored in increasing image code
             const int want = image_neg(c);
+            int lo = im.gstart[i], hi = im.gstart[i + 1] - 1;
             tt = -1;
-            for (int g = im.gstart[i]; g < im.gstart[i + 1]; ++g)
-              if (im.image[g] == want) {
-                tt = g;
+            while (lo <= hi) {
+              const int mid = (lo + hi) >> 1;
+              const int v = im.image[mid];
+              if (v == want) {
+                tt = mid;
                 break;
               }
+              if (v < want) lo = mid + 1;
+              else hi = mid - 1;
+            }
           }
         }
       }

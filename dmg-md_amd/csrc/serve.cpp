@@ -425,10 +425,12 @@ void Server::make_pack(Request& rq, const int* type, const double* mass, const d
   // Euclidean distance, so the shell is a superset of what the rows need);
   // an owned atom's Verlet neighbours at any rebuild lie within
   // rc + skin of the box plus the request's motion, covered by the margin
+  // (an atom's images in increasing image code, c then b then a: the
+  // reverse search of the Verlet build looks them up by bisection)
   for (int i = 0; i < N; ++i)
-    for (int ia = -K[0]; ia <= K[0]; ++ia)
+    for (int ic = -K[2]; ic <= K[2]; ++ic)
       for (int ib = -K[1]; ib <= K[1]; ++ib)
-        for (int ic = -K[2]; ic <= K[2]; ++ic) {
+        for (int ia = -K[0]; ia <= K[0]; ++ia) {
           if (ia == 0 && ib == 0 && ic == 0) continue;
           const int img[3] = {ia, ib, ic};
           double far = 0.0;
