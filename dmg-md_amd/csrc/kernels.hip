@@ -1524,12 +1524,26 @@ __global__ __launch_bounds__(kBlock) void k_gather(DevBox b, DevModel m, int n1,
           // i's Verlet position q in j's row (qv = j MV + q); its angular
           // slot is the number of angular entries of j's row before it
           const int q = qv[r] - jj[r] * w.MV;
-          const unsigned* mk = w.amask + (size_t)jj[r] * ((w.MV + 31) >> 5);
-          const unsigned word = mk[q >> 5];
-          if ((word >> (q & 31)) & 1u) {
-            int c = __popc(word & ((1u << (q & 31)) - 1u));
-            for (int k = 0; k < (q >> 5); ++k) c += __popc(mk[k]);
-            e2[r] = c;
+          const int mw = (w.MV + 31) >> 5;
+          const unsigned* mk = w.amask + (size_t)jj[r] * mw;
+          if (mw == 4) {
+            // rows of up to 128 entries: the row's four words in one load
+            const uint4 m4 = *reinterpret_cast<const uint4*>(mk);
+            const unsigned wd[4] = {m4.x, m4.y, m4.z, m4.w};
+            const int wq = q >> 5;
+            const unsigned word = wd[wq];
+            if ((word >> (q & 31)) & 1u) {
+              int c = __popc(word & ((1u << (q & 31)) - 1u));
+              c += (wq > 0 ? __popc(m4.x) : 0) + (wq > 1 ? __popc(m4.y) : 0) + (wq > 2 ? __popc(m4.z) : 0);
+              e2[r] = c;
+            }
+          } else {
+            const unsigned word = mk[q >> 5];
+            if ((word >> (q & 31)) & 1u) {
+              int c = __popc(word & ((1u << (q & 31)) - 1u));
+              for (int k = 0; k < (q >> 5); ++k) c += __popc(mk[k]);
+              e2[r] = c;
+            }
           }
         }
       }
