@@ -15,7 +15,6 @@
 // works on one centre reads and writes contiguous memory.  Build, sort and
 // reverse search run with kRowLanes lanes per centre (4 centres per wave).
 #include <algorithm>
-#include <climits>
 #include "device_common.hpp"
 
 namespace dmgx {
@@ -376,11 +375,11 @@ __global__ __launch_bounds__(256) void k_verlet_build(const int* __restrict__ go
   int* sorted = row + w.MV;
   int* rs = reinterpret_cast<int*>(shk + (size_t)groups * per) + g * 2 * kRanges;
   int* rp = rs + kRanges;
-  // centres c1 + k (one pass), or with compacted lists the listed centres
-  // (of the requests this rebuild remakes) inside [c1, c2), a pass per
-  // group of `groups` centres
+  // centres c1 + k, or with compacted lists the listed centres (of the
+  // requests this rebuild remakes) inside [c1, c2): a pass per group of
+  // `groups` centres, the grid striding over the groups
   const int ncent = LISTED ? w.map.req_off[2 * w.map.n_req] : c2 - c1;
-  for (int pass = 0; pass < (LISTED ? INT_MAX : 1); ++pass) {
+  for (int pass = 0;; ++pass) {
   const int base = (blockIdx.x + pass * gridDim.x) * groups;
   if (base >= ncent) break;
   const int kc = base + g;
@@ -651,7 +650,9 @@ __global__ __launch_bounds__(256) void k_verlet_reverse(const int* __restrict__ 
 }
 
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
-// workgroups of a rebuild kernel that strides over compacted lists
+// workgroups of 256 lanes of a rebuild kernel, which strides over its
+// centres or slots: 2,048 of them fill 256 CUs at 32 waves each, and a
+// launch that finds no rebuild asked returns in about a microsecond
 constexpr int kListBlocks = 2048;
 
 // LDS of one centre of the build: its row buffers and candidate ranges
@@ -699,14 +700,14 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
   const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
   const int r[4] = {x0_ranges ? x0_ranges[0] : 0, x0_ranges ? x0_ranges[1] : N, x0_ranges ? x0_ranges[2] : 0,
                     x0_ranges ? x0_ranges[3] : 0};
-  // with compacted lists the work grids are capped (their kernels stride
-  // over the listed centres and slots, counted on the device)
+  // the work grids are capped: their kernels stride over the centres and
+  // slots (with compacted lists, the listed ones, counted on the device)
   const bool listed = rebuild_listed(w.map);
   if (listed) {
     k_list_scan<<<1, 1024, 0, s>>>(go, w.map);
     k_list_fill<<<w.map.n_req, kBlock, 0, s>>>(go, w.map);
   }
-  const int slot_blocks = listed ? std::min(nblk(N), kListBlocks) : nblk(N);
+  const int slot_blocks = std::min(nblk(N), kListBlocks);
   if (N > 0)
     k_cell_count<<<slot_blocks, kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, r[0],
                                                 r[1], r[2], r[3], w.flags, w.map);
@@ -726,13 +727,10 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
   if (c2 > c1) {
     const bool keyed = gid != nullptr;
     const int groups = build_groups(w.MV, keyed);
-    int nb = (c2 - c1 + groups - 1) / groups;
     constexpr int rows_per_block = 256 / kRowLanes;
-    int nr = (c2 - c1 + rows_per_block - 1) / rows_per_block;
-    if (listed) {
-      nb = std::min(nb, kListBlocks);
-      nr = std::min(nr, kListBlocks);
-    }
+    // build workgroups hold groups * kRowLanes lanes
+    const int nb = std::min((c2 - c1 + groups - 1) / groups, kListBlocks * (rows_per_block / groups));
+    const int nr = std::min((c2 - c1 + rows_per_block - 1) / rows_per_block, kListBlocks);
     const size_t lds = build_lds_per_centre(w.MV, keyed) * groups;
     if (listed) k_verlet_build<true><<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
     else k_verlet_build<false><<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
