@@ -569,7 +569,9 @@ void NepEngine::launch_force(const Plan& p, int N, int n1, int n2, int nd1, int 
   const bool by_gid = (flags & 8u) != 0;
   const double rv = model_.rcR_max + kSkin;
   prof_.begin(kProfRebuild, s);
-  check(launch_rebuild(p.box, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w_, s), "rebuild");
+  ++evaluations_;
+  check(launch_rebuild(p.box, N, nd1, nd2, rv * rv, by_gid ? gid : nullptr, w_, s, nullptr, rare_rebuilds_),
+        "rebuild");
   if (w_.split) check(launch_split(p.box, dm_, nd1, nd2, w_, s, tune_.filter_lanes), "candidate rows");
   prof_.end(s);
   const bool acc = (flags & 1u) != 0;
@@ -737,6 +739,15 @@ bool NepEngine::settle(hipStream_t s)
   if (hflags_[kFlagRebuildCount] != seen_rebuilds_) {  // a rebuild ran: fresh references
     seen_rebuilds_ = hflags_[kFlagRebuildCount];
     scale_since_build_ = 1.0;
+  }
+  // the rebuild rate over at least 64 evaluations picks the rebuild grids
+  // (both give the same rows); the count restarts with a new allocation
+  const long long de = evaluations_ - rate_evaluations_;
+  const int dr = hflags_[kFlagRebuildCount] - rate_rebuilds_;
+  if (de >= 64 || dr < 0) {
+    if (dr >= 0) rare_rebuilds_ = 16LL * dr < de;
+    rate_evaluations_ = evaluations_;
+    rate_rebuilds_ = hflags_[kFlagRebuildCount];
   }
   if (hflags_[kFlagBadType]) {
     check(hipMemsetAsync(w_.flags + kFlagBadType, 0, sizeof(int), s), "clear flag");

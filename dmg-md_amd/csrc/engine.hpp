@@ -238,6 +238,12 @@ class NepEngine {
   int mv_base_ = 0;  // row capacity asked for, before the layout's rounding
   double scale_since_build_ = 1.0;  // isotropic box scale accumulated since the last Verlet build
   int seen_rebuilds_ = 0;           // device rebuild count when last read
+  // rebuild rate: evaluations enqueued, and the evaluation and rebuild
+  // counts where the rate was last taken; with fewer than one rebuild per
+  // 16 evaluations the rebuild kernels run on capped grids
+  long long evaluations_ = 0, rate_evaluations_ = 0;
+  int rate_rebuilds_ = 0;
+  bool rare_rebuilds_ = false;
   bool last_small_ = false;         // the last evaluation ran on a replicated small box
   int padded_rows(int mv, int layout) const;
   void launch_step_kernels(const DevBox& b, int N, int nc, const int* type, double* pe, double* force,

@@ -187,8 +187,12 @@ hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const i
 // x0_ranges (optional): {a1, b1, a2, b2}, the slots whose displacement
 // references this rebuild resets (default every slot); the others keep
 // theirs (rows carried over from an earlier build)
+// capped: grids of at most 2,048 workgroups striding over the centres and
+// slots, for callers that rebuild rarely (an empty launch then costs about a
+// microsecond; a rebuild takes about 1.3x as long as on full grids); a
+// rebuild from compacted lists always uses capped grids
 hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, const uint64_t* gid, DevWork& w,
-                          hipStream_t s, const int* x0_ranges = nullptr);
+                          hipStream_t s, const int* x0_ranges = nullptr, bool capped = false);
 
 // Force evaluation (per step)
 // lanes: lanes per centre, 8, 16 (default) or 32

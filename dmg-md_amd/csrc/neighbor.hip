@@ -694,20 +694,21 @@ hipError_t launch_pack(const DevBox& b, int N, int T, const double* pos, const i
 }
 
 hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, const uint64_t* gid, DevWork& w,
-                          hipStream_t s, const int* x0_ranges)
+                          hipStream_t s, const int* x0_ranges, bool capped)
 {
   const int* go = w.flags + kFlagRebuild;
   const int ncell = b.nc[0] * b.nc[1] * b.nc[2];
   const int r[4] = {x0_ranges ? x0_ranges[0] : 0, x0_ranges ? x0_ranges[1] : N, x0_ranges ? x0_ranges[2] : 0,
                     x0_ranges ? x0_ranges[3] : 0};
-  // the work grids are capped: their kernels stride over the centres and
-  // slots (with compacted lists, the listed ones, counted on the device)
+  // the kernels stride over the centres and slots (with compacted lists,
+  // the listed ones, counted on the device), so the grids may be capped
   const bool listed = rebuild_listed(w.map);
   if (listed) {
     k_list_scan<<<1, 1024, 0, s>>>(go, w.map);
     k_list_fill<<<w.map.n_req, kBlock, 0, s>>>(go, w.map);
   }
-  const int slot_blocks = std::min(nblk(N), kListBlocks);
+  const bool cap = capped || listed;
+  const int slot_blocks = cap ? std::min(nblk(N), kListBlocks) : nblk(N);
   if (N > 0)
     k_cell_count<<<slot_blocks, kBlock, 0, s>>>(go, b, N, w.xq, w.cell_of, w.cell_rank, w.cell_count, w.x0, r[0],
                                                 r[1], r[2], r[3], w.flags, w.map);
@@ -729,8 +730,12 @@ hipError_t launch_rebuild(const DevBox& b, int N, int c1, int c2, double rv2, co
     const int groups = build_groups(w.MV, keyed);
     constexpr int rows_per_block = 256 / kRowLanes;
     // build workgroups hold groups * kRowLanes lanes
-    const int nb = std::min((c2 - c1 + groups - 1) / groups, kListBlocks * (rows_per_block / groups));
-    const int nr = std::min((c2 - c1 + rows_per_block - 1) / rows_per_block, kListBlocks);
+    int nb = (c2 - c1 + groups - 1) / groups;
+    int nr = (c2 - c1 + rows_per_block - 1) / rows_per_block;
+    if (cap) {
+      nb = std::min(nb, kListBlocks * (rows_per_block / groups));
+      nr = std::min(nr, kListBlocks);
+    }
     const size_t lds = build_lds_per_centre(w.MV, keyed) * groups;
     if (listed) k_verlet_build<true><<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
     else k_verlet_build<false><<<nb, groups * kRowLanes, lds, s>>>(go, b, c1, c2, groups, w.xq, rv2, gid, w);
