@@ -133,6 +133,31 @@ def test_verlet_reuse_within_skin_and_rebuild(dmgx, workloads):
     assert nep.neighbor_stats()["rebuilds"] == r0 + 1
 
 
+def test_rows_rebuilt_on_capped_grids(dmgx, workloads):
+    """After 64 evaluations without a rebuild the engine runs its rebuild
+    kernels on capped, striding grids; the rows they build equal the
+    oracle's, and the forces stay at the oracle's."""
+    s, path = _system(workloads, "cu")
+    nep = dmgx.NEP(path)
+    gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"])
+    r0 = nep.neighbor_stats()["rebuilds"]
+    for _ in range(80):  # 80 evaluations on the same rows
+        gpu_compute(dmgx, nep, s["box"], s["types"], s["pos"])
+    assert nep.neighbor_stats()["rebuilds"] == r0
+    rng = np.random.default_rng(11)
+    moved = s["pos"] + rng.uniform(-0.15, 0.15, s["pos"].shape)
+    moved[:, 0] += 0.6  # past skin/2: a rebuild
+    out = gpu_compute(dmgx, nep, s["box"], s["types"], moved)
+    assert nep.neighbor_stats()["rebuilds"] == r0 + 1
+    N = len(s["types"])
+    rows_ref = neighbor_rows(s["box"].h, s["box"].pbc, moved, nep.info.rc_radial_max + 1.0, max_nb=1024)
+    rows_gpu = nep.verlet_rows(0, N)
+    bad = [i for i in range(N) if not np.array_equal(rows_gpu[i], rows_ref[i])]
+    assert not bad, bad[:5]
+    ref = OracleNEP(path, round_params_to_float=True).compute(s["box"].h, s["box"].pbc, s["types"], moved)
+    check(errors(out, ref))
+
+
 def test_domain_ranges(dmgx, workloads):
     """Force centres [0,n2) of a full-dependency evaluation equal the plain
     call; dependency centres restricted to what the force centres need."""
